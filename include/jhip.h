@@ -1,0 +1,197 @@
+/*
+ * jhip.h -- C-ABI of the MI355X (gfx950) FDK reconstruction back end.
+ *
+ * libjhip.so exports plain C entry points: plain pointers and sizes, no
+ * exceptions, no C++ or torch types.  Device buffers passed in are owned by
+ * the caller; plans own their internal tables and accumulators.  Every call
+ * that takes a stream enqueues work on it and returns without synchronising,
+ * except the *_create calls (which finish their set-up before returning) and
+ * the calls that copy tables back to the host.  A failing call returns a
+ * non-zero jh_status and leaves a message for jh_last_error() on the calling
+ * thread.  A plan may be used from one host thread at a time.
+ *
+ * Each entry point names the reference interface it replaces (paths are
+ * relative to the root of jay6987/JEnhine).
+ */
+#ifndef JHIP_H
+#define JHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int jh_status;
+#define JH_OK 0
+#define JH_ERR_INVALID 1 /* bad argument or shape */
+#define JH_ERR_OOM 2     /* device allocation failed (message starts "HIP failed to malloc ") */
+#define JH_ERR_HIP 3     /* HIP runtime error */
+
+/* Storage type of backprojection input (filtered, uW-weighted projections). */
+#define JH_F32 0
+#define JH_BF16 1
+#define JH_F16 2
+
+/* Row-filter arithmetic. */
+#define JH_FILTER_FP32 0        /* FP32 Toeplitz GEMM on the vector ALUs */
+#define JH_FILTER_MFMA_BF16X3 1 /* Toeplitz GEMM on MFMA, hi/lo split BF16 (3 products), FP32 accumulate */
+
+/* Filter windows. */
+#define JH_WINDOW_REFERENCE 0   /* FilterCore.cpp:96-134 cut-off semantics */
+#define JH_WINDOW_SHEPP_LOGAN 1 /* extension: sinc(f / (2 f_N)), zero above f_N */
+
+/* ProjectionMatrixModifier operations (TransformMatrix/ProjectionMatrixModifier.cpp:5-43). */
+#define JH_PTM_MOVE_U 0
+#define JH_PTM_MOVE_V 1
+#define JH_PTM_REVERSE_X 2
+#define JH_PTM_REVERSE_Y 3
+#define JH_PTM_MOVE_Z 4
+
+typedef void* jh_stream_t; /* a hipStream_t (NULL = the default stream) */
+
+typedef struct jh_pre_plan jh_pre_plan;
+typedef struct jh_filter_plan jh_filter_plan;
+typedef struct jh_bp_plan jh_bp_plan;
+
+/* Last error message of the calling thread ("" when none).
+ * Replaces Common/Exception.h:12-41 (ThrowExceptionAndLog) across the ABI. */
+const char* jh_last_error(void);
+
+/* Library version, major*10000 + minor*100 + patch. */
+int jh_version(void);
+
+/* Number of visible devices (0 on a host without GPUs). */
+int jh_device_count(void);
+
+/* ---------------------------------------------------------------------- */
+/* Geometry                                                                */
+/* ---------------------------------------------------------------------- */
+
+/* Apply one ProjectionMatrixModifier operation, in FP64 on the device, to n
+ * row-major 3x4 projection matrices in device memory (d_ptm, n*12 doubles).
+ * Replaces TransformMatrix/ProjectionMatrixModifier.cpp:5-43 (float, host)
+ * as called from JEngine/JEngine.cpp:139-150. */
+jh_status jh_ptm_modify(double* d_ptm, int n, int op, double value, jh_stream_t stream);
+
+/* ---------------------------------------------------------------------- */
+/* Preprocessing: u16 -> line integrals                                    */
+/* ---------------------------------------------------------------------- */
+
+/* Replaces PreProcessingAgent/PreProcessingCore.cpp:15-65 (constructor):
+ * input frame inW x inH, borders (up, down, left, right), output frame
+ * outW x outH (crop, or linear resize when the cropped size differs),
+ * bright field and beam-hardening coefficients bh[3] (skipped iff {1,0,0}). */
+jh_status jh_pre_create(jh_pre_plan** plan, int device, int inW, int inH, int bUp, int bDown,
+                        int bLeft, int bRight, int outW, int outH, float brightField,
+                        const float* bh);
+
+/* Replaces PreProcessingCore::Process (PreProcessingCore.cpp:67-86) for
+ * nviews frames at once: d_raw (nviews x inH x inW u16) -> d_out
+ * (nviews x outH x outW f32).  d_dark (inH x inW u16, subtracted as
+ * integers) and d_gain (outH x outW f32, divisor BrightField*gain) are
+ * optional extensions; NULL gives the reference chain exactly. */
+jh_status jh_pre_run(jh_pre_plan* plan, const uint16_t* d_raw, const uint16_t* d_dark,
+                     const float* d_gain, float* d_out, int nviews, jh_stream_t stream);
+
+jh_status jh_pre_destroy(jh_pre_plan* plan);
+
+/* ---------------------------------------------------------------------- */
+/* Row ramp filter with pre-filter weight                                  */
+/* ---------------------------------------------------------------------- */
+
+/* Replaces FilterAgent/FilterCore.cpp:13-61 (design), the per-view pre-weight
+ * of FilterAgent/PreFilterWeightGenerator.cpp:10-124 with its refresh
+ * schedule (FilterAgent.cpp:101-126), for frames of W x V.  h_ptm: nviews
+ * row-major 3x4 matrices on the host (FP64).  Pre-weight maps are built on
+ * the device in FP64.  adjPts/adjDB: nAdj dB-adjust knots (may be NULL when
+ * nAdj = 0).  Output width = W, yMirroring = false (the oracle chain). */
+jh_status jh_filter_create(jh_filter_plan** plan, int device, int W, int V, int nviews,
+                           const double* h_ptm, float dso, float halfSampleRate, float cutStart,
+                           float cutEnd, const float* adjPts, const float* adjDB, int nAdj,
+                           int window, int mode);
+
+/* Copies the frequency response H[0..L/2] (L = FFT length of the reference,
+ * FilterCore.cpp:254-260) to the host; *L receives L. */
+jh_status jh_filter_response(jh_filter_plan* plan, double* h_H, int* L);
+
+/* Copies the pre-weight map used for view `view` (V rows x W columns) to
+ * device memory d_out. */
+jh_status jh_filter_preweight(jh_filter_plan* plan, int view, float* d_out, jh_stream_t stream);
+
+/* Replaces FilterCore::ProcessRow (FilterCore.cpp:189-222) over all rows of
+ * nviews frames: d_in, d_out are nviews x V x W f32; frame i is view
+ * (view0 + i) of the scan (selects its pre-weight map). */
+jh_status jh_filter_run(jh_filter_plan* plan, const float* d_in, float* d_out, int view0,
+                        int nviews, jh_stream_t stream);
+
+jh_status jh_filter_destroy(jh_filter_plan* plan);
+
+/* ---------------------------------------------------------------------- */
+/* Backprojection                                                          */
+/* ---------------------------------------------------------------------- */
+
+/* Replaces BPAgent/BPCore.cpp:15-101 and BPCUDAAgent/BPCUDACore.cpp:17-127
+ * (constructors): nviews projection matrices (h_ptm, host, FP64, row-major
+ * 3x4), detector W x V, volume X x Y x Z (memory order vol[z][y][x]) at
+ * pitches pxy, pz.  Per-view geometry and detector weights are set up on the
+ * device in FP64. */
+jh_status jh_bp_create(jh_bp_plan** plan, int device, const double* h_ptm, int nviews, int W,
+                       int V, int X, int Y, int Z, float pxy, float pz);
+
+/* Copies the per-view detector weights uW (nviews x W, BPCore.cpp:68-100)
+ * to the host. */
+jh_status jh_bp_uweights(jh_bp_plan* plan, float* h_out);
+
+/* Bytes of staged backprojection input for nviews views in `dtype`. */
+jh_status jh_bp_stage_bytes(jh_bp_plan* plan, int dtype, int nviews, size_t* bytes);
+
+/* Replaces BPCore::InitView's projection weighting (BPCore.cpp:147-151):
+ * q = filtered * uW for views view0..view0+nviews-1, written in the
+ * backprojector's storage layout and type (d_q, jh_bp_stage_bytes bytes). */
+jh_status jh_bp_stage(jh_bp_plan* plan, const float* d_filtered, int view0, int nviews, int dtype,
+                      void* d_q, jh_stream_t stream);
+
+/* One-shot normalised backprojection of all views: d_vol (Z x Y x X f32) =
+ * sum_v bilinear(q_v) / sum_v bilinear(uW_v).  Replaces the whole
+ * BPCore InitShot/InitView/ProcessSlice/DoneSlice sequence
+ * (BPCore.cpp:108-205) and BPCUDACore's per-view loop
+ * (BPCUDAAgent/BPCUDAAgent.cpp:231-287). */
+jh_status jh_bp_run(jh_bp_plan* plan, const void* d_q, int dtype, float* d_vol, jh_stream_t stream);
+
+/* Streaming form, used by the agent: zero the accumulators (BPCore::InitShot,
+ * BPCore.cpp:108-113; BPCUDACore::InitShot_Synced, BPCUDACore.cpp:134-138). */
+jh_status jh_bp_reset(jh_bp_plan* plan, jh_stream_t stream);
+
+/* Accumulate views view0..view0+nviews-1 (d_q holds exactly those views) into
+ * the plan's accumulator and count volumes.  Replaces BPCore::ProcessSlice
+ * (BPCore.cpp:157-196) and BPCUDACore::DeployBackProj (BPCUDACore.cu:89-114). */
+jh_status jh_bp_accumulate(jh_bp_plan* plan, const void* d_q, int dtype, int view0, int nviews,
+                           jh_stream_t stream);
+
+/* Write slices z0..z0+nz-1 of acc/count to d_vol (nz x Y x X f32).
+ * Replaces BPCore::DoneSlice (BPCore.cpp:198-205) and
+ * BPCUDACore::DeployUpdateOutput + Div2 (BPCUDACore.cu:68-87,177-215). */
+jh_status jh_bp_finalize(jh_bp_plan* plan, float* d_vol, int z0, int nz, jh_stream_t stream);
+
+/* Tile shape of the backprojector: z voxels per thread (8, 16 or 32) and
+ * voxel columns per workgroup along y (1, 2 or 4; 64 along x).  0 keeps the
+ * current value.  No counterpart in the reference (its launch shape is fixed
+ * at BPCUDACore.cu:99-112); used by the autotuner. */
+jh_status jh_bp_set_tile(jh_bp_plan* plan, int zc, int ty);
+
+/* Current tile shape. */
+jh_status jh_bp_get_tile(jh_bp_plan* plan, int* zc, int* ty);
+
+/* Geometry class picked at create time: 0 = u and w constant along z,
+ * 1 = w constant along z, 2 = general. */
+jh_status jh_bp_geometry_mode(jh_bp_plan* plan, int* mode);
+
+jh_status jh_bp_destroy(jh_bp_plan* plan);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JHIP_H */

@@ -1,0 +1,102 @@
+"""ctypes binding of libjhip.so (the C-ABI declared in include/jhip.h).
+
+This is the binding a Python host uses; the C++ agents link the same library
+directly.  Loading fails loudly when the library is missing: there is no
+fallback implementation.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libjhip.so")
+
+JH_OK, JH_ERR_INVALID, JH_ERR_OOM, JH_ERR_HIP = 0, 1, 2, 3
+JH_F32, JH_BF16, JH_F16 = 0, 1, 2
+JH_FILTER_FP32, JH_FILTER_MFMA_BF16X3 = 0, 1
+JH_WINDOW_REFERENCE, JH_WINDOW_SHEPP_LOGAN = 0, 1
+JH_PTM_MOVE_U, JH_PTM_MOVE_V, JH_PTM_REVERSE_X, JH_PTM_REVERSE_Y, JH_PTM_MOVE_Z = range(5)
+
+DTYPES = {"f32": JH_F32, "bf16": JH_BF16, "f16": JH_F16}
+
+# Every symbol include/jhip.h declares (tests check the library exports all).
+EXPORTS = [
+    "jh_last_error", "jh_version", "jh_device_count", "jh_ptm_modify",
+    "jh_pre_create", "jh_pre_run", "jh_pre_destroy",
+    "jh_filter_create", "jh_filter_response", "jh_filter_preweight", "jh_filter_run", "jh_filter_destroy",
+    "jh_bp_create", "jh_bp_uweights", "jh_bp_stage_bytes", "jh_bp_stage", "jh_bp_run", "jh_bp_reset",
+    "jh_bp_accumulate", "jh_bp_finalize", "jh_bp_set_tile", "jh_bp_get_tile", "jh_bp_geometry_mode",
+    "jh_bp_destroy",
+]
+
+
+class JhError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"jhip error {status}: {msg}")
+        self.status = status
+
+
+class JhOutOfMemory(JhError):
+    pass
+
+
+_lib = None
+
+c_int, c_float, c_double, c_void_p, c_size_t = (ctypes.c_int, ctypes.c_float, ctypes.c_double,
+                                               ctypes.c_void_p, ctypes.c_size_t)
+P = c_void_p
+
+_SIGS = {
+    "jh_last_error": (ctypes.c_char_p, []),
+    "jh_version": (c_int, []),
+    "jh_device_count": (c_int, []),
+    "jh_ptm_modify": (c_int, [P, c_int, c_int, c_double, P]),
+    "jh_pre_create": (c_int, [P, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_float, P]),
+    "jh_pre_run": (c_int, [P, P, P, P, P, c_int, P]),
+    "jh_pre_destroy": (c_int, [P]),
+    "jh_filter_create": (c_int, [P, c_int, c_int, c_int, c_int, P, c_float, c_float, c_float, c_float, P, P,
+                                 c_int, c_int, c_int]),
+    "jh_filter_response": (c_int, [P, P, P]),
+    "jh_filter_preweight": (c_int, [P, c_int, P, P]),
+    "jh_filter_run": (c_int, [P, P, P, c_int, c_int, P]),
+    "jh_filter_destroy": (c_int, [P]),
+    "jh_bp_create": (c_int, [P, c_int, P, c_int, c_int, c_int, c_int, c_int, c_int, c_float, c_float]),
+    "jh_bp_uweights": (c_int, [P, P]),
+    "jh_bp_stage_bytes": (c_int, [P, c_int, c_int, P]),
+    "jh_bp_stage": (c_int, [P, P, c_int, c_int, c_int, P, P]),
+    "jh_bp_run": (c_int, [P, P, c_int, P, P]),
+    "jh_bp_reset": (c_int, [P, P]),
+    "jh_bp_accumulate": (c_int, [P, P, c_int, c_int, c_int, P]),
+    "jh_bp_finalize": (c_int, [P, P, c_int, c_int, P]),
+    "jh_bp_set_tile": (c_int, [P, c_int, c_int]),
+    "jh_bp_get_tile": (c_int, [P, P, P]),
+    "jh_bp_geometry_mode": (c_int, [P, P]),
+    "jh_bp_destroy": (c_int, [P]),
+}
+
+
+def lib():
+    """The loaded library; raises if libjhip.so has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m jenhine_amd.build` "
+                               "(there is no fallback path)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status):
+    if status != JH_OK:
+        msg = lib().jh_last_error().decode(errors="replace")
+        if status == JH_ERR_OOM:
+            raise JhOutOfMemory(status, msg)
+        raise JhError(status, msg)
+
+
+def call(name, *args):
+    check(getattr(lib(), name)(*args))

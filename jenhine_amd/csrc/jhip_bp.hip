@@ -1,0 +1,653 @@
+// Voxel-driven normalised backprojection for gfx950.
+//
+// Semantics (BPAgent/BPCore.cpp:108-205): for every voxel
+//   vol = sum_v bilinear(q_v, su, sv) / sum_v bilinear(uW_v, su, sv),
+//   su = U/W + 0.5, sv = V/W + 0.5, (U, V, W) = P_v (x, y, z, 1),
+// where q_v = filtered_v * uW_v row-wise and a sample whose coordinate falls
+// outside [0, W-1] x [0, V-1] contributes to neither sum.
+//
+// Design:
+//  * per-view geometry is set up on the device in FP64 and rounded once to
+//    float coefficients in voxel-index space;
+//  * each thread owns one (x, y) voxel column and ZC consecutive z voxels,
+//    keeping ZC accumulators and ZC counts in registers across all views of
+//    a launch, so the volume is written once per launch (one-shot) or
+//    read-modify-written once per view chunk (streaming);
+//  * projections are stored in a "pair" layout, pixel (u, v) holding
+//    (q[v][u], q[v][u+1]) in FP32, BF16 or FP16, so one bilinear sample is two
+//    aligned gathers that hit L1/L2;
+//  * when u and w do not depend on z (circular orbits) the u-side of the
+//    bilinear weights and the count weight are hoisted out of the z loop;
+//  * 64 x TY voxel columns per workgroup, workgroups remapped so that each of
+//    the 8 XCDs owns a contiguous z-range of tiles (its L2 then holds the
+//    detector rows of that z-range).
+#include <cmath>
+#include <cstring>
+
+#include "jhip_internal.h"
+
+namespace jh {
+
+struct Geo {
+    float c[12];  // cu0 cui cuj cuk | cv0 cvi cvj cvk | cw0 cwi cwj cwk
+};
+
+}  // namespace jh
+
+struct jh_bp_plan {
+    int device;
+    int N, W, V, X, Y, Z;
+    float pxy, pz;
+    int mode;
+    int zc, ty;
+    jh::DevBuf ptm;  // N x 12 double
+    jh::DevBuf geo;  // N x Geo
+    jh::DevBuf uw;   // N x W float
+    jh::DevBuf uwp;  // N x W float2 count-weight pairs
+    jh::DevBuf acc;  // X*Y*Z float (streaming form, allocated on first use)
+    jh::DevBuf cnt;  // X*Y*Z float
+};
+
+namespace jh {
+
+// ---------------------------------------------------------------------------
+// Per-view set-up, FP64.
+// ---------------------------------------------------------------------------
+
+// Index-space coefficients: U(i,j,k) = cu0 + cui*i + cuj*j + cuk*k with the
+// centred axes a[i] = (i - (n-1)/2) * pitch of BPCore.cpp:47-66.
+__global__ void bp_geometry_kernel(const double* __restrict__ ptm, int N, int X, int Y, int Z,
+                                   double pxy, double pz, Geo* __restrict__ geo)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    const double* P = ptm + 12 * (size_t)v;
+    const double ax0 = -((double)X - 1.0) * 0.5 * pxy;
+    const double ay0 = -((double)Y - 1.0) * 0.5 * pxy;
+    const double az0 = -((double)Z - 1.0) * 0.5 * pz;
+    Geo g;
+    for (int r = 0; r < 3; ++r) {
+        const double* p = P + 4 * r;
+        g.c[4 * r + 0] = (float)(p[0] * ax0 + p[1] * ay0 + p[2] * az0 + p[3]);
+        g.c[4 * r + 1] = (float)(p[0] * pxy);
+        g.c[4 * r + 2] = (float)(p[1] * pxy);
+        g.c[4 * r + 3] = (float)(p[2] * pz);
+    }
+    geo[v] = g;
+}
+
+// Detector weight uW across u (BPCore.cpp:68-100), FP64 then rounded.
+__global__ void bp_uweight_kernel(const double* __restrict__ ptm, int N, int W, float* __restrict__ uw)
+{
+    const int v = blockIdx.y;
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N || u >= W) return;
+    const double* P = ptm + 12 * (size_t)v;
+    const double uu = P[3] / P[11];
+    const long u0 = uu > 0.0 ? (long)uu : 0;
+    const double pi = 3.14159265358979323846;
+    double w = 1.0;
+    if (u0 > (long)(W / 2)) {
+        const long nT = u0 < (long)W ? ((long)W - u0) * 2 : 0;
+        const long i = u - (W - nT);
+        if (i >= 0 && i < nT) w = cos((double)i * pi / (double)nT) * 0.5 + 0.5;
+    } else {
+        long nT = u0 * 2;
+        if (nT > W) nT = W;
+        if (u < nT) w = cos((double)u * pi / (double)nT) * -0.5 + 0.5;
+    }
+    uw[(size_t)v * W + u] = (float)w;
+}
+
+// ---------------------------------------------------------------------------
+// Staging: q = filtered * uW in the pair layout.
+//
+// Frame layout: V+1 rows of W pixels; pixel (u, v) holds the pair
+// (q[v][u], q[v][u+1]) with q[v][W] = 0, and row V is all zeros.  A bilinear
+// sample at integer (iu, iv) with iu <= W-1, iv <= V-1 then reads pixels
+// (iu, iv) and (iu, iv+1) without clamping: at the far edges the weight of
+// the padding is exactly 0.
+// ---------------------------------------------------------------------------
+
+__device__ inline uint32_t f32_to_bf16_rne(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (u >> 16) | 0x40u;  // quiet NaN
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return u >> 16;
+}
+
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
+
+template <int QT>
+__global__ void __launch_bounds__(256) bp_stage_kernel(const float* __restrict__ filt,
+                                                       const float* __restrict__ uw, int W, int V,
+                                                       int nviews, void* __restrict__ out)
+{
+    const size_t frame_in = (size_t)W * V;
+    const size_t frame_out = (size_t)W * (V + 1);
+    const size_t total = frame_out * nviews;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (size_t)gridDim.x * blockDim.x) {
+        const size_t view = t / frame_out;
+        const int pix = (int)(t - view * frame_out);
+        const int row = pix / W;
+        const int u = pix - row * W;
+        float a = 0.0f, b = 0.0f;
+        if (row < V) {
+            const float* w = uw + view * W;
+            const float* f = filt + view * frame_in + (size_t)row * W;
+            a = f[u] * w[u];
+            if (u + 1 < W) b = f[u + 1] * w[u + 1];
+        }
+        if (QT == JH_F32) {
+            reinterpret_cast<float2*>(out)[t] = make_float2(a, b);
+        } else if (QT == JH_BF16) {
+            reinterpret_cast<uint32_t*>(out)[t] = f32_to_bf16_rne(a) | (f32_to_bf16_rne(b) << 16);
+        } else {
+            half2_t h;
+            h.x = (_Float16)a;
+            h.y = (_Float16)b;
+            reinterpret_cast<half2_t*>(out)[t] = h;
+        }
+    }
+}
+
+// Count-weight pairs (uW[u], uW[u+1]) per view; uW[W] := uW[W-1] (its weight
+// is always 0).
+__global__ void bp_uwpair_kernel(const float* __restrict__ uw, int N, int W, float2* __restrict__ uwp)
+{
+    const size_t n = (size_t)N * W;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+        const int u = (int)(t % W);
+        uwp[t] = make_float2(uw[t], u + 1 < W ? uw[t + 1] : uw[t]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Backprojection kernels.
+// ---------------------------------------------------------------------------
+
+constexpr int kBufferFlags = 0x00020000;  // raw buffer, 32-bit dwords (gfx9 family)
+
+// Bilinear value from the two pair loads of one sample.
+template <int QT>
+struct Sampler;
+
+template <>
+struct Sampler<JH_F32> {
+    typedef u32x2_t raw_t;
+    static constexpr int bytes = 8;
+    __device__ static raw_t load(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff)
+    {
+        return __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0);
+    }
+    __device__ static float lerp(raw_t p, float fu)
+    {
+        const float a = __uint_as_float(p.x), b = __uint_as_float(p.y);
+        return fmaf(fu, b - a, a);
+    }
+};
+
+template <>
+struct Sampler<JH_BF16> {
+    typedef uint32_t raw_t;
+    static constexpr int bytes = 4;
+    __device__ static raw_t load(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff)
+    {
+        return __builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0);
+    }
+    __device__ static float lerp(raw_t p, float fu)
+    {
+        const float a = __uint_as_float(p << 16), b = __uint_as_float(p & 0xffff0000u);
+        return fmaf(fu, b - a, a);
+    }
+};
+
+template <>
+struct Sampler<JH_F16> {
+    typedef uint32_t raw_t;
+    static constexpr int bytes = 4;
+    __device__ static raw_t load(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff)
+    {
+        return __builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0);
+    }
+    __device__ static float lerp(raw_t p, float fu)
+    {
+        // a*(1-fu) + b*fu with the f16 -> f32 conversions folded into v_fma_mix
+        const half2_t h = __builtin_bit_cast(half2_t, p);
+        return fmaf((float)h.y, fu, (float)h.x * (1.0f - fu));
+    }
+};
+
+struct BPArgs {
+    const void* q;       // views view0..view0+nviews-1, pair layout, V+1 rows
+    const Geo* geo;      // all views
+    const float2* uwp;   // all views, N x W count-weight pairs
+    float* vol;          // one-shot output
+    float* acc;          // streaming accumulators
+    float* cnt;
+    int W, V, X, Y, Z;
+    int view0, nviews;
+    int ntx, nty, ntz;
+    uint32_t frame_bytes;
+};
+
+constexpr int kZB = 8;  // samples whose loads are issued together
+
+template <int MODE, int ZC, int QT, bool FINAL>
+__global__ void __launch_bounds__(256) bp_kernel(BPArgs a)
+{
+    typedef Sampler<QT> S;
+    const unsigned ntiles = (unsigned)a.ntx * a.nty * a.ntz;
+    const unsigned per = gridDim.x >> 3;  // gridDim.x is a multiple of 8
+    const unsigned tile = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+    if (tile >= ntiles) return;
+    const int tx = tile % a.ntx;
+    const unsigned rest = tile / a.ntx;
+    const int tyi = rest % a.nty;
+    const int tz = rest / a.nty;
+    const int i = tx * 64 + threadIdx.x;
+    const int j = tyi * blockDim.y + threadIdx.y;
+    const int k0 = tz * ZC;
+    const float fi = (float)min(i, a.X - 1);
+    const float fj = (float)min(j, a.Y - 1);
+    const float fk0 = (float)k0;
+    const float Wm1 = (float)(a.W - 1), Vm1 = (float)(a.V - 1);
+    const uint32_t rowb = (uint32_t)a.W * S::bytes;
+
+    float acc[ZC], cnt[ZC];
+#pragma unroll
+    for (int kk = 0; kk < ZC; ++kk) { acc[kk] = 0.0f; cnt[kk] = 0.0f; }
+
+    for (int v = 0; v < a.nviews; ++v) {
+        const Geo g = a.geo[a.view0 + v];
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<char*>(reinterpret_cast<const char*>(a.q)) + (size_t)v * a.frame_bytes, (short)0,
+            (int)a.frame_bytes, kBufferFlags);
+        const float2* uwv = a.uwp + (size_t)(a.view0 + v) * a.W;
+        const float Uc = fmaf(g.c[3], fk0, fmaf(g.c[1], fi, fmaf(g.c[2], fj, g.c[0])));
+        const float Vc = fmaf(g.c[7], fk0, fmaf(g.c[5], fi, fmaf(g.c[6], fj, g.c[4])));
+        const float Wc = fmaf(g.c[11], fk0, fmaf(g.c[9], fi, fmaf(g.c[10], fj, g.c[8])));
+        if (MODE == 0) {
+            // u and w are constant along the column: hoist the u side
+            const float rw = 1.0f / Wc;
+            const float su = fmaf(Uc, rw, 0.5f);
+            if (su >= 0.0f && su <= Wm1) {
+                const int iu = (int)su;
+                const float fu = __builtin_amdgcn_fractf(su);
+                const float2 uwq = uwv[iu];
+                const float cw = fmaf(fu, uwq.y - uwq.x, uwq.x);
+                const float sv0 = fmaf(Vc, rw, 0.5f);
+                const float dsv = g.c[7] * rw;
+                const uint32_t ub = (uint32_t)iu * S::bytes;
+#pragma unroll
+                for (int kb = 0; kb < ZC; kb += kZB) {
+                    uint32_t off[kZB];
+                    float fv[kZB];
+                    bool in[kZB];
+#pragma unroll
+                    for (int q = 0; q < kZB; ++q) {
+                        const float sv = fmaf((float)(kb + q), dsv, sv0);
+                        const float t = __builtin_amdgcn_fmed3f(sv, 0.0f, Vm1);
+                        in[q] = (t == sv);
+                        fv[q] = __builtin_amdgcn_fractf(t);
+                        off[q] = __umul24((uint32_t)(int)t, rowb) + ub;
+                    }
+                    typename S::raw_t p0[kZB], p1[kZB];
+#pragma unroll
+                    for (int q = 0; q < kZB; ++q) {
+                        p0[q] = S::load(rs, off[q], 0);
+                        p1[q] = S::load(rs, off[q], rowb);
+                    }
+#pragma unroll
+                    for (int q = 0; q < kZB; ++q) {
+                        const float top = S::lerp(p0[q], fu);
+                        const float bot = S::lerp(p1[q], fu);
+                        const float val = fmaf(fv[q], bot - top, top);
+                        acc[kb + q] += in[q] ? val : 0.0f;
+                        cnt[kb + q] += in[q] ? cw : 0.0f;
+                    }
+                }
+            }
+        } else {
+            // MODE 1: w constant along the column, u and v linear in z;
+            // MODE 2: general, one reciprocal per sample.
+            const float rw = 1.0f / Wc;
+            const float su0 = fmaf(Uc, rw, 0.5f), dsu = g.c[3] * rw;
+            const float sv0 = fmaf(Vc, rw, 0.5f), dsv = g.c[7] * rw;
+#pragma unroll
+            for (int kb = 0; kb < ZC; kb += kZB) {
+                uint32_t off[kZB];
+                float fu[kZB], fv[kZB];
+                bool in[kZB];
+                float2 uwq[kZB];
+#pragma unroll
+                for (int q = 0; q < kZB; ++q) {
+                    const float fk = (float)(kb + q);
+                    float su, sv;
+                    if (MODE == 1) {
+                        su = fmaf(fk, dsu, su0);
+                        sv = fmaf(fk, dsv, sv0);
+                    } else {
+                        const float r = __builtin_amdgcn_rcpf(fmaf(fk, g.c[11], Wc));
+                        su = fmaf(fmaf(fk, g.c[3], Uc), r, 0.5f);
+                        sv = fmaf(fmaf(fk, g.c[7], Vc), r, 0.5f);
+                    }
+                    const float tu = __builtin_amdgcn_fmed3f(su, 0.0f, Wm1);
+                    const float t = __builtin_amdgcn_fmed3f(sv, 0.0f, Vm1);
+                    in[q] = (t == sv) && (tu == su);
+                    const int iu = (int)tu;
+                    fu[q] = __builtin_amdgcn_fractf(tu);
+                    fv[q] = __builtin_amdgcn_fractf(t);
+                    off[q] = __umul24((uint32_t)(int)t, rowb) + (uint32_t)iu * S::bytes;
+                    uwq[q] = uwv[iu];
+                }
+                typename S::raw_t p0[kZB], p1[kZB];
+#pragma unroll
+                for (int q = 0; q < kZB; ++q) {
+                    p0[q] = S::load(rs, off[q], 0);
+                    p1[q] = S::load(rs, off[q], rowb);
+                }
+#pragma unroll
+                for (int q = 0; q < kZB; ++q) {
+                    const float top = S::lerp(p0[q], fu[q]);
+                    const float bot = S::lerp(p1[q], fu[q]);
+                    const float val = fmaf(fv[q], bot - top, top);
+                    const float cw = fmaf(fu[q], uwq[q].y - uwq[q].x, uwq[q].x);
+                    acc[kb + q] += in[q] ? val : 0.0f;
+                    cnt[kb + q] += in[q] ? cw : 0.0f;
+                }
+            }
+        }
+    }
+
+    if (i >= a.X || j >= a.Y) return;
+    const size_t plane = (size_t)a.X * a.Y;
+    size_t idx = ((size_t)k0 * a.Y + j) * a.X + i;
+#pragma unroll
+    for (int kk = 0; kk < ZC; ++kk) {
+        if (k0 + kk < a.Z) {
+            if (FINAL) {
+                a.vol[idx] = acc[kk] / cnt[kk];
+            } else {
+                a.acc[idx] += acc[kk];
+                a.cnt[idx] += cnt[kk];
+            }
+        }
+        idx += plane;
+    }
+}
+
+__global__ void bp_finalize_kernel(const float* __restrict__ acc, const float* __restrict__ cnt,
+                                   float* __restrict__ vol, size_t n)
+{
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+        vol[t] = acc[t] / cnt[t];
+}
+
+__global__ void fill_zero_kernel(float* __restrict__ p, size_t n)
+{
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
+        p[t] = 0.0f;
+}
+
+static unsigned grid_for(size_t n, unsigned block)
+{
+    return (unsigned)std::min<size_t>((n + block - 1) / block, (size_t)65536);
+}
+
+typedef void (*bp_kernel_t)(BPArgs);
+
+template <int MODE, int ZC, bool FINAL>
+static bp_kernel_t pick_q(int dtype)
+{
+    switch (dtype) {
+    case JH_F32: return bp_kernel<MODE, ZC, JH_F32, FINAL>;
+    case JH_BF16: return bp_kernel<MODE, ZC, JH_BF16, FINAL>;
+    default: return bp_kernel<MODE, ZC, JH_F16, FINAL>;
+    }
+}
+
+template <int MODE, bool FINAL>
+static bp_kernel_t pick_zc(int zc, int dtype)
+{
+    switch (zc) {
+    case 8: return pick_q<MODE, 8, FINAL>(dtype);
+    case 16: return pick_q<MODE, 16, FINAL>(dtype);
+    default: return pick_q<MODE, 32, FINAL>(dtype);
+    }
+}
+
+template <bool FINAL>
+static bp_kernel_t pick_kernel(int mode, int zc, int dtype)
+{
+    switch (mode) {
+    case 0: return pick_zc<0, FINAL>(zc, dtype);
+    case 1: return pick_zc<1, FINAL>(zc, dtype);
+    default: return pick_zc<2, FINAL>(zc, dtype);
+    }
+}
+
+static void launch_bp(jh_bp_plan* p, const void* q, int dtype, int view0, int nviews, float* vol,
+                      bool final_, hipStream_t stream)
+{
+    BPArgs a;
+    a.q = q;
+    a.geo = p->geo.as<Geo>();
+    a.uwp = p->uwp.as<float2>();
+    a.vol = vol;
+    a.acc = p->acc.as<float>();
+    a.cnt = p->cnt.as<float>();
+    a.W = p->W; a.V = p->V; a.X = p->X; a.Y = p->Y; a.Z = p->Z;
+    a.view0 = view0; a.nviews = nviews;
+    a.ntx = (p->X + 63) / 64;
+    a.nty = (p->Y + p->ty - 1) / p->ty;
+    a.ntz = (p->Z + p->zc - 1) / p->zc;
+    const size_t fb = (size_t)p->W * (p->V + 1) * dtype_bytes_pair(dtype);
+    require(fb < (1ull << 31), "backprojection frame too large");
+    a.frame_bytes = (uint32_t)fb;
+    const size_t ntiles = (size_t)a.ntx * a.nty * a.ntz;
+    require(ntiles < (1ull << 31), "backprojection grid too large");
+    const unsigned grid = (unsigned)((ntiles + 7) / 8 * 8);
+    bp_kernel_t k = final_ ? pick_kernel<true>(p->mode, p->zc, dtype) : pick_kernel<false>(p->mode, p->zc, dtype);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(64, p->ty), 0, stream, a);
+    check_launch("bp_kernel");
+}
+
+}  // namespace jh
+
+using namespace jh;
+
+extern "C" {
+
+jh_status jh_bp_create(jh_bp_plan** plan, int device, const double* h_ptm, int nviews, int W,
+                       int V, int X, int Y, int Z, float pxy, float pz)
+{
+    return guard([&] {
+        require(plan != nullptr, "jh_bp_create: plan is NULL");
+        *plan = nullptr;
+        require(h_ptm != nullptr && nviews > 0, "jh_bp_create: no projection matrices");
+        require(W >= 2 && V >= 2, "jh_bp_create: detector must be at least 2 x 2");
+        require(X > 0 && Y > 0 && Z > 0, "jh_bp_create: empty volume");
+        require((size_t)W * V < (1ull << 31), "jh_bp_create: detector too large");
+        require(pxy > 0.0f && pz > 0.0f, "jh_bp_create: pitches must be positive");
+        int n = 0;
+        JH_HIP(hipGetDeviceCount(&n));
+        require(device >= 0 && device < n, "jh_bp_create: no such device");
+        DeviceScope ds(device);
+        auto* p = new jh_bp_plan;
+        try {
+            p->device = device;
+            p->N = nviews; p->W = W; p->V = V; p->X = X; p->Y = Y; p->Z = Z;
+            p->pxy = pxy; p->pz = pz;
+            p->zc = 32;
+            p->ty = 4;
+            p->ptm.alloc(sizeof(double) * 12 * nviews);
+            p->geo.alloc(sizeof(Geo) * nviews);
+            p->uw.alloc(sizeof(float) * (size_t)nviews * W);
+            JH_HIP(hipMemcpy(p->ptm.get(), h_ptm, sizeof(double) * 12 * nviews, hipMemcpyHostToDevice));
+            bp_geometry_kernel<<<(nviews + 63) / 64, 64>>>(p->ptm.as<double>(), nviews, X, Y, Z, pxy, pz,
+                                                            p->geo.as<Geo>());
+            check_launch("bp_geometry_kernel");
+            bp_uweight_kernel<<<dim3((W + 255) / 256, nviews), 256>>>(p->ptm.as<double>(), nviews, W,
+                                                                       p->uw.as<float>());
+            check_launch("bp_uweight_kernel");
+            p->uwp.alloc(sizeof(float2) * (size_t)nviews * W);
+            bp_uwpair_kernel<<<grid_for((size_t)nviews * W, 256), 256>>>(p->uw.as<float>(), nviews, W,
+                                                                          p->uwp.as<float2>());
+            check_launch("bp_uwpair_kernel");
+            std::vector<Geo> g(nviews);
+            JH_HIP(hipMemcpy(g.data(), p->geo.get(), sizeof(Geo) * nviews, hipMemcpyDeviceToHost));
+            bool zinv_u = true, zinv_w = true;
+            for (const Geo& e : g) {
+                if (e.c[3] != 0.0f) zinv_u = false;
+                if (e.c[11] != 0.0f) zinv_w = false;
+            }
+            p->mode = (zinv_u && zinv_w) ? 0 : (zinv_w ? 1 : 2);
+        } catch (...) {
+            delete p;
+            throw;
+        }
+        *plan = p;
+    });
+}
+
+jh_status jh_bp_uweights(jh_bp_plan* p, float* h_out)
+{
+    return guard([&] {
+        require(p != nullptr && h_out != nullptr, "jh_bp_uweights: bad arguments");
+        DeviceScope ds(p->device);
+        JH_HIP(hipMemcpy(h_out, p->uw.get(), sizeof(float) * (size_t)p->N * p->W, hipMemcpyDeviceToHost));
+    });
+}
+
+jh_status jh_bp_stage_bytes(jh_bp_plan* p, int dtype, int nviews, size_t* bytes)
+{
+    return guard([&] {
+        require(p != nullptr && bytes != nullptr && nviews >= 0, "jh_bp_stage_bytes: bad arguments");
+        require(dtype == JH_F32 || dtype == JH_BF16 || dtype == JH_F16, "jh_bp_stage_bytes: unknown dtype");
+        *bytes = (size_t)p->W * (p->V + 1) * nviews * dtype_bytes_pair(dtype);
+    });
+}
+
+jh_status jh_bp_stage(jh_bp_plan* p, const float* d_filtered, int view0, int nviews, int dtype,
+                      void* d_q, jh_stream_t stream)
+{
+    return guard([&] {
+        require(p != nullptr && d_filtered != nullptr && d_q != nullptr, "jh_bp_stage: bad arguments");
+        require(view0 >= 0 && nviews >= 0 && view0 + nviews <= p->N, "jh_bp_stage: view range outside the scan");
+        require(dtype == JH_F32 || dtype == JH_BF16 || dtype == JH_F16, "jh_bp_stage: unknown dtype");
+        if (nviews == 0) return;
+        DeviceScope ds(p->device);
+        const size_t total = (size_t)p->W * (p->V + 1) * nviews;
+        const float* uw = p->uw.as<float>() + (size_t)view0 * p->W;
+        const unsigned grid = grid_for(total, 256);
+        hipStream_t s = as_stream(stream);
+        if (dtype == JH_F32) bp_stage_kernel<JH_F32><<<grid, 256, 0, s>>>(d_filtered, uw, p->W, p->V, nviews, d_q);
+        else if (dtype == JH_BF16) bp_stage_kernel<JH_BF16><<<grid, 256, 0, s>>>(d_filtered, uw, p->W, p->V, nviews, d_q);
+        else bp_stage_kernel<JH_F16><<<grid, 256, 0, s>>>(d_filtered, uw, p->W, p->V, nviews, d_q);
+        check_launch("bp_stage_kernel");
+    });
+}
+
+jh_status jh_bp_run(jh_bp_plan* p, const void* d_q, int dtype, float* d_vol, jh_stream_t stream)
+{
+    return guard([&] {
+        require(p != nullptr && d_q != nullptr && d_vol != nullptr, "jh_bp_run: bad arguments");
+        require(dtype == JH_F32 || dtype == JH_BF16 || dtype == JH_F16, "jh_bp_run: unknown dtype");
+        DeviceScope ds(p->device);
+        launch_bp(p, d_q, dtype, 0, p->N, d_vol, true, as_stream(stream));
+    });
+}
+
+static void ensure_accumulators(jh_bp_plan* p)
+{
+    const size_t n = (size_t)p->X * p->Y * p->Z;
+    if (!p->acc.get()) p->acc.alloc(sizeof(float) * n);
+    if (!p->cnt.get()) p->cnt.alloc(sizeof(float) * n);
+}
+
+jh_status jh_bp_reset(jh_bp_plan* p, jh_stream_t stream)
+{
+    return guard([&] {
+        require(p != nullptr, "jh_bp_reset: plan is NULL");
+        DeviceScope ds(p->device);
+        ensure_accumulators(p);
+        const size_t n = (size_t)p->X * p->Y * p->Z;
+        hipStream_t s = as_stream(stream);
+        fill_zero_kernel<<<grid_for(n, 256), 256, 0, s>>>(p->acc.as<float>(), n);
+        fill_zero_kernel<<<grid_for(n, 256), 256, 0, s>>>(p->cnt.as<float>(), n);
+        check_launch("fill_zero_kernel");
+    });
+}
+
+jh_status jh_bp_accumulate(jh_bp_plan* p, const void* d_q, int dtype, int view0, int nviews,
+                           jh_stream_t stream)
+{
+    return guard([&] {
+        require(p != nullptr && d_q != nullptr, "jh_bp_accumulate: bad arguments");
+        require(dtype == JH_F32 || dtype == JH_BF16 || dtype == JH_F16, "jh_bp_accumulate: unknown dtype");
+        require(view0 >= 0 && nviews >= 0 && view0 + nviews <= p->N, "jh_bp_accumulate: view range outside the scan");
+        require(p->acc.get() != nullptr, "jh_bp_accumulate: call jh_bp_reset first");
+        if (nviews == 0) return;
+        DeviceScope ds(p->device);
+        launch_bp(p, d_q, dtype, view0, nviews, nullptr, false, as_stream(stream));
+    });
+}
+
+jh_status jh_bp_finalize(jh_bp_plan* p, float* d_vol, int z0, int nz, jh_stream_t stream)
+{
+    return guard([&] {
+        require(p != nullptr && d_vol != nullptr, "jh_bp_finalize: bad arguments");
+        require(z0 >= 0 && nz >= 0 && z0 + nz <= p->Z, "jh_bp_finalize: slice range outside the volume");
+        require(p->acc.get() != nullptr, "jh_bp_finalize: call jh_bp_reset first");
+        if (nz == 0) return;
+        DeviceScope ds(p->device);
+        const size_t plane = (size_t)p->X * p->Y;
+        const size_t n = plane * nz;
+        bp_finalize_kernel<<<grid_for(n, 256), 256, 0, as_stream(stream)>>>(
+            p->acc.as<float>() + plane * z0, p->cnt.as<float>() + plane * z0, d_vol, n);
+        check_launch("bp_finalize_kernel");
+    });
+}
+
+jh_status jh_bp_set_tile(jh_bp_plan* p, int zc, int ty)
+{
+    return guard([&] {
+        require(p != nullptr, "jh_bp_set_tile: plan is NULL");
+        require(zc == 0 || zc == 8 || zc == 16 || zc == 32, "jh_bp_set_tile: zc must be 8, 16 or 32");
+        require(ty == 0 || ty == 1 || ty == 2 || ty == 4, "jh_bp_set_tile: ty must be 1, 2 or 4");
+        if (zc) p->zc = zc;
+        if (ty) p->ty = ty;
+    });
+}
+
+jh_status jh_bp_get_tile(jh_bp_plan* p, int* zc, int* ty)
+{
+    return guard([&] {
+        require(p != nullptr && zc != nullptr && ty != nullptr, "jh_bp_get_tile: bad arguments");
+        *zc = p->zc;
+        *ty = p->ty;
+    });
+}
+
+jh_status jh_bp_geometry_mode(jh_bp_plan* p, int* mode)
+{
+    return guard([&] {
+        require(p != nullptr && mode != nullptr, "jh_bp_geometry_mode: bad arguments");
+        *mode = p->mode;
+    });
+}
+
+jh_status jh_bp_destroy(jh_bp_plan* p)
+{
+    return guard([&] {
+        if (!p) return;
+        DeviceScope ds(p->device);
+        delete p;
+    });
+}
+
+}  // extern "C"

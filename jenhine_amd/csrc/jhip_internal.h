@@ -1,0 +1,156 @@
+// Internal declarations shared by the translation units of libjhip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/jhip.h"
+
+namespace jh {
+
+// Error raised inside the library and turned into a jh_status at the ABI.
+struct Error {
+    int code;
+    std::string msg;
+};
+
+void set_last_error(const std::string& msg);
+
+[[noreturn]] inline void fail(int code, const std::string& msg) { throw Error{code, msg}; }
+
+inline void check_hip(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) fail(JH_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define JH_HIP(expr) ::jh::check_hip((expr), #expr)
+
+inline void require(bool ok, const std::string& msg)
+{
+    if (!ok) fail(JH_ERR_INVALID, msg);
+}
+
+// Runs f(); converts library errors and std exceptions to a status.
+template <class F>
+jh_status guard(F&& f)
+{
+    try {
+        f();
+        set_last_error("");
+        return JH_OK;
+    } catch (const Error& e) {
+        set_last_error(e.msg);
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed");
+        return JH_ERR_OOM;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return JH_ERR_INVALID;
+    } catch (...) {
+        set_last_error("unknown error");
+        return JH_ERR_INVALID;
+    }
+}
+
+// Owned device allocation.  An allocation failure raises JH_ERR_OOM with the
+// message prefix "HIP failed to malloc " (the host's part-retry keys on it).
+class DevBuf {
+public:
+    DevBuf() = default;
+    explicit DevBuf(size_t bytes) { alloc(bytes); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept
+    {
+        if (this != &o) { release(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t bytes)
+    {
+        release();
+        if (bytes == 0) return;
+        if (hipMalloc(&p_, bytes) != hipSuccess || p_ == nullptr) {
+            (void)hipGetLastError();
+            p_ = nullptr;
+            fail(JH_ERR_OOM, "HIP failed to malloc " + std::to_string(bytes) + " bytes");
+        }
+        n_ = bytes;
+    }
+    void release()
+    {
+        if (p_) (void)hipFree(p_);
+        p_ = nullptr;
+        n_ = 0;
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(p_); }
+    void* get() const { return p_; }
+    size_t size() const { return n_; }
+
+private:
+    void* p_ = nullptr;
+    size_t n_ = 0;
+};
+
+// Makes `device` current for the lifetime of the scope.
+class DeviceScope {
+public:
+    explicit DeviceScope(int device)
+    {
+        JH_HIP(hipGetDevice(&prev_));
+        if (device != prev_) JH_HIP(hipSetDevice(device));
+        changed_ = device != prev_;
+    }
+    ~DeviceScope()
+    {
+        if (changed_) (void)hipSetDevice(prev_);
+    }
+
+private:
+    int prev_ = 0;
+    bool changed_ = false;
+};
+
+inline hipStream_t as_stream(jh_stream_t s) { return static_cast<hipStream_t>(s); }
+
+inline void check_launch(const char* what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) fail(JH_ERR_HIP, std::string("launch of ") + what + ": " + hipGetErrorString(e));
+}
+
+inline int dtype_bytes_pair(int dtype)
+{
+    // Storage bytes per detector pixel in the pair layout (p[u], p[u+1]).
+    return dtype == JH_F32 ? 8 : 4;
+}
+
+// Host filter design (FP64): H[k], k = 0..L/2, incl. the 2*pi*f_N gain.
+std::vector<double> design_filter(int W, double hsr, double cutStart, double cutEnd,
+                                  const std::vector<double>& adjPts,
+                                  const std::vector<double>& adjDB, int window);
+
+inline int fft_length(int W)
+{
+    int L = 1;
+    while (L < W * 2 - 1) L *= 2;
+    return L;
+}
+
+}  // namespace jh
+
+// Opaque plan types of the C-ABI.
+struct jh_pre_plan {
+    int device;
+    int inW, inH, bUp, bDown, bLeft, bRight, outW, outH;
+    float brightField;
+    float bh[3];
+    bool doBH;
+};

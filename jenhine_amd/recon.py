@@ -1,0 +1,223 @@
+"""Python host of the HIP back end: the three reconstruction cores over the
+C-ABI, with torch tensors as device memory.
+
+The class names follow the reference cores they replace:
+
+* ``PreProcessingCore`` -- PreProcessingAgent/PreProcessingCore.cpp
+* ``FilterCore``        -- FilterAgent/FilterCore.cpp (+ PreFilterWeightGenerator)
+* ``BPCore``            -- BPAgent/BPCore.cpp and BPCUDAAgent/BPCUDACore.cpp
+* ``FDK``               -- the chain PreProcessingCore -> FilterCore -> BPCore
+
+All work is enqueued on torch's current stream.  Every method raises on a
+failed call; none falls back to a CPU path.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import call, DTYPES, JH_FILTER_FP32, JH_FILTER_MFMA_BF16X3
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _dev_index(device):
+    d = torch.device(device)
+    return d.index if d.index is not None else torch.cuda.current_device()
+
+
+def _require_cuda(t, name, dtype=None):
+    if t is None:
+        return
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a device tensor")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if dtype is not None and t.dtype != dtype:
+        raise ValueError(f"{name} must be {dtype}, got {t.dtype}")
+
+
+class PreProcessingCore:
+    """u16 raw frames -> line integrals (PreProcessingCore.cpp:15-86)."""
+
+    def __init__(self, in_w, in_h, borders=(0, 0, 0, 0), out_w=None, out_h=None, bright_field=4000.0,
+                 bh=(1.0, 0.0, 0.0), device="cuda:0"):
+        bu, bd, bl, br = borders
+        self.out_w = out_w if out_w is not None else in_w - bl - br
+        self.out_h = out_h if out_h is not None else in_h - bu - bd
+        self.in_w, self.in_h = in_w, in_h
+        self.device = torch.device(device)
+        h = ctypes.c_void_p()
+        bhv = (ctypes.c_float * 3)(*bh)
+        call("jh_pre_create", ctypes.byref(h), _dev_index(device), in_w, in_h, bu, bd, bl, br, self.out_w,
+             self.out_h, ctypes.c_float(bright_field), bhv)
+        self._h = h
+
+    def process(self, raw, dark=None, gain=None, out=None):
+        """raw: (N, in_h, in_w) int16/uint16 device tensor holding u16 counts."""
+        _require_cuda(raw, "raw")
+        if raw.element_size() != 2:
+            raise ValueError("raw must hold 16-bit counts")
+        n = raw.shape[0]
+        if out is None:
+            out = torch.empty((n, self.out_h, self.out_w), dtype=torch.float32, device=raw.device)
+        _require_cuda(out, "out", torch.float32)
+        _require_cuda(dark, "dark")
+        _require_cuda(gain, "gain", torch.float32)
+        call("jh_pre_run", self._h, _ptr(raw), _ptr(dark), _ptr(gain), _ptr(out), n, _stream())
+        return out
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.lib().jh_pre_destroy(self._h)
+            self._h = None
+
+
+class FilterCore:
+    """Row ramp filter with pre-filter weight (FilterCore.cpp:13-222)."""
+
+    def __init__(self, W, V, ptm, dso, half_sample_rate, cut_start=0.0, cut_end=0.0, adj_pts=(),
+                 adj_db=(), window=0, mode="fp32", device="cuda:0"):
+        self.W, self.V = W, V
+        ptm = np.ascontiguousarray(ptm, np.float64).reshape(-1, 12)
+        self.n_views = ptm.shape[0]
+        m = {"fp32": JH_FILTER_FP32, "mfma": JH_FILTER_MFMA_BF16X3}[mode]
+        n = len(adj_pts)
+        pts = (ctypes.c_float * max(n, 1))(*adj_pts)
+        db = (ctypes.c_float * max(n, 1))(*adj_db)
+        h = ctypes.c_void_p()
+        call("jh_filter_create", ctypes.byref(h), _dev_index(device), W, V, self.n_views,
+             ptm.ctypes.data_as(ctypes.c_void_p), ctypes.c_float(dso), ctypes.c_float(half_sample_rate),
+             ctypes.c_float(cut_start), ctypes.c_float(cut_end), pts if n else None, db if n else None, n,
+             window, m)
+        self._h = h
+
+    def response(self):
+        L = ctypes.c_int()
+        H = np.zeros(4 * self.W + 8, np.float64)
+        call("jh_filter_response", self._h, H.ctypes.data_as(ctypes.c_void_p), ctypes.byref(L))
+        return H[: L.value // 2 + 1].copy(), L.value
+
+    def preweight(self, view):
+        out = torch.empty((self.V, self.W), dtype=torch.float32, device="cuda")
+        call("jh_filter_preweight", self._h, view, _ptr(out), _stream())
+        return out
+
+    def process(self, frames, view0=0, out=None):
+        _require_cuda(frames, "frames", torch.float32)
+        n = frames.shape[0]
+        if out is None:
+            out = torch.empty_like(frames)
+        _require_cuda(out, "out", torch.float32)
+        call("jh_filter_run", self._h, _ptr(frames), _ptr(out), view0, n, _stream())
+        return out
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.lib().jh_filter_destroy(self._h)
+            self._h = None
+
+
+class BPCore:
+    """Normalised voxel-driven backprojection (BPCore.cpp:15-205)."""
+
+    def __init__(self, ptm, W, V, X, Y, Z, pxy, pz, device="cuda:0"):
+        ptm = np.ascontiguousarray(ptm, np.float64).reshape(-1, 12)
+        self.n_views = ptm.shape[0]
+        self.W, self.V, self.X, self.Y, self.Z = W, V, X, Y, Z
+        h = ctypes.c_void_p()
+        call("jh_bp_create", ctypes.byref(h), _dev_index(device), ptm.ctypes.data_as(ctypes.c_void_p),
+             self.n_views, W, V, X, Y, Z, ctypes.c_float(pxy), ctypes.c_float(pz))
+        self._h = h
+
+    def uweights(self):
+        out = np.zeros((self.n_views, self.W), np.float32)
+        call("jh_bp_uweights", self._h, out.ctypes.data_as(ctypes.c_void_p))
+        return out
+
+    def geometry_mode(self):
+        m = ctypes.c_int()
+        call("jh_bp_geometry_mode", self._h, ctypes.byref(m))
+        return m.value
+
+    def set_tile(self, zc=0, ty=0):
+        call("jh_bp_set_tile", self._h, zc, ty)
+
+    def tile(self):
+        zc, ty = ctypes.c_int(), ctypes.c_int()
+        call("jh_bp_get_tile", self._h, ctypes.byref(zc), ctypes.byref(ty))
+        return zc.value, ty.value
+
+    def stage_bytes(self, dtype, n_views):
+        b = ctypes.c_size_t()
+        call("jh_bp_stage_bytes", self._h, DTYPES[dtype], n_views, ctypes.byref(b))
+        return b.value
+
+    def stage(self, filtered, dtype="f32", view0=0, out=None):
+        """q = filtered * uW in the backprojector's layout (a byte tensor)."""
+        _require_cuda(filtered, "filtered", torch.float32)
+        n = filtered.shape[0]
+        if out is None:
+            out = torch.empty(self.stage_bytes(dtype, n), dtype=torch.uint8, device=filtered.device)
+        call("jh_bp_stage", self._h, _ptr(filtered), view0, n, DTYPES[dtype], _ptr(out), _stream())
+        return out
+
+    def run(self, q, dtype="f32", out=None):
+        _require_cuda(q, "q")
+        if out is None:
+            out = torch.empty((self.Z, self.Y, self.X), dtype=torch.float32, device=q.device)
+        _require_cuda(out, "out", torch.float32)
+        call("jh_bp_run", self._h, _ptr(q), DTYPES[dtype], _ptr(out), _stream())
+        return out
+
+    def reset(self):
+        call("jh_bp_reset", self._h, _stream())
+
+    def accumulate(self, q, dtype, view0, n_views):
+        _require_cuda(q, "q")
+        call("jh_bp_accumulate", self._h, _ptr(q), DTYPES[dtype], view0, n_views, _stream())
+
+    def finalize(self, z0=0, nz=None, out=None):
+        nz = self.Z - z0 if nz is None else nz
+        if out is None:
+            out = torch.empty((nz, self.Y, self.X), dtype=torch.float32, device="cuda")
+        call("jh_bp_finalize", self._h, _ptr(out), z0, nz, _stream())
+        return out
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.lib().jh_bp_destroy(self._h)
+            self._h = None
+
+
+class FDK:
+    """PreProcessingCore -> FilterCore -> BPCore on one device."""
+
+    def __init__(self, scan, recon, ptm, filter_mode="fp32", storage="f32", device="cuda:0"):
+        self.scan, self.recon, self.storage = scan, recon, storage
+        W, V = scan.NumUsedDetsU, scan.NumUsedDetsV
+        self.pre = PreProcessingCore(scan.NumDetsU, scan.NumDetsV,
+                                     (scan.BorderSizeUp, scan.BorderSizeDown, scan.BorderSizeLeft,
+                                      scan.BorderSizeRight), W, V, scan.BrightField,
+                                     scan.BeamHardeningParams, device)
+        self.filter = FilterCore(W, V, ptm, scan.DSO, scan.HalfSampleRate, recon.FilterCutOffStart,
+                                 recon.FilterCutOffEnd, recon.FilterAdjustPoints, recon.FilterAdjustLevelInDB,
+                                 recon.FilterWindow, filter_mode, device)
+        self.bp = BPCore(ptm, W, V, recon.NumPixelsX, recon.NumPixelsY, recon.NumPixelsZ, recon.PitchXY,
+                         recon.PitchZ, device)
+
+    def __call__(self, raw, dark=None, gain=None, return_stages=False):
+        p = self.pre.process(raw, dark, gain)
+        f = self.filter.process(p)
+        q = self.bp.stage(f, self.storage)
+        vol = self.bp.run(q, self.storage)
+        if return_stages:
+            return vol, dict(pre=p, filt=f, q=q)
+        return vol

@@ -1,0 +1,205 @@
+"""ctypes front end of the CPU oracle (test infrastructure only).
+
+Only tests/, ``__graft_entry__.smoke()`` and the CPU-baseline leg of bench.py
+import this module, and only as the checker; the shipped library never does.
+Every function here is a thin wrapper around ``libjh_oracle.so`` (built from
+jh_oracle.c by the Makefile beside it).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "libjh_oracle.so")
+        if not os.path.exists(path):
+            build()
+        _LIB = ctypes.CDLL(path)
+    return _LIB
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _dt(fp64):
+    return np.float64 if fp64 else np.float32
+
+
+def _sfx(fp64):
+    return "_f64" if fp64 else "_f32"
+
+
+def _real(fp64):
+    return ctypes.c_double if fp64 else ctypes.c_float
+
+
+def fft_length(W):
+    return int(lib().jo_fft_length(ctypes.c_int(W)))
+
+
+def axis(n, pitch, fp64=False):
+    out = np.zeros(n, _dt(fp64))
+    getattr(lib(), "jo_axis" + _sfx(fp64))(ctypes.c_int(n), _real(fp64)(pitch), _p(out))
+    return out
+
+
+def uweights(ptm, W, fp64=False):
+    ptm = np.ascontiguousarray(ptm, _dt(fp64)).reshape(-1, 12)
+    out = np.zeros((ptm.shape[0], W), _dt(fp64))
+    getattr(lib(), "jo_uweights" + _sfx(fp64))(_p(ptm), ctypes.c_int(ptm.shape[0]), ctypes.c_int(W), _p(out))
+    return out
+
+
+def rfft(x, fp64=False):
+    x = np.ascontiguousarray(x, _dt(fp64))
+    n = x.size
+    out = np.zeros(n // 2 * 2 + 2, _dt(fp64))
+    getattr(lib(), "jo_rfft" + _sfx(fp64))(ctypes.c_int(n), _p(x), _p(out))
+    return out
+
+
+def irfft(ccs, n, fp64=False):
+    ccs = np.ascontiguousarray(ccs, _dt(fp64))
+    out = np.zeros(n, _dt(fp64))
+    getattr(lib(), "jo_irfft" + _sfx(fp64))(ctypes.c_int(n), _p(ccs), _p(out))
+    return out
+
+
+def resize_linear(src, sroi, dshape, fp64=False):
+    """src full image, sroi=(x, y, w, h); returns a dshape=(h, w) image."""
+    src = np.ascontiguousarray(src, _dt(fp64))
+    x, y, w, h = sroi
+    dh, dw = dshape
+    out = np.zeros((dh, dw), _dt(fp64))
+    base = src.ctypes.data + (y * src.shape[1] + x) * src.itemsize
+    getattr(lib(), "jo_resize_linear" + _sfx(fp64))(
+        ctypes.c_void_p(base), ctypes.c_int(src.shape[1]), ctypes.c_int(w), ctypes.c_int(h),
+        _p(out), ctypes.c_int(dw), ctypes.c_int(dw), ctypes.c_int(dh))
+    return out
+
+
+def filter_design(W, half_sample_rate, cut_start=0.0, cut_end=0.0, adj_pts=(), adj_db=(),
+                  window=0, fp64=False):
+    L = fft_length(W)
+    H = np.zeros(L // 2 + 1, _dt(fp64))
+    pts = np.ascontiguousarray(adj_pts, _dt(fp64))
+    db = np.ascontiguousarray(adj_db, _dt(fp64))
+    assert pts.size == db.size
+    R = _real(fp64)
+    getattr(lib(), "jo_filter_design" + _sfx(fp64))(
+        ctypes.c_int(W), R(half_sample_rate), R(cut_start), R(cut_end),
+        _p(pts) if pts.size else None, _p(db) if db.size else None, ctypes.c_int(pts.size),
+        ctypes.c_int(window), _p(H))
+    return H
+
+
+def prefilter_weight(ptm, W, V, dso, detector_on_right=False, fp64=False):
+    ptm = np.ascontiguousarray(ptm, _dt(fp64)).reshape(12)
+    L = fft_length(W)
+    out = np.zeros((V, L), _dt(fp64))
+    getattr(lib(), "jo_prefilter_weight" + _sfx(fp64))(
+        _p(ptm), ctypes.c_int(W), ctypes.c_int(V), ctypes.c_int(int(detector_on_right)),
+        _real(fp64)(dso), _p(out))
+    return out
+
+
+def preweight_schedule(nviews):
+    nb = (nviews + 8) // 9
+    src = np.zeros(nb, np.int32)
+    blk = np.zeros(nviews, np.int32)
+    n = lib().jo_preweight_schedule(ctypes.c_int(nviews), _p(src), _p(blk))
+    assert n == nb
+    return src, blk
+
+
+def preprocess(raw, out_hw=None, borders=(0, 0, 0, 0), bright_field=4000.0, bh=(1.0, 0.0, 0.0),
+               dark=None, gain=None, fp64=False, nthreads=8):
+    """raw: (N, inH, inW) uint16. borders = (up, down, left, right)."""
+    raw = np.ascontiguousarray(raw, np.uint16)
+    N, inH, inW = raw.shape
+    bu, bd, bl, br = borders
+    if out_hw is None:
+        out_hw = (inH - bu - bd, inW - bl - br)
+    outH, outW = out_hw
+    out = np.zeros((N, outH, outW), _dt(fp64))
+    bhv = np.ascontiguousarray(bh, np.float64)
+    if dark is not None:
+        dark = np.ascontiguousarray(dark, np.uint16)
+    if gain is not None:
+        gain = np.ascontiguousarray(gain, np.float32)
+    rc = lib().jo_preprocess(_p(raw), _p(dark), _p(gain), inW, inH, bu, bd, bl, br, outW, outH,
+                             ctypes.c_double(bright_field), _p(bhv), _p(out), N, int(fp64), nthreads)
+    assert rc == 0
+    return out
+
+
+def filter_rows(frames, H, preweights=None, pw_index=None, fp64=False, nthreads=8):
+    """frames: (N, V, W). preweights: (nblocks, V, L) or None."""
+    frames = np.ascontiguousarray(frames, _dt(fp64))
+    N, V, W = frames.shape
+    H = np.ascontiguousarray(H, _dt(fp64))
+    out = np.zeros_like(frames)
+    pw = None if preweights is None else np.ascontiguousarray(preweights, _dt(fp64))
+    idx = None if pw_index is None else np.ascontiguousarray(pw_index, np.int32)
+    rc = lib().jo_filter(_p(frames), _p(pw), _p(idx), _p(H), W, V, N, _p(out), int(fp64), nthreads)
+    assert rc == 0
+    return out
+
+
+def backproject(proj, ptm, vol_shape, pitch_xy, pitch_z, z0=0, nz=None, fp64=False, nthreads=8,
+                return_parts=False):
+    """proj: (N, V, W) filtered projections; vol_shape = (Z, Y, X).
+    Returns vol[z0:z0+nz] (and acc, count when return_parts)."""
+    proj = np.ascontiguousarray(proj, _dt(fp64))
+    N, V, W = proj.shape
+    ptm = np.ascontiguousarray(ptm, _dt(fp64)).reshape(N, 12)
+    Z, Y, X = vol_shape
+    if nz is None:
+        nz = Z - z0
+    vol = np.zeros((nz, Y, X), _dt(fp64))
+    acc = np.zeros_like(vol)
+    cnt = np.zeros_like(vol)
+    R = _real(fp64)
+    rc = getattr(lib(), "jo_bp" + _sfx(fp64))(
+        _p(proj), _p(ptm), N, W, V, X, Y, Z, R(pitch_xy), R(pitch_z), z0, nz,
+        _p(vol), _p(acc), _p(cnt), nthreads)
+    assert rc == 0
+    if return_parts:
+        return vol, acc, cnt
+    return vol
+
+
+def fdk(raw, ptm, scan, recon, fp64=False, nthreads=8, z0=0, nz=None, dark=None, gain=None,
+        return_stages=False):
+    """Full oracle chain PreProcessingCore -> FilterCore -> BPCore.
+
+    scan: dict with W, V, bright_field, bh, hsr, dso, borders (optional).
+    recon: dict with X, Y, Z, pxy, pz, cut_start, cut_end, window."""
+    p = preprocess(raw, borders=scan.get("borders", (0, 0, 0, 0)), bright_field=scan["bright_field"],
+                   bh=scan.get("bh", (1.0, 0.0, 0.0)), dark=dark, gain=gain, fp64=fp64,
+                   nthreads=nthreads)
+    N, V, W = p.shape
+    H = filter_design(W, scan["hsr"], recon.get("cut_start", 0.0), recon.get("cut_end", 0.0),
+                      recon.get("adj_pts", ()), recon.get("adj_db", ()), recon.get("window", 0),
+                      fp64=fp64)
+    src, blk = preweight_schedule(N)
+    ptm = np.asarray(ptm).reshape(N, 12)
+    pws = np.stack([prefilter_weight(ptm[s], W, V, scan["dso"], fp64=fp64) for s in src])
+    f = filter_rows(p, H, pws, blk, fp64=fp64, nthreads=nthreads)
+    vol = backproject(f, ptm, (recon["Z"], recon["Y"], recon["X"]), recon["pxy"], recon["pz"],
+                      z0=z0, nz=nz, fp64=fp64, nthreads=nthreads)
+    if return_stages:
+        return vol, dict(pre=p, H=H, pw=pws, pw_index=blk, filt=f)
+    return vol
