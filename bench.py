@@ -1,0 +1,199 @@
+#!/usr/bin/env python3
+"""Benchmark of the FDK back end on one MI355X.
+
+Workload (dental512): 512^3 FP32 volume at 0.15 mm from 600 views of 768 x 768
+uint16 at 0.2 mm, SOD 350 mm, SDD 600 mm, Shepp-Logan window, cubic beam
+hardening, seeded 40-ellipsoid + tooth phantom, per-pixel gain and dark.
+
+One step = the whole reconstruction on the device with the raw counts already
+resident in HBM: preprocess -> row filter -> stage -> backprojection ->
+normalised volume.  Two forms are timed:
+  * fp32: FP32 Toeplitz filter on the vector ALUs, FP32 projection storage;
+  * bf16: MFMA hi/lo split-BF16 filter, BF16 projection storage (headline).
+value = voxel-updates per second of the headline form over the whole step
+(X*Y*Z*N / step time), in GUPS.
+
+The roofline object covers the dominant kernel (bp_kernel) timed with events
+on the stream it runs on; its algorithmic bytes are 8 B per voxel-update (an
+FP32 accumulator read + write per update, the reference's per-view
+formulation).  cpu_baseline times the CPU oracle's FP32 backprojection on a
+bounded sample of slices on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def rel_rmse(a, ref):
+    a = np.asarray(a, np.float64)
+    ref = np.asarray(ref, np.float64)
+    m = np.isfinite(a) & np.isfinite(ref)
+    return float(np.sqrt(np.mean((a[m] - ref[m]) ** 2)) / np.sqrt(np.mean(ref[m] ** 2)))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--views", type=int, default=None, help="override the view count (smoke runs)")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the FP64 oracle slices")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-slices", type=int, default=16)
+    ap.add_argument("--acc-slices", type=int, default=4)
+    ap.add_argument("--kernel-only", action="store_true", help="only the bp kernel loop (profiling)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+    from jenhine_amd import configs
+    from jenhine_amd.recon import FDK
+
+    if not torch.cuda.is_available():
+        sys.exit("bench.py needs a GPU")
+    dev = torch.device("cuda:0")
+    w = configs.DENTAL512
+    if args.views:
+        w = w.scaled(n_views=args.views)
+    t0 = time.time()
+    d = w.data()
+    log(f"synthetic data {w.n_views} x {w.n_v} x {w.n_u} in {time.time() - t0:.1f} s")
+    raw = torch.from_numpy(d["raw"].view(np.int16)).to(dev)
+    dark = torch.from_numpy(d["dark"].view(np.int16)).to(dev)
+    gain = torch.from_numpy(d["gain"]).to(dev)
+    scan, rec = w.scan_params(), w.recon_params()
+    forms = {"fp32": FDK(scan, rec, d["ptm"], "fp32", "f32", dev),
+             "bf16": FDK(scan, rec, d["ptm"], "mfma", "bf16", dev)}
+    units = w.gups_units
+
+    results = {}
+    vols = {}
+    for name, fdk in forms.items():
+        torch.cuda.synchronize()
+        for _ in range(args.warmup):
+            fdk(raw, dark, gain)
+        torch.cuda.synchronize()
+        # whole step, and the bp kernel alone (events on the same stream)
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            vol = fdk(raw, dark, gain)
+        torch.cuda.synchronize()
+        step = (time.perf_counter() - t) / args.steps
+        # stages of one more pass, timed with events
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+        ev[0].record()
+        p = fdk.pre.process(raw, dark, gain)
+        ev[1].record()
+        f = fdk.filter.process(p)
+        ev[2].record()
+        q = fdk.bp.stage(f, fdk.storage)
+        ev[3].record()
+        vol = fdk.bp.run(q, fdk.storage)
+        ev[4].record()
+        torch.cuda.synchronize()
+        st = [ev[i].elapsed_time(ev[i + 1]) for i in range(4)]
+        # bp kernel alone, averaged over `steps` launches
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.steps):
+            fdk.bp.run(q, fdk.storage, out=vol)
+        e1.record()
+        torch.cuda.synchronize()
+        bp_ms = e0.elapsed_time(e1) / args.steps
+        vols[name] = vol.cpu().numpy()
+        results[name] = dict(step_ms=step * 1e3, gups=units / step / 1e9, bp_ms=bp_ms,
+                             bp_gups=units / (bp_ms * 1e-3) / 1e9,
+                             stage_ms=dict(pre=st[0], filter=st[1], stage=st[2], bp=st[3]))
+        log(name, json.dumps(results[name]))
+        del p, f, q
+
+    # accuracy on a few slices against the FP64 oracle, and the CPU baseline
+    acc = {}
+    cpu = None
+    if not (args.no_accuracy and args.no_cpu):
+        from oracle import oracle as o
+        sc = dict(bright_field=scan.BrightField, hsr=scan.HalfSampleRate, dso=scan.DSO,
+                  bh=scan.BeamHardeningParams)
+        rc = dict(X=w.X, Y=w.Y, Z=w.Z, pxy=w.pxy, pz=w.pz, cut_start=rec.FilterCutOffStart,
+                  cut_end=rec.FilterCutOffEnd, window=rec.FilterWindow)
+        nth = min(os.cpu_count() or 1, 16)
+        if not args.no_accuracy:
+            z0 = w.Z // 2 - args.acc_slices // 2
+            t = time.time()
+            v64 = o.fdk(d["raw"], d["ptm"], sc, rc, fp64=True, nthreads=nth, z0=z0, nz=args.acc_slices,
+                        dark=d["dark"], gain=d["gain"])
+            log(f"FP64 oracle slices in {time.time() - t:.1f} s")
+            for name in vols:
+                g = vols[name][z0:z0 + args.acc_slices]
+                acc[name] = dict(rel_rmse_vs_fp64=rel_rmse(g, v64),
+                                 max_abs_vs_fp64=float(np.nanmax(np.abs(g.astype(np.float64) - v64))),
+                                 slices=[z0, z0 + args.acc_slices])
+            log("accuracy", json.dumps(acc))
+        if not args.no_cpu:
+            # FP32-ref oracle: preprocess + filter all views, then time its
+            # backprojection of cpu_slices centre slices
+            p32 = o.preprocess(d["raw"], bright_field=sc["bright_field"], bh=sc["bh"], dark=d["dark"],
+                               gain=d["gain"], nthreads=nth)
+            H = o.filter_design(w.n_u, sc["hsr"], rc["cut_start"], rc["cut_end"], window=rc["window"])
+            src, blk = o.preweight_schedule(w.n_views)
+            pws = np.stack([o.prefilter_weight(d["ptm"][s], w.n_u, w.n_v, sc["dso"]) for s in src])
+            f32 = o.filter_rows(p32, H, pws, blk, nthreads=nth)
+            del p32
+            z0 = w.Z // 2 - args.cpu_slices // 2
+            t = time.perf_counter()
+            v32 = o.backproject(f32, d["ptm"], (w.Z, w.Y, w.X), w.pxy, w.pz, z0=z0, nz=args.cpu_slices,
+                                nthreads=nth)
+            dt = time.perf_counter() - t
+            n_upd = w.X * w.Y * args.cpu_slices * w.n_views
+            cpu = dict(value=n_upd / dt / 1e9, unit="GUPS", cores=nth, kind="port",
+                       sample=f"CPU oracle FP32-ref backprojection of {args.cpu_slices} centre slices "
+                              f"(512 x 512 x {args.cpu_slices}) x {w.n_views} views, {nth} threads, "
+                              f"{dt:.1f} s")
+            acc["fp32_vs_fp32ref_rel_rmse"] = rel_rmse(vols["fp32"][z0:z0 + args.cpu_slices], v32)
+            log("cpu", json.dumps(cpu), "fp32 vs FP32-ref", acc["fp32_vs_fp32ref_rel_rmse"])
+
+    head = results["bf16"]
+    bp_bytes = 8.0 * units
+    achieved = bp_bytes / (head["bp_ms"] * 1e-3) / 1e9
+    line = {
+        "metric": "backprojection GUPS at 512^3 x 600 views; FP32 vs BF16/MFMA path at <=1e-3 rel-RMSE",
+        "value": head["gups"],
+        "unit": "GUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["step_ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16-storage/f32-accumulate",
+        "data": "synthetic (seeded ellipsoid phantom, analytic line integrals, Poisson counts)",
+        "config": {"workload": f"dental512: FDK {w.X}x{w.Y}x{w.Z} from {w.n_views} views of {w.n_u}x{w.n_v} u16",
+                   "global_batch": 1, "views": w.n_views, "detector": [w.n_u, w.n_v],
+                   "volume": [w.X, w.Y, w.Z], "parallelism": "single-gpu"},
+        "forms": results,
+        "accuracy": acc,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
+                     "frac": achieved / 8000.0, "traffic": None,
+                     "kernel": "bp_kernel (bf16 pair storage)", "bytes_per_update": 8},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

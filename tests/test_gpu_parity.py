@@ -1,0 +1,262 @@
+"""GPU parity of the HIP back end against the CPU oracle, through the C-ABI.
+
+Tolerances:
+  * preprocessing: integer steps bit-exact; float result within 2 ulp of the
+    FP32-ref oracle (the only non-exact step is ln: OCML logf vs glibc logf);
+  * filter FP32: rel-RMSE <= 2e-6 vs the FP64 oracle; MFMA hi/lo BF16:
+    rel-RMSE <= 2e-5;
+  * backprojection FP32: rel-RMSE <= 1e-4 vs the FP32-ref oracle volume (the
+    north-star bar) -- observed ~1e-6; BF16 / FP16 storage: <= 1e-3 vs FP64.
+"""
+import numpy as np
+import pytest
+import torch
+
+from jenhine_amd import configs, geometry, synth
+from oracle import oracle as o
+from tests.helpers import max_abs, oracle_params, rel_rmse, ulp_diff_f32
+
+pytestmark = pytest.mark.gpu
+
+
+def _u16(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a, np.uint16).view(np.int16)).to(dev)
+
+
+SMALL = configs.CBCT128.scaled(n_views=24, name="cbct128-24v")
+
+
+@pytest.fixture(scope="module")
+def small_data():
+    return SMALL.data()
+
+
+# --------------------------------------------------------------------------
+# Preprocessing
+# --------------------------------------------------------------------------
+
+def test_preprocess_all_u16_values(gpu):
+    from jenhine_amd.recon import PreProcessingCore
+    raw = np.arange(65536, dtype=np.uint16).reshape(1, 256, 256)
+    pre = PreProcessingCore(256, 256, bright_field=1000.0, device=gpu)
+    out = pre.process(_u16(raw, gpu)).cpu().numpy()
+    ref = o.preprocess(raw, bright_field=1000.0)
+    assert out.shape == ref.shape
+    assert ulp_diff_f32(out, ref).max() <= 2
+    # the clamp of LowerBound(1): counts 0 and 1 both give -ln(1/BF)
+    assert out[0, 0, 0] == out[0, 0, 1]
+
+
+def test_preprocess_dark_gain_bh_borders(gpu, small_data):
+    from jenhine_amd.recon import PreProcessingCore
+    d = small_data
+    raw = d["raw"][:6]
+    bh = (1.0, 0.03, 0.004)
+    borders = (3, 2, 4, 1)
+    H, W = raw.shape[1:]
+    oh, ow = H - 5, W - 5
+    gain = d["gain"][3:3 + oh, 4:4 + ow].copy()
+    pre = PreProcessingCore(W, H, borders, bright_field=4000.0, bh=bh, device=gpu)
+    out = pre.process(_u16(raw, gpu), _u16(d["dark"], gpu), torch.from_numpy(gain).to(gpu)).cpu().numpy()
+    ref = o.preprocess(raw, borders=borders, bright_field=4000.0, bh=bh, dark=d["dark"], gain=gain)
+    assert ulp_diff_f32(out, ref).max() <= 4
+
+
+def test_preprocess_dark_below_raw_clamps(gpu):
+    from jenhine_amd.recon import PreProcessingCore
+    raw = np.full((1, 8, 16), 100, np.uint16)
+    dark = np.full((8, 16), 150, np.uint16)
+    pre = PreProcessingCore(16, 8, bright_field=50.0, device=gpu)
+    out = pre.process(_u16(raw, gpu), _u16(dark, gpu)).cpu().numpy()
+    assert np.all(out == np.float32(-np.log(np.float32(1.0) / np.float32(50.0))))
+
+
+def test_preprocess_resize(gpu):
+    from jenhine_amd.recon import PreProcessingCore
+    raw = synth.uniform(3, (2, 40, 48), 50, 3000).astype(np.uint16)
+    pre = PreProcessingCore(48, 40, (2, 2, 4, 4), out_w=20, out_h=18, bright_field=4000.0, device=gpu)
+    out = pre.process(_u16(raw, gpu)).cpu().numpy()
+    ref = o.preprocess(raw, out_hw=(18, 20), borders=(2, 2, 4, 4), bright_field=4000.0)
+    np.testing.assert_allclose(out, ref, rtol=2e-6, atol=1e-6)
+
+
+# --------------------------------------------------------------------------
+# Filter
+# --------------------------------------------------------------------------
+
+def _filter_case(gpu, small_data, mode):
+    from jenhine_amd.recon import FilterCore
+    w = SMALL
+    scan, rec = w.scan_params(), w.recon_params()
+    pre = o.preprocess(small_data["raw"], bright_field=scan.BrightField, dark=small_data["dark"],
+                       gain=small_data["gain"], fp64=True)
+    ptm = small_data["ptm"]
+    flt = FilterCore(w.n_u, w.n_v, ptm, scan.DSO, scan.HalfSampleRate, rec.FilterCutOffStart,
+                     rec.FilterCutOffEnd, window=rec.FilterWindow, mode=mode, device=gpu)
+    out = flt.process(torch.from_numpy(pre.astype(np.float32)).to(gpu)).cpu().numpy()
+    H = o.filter_design(w.n_u, scan.HalfSampleRate, rec.FilterCutOffStart, rec.FilterCutOffEnd,
+                        window=rec.FilterWindow, fp64=True)
+    src, blk = o.preweight_schedule(w.n_views)
+    pws = np.stack([o.prefilter_weight(ptm[s], w.n_u, w.n_v, scan.DSO, fp64=True) for s in src])
+    ref = o.filter_rows(pre, H, pws, blk, fp64=True)
+    return flt, out, ref, H, pws
+
+
+def test_filter_fp32_vs_fp64_oracle(gpu, small_data):
+    flt, out, ref, H, pws = _filter_case(gpu, small_data, "fp32")
+    Hd, L = flt.response()
+    assert L == o.fft_length(SMALL.n_u)
+    np.testing.assert_allclose(Hd, H, rtol=1e-9, atol=1e-12 * np.abs(H).max())
+    pw = flt.preweight(13).cpu().numpy()
+    np.testing.assert_allclose(pw, pws[13 // 9][:, :SMALL.n_u], rtol=2e-7)
+    e = rel_rmse(out, ref)
+    print("filter fp32 rel-RMSE", e, "max-abs", max_abs(out, ref))
+    assert e <= 2e-6
+
+
+def test_filter_mfma_bf16x3_vs_fp64_oracle(gpu, small_data):
+    flt, out, ref, H, pws = _filter_case(gpu, small_data, "mfma")
+    e = rel_rmse(out, ref)
+    print("filter mfma rel-RMSE", e, "max-abs", max_abs(out, ref))
+    assert e <= 2e-5
+
+
+# --------------------------------------------------------------------------
+# Backprojection
+# --------------------------------------------------------------------------
+
+def _bp_gpu(gpu, ptm, filt, w, dtype="f32", chunks=None, tile=None):
+    from jenhine_amd.recon import BPCore
+    bp = BPCore(ptm, filt.shape[2], filt.shape[1], w[0], w[1], w[2], w[3], w[4], device=gpu)
+    if tile:
+        bp.set_tile(*tile)
+    f = torch.from_numpy(np.ascontiguousarray(filt, np.float32)).to(gpu)
+    if chunks is None:
+        q = bp.stage(f, dtype)
+        return bp.run(q, dtype).cpu().numpy(), bp
+    bp.reset()
+    for v0 in range(0, filt.shape[0], chunks):
+        n = min(chunks, filt.shape[0] - v0)
+        q = bp.stage(f[v0:v0 + n], dtype, view0=v0)
+        bp.accumulate(q, dtype, v0, n)
+    return bp.finalize().cpu().numpy(), bp
+
+
+def test_bp_uweights_match_oracle(gpu, small_data):
+    from jenhine_amd.recon import BPCore
+    w = SMALL
+    bp = BPCore(small_data["ptm"], w.n_u, w.n_v, w.X, w.Y, w.Z, w.pxy, w.pz, device=gpu)
+    ref = o.uweights(small_data["ptm"], w.n_u, fp64=True)
+    np.testing.assert_allclose(bp.uweights(), ref, rtol=1e-7, atol=1e-7)
+    assert bp.geometry_mode() == 0
+
+
+@pytest.mark.parametrize("tile", [(32, 4), (16, 2), (8, 1)])
+def test_bp_fp32_vs_fp32_ref(gpu, tile):
+    # random filtered projections on the cbct128 geometry at 24 views
+    w = SMALL
+    ptm = w.ptms()
+    filt = synth.normal(11, (w.n_views, w.n_v, w.n_u))
+    shape = (w.X, w.Y, w.Z, w.pxy, w.pz)
+    vol, _ = _bp_gpu(gpu, ptm, filt, shape, tile=tile)
+    ref = o.backproject(filt, ptm, (w.Z, w.Y, w.X), w.pxy, w.pz)
+    assert np.mean(np.isnan(vol) != np.isnan(ref)) < 1e-4
+    e = rel_rmse(vol, ref)
+    print("bp fp32 tile", tile, "rel-RMSE", e, "max-abs", max_abs(vol, ref))
+    assert e <= 1e-4
+
+
+def test_bp_streaming_equals_one_shot(gpu):
+    w = SMALL
+    ptm = w.ptms()
+    filt = synth.normal(12, (w.n_views, w.n_v, w.n_u))
+    shape = (w.X, w.Y, w.Z, w.pxy, w.pz)
+    one, _ = _bp_gpu(gpu, ptm, filt, shape)
+    st, _ = _bp_gpu(gpu, ptm, filt, shape, chunks=5)
+    assert rel_rmse(st, one) <= 1e-6
+
+
+def test_bp_general_geometry_modes(gpu):
+    # the reference's own test matrix (UTBPCUDACore.cpp:20-32) has u depending
+    # on z (mode 1); a tilted copy also makes w depend on z (mode 2)
+    P = np.array([-1.33897, -7.81194, 0.01236, 575.79443, -0.97515, -0.03862, 7.51861, 346.26642,
+                  -0.00226, -0.00004, 0.0, 1.0], np.float64)
+    tilt = P.copy()
+    tilt[10] = 0.0003
+    for ptm, mode in ((np.stack([P, P * 1.0001]), 1), (np.stack([tilt, P]), 2)):
+        ptm = ptm.reshape(-1, 12)
+        filt = synth.normal(13, (2, 120, 110))
+        shape = (96, 80, 20, 0.25, 1.0)
+        vol, bp = _bp_gpu(gpu, ptm, filt, shape)
+        assert bp.geometry_mode() == mode
+        ref = o.backproject(filt, ptm.astype(np.float32), (20, 80, 96), 0.25, 1.0)
+        assert np.mean(np.isnan(vol) != np.isnan(ref)) < 1e-3
+        e = rel_rmse(vol, ref)
+        print("mode", mode, "rel-RMSE", e)
+        assert e <= 1e-4
+
+
+def test_bp_reference_gpu_case(gpu):
+    # UTBPCUDACore.cpp:16-201: one view, 644 x 686 detector, 500 x 500 x 50
+    # volume; projection 0.5 on its first half and 1.0 on the second; at most
+    # N/500 voxels may differ from the CPU result by more than 0.05.
+    P = np.array([-1.33897, -7.81194, 0.01236, 575.79443, -0.97515, -0.03862, 7.51861, 346.26642,
+                  -0.00226, -0.00004, 0.0, 1.0], np.float64).reshape(1, 12)
+    U, V = 644, 686
+    proj = np.ones((1, V, U), np.float32)
+    proj.reshape(-1)[: U * V // 2] = 0.5
+    vol, _ = _bp_gpu(gpu, P, proj, (500, 500, 50, 0.25, 1.0))
+    ref = o.backproject(proj, P.astype(np.float32), (50, 500, 500), 0.25, 1.0)
+    both = np.isfinite(vol) & np.isfinite(ref)
+    bad = np.sum(np.abs(vol[both] - ref[both]) > 0.05) + np.sum(np.isfinite(vol) != np.isfinite(ref))
+    print("voxels off by more than 0.05 or NaN mismatched:", bad, "of", vol.size)
+    assert bad <= vol.size // 500
+    assert rel_rmse(vol, ref) <= 1e-5
+
+
+def test_bp_constant_projection_is_constant_volume(gpu):
+    # normalisation invariant: vol = c wherever any view saw the voxel
+    w = SMALL
+    ptm = w.ptms()
+    filt = np.full((w.n_views, w.n_v, w.n_u), 0.75, np.float32)
+    vol, _ = _bp_gpu(gpu, ptm, filt, (w.X, w.Y, w.Z, w.pxy, w.pz))
+    m = np.isfinite(vol)
+    assert m.mean() > 0.9
+    np.testing.assert_allclose(vol[m], 0.75, rtol=2e-6)
+
+
+@pytest.mark.parametrize("dtype,tol", [("bf16", 1e-3), ("f16", 2e-4)])
+def test_bp_reduced_storage_vs_fp64(gpu, small_data, dtype, tol):
+    w = SMALL
+    sc, rc = oracle_params(w)
+    vol64, st = o.fdk(small_data["raw"], small_data["ptm"], sc, rc, fp64=True, dark=small_data["dark"],
+                      gain=small_data["gain"], return_stages=True)
+    filt = st["filt"].astype(np.float32)
+    vol, _ = _bp_gpu(gpu, small_data["ptm"], filt, (w.X, w.Y, w.Z, w.pxy, w.pz), dtype=dtype)
+    e = rel_rmse(vol, vol64)
+    print(dtype, "rel-RMSE vs FP64", e, "max-abs", max_abs(vol, vol64))
+    assert e <= tol
+
+
+# --------------------------------------------------------------------------
+# Full chain
+# --------------------------------------------------------------------------
+
+@pytest.mark.parametrize("filter_mode,storage,tol64", [("fp32", "f32", 1e-4), ("mfma", "bf16", 1e-3),
+                                                        ("mfma", "f16", 1e-3)])
+def test_fdk_cbct128_vs_oracle(gpu, filter_mode, storage, tol64):
+    from jenhine_amd.recon import FDK
+    w = configs.CBCT128
+    d = w.data()
+    sc, rc = oracle_params(w)
+    fdk = FDK(w.scan_params(), w.recon_params(), d["ptm"], filter_mode, storage, device=gpu)
+    vol = fdk(_u16(d["raw"], gpu), _u16(d["dark"], gpu), torch.from_numpy(d["gain"]).to(gpu)).cpu().numpy()
+    v64 = o.fdk(d["raw"], d["ptm"], sc, rc, fp64=True, dark=d["dark"], gain=d["gain"])
+    e64 = rel_rmse(vol, v64)
+    print(filter_mode, storage, "rel-RMSE vs FP64", e64, "max-abs", max_abs(vol, v64))
+    assert e64 <= tol64
+    if storage == "f32":
+        v32 = o.fdk(d["raw"], d["ptm"], sc, rc, fp64=False, dark=d["dark"], gain=d["gain"])
+        e32 = rel_rmse(vol, v32)
+        print("rel-RMSE vs FP32-ref", e32)
+        assert e32 <= 1e-4
