@@ -175,6 +175,27 @@ jh_status jh_bp_accumulate(jh_bp_plan* plan, const void* d_q, int dtype, int vie
  * BPCUDACore::DeployUpdateOutput + Div2 (BPCUDACore.cu:68-87,177-215). */
 jh_status jh_bp_finalize(jh_bp_plan* plan, float* d_vol, int z0, int nz, jh_stream_t stream);
 
+/* Packs frames (nviews x V x W f32) into the backprojector's layout without
+ * the uW weighting, for use with jh_bp_backproject.  No counterpart in the
+ * reference. */
+jh_status jh_bp_pack(jh_bp_plan* plan, const float* d_frames, int view0, int nviews, int dtype,
+                     void* d_q, jh_stream_t stream);
+
+/* Un-normalised backprojection B of views view0..view0+nviews-1 of an FP32
+ * packed or staged buffer: d_out (Z x Y x X) = sum_v bilinear(q_v).  This is
+ * the accumulation of BPCore::ProcessSlice (BPCore.cpp:170-192) without the
+ * count division, i.e. the exact transpose of jh_fp_project. */
+jh_status jh_bp_backproject(jh_bp_plan* plan, const void* d_q, int view0, int nviews, float* d_out,
+                            jh_stream_t stream);
+
+/* Forward projector, the exact transpose of jh_bp_backproject: d_proj
+ * (nviews x V x W f32, plain row-major frames) = B^T d_vol for views
+ * view0..view0+nviews-1, multiplied per pixel column by uW when weighted != 0
+ * (then it is the transpose of staging followed by B).  The reference has no
+ * forward projector (its call was removed, JEngine/JEngine.cpp:27). */
+jh_status jh_fp_project(jh_bp_plan* plan, const float* d_vol, float* d_proj, int view0, int nviews,
+                        int weighted, jh_stream_t stream);
+
 /* Tile shape of the backprojector: z voxels per thread (8, 16 or 32) and
  * voxel columns per workgroup along y (1, 2 or 4; 64 along x).  0 keeps the
  * current value.  No counterpart in the reference (its launch shape is fixed

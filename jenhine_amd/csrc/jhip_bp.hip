@@ -26,27 +26,7 @@
 
 #include "jhip_internal.h"
 
-namespace jh {
-
-struct Geo {
-    float c[12];  // cu0 cui cuj cuk | cv0 cvi cvj cvk | cw0 cwi cwj cwk
-};
-
-}  // namespace jh
-
-struct jh_bp_plan {
-    int device;
-    int N, W, V, X, Y, Z;
-    float pxy, pz;
-    int mode;
-    int zc, ty;
-    jh::DevBuf ptm;  // N x 12 double
-    jh::DevBuf geo;  // N x Geo
-    jh::DevBuf uw;   // N x W float
-    jh::DevBuf uwp;  // N x W float2 count-weight pairs
-    jh::DevBuf acc;  // X*Y*Z float (streaming form, allocated on first use)
-    jh::DevBuf cnt;  // X*Y*Z float
-};
+#include "jhip_bp_plan.h"
 
 namespace jh {
 
@@ -124,6 +104,7 @@ template <int QT>
 __global__ void __launch_bounds__(256) bp_stage_kernel(const float* __restrict__ filt,
                                                        const float* __restrict__ uw, int W, int V,
                                                        int nviews, void* __restrict__ out)
+// uw == NULL packs the frames without weighting.
 {
     const size_t frame_in = (size_t)W * V;
     const size_t frame_out = (size_t)W * (V + 1);
@@ -136,10 +117,15 @@ __global__ void __launch_bounds__(256) bp_stage_kernel(const float* __restrict__
         const int u = pix - row * W;
         float a = 0.0f, b = 0.0f;
         if (row < V) {
-            const float* w = uw + view * W;
             const float* f = filt + view * frame_in + (size_t)row * W;
-            a = f[u] * w[u];
-            if (u + 1 < W) b = f[u + 1] * w[u + 1];
+            if (uw) {
+                const float* w = uw + view * W;
+                a = f[u] * w[u];
+                if (u + 1 < W) b = f[u + 1] * w[u + 1];
+            } else {
+                a = f[u];
+                if (u + 1 < W) b = f[u + 1];
+            }
         }
         if (QT == JH_F32) {
             reinterpret_cast<float2*>(out)[t] = make_float2(a, b);
@@ -236,7 +222,12 @@ struct BPArgs {
 
 constexpr int kZB = 8;  // samples whose loads are issued together
 
-template <int MODE, int ZC, int QT, bool FINAL>
+// Output forms of the backprojection kernel.
+constexpr int kOutNormalised = 0;  // vol = acc / count (one-shot)
+constexpr int kOutAccumulate = 1;  // acc += ..., count += ... (streaming)
+constexpr int kOutRaw = 2;         // vol = acc (the un-normalised operator, FP32 storage only)
+
+template <int MODE, int ZC, int QT, int OUT>
 __global__ void __launch_bounds__(256) bp_kernel(BPArgs a)
 {
     typedef Sampler<QT> S;
@@ -369,11 +360,13 @@ __global__ void __launch_bounds__(256) bp_kernel(BPArgs a)
 #pragma unroll
     for (int kk = 0; kk < ZC; ++kk) {
         if (k0 + kk < a.Z) {
-            if (FINAL) {
+            if (OUT == kOutNormalised) {
                 a.vol[idx] = acc[kk] / cnt[kk];
-            } else {
+            } else if (OUT == kOutAccumulate) {
                 a.acc[idx] += acc[kk];
                 a.cnt[idx] += cnt[kk];
+            } else {
+                a.vol[idx] = acc[kk];
             }
         }
         idx += plane;
@@ -400,38 +393,39 @@ static unsigned grid_for(size_t n, unsigned block)
 
 typedef void (*bp_kernel_t)(BPArgs);
 
-template <int MODE, int ZC, bool FINAL>
+template <int MODE, int ZC, int OUT>
 static bp_kernel_t pick_q(int dtype)
 {
+    if (OUT == kOutRaw) return bp_kernel<MODE, ZC, JH_F32, kOutRaw>;
     switch (dtype) {
-    case JH_F32: return bp_kernel<MODE, ZC, JH_F32, FINAL>;
-    case JH_BF16: return bp_kernel<MODE, ZC, JH_BF16, FINAL>;
-    default: return bp_kernel<MODE, ZC, JH_F16, FINAL>;
+    case JH_F32: return bp_kernel<MODE, ZC, JH_F32, OUT>;
+    case JH_BF16: return bp_kernel<MODE, ZC, JH_BF16, OUT>;
+    default: return bp_kernel<MODE, ZC, JH_F16, OUT>;
     }
 }
 
-template <int MODE, bool FINAL>
+template <int MODE, int OUT>
 static bp_kernel_t pick_zc(int zc, int dtype)
 {
     switch (zc) {
-    case 8: return pick_q<MODE, 8, FINAL>(dtype);
-    case 16: return pick_q<MODE, 16, FINAL>(dtype);
-    default: return pick_q<MODE, 32, FINAL>(dtype);
+    case 8: return pick_q<MODE, 8, OUT>(dtype);
+    case 16: return pick_q<MODE, 16, OUT>(dtype);
+    default: return pick_q<MODE, 32, OUT>(dtype);
     }
 }
 
-template <bool FINAL>
+template <int OUT>
 static bp_kernel_t pick_kernel(int mode, int zc, int dtype)
 {
     switch (mode) {
-    case 0: return pick_zc<0, FINAL>(zc, dtype);
-    case 1: return pick_zc<1, FINAL>(zc, dtype);
-    default: return pick_zc<2, FINAL>(zc, dtype);
+    case 0: return pick_zc<0, OUT>(zc, dtype);
+    case 1: return pick_zc<1, OUT>(zc, dtype);
+    default: return pick_zc<2, OUT>(zc, dtype);
     }
 }
 
 static void launch_bp(jh_bp_plan* p, const void* q, int dtype, int view0, int nviews, float* vol,
-                      bool final_, hipStream_t stream)
+                      int out, hipStream_t stream)
 {
     BPArgs a;
     a.q = q;
@@ -451,7 +445,9 @@ static void launch_bp(jh_bp_plan* p, const void* q, int dtype, int view0, int nv
     const size_t ntiles = (size_t)a.ntx * a.nty * a.ntz;
     require(ntiles < (1ull << 31), "backprojection grid too large");
     const unsigned grid = (unsigned)((ntiles + 7) / 8 * 8);
-    bp_kernel_t k = final_ ? pick_kernel<true>(p->mode, p->zc, dtype) : pick_kernel<false>(p->mode, p->zc, dtype);
+    bp_kernel_t k = out == kOutNormalised   ? pick_kernel<kOutNormalised>(p->mode, p->zc, dtype)
+                    : out == kOutAccumulate ? pick_kernel<kOutAccumulate>(p->mode, p->zc, dtype)
+                                            : pick_kernel<kOutRaw>(p->mode, p->zc, dtype);
     hipLaunchKernelGGL(k, dim3(grid), dim3(64, p->ty), 0, stream, a);
     check_launch("bp_kernel");
 }
@@ -532,23 +528,47 @@ jh_status jh_bp_stage_bytes(jh_bp_plan* p, int dtype, int nviews, size_t* bytes)
     });
 }
 
-jh_status jh_bp_stage(jh_bp_plan* p, const float* d_filtered, int view0, int nviews, int dtype,
-                      void* d_q, jh_stream_t stream)
+static void stage(jh_bp_plan* p, const float* d_filtered, int view0, int nviews, int dtype, void* d_q,
+                  bool weighted, jh_stream_t stream)
 {
-    return guard([&] {
+    {
         require(p != nullptr && d_filtered != nullptr && d_q != nullptr, "jh_bp_stage: bad arguments");
         require(view0 >= 0 && nviews >= 0 && view0 + nviews <= p->N, "jh_bp_stage: view range outside the scan");
         require(dtype == JH_F32 || dtype == JH_BF16 || dtype == JH_F16, "jh_bp_stage: unknown dtype");
         if (nviews == 0) return;
         DeviceScope ds(p->device);
         const size_t total = (size_t)p->W * (p->V + 1) * nviews;
-        const float* uw = p->uw.as<float>() + (size_t)view0 * p->W;
+        const float* uw = weighted ? p->uw.as<float>() + (size_t)view0 * p->W : nullptr;
         const unsigned grid = grid_for(total, 256);
         hipStream_t s = as_stream(stream);
         if (dtype == JH_F32) bp_stage_kernel<JH_F32><<<grid, 256, 0, s>>>(d_filtered, uw, p->W, p->V, nviews, d_q);
         else if (dtype == JH_BF16) bp_stage_kernel<JH_BF16><<<grid, 256, 0, s>>>(d_filtered, uw, p->W, p->V, nviews, d_q);
         else bp_stage_kernel<JH_F16><<<grid, 256, 0, s>>>(d_filtered, uw, p->W, p->V, nviews, d_q);
         check_launch("bp_stage_kernel");
+    }
+}
+
+jh_status jh_bp_stage(jh_bp_plan* p, const float* d_filtered, int view0, int nviews, int dtype,
+                      void* d_q, jh_stream_t stream)
+{
+    return guard([&] { stage(p, d_filtered, view0, nviews, dtype, d_q, true, stream); });
+}
+
+jh_status jh_bp_pack(jh_bp_plan* p, const float* d_frames, int view0, int nviews, int dtype, void* d_q,
+                     jh_stream_t stream)
+{
+    return guard([&] { stage(p, d_frames, view0, nviews, dtype, d_q, false, stream); });
+}
+
+jh_status jh_bp_backproject(jh_bp_plan* p, const void* d_q, int view0, int nviews, float* d_out,
+                            jh_stream_t stream)
+{
+    return guard([&] {
+        require(p != nullptr && d_q != nullptr && d_out != nullptr, "jh_bp_backproject: bad arguments");
+        require(view0 >= 0 && nviews >= 0 && view0 + nviews <= p->N,
+                "jh_bp_backproject: view range outside the scan");
+        DeviceScope ds(p->device);
+        launch_bp(p, d_q, JH_F32, view0, nviews, d_out, kOutRaw, as_stream(stream));
     });
 }
 
@@ -558,7 +578,7 @@ jh_status jh_bp_run(jh_bp_plan* p, const void* d_q, int dtype, float* d_vol, jh_
         require(p != nullptr && d_q != nullptr && d_vol != nullptr, "jh_bp_run: bad arguments");
         require(dtype == JH_F32 || dtype == JH_BF16 || dtype == JH_F16, "jh_bp_run: unknown dtype");
         DeviceScope ds(p->device);
-        launch_bp(p, d_q, dtype, 0, p->N, d_vol, true, as_stream(stream));
+        launch_bp(p, d_q, dtype, 0, p->N, d_vol, kOutNormalised, as_stream(stream));
     });
 }
 
@@ -593,7 +613,7 @@ jh_status jh_bp_accumulate(jh_bp_plan* p, const void* d_q, int dtype, int view0,
         require(p->acc.get() != nullptr, "jh_bp_accumulate: call jh_bp_reset first");
         if (nviews == 0) return;
         DeviceScope ds(p->device);
-        launch_bp(p, d_q, dtype, view0, nviews, nullptr, false, as_stream(stream));
+        launch_bp(p, d_q, dtype, view0, nviews, nullptr, kOutAccumulate, as_stream(stream));
     });
 }
 

@@ -1,0 +1,30 @@
+// Backprojection plan shared by the backprojector and its adjoint.
+#pragma once
+
+#include "jhip_internal.h"
+
+namespace jh {
+
+// Per-view geometry in voxel-index space, rounded once from FP64:
+// U(i,j,k) = c[0] + c[1] i + c[2] j + c[3] k, V likewise with c[4..7],
+// W with c[8..11].
+struct Geo {
+    float c[12];
+};
+
+}  // namespace jh
+
+struct jh_bp_plan {
+    int device;
+    int N, W, V, X, Y, Z;
+    float pxy, pz;
+    int mode;        // 0: u and w constant along z, 1: w constant along z, 2: general
+    int zc, ty;      // backprojector tile: z voxels per thread, columns along y
+    jh::DevBuf ptm;  // N x 12 double
+    jh::DevBuf geo;  // N x Geo
+    jh::DevBuf uw;   // N x W float
+    jh::DevBuf uwp;  // N x W float2 count-weight pairs
+    jh::DevBuf acc;  // X*Y*Z float (streaming form, allocated on first use)
+    jh::DevBuf cnt;  // X*Y*Z float
+    jh::DevBuf xt;   // X*Y*Z float, z-fastest copy of the volume for the projector
+};

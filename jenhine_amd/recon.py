@@ -169,6 +169,33 @@ class BPCore:
         call("jh_bp_stage", self._h, _ptr(filtered), view0, n, DTYPES[dtype], _ptr(out), _stream())
         return out
 
+    def pack(self, frames, dtype="f32", view0=0, out=None):
+        """frames in the backprojector's layout without uW weighting."""
+        _require_cuda(frames, "frames", torch.float32)
+        n = frames.shape[0]
+        if out is None:
+            out = torch.empty(self.stage_bytes(dtype, n), dtype=torch.uint8, device=frames.device)
+        call("jh_bp_pack", self._h, _ptr(frames), view0, n, DTYPES[dtype], _ptr(out), _stream())
+        return out
+
+    def backproject(self, q, view0, n_views, out=None):
+        """Un-normalised backprojection B of an FP32 packed/staged buffer."""
+        _require_cuda(q, "q")
+        if out is None:
+            out = torch.empty((self.Z, self.Y, self.X), dtype=torch.float32, device=q.device)
+        call("jh_bp_backproject", self._h, _ptr(q), view0, n_views, _ptr(out), _stream())
+        return out
+
+    def project(self, vol, view0=0, n_views=None, weighted=False, out=None):
+        """Forward projection B^T vol (frames n_views x V x W)."""
+        _require_cuda(vol, "vol", torch.float32)
+        n = self.n_views - view0 if n_views is None else n_views
+        if out is None:
+            out = torch.empty((n, self.V, self.W), dtype=torch.float32, device=vol.device)
+        _require_cuda(out, "out", torch.float32)
+        call("jh_fp_project", self._h, _ptr(vol), _ptr(out), view0, n, int(weighted), _stream())
+        return out
+
     def run(self, q, dtype="f32", out=None):
         _require_cuda(q, "q")
         if out is None:

@@ -581,6 +581,90 @@ int FN(jo_bp)(const R* proj, const R* ptm, int N, int W, int V, int X, int Y, in
     return 0;
 }
 
+
+/* ------------------------------------------------------------------------ */
+/* Un-normalised backprojection and its transpose (differentiation form)     */
+/* ------------------------------------------------------------------------ */
+
+/* Sample coordinate of voxel (i, j, k) in view P, in the operation order of
+ * BPCore (BPCore.cpp:120-145, 207-220). */
+static inline void FN(voxel_uv)(const R* P, const R* ax, const R* ay, const R* az, int i, int j, int k,
+                                R* su, R* sv)
+{
+    R U = ax[i] * P[0] + (ay[j] * P[1] + P[3]);
+    R Vv = ax[i] * P[4] + (ay[j] * P[5] + P[7]);
+    R Wd = ax[i] * P[8] + (ay[j] * P[9] + P[11]);
+    U = U + az[k] * P[2]; Vv = Vv + az[k] * P[6]; Wd = Wd + az[k] * P[10];
+    *su = U / Wd + (R)0.5;
+    *sv = Vv / Wd + (R)0.5;
+}
+
+/* acc = sum_v remap(proj_v * (weighted ? uW_v : 1)) over the whole volume,
+ * i.e. the accumulation of BPCore::ProcessSlice without the count division.
+ * Single-threaded; for small test volumes. */
+void FN(jo_bp_acc)(const R* proj, const R* ptm, int N, int W, int V, int X, int Y, int Z, R pxy, R pz,
+                   int weighted, R* acc)
+{
+    R* uw = (R*)malloc(sizeof(R) * (size_t)N * W);
+    R* ax = (R*)malloc(sizeof(R) * X); R* ay = (R*)malloc(sizeof(R) * Y); R* az = (R*)malloc(sizeof(R) * Z);
+    R* q = (R*)malloc(sizeof(R) * (size_t)W * V);
+    FN(jo_uweights)(ptm, N, W, uw);
+    FN(jo_axis)(X, pxy, ax); FN(jo_axis)(Y, pxy, ay); FN(jo_axis)(Z, pz, az);
+    memset(acc, 0, sizeof(R) * (size_t)X * Y * Z);
+    for (int v = 0; v < N; ++v) {
+        const R* P = ptm + 12 * (size_t)v;
+        for (int r = 0; r < V; ++r)
+            for (int u = 0; u < W; ++u)
+                q[(size_t)r * W + u] = proj[((size_t)v * V + r) * W + u] * (weighted ? uw[(size_t)v * W + u] : (R)1);
+        for (int k = 0; k < Z; ++k)
+            for (int j = 0; j < Y; ++j)
+                for (int i = 0; i < X; ++i) {
+                    R su, sv, s;
+                    FN(voxel_uv)(P, ax, ay, az, i, j, k, &su, &sv);
+                    if (FN(remap_sample)(q, W, V, su, sv, &s)) acc[((size_t)k * Y + j) * X + i] += s;
+                }
+    }
+    free(uw); free(ax); free(ay); free(az); free(q);
+}
+
+/* Transpose of jo_bp_acc: out_v(u, r) = (weighted ? uW_v(u) : 1) *
+ * sum over voxels of x * (bilinear weight of pixel (u, r) in the voxel's
+ * sample), with the same edge rule (samples outside [0,W-1]x[0,V-1] skipped;
+ * the clamped stencil of remap_sample). */
+void FN(jo_fp)(const R* x, const R* ptm, int N, int W, int V, int X, int Y, int Z, R pxy, R pz,
+               int weighted, R* out)
+{
+    R* uw = (R*)malloc(sizeof(R) * (size_t)N * W);
+    R* ax = (R*)malloc(sizeof(R) * X); R* ay = (R*)malloc(sizeof(R) * Y); R* az = (R*)malloc(sizeof(R) * Z);
+    FN(jo_uweights)(ptm, N, W, uw);
+    FN(jo_axis)(X, pxy, ax); FN(jo_axis)(Y, pxy, ay); FN(jo_axis)(Z, pz, az);
+    memset(out, 0, sizeof(R) * (size_t)N * W * V);
+    for (int v = 0; v < N; ++v) {
+        const R* P = ptm + 12 * (size_t)v;
+        R* o = out + (size_t)v * W * V;
+        for (int k = 0; k < Z; ++k)
+            for (int j = 0; j < Y; ++j)
+                for (int i = 0; i < X; ++i) {
+                    R su, sv;
+                    FN(voxel_uv)(P, ax, ay, az, i, j, k, &su, &sv);
+                    if (!(su >= (R)0 && su <= (R)(W - 1) && sv >= (R)0 && sv <= (R)(V - 1))) continue;
+                    int x0 = (int)su, y0 = (int)sv;
+                    if (x0 > W - 2) x0 = W - 2;
+                    if (y0 > V - 2) y0 = V - 2;
+                    const R fx = su - (R)x0, fy = sv - (R)y0;
+                    const R val = x[((size_t)k * Y + j) * X + i];
+                    o[(size_t)y0 * W + x0] += ((R)1 - fx) * ((R)1 - fy) * val;
+                    o[(size_t)y0 * W + x0 + 1] += fx * ((R)1 - fy) * val;
+                    o[(size_t)(y0 + 1) * W + x0] += ((R)1 - fx) * fy * val;
+                    o[(size_t)(y0 + 1) * W + x0 + 1] += fx * fy * val;
+                }
+        if (weighted)
+            for (int r = 0; r < V; ++r)
+                for (int u = 0; u < W; ++u) o[(size_t)r * W + u] *= uw[(size_t)v * W + u];
+    }
+    free(uw); free(ax); free(ay); free(az);
+}
+
 #undef FN
 #undef CAT
 #undef CAT_

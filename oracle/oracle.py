@@ -203,3 +203,30 @@ def fdk(raw, ptm, scan, recon, fp64=False, nthreads=8, z0=0, nz=None, dark=None,
     if return_stages:
         return vol, dict(pre=p, H=H, pw=pws, pw_index=blk, filt=f)
     return vol
+
+
+def bp_acc(proj, ptm, vol_shape, pitch_xy, pitch_z, weighted=True, fp64=True):
+    """Un-normalised backprojection sum_v remap(proj_v [* uW_v]) (small sizes)."""
+    proj = np.ascontiguousarray(proj, _dt(fp64))
+    N, V, W = proj.shape
+    ptm = np.ascontiguousarray(ptm, _dt(fp64)).reshape(N, 12)
+    Z, Y, X = vol_shape
+    acc = np.zeros((Z, Y, X), _dt(fp64))
+    R = _real(fp64)
+    getattr(lib(), "jo_bp_acc" + _sfx(fp64))(_p(proj), _p(ptm), N, W, V, X, Y, Z, R(pitch_xy), R(pitch_z),
+                                              int(weighted), _p(acc))
+    return acc
+
+
+def forward_project(x, ptm, det_shape, pitch_xy, pitch_z, weighted=True, fp64=True):
+    """Transpose of bp_acc: (N, V, W) frames from a (Z, Y, X) volume."""
+    x = np.ascontiguousarray(x, _dt(fp64))
+    Z, Y, X = x.shape
+    ptm = np.ascontiguousarray(ptm, _dt(fp64)).reshape(-1, 12)
+    N = ptm.shape[0]
+    V, W = det_shape
+    out = np.zeros((N, V, W), _dt(fp64))
+    R = _real(fp64)
+    getattr(lib(), "jo_fp" + _sfx(fp64))(_p(x), _p(ptm), N, W, V, X, Y, Z, R(pitch_xy), R(pitch_z),
+                                         int(weighted), _p(out))
+    return out

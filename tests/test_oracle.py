@@ -307,3 +307,21 @@ def test_fdk_fp64_reconstructs_the_phantom():
     assert abs(vol.mean() / ph.mean() - 1.0) < 0.05
     v32 = o.fdk(d["raw"], d["ptm"], sc, rc, fp64=False, dark=d["dark"], gain=d["gain"])
     assert rel_rmse(v32, vol) < 1e-4
+
+
+# ---------------------------------------------------------- adjoint pair ----
+
+@pytest.mark.parametrize("weighted", [0, 1])
+def test_oracle_projector_is_transpose_of_backprojector(weighted):
+    w = configs.CBCT128.scaled(n_views=6, n_u=40, n_v=36, X=20, Y=18, Z=12, du=1.2)
+    ptm = w.ptms()
+    x = synth.normal(1, (12, 18, 20)).astype(np.float64)
+    y = synth.normal(2, (6, 36, 40)).astype(np.float64)
+    Ax = o.forward_project(x, ptm, (36, 40), w.pxy, w.pz, weighted=weighted)
+    By = o.bp_acc(y, ptm, (12, 18, 20), w.pxy, w.pz, weighted=weighted)
+    a, b = (Ax * y).sum(), (x * By).sum()
+    assert abs(a - b) <= 1e-12 * abs(a)
+    # bp_acc with weights equals the accumulator of the normalised backprojector
+    _, acc, _ = o.backproject(y, ptm, (12, 18, 20), w.pxy, w.pz, fp64=True, return_parts=True)
+    if weighted:
+        np.testing.assert_allclose(By, acc, rtol=1e-12, atol=1e-12)
