@@ -9,7 +9,8 @@ One step = the whole reconstruction on the device with the raw counts already
 resident in HBM: preprocess -> row filter -> stage -> backprojection ->
 normalised volume.  Two forms are timed:
   * fp32: FP32 Toeplitz filter on the vector ALUs, FP32 projection storage;
-  * bf16: MFMA hi/lo split-BF16 filter, BF16 projection storage (headline).
+  * bf16: MFMA hi/lo split-BF16 filter, BF16 projection storage (headline);
+  * f16: the same filter, FP16 projection storage.
 value = voxel-updates per second of the headline form over the whole step
 (X*Y*Z*N / step time), in GUPS.
 
@@ -77,7 +78,8 @@ def main():
     gain = torch.from_numpy(d["gain"]).to(dev)
     scan, rec = w.scan_params(), w.recon_params()
     forms = {"fp32": FDK(scan, rec, d["ptm"], "fp32", "f32", dev),
-             "bf16": FDK(scan, rec, d["ptm"], "mfma", "bf16", dev)}
+             "bf16": FDK(scan, rec, d["ptm"], "mfma", "bf16", dev),
+             "f16": FDK(scan, rec, d["ptm"], "mfma", "f16", dev)}
     units = w.gups_units
 
     results = {}

@@ -8,6 +8,7 @@
 namespace jh {
 
 static thread_local std::string g_last_error;
+std::atomic<size_t> g_device_bytes{0};
 
 void set_last_error(const std::string& msg) { g_last_error = msg; }
 

@@ -107,8 +107,8 @@ __global__ void __launch_bounds__(256) fp_gather_kernel(FPArgs a)
             int k0, k1;
             if (fabsf(dsv) > 1e-6f) {
                 const float ka = ((float)iv - 1.0f - svA) / dsv, kb = ((float)iv + 1.0f - svA) / dsv;
-                k0 = max(0, (int)floorf(fminf(ka, kb)) - 1);
-                k1 = min(a.Z - 1, (int)ceilf(fmaxf(ka, kb)) + 1);
+                k0 = (int)fmaxf(0.0f, floorf(fminf(ka, kb)) - 1.0f);
+                k1 = (int)fminf((float)(a.Z - 1), ceilf(fmaxf(ka, kb)) + 1.0f);
             } else {
                 k0 = 0;
                 k1 = a.Z - 1;

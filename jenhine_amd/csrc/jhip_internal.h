@@ -3,8 +3,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -57,6 +59,17 @@ jh_status guard(F&& f)
     }
 }
 
+// Bytes currently held by the library's plans, and the optional cap set by
+// the environment variable JH_MAX_DEVICE_BYTES (a test hook that lets a
+// host exercise its out-of-memory path on any device).
+extern std::atomic<size_t> g_device_bytes;
+
+inline size_t device_byte_cap()
+{
+    const char* s = std::getenv("JH_MAX_DEVICE_BYTES");
+    return s ? (size_t)std::strtoull(s, nullptr, 10) : SIZE_MAX;
+}
+
 // Owned device allocation.  An allocation failure raises JH_ERR_OOM with the
 // message prefix "HIP failed to malloc " (the host's part-retry keys on it).
 class DevBuf {
@@ -76,16 +89,22 @@ public:
     {
         release();
         if (bytes == 0) return;
+        if (g_device_bytes.load() + bytes > device_byte_cap())
+            fail(JH_ERR_OOM, "HIP failed to malloc " + std::to_string(bytes) + " bytes (cap)");
         if (hipMalloc(&p_, bytes) != hipSuccess || p_ == nullptr) {
             (void)hipGetLastError();
             p_ = nullptr;
             fail(JH_ERR_OOM, "HIP failed to malloc " + std::to_string(bytes) + " bytes");
         }
         n_ = bytes;
+        g_device_bytes += bytes;
     }
     void release()
     {
-        if (p_) (void)hipFree(p_);
+        if (p_) {
+            (void)hipFree(p_);
+            g_device_bytes -= n_;
+        }
         p_ = nullptr;
         n_ = 0;
     }

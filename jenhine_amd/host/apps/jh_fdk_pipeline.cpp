@@ -1,0 +1,232 @@
+// jh_fdk_pipeline: FDK reconstruction of one scan through the agent graph
+//
+//   MemoryInputAgent -> ProjectionUploadAgent -> PreProcessingHIPAgent ->
+//   FilterHIPAgent -> BPHIPAgent -> DeviceMemoryDownloadAgent -> SliceOutputAgent
+//
+// wired by pipe name ("Proj", "Slice") the way JEngine/JEngine.cpp:185-346
+// wires its agents, with the reference's retry on device shortage
+// (JEngine.cpp:167-176, 413-426): when set-up fails with the device
+// out-of-memory prefix the graph is rebuilt with one more z-slab ("part") and
+// the input is streamed once per part.
+//
+// Input files (little-endian, raw): raw counts N x NumDetsV x NumDetsU u16,
+// matrices N x 12 f64, optional dark (u16) and gain (f32) frames.  Output: the
+// volume Z x Y x X f32.  Parameters are key=value arguments.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <map>
+#include <sstream>
+#include <string>
+
+#include "jh/HipAgents.h"
+
+using namespace jh;
+
+namespace {
+
+template <typename T>
+std::vector<T> ReadFile(const std::string& path, size_t count)
+{
+    std::vector<T> v(count);
+    std::ifstream is(path, std::ios::binary);
+    if (!is) ThrowException("cannot open " + path);
+    is.read(reinterpret_cast<char*>(v.data()), (std::streamsize)(count * sizeof(T)));
+    if ((size_t)is.gcount() != count * sizeof(T)) ThrowException("short read from " + path);
+    return v;
+}
+
+// Streams the scan from memory, once per part, one shot per pass
+// (JEngine/InputAgent.cpp:39-91 streams files the same way).
+class MemoryInputAgent : public SequentialAgentBase {
+public:
+    MemoryInputAgent(const std::vector<uint16_t>& raw, size_t frame, size_t views, size_t passes)
+        : SequentialAgentBase("MemoryInputAgent", 1), raw_(raw), frame_(frame), views_(views), passes_(passes)
+    {
+    }
+
+private:
+    void SetPipesImpl() override
+    {
+        out_ = CreateNewPipe<UINT16Vec>("Proj");
+        out_->SetTemplate(UINT16Vec(frame_), {frame_});
+        out_->SetProducer(GetAgentName(), 1, 1);
+        MergePipeToTrunk(out_);
+    }
+    void WorkFlow0() override
+    {
+        for (size_t p = 0; p < passes_; ++p)
+            for (size_t v = 0; v < views_; ++v) {
+                auto wt = out_->GetWriteToken(1, v + 1 == views_);
+                ThreadOccupiedScope occ(this);
+                std::memcpy(wt.GetBuffer(0).data(), raw_.data() + v * frame_, frame_ * sizeof(uint16_t));
+            }
+        out_->Close();
+    }
+    const std::vector<uint16_t>& raw_;
+    const size_t frame_, views_, passes_;
+    std::shared_ptr<Pipe<UINT16Vec>> out_;
+};
+
+// Collects slices into the output volume (JEngine/OutputAgent.cpp:37-82
+// writes them to files instead).
+class SliceOutputAgent : public SequentialAgentBase {
+public:
+    SliceOutputAgent(std::vector<float>& vol, size_t plane, const BPHIPAgent& bp, size_t parts)
+        : SequentialAgentBase("SliceOutputAgent", 1), vol_(vol), plane_(plane), bp_(bp), parts_(parts)
+    {
+    }
+
+private:
+    void SetPipesImpl() override
+    {
+        in_ = BranchPipeFromTrunk<FloatVec>("Slice");
+        in_->SetConsumer(GetAgentName(), 1, 1, 0);
+    }
+    void WorkFlow0() override
+    {
+        size_t part = 0, z = 0;
+        try {
+            for (;;) {
+                auto rt = in_->GetReadToken();
+                ThreadOccupiedScope occ(this);
+                const size_t slice = bp_.PartStart(part) + z;
+                std::memcpy(vol_.data() + slice * plane_, rt.GetBuffer(0).data(), plane_ * sizeof(float));
+                ++z;
+                if (rt.IsShotEnd()) {
+                    ++part;
+                    z = 0;
+                }
+            }
+        } catch (PipeClosedAndEmptySignal&) {
+        }
+        if (part != parts_) ThrowExceptionAndLog("SliceOutputAgent: received " + std::to_string(part) + " of " +
+                                                 std::to_string(parts_) + " parts");
+    }
+    std::vector<float>& vol_;
+    const size_t plane_;
+    const BPHIPAgent& bp_;
+    const size_t parts_;
+    std::shared_ptr<Pipe<FloatVec>> in_;
+};
+
+struct Args {
+    std::map<std::string, std::string> kv;
+    std::string S(const std::string& k, const std::string& d = "") const
+    {
+        auto it = kv.find(k);
+        if (it == kv.end()) {
+            if (d.empty()) ThrowException("missing argument " + k);
+            return d;
+        }
+        return it->second;
+    }
+    double D(const std::string& k, const std::string& d = "") const { return std::atof(S(k, d).c_str()); }
+    long I(const std::string& k, const std::string& d = "") const { return std::atol(S(k, d).c_str()); }
+};
+
+std::vector<float> ParseList(const std::string& s)
+{
+    std::vector<float> v;
+    std::stringstream ss(s);
+    std::string item;
+    while (std::getline(ss, item, '|'))
+        if (!item.empty()) v.push_back((float)std::atof(item.c_str()));
+    return v;
+}
+
+}  // namespace
+
+int main(int argc, char** argv)
+{
+    try {
+        Args a;
+        for (int i = 1; i < argc; ++i) {
+            const char* eq = std::strchr(argv[i], '=');
+            if (!eq) ThrowException(std::string("argument without '=': ") + argv[i]);
+            a.kv[std::string(argv[i], (size_t)(eq - argv[i]))] = eq + 1;
+        }
+        ScanParams scan;
+        scan.NumViews = (size_t)a.I("views");
+        scan.DSO = (float)a.D("dso");
+        scan.DSD = (float)a.D("dsd");
+        scan.NumDetsU = (size_t)a.I("detu");
+        scan.NumDetsV = (size_t)a.I("detv");
+        scan.DetectorPixelSize = (float)a.D("pitch");
+        scan.BorderSizeUp = (size_t)a.I("border_up", "0");
+        scan.BorderSizeDown = (size_t)a.I("border_down", "0");
+        scan.BorderSizeLeft = (size_t)a.I("border_left", "0");
+        scan.BorderSizeRight = (size_t)a.I("border_right", "0");
+        scan.BrightField = (float)a.D("bright_field");
+        scan.BeamHardeningParams = ParseList(a.S("bh", "1|0|0"));
+        scan.Derive();
+        ReconParams recon;
+        recon.NumPixelsX = (size_t)a.I("x");
+        recon.NumPixelsY = (size_t)a.I("y");
+        recon.NumPixelsZ = (size_t)a.I("z");
+        recon.PitchXY = (float)a.D("pitch_xy");
+        recon.PitchZ = (float)a.D("pitch_z");
+        recon.FilterCutOffStart = (float)a.D("cut_start", "0");
+        recon.FilterCutOffEnd = (float)a.D("cut_end", "0");
+        recon.FilterAdjustPoints = ParseList(a.S("adj_pts", "|"));
+        recon.FilterAdjustLevelInDB = ParseList(a.S("adj_db", "|"));
+        recon.FilterWindow = (int)a.I("window", "0");
+        recon.FilterMode = (int)a.I("filter", "0");
+        recon.StorageType = (int)a.I("storage", "0");
+        recon.ViewsPerChunk = (size_t)a.I("chunk", "8");
+        const int device = (int)a.I("device", "0");
+        const size_t frame = scan.NumDetsU * scan.NumDetsV;
+        std::vector<uint16_t> raw = ReadFile<uint16_t>(a.S("raw"), frame * scan.NumViews);
+        std::vector<double> ptm = ReadFile<double>(a.S("ptm"), 12 * scan.NumViews);
+        std::vector<uint16_t> dark;
+        std::vector<float> gain;
+        if (a.kv.count("dark")) dark = ReadFile<uint16_t>(a.S("dark"), frame);
+        if (a.kv.count("gain")) gain = ReadFile<float>(a.S("gain"), scan.NumUsedDetsU * scan.NumUsedDetsV);
+        const size_t plane = recon.NumPixelsX * recon.NumPixelsY;
+        std::vector<float> vol(plane * recon.NumPixelsZ, 0.0f);
+
+        for (size_t parts = (size_t)a.I("parts", "1"); parts <= recon.NumPixelsZ; ++parts) {
+            std::vector<std::shared_ptr<AgentBase>> agents;
+            auto bp = std::make_shared<BPHIPAgent>(scan, recon, ptm, parts, device);
+            agents.push_back(std::make_shared<MemoryInputAgent>(raw, frame, scan.NumViews, parts));
+            agents.push_back(std::make_shared<ProjectionUploadAgent>(scan.NumDetsU, scan.NumDetsV));
+            agents.push_back(std::make_shared<PreProcessingHIPAgent>(scan, dark, gain, device));
+            agents.push_back(std::make_shared<FilterHIPAgent>(scan, recon, ptm, device));
+            agents.push_back(bp);
+            agents.push_back(std::make_shared<DeviceMemoryDownloadAgent>(plane));
+            agents.push_back(std::make_shared<SliceOutputAgent>(vol, plane, *bp, parts));
+            try {
+                std::vector<std::shared_ptr<PipeBase>> trunk;
+                for (auto& ag : agents) ag->SetPipes(trunk);
+                for (auto& ag : agents) ag->GetReady();
+            } catch (Exception& e) {
+                if (std::strncmp(e.what(), kDeviceOOMPrefix, std::strlen(kDeviceOOMPrefix)) == 0 ||
+                    std::strstr(e.what(), kDeviceOOMPrefix)) {
+                    GLog("device memory short with " + std::to_string(parts) + " part(s): " + e.what());
+                    continue;
+                }
+                throw;
+            }
+            Timer t;
+            for (auto& ag : agents) ag->Start();
+            for (auto& ag : agents) ag->Join();
+            const double span = t.Toc();
+            for (auto& ag : agents)
+                if (ag->Failed()) ThrowException(ag->GetAgentName() + " failed; see the log");
+            for (auto& ag : agents)
+                std::cerr << ag->GetAgentName() << ": occupied " << ag->GetOccupiedTime() << " s, async "
+                          << ag->GetAsyncTime() << " s\n";
+            std::cout << "{\"parts\": " << parts << ", \"seconds\": " << span << "}" << std::endl;
+            std::ofstream os(a.S("out"), std::ios::binary);
+            os.write(reinterpret_cast<const char*>(vol.data()), (std::streamsize)(vol.size() * sizeof(float)));
+            if (!os) ThrowException("cannot write " + a.S("out"));
+            return 0;
+        }
+        ThrowException("device memory is short even with one slice per part");
+    } catch (std::exception& e) {
+        std::cerr << "jh_fdk_pipeline: " << e.what() << std::endl;
+        return 1;
+    }
+}

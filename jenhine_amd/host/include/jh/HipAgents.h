@@ -1,0 +1,214 @@
+// Device-side agents of the reconstruction pipeline on MI355X, built on the
+// C-ABI of include/jhip.h.  They take the places of the reference's CUDA
+// module and CPU cores in the agent graph (JEngine/JEngine.cpp:185-313):
+//
+//   ProjectionUploadAgent      Pipe<UINT16Vec> "Proj" -> Pipe<DeviceMemory<u16>> "Proj"
+//                              (replaces CUDA/Tex2DUploadAgent.cpp: raw counts
+//                              go up once, so every later stage stays on the device)
+//   PreProcessingHIPAgent      DeviceMemory<u16> -> DeviceMemory<float>
+//                              (replaces PreProcessingAgent/PreProcessingAgent.cpp)
+//   FilterHIPAgent             DeviceMemory<float> -> DeviceMemory<float>
+//                              (replaces FilterAgent/FilterAgent.cpp)
+//   BPHIPAgent                 DeviceMemory<float> "Proj" -> DeviceMemory<float> "Slice"
+//                              (replaces BPCUDAAgent/BPCUDAAgent.cpp)
+//   DeviceMemoryDownloadAgent  DeviceMemory<float> "Slice" -> FloatVec "Slice"
+//                              (replaces CUDA/DeviceMemoryDownloadAgent.cpp)
+//
+// Each agent owns one HIP stream created on its own thread and synchronises
+// it before releasing a write token, so the next agent may use the data
+// (the reference's ordering rule, BPCUDAAgent.cpp:240-261).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "jhip.h"
+#include "Agent.h"
+#include "Params.h"
+
+namespace jh {
+
+// Message prefix of a failed device allocation; the host's part-retry loop
+// keys on it (the reference keys on "CUDA failed to malloc ",
+// JEngine/JEngine.cpp:413-421).
+extern const char* const kDeviceOOMPrefix;
+
+// Throws jh::Exception carrying jh_last_error() when s != JH_OK.
+void CheckJh(jh_status s, const char* what);
+void CheckHip(hipError_t e, const char* what);
+
+namespace hip {
+
+// Throws with kDeviceOOMPrefix when the allocation fails or would pass the
+// byte cap set by JH_MAX_DEVICE_BYTES (a test hook for the part-retry path).
+void* DeviceMalloc(size_t bytes);
+void DeviceFree(void* p, size_t bytes);
+
+// Owned device buffer of n elements.  Copyable (a pipe copies its template
+// into every ring slot): a copy allocates and copies the contents device to
+// device.  Replaces CUDA/DeviceMemory.cpp:19-79.
+template <typename T>
+class DeviceMemory {
+public:
+    DeviceMemory() = default;
+    explicit DeviceMemory(size_t n) { Malloc(n); }
+    DeviceMemory(const DeviceMemory& o)
+    {
+        Malloc(o.n_);
+        if (n_) CheckHip(hipMemcpy(p_, o.p_, n_ * sizeof(T), hipMemcpyDeviceToDevice), "DeviceMemory copy");
+    }
+    DeviceMemory(DeviceMemory&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+    DeviceMemory& operator=(const DeviceMemory& o)
+    {
+        if (this != &o) {
+            Free();
+            Malloc(o.n_);
+            if (n_) CheckHip(hipMemcpy(p_, o.p_, n_ * sizeof(T), hipMemcpyDeviceToDevice), "DeviceMemory copy");
+        }
+        return *this;
+    }
+    DeviceMemory& operator=(DeviceMemory&& o) noexcept
+    {
+        if (this != &o) { Free(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
+        return *this;
+    }
+    ~DeviceMemory() { Free(); }
+
+    T* Data() const { return p_; }
+    size_t Size() const { return n_; }
+    size_t Bytes() const { return n_ * sizeof(T); }
+
+private:
+    void Malloc(size_t n);
+    void Free()
+    {
+        DeviceFree(p_, n_ * sizeof(T));
+        p_ = nullptr;
+        n_ = 0;
+    }
+    T* p_ = nullptr;
+    size_t n_ = 0;
+};
+
+template <typename T>
+void DeviceMemory<T>::Malloc(size_t n)
+{
+    n_ = n;
+    p_ = n ? static_cast<T*>(DeviceMalloc(n * sizeof(T))) : nullptr;
+}
+
+}  // namespace hip
+
+// Per-agent HIP stream, created on first use on the calling thread.
+class StreamHolder {
+public:
+    ~StreamHolder();
+    hipStream_t Get();
+    void Sync();
+
+private:
+    hipStream_t s_ = nullptr;
+};
+
+class ProjectionUploadAgent : public SequentialAgentBase {
+public:
+    ProjectionUploadAgent(size_t width, size_t height);
+
+private:
+    void SetPipesImpl() override;
+    void WorkFlow0() override;
+    const size_t width_, height_;
+    std::shared_ptr<Pipe<UINT16Vec>> in_;
+    std::shared_ptr<Pipe<hip::DeviceMemory<uint16_t>>> out_;
+    StreamHolder stream_;
+};
+
+class PreProcessingHIPAgent : public SequentialAgentBase {
+public:
+    // dark (inH x inW) and gain (outH x outW) are optional extensions
+    PreProcessingHIPAgent(const ScanParams& scan, const std::vector<uint16_t>& dark = {},
+                          const std::vector<float>& gain = {}, int device = 0);
+    ~PreProcessingHIPAgent() override;
+    void GetReady() override;
+
+private:
+    void SetPipesImpl() override;
+    void WorkFlow0() override;
+    ScanParams scan_;
+    std::vector<uint16_t> darkHost_;
+    std::vector<float> gainHost_;
+    int device_;
+    jh_pre_plan* plan_ = nullptr;
+    hip::DeviceMemory<uint16_t> dark_;
+    hip::DeviceMemory<float> gain_;
+    std::shared_ptr<Pipe<hip::DeviceMemory<uint16_t>>> in_;
+    std::shared_ptr<Pipe<hip::DeviceMemory<float>>> out_;
+    StreamHolder stream_;
+};
+
+class FilterHIPAgent : public SequentialAgentBase {
+public:
+    FilterHIPAgent(const ScanParams& scan, const ReconParams& recon, const std::vector<double>& ptms, int device = 0);
+    ~FilterHIPAgent() override;
+    void GetReady() override;
+
+private:
+    void SetPipesImpl() override;
+    void WorkFlow0() override;
+    ScanParams scan_;
+    ReconParams recon_;
+    std::vector<double> ptms_;
+    int device_;
+    jh_filter_plan* plan_ = nullptr;
+    std::shared_ptr<Pipe<hip::DeviceMemory<float>>> in_, out_;
+    StreamHolder stream_;
+};
+
+// Backprojection of the views of each shot into the volume, written out as
+// slices at the shot end.  With numParts > 1 the volume is split into
+// z-slabs (the reference's "recon parts", BPCUDACore.cpp:32-38): shot p of
+// the input stream is back-projected into slab p % numParts.
+class BPHIPAgent : public SequentialAgentBase {
+public:
+    BPHIPAgent(const ScanParams& scan, const ReconParams& recon, const std::vector<double>& ptms,
+               size_t numParts = 1, int device = 0);
+    ~BPHIPAgent() override;
+    void GetReady() override;
+
+    // first slice and slice count of slab p
+    size_t PartStart(size_t p) const;
+    size_t PartSlices(size_t p) const;
+
+private:
+    void SetPipesImpl() override;
+    void WorkFlow0() override;
+    jh_bp_plan* PlanFor(size_t part);
+    ScanParams scan_;
+    ReconParams recon_;
+    std::vector<double> ptms_;
+    size_t numParts_;
+    int device_;
+    size_t part_ = SIZE_MAX;
+    jh_bp_plan* plan_ = nullptr;
+    hip::DeviceMemory<uint8_t> q_;
+    std::shared_ptr<Pipe<hip::DeviceMemory<float>>> in_, out_;
+    StreamHolder stream_;
+};
+
+class DeviceMemoryDownloadAgent : public SequentialAgentBase {
+public:
+    explicit DeviceMemoryDownloadAgent(size_t elements);
+
+private:
+    void SetPipesImpl() override;
+    void WorkFlow0() override;
+    const size_t n_;
+    std::shared_ptr<Pipe<hip::DeviceMemory<float>>> in_;
+    std::shared_ptr<Pipe<FloatVec>> out_;
+    StreamHolder stream_;
+};
+
+}  // namespace jh

@@ -1,0 +1,232 @@
+// Pipe<T>: bounded multi-producer / multi-consumer ring of elements between
+// agents, with shot framing and read overlap, handed out as RAII tokens.
+//
+// Interface of the reference's Pipeline/Pipe.h:19-66, PipeBase.h:19-154 and
+// the tokens of PipeReadToken.h / PipeWriteToken.h:
+//   SetTemplate(T&&, dims), SetProducer(name, nWriters, writeSize),
+//   SetConsumer(name, nReaders, readSize incl. overlap, overlap),
+//   SetBufferSize, GetWriteToken(size, isShotEnd), GetReadToken(), Close().
+// Semantics kept from the reference (PipeBase.cpp:147-331): positions are
+// global element indices; a write token waits until the ring has room both
+// behind unfinished writes and behind the oldest element a reader may still
+// need; a read token covers readSize elements (fewer at a shot end) and
+// repeats the last `overlap` elements of the previous token of the same shot;
+// tokens are released in order, when the last copy of a token is destroyed;
+// GetReadToken throws PipeClosedAndEmptySignal once the pipe is closed and
+// drained; a writer that can never make progress gets "Pipe deadlock".
+// Differences: tokens notify the pipe from their shared control block
+// instead of being polled by use count; a closed pipe whose last shot was
+// never finished drains to PipeClosedAndEmptySignal instead of blocking.
+#pragma once
+
+#include <condition_variable>
+#include <cstdint>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "Common.h"
+
+namespace jh {
+
+struct PipeClosedAndEmptySignal {};
+
+class PipeBase {
+public:
+    explicit PipeBase(std::string name);
+    virtual ~PipeBase();
+    PipeBase(const PipeBase&) = delete;
+    PipeBase& operator=(const PipeBase&) = delete;
+
+    void SetConsumer(const std::string& name, size_t number, size_t sizeIncludingOverlap, size_t overlapSize);
+    void SetProducer(const std::string& name, size_t number, size_t size);
+    void SetBufferSize(size_t size);
+
+    const std::string& GetName() const { return name_; }
+    const std::string& GetConsumer() const { return consumer_; }
+    const std::string& GetProducer() const { return producer_; }
+    size_t GetNumReaders() const { return numReaders_; }
+    size_t GetReadSize() const { return readSize_; }
+    size_t GetOverlapSize() const { return overlap_; }
+    size_t GetNumWriters() const { return numWriters_; }
+    size_t GetWriteSize() const { return writeSize_; }
+    size_t GetBufferSize() const { return bufferSize_; }
+    std::vector<size_t> GetElementShape() const { return shape_; }
+
+    size_t GetWritingPos();
+    size_t GetWriteDonePos();
+    size_t GetReadingPos();
+    size_t GetReadDonePos();
+
+    void Close();
+    bool Closed();
+    double GetProducerBlockedTime() const { return blocked_; }
+
+    // Token bookkeeping shared by the typed tokens.
+    struct Span {
+        size_t start = 0, size = 0, overlap = 0;
+        bool shotStart = false, shotEnd = false;
+    };
+    struct TokenState {
+        PipeBase* pipe;
+        bool write;
+        uint64_t id;
+        Span span;
+        ~TokenState() { pipe->Release(write, id); }
+    };
+
+protected:
+    void SetElementShape(const std::vector<size_t>& d) { shape_ = d; }
+    std::shared_ptr<TokenState> AcquireWrite(size_t size, bool isShotEnd);
+    std::shared_ptr<TokenState> AcquireRead();
+    // Called (under the lock) once producer, consumer and template are known.
+    virtual void InitBuffers() = 0;
+    virtual bool HasTemplate() const = 0;
+    void TryInit();
+    size_t bufferSize_ = 0;
+
+private:
+    friend struct TokenState;
+    void Release(bool write, uint64_t id);
+    size_t ReadStart() const;
+    size_t ReadEnd() const;
+    bool DataReady() const;
+    bool RoomToWrite(size_t size);
+
+    struct Slot {
+        uint64_t id;
+        size_t start, size;
+        bool released;
+    };
+    struct Shot {
+        size_t start, end;
+    };
+
+    const std::string name_;
+    std::string consumer_, producer_;
+    size_t numWriters_ = 0, writeSize_ = 0, numReaders_ = 0, readSize_ = 0, overlap_ = 0;
+    std::vector<size_t> shape_;
+    bool ready_ = false;
+    bool closed_ = false;
+    size_t writingPos_ = 0, writeDonePos_ = 0, readingPos_ = 0, readDonePos_ = 0;
+    uint64_t nextId_ = 1;
+    std::deque<Slot> writes_, reads_;
+    std::deque<Shot> shots_;
+    double blocked_ = 0.0;
+    std::mutex m_;
+    std::condition_variable cv_;
+};
+
+template <typename T>
+class PipeWriteToken {
+public:
+    PipeWriteToken() = default;
+    PipeWriteToken(std::shared_ptr<PipeBase::TokenState> st, T* const* ptrs) : st_(std::move(st)), ptrs_(ptrs) {}
+    PipeWriteToken(const PipeWriteToken&) = default;
+    PipeWriteToken(PipeWriteToken&&) = default;
+    PipeWriteToken& operator=(const PipeWriteToken& o)
+    {
+        if (st_) ThrowException("WriteToken is already assigned.");
+        st_ = o.st_;
+        ptrs_ = o.ptrs_;
+        return *this;
+    }
+    bool Null() const { return !st_; }
+    size_t GetStartIndex() const { return st_->span.start; }
+    size_t GetSize() const { return st_->span.size; }
+    bool IsShotStart() const { return st_->span.shotStart; }
+    bool IsShotEnd() const { return st_->span.shotEnd; }
+    T* const* GetBufferPtrs() const { return ptrs_; }
+    T& GetBuffer(size_t i) const { return *ptrs_[i]; }
+
+private:
+    std::shared_ptr<PipeBase::TokenState> st_;
+    T* const* ptrs_ = nullptr;
+};
+
+template <typename T>
+class PipeReadToken {
+public:
+    PipeReadToken() = default;
+    PipeReadToken(std::shared_ptr<PipeBase::TokenState> st, T* const* ptrs) : st_(std::move(st)), ptrs_(ptrs) {}
+    PipeReadToken(const PipeReadToken&) = default;
+    PipeReadToken(PipeReadToken&&) = default;
+    PipeReadToken& operator=(const PipeReadToken& o)
+    {
+        if (st_) ThrowException("ReadToken is already assigned.");
+        st_ = o.st_;
+        ptrs_ = o.ptrs_;
+        return *this;
+    }
+    void Reset() { st_.reset(); ptrs_ = nullptr; }
+    bool Null() const { return !st_; }
+    size_t GetStartIndex() const { return st_->span.start; }
+    size_t GetSize() const { return st_->span.size; }
+    size_t GetOverlapSize() const { return st_->span.overlap; }
+    bool IsShotStart() const { return st_->span.shotStart; }
+    bool IsShotEnd() const { return st_->span.shotEnd; }
+    const T* const* GetBufferPtrs() const { return ptrs_; }
+    const T& GetBuffer(size_t i) const { return *ptrs_[i]; }
+    T& GetMutableBuffer(size_t i) const { return *ptrs_[i]; }
+
+private:
+    std::shared_ptr<PipeBase::TokenState> st_;
+    T* const* ptrs_ = nullptr;
+};
+
+template <typename T>
+class Pipe : public PipeBase {
+public:
+    typedef PipeWriteToken<T> WriteToken;
+    typedef PipeReadToken<T> ReadToken;
+
+    explicit Pipe(const std::string& name) : PipeBase(name) {}
+
+    void SetTemplate(T&& elementTemplate, std::vector<size_t>&& dims)
+    {
+        if (!buffers_.empty() || hasTemplate_) ThrowExceptionAndLog("Pipe: buffer is already initialized.");
+        template_.push_back(std::move(elementTemplate));
+        hasTemplate_ = true;
+        SetElementShape(dims);
+        TryInit();
+    }
+
+    T GetTemplate() const
+    {
+        if (template_.empty()) ThrowException("Pipe: pipe is not initialized.");
+        return template_[0];
+    }
+
+    WriteToken GetWriteToken(size_t size, bool isShotEnd)
+    {
+        auto st = AcquireWrite(size, isShotEnd);
+        return WriteToken(st, ptrs_.data() + st->span.start % bufferSize_);
+    }
+
+    ReadToken GetReadToken()
+    {
+        auto st = AcquireRead();
+        return ReadToken(st, ptrs_.data() + st->span.start % bufferSize_);
+    }
+
+protected:
+    bool HasTemplate() const override { return hasTemplate_; }
+    void InitBuffers() override
+    {
+        buffers_.assign(bufferSize_, template_[0]);
+        // pointers repeated once so that a token crossing the end of the
+        // ring still sees contiguous element pointers
+        ptrs_.resize(2 * bufferSize_ - 1);
+        for (size_t i = 0; i < ptrs_.size(); ++i) ptrs_[i] = &buffers_[i % bufferSize_];
+    }
+
+private:
+    bool hasTemplate_ = false;
+    std::vector<T> template_;
+    std::vector<T> buffers_;
+    std::vector<T*> ptrs_;
+};
+
+}  // namespace jh

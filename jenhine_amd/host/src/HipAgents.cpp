@@ -1,0 +1,410 @@
+// Device-side agents (see HipAgents.h).
+#include "jh/HipAgents.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+
+namespace jh {
+
+const char* const kDeviceOOMPrefix = "HIP failed to malloc ";
+
+void CheckJh(jh_status s, const char* what)
+{
+    if (s != JH_OK) throw Exception(std::string(what) + ": " + jh_last_error());
+}
+
+void CheckHip(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) throw Exception(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+namespace hip {
+
+static std::atomic<size_t> g_allocated{0};
+
+static size_t ByteCap()
+{
+    const char* s = std::getenv("JH_MAX_DEVICE_BYTES");
+    return s ? (size_t)std::strtoull(s, nullptr, 10) : SIZE_MAX;
+}
+
+void* DeviceMalloc(size_t bytes)
+{
+    const size_t cap = ByteCap();
+    if (g_allocated.load() + bytes > cap)
+        throw Exception(std::string(kDeviceOOMPrefix) + std::to_string(bytes) + " bytes (cap)");
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess || !p) {
+        (void)hipGetLastError();
+        throw Exception(std::string(kDeviceOOMPrefix) + std::to_string(bytes) + " bytes");
+    }
+    g_allocated += bytes;
+    return p;
+}
+
+void DeviceFree(void* p, size_t bytes)
+{
+    if (p) {
+        (void)hipFree(p);
+        g_allocated -= bytes;
+    }
+}
+
+}  // namespace hip
+
+StreamHolder::~StreamHolder()
+{
+    if (s_) (void)hipStreamDestroy(s_);
+}
+
+hipStream_t StreamHolder::Get()
+{
+    if (!s_) CheckHip(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking), "hipStreamCreate");
+    return s_;
+}
+
+void StreamHolder::Sync() { CheckHip(hipStreamSynchronize(Get()), "hipStreamSynchronize"); }
+
+// ---------------------------------------------------------------------------
+// ProjectionUploadAgent
+// ---------------------------------------------------------------------------
+
+ProjectionUploadAgent::ProjectionUploadAgent(size_t width, size_t height)
+    : SequentialAgentBase("ProjectionUploadAgent", 1), width_(width), height_(height)
+{
+}
+
+void ProjectionUploadAgent::SetPipesImpl()
+{
+    in_ = BranchPipeFromTrunk<UINT16Vec>("Proj");
+    in_->SetConsumer(GetAgentName(), 1, 1, 0);
+    out_ = CreateNewPipe<hip::DeviceMemory<uint16_t>>("Proj");
+    out_->SetTemplate(hip::DeviceMemory<uint16_t>(width_ * height_), {width_, height_});
+    out_->SetProducer(GetAgentName(), 1, 1);
+    MergePipeToTrunk(out_);
+}
+
+void ProjectionUploadAgent::WorkFlow0()
+{
+    try {
+        for (;;) {
+            auto rt = in_->GetReadToken();
+            auto wt = out_->GetWriteToken(1, rt.IsShotEnd());
+            ThreadOccupiedScope occ(this);
+            const UINT16Vec& src = rt.GetBuffer(0);
+            if (src.size() != width_ * height_) ThrowExceptionAndLog(GetAgentName() + ": frame size mismatch");
+            CheckHip(hipMemcpyAsync(wt.GetBuffer(0).Data(), src.data(), src.size() * sizeof(uint16_t),
+                                    hipMemcpyHostToDevice, stream_.Get()),
+                     "upload");
+            WaitAsyncScope wait(&occ);
+            stream_.Sync();
+        }
+    } catch (PipeClosedAndEmptySignal&) {
+    }
+    out_->Close();
+}
+
+// ---------------------------------------------------------------------------
+// PreProcessingHIPAgent
+// ---------------------------------------------------------------------------
+
+PreProcessingHIPAgent::PreProcessingHIPAgent(const ScanParams& scan, const std::vector<uint16_t>& dark,
+                                             const std::vector<float>& gain, int device)
+    : SequentialAgentBase("PreProcessingHIPAgent", 1), scan_(scan), darkHost_(dark), gainHost_(gain), device_(device)
+{
+    scan_.Derive();
+}
+
+PreProcessingHIPAgent::~PreProcessingHIPAgent()
+{
+    if (plan_) jh_pre_destroy(plan_);
+}
+
+void PreProcessingHIPAgent::SetPipesImpl()
+{
+    in_ = BranchPipeFromTrunk<hip::DeviceMemory<uint16_t>>("Proj");
+    in_->SetConsumer(GetAgentName(), 1, 1, 0);
+    out_ = CreateNewPipe<hip::DeviceMemory<float>>("Proj");
+    out_->SetTemplate(hip::DeviceMemory<float>(scan_.NumUsedDetsU * scan_.NumUsedDetsV),
+                      {scan_.NumUsedDetsU, scan_.NumUsedDetsV});
+    out_->SetProducer(GetAgentName(), 1, 1);
+    MergePipeToTrunk(out_);
+}
+
+void PreProcessingHIPAgent::GetReady()
+{
+    CheckHip(hipSetDevice(device_), "hipSetDevice");
+    const float* bh = scan_.BeamHardeningParams.data();
+    if (scan_.BeamHardeningParams.size() != 3) ThrowException("The number of BeamHardeningParams should be 3.");
+    CheckJh(jh_pre_create(&plan_, device_, (int)scan_.NumDetsU, (int)scan_.NumDetsV, (int)scan_.BorderSizeUp,
+                          (int)scan_.BorderSizeDown, (int)scan_.BorderSizeLeft, (int)scan_.BorderSizeRight,
+                          (int)scan_.NumUsedDetsU, (int)scan_.NumUsedDetsV, scan_.BrightField, bh),
+            "jh_pre_create");
+    if (!darkHost_.empty()) {
+        dark_ = hip::DeviceMemory<uint16_t>(darkHost_.size());
+        CheckHip(hipMemcpy(dark_.Data(), darkHost_.data(), dark_.Bytes(), hipMemcpyHostToDevice), "dark upload");
+    }
+    if (!gainHost_.empty()) {
+        gain_ = hip::DeviceMemory<float>(gainHost_.size());
+        CheckHip(hipMemcpy(gain_.Data(), gainHost_.data(), gain_.Bytes(), hipMemcpyHostToDevice), "gain upload");
+    }
+}
+
+void PreProcessingHIPAgent::WorkFlow0()
+{
+    CheckHip(hipSetDevice(device_), "hipSetDevice");
+    try {
+        for (;;) {
+            auto rt = in_->GetReadToken();
+            auto wt = out_->GetWriteToken(1, rt.IsShotEnd());
+            ThreadOccupiedScope occ(this);
+            CheckJh(jh_pre_run(plan_, rt.GetBuffer(0).Data(), dark_.Data(), gain_.Data(), wt.GetBuffer(0).Data(), 1,
+                               stream_.Get()),
+                    "jh_pre_run");
+            WaitAsyncScope wait(&occ);
+            stream_.Sync();
+        }
+    } catch (PipeClosedAndEmptySignal&) {
+    }
+    out_->Close();
+}
+
+// ---------------------------------------------------------------------------
+// FilterHIPAgent
+// ---------------------------------------------------------------------------
+
+FilterHIPAgent::FilterHIPAgent(const ScanParams& scan, const ReconParams& recon, const std::vector<double>& ptms,
+                               int device)
+    : SequentialAgentBase("FilterHIPAgent", 1), scan_(scan), recon_(recon), ptms_(ptms), device_(device)
+{
+    scan_.Derive();
+}
+
+FilterHIPAgent::~FilterHIPAgent()
+{
+    if (plan_) jh_filter_destroy(plan_);
+}
+
+void FilterHIPAgent::SetPipesImpl()
+{
+    in_ = BranchPipeFromTrunk<hip::DeviceMemory<float>>("Proj");
+    in_->SetConsumer(GetAgentName(), 1, 1, 0);
+    out_ = CreateNewPipe<hip::DeviceMemory<float>>("Proj");
+    out_->SetTemplate(hip::DeviceMemory<float>(scan_.NumUsedDetsU * scan_.NumUsedDetsV),
+                      {scan_.NumUsedDetsU, scan_.NumUsedDetsV});
+    out_->SetProducer(GetAgentName(), 1, 1);
+    MergePipeToTrunk(out_);
+}
+
+void FilterHIPAgent::GetReady()
+{
+    CheckHip(hipSetDevice(device_), "hipSetDevice");
+    const int nv = (int)(ptms_.size() / 12);
+    const int nAdj = (int)recon_.FilterAdjustPoints.size();
+    if (recon_.FilterAdjustLevelInDB.size() != (size_t)nAdj) ThrowException("filter adjust points and levels differ");
+    CheckJh(jh_filter_create(&plan_, device_, (int)scan_.NumUsedDetsU, (int)scan_.NumUsedDetsV, nv, ptms_.data(),
+                             scan_.DSO, scan_.HalfSampleRate, recon_.FilterCutOffStart, recon_.FilterCutOffEnd,
+                             nAdj ? recon_.FilterAdjustPoints.data() : nullptr,
+                             nAdj ? recon_.FilterAdjustLevelInDB.data() : nullptr, nAdj, recon_.FilterWindow,
+                             recon_.FilterMode),
+            "jh_filter_create");
+}
+
+void FilterHIPAgent::WorkFlow0()
+{
+    CheckHip(hipSetDevice(device_), "hipSetDevice");
+    const size_t nv = ptms_.size() / 12;
+    try {
+        for (;;) {
+            auto rt = in_->GetReadToken();
+            auto wt = out_->GetWriteToken(1, rt.IsShotEnd());
+            ThreadOccupiedScope occ(this);
+            const int view = (int)(rt.GetStartIndex() % nv);
+            CheckJh(jh_filter_run(plan_, rt.GetBuffer(0).Data(), wt.GetBuffer(0).Data(), view, 1, stream_.Get()),
+                    "jh_filter_run");
+            WaitAsyncScope wait(&occ);
+            stream_.Sync();
+        }
+    } catch (PipeClosedAndEmptySignal&) {
+    }
+    out_->Close();
+}
+
+// ---------------------------------------------------------------------------
+// BPHIPAgent
+// ---------------------------------------------------------------------------
+
+BPHIPAgent::BPHIPAgent(const ScanParams& scan, const ReconParams& recon, const std::vector<double>& ptms,
+                       size_t numParts, int device)
+    : SequentialAgentBase("BPHIPAgent", 1), scan_(scan), recon_(recon), ptms_(ptms), numParts_(numParts),
+      device_(device)
+{
+    scan_.Derive();
+    if (numParts_ < 1 || numParts_ > recon_.NumPixelsZ) ThrowException("BPHIPAgent: bad number of parts");
+    if (recon_.ViewsPerChunk < 1) ThrowException("BPHIPAgent: ViewsPerChunk must be positive");
+}
+
+BPHIPAgent::~BPHIPAgent()
+{
+    if (plan_) jh_bp_destroy(plan_);
+}
+
+size_t BPHIPAgent::PartStart(size_t p) const
+{
+    const size_t per = (recon_.NumPixelsZ + numParts_ - 1) / numParts_;
+    return std::min(p * per, recon_.NumPixelsZ);
+}
+
+size_t BPHIPAgent::PartSlices(size_t p) const
+{
+    const size_t per = (recon_.NumPixelsZ + numParts_ - 1) / numParts_;
+    return std::min(per, recon_.NumPixelsZ - PartStart(p));
+}
+
+void BPHIPAgent::SetPipesImpl()
+{
+    in_ = BranchPipeFromTrunk<hip::DeviceMemory<float>>("Proj");
+    in_->SetConsumer(GetAgentName(), 1, recon_.ViewsPerChunk, 0);
+    out_ = CreateNewPipe<hip::DeviceMemory<float>>("Slice");
+    out_->SetTemplate(hip::DeviceMemory<float>(recon_.NumPixelsX * recon_.NumPixelsY),
+                      {recon_.NumPixelsX, recon_.NumPixelsY});
+    out_->SetProducer(GetAgentName(), 1, 1);
+    MergePipeToTrunk(out_);
+}
+
+// Plan of slab p: its slices, with the z offset of the slab folded into the
+// matrices (ProjectionMatrixModifier::MoveZAxis semantics, in FP64 on the
+// device).
+jh_bp_plan* BPHIPAgent::PlanFor(size_t p)
+{
+    if (part_ == p) return plan_;
+    if (plan_) {
+        CheckJh(jh_bp_destroy(plan_), "jh_bp_destroy");
+        plan_ = nullptr;
+    }
+    const size_t nv = ptms_.size() / 12;
+    const size_t z0 = PartStart(p), nz = PartSlices(p);
+    const double Z = (double)recon_.NumPixelsZ;
+    const double dz = ((double)z0 + ((double)nz - 1.0) / 2.0 - (Z - 1.0) / 2.0) * (double)recon_.PitchZ;
+    std::vector<double> ptm(ptms_);
+    if (dz != 0.0) {
+        double* d = nullptr;
+        CheckHip(hipMalloc(&d, ptm.size() * sizeof(double)), "hipMalloc ptm");
+        CheckHip(hipMemcpy(d, ptm.data(), ptm.size() * sizeof(double), hipMemcpyHostToDevice), "ptm upload");
+        const jh_status s = jh_ptm_modify(d, (int)nv, JH_PTM_MOVE_Z, dz, nullptr);
+        if (s == JH_OK) (void)hipMemcpy(ptm.data(), d, ptm.size() * sizeof(double), hipMemcpyDeviceToHost);
+        (void)hipFree(d);
+        CheckJh(s, "jh_ptm_modify");
+    }
+    CheckJh(jh_bp_create(&plan_, device_, ptm.data(), (int)nv, (int)scan_.NumUsedDetsU, (int)scan_.NumUsedDetsV,
+                         (int)recon_.NumPixelsX, (int)recon_.NumPixelsY, (int)nz, recon_.PitchXY, recon_.PitchZ),
+            "jh_bp_create");
+    // the accumulators are allocated here so that a shortage shows up now
+    CheckJh(jh_bp_reset(plan_, stream_.Get()), "jh_bp_reset");
+    stream_.Sync();
+    part_ = p;
+    return plan_;
+}
+
+void BPHIPAgent::GetReady()
+{
+    CheckHip(hipSetDevice(device_), "hipSetDevice");
+    PlanFor(0);
+    size_t bytes = 0;
+    CheckJh(jh_bp_stage_bytes(plan_, recon_.StorageType, (int)recon_.ViewsPerChunk, &bytes), "jh_bp_stage_bytes");
+    q_ = hip::DeviceMemory<uint8_t>(bytes);
+}
+
+void BPHIPAgent::WorkFlow0()
+{
+    CheckHip(hipSetDevice(device_), "hipSetDevice");
+    const size_t nv = ptms_.size() / 12;
+    size_t frameBytes = 0;
+    size_t shot = 0;
+    try {
+        for (;;) {
+            auto rt = in_->GetReadToken();
+            ThreadOccupiedScope occ(this);
+            jh_bp_plan* plan = PlanFor(shot % numParts_);
+            if (!frameBytes) CheckJh(jh_bp_stage_bytes(plan, recon_.StorageType, 1, &frameBytes), "stage bytes");
+            if (rt.IsShotStart()) CheckJh(jh_bp_reset(plan, stream_.Get()), "jh_bp_reset");
+            // stage the chunk's views contiguously, then accumulate runs of
+            // consecutive view indices
+            const size_t n = rt.GetSize();
+            size_t runStart = 0;
+            for (size_t i = 0; i < n; ++i) {
+                const int view = (int)((rt.GetStartIndex() + i) % nv);
+                CheckJh(jh_bp_stage(plan, rt.GetBuffer(i).Data(), view, 1, recon_.StorageType,
+                                    q_.Data() + i * frameBytes, stream_.Get()),
+                        "jh_bp_stage");
+                const bool last = i + 1 == n;
+                const bool wraps = !last && (int)((rt.GetStartIndex() + i + 1) % nv) != view + 1;
+                if (last || wraps) {
+                    const int v0 = (int)((rt.GetStartIndex() + runStart) % nv);
+                    CheckJh(jh_bp_accumulate(plan, q_.Data() + runStart * frameBytes, recon_.StorageType, v0,
+                                             (int)(i + 1 - runStart), stream_.Get()),
+                            "jh_bp_accumulate");
+                    runStart = i + 1;
+                }
+            }
+            {
+                WaitAsyncScope wait(&occ);
+                stream_.Sync();
+            }
+            if (rt.IsShotEnd()) {
+                rt.Reset();  // views are no longer needed
+                const size_t nz = PartSlices(shot % numParts_);
+                for (size_t z = 0; z < nz; ++z) {
+                    auto wt = out_->GetWriteToken(1, z + 1 == nz);
+                    CheckJh(jh_bp_finalize(plan, wt.GetBuffer(0).Data(), (int)z, 1, stream_.Get()), "jh_bp_finalize");
+                    stream_.Sync();
+                }
+                ++shot;
+            }
+        }
+    } catch (PipeClosedAndEmptySignal&) {
+    }
+    out_->Close();
+}
+
+// ---------------------------------------------------------------------------
+// DeviceMemoryDownloadAgent
+// ---------------------------------------------------------------------------
+
+DeviceMemoryDownloadAgent::DeviceMemoryDownloadAgent(size_t elements)
+    : SequentialAgentBase("DeviceMemoryDownloadAgent", 1), n_(elements)
+{
+}
+
+void DeviceMemoryDownloadAgent::SetPipesImpl()
+{
+    in_ = BranchPipeFromTrunk<hip::DeviceMemory<float>>("Slice");
+    in_->SetConsumer(GetAgentName(), 1, 1, 0);
+    out_ = CreateNewPipe<FloatVec>("Slice");
+    out_->SetTemplate(FloatVec(n_), {n_});
+    out_->SetProducer(GetAgentName(), 1, 1);
+    MergePipeToTrunk(out_);
+}
+
+void DeviceMemoryDownloadAgent::WorkFlow0()
+{
+    try {
+        for (;;) {
+            auto rt = in_->GetReadToken();
+            auto wt = out_->GetWriteToken(1, rt.IsShotEnd());
+            ThreadOccupiedScope occ(this);
+            CheckHip(hipMemcpyAsync(wt.GetBuffer(0).data(), rt.GetBuffer(0).Data(), n_ * sizeof(float),
+                                    hipMemcpyDeviceToHost, stream_.Get()),
+                     "download");
+            WaitAsyncScope wait(&occ);
+            stream_.Sync();
+        }
+    } catch (PipeClosedAndEmptySignal&) {
+    }
+    out_->Close();
+}
+
+}  // namespace jh

@@ -1,0 +1,85 @@
+"""The C++ agent graph on the GPU (jh_fdk_pipeline): upload -> preprocess ->
+filter -> BPHIPAgent -> download, against the Python path over the same C-ABI
+and against the CPU oracle; and the retry with more z-slabs when set-up runs
+out of device memory (forced with the JH_MAX_DEVICE_BYTES test hook)."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+import torch
+
+from jenhine_amd import configs
+from oracle import oracle as o
+from tests.helpers import oracle_params, rel_rmse
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+APP = os.path.join(ROOT, "jenhine_amd", "host", "build", "jh_fdk_pipeline")
+W = configs.CBCT128.scaled(n_views=24, X=64, Y=64, Z=48, pxy=0.5, pz=0.5)
+
+
+def _run(tmp_path, d, storage=0, filt=0, chunk=5, env=None, parts=1):
+    paths = {}
+    for k, dt in (("raw", np.uint16), ("ptm", np.float64), ("dark", np.uint16), ("gain", np.float32)):
+        paths[k] = str(tmp_path / f"{k}.bin")
+        np.ascontiguousarray(d[k], dt).tofile(paths[k])
+    out = str(tmp_path / "vol.bin")
+    scan, rec = W.scan_params(), W.recon_params()
+    args = [APP, f"views={W.n_views}", f"dso={W.sod}", f"dsd={W.sdd}", f"detu={W.n_u}", f"detv={W.n_v}",
+            f"pitch={W.du}", f"bright_field={scan.BrightField}",
+            "bh=" + "|".join(str(b) for b in scan.BeamHardeningParams), f"x={W.X}", f"y={W.Y}", f"z={W.Z}",
+            f"pitch_xy={W.pxy}", f"pitch_z={W.pz}", f"cut_start={rec.FilterCutOffStart}",
+            f"cut_end={rec.FilterCutOffEnd}", f"window={rec.FilterWindow}", f"filter={filt}",
+            f"storage={storage}", f"chunk={chunk}", f"parts={parts}", f"out={out}"] + \
+           [f"{k}={v}" for k, v in paths.items()]
+    e = dict(os.environ)
+    e.update(env or {})
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120, env=e)
+    print(r.stdout, r.stderr[-2000:])
+    assert r.returncode == 0, r.stderr[-2000:]
+    info = json.loads(r.stdout.strip().splitlines()[-1])
+    return np.fromfile(out, np.float32).reshape(W.Z, W.Y, W.X), info
+
+
+@pytest.fixture(scope="module")
+def data():
+    return W.data()
+
+
+def test_agent_graph_matches_python_path_and_oracle(gpu, data, tmp_path):
+    if not os.path.exists(APP):
+        pytest.fail("jh_fdk_pipeline is not built")
+    vol, info = _run(tmp_path, data)
+    assert info["parts"] == 1
+    from jenhine_amd.recon import FDK
+    fdk = FDK(W.scan_params(), W.recon_params(), data["ptm"], "fp32", "f32", gpu)
+    ref = fdk(torch.from_numpy(data["raw"].view(np.int16)).to(gpu), torch.from_numpy(data["dark"].view(np.int16)).to(gpu),
+              torch.from_numpy(data["gain"]).to(gpu)).cpu().numpy()
+    assert rel_rmse(vol, ref) <= 1e-6
+    sc, rc = oracle_params(W)
+    v32 = o.fdk(data["raw"], data["ptm"], sc, rc, dark=data["dark"], gain=data["gain"])
+    e = rel_rmse(vol, v32)
+    print("agent graph vs FP32-ref oracle", e)
+    assert e <= 1e-4
+
+
+def test_agent_graph_bf16_mfma(gpu, data, tmp_path):
+    vol, _ = _run(tmp_path, data, storage=1, filt=1, chunk=8)
+    sc, rc = oracle_params(W)
+    v64 = o.fdk(data["raw"], data["ptm"], sc, rc, fp64=True, dark=data["dark"], gain=data["gain"])
+    e = rel_rmse(vol, v64)
+    print("agent graph bf16/mfma vs FP64", e)
+    assert e <= 1e-3
+
+
+def test_device_shortage_retries_with_more_parts(gpu, data, tmp_path):
+    full, _ = _run(tmp_path, data)
+    # the accumulator and count volumes of one part need 64*64*48*8 bytes;
+    # cap the library below that so set-up fails and is retried in slabs
+    cap = 64 * 64 * 48 * 8 * 2 // 3
+    vol, info = _run(tmp_path, data, env={"JH_MAX_DEVICE_BYTES": str(cap)})
+    assert info["parts"] >= 2
+    assert rel_rmse(vol, full) <= 1e-6
