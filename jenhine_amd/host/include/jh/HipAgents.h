@@ -42,8 +42,9 @@ void CheckHip(hipError_t e, const char* what);
 
 namespace hip {
 
-// Throws with kDeviceOOMPrefix when the allocation fails or would pass the
-// byte cap set by JH_MAX_DEVICE_BYTES (a test hook for the part-retry path).
+// Throws with kDeviceOOMPrefix when the allocation fails.  (The library's
+// plans, which hold the large buffers, also honour the JH_MAX_DEVICE_BYTES
+// test cap; pipe frames allocated here do not.)
 void* DeviceMalloc(size_t bytes);
 void DeviceFree(void* p, size_t bytes);
 

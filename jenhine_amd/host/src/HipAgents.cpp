@@ -25,17 +25,8 @@ namespace hip {
 
 static std::atomic<size_t> g_allocated{0};
 
-static size_t ByteCap()
-{
-    const char* s = std::getenv("JH_MAX_DEVICE_BYTES");
-    return s ? (size_t)std::strtoull(s, nullptr, 10) : SIZE_MAX;
-}
-
 void* DeviceMalloc(size_t bytes)
 {
-    const size_t cap = ByteCap();
-    if (g_allocated.load() + bytes > cap)
-        throw Exception(std::string(kDeviceOOMPrefix) + std::to_string(bytes) + " bytes (cap)");
     void* p = nullptr;
     if (hipMalloc(&p, bytes) != hipSuccess || !p) {
         (void)hipGetLastError();

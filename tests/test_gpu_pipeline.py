@@ -77,9 +77,12 @@ def test_agent_graph_bf16_mfma(gpu, data, tmp_path):
 
 def test_device_shortage_retries_with_more_parts(gpu, data, tmp_path):
     full, _ = _run(tmp_path, data)
-    # the accumulator and count volumes of one part need 64*64*48*8 bytes;
-    # cap the library below that so set-up fails and is retried in slabs
-    cap = 64 * 64 * 48 * 8 * 2 // 3
+    # the library's plans need ~1.1 MB for the filter (Toeplitz operand and
+    # pre-weight maps) plus 64*64*48*8 = 1.57 MB of accumulator and count
+    # volumes for one part; a 2.2 MB cap makes one part fail and two fit
+    cap = 2_200_000
     vol, info = _run(tmp_path, data, env={"JH_MAX_DEVICE_BYTES": str(cap)})
     assert info["parts"] >= 2
-    assert rel_rmse(vol, full) <= 1e-6
+    # a slab's matrices carry its z offset, so its float coordinates round
+    # differently from the one-part volume's (observed ~4e-6)
+    assert rel_rmse(vol, full) <= 2e-5
