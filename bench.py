@@ -8,8 +8,9 @@ hardening, seeded 40-ellipsoid + tooth phantom, per-pixel gain and dark.
 One step = the whole reconstruction on the device with the raw counts already
 resident in HBM: preprocess -> row filter -> stage -> backprojection ->
 normalised volume.  Two forms are timed:
-  * fp32: FP32 Toeplitz filter on the vector ALUs, FP32 projection storage;
-  * bf16: MFMA hi/lo split-BF16 filter, BF16 projection storage (headline);
+  * fp32: FP32 Toeplitz filter on MFMA, FP32 projection storage;
+  * bf16: hi/mid/lo split-BF16 Toeplitz filter on MFMA (6 products, FP32
+    accumulation), BF16 projection storage (headline);
   * f16: the same filter, FP16 projection storage.
 value = voxel-updates per second of the headline form over the whole step
 (X*Y*Z*N / step time), in GUPS.
@@ -53,7 +54,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-slices", type=int, default=16)
     ap.add_argument("--acc-slices", type=int, default=4)
-    ap.add_argument("--kernel-only", action="store_true", help="only the bp kernel loop (profiling)")
+    ap.add_argument("--kernel-only", action="store_true",
+                    help="profiling run: a 1 GiB copy, then only the BF16 backprojection kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -78,8 +80,27 @@ def main():
     gain = torch.from_numpy(d["gain"]).to(dev)
     scan, rec = w.scan_params(), w.recon_params()
     forms = {"fp32": FDK(scan, rec, d["ptm"], "fp32", "f32", dev),
-             "bf16": FDK(scan, rec, d["ptm"], "mfma", "bf16", dev),
-             "f16": FDK(scan, rec, d["ptm"], "mfma", "f16", dev)}
+             "bf16": FDK(scan, rec, d["ptm"], "mfma6", "bf16", dev),
+             "f16": FDK(scan, rec, d["ptm"], "mfma6", "f16", dev)}
+    if args.kernel_only:
+        # profiling run: a plain copy of known size (scales the HBM counters),
+        # then the BF16 backprojection kernel `steps` times
+        src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        fdk = forms["bf16"]
+        q = fdk.bp.stage(fdk.filter.process(fdk.pre.process(raw, dark, gain)), "bf16")
+        vol = fdk.bp.run(q, "bf16")
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            fdk.bp.run(q, "bf16", out=vol)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t) / args.steps * 1e3
+        log(f"bp_kernel bf16: {ms:.2f} ms/launch, copy kernel moved {2 * src.numel() * 4} bytes")
+        print(json.dumps({"kernel_only": True, "bp_ms": ms, "copy_bytes": 2 * src.numel() * 4}))
+        return
     units = w.gups_units
 
     results = {}

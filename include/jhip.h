@@ -34,9 +34,11 @@ typedef int jh_status;
 #define JH_BF16 1
 #define JH_F16 2
 
-/* Row-filter arithmetic. */
-#define JH_FILTER_FP32 0        /* FP32 Toeplitz GEMM on the vector ALUs */
-#define JH_FILTER_MFMA_BF16X3 1 /* Toeplitz GEMM on MFMA, hi/lo split BF16 (3 products), FP32 accumulate */
+/* Row-filter arithmetic (all forms: Toeplitz GEMM, FP32 accumulation). */
+#define JH_FILTER_FP32 0        /* FP32 operands on MFMA (v_mfma_f32_16x16x4_f32) */
+#define JH_FILTER_MFMA_BF16X3 1 /* hi/lo split BF16 on MFMA, 3 products */
+#define JH_FILTER_MFMA_BF16X6 2 /* hi/mid/lo split BF16 on MFMA, 6 products */
+#define JH_FILTER_FP32_VALU 3   /* FP32 FMA on the vector ALUs (cross-check) */
 
 /* Filter windows. */
 #define JH_WINDOW_REFERENCE 0   /* FilterCore.cpp:96-134 cut-off semantics */

@@ -14,10 +14,14 @@
 // A is formed on the fly from the frame and the pre-weight map; B is built
 // once per plan (FP64 on the host, rounded once).
 //
-// Two arithmetic forms:
-//  * JH_FILTER_FP32: FP32 FMA on the vector ALUs (LDS-tiled SIMT GEMM);
+// Arithmetic forms:
+//  * JH_FILTER_FP32: FP32 on MFMA (v_mfma_f32_16x16x4_f32);
 //  * JH_FILTER_MFMA_BF16X3: MFMA 16x16x32 BF16 with hi/lo split operands,
-//    hi*hi + hi*lo + lo*hi products accumulated in FP32.
+//    hi*hi + hi*lo + lo*hi products accumulated in FP32;
+//  * JH_FILTER_MFMA_BF16X6: hi/mid/lo split, the 6 products of total order
+//    <= 2 (hh, hm, mh, hl, lh, mm);
+//  * JH_FILTER_FP32_VALU: FP32 FMA on the vector ALUs (LDS-tiled SIMT GEMM),
+//    kept as a cross-check of the MFMA forms.
 //
 // Pre-filter weight maps (FilterAgent/PreFilterWeightGenerator.cpp:10-124)
 // are built on the device in FP64, one per refresh block of 9 views
@@ -34,8 +38,8 @@ struct jh_filter_plan {
     int nblocks;
     std::vector<int> view_block;  // block index of every view
     std::vector<double> H;        // L/2 + 1
-    jh::DevBuf B;                 // Kp x W float (FP32 form)
-    jh::DevBuf Bhl;               // MFMA form: packed hi/lo BF16 operand
+    jh::DevBuf B;                 // Kp x W float (vector-ALU form)
+    jh::DevBuf Bhl;               // MFMA forms: packed FP32 or BF16-part operand
     jh::DevBuf pw;                // nblocks x V x W float
     jh::DevBuf blk;               // N int: block of every view (device copy)
 };
@@ -204,99 +208,186 @@ __global__ void __launch_bounds__(256) filter_fp32_kernel(FiltArgs a)
 }
 
 // ---------------------------------------------------------------------------
-// MFMA Toeplitz GEMM with split BF16 (hi/lo).
+// Toeplitz GEMM on MFMA.
 //
-// v_mfma_f32_16x16x32_bf16: lane l supplies A[row = l%16][k = 8*(l/16) .. +7]
-// and B[k = 8*(l/16) .. +7][col = l%16]; it receives D[row = 4*(l/16) + i]
-// [col = l%16], i = 0..3.
-// Workgroup: 256 threads = 4 waves, tile 64 rows x 64 columns; wave w owns
-// rows 16*w..16*w+15 and all 4 column blocks of 16.  K step 32.
-// B operand layout (built on the host): for each (k-step s, column block cb,
-// lane l) 16 bf16 values = hi[8] then lo[8] of B[32 s + 8 (l/16) + 0..7]
-// [16 cb + l%16], i.e. one 32-byte load per lane per column block.
+// Workgroup: 256 threads = 4 waves, tile 128 rows x 128 columns; wave w owns
+// rows 32w..32w+31 (two 16-row blocks) and all 8 column blocks of 16.  K step
+// 32.  In every form lane l supplies, for its row (l % 16) and column
+// (l % 16), the 8 consecutive k values 32 s + 8 (l / 16) + 0..7:
+//  * BF16 forms use v_mfma_f32_16x16x32_bf16, whose operand layout is exactly
+//    that; the operands are split into P = 2 (hi, lo) or P = 3 (hi, mid, lo)
+//    BF16 parts and every product of parts whose orders sum to <= P-1 is
+//    accumulated in FP32 (3 or 6 products), smallest first;
+//  * the FP32 form issues 8 v_mfma_f32_16x16x4_f32, sub-step q taking
+//    k = 32 s + 8 (l / 16) + q (a permutation of k applied to both operands).
+// D layout (16x16 family): lane l holds D[4 (l / 16) + i][l % 16], i = 0..3.
+// The B operand is packed on the host per (k step, column block, lane) and
+// staged through LDS once per workgroup and k step, with the next step's
+// tile prefetched into registers during the MFMAs.
 // ---------------------------------------------------------------------------
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 
-__device__ inline void split_bf16(float f, __bf16& hi, __bf16& lo)
+constexpr int kModeF32 = 0, kModeX3 = 2, kModeX6 = 3;
+
+template <int MODE>
+struct FiltMode;
+template <>
+struct FiltMode<kModeF32> { static constexpr int parts = 1; static constexpr int lane_bytes = 32; };
+template <>
+struct FiltMode<kModeX3> { static constexpr int parts = 2; static constexpr int lane_bytes = 32; };
+template <>
+struct FiltMode<kModeX6> { static constexpr int parts = 3; static constexpr int lane_bytes = 48; };
+
+__device__ inline float bf16_round(float f)
 {
-    // hi = round-to-nearest-even BF16 of f, lo = BF16 of the remainder
     uint32_t u = __float_as_uint(f);
-    uint32_t r = (u + 0x7fffu + ((u >> 16) & 1u)) & 0xffff0000u;
-    const float fh = __uint_as_float(r);
-    const float rem = f - fh;
-    uint32_t v = __float_as_uint(rem);
-    v = (v + 0x7fffu + ((v >> 16) & 1u)) & 0xffff0000u;
-    hi = __builtin_bit_cast(__bf16, (unsigned short)(r >> 16));
-    lo = __builtin_bit_cast(__bf16, (unsigned short)(v >> 16));
+    u = (u + 0x7fffu + ((u >> 16) & 1u)) & 0xffff0000u;
+    return __uint_as_float(u);
 }
 
+__device__ inline __bf16 to_bf16(float rounded) { return __builtin_bit_cast(__bf16, (unsigned short)(__float_as_uint(rounded) >> 16)); }
+
+// A operand element (r, m): pre-weighted input, or one of the three edge
+// columns (unweighted x0, x1, x2), or padding.
+__device__ inline void load_a8(const FiltArgs& a, const float* xr, const float* pwr, int kb, float e[8])
+{
+    if (kb + 8 <= a.W) {
+        const float4 x0 = *reinterpret_cast<const float4*>(xr + kb);
+        const float4 x1 = *reinterpret_cast<const float4*>(xr + kb + 4);
+        const float4 w0 = *reinterpret_cast<const float4*>(pwr + kb);
+        const float4 w1 = *reinterpret_cast<const float4*>(pwr + kb + 4);
+        e[0] = x0.x * w0.x; e[1] = x0.y * w0.y; e[2] = x0.z * w0.z; e[3] = x0.w * w0.w;
+        e[4] = x1.x * w1.x; e[5] = x1.y * w1.y; e[6] = x1.z * w1.z; e[7] = x1.w * w1.w;
+    } else {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int m = kb + q;
+            e[q] = m < a.W ? xr[m] * pwr[m] : (m < a.W + 3 ? xr[m - a.W] : 0.0f);
+        }
+    }
+}
+
+template <int MODE>
 __global__ void __launch_bounds__(256) filter_mfma_kernel(FiltArgs a)
 {
+    typedef FiltMode<MODE> M;
+    constexpr int P = M::parts;
+    constexpr int LB = M::lane_bytes;                 // B bytes per lane per column block
+    constexpr int TILE_BYTES = 8 * 64 * LB;           // B bytes per k step per workgroup
+    constexpr int CHUNKS = TILE_BYTES / 16 / 256;     // 16-byte chunks per thread
+    __shared__ __attribute__((aligned(16))) unsigned char lds[TILE_BYTES];
+
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int row0 = blockIdx.y * 64 + wave * 16;
-    const int col0 = blockIdx.x * 64;
-    const int ncb = (a.W + 15) / 16;  // column blocks of 16 in the whole matrix
-    const int cb0 = col0 / 16;
-    // this lane's A row
-    const int r = min(row0 + (lane & 15), a.rows - 1);
-    const int view = r / a.V, vrow = r - view * a.V;
-    const float* xr = a.x + (size_t)r * a.W;
-    const float* pwr = a.pw + ((size_t)a.blk[a.view0 + view] * a.V + vrow) * a.W;
+    const int ncb8 = (a.W + 127) / 128 * 8;            // packed column blocks (multiple of 8)
+    const int cb0 = blockIdx.x * 8;
+    const int rowTile = blockIdx.y * 128 + wave * 32;
     const int kq = 8 * (lane >> 4);
-    f32x4_t acc[4];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) acc[c] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
-    const bf16x8_t* Bp = reinterpret_cast<const bf16x8_t*>(a.Bhl);
     const int nsteps = a.Kp / 32;
+
+    const float* xr[2];
+    const float* pwr[2];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb) {
+        const int r = min(rowTile + rb * 16 + (lane & 15), a.rows - 1);
+        const int view = r / a.V, vrow = r - view * a.V;
+        xr[rb] = a.x + (size_t)r * a.W;
+        pwr[rb] = a.pw + ((size_t)a.blk[a.view0 + view] * a.V + vrow) * a.W;
+    }
+
+    f32x4_t acc[2][8];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int c = 0; c < 8; ++c) acc[rb][c] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
+
+    const uint4* gB = reinterpret_cast<const uint4*>(a.Bhl);
+    auto gsrc = [&](int s) { return gB + ((size_t)s * ncb8 + cb0) * 64 * LB / 16; };
+    uint4 pre[CHUNKS];
+#pragma unroll
+    for (int c = 0; c < CHUNKS; ++c) pre[c] = gsrc(0)[threadIdx.x + c * 256];
+#pragma unroll
+    for (int c = 0; c < CHUNKS; ++c) reinterpret_cast<uint4*>(lds)[threadIdx.x + c * 256] = pre[c];
+    __syncthreads();
+
     for (int s = 0; s < nsteps; ++s) {
-        bf16x8_t ahi, alo;
+        if (s + 1 < nsteps) {
+#pragma unroll
+            for (int c = 0; c < CHUNKS; ++c) pre[c] = gsrc(s + 1)[threadIdx.x + c * 256];
+        }
         const int kb = 32 * s + kq;
-        if (kb + 8 <= a.W) {
-            const float4 x0 = *reinterpret_cast<const float4*>(xr + kb);
-            const float4 x1 = *reinterpret_cast<const float4*>(xr + kb + 4);
-            const float4 w0 = *reinterpret_cast<const float4*>(pwr + kb);
-            const float4 w1 = *reinterpret_cast<const float4*>(pwr + kb + 4);
-            const float e[8] = {x0.x * w0.x, x0.y * w0.y, x0.z * w0.z, x0.w * w0.w,
-                                x1.x * w1.x, x1.y * w1.y, x1.z * w1.z, x1.w * w1.w};
+        if (MODE == kModeF32) {
+            float ea[2][8];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) { __bf16 h, l; split_bf16(e[q], h, l); ahi[q] = h; alo[q] = l; }
+            for (int rb = 0; rb < 2; ++rb) load_a8(a, xr[rb], pwr[rb], kb, ea[rb]);
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const float4* bp = reinterpret_cast<const float4*>(lds + (c * 64 + lane) * LB);
+                const float4 b0 = bp[0], b1 = bp[1];
+                const float eb[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+#pragma unroll
+                for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
+                        acc[rb][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(ea[rb][q], eb[q], acc[rb][c], 0, 0, 0);
+            }
         } else {
+            bf16x8_t ap[2][P];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const int m = kb + q;
-                float e = 0.0f;
-                if (m < a.W) e = xr[m] * pwr[m];
-                else if (m < a.W + 3) e = xr[m - a.W];
-                __bf16 h, l;
-                split_bf16(e, h, l);
-                ahi[q] = h; alo[q] = l;
+            for (int rb = 0; rb < 2; ++rb) {
+                float e[8];
+                load_a8(a, xr[rb], pwr[rb], kb, e);
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    float r = e[q];
+#pragma unroll
+                    for (int p = 0; p < P; ++p) {
+                        const float h = bf16_round(r);
+                        ap[rb][p][q] = to_bf16(h);
+                        r = r - h;
+                    }
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const bf16x8_t* bp = reinterpret_cast<const bf16x8_t*>(lds + (c * 64 + lane) * LB);
+                bf16x8_t bq[P];
+#pragma unroll
+                for (int p = 0; p < P; ++p) bq[p] = bp[p];
+#pragma unroll
+                for (int rb = 0; rb < 2; ++rb) {
+                    if (P == 3) {
+                        acc[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[rb][1], bq[1], acc[rb][c], 0, 0, 0);
+                        acc[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[rb][0], bq[2], acc[rb][c], 0, 0, 0);
+                        acc[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[rb][2], bq[0], acc[rb][c], 0, 0, 0);
+                    }
+                    acc[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[rb][1], bq[0], acc[rb][c], 0, 0, 0);
+                    acc[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[rb][0], bq[1], acc[rb][c], 0, 0, 0);
+                    acc[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[rb][0], bq[0], acc[rb][c], 0, 0, 0);
+                }
             }
         }
+        __syncthreads();
+        if (s + 1 < nsteps) {
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const int cb = cb0 + c;
-            if (cb >= ncb) break;
-            const bf16x8_t* bp = Bp + (((size_t)s * ncb + cb) * 64 + lane) * 2;
-            const bf16x8_t bhi = bp[0];
-            const bf16x8_t blo = bp[1];
-            acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, bhi, acc[c], 0, 0, 0);
-            acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, blo, acc[c], 0, 0, 0);
-            acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, bhi, acc[c], 0, 0, 0);
+            for (int c = 0; c < CHUNKS; ++c) reinterpret_cast<uint4*>(lds)[threadIdx.x + c * 256] = pre[c];
         }
+        __syncthreads();
     }
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-        const int col = col0 + 16 * c + (lane & 15);
-        if (col >= a.W) continue;
+    for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int rr = blockIdx.y * 64 + wave * 16 + 4 * (lane >> 4) + q;
-            if (rr < a.rows) a.out[(size_t)rr * a.W + col] = acc[c][q];
+        for (int c = 0; c < 8; ++c) {
+            const int col = (cb0 + c) * 16 + (lane & 15);
+            if (col >= a.W) continue;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int rr = rowTile + rb * 16 + 4 * (lane >> 4) + q;
+                if (rr < a.rows) a.out[(size_t)rr * a.W + col] = acc[rb][c][q];
+            }
         }
-    }
 }
 
 static unsigned short bf16_rne_bits(float f)
@@ -334,8 +425,10 @@ jh_status jh_filter_create(jh_filter_plan** plan, int device, int W, int V, int 
         require(dso > 1.0f && halfSampleRate > 0.0f, "jh_filter_create: bad DSO or sample rate");
         require(nAdj >= 0 && (nAdj == 0 || (adjPts && adjDB)), "jh_filter_create: bad adjust points");
         require(window == JH_WINDOW_REFERENCE || window == JH_WINDOW_SHEPP_LOGAN, "jh_filter_create: unknown window");
-        require(mode == JH_FILTER_FP32 || mode == JH_FILTER_MFMA_BF16X3, "jh_filter_create: unknown mode");
-        require(W % 8 == 0 || mode == JH_FILTER_FP32, "jh_filter_create: the MFMA form needs W % 8 == 0");
+        require(mode == JH_FILTER_FP32 || mode == JH_FILTER_MFMA_BF16X3 || mode == JH_FILTER_MFMA_BF16X6 ||
+                    mode == JH_FILTER_FP32_VALU,
+                "jh_filter_create: unknown mode");
+        require(W % 4 == 0 || mode == JH_FILTER_FP32_VALU, "jh_filter_create: the MFMA forms need W % 4 == 0");
         int ndev = 0;
         JH_HIP(hipGetDeviceCount(&ndev));
         require(device >= 0 && device < ndev, "jh_filter_create: no such device");
@@ -374,28 +467,47 @@ jh_status jh_filter_create(jh_filter_plan** plan, int device, int W, int V, int 
                 for (int n = 0; n < W; ++n) Bh[(size_t)(W + e) * W + n] = (float)(rr[n] / 3.0);
             p->B.alloc(sizeof(float) * Bh.size());
             JH_HIP(hipMemcpy(p->B.get(), Bh.data(), sizeof(float) * Bh.size(), hipMemcpyHostToDevice));
-            if (mode == JH_FILTER_MFMA_BF16X3) {
-                const int ncb = (W + 15) / 16, nst = p->Kp / 32;
-                std::vector<unsigned short> pk((size_t)nst * ncb * 64 * 16);
-                for (int s = 0; s < nst; ++s)
-                    for (int cb = 0; cb < ncb; ++cb)
-                        for (int l = 0; l < 64; ++l) {
-                            unsigned short* dst = &pk[(((size_t)s * ncb + cb) * 64 + l) * 16];
-                            const int col = 16 * cb + (l & 15);
-                            for (int q = 0; q < 8; ++q) {
-                                const int k = 32 * s + 8 * (l >> 4) + q;
-                                // hi/lo from the FP64 value for the best split
-                                double bv = 0.0;
-                                if (col < W && k < W) bv = h[std::abs(col - k)];
-                                else if (col < W && k < W + 3) bv = rr[col] / 3.0;
-                                const unsigned short hb = bf16_rne_bits((float)bv);
-                                const unsigned short lb = bf16_rne_bits((float)(bv - (double)bf16_to_float(hb)));
-                                dst[q] = hb;
-                                dst[8 + q] = lb;
+            if (mode != JH_FILTER_FP32_VALU) {
+                // operand packed per (k step s, column block cb, lane l): the 8
+                // values B[32 s + 8 (l / 16) + q][16 cb + l % 16], q = 0..7, as
+                // FP32, or as P BF16 parts (hi, [mid,] lo) split from the FP64 value
+                const int ncb8 = (W + 127) / 128 * 8, nst = p->Kp / 32;
+                const int P = mode == JH_FILTER_MFMA_BF16X6 ? 3 : (mode == JH_FILTER_MFMA_BF16X3 ? 2 : 0);
+                auto bval = [&](int k, int col) -> double {
+                    if (col >= W) return 0.0;
+                    if (k < W) return h[std::abs(col - k)];
+                    if (k < W + 3) return rr[col] / 3.0;
+                    return 0.0;
+                };
+                const size_t lanes = (size_t)nst * ncb8 * 64;
+                if (P == 0) {
+                    std::vector<float> pk(lanes * 8);
+                    for (int s = 0; s < nst; ++s)
+                        for (int cb = 0; cb < ncb8; ++cb)
+                            for (int l = 0; l < 64; ++l)
+                                for (int q = 0; q < 8; ++q)
+                                    pk[(((size_t)s * ncb8 + cb) * 64 + l) * 8 + q] =
+                                        (float)bval(32 * s + 8 * (l >> 4) + q, 16 * cb + (l & 15));
+                    p->Bhl.alloc(sizeof(float) * pk.size());
+                    JH_HIP(hipMemcpy(p->Bhl.get(), pk.data(), sizeof(float) * pk.size(), hipMemcpyHostToDevice));
+                } else {
+                    std::vector<unsigned short> pk(lanes * 8 * P);
+                    for (int s = 0; s < nst; ++s)
+                        for (int cb = 0; cb < ncb8; ++cb)
+                            for (int l = 0; l < 64; ++l) {
+                                unsigned short* dst = &pk[(((size_t)s * ncb8 + cb) * 64 + l) * 8 * P];
+                                for (int q = 0; q < 8; ++q) {
+                                    double r = bval(32 * s + 8 * (l >> 4) + q, 16 * cb + (l & 15));
+                                    for (int part = 0; part < P; ++part) {
+                                        const unsigned short b = bf16_rne_bits((float)r);
+                                        dst[part * 8 + q] = b;
+                                        r -= (double)bf16_to_float(b);
+                                    }
+                                }
                             }
-                        }
-                p->Bhl.alloc(sizeof(unsigned short) * pk.size());
-                JH_HIP(hipMemcpy(p->Bhl.get(), pk.data(), sizeof(unsigned short) * pk.size(), hipMemcpyHostToDevice));
+                    p->Bhl.alloc(sizeof(unsigned short) * pk.size());
+                    JH_HIP(hipMemcpy(p->Bhl.get(), pk.data(), sizeof(unsigned short) * pk.size(), hipMemcpyHostToDevice));
+                }
             }
             // pre-weight refresh schedule (FilterAgent.cpp:101-126)
             const int per = 9;
@@ -472,13 +584,18 @@ jh_status jh_filter_run(jh_filter_plan* p, const float* d_in, float* d_out, int 
         a.Kp = p->Kp;
         a.view0 = view0;
         require((size_t)p->V * nviews < (1u << 31), "jh_filter_run: too many rows");
-        dim3 grid((p->W + 63) / 64, (a.rows + 63) / 64);
-        require(grid.y <= 65535u * 16u, "jh_filter_run: too many rows for one launch");
-        if (p->mode == JH_FILTER_FP32) {
-            filter_fp32_kernel<<<grid, 256, 0, as_stream(stream)>>>(a);
+        hipStream_t st = as_stream(stream);
+        if (p->mode == JH_FILTER_FP32_VALU) {
+            dim3 grid((p->W + 63) / 64, (a.rows + 63) / 64);
+            require(grid.y <= 65535u, "jh_filter_run: too many rows for one launch");
+            filter_fp32_kernel<<<grid, 256, 0, st>>>(a);
             check_launch("filter_fp32_kernel");
         } else {
-            filter_mfma_kernel<<<grid, 256, 0, as_stream(stream)>>>(a);
+            dim3 grid((p->W + 127) / 128, (a.rows + 127) / 128);
+            require(grid.y <= 65535u, "jh_filter_run: too many rows for one launch");
+            if (p->mode == JH_FILTER_FP32) filter_mfma_kernel<kModeF32><<<grid, 256, 0, st>>>(a);
+            else if (p->mode == JH_FILTER_MFMA_BF16X3) filter_mfma_kernel<kModeX3><<<grid, 256, 0, st>>>(a);
+            else filter_mfma_kernel<kModeX6><<<grid, 256, 0, st>>>(a);
             check_launch("filter_mfma_kernel");
         }
     });

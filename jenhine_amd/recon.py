@@ -88,7 +88,8 @@ class FilterCore:
         self.W, self.V = W, V
         ptm = np.ascontiguousarray(ptm, np.float64).reshape(-1, 12)
         self.n_views = ptm.shape[0]
-        m = {"fp32": JH_FILTER_FP32, "mfma": JH_FILTER_MFMA_BF16X3}[mode]
+        m = {"fp32": _lib.JH_FILTER_FP32, "mfma": _lib.JH_FILTER_MFMA_BF16X3, "mfma6": _lib.JH_FILTER_MFMA_BF16X6,
+             "fp32valu": _lib.JH_FILTER_FP32_VALU}[mode]
         n = len(adj_pts)
         pts = (ctypes.c_float * max(n, 1))(*adj_pts)
         db = (ctypes.c_float * max(n, 1))(*adj_db)

@@ -3,9 +3,10 @@
 Tolerances:
   * preprocessing: integer steps bit-exact; float result within 2 ulp of the
     FP32-ref oracle (the only non-exact step is ln: OCML logf vs glibc logf);
-  * filter FP32: rel-RMSE <= 2e-6 vs the FP64 oracle; MFMA hi/lo BF16:
-    rel-RMSE <= 3e-4 (each operand keeps ~17 significant bits, and the ramp
-    filter's cancellation, sum|terms| / |result| ~ 20, amplifies that);
+  * filter FP32 (MFMA and vector ALU) and hi/mid/lo BF16 (6 products):
+    rel-RMSE <= 2e-6 vs the FP64 oracle; hi/lo BF16 (3 products): <= 3e-4
+    (each operand keeps ~17 significant bits, and the ramp filter's
+    cancellation, sum|terms| / |result| ~ 20, amplifies that);
   * backprojection FP32: rel-RMSE <= 1e-4 vs the FP32-ref oracle volume (the
     north-star bar) -- observed ~1e-6; BF16 / FP16 storage: <= 1e-3 vs FP64.
 """
@@ -119,11 +120,29 @@ def test_filter_fp32_vs_fp64_oracle(gpu, small_data):
     assert e <= 2e-6
 
 
-def test_filter_mfma_bf16x3_vs_fp64_oracle(gpu, small_data):
-    flt, out, ref, H, pws = _filter_case(gpu, small_data, "mfma")
+@pytest.mark.parametrize("mode,tol", [("mfma", 3e-4), ("mfma6", 2e-6), ("fp32valu", 2e-6)])
+def test_filter_other_forms_vs_fp64_oracle(gpu, small_data, mode, tol):
+    flt, out, ref, H, pws = _filter_case(gpu, small_data, mode)
     e = rel_rmse(out, ref)
-    print("filter mfma rel-RMSE", e, "max-abs", max_abs(out, ref))
-    assert e <= 3e-4
+    print("filter", mode, "rel-RMSE", e, "max-abs", max_abs(out, ref))
+    assert e <= tol
+
+
+def test_filter_mfma_partial_column_tile(gpu):
+    # W = 200: the last 128-column tile is partial and K = 203 -> 224
+    from jenhine_amd.recon import FilterCore
+    w = SMALL.scaled(n_views=3, n_u=200, n_v=40)
+    scan = w.scan_params()
+    frames = np.ascontiguousarray(synth.uniform(8, (3, 40, 200), 0.0, 3.0))
+    ptm = w.ptms()
+    H = o.filter_design(200, scan.HalfSampleRate, window=1, fp64=True)
+    src, blk = o.preweight_schedule(3)
+    pws = np.stack([o.prefilter_weight(ptm[s], 200, 40, scan.DSO, fp64=True) for s in src])
+    ref = o.filter_rows(frames.astype(np.float64), H, pws, blk, fp64=True)
+    for mode, tol in (("fp32", 2e-6), ("mfma6", 2e-6), ("mfma", 3e-4)):
+        flt = FilterCore(200, 40, ptm, scan.DSO, scan.HalfSampleRate, window=1, mode=mode, device=gpu)
+        out = flt.process(torch.from_numpy(frames).to(gpu)).cpu().numpy()
+        assert rel_rmse(out, ref) <= tol, mode
 
 
 # --------------------------------------------------------------------------
@@ -252,7 +271,7 @@ def test_bp_reduced_storage_vs_fp64(gpu, small_data, dtype, tol):
 # --------------------------------------------------------------------------
 
 @pytest.mark.parametrize("filter_mode,storage,tol64", [("fp32", "f32", 1e-4), ("mfma", "bf16", 1e-3),
-                                                        ("mfma", "f16", 1e-3)])
+                                                        ("mfma6", "bf16", 1e-3), ("mfma6", "f16", 1e-3)])
 def test_fdk_cbct128_vs_oracle(gpu, filter_mode, storage, tol64):
     from jenhine_amd.recon import FDK
     w = configs.CBCT128
