@@ -204,6 +204,21 @@ jh_status jh_fp_project(jh_bp_plan* plan, const float* d_vol, float* d_proj, int
  * at BPCUDACore.cu:99-112); used by the autotuner. */
 jh_status jh_bp_set_tile(jh_bp_plan* plan, int zc, int ty);
 
+/* Launch schedule of the backprojector: persistent = 1 sizes the grid to the
+ * workgroups the device holds at once, which then sweep the tiles of their
+ * XCD in rounds (default); 0 launches one workgroup per tile.  wgPerCU > 0
+ * caps the resident workgroups per CU counted for the persistent grid. */
+jh_status jh_bp_set_schedule(jh_bp_plan* plan, int persistent, int wgPerCU);
+
+/* Autotuner: times the normalised backprojection of views 0..nviews-1 of
+ * d_q (reps launches each, after one warm-up) for every tile shape
+ * zc in {8, 16, 32} x ty in {1, 2, 4} and schedule persistent in {0, 1}, and
+ * keeps the fastest in the plan.  h_ms (18 floats, may be NULL) receives the
+ * milliseconds per launch in the order persistent, zc, ty (outer to inner).
+ * Writes d_vol; synchronises the stream.  No counterpart in the reference. */
+jh_status jh_bp_autotune(jh_bp_plan* plan, const void* d_q, int dtype, int nviews, float* d_vol, int reps,
+                         jh_stream_t stream, float* h_ms);
+
 /* Current tile shape. */
 jh_status jh_bp_get_tile(jh_bp_plan* plan, int* zc, int* ty);
 

@@ -227,14 +227,12 @@ constexpr int kOutNormalised = 0;  // vol = acc / count (one-shot)
 constexpr int kOutAccumulate = 1;  // acc += ..., count += ... (streaming)
 constexpr int kOutRaw = 2;         // vol = acc (the un-normalised operator, FP32 storage only)
 
+// One tile: 64 x blockDim.y voxel columns, ZC voxels deep, all views of the
+// launch.
 template <int MODE, int ZC, int QT, int OUT>
-__global__ void __launch_bounds__(256) bp_kernel(BPArgs a)
+__device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
 {
     typedef Sampler<QT> S;
-    const unsigned ntiles = (unsigned)a.ntx * a.nty * a.ntz;
-    const unsigned per = gridDim.x >> 3;  // gridDim.x is a multiple of 8
-    const unsigned tile = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
-    if (tile >= ntiles) return;
     const int tx = tile % a.ntx;
     const unsigned rest = tile / a.ntx;
     const int tyi = rest % a.nty;
@@ -373,6 +371,27 @@ __global__ void __launch_bounds__(256) bp_kernel(BPArgs a)
     }
 }
 
+// Tile schedule.  Workgroups are dispatched round-robin over the 8 XCDs, so
+// workgroup b runs on XCD b % 8.  XCD x owns the contiguous tile range
+// [x T8, (x+1) T8) -- a z-slab, whose detector rows then stay in that XCD's
+// L2 -- and its gridDim.x / 8 workgroups sweep the range in rounds.  With a
+// grid of one workgroup per tile every workgroup has one tile; with a grid
+// sized to what the device holds at once ("persistent"), all resident
+// workgroups of an XCD work on neighbouring tiles and advance through the
+// views together, which keeps the per-view working set small enough for L2.
+template <int MODE, int ZC, int QT, int OUT>
+__global__ void __launch_bounds__(256) bp_kernel(BPArgs a)
+{
+    const unsigned ntiles = (unsigned)a.ntx * a.nty * a.ntz;
+    const unsigned g8 = gridDim.x >> 3;  // gridDim.x is a multiple of 8
+    const unsigned t8 = (ntiles + 7) >> 3;
+    const unsigned xcd = blockIdx.x & 7u;
+    for (unsigned t = blockIdx.x >> 3; t < t8; t += g8) {
+        const unsigned tile = xcd * t8 + t;
+        if (tile < ntiles) bp_tile<MODE, ZC, QT, OUT>(a, tile);
+    }
+}
+
 __global__ void bp_finalize_kernel(const float* __restrict__ acc, const float* __restrict__ cnt,
                                    float* __restrict__ vol, size_t n)
 {
@@ -444,11 +463,20 @@ static void launch_bp(jh_bp_plan* p, const void* q, int dtype, int view0, int nv
     a.frame_bytes = (uint32_t)fb;
     const size_t ntiles = (size_t)a.ntx * a.nty * a.ntz;
     require(ntiles < (1ull << 31), "backprojection grid too large");
-    const unsigned grid = (unsigned)((ntiles + 7) / 8 * 8);
+    size_t grid = (ntiles + 7) / 8 * 8;
     bp_kernel_t k = out == kOutNormalised   ? pick_kernel<kOutNormalised>(p->mode, p->zc, dtype)
                     : out == kOutAccumulate ? pick_kernel<kOutAccumulate>(p->mode, p->zc, dtype)
                                             : pick_kernel<kOutRaw>(p->mode, p->zc, dtype);
-    hipLaunchKernelGGL(k, dim3(grid), dim3(64, p->ty), 0, stream, a);
+    if (p->persistent) {
+        // as many workgroups as the device holds at once (a multiple of 8)
+        int cus = 0, perCU = 0;
+        JH_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device));
+        JH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, reinterpret_cast<const void*>(k), 64 * p->ty, 0));
+        if (p->wgPerCU > 0) perCU = std::min(perCU, p->wgPerCU);
+        const size_t resident = (size_t)std::max(1, cus) * (size_t)std::max(1, perCU) / 8 * 8;
+        if (resident >= 8) grid = std::min(grid, resident);
+    }
+    hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(64, p->ty), 0, stream, a);
     check_launch("bp_kernel");
 }
 
@@ -480,6 +508,8 @@ jh_status jh_bp_create(jh_bp_plan** plan, int device, const double* h_ptm, int n
             p->pxy = pxy; p->pz = pz;
             p->zc = 32;
             p->ty = 4;
+            p->persistent = 1;
+            p->wgPerCU = 0;
             p->ptm.alloc(sizeof(double) * 12 * nviews);
             p->geo.alloc(sizeof(Geo) * nviews);
             p->uw.alloc(sizeof(float) * (size_t)nviews * W);
@@ -641,6 +671,65 @@ jh_status jh_bp_set_tile(jh_bp_plan* p, int zc, int ty)
         require(ty == 0 || ty == 1 || ty == 2 || ty == 4, "jh_bp_set_tile: ty must be 1, 2 or 4");
         if (zc) p->zc = zc;
         if (ty) p->ty = ty;
+    });
+}
+
+jh_status jh_bp_set_schedule(jh_bp_plan* p, int persistent, int wgPerCU)
+{
+    return guard([&] {
+        require(p != nullptr, "jh_bp_set_schedule: plan is NULL");
+        require(persistent == 0 || persistent == 1, "jh_bp_set_schedule: persistent must be 0 or 1");
+        require(wgPerCU >= 0 && wgPerCU <= 64, "jh_bp_set_schedule: bad workgroups per CU");
+        p->persistent = persistent;
+        p->wgPerCU = wgPerCU;
+    });
+}
+
+jh_status jh_bp_autotune(jh_bp_plan* p, const void* d_q, int dtype, int nviews, float* d_vol, int reps,
+                         jh_stream_t stream, float* h_ms)
+{
+    return guard([&] {
+        require(p != nullptr && d_q != nullptr && d_vol != nullptr, "jh_bp_autotune: bad arguments");
+        require(dtype == JH_F32 || dtype == JH_BF16 || dtype == JH_F16, "jh_bp_autotune: unknown dtype");
+        require(nviews >= 1 && nviews <= p->N && reps >= 1, "jh_bp_autotune: bad view count or repetitions");
+        DeviceScope ds(p->device);
+        hipStream_t s = as_stream(stream);
+        hipEvent_t e0, e1;
+        JH_HIP(hipEventCreate(&e0));
+        JH_HIP(hipEventCreate(&e1));
+        const int zcs[3] = {8, 16, 32}, tys[3] = {1, 2, 4};
+        float best = 1e30f;
+        int bz = p->zc, bt = p->ty, bp = p->persistent;
+        int n = 0;
+        try {
+            for (int pers = 0; pers < 2; ++pers)
+                for (int zi = 0; zi < 3; ++zi)
+                    for (int ti = 0; ti < 3; ++ti) {
+                        p->zc = zcs[zi];
+                        p->ty = tys[ti];
+                        p->persistent = pers;
+                        launch_bp(p, d_q, dtype, 0, nviews, d_vol, kOutNormalised, s);  // warm-up
+                        JH_HIP(hipEventRecord(e0, s));
+                        for (int r = 0; r < reps; ++r) launch_bp(p, d_q, dtype, 0, nviews, d_vol, kOutNormalised, s);
+                        JH_HIP(hipEventRecord(e1, s));
+                        JH_HIP(hipEventSynchronize(e1));
+                        float ms = 0.0f;
+                        JH_HIP(hipEventElapsedTime(&ms, e0, e1));
+                        ms /= (float)reps;
+                        if (h_ms) h_ms[n] = ms;
+                        ++n;
+                        if (ms < best) { best = ms; bz = p->zc; bt = p->ty; bp = pers; }
+                    }
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        p->zc = bz;
+        p->ty = bt;
+        p->persistent = bp;
     });
 }
 

@@ -151,6 +151,19 @@ class BPCore:
     def set_tile(self, zc=0, ty=0):
         call("jh_bp_set_tile", self._h, zc, ty)
 
+    def set_schedule(self, persistent=1, wg_per_cu=0):
+        call("jh_bp_set_schedule", self._h, int(persistent), int(wg_per_cu))
+
+    def autotune(self, q, dtype="bf16", n_views=None, reps=1):
+        """Times every tile shape and schedule on views 0..n_views-1 of q and
+        keeps the fastest; returns {(persistent, zc, ty): ms}."""
+        n = self.n_views if n_views is None else n_views
+        vol = torch.empty((self.Z, self.Y, self.X), dtype=torch.float32, device=q.device)
+        ms = (ctypes.c_float * 18)()
+        call("jh_bp_autotune", self._h, _ptr(q), DTYPES[dtype], n, _ptr(vol), reps, _stream(), ms)
+        keys = [(p, z, t) for p in (0, 1) for z in (8, 16, 32) for t in (1, 2, 4)]
+        return dict(zip(keys, list(ms)))
+
     def tile(self):
         zc, ty = ctypes.c_int(), ctypes.c_int()
         call("jh_bp_get_tile", self._h, ctypes.byref(zc), ctypes.byref(ty))

@@ -153,7 +153,9 @@ def _bp_gpu(gpu, ptm, filt, w, dtype="f32", chunks=None, tile=None):
     from jenhine_amd.recon import BPCore
     bp = BPCore(ptm, filt.shape[2], filt.shape[1], w[0], w[1], w[2], w[3], w[4], device=gpu)
     if tile:
-        bp.set_tile(*tile)
+        bp.set_tile(*tile[:2])
+        if len(tile) > 2:
+            bp.set_schedule(tile[2])
     f = torch.from_numpy(np.ascontiguousarray(filt, np.float32)).to(gpu)
     if chunks is None:
         q = bp.stage(f, dtype)
@@ -175,7 +177,7 @@ def test_bp_uweights_match_oracle(gpu, small_data):
     assert bp.geometry_mode() == 0
 
 
-@pytest.mark.parametrize("tile", [(32, 4), (16, 2), (8, 1)])
+@pytest.mark.parametrize("tile", [(32, 4, 1), (16, 2, 0), (8, 1, 1), (32, 1, 0)])
 def test_bp_fp32_vs_fp32_ref(gpu, tile):
     # random filtered projections on the cbct128 geometry at 24 views
     w = SMALL
@@ -288,3 +290,18 @@ def test_fdk_cbct128_vs_oracle(gpu, filter_mode, storage, tol64):
         e32 = rel_rmse(vol, v32)
         print("rel-RMSE vs FP32-ref", e32)
         assert e32 <= 1e-4
+
+
+def test_bp_autotune_keeps_results(gpu):
+    w = SMALL
+    ptm = w.ptms()
+    filt = torch.from_numpy(synth.normal(14, (w.n_views, w.n_v, w.n_u))).to(gpu)
+    from jenhine_amd.recon import BPCore
+    bp = BPCore(ptm, w.n_u, w.n_v, w.X, w.Y, w.Z, w.pxy, w.pz, device=gpu)
+    q = bp.stage(filt, "bf16")
+    before = bp.run(q, "bf16").cpu().numpy()
+    table = bp.autotune(q, "bf16", reps=1)
+    assert len(table) == 18 and all(v > 0 for v in table.values())
+    after = bp.run(q, "bf16").cpu().numpy()
+    print("autotuned tile", bp.tile(), "rel-RMSE vs default", rel_rmse(after, before))
+    assert rel_rmse(after, before) <= 1e-6
