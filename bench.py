@@ -55,7 +55,9 @@ def main():
     ap.add_argument("--cpu-slices", type=int, default=16)
     ap.add_argument("--acc-slices", type=int, default=4)
     ap.add_argument("--kernel-only", action="store_true",
-                    help="profiling run: a 1 GiB copy, then only the BF16 backprojection kernel")
+                    help="profiling run: a 1 GiB copy, then only the backprojection kernel")
+    ap.add_argument("--kernel-form", choices=("fp32", "bf16", "f16"), default="bf16",
+                    help="storage form of the --kernel-only backprojection")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -84,21 +86,22 @@ def main():
              "f16": FDK(scan, rec, d["ptm"], "mfma6", "f16", dev)}
     if args.kernel_only:
         # profiling run: a plain copy of known size (scales the HBM counters),
-        # then the BF16 backprojection kernel `steps` times
+        # then the backprojection kernel of one form `steps` times
         src = torch.empty(1 << 28, dtype=torch.float32, device=dev).fill_(1.0)
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
-        fdk = forms["bf16"]
-        q = fdk.bp.stage(fdk.filter.process(fdk.pre.process(raw, dark, gain)), "bf16")
-        vol = fdk.bp.run(q, "bf16")
+        fdk = forms[args.kernel_form]
+        q = fdk.bp.stage(fdk.filter.process(fdk.pre.process(raw, dark, gain)), fdk.storage)
+        vol = fdk.bp.run(q, fdk.storage)
         torch.cuda.synchronize()
         t = time.perf_counter()
         for _ in range(args.steps):
-            fdk.bp.run(q, "bf16", out=vol)
+            fdk.bp.run(q, fdk.storage, out=vol)
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t) / args.steps * 1e3
-        log(f"bp_kernel bf16: {ms:.2f} ms/launch, copy kernel moved {2 * src.numel() * 4} bytes")
+        log(f"bp_kernel {fdk.storage}: {ms:.2f} ms/launch (tile {fdk.bp.tile()}), "
+            f"copy kernel moved {2 * src.numel() * 4} bytes")
         print(json.dumps({"kernel_only": True, "bp_ms": ms, "copy_bytes": 2 * src.numel() * 4}))
         return
     units = w.gups_units
@@ -110,9 +113,10 @@ def main():
         # plan set-up: pick the backprojector's tile shape and schedule on this
         # geometry (not part of the timed step, like any plan creation)
         q0 = fdk.bp.stage(fdk.filter.process(fdk.pre.process(raw, dark, gain)), fdk.storage)
-        table = fdk.bp.autotune(q0, fdk.storage, n_views=min(w.n_views, 120), reps=1)
+        # per-view cost depends on the view angle: tune on the whole orbit
+        table = fdk.bp.autotune(q0, fdk.storage, reps=1)
         tuned[name] = dict(zip(("zc", "ty"), fdk.bp.tile()))
-        tuned[name]["table_ms_120_views"] = {f"p{k[0]}_zc{k[1]}_ty{k[2]}": round(v, 3) for k, v in table.items()}
+        tuned[name]["table_ms"] = {f"zc{k[0]}_ty{k[1]}": round(v, 3) for k, v in table.items()}
         log(name, "autotune", json.dumps(tuned[name]))
         del q0
         torch.cuda.synchronize()

@@ -56,6 +56,7 @@ typedef void* jh_stream_t; /* a hipStream_t (NULL = the default stream) */
 typedef struct jh_pre_plan jh_pre_plan;
 typedef struct jh_filter_plan jh_filter_plan;
 typedef struct jh_bp_plan jh_bp_plan;
+typedef struct jh_graph jh_graph;
 
 /* Last error message of the calling thread ("" when none).
  * Replaces Common/Exception.h:12-41 (ThrowExceptionAndLog) across the ABI. */
@@ -204,17 +205,12 @@ jh_status jh_fp_project(jh_bp_plan* plan, const float* d_vol, float* d_proj, int
  * at BPCUDACore.cu:99-112); used by the autotuner. */
 jh_status jh_bp_set_tile(jh_bp_plan* plan, int zc, int ty);
 
-/* Launch schedule of the backprojector: persistent = 1 sizes the grid to the
- * workgroups the device holds at once, which then sweep the tiles of their
- * XCD in rounds (default); 0 launches one workgroup per tile.  wgPerCU > 0
- * caps the resident workgroups per CU counted for the persistent grid. */
-jh_status jh_bp_set_schedule(jh_bp_plan* plan, int persistent, int wgPerCU);
-
 /* Autotuner: times the normalised backprojection of views 0..nviews-1 of
  * d_q (reps launches each, after one warm-up) for every tile shape
- * zc in {8, 16, 32} x ty in {1, 2, 4} and schedule persistent in {0, 1}, and
- * keeps the fastest in the plan.  h_ms (18 floats, may be NULL) receives the
- * milliseconds per launch in the order persistent, zc, ty (outer to inner).
+ * zc in {8, 16, 32} x ty in {1, 2, 4}, and keeps the fastest in the plan.
+ * h_ms (9 floats, may be NULL) receives the milliseconds per launch in the
+ * order zc, ty (outer to inner).  The per-view cost depends on the view
+ * angle, so nviews should cover the whole orbit.
  * Writes d_vol; synchronises the stream.  No counterpart in the reference. */
 jh_status jh_bp_autotune(jh_bp_plan* plan, const void* d_q, int dtype, int nviews, float* d_vol, int reps,
                          jh_stream_t stream, float* h_ms);
@@ -227,6 +223,55 @@ jh_status jh_bp_get_tile(jh_bp_plan* plan, int* zc, int* ty);
 jh_status jh_bp_geometry_mode(jh_bp_plan* plan, int* mode);
 
 jh_status jh_bp_destroy(jh_bp_plan* plan);
+
+/* ---------------------------------------------------------------------- */
+/* Batches: many scans captured into one hipGraph                          */
+/* ---------------------------------------------------------------------- */
+
+/* A batch of independent scans reconstructed by one graph launch.  The
+ * reference reconstructs one scan per run of its agent graph
+ * (JEngine/JEngine.cpp:185-313, recon-part loop :400-425); the batch chains,
+ * per scan, the entry points that replace its agents -- jh_pre_run
+ * (PreProcessingAgent), jh_filter_run (FilterAgent), jh_bp_stage +
+ * jh_bp_accumulate per chunk of views (BPCUDAAgent::ProcessView loop,
+ * BPCUDAAgent.cpp:231-287) and jh_bp_finalize (DeployUpdateOutput). */
+jh_status jh_graph_create(jh_graph** graph, int device);
+
+/* Adds one scan.  The plans must describe the same detector and view count
+ * (pre output = filter frame = backprojector detector) and outlive the
+ * graph; each backprojection plan and each d_vol may appear once.  d_raw:
+ * nviews x inH x inW u16 (read at every launch), d_dark/d_gain as in
+ * jh_pre_run (may be NULL), d_vol: Z x Y x X f32 output.  dtype: storage
+ * of the staged projections.  viewsPerChunk: views per chunk, 0 = chosen by
+ * jh_graph_autotune (128 if it is not run).  *index (may be NULL) receives
+ * the scan's index. */
+jh_status jh_graph_add_scan(jh_graph* graph, jh_pre_plan* pre, jh_filter_plan* filter, jh_bp_plan* bp,
+                            const uint16_t* d_raw, const uint16_t* d_dark, const float* d_gain, int dtype,
+                            int viewsPerChunk, float* d_vol, int* index);
+
+/* Autotuner for a batch: per scan, the backprojector tile (jh_bp_autotune
+ * over the whole orbit on a zero-filled staging buffer; the timing depends
+ * on the geometry and the tile, not on the values) and, for scans added with
+ * viewsPerChunk = 0, the smallest chunk of {16, 32, ..., 256, min(N, 512)}
+ * whose accumulation time per view is within 3 % of the best.  Writes the
+ * scans' d_vol; synchronises the stream. */
+jh_status jh_graph_autotune(jh_graph* graph, int reps, jh_stream_t stream);
+
+/* The chunk and tile a scan will run with. */
+jh_status jh_graph_scan_tuning(jh_graph* graph, int index, int* viewsPerChunk, int* zc, int* ty);
+
+/* Assigns the scans to nbranches (1..16) parallel branches (longest first,
+ * to the least-loaded branch), allocates one set of chunk buffers per
+ * branch and captures the whole batch into one hipGraph.  No scans can be
+ * added afterwards. */
+jh_status jh_graph_instantiate(jh_graph* graph, int nbranches);
+
+/* Replays the batch on `stream`: every scan's volume is recomputed from its
+ * input buffers as they are at launch time.  Launches must not overlap
+ * (they share the chunk buffers and the plans' accumulators). */
+jh_status jh_graph_launch(jh_graph* graph, jh_stream_t stream);
+
+jh_status jh_graph_destroy(jh_graph* graph);
 
 #ifdef __cplusplus
 }

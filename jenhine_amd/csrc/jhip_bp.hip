@@ -14,8 +14,8 @@
 //    a launch, so the volume is written once per launch (one-shot) or
 //    read-modify-written once per view chunk (streaming);
 //  * projections are stored in a "pair" layout, pixel (u, v) holding
-//    (q[v][u], q[v][u+1]) in FP32, BF16 or FP16, so one bilinear sample is two
-//    aligned gathers that hit L1/L2;
+//    (q[v][u], q[v][u+1] - q[v][u]) in FP32 and (q[v][u], q[v][u+1]) in BF16
+//    or FP16, so one bilinear sample is two aligned gathers that hit L1/L2;
 //  * when u and w do not depend on z (circular orbits) the u-side of the
 //    bilinear weights and the count weight are hoisted out of the z loop;
 //  * 64 x TY voxel columns per workgroup, workgroups remapped so that each of
@@ -80,13 +80,23 @@ __global__ void bp_uweight_kernel(const double* __restrict__ ptm, int N, int W, 
 }
 
 // ---------------------------------------------------------------------------
-// Staging: q = filtered * uW in the pair layout.
+// Staging: q = filtered * uW in the quad layout.
 //
-// Frame layout: V+1 rows of W pixels; pixel (u, v) holds the pair
-// (q[v][u], q[v][u+1]) with q[v][W] = 0, and row V is all zeros.  A bilinear
-// sample at integer (iu, iv) with iu <= W-1, iv <= V-1 then reads pixels
-// (iu, iv) and (iu, iv+1) without clamping: at the far edges the weight of
-// the padding is exactly 0.
+// Frame layout: V rows of W pixels; pixel (u, v) holds the 2 x 2 neighbourhood
+// of a bilinear sample whose integer part is (u, v), with q[v][W] = 0 and
+// q[V][u] = 0 (so a sample at u <= W-1, v <= V-1 needs no clamping: at the
+// far edges the weight of the padding is exactly 0), and one sample is one
+// aligned gather.  The gathers are limited by the cache lines each wave
+// instruction touches, not by bytes, so the 2x-redundant layout halves
+// their cost.
+//  * FP32 (16 B): (q[v][u], q[v][u+1] - q[v][u], q[v+1][u], q[v+1][u+1] -
+//    q[v+1][u]).  Each row interpolation is one FMA, q + fu d, and d is the
+//    difference the interpolation would form anyway (same result to the bit).
+//  * BF16 / FP16 (8 B): (q[v][u], q[v][u+1], q[v+1][u], q[v+1][u+1]), each
+//    rounded once.  Storing differences instead makes the rounding error
+//    scale with |d|, which exceeds |q| on ramp-filtered noise, and a BF16
+//    weight for v_dot2 would move the sample by up to 2^-9 pixel; both
+//    measurably raise the error.
 // ---------------------------------------------------------------------------
 
 __device__ inline uint32_t f32_to_bf16_rne(float f)
@@ -99,6 +109,15 @@ __device__ inline uint32_t f32_to_bf16_rne(float f)
 
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+
+__device__ inline uint32_t pack_f16x2(float a, float b)
+{
+    half2_t h;
+    h.x = (_Float16)a;
+    h.y = (_Float16)b;
+    return __builtin_bit_cast(uint32_t, h);
+}
 
 template <int QT>
 __global__ void __launch_bounds__(256) bp_stage_kernel(const float* __restrict__ filt,
@@ -106,36 +125,35 @@ __global__ void __launch_bounds__(256) bp_stage_kernel(const float* __restrict__
                                                        int nviews, void* __restrict__ out)
 // uw == NULL packs the frames without weighting.
 {
-    const size_t frame_in = (size_t)W * V;
-    const size_t frame_out = (size_t)W * (V + 1);
-    const size_t total = frame_out * nviews;
+    const size_t frame = (size_t)W * V;
+    const size_t total = frame * nviews;
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
          t += (size_t)gridDim.x * blockDim.x) {
-        const size_t view = t / frame_out;
-        const int pix = (int)(t - view * frame_out);
+        const size_t view = t / frame;
+        const int pix = (int)(t - view * frame);
         const int row = pix / W;
         const int u = pix - row * W;
-        float a = 0.0f, b = 0.0f;
-        if (row < V) {
-            const float* f = filt + view * frame_in + (size_t)row * W;
-            if (uw) {
-                const float* w = uw + view * W;
-                a = f[u] * w[u];
-                if (u + 1 < W) b = f[u + 1] * w[u + 1];
-            } else {
-                a = f[u];
-                if (u + 1 < W) b = f[u + 1];
-            }
-        }
+        const float* f = filt + view * frame + (size_t)row * W;
+        const float* w = uw ? uw + view * W : nullptr;
+        const bool right = u + 1 < W, below = row + 1 < V;
+        const float wa = w ? w[u] : 1.0f;
+        const float wb = w && right ? w[u + 1] : 1.0f;
+        const float a0 = f[u] * wa;
+        const float b0 = right ? f[u + 1] * wb : 0.0f;
+        const float a1 = below ? f[W + u] * wa : 0.0f;
+        const float b1 = below && right ? f[W + u + 1] * wb : 0.0f;
         if (QT == JH_F32) {
-            reinterpret_cast<float2*>(out)[t] = make_float2(a, b);
+            reinterpret_cast<float4*>(out)[t] = make_float4(a0, b0 - a0, a1, b1 - a1);
         } else if (QT == JH_BF16) {
-            reinterpret_cast<uint32_t*>(out)[t] = f32_to_bf16_rne(a) | (f32_to_bf16_rne(b) << 16);
+            u32x2_t o;
+            o.x = f32_to_bf16_rne(a0) | (f32_to_bf16_rne(b0) << 16);
+            o.y = f32_to_bf16_rne(a1) | (f32_to_bf16_rne(b1) << 16);
+            reinterpret_cast<u32x2_t*>(out)[t] = o;
         } else {
-            half2_t h;
-            h.x = (_Float16)a;
-            h.y = (_Float16)b;
-            reinterpret_cast<half2_t*>(out)[t] = h;
+            u32x2_t o;
+            o.x = pack_f16x2(a0, b0);
+            o.y = pack_f16x2(a1, b1);
+            reinterpret_cast<u32x2_t*>(out)[t] = o;
         }
     }
 }
@@ -157,58 +175,62 @@ __global__ void bp_uwpair_kernel(const float* __restrict__ uw, int N, int W, flo
 
 constexpr int kBufferFlags = 0x00020000;  // raw buffer, 32-bit dwords (gfx9 family)
 
-// Bilinear value from the two pair loads of one sample.
+// One quad load per sample; row r (0 = top, 1 = bottom) interpolated at
+// fractional offset fu.
 template <int QT>
 struct Sampler;
 
 template <>
 struct Sampler<JH_F32> {
-    typedef u32x2_t raw_t;
-    static constexpr int bytes = 8;
-    __device__ static raw_t load(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff)
+    typedef u32x4_t raw_t;
+    static constexpr int bytes = 16;
+    __device__ static raw_t load(__amdgpu_buffer_rsrc_t r, uint32_t off)
     {
-        return __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0);
+        return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
     }
-    __device__ static float lerp(raw_t p, float fu)
+    // (q, d): q + fu d
+    __device__ static float lerp(raw_t p, int r, float fu)
     {
-        const float a = __uint_as_float(p.x), b = __uint_as_float(p.y);
-        return fmaf(fu, b - a, a);
+        return r == 0 ? fmaf(fu, __uint_as_float(p.y), __uint_as_float(p.x))
+                      : fmaf(fu, __uint_as_float(p.w), __uint_as_float(p.z));
     }
 };
 
 template <>
 struct Sampler<JH_BF16> {
-    typedef uint32_t raw_t;
-    static constexpr int bytes = 4;
-    __device__ static raw_t load(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff)
+    typedef u32x2_t raw_t;
+    static constexpr int bytes = 8;
+    __device__ static raw_t load(__amdgpu_buffer_rsrc_t r, uint32_t off)
     {
-        return __builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0);
+        return __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
     }
-    __device__ static float lerp(raw_t p, float fu)
+    __device__ static float lerp(raw_t p, int r, float fu)
     {
-        const float a = __uint_as_float(p << 16), b = __uint_as_float(p & 0xffff0000u);
+        const uint32_t x = r == 0 ? p.x : p.y;
+        const float a = __uint_as_float(x << 16), b = __uint_as_float(x & 0xffff0000u);
         return fmaf(fu, b - a, a);
     }
 };
 
 template <>
 struct Sampler<JH_F16> {
-    typedef uint32_t raw_t;
-    static constexpr int bytes = 4;
-    __device__ static raw_t load(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff)
+    typedef u32x2_t raw_t;
+    static constexpr int bytes = 8;
+    __device__ static raw_t load(__amdgpu_buffer_rsrc_t r, uint32_t off)
     {
-        return __builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0);
+        return __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
     }
-    __device__ static float lerp(raw_t p, float fu)
+    __device__ static float lerp(raw_t p, int r, float fu)
     {
-        // a*(1-fu) + b*fu with the f16 -> f32 conversions folded into v_fma_mix
-        const half2_t h = __builtin_bit_cast(half2_t, p);
-        return fmaf((float)h.y, fu, (float)h.x * (1.0f - fu));
+        // b fu + (a - a fu): two v_fma_mix, which convert the f16 halves inside
+        const half2_t h = __builtin_bit_cast(half2_t, r == 0 ? p.x : p.y);
+        const float a = (float)h.x;
+        return fmaf((float)h.y, fu, fmaf(-a, fu, a));
     }
 };
 
 struct BPArgs {
-    const void* q;       // views view0..view0+nviews-1, pair layout, V+1 rows
+    const void* q;       // views view0..view0+nviews-1, quad layout
     const Geo* geo;      // all views
     const float2* uwp;   // all views, N x W count-weight pairs
     float* vol;          // one-shot output
@@ -227,8 +249,8 @@ constexpr int kOutNormalised = 0;  // vol = acc / count (one-shot)
 constexpr int kOutAccumulate = 1;  // acc += ..., count += ... (streaming)
 constexpr int kOutRaw = 2;         // vol = acc (the un-normalised operator, FP32 storage only)
 
-// One tile: 64 x blockDim.y voxel columns, ZC voxels deep, all views of the
-// launch.
+// One tile: 64 blockDim.y voxel columns (8 x 8 per wave), ZC voxels deep,
+// all views of the launch.
 template <int MODE, int ZC, int QT, int OUT>
 __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
 {
@@ -237,8 +259,15 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
     const unsigned rest = tile / a.ntx;
     const int tyi = rest % a.nty;
     const int tz = rest / a.nty;
-    const int i = tx * 64 + threadIdx.x;
-    const int j = tyi * blockDim.y + threadIdx.y;
+    // Each wave covers an 8 x 8 patch of voxel columns; the blockDim.y waves
+    // of a workgroup tile 8 x 8 (1 wave), 16 x 8 (2) or 16 x 16 (4).  A wave's
+    // 64 samples of one view then fall on a compact detector footprint: the
+    // gathers are limited by the number of cache lines each wave touches, which
+    // a 64 x 1 row of voxels spreads over ~4x more lines.
+    const int px = blockDim.y >= 2 ? 2 : 1;
+    const int wx = threadIdx.y % px, wy = threadIdx.y / px;
+    const int i = (tx * px + wx) * 8 + (threadIdx.x & 7);
+    const int j = (tyi * (blockDim.y / px) + wy) * 8 + (threadIdx.x >> 3);
     const int k0 = tz * ZC;
     const float fi = (float)min(i, a.X - 1);
     const float fj = (float)min(j, a.Y - 1);
@@ -249,6 +278,7 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
     float acc[ZC], cnt[ZC];
 #pragma unroll
     for (int kk = 0; kk < ZC; ++kk) { acc[kk] = 0.0f; cnt[kk] = 0.0f; }
+    float cfull = 0.0f;  // count weight of the views whose samples all count
 
     for (int v = 0; v < a.nviews; ++v) {
         const Geo g = a.geo[a.view0 + v];
@@ -271,32 +301,57 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
                 const float sv0 = fmaf(Vc, rw, 0.5f);
                 const float dsv = g.c[7] * rw;
                 const uint32_t ub = (uint32_t)iu * S::bytes;
+                // sv is monotonic in k, so the column is inside the detector
+                // rows iff its first and last samples are.
+                const float svl = fmaf((float)(ZC - 1), dsv, sv0);
+                if (fminf(sv0, svl) >= 0.0f && fmaxf(sv0, svl) <= Vm1) {
+                    // every sample counts: the count weight is added once.
+                    // All gathers of the column are issued before any is
+                    // consumed: the kernel is bound by gather latency, and
+                    // this keeps 2 ZC of them in flight per wave.
+                    cfull += cw;
+                    uint32_t off[ZC];
+                    float fv[ZC];
 #pragma unroll
-                for (int kb = 0; kb < ZC; kb += kZB) {
-                    uint32_t off[kZB];
-                    float fv[kZB];
-                    bool in[kZB];
-#pragma unroll
-                    for (int q = 0; q < kZB; ++q) {
-                        const float sv = fmaf((float)(kb + q), dsv, sv0);
-                        const float t = __builtin_amdgcn_fmed3f(sv, 0.0f, Vm1);
-                        in[q] = (t == sv);
-                        fv[q] = __builtin_amdgcn_fractf(t);
-                        off[q] = __umul24((uint32_t)(int)t, rowb) + ub;
+                    for (int k = 0; k < ZC; ++k) {
+                        const float sv = fmaf((float)k, dsv, sv0);
+                        fv[k] = __builtin_amdgcn_fractf(sv);
+                        off[k] = __umul24((uint32_t)(int)sv, rowb) + ub;
                     }
-                    typename S::raw_t p0[kZB], p1[kZB];
+                    typename S::raw_t p[ZC];
 #pragma unroll
-                    for (int q = 0; q < kZB; ++q) {
-                        p0[q] = S::load(rs, off[q], 0);
-                        p1[q] = S::load(rs, off[q], rowb);
+                    for (int k = 0; k < ZC; ++k) p[k] = S::load(rs, off[k]);
+#pragma unroll
+                    for (int k = 0; k < ZC; ++k) {
+                        const float top = S::lerp(p[k], 0, fu);
+                        const float bot = S::lerp(p[k], 1, fu);
+                        acc[k] += fmaf(fv[k], bot - top, top);
                     }
+                } else {
 #pragma unroll
-                    for (int q = 0; q < kZB; ++q) {
-                        const float top = S::lerp(p0[q], fu);
-                        const float bot = S::lerp(p1[q], fu);
-                        const float val = fmaf(fv[q], bot - top, top);
-                        acc[kb + q] += in[q] ? val : 0.0f;
-                        cnt[kb + q] += in[q] ? cw : 0.0f;
+                    for (int kb = 0; kb < ZC; kb += kZB) {
+                        uint32_t off[kZB];
+                        float fv[kZB];
+                        bool in[kZB];
+#pragma unroll
+                        for (int q = 0; q < kZB; ++q) {
+                            const float sv = fmaf((float)(kb + q), dsv, sv0);
+                            const float t = __builtin_amdgcn_fmed3f(sv, 0.0f, Vm1);
+                            in[q] = (t == sv);
+                            fv[q] = __builtin_amdgcn_fractf(t);
+                            off[q] = __umul24((uint32_t)(int)t, rowb) + ub;
+                        }
+                        typename S::raw_t p[kZB];
+#pragma unroll
+                        for (int q = 0; q < kZB; ++q) p[q] = S::load(rs, off[q]);
+#pragma unroll
+                        for (int q = 0; q < kZB; ++q) {
+                            const float top = S::lerp(p[q], 0, fu);
+                            const float bot = S::lerp(p[q], 1, fu);
+                            const float val = fmaf(fv[q], bot - top, top);
+                            acc[kb + q] += in[q] ? val : 0.0f;
+                            cnt[kb + q] += in[q] ? cw : 0.0f;
+                        }
                     }
                 }
             }
@@ -333,16 +388,13 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
                     off[q] = __umul24((uint32_t)(int)t, rowb) + (uint32_t)iu * S::bytes;
                     uwq[q] = uwv[iu];
                 }
-                typename S::raw_t p0[kZB], p1[kZB];
+                typename S::raw_t p[kZB];
+#pragma unroll
+                for (int q = 0; q < kZB; ++q) p[q] = S::load(rs, off[q]);
 #pragma unroll
                 for (int q = 0; q < kZB; ++q) {
-                    p0[q] = S::load(rs, off[q], 0);
-                    p1[q] = S::load(rs, off[q], rowb);
-                }
-#pragma unroll
-                for (int q = 0; q < kZB; ++q) {
-                    const float top = S::lerp(p0[q], fu[q]);
-                    const float bot = S::lerp(p1[q], fu[q]);
+                    const float top = S::lerp(p[q], 0, fu[q]);
+                    const float bot = S::lerp(p[q], 1, fu[q]);
                     const float val = fmaf(fv[q], bot - top, top);
                     const float cw = fmaf(fu[q], uwq[q].y - uwq[q].x, uwq[q].x);
                     acc[kb + q] += in[q] ? val : 0.0f;
@@ -353,6 +405,8 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
     }
 
     if (i >= a.X || j >= a.Y) return;
+#pragma unroll
+    for (int kk = 0; kk < ZC; ++kk) cnt[kk] += cfull;
     const size_t plane = (size_t)a.X * a.Y;
     size_t idx = ((size_t)k0 * a.Y + j) * a.X + i;
 #pragma unroll
@@ -372,24 +426,18 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
 }
 
 // Tile schedule.  Workgroups are dispatched round-robin over the 8 XCDs, so
-// workgroup b runs on XCD b % 8.  XCD x owns the contiguous tile range
-// [x T8, (x+1) T8) -- a z-slab, whose detector rows then stay in that XCD's
-// L2 -- and its gridDim.x / 8 workgroups sweep the range in rounds.  With a
-// grid of one workgroup per tile every workgroup has one tile; with a grid
-// sized to what the device holds at once ("persistent"), all resident
-// workgroups of an XCD work on neighbouring tiles and advance through the
-// views together, which keeps the per-view working set small enough for L2.
+// workgroup b runs on XCD b % 8.  XCD x takes the contiguous tile range
+// [x T8, (x+1) T8): a z-slab, whose detector rows then stay in that XCD's L2.
+// (A persistent grid that sweeps the tiles in rounds was measured slower in
+// every tile shape on dental512 and made the compiler emit ~45 % more vector
+// instructions, so each workgroup runs exactly one tile.)
 template <int MODE, int ZC, int QT, int OUT>
 __global__ void __launch_bounds__(256) bp_kernel(BPArgs a)
 {
     const unsigned ntiles = (unsigned)a.ntx * a.nty * a.ntz;
-    const unsigned g8 = gridDim.x >> 3;  // gridDim.x is a multiple of 8
-    const unsigned t8 = (ntiles + 7) >> 3;
-    const unsigned xcd = blockIdx.x & 7u;
-    for (unsigned t = blockIdx.x >> 3; t < t8; t += g8) {
-        const unsigned tile = xcd * t8 + t;
-        if (tile < ntiles) bp_tile<MODE, ZC, QT, OUT>(a, tile);
-    }
+    const unsigned t8 = gridDim.x >> 3;  // gridDim.x = 8 * ceil(ntiles / 8)
+    const unsigned tile = (blockIdx.x & 7u) * t8 + (blockIdx.x >> 3);
+    if (tile < ntiles) bp_tile<MODE, ZC, QT, OUT>(a, tile);
 }
 
 __global__ void bp_finalize_kernel(const float* __restrict__ acc, const float* __restrict__ cnt,
@@ -455,27 +503,20 @@ static void launch_bp(jh_bp_plan* p, const void* q, int dtype, int view0, int nv
     a.cnt = p->cnt.as<float>();
     a.W = p->W; a.V = p->V; a.X = p->X; a.Y = p->Y; a.Z = p->Z;
     a.view0 = view0; a.nviews = nviews;
-    a.ntx = (p->X + 63) / 64;
-    a.nty = (p->Y + p->ty - 1) / p->ty;
+    // workgroup extent: 8 x 8 voxel columns per wave (see bp_tile)
+    const int ex = p->ty >= 2 ? 16 : 8, ey = p->ty >= 4 ? 16 : 8;
+    a.ntx = (p->X + ex - 1) / ex;
+    a.nty = (p->Y + ey - 1) / ey;
     a.ntz = (p->Z + p->zc - 1) / p->zc;
-    const size_t fb = (size_t)p->W * (p->V + 1) * dtype_bytes_pair(dtype);
+    const size_t fb = (size_t)p->W * p->V * dtype_bytes_quad(dtype);
     require(fb < (1ull << 31), "backprojection frame too large");
     a.frame_bytes = (uint32_t)fb;
     const size_t ntiles = (size_t)a.ntx * a.nty * a.ntz;
     require(ntiles < (1ull << 31), "backprojection grid too large");
-    size_t grid = (ntiles + 7) / 8 * 8;
+    const size_t grid = (ntiles + 7) / 8 * 8;
     bp_kernel_t k = out == kOutNormalised   ? pick_kernel<kOutNormalised>(p->mode, p->zc, dtype)
                     : out == kOutAccumulate ? pick_kernel<kOutAccumulate>(p->mode, p->zc, dtype)
                                             : pick_kernel<kOutRaw>(p->mode, p->zc, dtype);
-    if (p->persistent) {
-        // as many workgroups as the device holds at once (a multiple of 8)
-        int cus = 0, perCU = 0;
-        JH_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device));
-        JH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, reinterpret_cast<const void*>(k), 64 * p->ty, 0));
-        if (p->wgPerCU > 0) perCU = std::min(perCU, p->wgPerCU);
-        const size_t resident = (size_t)std::max(1, cus) * (size_t)std::max(1, perCU) / 8 * 8;
-        if (resident >= 8) grid = std::min(grid, resident);
-    }
     hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(64, p->ty), 0, stream, a);
     check_launch("bp_kernel");
 }
@@ -508,8 +549,6 @@ jh_status jh_bp_create(jh_bp_plan** plan, int device, const double* h_ptm, int n
             p->pxy = pxy; p->pz = pz;
             p->zc = 32;
             p->ty = 4;
-            p->persistent = 1;
-            p->wgPerCU = 0;
             p->ptm.alloc(sizeof(double) * 12 * nviews);
             p->geo.alloc(sizeof(Geo) * nviews);
             p->uw.alloc(sizeof(float) * (size_t)nviews * W);
@@ -554,7 +593,7 @@ jh_status jh_bp_stage_bytes(jh_bp_plan* p, int dtype, int nviews, size_t* bytes)
     return guard([&] {
         require(p != nullptr && bytes != nullptr && nviews >= 0, "jh_bp_stage_bytes: bad arguments");
         require(dtype == JH_F32 || dtype == JH_BF16 || dtype == JH_F16, "jh_bp_stage_bytes: unknown dtype");
-        *bytes = (size_t)p->W * (p->V + 1) * nviews * dtype_bytes_pair(dtype);
+        *bytes = (size_t)p->W * p->V * nviews * dtype_bytes_quad(dtype);
     });
 }
 
@@ -567,7 +606,7 @@ static void stage(jh_bp_plan* p, const float* d_filtered, int view0, int nviews,
         require(dtype == JH_F32 || dtype == JH_BF16 || dtype == JH_F16, "jh_bp_stage: unknown dtype");
         if (nviews == 0) return;
         DeviceScope ds(p->device);
-        const size_t total = (size_t)p->W * (p->V + 1) * nviews;
+        const size_t total = (size_t)p->W * p->V * nviews;
         const float* uw = weighted ? p->uw.as<float>() + (size_t)view0 * p->W : nullptr;
         const unsigned grid = grid_for(total, 256);
         hipStream_t s = as_stream(stream);
@@ -674,17 +713,6 @@ jh_status jh_bp_set_tile(jh_bp_plan* p, int zc, int ty)
     });
 }
 
-jh_status jh_bp_set_schedule(jh_bp_plan* p, int persistent, int wgPerCU)
-{
-    return guard([&] {
-        require(p != nullptr, "jh_bp_set_schedule: plan is NULL");
-        require(persistent == 0 || persistent == 1, "jh_bp_set_schedule: persistent must be 0 or 1");
-        require(wgPerCU >= 0 && wgPerCU <= 64, "jh_bp_set_schedule: bad workgroups per CU");
-        p->persistent = persistent;
-        p->wgPerCU = wgPerCU;
-    });
-}
-
 jh_status jh_bp_autotune(jh_bp_plan* p, const void* d_q, int dtype, int nviews, float* d_vol, int reps,
                          jh_stream_t stream, float* h_ms)
 {
@@ -699,27 +727,25 @@ jh_status jh_bp_autotune(jh_bp_plan* p, const void* d_q, int dtype, int nviews, 
         JH_HIP(hipEventCreate(&e1));
         const int zcs[3] = {8, 16, 32}, tys[3] = {1, 2, 4};
         float best = 1e30f;
-        int bz = p->zc, bt = p->ty, bp = p->persistent;
+        int bz = p->zc, bt = p->ty;
         int n = 0;
         try {
-            for (int pers = 0; pers < 2; ++pers)
-                for (int zi = 0; zi < 3; ++zi)
-                    for (int ti = 0; ti < 3; ++ti) {
-                        p->zc = zcs[zi];
-                        p->ty = tys[ti];
-                        p->persistent = pers;
-                        launch_bp(p, d_q, dtype, 0, nviews, d_vol, kOutNormalised, s);  // warm-up
-                        JH_HIP(hipEventRecord(e0, s));
-                        for (int r = 0; r < reps; ++r) launch_bp(p, d_q, dtype, 0, nviews, d_vol, kOutNormalised, s);
-                        JH_HIP(hipEventRecord(e1, s));
-                        JH_HIP(hipEventSynchronize(e1));
-                        float ms = 0.0f;
-                        JH_HIP(hipEventElapsedTime(&ms, e0, e1));
-                        ms /= (float)reps;
-                        if (h_ms) h_ms[n] = ms;
-                        ++n;
-                        if (ms < best) { best = ms; bz = p->zc; bt = p->ty; bp = pers; }
-                    }
+            for (int zi = 0; zi < 3; ++zi)
+                for (int ti = 0; ti < 3; ++ti) {
+                    p->zc = zcs[zi];
+                    p->ty = tys[ti];
+                    launch_bp(p, d_q, dtype, 0, nviews, d_vol, kOutNormalised, s);  // warm-up
+                    JH_HIP(hipEventRecord(e0, s));
+                    for (int r = 0; r < reps; ++r) launch_bp(p, d_q, dtype, 0, nviews, d_vol, kOutNormalised, s);
+                    JH_HIP(hipEventRecord(e1, s));
+                    JH_HIP(hipEventSynchronize(e1));
+                    float ms = 0.0f;
+                    JH_HIP(hipEventElapsedTime(&ms, e0, e1));
+                    ms /= (float)reps;
+                    if (h_ms) h_ms[n] = ms;
+                    ++n;
+                    if (ms < best) { best = ms; bz = p->zc; bt = p->ty; }
+                }
         } catch (...) {
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
@@ -729,7 +755,6 @@ jh_status jh_bp_autotune(jh_bp_plan* p, const void* d_q, int dtype, int nviews, 
         (void)hipEventDestroy(e1);
         p->zc = bz;
         p->ty = bt;
-        p->persistent = bp;
     });
 }
 

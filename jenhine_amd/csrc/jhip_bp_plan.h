@@ -20,8 +20,6 @@ struct jh_bp_plan {
     float pxy, pz;
     int mode;        // 0: u and w constant along z, 1: w constant along z, 2: general
     int zc, ty;      // backprojector tile: z voxels per thread, columns along y
-    int persistent;  // 1: grid sized to the resident workgroups (tile rounds)
-    int wgPerCU;     // cap on resident workgroups per CU in the persistent grid (0: occupancy limit)
     jh::DevBuf ptm;  // N x 12 double
     jh::DevBuf geo;  // N x Geo
     jh::DevBuf uw;   // N x W float

@@ -410,6 +410,18 @@ static float bf16_to_float(unsigned short b)
 
 using namespace jh;
 
+namespace jh {
+
+void filter_plan_shape(const jh_filter_plan* p, int* device, int* W, int* V, int* nviews)
+{
+    *device = p->device;
+    *W = p->W;
+    *V = p->V;
+    *nviews = p->N;
+}
+
+}  // namespace jh
+
 extern "C" {
 
 jh_status jh_filter_create(jh_filter_plan** plan, int device, int W, int V, int nviews,

@@ -145,16 +145,20 @@ inline void check_launch(const char* what)
     if (e != hipSuccess) fail(JH_ERR_HIP, std::string("launch of ") + what + ": " + hipGetErrorString(e));
 }
 
-inline int dtype_bytes_pair(int dtype)
+inline int dtype_bytes_quad(int dtype)
 {
-    // Storage bytes per detector pixel in the pair layout (p[u], p[u+1]).
-    return dtype == JH_F32 ? 8 : 4;
+    // Storage bytes per detector pixel in the backprojector's quad layout
+    // (the 2 x 2 neighbourhood of a bilinear sample).
+    return dtype == JH_F32 ? 16 : 8;
 }
 
 // Host filter design (FP64): H[k], k = 0..L/2, incl. the 2*pi*f_N gain.
 std::vector<double> design_filter(int W, double hsr, double cutStart, double cutEnd,
                                   const std::vector<double>& adjPts,
                                   const std::vector<double>& adjDB, int window);
+
+// Device and frame shape of a filter plan (W x V frames, nviews views).
+void filter_plan_shape(const jh_filter_plan* p, int* device, int* W, int* V, int* nviews);
 
 inline int fft_length(int W)
 {

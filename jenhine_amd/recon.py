@@ -151,17 +151,14 @@ class BPCore:
     def set_tile(self, zc=0, ty=0):
         call("jh_bp_set_tile", self._h, zc, ty)
 
-    def set_schedule(self, persistent=1, wg_per_cu=0):
-        call("jh_bp_set_schedule", self._h, int(persistent), int(wg_per_cu))
-
     def autotune(self, q, dtype="bf16", n_views=None, reps=1):
-        """Times every tile shape and schedule on views 0..n_views-1 of q and
-        keeps the fastest; returns {(persistent, zc, ty): ms}."""
+        """Times every tile shape on views 0..n_views-1 of q and keeps the
+        fastest; returns {(zc, ty): ms}."""
         n = self.n_views if n_views is None else n_views
         vol = torch.empty((self.Z, self.Y, self.X), dtype=torch.float32, device=q.device)
-        ms = (ctypes.c_float * 18)()
+        ms = (ctypes.c_float * 9)()
         call("jh_bp_autotune", self._h, _ptr(q), DTYPES[dtype], n, _ptr(vol), reps, _stream(), ms)
-        keys = [(p, z, t) for p in (0, 1) for z in (8, 16, 32) for t in (1, 2, 4)]
+        keys = [(z, t) for z in (8, 16, 32) for t in (1, 2, 4)]
         return dict(zip(keys, list(ms)))
 
     def tile(self):
@@ -262,3 +259,50 @@ class FDK:
         if return_stages:
             return vol, dict(pre=p, filt=f, q=q)
         return vol
+
+
+class BatchGraph:
+    """Many scans of uneven sizes reconstructed by one hipGraph launch
+    (the ``jh_graph_*`` entry points).  Each scan is an ``FDK`` (its plans)
+    with bound device buffers; the graph reads the buffers at every launch.
+    """
+
+    def __init__(self, device="cuda:0"):
+        h = ctypes.c_void_p()
+        call("jh_graph_create", ctypes.byref(h), _dev_index(device))
+        self._h = h
+        self._keep = []  # plans and buffers the graph refers to
+
+    def add(self, fdk, raw, vol, dark=None, gain=None, views_per_chunk=0):
+        """Adds one scan: raw (N, inH, inW) 16-bit counts, vol (Z, Y, X) f32."""
+        _require_cuda(raw, "raw")
+        if raw.element_size() != 2:
+            raise ValueError("raw must hold 16-bit counts")
+        _require_cuda(vol, "vol", torch.float32)
+        _require_cuda(dark, "dark")
+        _require_cuda(gain, "gain", torch.float32)
+        idx = ctypes.c_int()
+        call("jh_graph_add_scan", self._h, fdk.pre._h, fdk.filter._h, fdk.bp._h, _ptr(raw), _ptr(dark),
+             _ptr(gain), DTYPES[fdk.storage], views_per_chunk, _ptr(vol), ctypes.byref(idx))
+        self._keep.append((fdk, raw, vol, dark, gain))
+        return idx.value
+
+    def autotune(self, reps=1):
+        call("jh_graph_autotune", self._h, reps, _stream())
+
+    def tuning(self, index):
+        """(views per chunk, zc, ty) of scan `index`."""
+        c, z, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        call("jh_graph_scan_tuning", self._h, index, ctypes.byref(c), ctypes.byref(z), ctypes.byref(t))
+        return c.value, z.value, t.value
+
+    def instantiate(self, branches=4):
+        call("jh_graph_instantiate", self._h, branches)
+
+    def launch(self):
+        call("jh_graph_launch", self._h, _stream())
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.lib().jh_graph_destroy(self._h)
+            self._h = None

@@ -1,0 +1,179 @@
+"""Batched reconstruction: scans of uneven sizes captured into one hipGraph.
+
+The graph chains, per scan, the single-scan entry points over chunks of
+views (preprocess -> filter -> stage -> accumulate, then finalise).  It is
+checked against the same chain run eagerly (to the bit: the kernels and
+their order are the same), against the FP64 oracle per scan, over repeated
+replays with other work in between, and after the bound inputs change.
+"""
+import numpy as np
+import pytest
+import torch
+
+from jenhine_amd import configs
+from jenhine_amd._lib import JhError
+from oracle import oracle as o
+from tests.helpers import max_abs, oracle_params, rel_rmse
+
+pytestmark = pytest.mark.gpu
+
+
+def _u16(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a, np.uint16).view(np.int16)).to(dev)
+
+
+def _workload(i, n, views, det):
+    fov = n * 0.25 * 1.25
+    return configs.Workload(f"small{i}", 300.0, 500.0, views, det, det, fov * (500.0 / 300.0) / det, n, n, n,
+                            0.25, 0.25, n_ellipsoids=6, window=0, ramlak=True, seed=900 + i)
+
+
+# (volume edge, views, detector edge, storage, filter form, views per chunk)
+SPECS = [(48, 60, 96, "bf16", "mfma6", 16), (64, 90, 128, "bf16", "mfma6", 25), (40, 45, 80, "f32", "fp32", 45),
+         (56, 72, 112, "f16", "mfma6", 32), (32, 30, 64, "bf16", "mfma", 7)]
+
+
+@pytest.fixture(scope="module")
+def scans(gpu):
+    from jenhine_amd.recon import FDK
+    out = []
+    for i, (n, views, det, storage, form, chunk) in enumerate(SPECS):
+        w = _workload(i, n, views, det)
+        d = w.data()
+        fdk = FDK(w.scan_params(), w.recon_params(), d["ptm"], form, storage, device=gpu)
+        out.append(dict(w=w, d=d, fdk=fdk, chunk=chunk, raw=_u16(d["raw"], gpu), dark=_u16(d["dark"], gpu),
+                        gain=torch.from_numpy(d["gain"]).to(gpu),
+                        vol=torch.empty((n, n, n), dtype=torch.float32, device=gpu)))
+    return out
+
+
+def _eager(s):
+    """The graph's chain for one scan, run call by call."""
+    fdk, raw, chunk = s["fdk"], s["raw"], s["chunk"]
+    n = raw.shape[0]
+    fdk.bp.reset()
+    for v0 in range(0, n, chunk):
+        c = min(chunk, n - v0)
+        p = fdk.pre.process(raw[v0:v0 + c], s["dark"], s["gain"])
+        f = fdk.filter.process(p, view0=v0)
+        q = fdk.bp.stage(f, fdk.storage, view0=v0)
+        fdk.bp.accumulate(q, fdk.storage, v0, c)
+    return fdk.bp.finalize()
+
+
+@pytest.fixture(scope="module")
+def graph(gpu, scans):
+    from jenhine_amd.recon import BatchGraph
+    g = BatchGraph(device=gpu)
+    for s in scans:
+        g.add(s["fdk"], s["raw"], s["vol"], s["dark"], s["gain"], views_per_chunk=s["chunk"])
+    g.instantiate(branches=3)
+    return g
+
+
+def test_graph_equals_eager_chain(gpu, scans, graph):
+    graph.launch()
+    torch.cuda.synchronize()
+    got = [s["vol"].cpu().numpy() for s in scans]
+    for i, s in enumerate(scans):
+        ref = _eager(s).cpu().numpy()
+        assert np.isfinite(got[i]).all()
+        np.testing.assert_array_equal(got[i], ref, err_msg=f"scan {i}")
+
+
+def test_graph_scans_vs_fp64_oracle(gpu, scans, graph):
+    graph.launch()
+    torch.cuda.synchronize()
+    for i, s in enumerate(scans):
+        w, d = s["w"], s["d"]
+        sc, rc = oracle_params(w)
+        v64 = o.fdk(d["raw"], d["ptm"], sc, rc, fp64=True, dark=d["dark"], gain=d["gain"])
+        vol = s["vol"].cpu().numpy()
+        e = rel_rmse(vol, v64)
+        print(f"scan {i} {w.X}^3 x {w.n_views} views of {w.n_u}^2 {s['fdk'].storage}: rel-RMSE vs FP64 {e:.3e} "
+              f"max-abs {max_abs(vol, v64):.3e}")
+        assert e <= (1e-4 if s["fdk"].storage == "f32" else 1e-3)
+
+
+def test_graph_replays_agree_with_work_in_between(gpu, scans, graph):
+    graph.launch()
+    torch.cuda.synchronize()
+    first = [s["vol"].cpu().numpy() for s in scans]
+    for r in range(3):
+        # unrelated eager work and synchronisation between the replays
+        x = torch.randn(1 << 20, device=gpu)
+        _ = (x * 2.0).sum().item()
+        for s in scans:
+            s["vol"].fill_(float("nan"))
+        graph.launch()
+        torch.cuda.synchronize()
+        for i, s in enumerate(scans):
+            np.testing.assert_array_equal(s["vol"].cpu().numpy(), first[i], err_msg=f"replay {r} scan {i}")
+
+
+def test_graph_reads_inputs_at_launch(gpu, scans, graph):
+    s0 = scans[0]
+    keep = s0["raw"].clone()
+    try:
+        # new counts in the bound buffer: half the signal, same dark
+        raw = s0["d"]["raw"].astype(np.int64)
+        dark = s0["d"]["dark"].astype(np.int64)
+        new = np.clip(dark + (raw - dark) // 2, 0, 65535).astype(np.uint16)
+        s0["raw"].copy_(_u16(new, gpu))
+        graph.launch()
+        torch.cuda.synchronize()
+        got = s0["vol"].cpu().numpy()
+        ref = _eager(s0).cpu().numpy()
+        np.testing.assert_array_equal(got, ref)
+        s1 = scans[1]
+        np.testing.assert_array_equal(s1["vol"].cpu().numpy(), _eager(s1).cpu().numpy())
+    finally:
+        s0["raw"].copy_(keep)
+
+
+def test_graph_autotune_picks_tiles_and_chunks(gpu):
+    from jenhine_amd.recon import FDK, BatchGraph
+    g = BatchGraph(device=gpu)
+    keep = []
+    for i, (n, views, det) in enumerate([(64, 120, 128), (48, 40, 96)]):
+        w = _workload(10 + i, n, views, det)
+        d = w.data()
+        fdk = FDK(w.scan_params(), w.recon_params(), d["ptm"], "mfma6", "bf16", device=gpu)
+        vol = torch.empty((n, n, n), dtype=torch.float32, device=gpu)
+        g.add(fdk, _u16(d["raw"], gpu), vol, _u16(d["dark"], gpu), torch.from_numpy(d["gain"]).to(gpu))
+        keep.append((w, d, vol))
+    g.autotune(reps=1)
+    for i, (w, _, _) in enumerate(keep):
+        chunk, zc, ty = g.tuning(i)
+        cands = [c for c in (16, 32, 64, 128, 256) if c < w.n_views] + [min(w.n_views, 512)]
+        print(f"scan {i}: chunk {chunk}, tile zc {zc} ty {ty}")
+        assert chunk in cands and zc in (8, 16, 32) and ty in (1, 2, 4)
+    g.instantiate(branches=2)
+    g.launch()
+    torch.cuda.synchronize()
+    for w, d, vol in keep:
+        sc, rc = oracle_params(w)
+        v64 = o.fdk(d["raw"], d["ptm"], sc, rc, fp64=True, dark=d["dark"], gain=d["gain"])
+        assert rel_rmse(vol.cpu().numpy(), v64) <= 1e-3
+
+
+def test_graph_rejects_bad_use(gpu, scans):
+    from jenhine_amd.recon import BatchGraph
+    s0, s1 = scans[0], scans[1]
+    g = BatchGraph(device=gpu)
+    with pytest.raises(JhError):
+        g.launch()  # not instantiated
+    with pytest.raises(JhError):
+        g.instantiate()  # no scans
+
+    class Mixed:  # filter plan of another detector
+        pre, filter, bp, storage = s0["fdk"].pre, s1["fdk"].filter, s0["fdk"].bp, "bf16"
+
+    with pytest.raises(JhError):
+        g.add(Mixed, s0["raw"], s0["vol"])
+    g.add(s0["fdk"], s0["raw"], s0["vol"], views_per_chunk=8)
+    with pytest.raises(JhError):
+        g.add(s0["fdk"], s0["raw"], s1["vol"])  # the same plan twice
+    g.instantiate(branches=1)
+    with pytest.raises(JhError):
+        g.add(s1["fdk"], s1["raw"], s1["vol"])  # after instantiation
