@@ -228,6 +228,9 @@ __global__ void __launch_bounds__(256) filter_fp32_kernel(FiltArgs a)
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
+// Native 16-byte vector for the B prefetch: HIP's uint4 is a struct whose
+// copies go through memory and would keep the prefetch in scratch.
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
 
 constexpr int kModeF32 = 0, kModeX3 = 2, kModeX6 = 3;
 
@@ -303,13 +306,13 @@ __global__ void __launch_bounds__(256) filter_mfma_kernel(FiltArgs a)
 #pragma unroll
         for (int c = 0; c < 8; ++c) acc[rb][c] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
 
-    const uint4* gB = reinterpret_cast<const uint4*>(a.Bhl);
+    const u32x4_t* gB = reinterpret_cast<const u32x4_t*>(a.Bhl);
     auto gsrc = [&](int s) { return gB + ((size_t)s * ncb8 + cb0) * 64 * LB / 16; };
-    uint4 pre[CHUNKS];
+    u32x4_t pre[CHUNKS];
 #pragma unroll
     for (int c = 0; c < CHUNKS; ++c) pre[c] = gsrc(0)[threadIdx.x + c * 256];
 #pragma unroll
-    for (int c = 0; c < CHUNKS; ++c) reinterpret_cast<uint4*>(lds)[threadIdx.x + c * 256] = pre[c];
+    for (int c = 0; c < CHUNKS; ++c) reinterpret_cast<u32x4_t*>(lds)[threadIdx.x + c * 256] = pre[c];
     __syncthreads();
 
     for (int s = 0; s < nsteps; ++s) {
@@ -372,7 +375,7 @@ __global__ void __launch_bounds__(256) filter_mfma_kernel(FiltArgs a)
         __syncthreads();
         if (s + 1 < nsteps) {
 #pragma unroll
-            for (int c = 0; c < CHUNKS; ++c) reinterpret_cast<uint4*>(lds)[threadIdx.x + c * 256] = pre[c];
+            for (int c = 0; c < CHUNKS; ++c) reinterpret_cast<u32x4_t*>(lds)[threadIdx.x + c * 256] = pre[c];
         }
         __syncthreads();
     }
