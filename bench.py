@@ -15,6 +15,12 @@ normalised volume.  Two forms are timed:
 value = voxel-updates per second of the headline form over the whole step
 (X*Y*Z*N / step time), in GUPS.
 
+--workload batch16 times the batched form instead: 16 scans of uneven sizes
+(volumes 128^3..400^3, 180..720 views, detectors 256^2..1024^2) reconstructed
+by one hipGraph launch (BF16 storage, 6-product filter), tiles and view chunks
+from the batch autotuner; value = aggregate GUPS of one launch, with per-scan
+rel-RMSE against the FP64 oracle on the centre slices.
+
 The roofline object covers the dominant kernel (bp_kernel) timed with events
 on the stream it runs on; its algorithmic bytes are 8 B per voxel-update (an
 FP32 accumulator read + write per update, the reference's per-view
@@ -58,6 +64,8 @@ def main():
                     help="profiling run: a 1 GiB copy, then only the backprojection kernel")
     ap.add_argument("--kernel-form", choices=("fp32", "bf16", "f16"), default="bf16",
                     help="storage form of the --kernel-only backprojection")
+    ap.add_argument("--workload", choices=("dental512", "batch16"), default="dental512")
+    ap.add_argument("--branches", type=int, default=4, help="parallel graph branches (batch16)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -71,6 +79,8 @@ def main():
     if not torch.cuda.is_available():
         sys.exit("bench.py needs a GPU")
     dev = torch.device("cuda:0")
+    if args.workload == "batch16":
+        return run_batch(args, dev)
     w = configs.DENTAL512
     if args.views:
         w = w.scaled(n_views=args.views)
@@ -205,6 +215,14 @@ def main():
     head = results["bf16"]
     bp_bytes = 8.0 * units
     achieved = bp_bytes / (head["bp_ms"] * 1e-3) / 1e9
+    # HBM bytes per launch of this kernel from the committed rocprofv3 --pmc
+    # summary (FETCH_SIZE and WRITE_SIZE, each scaled by what the counter
+    # reads for a 1 GiB device copy); null when the summary is absent
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "bp_dental512_bf16_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as fh:
+            traffic = json.load(fh)["hbm_bytes"]
     line = {
         "metric": "backprojection GUPS at 512^3 x 600 views; FP32 vs BF16/MFMA path at <=1e-3 rel-RMSE",
         "value": head["gups"],
@@ -225,9 +243,100 @@ def main():
         "autotune": tuned,
         "accuracy": acc,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
-                     "frac": achieved / 8000.0, "traffic": None,
-                     "kernel": "bp_kernel (bf16 pair storage)", "bytes_per_update": 8},
+                     "frac": achieved / 8000.0, "traffic": traffic,
+                     "kernel": "bp_kernel (bf16 quad storage)", "bytes_per_update": 8,
+                     "note": "achieved counts the per-view formulation's 8 B/voxel-update (FP32 accumulator "
+                             "read + write); frac > 1 means the register-accumulating kernel beats that roof. "
+                             "Measured limiter: gather addresses (texture addresser ~97% busy, "
+                             "profiles/bp_dental512_pmc.md)"},
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def run_batch(args, dev):
+    """16 uneven scans, one hipGraph launch per step."""
+    import torch
+    from jenhine_amd import configs
+    from jenhine_amd.recon import FDK, BatchGraph
+
+    t0 = time.time()
+    entries = []
+    for w in configs.batch16():
+        d = w.data()
+        del d["lineint"]
+        e = dict(w=w, d=d, raw=torch.from_numpy(d["raw"].view(np.int16)).to(dev),
+                 dark=torch.from_numpy(d["dark"].view(np.int16)).to(dev), gain=torch.from_numpy(d["gain"]).to(dev),
+                 vol=torch.empty((w.Z, w.Y, w.X), dtype=torch.float32, device=dev))
+        e["fdk"] = FDK(w.scan_params(), w.recon_params(), d["ptm"], "mfma6", "bf16", dev)
+        entries.append(e)
+    units = sum(e["w"].gups_units for e in entries)
+    log(f"16 scans generated in {time.time() - t0:.1f} s; {units / 1e9:.1f} G voxel-updates per launch")
+    g = BatchGraph(device=dev)
+    for e in entries:
+        g.add(e["fdk"], e["raw"], e["vol"], e["dark"], e["gain"])
+    t = time.time()
+    g.autotune(reps=1)
+    log(f"autotune {time.time() - t:.1f} s")
+    g.instantiate(branches=args.branches)
+    for _ in range(args.warmup):
+        g.launch()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        g.launch()
+    torch.cuda.synchronize()
+    step = (time.perf_counter() - t) / args.steps
+    log(f"graph launch {step * 1e3:.1f} ms, {units / step / 1e9:.1f} GUPS")
+    # the same scans one after another through the one-shot chain (no graph)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for e in entries:
+        e["fdk"](e["raw"], e["dark"], e["gain"])
+    torch.cuda.synchronize()
+    seq = time.perf_counter() - t
+    log(f"sequential one-shot FDK {seq * 1e3:.1f} ms, {units / seq / 1e9:.1f} GUPS")
+
+    per_scan = []
+    if not args.no_accuracy:
+        from oracle import oracle as o
+        nth = min(os.cpu_count() or 1, 16)
+        t = time.time()
+        for i, e in enumerate(entries):
+            w, d = e["w"], e["d"]
+            scan, rec = w.scan_params(), w.recon_params()
+            sc = dict(bright_field=scan.BrightField, hsr=scan.HalfSampleRate, dso=scan.DSO,
+                      bh=scan.BeamHardeningParams)
+            rc = dict(X=w.X, Y=w.Y, Z=w.Z, pxy=w.pxy, pz=w.pz, cut_start=rec.FilterCutOffStart,
+                      cut_end=rec.FilterCutOffEnd, window=rec.FilterWindow)
+            z0 = w.Z // 2 - 1
+            v64 = o.fdk(d["raw"], d["ptm"], sc, rc, fp64=True, nthreads=nth, z0=z0, nz=2, dark=d["dark"],
+                        gain=d["gain"])
+            got = e["vol"][z0:z0 + 2].cpu().numpy()
+            chunk, zc, ty = g.tuning(i)
+            per_scan.append(dict(volume=w.X, views=w.n_views, detector=w.n_u, chunk=chunk, tile=[zc, ty],
+                                 rel_rmse_vs_fp64=rel_rmse(got, v64),
+                                 max_abs_vs_fp64=float(np.nanmax(np.abs(got.astype(np.float64) - v64)))))
+            log("scan", i, json.dumps(per_scan[-1]))
+        log(f"FP64 oracle slices in {time.time() - t:.1f} s")
+    line = {
+        "metric": "batched FDK: aggregate GUPS of 16 uneven scans in one hipGraph launch",
+        "value": units / step / 1e9,
+        "unit": "GUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": step * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16-storage/f32-accumulate",
+        "data": "synthetic (seeded ellipsoid phantoms, analytic line integrals, Poisson counts)",
+        "config": {"workload": "batch16: 16 scans, volumes 128^3..400^3, 180..720 views, detectors 256^2..1024^2",
+                   "scans": len(entries), "voxel_updates": units, "branches": args.branches,
+                   "parallelism": "single-gpu"},
+        "sequential_oneshot_ms": seq * 1e3,
+        "per_scan": per_scan,
     }
     print(json.dumps(line), flush=True)
 
