@@ -126,7 +126,8 @@ def main():
         # per-view cost depends on the view angle: tune on the whole orbit
         table = fdk.bp.autotune(q0, fdk.storage, reps=1)
         tuned[name] = dict(zip(("zc", "ty"), fdk.bp.tile()))
-        tuned[name]["table_ms"] = {f"zc{k[0]}_ty{k[1]}": round(v, 3) for k, v in table.items()}
+        tuned[name]["fetch"] = fdk.bp.fetch()
+        tuned[name]["table_ms"] = {f"{k[0]}_zc{k[1]}_ty{k[2]}": round(v, 3) for k, v in table.items()}
         log(name, "autotune", json.dumps(tuned[name]))
         del q0
         torch.cuda.synchronize()

@@ -41,6 +41,9 @@ typedef int jh_status;
 #define JH_FILTER_FP32_VALU 3   /* FP32 FMA on the vector ALUs (cross-check) */
 
 /* Filter windows. */
+#define JH_FETCH_GATHER 0 /* backprojector sample fetch forms (jh_bp_set_fetch) */
+#define JH_FETCH_LDS 1
+
 #define JH_WINDOW_REFERENCE 0   /* FilterCore.cpp:96-134 cut-off semantics */
 #define JH_WINDOW_SHEPP_LOGAN 1 /* extension: sinc(f / (2 f_N)), zero above f_N */
 
@@ -200,17 +203,27 @@ jh_status jh_fp_project(jh_bp_plan* plan, const float* d_vol, float* d_proj, int
                         int weighted, jh_stream_t stream);
 
 /* Tile shape of the backprojector: z voxels per thread (8, 16 or 32) and
- * voxel columns per workgroup along y (1, 2 or 4; 64 along x).  0 keeps the
- * current value.  No counterpart in the reference (its launch shape is fixed
- * at BPCUDACore.cu:99-112); used by the autotuner. */
+ * waves per workgroup (1, 2 or 4; each wave covers 8 x 8 voxel columns, so a
+ * workgroup covers 8 x 8, 16 x 8 or 16 x 16).  0 keeps the current value.  No
+ * counterpart in the reference (its launch shape is fixed at
+ * BPCUDACore.cu:99-112); used by the autotuner. */
 jh_status jh_bp_set_tile(jh_bp_plan* plan, int zc, int ty);
 
+/* How the backprojector fetches detector samples: JH_FETCH_GATHER (one
+ * gather per sample through the vector memory path) or JH_FETCH_LDS (each
+ * workgroup stages its detector footprint of the view into LDS once, then
+ * samples from LDS; geometry class 0 only, other classes gather).  The
+ * reference samples a texture (BPCUDACore.cu:62); no counterpart. */
+jh_status jh_bp_set_fetch(jh_bp_plan* plan, int fetch);
+jh_status jh_bp_get_fetch(jh_bp_plan* plan, int* fetch);
+
 /* Autotuner: times the normalised backprojection of views 0..nviews-1 of
- * d_q (reps launches each, after one warm-up) for every tile shape
- * zc in {8, 16, 32} x ty in {1, 2, 4}, and keeps the fastest in the plan.
- * h_ms (9 floats, may be NULL) receives the milliseconds per launch in the
- * order zc, ty (outer to inner).  The per-view cost depends on the view
- * angle, so nviews should cover the whole orbit.
+ * d_q (reps launches each, after one warm-up) for every fetch form x tile
+ * shape zc in {8, 16, 32} x ty in {1, 2, 4}, and keeps the fastest in the
+ * plan.  h_ms (18 floats, may be NULL) receives the milliseconds per launch
+ * in the order fetch, zc, ty (outer to inner); -1 marks a combination that
+ * does not apply (the LDS fetch outside geometry class 0).  The per-view cost
+ * depends on the view angle, so nviews should cover the whole orbit.
  * Writes d_vol; synchronises the stream.  No counterpart in the reference. */
 jh_status jh_bp_autotune(jh_bp_plan* plan, const void* d_q, int dtype, int nviews, float* d_vol, int reps,
                          jh_stream_t stream, float* h_ms);

@@ -194,6 +194,12 @@ struct Sampler<JH_F32> {
         return r == 0 ? fmaf(fu, __uint_as_float(p.y), __uint_as_float(p.x))
                       : fmaf(fu, __uint_as_float(p.w), __uint_as_float(p.z));
     }
+    // bilinear coefficients (q00, q01 - q00, q10 - q00, (q11 - q10) - (q01 - q00))
+    __device__ static float4 abcd(raw_t p)
+    {
+        const float a = __uint_as_float(p.x), b = __uint_as_float(p.y);
+        return make_float4(a, b, __uint_as_float(p.z) - a, __uint_as_float(p.w) - b);
+    }
 };
 
 template <>
@@ -210,6 +216,13 @@ struct Sampler<JH_BF16> {
         const float a = __uint_as_float(x << 16), b = __uint_as_float(x & 0xffff0000u);
         return fmaf(fu, b - a, a);
     }
+    __device__ static float4 abcd(raw_t p)
+    {
+        const float q00 = __uint_as_float(p.x << 16), q01 = __uint_as_float(p.x & 0xffff0000u);
+        const float q10 = __uint_as_float(p.y << 16), q11 = __uint_as_float(p.y & 0xffff0000u);
+        const float b = q01 - q00;
+        return make_float4(q00, b, q10 - q00, (q11 - q10) - b);
+    }
 };
 
 template <>
@@ -223,9 +236,20 @@ struct Sampler<JH_F16> {
     __device__ static float lerp(raw_t p, int r, float fu)
     {
         // b fu + (a - a fu): two v_fma_mix, which convert the f16 halves inside
-        const half2_t h = __builtin_bit_cast(half2_t, r == 0 ? p.x : p.y);
+        const uint32_t x = r == 0 ? p.x : p.y;
+        const half2_t h = __builtin_bit_cast(half2_t, x);
         const float a = (float)h.x;
         return fmaf((float)h.y, fu, fmaf(-a, fu, a));
+    }
+    __device__ static float4 abcd(raw_t p)
+    {
+        // the halves are copied out of the vector first: a bit cast taken
+        // directly on the element p.y was compiled as one of p.x
+        const uint32_t px = p.x, py = p.y;
+        const half2_t t = __builtin_bit_cast(half2_t, px), u = __builtin_bit_cast(half2_t, py);
+        const float q00 = (float)t.x, q01 = (float)t.y, q10 = (float)u.x, q11 = (float)u.y;
+        const float b = q01 - q00;
+        return make_float4(q00, b, q10 - q00, (q11 - q10) - b);
     }
 };
 
@@ -243,6 +267,7 @@ struct BPArgs {
 };
 
 constexpr int kZB = 8;  // samples whose loads are issued together
+constexpr int kModeStaged = 3;  // geometry class 0 with the LDS-staged fetch
 
 // Output forms of the backprojection kernel.
 constexpr int kOutNormalised = 0;  // vol = acc / count (one-shot)
@@ -425,6 +450,226 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// LDS-staged fetch (geometry class 0: u and w constant along z).
+//
+// Per view the workgroup computes the detector footprint of its voxel block
+// from the block's corners -- u/w and v/w are linear-fractional in the voxel
+// coordinates, so their extremes over the block are at its corners -- widened
+// by one pixel against rounding, and copies that rectangle of quads once into
+// LDS as FP32 bilinear coefficients (a, b, c, d):
+//   value(fu, fv) = a + fu b + fv (c + fu d).
+// Every sample is then one ds_read_b128 and three FMAs; the per-sample
+// gathers no longer pass through the texture addresser, which bounds the
+// gather form.  A footprint larger than the LDS budget falls back to the
+// gather form for that view.  Sample positions are computed exactly as in the
+// gather form.
+// ---------------------------------------------------------------------------
+
+constexpr int kLdsPixels = 2048;  // 32 KiB of FP32 coefficients per workgroup
+
+template <int ZC, int QT, int OUT>
+__device__ __forceinline__ void bp_tile_lds(const BPArgs& a, unsigned tile)
+{
+    typedef Sampler<QT> S;
+    __shared__ float4 lds[kLdsPixels];
+    __shared__ int box[4];
+    const int tx = tile % a.ntx;
+    const unsigned rest = tile / a.ntx;
+    const int tyi = rest % a.nty;
+    const int tz = rest / a.nty;
+    const int px = blockDim.y >= 2 ? 2 : 1;
+    const int wx = threadIdx.y % px, wy = threadIdx.y / px;
+    const int ex = px * 8, ey = (blockDim.y / px) * 8;
+    const int i = tx * ex + wx * 8 + (threadIdx.x & 7);
+    const int j = tyi * ey + wy * 8 + (threadIdx.x >> 3);
+    const int k0 = tz * ZC;
+    const float fi = (float)min(i, a.X - 1);
+    const float fj = (float)min(j, a.Y - 1);
+    const float fk0 = (float)k0;
+    // corner columns of the block (clamped like the threads' own columns)
+    const float cx0 = (float)min(tx * ex, a.X - 1), cx1 = (float)min(tx * ex + ex - 1, a.X - 1);
+    const float cy0 = (float)min(tyi * ey, a.Y - 1), cy1 = (float)min(tyi * ey + ey - 1, a.Y - 1);
+    const float Wm1 = (float)(a.W - 1), Vm1 = (float)(a.V - 1);
+    const uint32_t rowb = (uint32_t)a.W * S::bytes;
+    const int nthr = 64 * blockDim.y;
+    const int tid = threadIdx.y * 64 + threadIdx.x;
+
+    float acc[ZC], cnt[ZC];
+#pragma unroll
+    for (int kk = 0; kk < ZC; ++kk) { acc[kk] = 0.0f; cnt[kk] = 0.0f; }
+    float cfull = 0.0f;
+
+    for (int v = 0; v < a.nviews; ++v) {
+        const Geo g = a.geo[a.view0 + v];
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<char*>(reinterpret_cast<const char*>(a.q)) + (size_t)v * a.frame_bytes, (short)0,
+            (int)a.frame_bytes, kBufferFlags);
+
+        // Footprint of the block, computed by wave 0 with one lane per
+        // corner column (reciprocal instead of division: the one-pixel margin
+        // absorbs the difference) and handed to the workgroup through LDS.
+        if (threadIdx.y == 0) {
+            const int c = threadIdx.x & 3;
+            const float cx = (c & 1) ? cx1 : cx0, cy = (c & 2) ? cy1 : cy0;
+            const float U = fmaf(g.c[3], fk0, fmaf(g.c[1], cx, fmaf(g.c[2], cy, g.c[0])));
+            const float Vv = fmaf(g.c[7], fk0, fmaf(g.c[5], cx, fmaf(g.c[6], cy, g.c[4])));
+            const float Ww = fmaf(g.c[11], fk0, fmaf(g.c[9], cx, fmaf(g.c[10], cy, g.c[8])));
+            const float r = __builtin_amdgcn_rcpf(Ww);
+            const float su = fmaf(U, r, 0.5f);
+            const float s0 = fmaf(Vv, r, 0.5f);
+            const float sl = fmaf((float)(ZC - 1), g.c[7] * r, s0);
+            float umin = su, umax = su, vmin = fminf(s0, sl), vmax = fmaxf(s0, sl);
+#pragma unroll
+            for (int m = 1; m <= 2; m <<= 1) {
+                umin = fminf(umin, __shfl_xor(umin, m));
+                umax = fmaxf(umax, __shfl_xor(umax, m));
+                vmin = fminf(vmin, __shfl_xor(vmin, m));
+                vmax = fmaxf(vmax, __shfl_xor(vmax, m));
+            }
+            if (threadIdx.x == 0) {
+                box[0] = (int)fminf(fmaxf(floorf(umin) - 1.0f, 0.0f), Wm1);
+                box[1] = (int)fminf(fmaxf(floorf(umax) + 1.0f, 0.0f), Wm1);
+                box[2] = (int)fminf(fmaxf(floorf(vmin) - 1.0f, 0.0f), Vm1);
+                box[3] = (int)fminf(fmaxf(floorf(vmax) + 1.0f, 0.0f), Vm1);
+            }
+        }
+        __syncthreads();  // the box is known and every read of the previous view's tile is done
+        const int u0 = box[0], u1 = box[1], v0 = box[2], v1 = box[3];
+        const int wdt = u1 - u0 + 1;
+        const int npx = wdt * (v1 - v0 + 1);
+        const bool staged = npx <= kLdsPixels;
+        if (staged) {
+            if (wdt <= 64) {
+                // rows of up to 64 pixels: lane group g of 2^lg lanes takes a
+                // row, no division
+                const int lg = wdt <= 16 ? 4 : (wdt <= 32 ? 5 : 6);
+                const int c = tid & ((1 << lg) - 1);
+                const int rstep = nthr >> lg;
+                if (c < wdt) {
+                    int r = tid >> lg;
+                    uint32_t goff = (uint32_t)(v0 + r) * rowb + (uint32_t)(u0 + c) * S::bytes;
+                    const uint32_t gstep = (uint32_t)rstep * rowb;
+                    int li = r * wdt + c;
+                    const int lstep = rstep * wdt;
+                    const int hgt = v1 - v0 + 1;
+                    for (; r + rstep < hgt; r += 2 * rstep) {
+                        const typename S::raw_t p0 = S::load(rs, goff);
+                        const typename S::raw_t p1 = S::load(rs, goff + gstep);
+                        lds[li] = S::abcd(p0);
+                        lds[li + lstep] = S::abcd(p1);
+                        goff += 2 * gstep;
+                        li += 2 * lstep;
+                    }
+                    if (r < hgt) lds[li] = S::abcd(S::load(rs, goff));
+                }
+            } else {
+                const float rinv = 1.0f / (float)wdt;
+                for (int b = tid; b < npx; b += nthr) {
+                    // exact row index: (b + 0.5) / wdt is at least 0.5 / wdt
+                    // away from an integer, far beyond float rounding here
+                    const int r = (int)(((float)b + 0.5f) * rinv);
+                    const int c = b - r * wdt;
+                    lds[b] = S::abcd(S::load(rs, (uint32_t)(v0 + r) * rowb + (uint32_t)(u0 + c) * S::bytes));
+                }
+            }
+        }
+        __syncthreads();  // the tile is in LDS
+
+        const float Uc = fmaf(g.c[3], fk0, fmaf(g.c[1], fi, fmaf(g.c[2], fj, g.c[0])));
+        const float Vc = fmaf(g.c[7], fk0, fmaf(g.c[5], fi, fmaf(g.c[6], fj, g.c[4])));
+        const float Wc = fmaf(g.c[11], fk0, fmaf(g.c[9], fi, fmaf(g.c[10], fj, g.c[8])));
+        const float rw = 1.0f / Wc;
+        const float su = fmaf(Uc, rw, 0.5f);
+        if (!(su >= 0.0f && su <= Wm1)) continue;
+        const int iu = (int)su;
+        const float fu = __builtin_amdgcn_fractf(su);
+        const float2 uwq = a.uwp[(size_t)(a.view0 + v) * a.W + iu];
+        const float cw = fmaf(fu, uwq.y - uwq.x, uwq.x);
+        const float sv0 = fmaf(Vc, rw, 0.5f);
+        const float dsv = g.c[7] * rw;
+        const float svl = fmaf((float)(ZC - 1), dsv, sv0);
+        const bool all_in = fminf(sv0, svl) >= 0.0f && fmaxf(sv0, svl) <= Vm1;
+        if (staged) {
+            // byte offset of sample row iv: iv * wdt * 16 + lb (mod 2^32)
+            const uint32_t lrow = (uint32_t)wdt * 16u;
+            const uint32_t lb = (uint32_t)(iu - u0) * 16u - (uint32_t)v0 * lrow;
+            const char* lbase = reinterpret_cast<const char*>(lds);
+            if (all_in) {
+                cfull += cw;
+#pragma unroll
+                for (int k = 0; k < ZC; ++k) {
+                    const float sv = fmaf((float)k, dsv, sv0);
+                    const float fv = __builtin_amdgcn_fractf(sv);
+                    const float4 c4 = *reinterpret_cast<const float4*>(lbase + __umul24((uint32_t)(int)sv, lrow) + lb);
+                    acc[k] += fmaf(fv, fmaf(fu, c4.w, c4.z), fmaf(fu, c4.y, c4.x));
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < ZC; ++k) {
+                    const float sv = fmaf((float)k, dsv, sv0);
+                    const float t = __builtin_amdgcn_fmed3f(sv, 0.0f, Vm1);
+                    const bool in = (t == sv);
+                    const float fv = __builtin_amdgcn_fractf(t);
+                    const float4 c4 = *reinterpret_cast<const float4*>(lbase + __umul24((uint32_t)(int)t, lrow) + lb);
+                    const float val = fmaf(fv, fmaf(fu, c4.w, c4.z), fmaf(fu, c4.y, c4.x));
+                    acc[k] += in ? val : 0.0f;
+                    cnt[k] += in ? cw : 0.0f;
+                }
+            }
+        } else {
+            // footprint larger than the LDS budget: gathers for this view
+            const uint32_t ub = (uint32_t)iu * S::bytes;
+#pragma unroll
+            for (int kb = 0; kb < ZC; kb += kZB) {
+                uint32_t off[kZB];
+                float fv[kZB];
+                bool in[kZB];
+#pragma unroll
+                for (int q = 0; q < kZB; ++q) {
+                    const float sv = fmaf((float)(kb + q), dsv, sv0);
+                    const float t = __builtin_amdgcn_fmed3f(sv, 0.0f, Vm1);
+                    in[q] = (t == sv);
+                    fv[q] = __builtin_amdgcn_fractf(t);
+                    off[q] = __umul24((uint32_t)(int)t, rowb) + ub;
+                }
+                typename S::raw_t p[kZB];
+#pragma unroll
+                for (int q = 0; q < kZB; ++q) p[q] = S::load(rs, off[q]);
+#pragma unroll
+                for (int q = 0; q < kZB; ++q) {
+                    const float top = S::lerp(p[q], 0, fu);
+                    const float bot = S::lerp(p[q], 1, fu);
+                    const float val = fmaf(fv[q], bot - top, top);
+                    acc[kb + q] += in[q] ? val : 0.0f;
+                    cnt[kb + q] += in[q] ? cw : 0.0f;
+                }
+            }
+        }
+    }
+
+    if (i >= a.X || j >= a.Y) return;
+#pragma unroll
+    for (int kk = 0; kk < ZC; ++kk) cnt[kk] += cfull;
+    const size_t plane = (size_t)a.X * a.Y;
+    size_t idx = ((size_t)k0 * a.Y + j) * a.X + i;
+#pragma unroll
+    for (int kk = 0; kk < ZC; ++kk) {
+        if (k0 + kk < a.Z) {
+            if (OUT == kOutNormalised) {
+                a.vol[idx] = acc[kk] / cnt[kk];
+            } else if (OUT == kOutAccumulate) {
+                a.acc[idx] += acc[kk];
+                a.cnt[idx] += cnt[kk];
+            } else {
+                a.vol[idx] = acc[kk];
+            }
+        }
+        idx += plane;
+    }
+}
+
 // Tile schedule.  Workgroups are dispatched round-robin over the 8 XCDs, so
 // workgroup b runs on XCD b % 8.  XCD x takes the contiguous tile range
 // [x T8, (x+1) T8): a z-slab, whose detector rows then stay in that XCD's L2.
@@ -437,7 +682,9 @@ __global__ void __launch_bounds__(256) bp_kernel(BPArgs a)
     const unsigned ntiles = (unsigned)a.ntx * a.nty * a.ntz;
     const unsigned t8 = gridDim.x >> 3;  // gridDim.x = 8 * ceil(ntiles / 8)
     const unsigned tile = (blockIdx.x & 7u) * t8 + (blockIdx.x >> 3);
-    if (tile < ntiles) bp_tile<MODE, ZC, QT, OUT>(a, tile);
+    if (tile >= ntiles) return;  // uniform per workgroup
+    if constexpr (MODE == kModeStaged) bp_tile_lds<ZC, QT, OUT>(a, tile);
+    else bp_tile<MODE, ZC, QT, OUT>(a, tile);
 }
 
 __global__ void bp_finalize_kernel(const float* __restrict__ acc, const float* __restrict__ cnt,
@@ -482,10 +729,10 @@ static bp_kernel_t pick_zc(int zc, int dtype)
 }
 
 template <int OUT>
-static bp_kernel_t pick_kernel(int mode, int zc, int dtype)
+static bp_kernel_t pick_kernel(int mode, int zc, int dtype, int fetch)
 {
     switch (mode) {
-    case 0: return pick_zc<0, OUT>(zc, dtype);
+    case 0: return fetch == JH_FETCH_LDS ? pick_zc<kModeStaged, OUT>(zc, dtype) : pick_zc<0, OUT>(zc, dtype);
     case 1: return pick_zc<1, OUT>(zc, dtype);
     default: return pick_zc<2, OUT>(zc, dtype);
     }
@@ -514,9 +761,9 @@ static void launch_bp(jh_bp_plan* p, const void* q, int dtype, int view0, int nv
     const size_t ntiles = (size_t)a.ntx * a.nty * a.ntz;
     require(ntiles < (1ull << 31), "backprojection grid too large");
     const size_t grid = (ntiles + 7) / 8 * 8;
-    bp_kernel_t k = out == kOutNormalised   ? pick_kernel<kOutNormalised>(p->mode, p->zc, dtype)
-                    : out == kOutAccumulate ? pick_kernel<kOutAccumulate>(p->mode, p->zc, dtype)
-                                            : pick_kernel<kOutRaw>(p->mode, p->zc, dtype);
+    bp_kernel_t k = out == kOutNormalised   ? pick_kernel<kOutNormalised>(p->mode, p->zc, dtype, p->fetch)
+                    : out == kOutAccumulate ? pick_kernel<kOutAccumulate>(p->mode, p->zc, dtype, p->fetch)
+                                            : pick_kernel<kOutRaw>(p->mode, p->zc, dtype, p->fetch);
     hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(64, p->ty), 0, stream, a);
     check_launch("bp_kernel");
 }
@@ -549,6 +796,7 @@ jh_status jh_bp_create(jh_bp_plan** plan, int device, const double* h_ptm, int n
             p->pxy = pxy; p->pz = pz;
             p->zc = 32;
             p->ty = 4;
+            p->fetch = JH_FETCH_GATHER;
             p->ptm.alloc(sizeof(double) * 12 * nviews);
             p->geo.alloc(sizeof(Geo) * nviews);
             p->uw.alloc(sizeof(float) * (size_t)nviews * W);
@@ -713,6 +961,23 @@ jh_status jh_bp_set_tile(jh_bp_plan* p, int zc, int ty)
     });
 }
 
+jh_status jh_bp_set_fetch(jh_bp_plan* p, int fetch)
+{
+    return guard([&] {
+        require(p != nullptr, "jh_bp_set_fetch: plan is NULL");
+        require(fetch == JH_FETCH_GATHER || fetch == JH_FETCH_LDS, "jh_bp_set_fetch: unknown fetch form");
+        p->fetch = fetch;
+    });
+}
+
+jh_status jh_bp_get_fetch(jh_bp_plan* p, int* fetch)
+{
+    return guard([&] {
+        require(p != nullptr && fetch != nullptr, "jh_bp_get_fetch: bad arguments");
+        *fetch = p->fetch;
+    });
+}
+
 jh_status jh_bp_autotune(jh_bp_plan* p, const void* d_q, int dtype, int nviews, float* d_vol, int reps,
                          jh_stream_t stream, float* h_ms)
 {
@@ -727,34 +992,41 @@ jh_status jh_bp_autotune(jh_bp_plan* p, const void* d_q, int dtype, int nviews, 
         JH_HIP(hipEventCreate(&e1));
         const int zcs[3] = {8, 16, 32}, tys[3] = {1, 2, 4};
         float best = 1e30f;
-        int bz = p->zc, bt = p->ty;
+        int bz = p->zc, bt = p->ty, bf = p->fetch;
         int n = 0;
         try {
-            for (int zi = 0; zi < 3; ++zi)
-                for (int ti = 0; ti < 3; ++ti) {
-                    p->zc = zcs[zi];
-                    p->ty = tys[ti];
-                    launch_bp(p, d_q, dtype, 0, nviews, d_vol, kOutNormalised, s);  // warm-up
-                    JH_HIP(hipEventRecord(e0, s));
-                    for (int r = 0; r < reps; ++r) launch_bp(p, d_q, dtype, 0, nviews, d_vol, kOutNormalised, s);
-                    JH_HIP(hipEventRecord(e1, s));
-                    JH_HIP(hipEventSynchronize(e1));
-                    float ms = 0.0f;
-                    JH_HIP(hipEventElapsedTime(&ms, e0, e1));
-                    ms /= (float)reps;
-                    if (h_ms) h_ms[n] = ms;
-                    ++n;
-                    if (ms < best) { best = ms; bz = p->zc; bt = p->ty; }
-                }
+            for (int f = 0; f < 2; ++f)
+                for (int zi = 0; zi < 3; ++zi)
+                    for (int ti = 0; ti < 3; ++ti, ++n) {
+                        if (f == JH_FETCH_LDS && p->mode != 0) {  // staged fetch: geometry class 0 only
+                            if (h_ms) h_ms[n] = -1.0f;
+                            continue;
+                        }
+                        p->fetch = f;
+                        p->zc = zcs[zi];
+                        p->ty = tys[ti];
+                        launch_bp(p, d_q, dtype, 0, nviews, d_vol, kOutNormalised, s);  // warm-up
+                        JH_HIP(hipEventRecord(e0, s));
+                        for (int r = 0; r < reps; ++r) launch_bp(p, d_q, dtype, 0, nviews, d_vol, kOutNormalised, s);
+                        JH_HIP(hipEventRecord(e1, s));
+                        JH_HIP(hipEventSynchronize(e1));
+                        float ms = 0.0f;
+                        JH_HIP(hipEventElapsedTime(&ms, e0, e1));
+                        ms /= (float)reps;
+                        if (h_ms) h_ms[n] = ms;
+                        if (ms < best) { best = ms; bz = p->zc; bt = p->ty; bf = p->fetch; }
+                    }
         } catch (...) {
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
+            p->zc = bz; p->ty = bt; p->fetch = bf;
             throw;
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         p->zc = bz;
         p->ty = bt;
+        p->fetch = bf;
     });
 }
 

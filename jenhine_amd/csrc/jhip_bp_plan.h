@@ -19,7 +19,8 @@ struct jh_bp_plan {
     int N, W, V, X, Y, Z;
     float pxy, pz;
     int mode;        // 0: u and w constant along z, 1: w constant along z, 2: general
-    int zc, ty;      // backprojector tile: z voxels per thread, columns along y
+    int zc, ty;      // backprojector tile: z voxels per thread, waves per workgroup
+    int fetch;       // JH_FETCH_GATHER or JH_FETCH_LDS (geometry class 0 only)
     jh::DevBuf ptm;  // N x 12 double
     jh::DevBuf geo;  // N x Geo
     jh::DevBuf uw;   // N x W float

@@ -75,9 +75,13 @@ class PreProcessingCore:
         return out
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            _lib.lib().jh_pre_destroy(self._h)
+        h = getattr(self, "_h", None)
+        if h:
             self._h = None
+            try:
+                _lib.lib().jh_pre_destroy(h)
+            except Exception:  # interpreter shutdown: the library may be gone already
+                pass
 
 
 class FilterCore:
@@ -121,9 +125,13 @@ class FilterCore:
         return out
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            _lib.lib().jh_filter_destroy(self._h)
+        h = getattr(self, "_h", None)
+        if h:
             self._h = None
+            try:
+                _lib.lib().jh_filter_destroy(h)
+            except Exception:  # interpreter shutdown: the library may be gone already
+                pass
 
 
 class BPCore:
@@ -151,15 +159,26 @@ class BPCore:
     def set_tile(self, zc=0, ty=0):
         call("jh_bp_set_tile", self._h, zc, ty)
 
+    FETCH = {"gather": 0, "lds": 1}
+
     def autotune(self, q, dtype="bf16", n_views=None, reps=1):
-        """Times every tile shape on views 0..n_views-1 of q and keeps the
-        fastest; returns {(zc, ty): ms}."""
+        """Times every fetch form and tile shape on views 0..n_views-1 of q and
+        keeps the fastest; returns {(fetch, zc, ty): ms} (forms that do not
+        apply to this geometry are left out)."""
         n = self.n_views if n_views is None else n_views
         vol = torch.empty((self.Z, self.Y, self.X), dtype=torch.float32, device=q.device)
-        ms = (ctypes.c_float * 9)()
+        ms = (ctypes.c_float * 18)()
         call("jh_bp_autotune", self._h, _ptr(q), DTYPES[dtype], n, _ptr(vol), reps, _stream(), ms)
-        keys = [(z, t) for z in (8, 16, 32) for t in (1, 2, 4)]
-        return dict(zip(keys, list(ms)))
+        keys = [(f, z, t) for f in ("gather", "lds") for z in (8, 16, 32) for t in (1, 2, 4)]
+        return {k: v for k, v in zip(keys, list(ms)) if v >= 0}
+
+    def set_fetch(self, fetch):
+        call("jh_bp_set_fetch", self._h, self.FETCH[fetch])
+
+    def fetch(self):
+        f = ctypes.c_int()
+        call("jh_bp_get_fetch", self._h, ctypes.byref(f))
+        return {v: k for k, v in self.FETCH.items()}[f.value]
 
     def tile(self):
         zc, ty = ctypes.c_int(), ctypes.c_int()
@@ -230,9 +249,13 @@ class BPCore:
         return out
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            _lib.lib().jh_bp_destroy(self._h)
+        h = getattr(self, "_h", None)
+        if h:
             self._h = None
+            try:
+                _lib.lib().jh_bp_destroy(h)
+            except Exception:  # interpreter shutdown: the library may be gone already
+                pass
 
 
 class FDK:
@@ -303,6 +326,10 @@ class BatchGraph:
         call("jh_graph_launch", self._h, _stream())
 
     def __del__(self):
-        if getattr(self, "_h", None):
-            _lib.lib().jh_graph_destroy(self._h)
+        h = getattr(self, "_h", None)
+        if h:
             self._h = None
+            try:
+                _lib.lib().jh_graph_destroy(h)
+            except Exception:  # interpreter shutdown: the library may be gone already
+                pass

@@ -299,9 +299,60 @@ def test_bp_autotune_keeps_results(gpu):
     q = bp.stage(filt, "bf16")
     before = bp.run(q, "bf16").cpu().numpy()
     table = bp.autotune(q, "bf16", reps=1)
-    assert len(table) == 9 and all(v > 0 for v in table.values())
+    assert len(table) == 18 and all(v > 0 for v in table.values())
     after = bp.run(q, "bf16").cpu().numpy()
-    print("autotuned tile", bp.tile(), "rel-RMSE vs default", rel_rmse(after, before))
+    print("autotuned", bp.fetch(), "tile", bp.tile(), "rel-RMSE vs default", rel_rmse(after, before))
     # the z tile origin enters the float sample coordinates, so another tile
     # shape rounds them differently; on white-noise input that shows at ~1e-5
     assert rel_rmse(after, before) <= 1e-4
+
+
+@pytest.mark.parametrize("dtype", ["f32", "bf16", "f16"])
+@pytest.mark.parametrize("tile", [(32, 4), (16, 2), (8, 1)])
+def test_bp_lds_fetch_matches_gather(gpu, dtype, tile):
+    """The LDS-staged fetch samples the same positions from the same values:
+    only the association of the bilinear terms differs."""
+    w = SMALL
+    ptm = w.ptms()
+    filt = torch.from_numpy(synth.normal(21, (w.n_views, w.n_v, w.n_u))).to(gpu)
+    from jenhine_amd.recon import BPCore
+    bp = BPCore(ptm, w.n_u, w.n_v, w.X, w.Y, w.Z, w.pxy, w.pz, device=gpu)
+    assert bp.geometry_mode() == 0
+    bp.set_tile(*tile)
+    q = bp.stage(filt, dtype)
+    res = {}
+    for f in ("gather", "lds"):
+        bp.set_fetch(f)
+        assert bp.fetch() == f
+        vol = bp.run(q, dtype).cpu().numpy()
+        bp.reset()
+        bp.accumulate(q[: q.numel() // 2], dtype, 0, w.n_views // 2)
+        bp.accumulate(q[q.numel() // 2:], dtype, w.n_views // 2, w.n_views - w.n_views // 2)
+        streamed = bp.finalize().cpu().numpy()
+        res[f] = (vol, streamed)
+    for k in range(2):
+        a, b = res["gather"][k], res["lds"][k]
+        np.testing.assert_array_equal(np.isnan(a), np.isnan(b))
+        e = rel_rmse(b, a)
+        print(dtype, tile, ["one-shot", "streamed"][k], "LDS vs gather rel-RMSE", e)
+        assert e <= 1e-6
+
+
+def test_bp_lds_fetch_falls_back_beyond_budget(gpu):
+    """A detector of very fine pixels makes every block's footprint exceed the
+    LDS budget: the staged kernel then gathers, with the same result."""
+    w = SMALL.scaled(du=0.06, n_u=160, n_v=160, name="fine-detector")
+    ptm = w.ptms()
+    filt = torch.from_numpy(synth.normal(22, (w.n_views, w.n_v, w.n_u))).to(gpu)
+    from jenhine_amd.recon import BPCore
+    bp = BPCore(ptm, w.n_u, w.n_v, w.X, w.Y, w.Z, w.pxy, w.pz, device=gpu)
+    q = bp.stage(filt, "bf16")
+    out = {}
+    for f in ("gather", "lds"):
+        bp.set_fetch(f)
+        out[f] = bp.run(q, "bf16").cpu().numpy()
+    assert np.isfinite(out["gather"]).sum() > 1000
+    # blocks whose footprint is clipped to a thin strip of the detector may
+    # still be staged, so the two agree to FP32 rounding, not to the bit
+    np.testing.assert_array_equal(np.isnan(out["gather"]), np.isnan(out["lds"]))
+    assert rel_rmse(out["lds"], out["gather"]) <= 1e-6
