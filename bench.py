@@ -66,6 +66,8 @@ def main():
                     help="storage form of the --kernel-only backprojection")
     ap.add_argument("--workload", choices=("dental512", "batch16"), default="dental512")
     ap.add_argument("--branches", type=int, default=4, help="parallel graph branches (batch16)")
+    ap.add_argument("--batch-storage", choices=("bf16", "f16", "f32"), default="bf16",
+                    help="projection storage of the batch16 scans")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -269,7 +271,8 @@ def run_batch(args, dev):
         e = dict(w=w, d=d, raw=torch.from_numpy(d["raw"].view(np.int16)).to(dev),
                  dark=torch.from_numpy(d["dark"].view(np.int16)).to(dev), gain=torch.from_numpy(d["gain"]).to(dev),
                  vol=torch.empty((w.Z, w.Y, w.X), dtype=torch.float32, device=dev))
-        e["fdk"] = FDK(w.scan_params(), w.recon_params(), d["ptm"], "mfma6", "bf16", dev)
+        e["fdk"] = FDK(w.scan_params(), w.recon_params(), d["ptm"], "fp32" if args.batch_storage == "f32" else "mfma6",
+                       args.batch_storage, dev)
         entries.append(e)
     units = sum(e["w"].gups_units for e in entries)
     log(f"16 scans generated in {time.time() - t0:.1f} s; {units / 1e9:.1f} G voxel-updates per launch")
@@ -331,7 +334,7 @@ def run_batch(args, dev):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16-storage/f32-accumulate",
+        "dtype": f"{args.batch_storage}-storage/f32-accumulate",
         "data": "synthetic (seeded ellipsoid phantoms, analytic line integrals, Poisson counts)",
         "config": {"workload": "batch16: 16 scans, volumes 128^3..400^3, 180..720 views, detectors 256^2..1024^2",
                    "scans": len(entries), "voxel_updates": units, "branches": args.branches,

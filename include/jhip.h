@@ -217,6 +217,20 @@ jh_status jh_bp_set_tile(jh_bp_plan* plan, int zc, int ty);
 jh_status jh_bp_set_fetch(jh_bp_plan* plan, int fetch);
 jh_status jh_bp_get_fetch(jh_bp_plan* plan, int* fetch);
 
+/* Fuses ZFilterAgent (FilterAgent/ZFilterAgent.cpp:49-71) into jh_bp_stage:
+ * with enable != 0 the filtered frames are smoothed along v, out row r =
+ * (0.5 in[r] + 0.25 in[max(r-1,0)]) + 0.25 in[min(r+1,V-1)], before the uW
+ * weight -- the reference's production chain Filter -> ZFilter -> BP
+ * (JEngine/JEngine.cpp:240-245).  jh_bp_pack is not affected. */
+jh_status jh_bp_set_zfilter(jh_bp_plan* plan, int enable);
+
+/* Fuses CTNumAgent (CTNumAgent/CTNumCore.cpp:22-36) into the normalised
+ * output of jh_bp_run and jh_bp_finalize: with enable != 0 each value x
+ * becomes x * (norm0 * 1000 / muWater) - 1000 and, above 300,
+ * min(8191, (x - 300) * norm1 + 300) (JEngine.cpp:297-305; MuWater defaults to
+ * 0.02, Config/ConfigHelper.cpp:70). */
+jh_status jh_bp_set_ctnum(jh_bp_plan* plan, int enable, float norm0, float norm1, float muWater);
+
 /* Autotuner: times the normalised backprojection of views 0..nviews-1 of
  * d_q (reps launches each, after one warm-up) for every fetch form x tile
  * shape zc in {8, 16, 32} x ty in {1, 2, 4}, and keeps the fastest in the
@@ -236,6 +250,14 @@ jh_status jh_bp_get_tile(jh_bp_plan* plan, int* zc, int* ty);
 jh_status jh_bp_geometry_mode(jh_bp_plan* plan, int* mode);
 
 jh_status jh_bp_destroy(jh_bp_plan* plan);
+
+/* Stand-alone ZFilterAgent::ProcessTask (FilterAgent/ZFilterAgent.cpp:49-71)
+ * over nviews frames of V rows x W (d_in != d_out), on the current device. */
+jh_status jh_zfilter_run(const float* d_in, float* d_out, int W, int V, int nviews, jh_stream_t stream);
+
+/* Stand-alone CTNumCore::Process (CTNumAgent/CTNumCore.cpp:22-36) in place
+ * over n values, on the current device. */
+jh_status jh_ctnum_run(float* d_inout, size_t n, float norm0, float norm1, float muWater, jh_stream_t stream);
 
 /* ---------------------------------------------------------------------- */
 /* Batches: many scans captured into one hipGraph                          */

@@ -26,7 +26,8 @@ EXPORTS = [
     "jh_bp_create", "jh_bp_uweights", "jh_bp_stage_bytes", "jh_bp_stage", "jh_bp_run", "jh_bp_reset",
     "jh_bp_accumulate", "jh_bp_finalize", "jh_bp_set_tile", "jh_bp_get_tile", "jh_bp_geometry_mode",
     "jh_bp_destroy", "jh_bp_pack", "jh_bp_backproject", "jh_fp_project", "jh_bp_autotune",
-    "jh_bp_set_fetch", "jh_bp_get_fetch",
+    "jh_bp_set_fetch", "jh_bp_get_fetch", "jh_bp_set_zfilter", "jh_bp_set_ctnum", "jh_zfilter_run",
+    "jh_ctnum_run",
     "jh_graph_create", "jh_graph_add_scan", "jh_graph_autotune", "jh_graph_scan_tuning", "jh_graph_instantiate",
     "jh_graph_launch", "jh_graph_destroy",
 ]
@@ -79,6 +80,10 @@ _SIGS = {
     "jh_fp_project": (c_int, [P, P, P, c_int, c_int, c_int, P]),
     "jh_bp_autotune": (c_int, [P, P, c_int, c_int, P, c_int, P, P]),
     "jh_bp_set_fetch": (c_int, [P, c_int]),
+    "jh_bp_set_zfilter": (c_int, [P, c_int]),
+    "jh_bp_set_ctnum": (c_int, [P, c_int, c_float, c_float, c_float]),
+    "jh_zfilter_run": (c_int, [P, P, c_int, c_int, c_int, P]),
+    "jh_ctnum_run": (c_int, [P, c_size_t, c_float, c_float, c_float, P]),
     "jh_bp_get_fetch": (c_int, [P, P]),
     "jh_graph_create": (c_int, [P, c_int]),
     "jh_graph_add_scan": (c_int, [P, P, P, P, P, P, P, c_int, c_int, P, P]),

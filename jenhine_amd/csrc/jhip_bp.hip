@@ -119,11 +119,54 @@ __device__ inline uint32_t pack_f16x2(float a, float b)
     return __builtin_bit_cast(uint32_t, h);
 }
 
+// ZFilterAgent's 1-2-1 smoothing along v (FilterAgent/ZFilterAgent.cpp:49-71)
+// of column u at row r of one frame, in the reference's operation order:
+// (0.5 in[r] + 0.25 in[max(r-1, 0)]) + 0.25 in[min(r+1, V-1)].
+__device__ __forceinline__ float zfilter_at(const float* __restrict__ frame, int W, int V, int r, int u)
+{
+#pragma clang fp contract(off)
+    const float a = frame[(size_t)r * W + u];
+    const float b = frame[(size_t)(r > 0 ? r - 1 : 0) * W + u];
+    const float c = frame[(size_t)(r + 1 < V ? r + 1 : V - 1) * W + u];
+    const float t = a * 0.5f + b * 0.25f;
+    return t + c * 0.25f;
+}
+
+// CTNumCore::Process (CTNumAgent/CTNumCore.cpp:22-36): x * scale - 1000 with
+// scale = norm0 * 1000 / muWater formed on the host in float as the reference
+// does; above 300 the value becomes min(8191, (x - 300) norm1 + 300).
+__device__ __forceinline__ float ct_number(float x, float scale, float norm1)
+{
+#pragma clang fp contract(off)
+    float y = x * scale;
+    y = y - 1000.0f;
+    if (y > 300.0f) {
+        float z = (y - 300.0f) * norm1;
+        z = z + 300.0f;
+        y = fminf(8191.0f, z);
+    }
+    return y;
+}
+
+__global__ void __launch_bounds__(256) zfilter_kernel(const float* __restrict__ in, float* __restrict__ out, int W,
+                                                      int V, size_t total)
+{
+    const size_t frame = (size_t)W * V;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const size_t f = t / frame;
+        const int pix = (int)(t - f * frame);
+        const int r = pix / W;
+        out[t] = zfilter_at(in + f * frame, W, V, r, pix - r * W);
+    }
+}
+
 template <int QT>
 __global__ void __launch_bounds__(256) bp_stage_kernel(const float* __restrict__ filt,
                                                        const float* __restrict__ uw, int W, int V,
-                                                       int nviews, void* __restrict__ out)
-// uw == NULL packs the frames without weighting.
+                                                       int nviews, int zf, void* __restrict__ out)
+// uw == NULL packs the frames without weighting; zf != 0 applies the v
+// smoothing of ZFilterAgent to the frames first (before the uW weight, as in
+// the reference chain Filter -> ZFilter -> BPCore::InitView).
 {
     const size_t frame = (size_t)W * V;
     const size_t total = frame * nviews;
@@ -133,15 +176,16 @@ __global__ void __launch_bounds__(256) bp_stage_kernel(const float* __restrict__
         const int pix = (int)(t - view * frame);
         const int row = pix / W;
         const int u = pix - row * W;
-        const float* f = filt + view * frame + (size_t)row * W;
+        const float* fr = filt + view * frame;
         const float* w = uw ? uw + view * W : nullptr;
         const bool right = u + 1 < W, below = row + 1 < V;
         const float wa = w ? w[u] : 1.0f;
         const float wb = w && right ? w[u + 1] : 1.0f;
-        const float a0 = f[u] * wa;
-        const float b0 = right ? f[u + 1] * wb : 0.0f;
-        const float a1 = below ? f[W + u] * wa : 0.0f;
-        const float b1 = below && right ? f[W + u + 1] * wb : 0.0f;
+        auto at = [&](int r, int c) { return zf ? zfilter_at(fr, W, V, r, c) : fr[(size_t)r * W + c]; };
+        const float a0 = at(row, u) * wa;
+        const float b0 = right ? at(row, u + 1) * wb : 0.0f;
+        const float a1 = below ? at(row + 1, u) * wa : 0.0f;
+        const float b1 = below && right ? at(row + 1, u + 1) * wb : 0.0f;
         if (QT == JH_F32) {
             reinterpret_cast<float4*>(out)[t] = make_float4(a0, b0 - a0, a1, b1 - a1);
         } else if (QT == JH_BF16) {
@@ -264,6 +308,8 @@ struct BPArgs {
     int view0, nviews;
     int ntx, nty, ntz;
     uint32_t frame_bytes;
+    int ct;              // apply the CT-number map to normalised output
+    float ct_scale, ct_norm1;
 };
 
 constexpr int kZB = 8;  // samples whose loads are issued together
@@ -438,7 +484,8 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
     for (int kk = 0; kk < ZC; ++kk) {
         if (k0 + kk < a.Z) {
             if (OUT == kOutNormalised) {
-                a.vol[idx] = acc[kk] / cnt[kk];
+                const float v = acc[kk] / cnt[kk];
+                a.vol[idx] = a.ct ? ct_number(v, a.ct_scale, a.ct_norm1) : v;
             } else if (OUT == kOutAccumulate) {
                 a.acc[idx] += acc[kk];
                 a.cnt[idx] += cnt[kk];
@@ -658,7 +705,8 @@ __device__ __forceinline__ void bp_tile_lds(const BPArgs& a, unsigned tile)
     for (int kk = 0; kk < ZC; ++kk) {
         if (k0 + kk < a.Z) {
             if (OUT == kOutNormalised) {
-                a.vol[idx] = acc[kk] / cnt[kk];
+                const float v = acc[kk] / cnt[kk];
+                a.vol[idx] = a.ct ? ct_number(v, a.ct_scale, a.ct_norm1) : v;
             } else if (OUT == kOutAccumulate) {
                 a.acc[idx] += acc[kk];
                 a.cnt[idx] += cnt[kk];
@@ -687,11 +735,19 @@ __global__ void __launch_bounds__(256) bp_kernel(BPArgs a)
     else bp_tile<MODE, ZC, QT, OUT>(a, tile);
 }
 
-__global__ void bp_finalize_kernel(const float* __restrict__ acc, const float* __restrict__ cnt,
-                                   float* __restrict__ vol, size_t n)
+__global__ void ctnum_kernel(float* __restrict__ x, size_t n, float scale, float norm1)
 {
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
-        vol[t] = acc[t] / cnt[t];
+        x[t] = ct_number(x[t], scale, norm1);
+}
+
+__global__ void bp_finalize_kernel(const float* __restrict__ acc, const float* __restrict__ cnt,
+                                   float* __restrict__ vol, size_t n, int ct, float ct_scale, float ct_norm1)
+{
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+        const float v = acc[t] / cnt[t];
+        vol[t] = ct ? ct_number(v, ct_scale, ct_norm1) : v;
+    }
 }
 
 __global__ void fill_zero_kernel(float* __restrict__ p, size_t n)
@@ -750,6 +806,9 @@ static void launch_bp(jh_bp_plan* p, const void* q, int dtype, int view0, int nv
     a.cnt = p->cnt.as<float>();
     a.W = p->W; a.V = p->V; a.X = p->X; a.Y = p->Y; a.Z = p->Z;
     a.view0 = view0; a.nviews = nviews;
+    a.ct = p->ct;
+    a.ct_scale = p->ct_scale;
+    a.ct_norm1 = p->ct_norm1;
     // workgroup extent: 8 x 8 voxel columns per wave (see bp_tile)
     const int ex = p->ty >= 2 ? 16 : 8, ey = p->ty >= 4 ? 16 : 8;
     a.ntx = (p->X + ex - 1) / ex;
@@ -797,6 +856,10 @@ jh_status jh_bp_create(jh_bp_plan** plan, int device, const double* h_ptm, int n
             p->zc = 32;
             p->ty = 4;
             p->fetch = JH_FETCH_GATHER;
+            p->zfilter = 0;
+            p->ct = 0;
+            p->ct_scale = 1.0f;
+            p->ct_norm1 = 1.0f;
             p->ptm.alloc(sizeof(double) * 12 * nviews);
             p->geo.alloc(sizeof(Geo) * nviews);
             p->uw.alloc(sizeof(float) * (size_t)nviews * W);
@@ -858,9 +921,10 @@ static void stage(jh_bp_plan* p, const float* d_filtered, int view0, int nviews,
         const float* uw = weighted ? p->uw.as<float>() + (size_t)view0 * p->W : nullptr;
         const unsigned grid = grid_for(total, 256);
         hipStream_t s = as_stream(stream);
-        if (dtype == JH_F32) bp_stage_kernel<JH_F32><<<grid, 256, 0, s>>>(d_filtered, uw, p->W, p->V, nviews, d_q);
-        else if (dtype == JH_BF16) bp_stage_kernel<JH_BF16><<<grid, 256, 0, s>>>(d_filtered, uw, p->W, p->V, nviews, d_q);
-        else bp_stage_kernel<JH_F16><<<grid, 256, 0, s>>>(d_filtered, uw, p->W, p->V, nviews, d_q);
+        const int zf = weighted ? p->zfilter : 0;
+        if (dtype == JH_F32) bp_stage_kernel<JH_F32><<<grid, 256, 0, s>>>(d_filtered, uw, p->W, p->V, nviews, zf, d_q);
+        else if (dtype == JH_BF16) bp_stage_kernel<JH_BF16><<<grid, 256, 0, s>>>(d_filtered, uw, p->W, p->V, nviews, zf, d_q);
+        else bp_stage_kernel<JH_F16><<<grid, 256, 0, s>>>(d_filtered, uw, p->W, p->V, nviews, zf, d_q);
         check_launch("bp_stage_kernel");
     }
 }
@@ -945,7 +1009,8 @@ jh_status jh_bp_finalize(jh_bp_plan* p, float* d_vol, int z0, int nz, jh_stream_
         const size_t plane = (size_t)p->X * p->Y;
         const size_t n = plane * nz;
         bp_finalize_kernel<<<grid_for(n, 256), 256, 0, as_stream(stream)>>>(
-            p->acc.as<float>() + plane * z0, p->cnt.as<float>() + plane * z0, d_vol, n);
+            p->acc.as<float>() + plane * z0, p->cnt.as<float>() + plane * z0, d_vol, n, p->ct, p->ct_scale,
+            p->ct_norm1);
         check_launch("bp_finalize_kernel");
     });
 }
@@ -975,6 +1040,50 @@ jh_status jh_bp_get_fetch(jh_bp_plan* p, int* fetch)
     return guard([&] {
         require(p != nullptr && fetch != nullptr, "jh_bp_get_fetch: bad arguments");
         *fetch = p->fetch;
+    });
+}
+
+jh_status jh_bp_set_zfilter(jh_bp_plan* p, int enable)
+{
+    return guard([&] {
+        require(p != nullptr, "jh_bp_set_zfilter: plan is NULL");
+        p->zfilter = enable ? 1 : 0;
+    });
+}
+
+jh_status jh_bp_set_ctnum(jh_bp_plan* p, int enable, float norm0, float norm1, float muWater)
+{
+    return guard([&] {
+        require(p != nullptr, "jh_bp_set_ctnum: plan is NULL");
+        require(!enable || (muWater != 0.0f && std::isfinite(norm0) && std::isfinite(norm1) && std::isfinite(muWater)),
+                "jh_bp_set_ctnum: bad CT-number parameters");
+        p->ct = enable ? 1 : 0;
+        p->ct_scale = norm0 * 1000.f / muWater;  // in float, as CTNumCore.cpp:24 forms it
+        p->ct_norm1 = norm1;
+    });
+}
+
+jh_status jh_zfilter_run(const float* d_in, float* d_out, int W, int V, int nviews, jh_stream_t stream)
+{
+    return guard([&] {
+        require(d_in != nullptr && d_out != nullptr && d_in != d_out, "jh_zfilter_run: bad buffers");
+        require(W >= 1 && V >= 1 && nviews >= 0, "jh_zfilter_run: bad frame size");
+        const size_t total = (size_t)W * V * nviews;
+        if (total == 0) return;
+        zfilter_kernel<<<grid_for(total, 256), 256, 0, as_stream(stream)>>>(d_in, d_out, W, V, total);
+        check_launch("zfilter_kernel");
+    });
+}
+
+jh_status jh_ctnum_run(float* d_inout, size_t n, float norm0, float norm1, float muWater, jh_stream_t stream)
+{
+    return guard([&] {
+        require(d_inout != nullptr || n == 0, "jh_ctnum_run: buffer is NULL");
+        require(muWater != 0.0f, "jh_ctnum_run: muWater is 0");
+        if (n == 0) return;
+        const float scale = norm0 * 1000.f / muWater;  // in float, as CTNumCore.cpp:24
+        ctnum_kernel<<<grid_for(n, 256), 256, 0, as_stream(stream)>>>(d_inout, n, scale, norm1);
+        check_launch("ctnum_kernel");
     });
 }
 

@@ -176,6 +176,15 @@ int main(int argc, char** argv)
         recon.FilterMode = (int)a.I("filter", "0");
         recon.StorageType = (int)a.I("storage", "0");
         recon.ViewsPerChunk = (size_t)a.I("chunk", "8");
+        recon.ZFilter = a.I("zfilter", "0") != 0;
+        if (a.kv.count("ct")) {  // ct=norm0|norm1|muWater
+            const std::vector<float> ct = ParseList(a.S("ct"));
+            if (ct.size() != 3) ThrowException("ct needs norm0|norm1|muWater");
+            recon.CTNumber = true;
+            recon.CTNumNorm0 = ct[0];
+            recon.CTNumNorm1 = ct[1];
+            recon.MuWater = ct[2];
+        }
         const int device = (int)a.I("device", "0");
         const size_t frame = scan.NumDetsU * scan.NumDetsV;
         std::vector<uint16_t> raw = ReadFile<uint16_t>(a.S("raw"), frame * scan.NumViews);

@@ -52,6 +52,14 @@ struct ReconParams {
     int FilterMode = 0;      // JH_FILTER_FP32, JH_FILTER_MFMA_BF16X3, JH_FILTER_MFMA_BF16X6 or JH_FILTER_FP32_VALU
     int StorageType = 0;     // JH_F32, JH_BF16 or JH_F16
     size_t ViewsPerChunk = 8;
+    // the production chain's two stages around the backprojection, fused
+    // into it (ZFilterAgent between filter and upload, CTNumAgent after the
+    // download: JEngine.cpp:240-245,297-305)
+    bool ZFilter = false;
+    bool CTNumber = false;
+    float CTNumNorm0 = 1.0f;  // ReconParams::CTNumNorm0 (ConfigHelper.cpp:91)
+    float CTNumNorm1 = 1.0f;  // ReconParams::CTNumNorm1 (ConfigHelper.cpp:92)
+    float MuWater = 0.02f;    // ReconParams::MuWater (ConfigHelper.cpp:70)
 };
 
 }  // namespace jh

@@ -293,6 +293,9 @@ jh_bp_plan* BPHIPAgent::PlanFor(size_t p)
     CheckJh(jh_bp_create(&plan_, device_, ptm.data(), (int)nv, (int)scan_.NumUsedDetsU, (int)scan_.NumUsedDetsV,
                          (int)recon_.NumPixelsX, (int)recon_.NumPixelsY, (int)nz, recon_.PitchXY, recon_.PitchZ),
             "jh_bp_create");
+    if (recon_.ZFilter) CheckJh(jh_bp_set_zfilter(plan_, 1), "jh_bp_set_zfilter");
+    if (recon_.CTNumber)
+        CheckJh(jh_bp_set_ctnum(plan_, 1, recon_.CTNumNorm0, recon_.CTNumNorm1, recon_.MuWater), "jh_bp_set_ctnum");
     // the accumulators are allocated here so that a shortage shows up now
     CheckJh(jh_bp_reset(plan_, stream_.Get()), "jh_bp_reset");
     stream_.Sync();

@@ -172,6 +172,15 @@ class BPCore:
         keys = [(f, z, t) for f in ("gather", "lds") for z in (8, 16, 32) for t in (1, 2, 4)]
         return {k: v for k, v in zip(keys, list(ms)) if v >= 0}
 
+    def set_zfilter(self, enable=True):
+        """Fuse ZFilterAgent's v smoothing into stage()."""
+        call("jh_bp_set_zfilter", self._h, int(bool(enable)))
+
+    def set_ctnum(self, enable=True, norm0=1.0, norm1=1.0, mu_water=0.02):
+        """Fuse CTNumAgent's map into normalised outputs (run, finalize)."""
+        call("jh_bp_set_ctnum", self._h, int(bool(enable)), ctypes.c_float(norm0), ctypes.c_float(norm1),
+             ctypes.c_float(mu_water))
+
     def set_fetch(self, fetch):
         call("jh_bp_set_fetch", self._h, self.FETCH[fetch])
 
@@ -258,10 +267,37 @@ class BPCore:
                 pass
 
 
-class FDK:
-    """PreProcessingCore -> FilterCore -> BPCore on one device."""
+def zfilter(frames, out=None):
+    """ZFilterAgent (FilterAgent/ZFilterAgent.cpp:49-71): 1-2-1 smoothing
+    along v with clamped edges, frames (N, V, W) f32 on the device."""
+    _require_cuda(frames, "frames", torch.float32)
+    if frames.dim() != 3:
+        raise ValueError("frames must be (N, V, W)")
+    n, V, W = frames.shape
+    if out is None:
+        out = torch.empty_like(frames)
+    _require_cuda(out, "out", torch.float32)
+    with torch.cuda.device(frames.device):
+        call("jh_zfilter_run", _ptr(frames), _ptr(out), W, V, n, _stream())
+    return out
 
-    def __init__(self, scan, recon, ptm, filter_mode="fp32", storage="f32", device="cuda:0"):
+
+def ctnum(x, norm0=1.0, norm1=1.0, mu_water=0.02):
+    """CTNumCore (CTNumAgent/CTNumCore.cpp:22-36), in place on a device tensor."""
+    _require_cuda(x, "x", torch.float32)
+    with torch.cuda.device(x.device):
+        call("jh_ctnum_run", _ptr(x), ctypes.c_size_t(x.numel()), ctypes.c_float(norm0), ctypes.c_float(norm1),
+             ctypes.c_float(mu_water), _stream())
+    return x
+
+
+class FDK:
+    """PreProcessingCore -> FilterCore [-> ZFilterAgent] -> BPCore [->
+    CTNumAgent] on one device.  zfilter and ct (norm0, norm1, mu_water) are
+    fused into the backprojector's staging and normalisation."""
+
+    def __init__(self, scan, recon, ptm, filter_mode="fp32", storage="f32", device="cuda:0", zfilter=False,
+                 ct=None):
         self.scan, self.recon, self.storage = scan, recon, storage
         W, V = scan.NumUsedDetsU, scan.NumUsedDetsV
         self.pre = PreProcessingCore(scan.NumDetsU, scan.NumDetsV,
@@ -273,6 +309,10 @@ class FDK:
                                  recon.FilterWindow, filter_mode, device)
         self.bp = BPCore(ptm, W, V, recon.NumPixelsX, recon.NumPixelsY, recon.NumPixelsZ, recon.PitchXY,
                          recon.PitchZ, device)
+        if zfilter:
+            self.bp.set_zfilter(True)
+        if ct is not None:
+            self.bp.set_ctnum(True, *ct)
 
     def __call__(self, raw, dark=None, gain=None, return_stages=False):
         p = self.pre.process(raw, dark, gain)

@@ -665,6 +665,49 @@ void FN(jo_fp)(const R* x, const R* ptm, int N, int W, int V, int X, int Y, int 
     free(uw); free(ax); free(ay); free(az);
 }
 
+/* ZFilterAgent::ProcessTask (FilterAgent/ZFilterAgent.cpp:49-71): per frame of
+ * V rows x W, out row i = (0.5 in[i] + 0.25 in[max(i-1, 0)]) + 0.25
+ * in[min(i+1, V-1)], in that operation order (Mul, then Add of a scaled
+ * neighbour, then Add of the other). */
+void FN(jo_zfilter)(const R* in, int nframes, int W, int V, R* out)
+{
+    for (int f = 0; f < nframes; ++f) {
+        const R* src = in + (size_t)f * W * V;
+        R* dst = out + (size_t)f * W * V;
+        for (int i = 0; i < V; ++i) {
+            const R* a = src + (size_t)i * W;
+            const R* b = src + (size_t)(i > 0 ? i - 1 : 0) * W;
+            const R* c = src + (size_t)(i + 1 < V ? i + 1 : V - 1) * W;
+            R* o = dst + (size_t)i * W;
+            for (int u = 0; u < W; ++u) {
+                R t = a[u] * (R)0.5;
+                const R bb = b[u] * (R)0.25;
+                t = t + bb;
+                const R cc = c[u] * (R)0.25;
+                o[u] = t + cc;
+            }
+        }
+    }
+}
+
+/* CTNumCore::Process (CTNumAgent/CTNumCore.cpp:22-36): x * (norm0 * 1000 /
+ * muWater) - 1000, then values above 300 become min(8191, (x - 300) norm1 +
+ * 300).  The factor is formed in R as the reference forms it in float. */
+void FN(jo_ctnum)(R* inout, size_t n, R norm0, R norm1, R muWater)
+{
+    const R s = norm0 * (R)1000 / muWater;
+    for (size_t t = 0; t < n; ++t) {
+        R x = inout[t] * s;
+        x = x - (R)1000;
+        if (x > (R)300) {
+            R y = (x - (R)300) * norm1;
+            y = y + (R)300;
+            x = y < (R)8191 ? y : (R)8191;
+        }
+        inout[t] = x;
+    }
+}
+
 #undef FN
 #undef CAT
 #undef CAT_

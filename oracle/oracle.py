@@ -182,11 +182,15 @@ def backproject(proj, ptm, vol_shape, pitch_xy, pitch_z, z0=0, nz=None, fp64=Fal
 
 
 def fdk(raw, ptm, scan, recon, fp64=False, nthreads=8, z0=0, nz=None, dark=None, gain=None,
-        return_stages=False):
-    """Full oracle chain PreProcessingCore -> FilterCore -> BPCore.
+        return_stages=False, zfilter_v=False, ct=None):
+    """Full oracle chain PreProcessingCore -> FilterCore [-> ZFilterAgent] ->
+    BPCore [-> CTNumCore].
 
     scan: dict with W, V, bright_field, bh, hsr, dso, borders (optional).
-    recon: dict with X, Y, Z, pxy, pz, cut_start, cut_end, window."""
+    recon: dict with X, Y, Z, pxy, pz, cut_start, cut_end, window.
+    zfilter_v: apply the 1-2-1 v smoothing between filter and backprojection
+    (JEngine.cpp:240-245); ct: (norm0, norm1, mu_water) for the CT-number map
+    (JEngine.cpp:297-305)."""
     p = preprocess(raw, borders=scan.get("borders", (0, 0, 0, 0)), bright_field=scan["bright_field"],
                    bh=scan.get("bh", (1.0, 0.0, 0.0)), dark=dark, gain=gain, fp64=fp64,
                    nthreads=nthreads)
@@ -198,11 +202,34 @@ def fdk(raw, ptm, scan, recon, fp64=False, nthreads=8, z0=0, nz=None, dark=None,
     ptm = np.asarray(ptm).reshape(N, 12)
     pws = np.stack([prefilter_weight(ptm[s], W, V, scan["dso"], fp64=fp64) for s in src])
     f = filter_rows(p, H, pws, blk, fp64=fp64, nthreads=nthreads)
+    if zfilter_v:
+        f = zfilter(f, fp64=fp64)
     vol = backproject(f, ptm, (recon["Z"], recon["Y"], recon["X"]), recon["pxy"], recon["pz"],
                       z0=z0, nz=nz, fp64=fp64, nthreads=nthreads)
+    if ct is not None:
+        vol = ctnum(vol, *ct, fp64=fp64)
     if return_stages:
         return vol, dict(pre=p, H=H, pw=pws, pw_index=blk, filt=f)
     return vol
+
+
+def zfilter(frames, fp64=False):
+    """ZFilterAgent: 1-2-1 smoothing along v with clamped edges, (N, V, W)."""
+    x = np.ascontiguousarray(frames, _dt(fp64))
+    if x.ndim == 2:
+        x = x[None]
+    N, V, W = x.shape
+    out = np.empty_like(x)
+    getattr(lib(), "jo_zfilter" + _sfx(fp64))(_p(x), N, W, V, _p(out))
+    return out
+
+
+def ctnum(vol, norm0, norm1, mu_water, fp64=False):
+    """CTNumCore: attenuation -> CT number (returns a new array)."""
+    x = np.array(vol, _dt(fp64), copy=True, order="C")
+    R = _real(fp64)
+    getattr(lib(), "jo_ctnum" + _sfx(fp64))(_p(x), ctypes.c_size_t(x.size), R(norm0), R(norm1), R(mu_water))
+    return x
 
 
 def bp_acc(proj, ptm, vol_shape, pitch_xy, pitch_z, weighted=True, fp64=True):

@@ -356,3 +356,81 @@ def test_bp_lds_fetch_falls_back_beyond_budget(gpu):
     # still be staged, so the two agree to FP32 rounding, not to the bit
     np.testing.assert_array_equal(np.isnan(out["gather"]), np.isnan(out["lds"]))
     assert rel_rmse(out["lds"], out["gather"]) <= 1e-6
+
+
+# --------------------------------------------------------------------------
+# ZFilterAgent and CTNumAgent (fused into staging and normalisation)
+# --------------------------------------------------------------------------
+
+def test_zfilter_kernel_matches_oracle_bit_exact(gpu):
+    from jenhine_amd.recon import zfilter
+    x = synth.normal(41, (3, 40, 57)) * 3.0
+    got = zfilter(torch.from_numpy(x).to(gpu)).cpu().numpy()
+    np.testing.assert_array_equal(got, o.zfilter(x))
+    one_row = synth.normal(42, (2, 1, 33))  # V = 1: every neighbour is the row itself
+    np.testing.assert_array_equal(zfilter(torch.from_numpy(one_row).to(gpu)).cpu().numpy(), o.zfilter(one_row))
+
+
+def test_ctnum_kernel_matches_oracle_bit_exact(gpu):
+    from jenhine_amd.recon import ctnum
+    x = synth.uniform(43, (10000,), -0.01, 0.2)
+    x[:3] = [np.nan, 0.0, 1.0]
+    for norm0, norm1, mu in ((1.0, 1.0, 0.02), (1.07, 0.62, 0.0193)):
+        got = ctnum(torch.from_numpy(x.copy()).to(gpu), norm0, norm1, mu).cpu().numpy()
+        np.testing.assert_array_equal(got, o.ctnum(x, norm0, norm1, mu))
+
+
+@pytest.mark.parametrize("dtype", ["f32", "bf16", "f16"])
+def test_stage_with_fused_zfilter_equals_zfilter_then_stage(gpu, dtype):
+    w = SMALL
+    from jenhine_amd.recon import BPCore, zfilter
+    filt = torch.from_numpy(synth.normal(44, (w.n_views, w.n_v, w.n_u))).to(gpu)
+    bp = BPCore(w.ptms(), w.n_u, w.n_v, w.X, w.Y, w.Z, w.pxy, w.pz, device=gpu)
+    ref = bp.stage(zfilter(filt), dtype).cpu().numpy()
+    bp.set_zfilter(True)
+    got = bp.stage(filt, dtype).cpu().numpy()
+    np.testing.assert_array_equal(got, ref)
+
+
+def test_bp_fused_ctnum_equals_ctnum_of_volume(gpu):
+    w = SMALL
+    from jenhine_amd.recon import BPCore, ctnum
+    filt = torch.from_numpy(np.abs(synth.normal(45, (w.n_views, w.n_v, w.n_u))) * 0.05).to(gpu)
+    bp = BPCore(w.ptms(), w.n_u, w.n_v, w.X, w.Y, w.Z, w.pxy, w.pz, device=gpu)
+    q = bp.stage(filt, "bf16")
+    ref = ctnum(bp.run(q, "bf16"), 1.07, 0.62, 0.02).cpu().numpy()
+    bp.reset()
+    bp.accumulate(q, "bf16", 0, w.n_views)
+    ref_s = ctnum(bp.finalize(), 1.07, 0.62, 0.02).cpu().numpy()
+    bp.set_ctnum(True, 1.07, 0.62, 0.02)
+    np.testing.assert_array_equal(bp.run(q, "bf16").cpu().numpy(), ref)
+    np.testing.assert_array_equal(bp.finalize().cpu().numpy(), ref_s)
+    assert (ref[np.isfinite(ref)] > 300).any()  # the compressed branch is exercised
+
+
+def test_fdk_production_chain_with_zfilter_and_ct_vs_oracle(gpu):
+    """Filter -> ZFilter -> BP -> CTNum (JEngine.cpp:240-305) against the
+    oracle chain with the same two stages."""
+    from jenhine_amd.recon import FDK
+    w = SMALL
+    d = small_data_cache()
+    sc, rc = oracle_params(w)
+    ct = (1.07, 0.62, 0.02)
+    fdk = FDK(w.scan_params(), w.recon_params(), d["ptm"], "fp32", "f32", device=gpu, zfilter=True, ct=ct)
+    vol = fdk(_u16(d["raw"], gpu), _u16(d["dark"], gpu), torch.from_numpy(d["gain"]).to(gpu)).cpu().numpy()
+    v32 = o.fdk(d["raw"], d["ptm"], sc, rc, fp64=False, dark=d["dark"], gain=d["gain"], zfilter_v=True, ct=ct)
+    v64 = o.fdk(d["raw"], d["ptm"], sc, rc, fp64=True, dark=d["dark"], gain=d["gain"], zfilter_v=True, ct=ct)
+    plain = o.fdk(d["raw"], d["ptm"], sc, rc, fp64=True, dark=d["dark"], gain=d["gain"], ct=ct)
+    e32, e64 = rel_rmse(vol, v32), rel_rmse(vol, v64)
+    print("zfilter+ct chain: rel-RMSE vs FP32-ref", e32, "vs FP64", e64, "max-abs HU vs FP64", max_abs(vol, v64))
+    assert e32 <= 1e-4 and e64 <= 1e-4
+    assert rel_rmse(v64, plain) > 1e-3  # the z filter changes the result
+
+
+_SMALL_DATA = {}
+
+
+def small_data_cache():
+    if not _SMALL_DATA:
+        _SMALL_DATA.update(SMALL.data())
+    return _SMALL_DATA

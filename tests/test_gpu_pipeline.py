@@ -21,7 +21,7 @@ APP = os.path.join(ROOT, "jenhine_amd", "host", "build", "jh_fdk_pipeline")
 W = configs.CBCT128.scaled(n_views=24, X=64, Y=64, Z=48, pxy=0.5, pz=0.5)
 
 
-def _run(tmp_path, d, storage=0, filt=0, chunk=5, env=None, parts=1):
+def _run(tmp_path, d, storage=0, filt=0, chunk=5, env=None, parts=1, extra=()):
     paths = {}
     for k, dt in (("raw", np.uint16), ("ptm", np.float64), ("dark", np.uint16), ("gain", np.float32)):
         paths[k] = str(tmp_path / f"{k}.bin")
@@ -34,7 +34,7 @@ def _run(tmp_path, d, storage=0, filt=0, chunk=5, env=None, parts=1):
             f"pitch_xy={W.pxy}", f"pitch_z={W.pz}", f"cut_start={rec.FilterCutOffStart}",
             f"cut_end={rec.FilterCutOffEnd}", f"window={rec.FilterWindow}", f"filter={filt}",
             f"storage={storage}", f"chunk={chunk}", f"parts={parts}", f"out={out}"] + \
-           [f"{k}={v}" for k, v in paths.items()]
+           [f"{k}={v}" for k, v in paths.items()] + list(extra)
     e = dict(os.environ)
     e.update(env or {})
     r = subprocess.run(args, capture_output=True, text=True, timeout=120, env=e)
@@ -86,3 +86,14 @@ def test_device_shortage_retries_with_more_parts(gpu, data, tmp_path):
     # a slab's matrices carry its z offset, so its float coordinates round
     # differently from the one-part volume's (observed ~4e-6)
     assert rel_rmse(vol, full) <= 2e-5
+
+
+def test_agent_graph_production_chain_zfilter_ct(gpu, data, tmp_path):
+    """ZFilterAgent and CTNumAgent fused into the BP agent (JEngine.cpp:240-305)."""
+    ct = (1.07, 0.62, 0.02)
+    vol, _ = _run(tmp_path, data, extra=["zfilter=1", "ct=" + "|".join(str(c) for c in ct)])
+    sc, rc = oracle_params(W)
+    v32 = o.fdk(data["raw"], data["ptm"], sc, rc, dark=data["dark"], gain=data["gain"], zfilter_v=True, ct=ct)
+    e = rel_rmse(vol, v32)
+    print("agent graph zfilter+ct vs FP32-ref oracle", e)
+    assert e <= 1e-4

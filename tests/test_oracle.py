@@ -325,3 +325,68 @@ def test_oracle_projector_is_transpose_of_backprojector(weighted):
     _, acc, _ = o.backproject(y, ptm, (12, 18, 20), w.pxy, w.pz, fp64=True, return_parts=True)
     if weighted:
         np.testing.assert_allclose(By, acc, rtol=1e-12, atol=1e-12)
+
+
+# ------------------------------------------------- ZFilter, CT numbers -------
+
+@pytest.mark.parametrize("fp64", [False, True])
+def test_zfilter_impulse_and_edges(fp64):
+    # ZFilterAgent.cpp:49-71: kernel (0.25, 0.5, 0.25) along v, edge rows
+    # replicated; an impulse in an interior row and in each edge row
+    V, W = 7, 3
+    x = np.zeros((3, V, W))
+    x[0, 3, 1] = 8.0
+    x[1, 0, 0] = 8.0
+    x[2, V - 1, 2] = 8.0
+    z = o.zfilter(x, fp64=fp64)
+    exp = np.zeros_like(x)
+    exp[0, 2:5, 1] = [2.0, 4.0, 2.0]
+    exp[1, 0, 0], exp[1, 1, 0] = 6.0, 2.0          # 0.5*8 + 0.25*8 (clamped) ; 0.25*8
+    exp[2, V - 1, 2], exp[2, V - 2, 2] = 6.0, 2.0
+    np.testing.assert_array_equal(z, exp)
+
+
+def test_zfilter_constant_and_linear_in_v():
+    # a constant frame is unchanged; a ramp linear in v is unchanged away from
+    # the edges (the kernel is symmetric and sums to 1), and the edges see the
+    # replicated row: row 0 -> 0.75 r0 + 0.25 r1
+    V, W = 9, 4
+    c = np.full((1, V, W), 3.25, np.float32)
+    np.testing.assert_array_equal(o.zfilter(c), c)
+    r = np.tile(np.arange(V, dtype=np.float64)[None, :, None] * 2.0 + 1.0, (1, 1, W))
+    z = o.zfilter(r, fp64=True)
+    np.testing.assert_allclose(z[0, 1:-1], r[0, 1:-1], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(z[0, 0], 0.75 * r[0, 0] + 0.25 * r[0, 1], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(z[0, -1], 0.75 * r[0, -1] + 0.25 * r[0, -2], rtol=0, atol=1e-12)
+
+
+def test_zfilter_fp32_matches_fp64():
+    x = synth.normal(31, (2, 40, 24))
+    a = o.zfilter(x)
+    b = o.zfilter(x.astype(np.float64), fp64=True)
+    assert np.abs(a - b).max() <= 2e-7 * np.abs(b).max()
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+def test_ctnum_closed_forms(fp64):
+    # CTNumCore.cpp:22-36 with norm0 = 1: water -> 0, air -> -1000; above 300
+    # compressed by norm1; saturated at 8191; NaN (no view) stays NaN
+    mu = 0.02
+    x = np.array([0.0, mu, mu * 1.2, mu * 1.3, mu * 1.9, mu * 100.0, np.nan])
+    y = o.ctnum(x, 1.0, 0.5, mu, fp64=fp64)
+    exp = [-1000.0, 0.0, 200.0, 300.0, 300.0 + 600.0 * 0.5, 8191.0]
+    np.testing.assert_allclose(y[:6], exp, rtol=0, atol=1e-3 if not fp64 else 1e-9)
+    assert np.isnan(y[6])
+
+
+def test_ctnum_inverts_the_reference_metal_threshold():
+    # ConfigHelper.cpp:139-142 maps a threshold in CT numbers (> 300) back to an
+    # attenuation through the inverse of CTNumCore's map: the forward map must
+    # return the threshold
+    norm0, norm1, mu = 1.07, 0.62, 0.02
+    for hu in (600.0, 2500.0, 4000.0):
+        h = (hu - 300.0) / norm1 + 300.0
+        h = (h + 1000.0) / norm0 - 1000.0
+        att = (h + 1000.0) / 1000.0 * mu
+        got = o.ctnum(np.array([att]), norm0, norm1, mu, fp64=True)[0]
+        assert abs(got - hu) < 1e-9 * hu
