@@ -58,6 +58,7 @@ typedef void* jh_stream_t; /* a hipStream_t (NULL = the default stream) */
 
 typedef struct jh_pre_plan jh_pre_plan;
 typedef struct jh_filter_plan jh_filter_plan;
+typedef struct jh_composite_plan jh_composite_plan;
 typedef struct jh_bp_plan jh_bp_plan;
 typedef struct jh_graph jh_graph;
 
@@ -118,21 +119,71 @@ jh_status jh_filter_create(jh_filter_plan** plan, int device, int W, int V, int 
                            float cutEnd, const float* adjPts, const float* adjDB, int nAdj,
                            int window, int mode);
 
+/* Replaces the FilterCore constructor with its widthOut and yMirroring
+ * arguments (FilterAgent/FilterCore.cpp:13-61) as FilterAgent builds it
+ * (FilterAgent.cpp:10-56): frames of Win x V in, Wout x V out.  With
+ * yMirroring the truncation ramp (R-1-i)/R follows the data, its edge is the
+ * mean of the last three inputs and the output is columns [Wout, 2 Wout) of
+ * the filtered row; the pre-weight u axis is shifted by Win / 2
+ * (PreFilterWeightGenerator.cpp:36-37).  JEngine's offset-detector chain
+ * uses Win = 2 NumUsedDetsU, Wout = NumUsedDetsU, yMirroring =
+ * detectorOnTheRight (JEngine.cpp:223-238).  jh_filter_create is the form
+ * Win = Wout = W, yMirroring = 0.  Requires (yMirroring ? 2 : 1) * Wout <= L. */
+jh_status jh_filter_create_ex(jh_filter_plan** plan, int device, int Win, int V, int Wout, int yMirroring,
+                              int nviews, const double* h_ptm, float dso, float halfSampleRate, float cutStart,
+                              float cutEnd, const float* adjPts, const float* adjDB, int nAdj, int window,
+                              int mode);
+
+/* Input width, output width and yMirroring of a filter plan (any may be NULL). */
+jh_status jh_filter_shape(const jh_filter_plan* plan, int* Win, int* Wout, int* yMirroring);
+
 /* Copies the frequency response H[0..L/2] (L = FFT length of the reference,
  * FilterCore.cpp:254-260) to the host; *L receives L. */
 jh_status jh_filter_response(jh_filter_plan* plan, double* h_H, int* L);
 
-/* Copies the pre-weight map used for view `view` (V rows x W columns) to
+/* Copies the pre-weight map used for view `view` (V rows x Win columns) to
  * device memory d_out. */
 jh_status jh_filter_preweight(jh_filter_plan* plan, int view, float* d_out, jh_stream_t stream);
 
 /* Replaces FilterCore::ProcessRow (FilterCore.cpp:189-222) over all rows of
- * nviews frames: d_in, d_out are nviews x V x W f32; frame i is view
- * (view0 + i) of the scan (selects its pre-weight map). */
+ * nviews frames: d_in is nviews x V x Win f32, d_out nviews x V x Wout;
+ * frame i is view (view0 + i) of the scan (selects its pre-weight map). */
 jh_status jh_filter_run(jh_filter_plan* plan, const float* d_in, float* d_out, int view0,
                         int nviews, jh_stream_t stream);
 
 jh_status jh_filter_destroy(jh_filter_plan* plan);
+
+/* ---------------------------------------------------------------------- */
+/* Offset-detector composite                                               */
+/* ---------------------------------------------------------------------- */
+
+/* JEngine/JEngine.cpp:153-161: 1 when the iso-centre projects left of the
+ * detector middle in the first matrix (ptm0: 12 doubles, row-major). */
+int jh_detector_on_right(const double* ptm0, int numUsedDetsU);
+
+/* Replaces CompositeAgent/CompositeCore.cpp:13-65 (constructor) for frames
+ * of W x V and nviews (even) views of a full rotation; h_ptm: nviews
+ * row-major 3x4 matrices on the host.  detectorOnRight < 0 decides it from
+ * the first matrix as jh_detector_on_right does.  The per-view maps from the
+ * direct view's detector, through the iso-centre plane (pseudo-depth 0),
+ * into the conjugate view (v + nviews/2) % nviews are composed on the device
+ * in FP64. */
+jh_status jh_composite_create(jh_composite_plan** plan, int device, int W, int V, int nviews,
+                              const double* h_ptm, float dso, int detectorOnRight);
+
+/* The detector side a composite plan uses. */
+jh_status jh_composite_side(const jh_composite_plan* plan, int* detectorOnRight);
+
+/* Replaces CompositeCore::Process (CompositeCore.cpp:66-150) for nviews
+ * views: frame i is view (view0 + i), its direct frame at
+ * d_dirc + i * inStride and its conjugate frame (view (view0 + i + N/2) % N,
+ * the pairing of JEngine/InputAgent.cpp:47-85) at d_conj + i * inStride
+ * (elements; W x V f32 each).  d_out: nviews x V x 2W f32, the direct frame
+ * in the right half when the detector is on the right, else in the left. */
+jh_status jh_composite_run(jh_composite_plan* plan, const float* d_dirc, const float* d_conj, long long inStride,
+                           float* d_out, int view0, int nviews, jh_stream_t stream);
+
+jh_status jh_composite_destroy(jh_composite_plan* plan);
 
 /* ---------------------------------------------------------------------- */
 /* Backprojection                                                          */

@@ -22,7 +22,9 @@ DTYPES = {"f32": JH_F32, "bf16": JH_BF16, "f16": JH_F16}
 EXPORTS = [
     "jh_last_error", "jh_version", "jh_device_count", "jh_ptm_modify",
     "jh_pre_create", "jh_pre_run", "jh_pre_destroy",
-    "jh_filter_create", "jh_filter_response", "jh_filter_preweight", "jh_filter_run", "jh_filter_destroy",
+    "jh_filter_create", "jh_filter_create_ex", "jh_filter_shape", "jh_filter_response", "jh_filter_preweight",
+    "jh_filter_run", "jh_filter_destroy",
+    "jh_detector_on_right", "jh_composite_create", "jh_composite_side", "jh_composite_run", "jh_composite_destroy",
     "jh_bp_create", "jh_bp_uweights", "jh_bp_stage_bytes", "jh_bp_stage", "jh_bp_run", "jh_bp_reset",
     "jh_bp_accumulate", "jh_bp_finalize", "jh_bp_set_tile", "jh_bp_get_tile", "jh_bp_geometry_mode",
     "jh_bp_destroy", "jh_bp_pack", "jh_bp_backproject", "jh_fp_project", "jh_bp_autotune",
@@ -59,10 +61,18 @@ _SIGS = {
     "jh_pre_destroy": (c_int, [P]),
     "jh_filter_create": (c_int, [P, c_int, c_int, c_int, c_int, P, c_float, c_float, c_float, c_float, P, P,
                                  c_int, c_int, c_int]),
+    "jh_filter_create_ex": (c_int, [P, c_int, c_int, c_int, c_int, c_int, c_int, P, c_float, c_float, c_float,
+                                    c_float, P, P, c_int, c_int, c_int]),
+    "jh_filter_shape": (c_int, [P, P, P, P]),
     "jh_filter_response": (c_int, [P, P, P]),
     "jh_filter_preweight": (c_int, [P, c_int, P, P]),
     "jh_filter_run": (c_int, [P, P, P, c_int, c_int, P]),
     "jh_filter_destroy": (c_int, [P]),
+    "jh_detector_on_right": (c_int, [P, c_int]),
+    "jh_composite_create": (c_int, [P, c_int, c_int, c_int, c_int, P, c_float, c_int]),
+    "jh_composite_side": (c_int, [P, P]),
+    "jh_composite_run": (c_int, [P, P, P, ctypes.c_longlong, P, c_int, c_int, P]),
+    "jh_composite_destroy": (c_int, [P]),
     "jh_bp_create": (c_int, [P, c_int, P, c_int, c_int, c_int, c_int, c_int, c_int, c_float, c_float]),
     "jh_bp_uweights": (c_int, [P, P]),
     "jh_bp_stage_bytes": (c_int, [P, c_int, c_int, P]),

@@ -16,7 +16,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 
-HIP_SOURCES = ["jhip_core.hip", "jhip_bp.hip", "jhip_filter.hip", "jhip_fp.hip", "jhip_graph.hip"]
+HIP_SOURCES = ["jhip_core.hip", "jhip_bp.hip", "jhip_filter.hip", "jhip_composite.hip", "jhip_fp.hip",
+               "jhip_graph.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -5,6 +5,8 @@
   cubic beam hardening, 40 ellipsoids plus a 0.3/mm tooth
 * ``adjoint256`` -- 256^3 at 0.3 mm, 360 views of 512^2 at 0.4 mm
 * ``batch16``   -- 16 scans of uneven sizes drawn from the seed
+* ``offset128``, ``offset_dental512`` -- half-fan (offset) detectors, the
+  composite chain
 
 Raw counts are I = Poisson(I0 * g * exp(-p)) + d with I0 = 4000, per-pixel gain
 g ~ U(0.9, 1.1) and dark d ~ U{90..110}.
@@ -39,6 +41,7 @@ class Workload:
     bh: Optional[List[float]] = None  # None = seeded cubic when tooth, else {1,0,0}
     I0: float = 4000.0
     seed: int = 1
+    u_shift: float = 0.0     # principal point left of the detector centre (offset detector), pixels
 
     @property
     def gups_units(self):
@@ -69,7 +72,8 @@ class Workload:
         return (1.0, float(np.float32(0.02 + 0.02 * u[0])), float(np.float32(0.002 + 0.003 * u[1])))
 
     def ptms(self):
-        return geometry.circular_ptms(self.n_views, self.sod, self.sdd, self.n_u, self.n_v, self.du)
+        return geometry.circular_ptms(self.n_views, self.sod, self.sdd, self.n_u, self.n_v, self.du,
+                                      u_shift=self.u_shift)
 
     def ellipsoids(self):
         half = (self.X * self.pxy / 2.0, self.Y * self.pxy / 2.0, self.Z * self.pz / 2.0)
@@ -95,7 +99,16 @@ DENTAL512 = Workload("dental512", 350.0, 600.0, 600, 768, 768, 0.2, 512, 512, 51
 ADJOINT256 = Workload("adjoint256", 300.0, 500.0, 360, 512, 512, 0.4, 256, 256, 256, 0.3, 0.3,
                       n_ellipsoids=10, window=0, ramlak=True)
 
-WORKLOADS = {w.name: w for w in (CBCT128, DENTAL512, ADJOINT256)}
+# Offset (half-fan) detector: the central ray hits column 31.5 of 160, so the
+# detector extends to the right of it and the reconstruction needs the
+# composite of every view with its conjugate (JEngine.cpp:152-238).
+OFFSET128 = Workload("offset128", 300.0, 500.0, 128, 160, 128, 0.5, 128, 128, 64, 0.25, 0.25,
+                     n_ellipsoids=10, window=0, ramlak=True, u_shift=48.0)
+# Dental-scale half-fan: 768 x 768 detector, central ray at column 95.5.
+OFFSET_DENTAL512 = Workload("offset_dental512", 350.0, 600.0, 600, 768, 768, 0.2, 512, 512, 512, 0.15, 0.15,
+                            n_ellipsoids=40, tooth=True, window=1, ramlak=False, u_shift=288.0)
+
+WORKLOADS = {w.name: w for w in (CBCT128, DENTAL512, ADJOINT256, OFFSET128, OFFSET_DENTAL512)}
 
 
 def batch16(seed=4):

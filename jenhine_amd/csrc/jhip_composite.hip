@@ -1,0 +1,242 @@
+// Offset-detector composite on gfx950 (CompositeAgent/CompositeCore.cpp).
+//
+// A half-fan scan puts the detector to one side of the central ray.  The
+// reference widens every preprocessed view to 2W columns before filtering:
+// the direct frame fills one half and the other half is the conjugate view
+// (v + N/2) % N re-sampled through the iso-centre plane (JEngine.cpp:213-238,
+// InputAgent.cpp:47-85).  Per output pixel (c, r) of the conjugate half the
+// reference forms the virtual detector point (u1, v1, 0, 1), u1 = c - W when
+// the detector is on the right, else c + W, back-projects it with the
+// inverse transform matrix of the direct view and projects it with the
+// conjugate view's matrix (CompositeCore.cpp:66-104).  Because the
+// pseudo-depth coordinate is 0, that chain is a 3 x 3 homography per view:
+//   (U, V, S) = G (u1, v1, 1),  G = P_conj * TM_direct^-1 [columns 0, 1, 3],
+// composed once per view in FP64 on the device and rounded to float; the
+// per-pixel work is then 3 multiply-add chains and two divisions in FP32.
+//
+// The conjugate frame is sampled bilinearly at (U/S, clamp(V/S, 0, V-1));
+// a sample outside the frame leaves 0 (the remap semantics of the
+// backprojector).  The 40 conjugate columns next to the direct half are
+// blended towards the mean of the 5 direct pixels at the shared border:
+// out = conj * maskConj[i] + edge * maskDirc[i] (CompositeCore.cpp:106-148).
+// The masks are formed on the host in double and rounded once.
+#include <cmath>
+#include <vector>
+
+#include "jhip_internal.h"
+
+struct jh_composite_plan {
+    int device;
+    int W, V, N;
+    int detRight;
+    jh::DevBuf G;       // N x 9 float: the per-view homography
+    float maskDirc[40];
+    float maskConj[40];
+};
+
+namespace jh {
+
+namespace {
+
+constexpr int kTransWidth = 40;  // CompositeCore.cpp:41
+constexpr int kEdgeWidth = 5;    // CompositeCore.cpp:42
+
+// G of every view in FP64: TM^-1 of the view, times the conjugate view's
+// matrix, keeping the columns that multiply u1, v1 and the homogeneous 1.
+__global__ void composite_setup_kernel(const double* __restrict__ ptm, int N, double dso, float* __restrict__ G)
+{
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= N) return;
+    double inv[16];
+    tm_inverse_fp64(ptm + 12 * (size_t)v, dso, inv);
+    const double* Pc = ptm + 12 * (size_t)((v + N / 2) % N);
+    const int cols[3] = {0, 1, 3};
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < 4; ++k) s += Pc[4 * i + k] * inv[4 * k + cols[j]];
+            G[9 * (size_t)v + 3 * i + j] = (float)s;
+        }
+}
+
+struct CompArgs {
+    const float* dirc;
+    const float* conj;
+    long long inStride;  // elements between consecutive input frames
+    float* out;          // nviews x V x 2W
+    const float* G;      // N x 9
+    int W, V, view0, nviews, detRight;
+    float md[kTransWidth];
+    float mc[kTransWidth];
+};
+
+#pragma clang fp contract(off)
+// Bilinear sample of frame img (W x V) at (su, sv); false outside the frame.
+__device__ __forceinline__ bool remap_at(const float* __restrict__ img, int W, int V, float su, float sv, float& val)
+{
+    if (!(su >= 0.0f && su <= (float)(W - 1) && sv >= 0.0f && sv <= (float)(V - 1))) return false;
+    int x0 = (int)su, y0 = (int)sv;
+    x0 = min(x0, W - 2);
+    y0 = min(y0, V - 2);
+    const float fx = su - (float)x0, fy = sv - (float)y0;
+    const float* r0 = img + (size_t)y0 * W + x0;
+    const float* r1 = r0 + W;
+    const float top = r0[0] + fx * (r0[1] - r0[0]);
+    const float bot = r1[0] + fx * (r1[1] - r1[0]);
+    val = top + fy * (bot - top);
+    return true;
+}
+
+// One thread per pixel of the conjugate half; the direct half is copied by
+// the same thread (column c of the direct frame).
+__global__ void __launch_bounds__(256) composite_kernel(CompArgs a)
+{
+    const int W = a.W, V = a.V, W2 = 2 * W;
+    const size_t total = (size_t)a.nviews * V * W;
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x) {
+        const int f = (int)(t / ((size_t)V * W));
+        const int rem = (int)(t - (size_t)f * V * W);
+        const int r = rem / W, c = rem - r * W;
+        const float* dirc = a.dirc + (size_t)f * a.inStride;
+        const float* conj = a.conj + (size_t)f * a.inStride;
+        float* o = a.out + ((size_t)f * V + r) * W2;
+        const float* g = a.G + 9 * (size_t)(a.view0 + f);
+        const float u1 = a.detRight ? (float)c - (float)W : (float)(c + W);
+        const float v1 = (float)r;
+        const float U = fmaf(g[0], u1, fmaf(g[1], v1, g[2]));
+        const float Vv = fmaf(g[3], u1, fmaf(g[4], v1, g[5]));
+        const float S = fmaf(g[6], u1, fmaf(g[7], v1, g[8]));
+        const float u2 = U / S;
+        float v2 = Vv / S;
+        if (v2 < 0.0f) v2 = 0.0f;
+        if (v2 > (float)(V - 1)) v2 = (float)(V - 1);
+        float val = 0.0f;
+        remap_at(conj, W, V, u2, v2, val);
+        const int conjOff = a.detRight ? 0 : W;
+        const int transOff = a.detRight ? W - kTransWidth : W;
+        const int oc = conjOff + c;
+        const float* drow = dirc + (size_t)r * W;
+        if (oc >= transOff && oc < transOff + kTransWidth) {
+            const int i = oc - transOff;
+            const float* e = a.detRight ? drow : drow + W - kEdgeWidth;
+            float sum = e[0];
+#pragma unroll
+            for (int k = 1; k < kEdgeWidth; ++k) sum = sum + e[k];
+            const float edge = sum / (float)kEdgeWidth;
+            const float x = val * a.mc[i];
+            const float y = edge * a.md[i];
+            val = x + y;
+        }
+        o[oc] = val;
+        o[(a.detRight ? W : 0) + c] = drow[c];
+    }
+}
+#pragma clang fp contract(on)
+
+unsigned grid_1d(size_t n)
+{
+    return (unsigned)std::min<size_t>((n + 255) / 256, (size_t)65536);
+}
+
+}  // namespace
+
+}  // namespace jh
+
+using namespace jh;
+
+extern "C" {
+
+int jh_detector_on_right(const double* ptm0, int numUsedDetsU)
+{
+    if (!ptm0) return 0;
+    // JEngine.cpp:153-161, in the reference's float arithmetic
+    const float originU = (float)ptm0[3] / (float)ptm0[11];
+    return originU < (float)numUsedDetsU / 2.0f ? 1 : 0;
+}
+
+jh_status jh_composite_create(jh_composite_plan** plan, int device, int W, int V, int nviews, const double* h_ptm,
+                              float dso, int detectorOnRight)
+{
+    return guard([&] {
+        require(plan != nullptr, "jh_composite_create: plan is NULL");
+        *plan = nullptr;
+        require(W >= kTransWidth && W >= kEdgeWidth && V >= 2, "jh_composite_create: frame narrower than 40 columns");
+        require(nviews >= 2 && nviews % 2 == 0, "jh_composite_create: the view count must be even (conjugate pairs)");
+        require(h_ptm != nullptr, "jh_composite_create: no projection matrices");
+        require(dso > 1.0f, "jh_composite_create: bad DSO");
+        require((size_t)nviews * V * 2 * W < ((size_t)1 << 40), "jh_composite_create: scan too large");
+        int ndev = 0;
+        JH_HIP(hipGetDeviceCount(&ndev));
+        require(device >= 0 && device < ndev, "jh_composite_create: no such device");
+        DeviceScope ds(device);
+        auto* p = new jh_composite_plan;
+        try {
+            p->device = device;
+            p->W = W; p->V = V; p->N = nviews;
+            p->detRight = detectorOnRight < 0 ? jh_detector_on_right(h_ptm, W) : (detectorOnRight ? 1 : 0);
+            const double pi = 3.14159265358979323846;
+            for (int i = 0; i < kTransWidth; ++i) {
+                const double c = std::cos((double)i / kTransWidth * pi);
+                const double md = p->detRight ? 0.5 - 0.5 * c : 0.5 + 0.5 * c;
+                p->maskDirc[i] = (float)md;
+                p->maskConj[i] = (float)(1.0 - md);
+            }
+            DevBuf dptm(sizeof(double) * 12 * nviews);
+            JH_HIP(hipMemcpy(dptm.get(), h_ptm, sizeof(double) * 12 * nviews, hipMemcpyHostToDevice));
+            p->G.alloc(sizeof(float) * 9 * nviews);
+            composite_setup_kernel<<<(nviews + 63) / 64, 64>>>(dptm.as<double>(), nviews, (double)dso, p->G.as<float>());
+            check_launch("composite_setup_kernel");
+            JH_HIP(hipDeviceSynchronize());
+        } catch (...) {
+            delete p;
+            throw;
+        }
+        *plan = p;
+    });
+}
+
+jh_status jh_composite_side(const jh_composite_plan* p, int* detectorOnRight)
+{
+    return guard([&] {
+        require(p != nullptr && detectorOnRight != nullptr, "jh_composite_side: bad arguments");
+        *detectorOnRight = p->detRight;
+    });
+}
+
+jh_status jh_composite_run(jh_composite_plan* p, const float* d_dirc, const float* d_conj, long long inStride,
+                           float* d_out, int view0, int nviews, jh_stream_t stream)
+{
+    return guard([&] {
+        require(p != nullptr && d_dirc != nullptr && d_conj != nullptr && d_out != nullptr,
+                "jh_composite_run: bad arguments");
+        require(view0 >= 0 && nviews >= 0 && view0 + nviews <= p->N, "jh_composite_run: view range outside the scan");
+        require(nviews <= 1 || inStride >= (long long)p->W * p->V, "jh_composite_run: input frames overlap");
+        if (nviews == 0) return;
+        DeviceScope ds(p->device);
+        CompArgs a;
+        a.dirc = d_dirc;
+        a.conj = d_conj;
+        a.inStride = inStride;
+        a.out = d_out;
+        a.G = p->G.as<float>();
+        a.W = p->W; a.V = p->V;
+        a.view0 = view0;
+        a.nviews = nviews;
+        a.detRight = p->detRight;
+        for (int i = 0; i < kTransWidth; ++i) { a.md[i] = p->maskDirc[i]; a.mc[i] = p->maskConj[i]; }
+        const size_t total = (size_t)nviews * p->V * p->W;
+        composite_kernel<<<grid_1d(total), 256, 0, as_stream(stream)>>>(a);
+        check_launch("composite_kernel");
+    });
+}
+
+jh_status jh_composite_destroy(jh_composite_plan* p)
+{
+    return guard([&] {
+        if (!p) return;
+        DeviceScope ds(p->device);
+        delete p;
+    });
+}
+
+}  // extern "C"

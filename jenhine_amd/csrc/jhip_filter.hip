@@ -33,12 +33,13 @@
 
 struct jh_filter_plan {
     int device;
-    int W, V, N, L, K, Kp;
+    int W, V, N, L, K, Kp;  // W: input width
+    int Wout, mirror;       // output width, yMirroring
     int mode;
     int nblocks;
     std::vector<int> view_block;  // block index of every view
     std::vector<double> H;        // L/2 + 1
-    jh::DevBuf B;                 // Kp x W float (vector-ALU form)
+    jh::DevBuf B;                 // Kp x Wout float (vector-ALU form)
     jh::DevBuf Bhl;               // MFMA forms: packed FP32 or BF16-part operand
     jh::DevBuf pw;                // nblocks x V x W float
     jh::DevBuf blk;               // N int: block of every view (device copy)
@@ -50,38 +51,13 @@ namespace jh {
 // Pre-filter weight maps, FP64.
 // ---------------------------------------------------------------------------
 
-// TM of TransformMatrix/MatrixConvertor.cpp:8-27 and its inverse (Gauss-Jordan
-// with partial pivoting), one thread per refresh block.
+// Inverse TM of every refresh block's source view, one thread per block.
 __global__ void pw_inverse_kernel(const double* __restrict__ ptm, const int* __restrict__ src_view,
                                   int nb, double dso, double* __restrict__ inv)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nb) return;
-    const double* P = ptm + 12 * (size_t)src_view[b];
-    const double far_ = (1.0 + 1.0 / dso) * dso, near_ = (1.0 - 1.0 / dso) * dso;
-    const double s20 = (far_ + near_) / (far_ - near_);
-    const double s23 = ((far_ + near_) - 2.0 * far_ * near_ / dso) / (far_ - near_) / dso;
-    double A[4][8];
-    for (int c = 0; c < 4; ++c) { A[0][c] = P[c]; A[1][c] = P[4 + c]; A[3][c] = P[8 + c]; }
-    A[2][0] = s20 * P[8]; A[2][1] = s20 * P[9]; A[2][2] = 0.0; A[2][3] = s23 * P[11];
-    for (int r = 0; r < 4; ++r)
-        for (int c = 0; c < 4; ++c) A[r][4 + c] = (r == c) ? 1.0 : 0.0;
-    for (int k = 0; k < 4; ++k) {
-        int p = k;
-        for (int r = k + 1; r < 4; ++r)
-            if (fabs(A[r][k]) > fabs(A[p][k])) p = r;
-        if (p != k)
-            for (int c = 0; c < 8; ++c) { double t = A[k][c]; A[k][c] = A[p][c]; A[p][c] = t; }
-        const double d = A[k][k];
-        for (int c = 0; c < 8; ++c) A[k][c] /= d;
-        for (int r = 0; r < 4; ++r) {
-            if (r == k) continue;
-            const double f = A[r][k];
-            for (int c = 0; c < 8; ++c) A[r][c] -= f * A[k][c];
-        }
-    }
-    for (int r = 0; r < 4; ++r)
-        for (int c = 0; c < 4; ++c) inv[16 * (size_t)b + 4 * r + c] = A[r][4 + c];
+    tm_inverse_fp64(ptm + 12 * (size_t)src_view[b], dso, inv + 16 * (size_t)b);
 }
 
 // Distance between the back-projections of detector point (u, v) at the two
@@ -144,17 +120,19 @@ struct FiltArgs {
     const float* x;      // rows x W (frames of V rows)
     const float* pw;     // nblocks x V x W
     const int* blk;      // block of every view of the scan
-    const float* B;      // Kp x W
+    const float* B;      // Kp x Wout
     const void* Bhl;     // MFMA operand
-    float* out;          // rows x W
+    float* out;          // rows x Wout
     int W, V, rows, Kp, view0;
+    int Wout;            // output width
+    int edge0;           // first of the three edge columns (0, or W - 3 with yMirroring)
 };
 
 // Element (r, m) of the A operand.
 __device__ inline float a_elem(const FiltArgs& a, int r, int m, const float* xr, const float* pwr)
 {
     if (m < a.W) return xr[m] * pwr[m];
-    if (m < a.W + 3) return xr[m - a.W];
+    if (m < a.W + 3) return xr[m - a.W + a.edge0];
     return 0.0f;
 }
 
@@ -178,9 +156,9 @@ __global__ void __launch_bounds__(256) filter_fp32_kernel(FiltArgs a)
 #pragma unroll
         for (int q = 0; q < 4; ++q) As[lk + q][lr] = a_elem(a, r, k0 + lk + q, xr, pwr);
         {
-            const float* bp = a.B + (size_t)(k0 + bk) * a.W + col0 + bn;
+            const float* bp = a.B + (size_t)(k0 + bk) * a.Wout + col0 + bn;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) Bs[bk][bn + q] = (col0 + bn + q < a.W) ? bp[q] : 0.0f;
+            for (int q = 0; q < 4; ++q) Bs[bk][bn + q] = (col0 + bn + q < a.Wout) ? bp[q] : 0.0f;
         }
         __syncthreads();
 #pragma unroll
@@ -202,7 +180,7 @@ __global__ void __launch_bounds__(256) filter_fp32_kernel(FiltArgs a)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int cc = col0 + tx * 4 + q;
-            if (cc < a.W) a.out[(size_t)rr * a.W + cc] = c[p][q];
+            if (cc < a.Wout) a.out[(size_t)rr * a.Wout + cc] = c[p][q];
         }
     }
 }
@@ -267,7 +245,7 @@ __device__ inline void load_a8(const FiltArgs& a, const float* xr, const float* 
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const int m = kb + q;
-            e[q] = m < a.W ? xr[m] * pwr[m] : (m < a.W + 3 ? xr[m - a.W] : 0.0f);
+            e[q] = m < a.W ? xr[m] * pwr[m] : (m < a.W + 3 ? xr[m - a.W + a.edge0] : 0.0f);
         }
     }
 }
@@ -284,7 +262,7 @@ __global__ void __launch_bounds__(256) filter_mfma_kernel(FiltArgs a)
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int ncb8 = (a.W + 127) / 128 * 8;            // packed column blocks (multiple of 8)
+    const int ncb8 = (a.Wout + 127) / 128 * 8;         // packed column blocks (multiple of 8)
     const int cb0 = blockIdx.x * 8;
     const int rowTile = blockIdx.y * 128 + wave * 32;
     const int kq = 8 * (lane >> 4);
@@ -384,11 +362,11 @@ __global__ void __launch_bounds__(256) filter_mfma_kernel(FiltArgs a)
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
             const int col = (cb0 + c) * 16 + (lane & 15);
-            if (col >= a.W) continue;
+            if (col >= a.Wout) continue;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 const int rr = rowTile + rb * 16 + 4 * (lane >> 4) + q;
-                if (rr < a.rows) a.out[(size_t)rr * a.W + col] = acc[rb][c][q];
+                if (rr < a.rows) a.out[(size_t)rr * a.Wout + col] = acc[rb][c][q];
             }
         }
 }
@@ -418,24 +396,28 @@ namespace jh {
 void filter_plan_shape(const jh_filter_plan* p, int* device, int* W, int* V, int* nviews)
 {
     *device = p->device;
-    *W = p->W;
+    *W = p->W == p->Wout && !p->mirror ? p->W : -1;  // the batch graph takes the plain form only
     *V = p->V;
     *nviews = p->N;
 }
 
 }  // namespace jh
 
-extern "C" {
+namespace {
 
-jh_status jh_filter_create(jh_filter_plan** plan, int device, int W, int V, int nviews,
-                           const double* h_ptm, float dso, float halfSampleRate, float cutStart,
-                           float cutEnd, const float* adjPts, const float* adjDB, int nAdj,
-                           int window, int mode)
+// FilterCore constructor with widthOut and yMirroring (FilterCore.cpp:13-61)
+// plus the FilterAgent's pre-weight generator and schedule
+// (FilterAgent.cpp:10-56, 101-126).
+jh_status create_filter(jh_filter_plan** plan, int device, int W, int V, int Wout, int mirror, int nviews,
+                        const double* h_ptm, float dso, float halfSampleRate, float cutStart, float cutEnd,
+                        const float* adjPts, const float* adjDB, int nAdj, int window, int mode)
 {
     return guard([&] {
         require(plan != nullptr, "jh_filter_create: plan is NULL");
         *plan = nullptr;
         require(W >= 3 && V >= 1 && nviews >= 1, "jh_filter_create: bad frame size or view count");
+        require(Wout >= 1 && (mirror ? 2 * Wout : Wout) <= fft_length(W),
+                "jh_filter_create: output columns beyond the FFT length");
         require(h_ptm != nullptr, "jh_filter_create: no projection matrices");
         require(dso > 1.0f && halfSampleRate > 0.0f, "jh_filter_create: bad DSO or sample rate");
         require(nAdj >= 0 && (nAdj == 0 || (adjPts && adjDB)), "jh_filter_create: bad adjust points");
@@ -452,6 +434,8 @@ jh_status jh_filter_create(jh_filter_plan** plan, int device, int W, int V, int 
         try {
             p->device = device;
             p->W = W; p->V = V; p->N = nviews; p->mode = mode;
+            p->Wout = Wout; p->mirror = mirror ? 1 : 0;
+            const int off = p->mirror ? Wout : 0;  // FilterCore.cpp:28
             p->L = fft_length(W);
             p->K = W + 3;
             p->Kp = (p->K + 31) / 32 * 32;
@@ -468,29 +452,35 @@ jh_status jh_filter_create(jh_filter_plan** plan, int device, int W, int V, int 
                 for (int k = 1; k < L / 2; ++k) s += 2.0 * p->H[k] * cs[(size_t)k * n % L];
                 h[n] = s / L;
             }
+            // h at a circular offset d (any sign)
+            auto hc = [&](long d) { return h[(size_t)(((d % L) + L) % L)]; };
+            // truncation ramp response at output column n (FilterCore.cpp:45-59,
+            // 205-215): ramp i/R at L-R+i, or (R-1-i)/R at W+i with yMirroring
             const int R = (L - W) / 2;
-            std::vector<double> rr(W, 0.0);
-            for (int n = 0; n < W; ++n) {
+            std::vector<double> rr(Wout, 0.0);
+            for (int n = 0; n < Wout; ++n) {
+                const long g = n + off;
                 double s = 0.0;
-                for (int i = 0; i < R; ++i) s += ((double)i / R) * h[(n + R - i) % L];
+                for (int i = 0; i < R; ++i)
+                    s += p->mirror ? ((double)(R - 1 - i) / R) * hc(g - (W + i)) : ((double)i / R) * hc(g - (L - R + i));
                 rr[n] = s;
             }
-            std::vector<float> Bh((size_t)p->Kp * W, 0.0f);
+            std::vector<float> Bh((size_t)p->Kp * Wout, 0.0f);
             for (int m = 0; m < W; ++m)
-                for (int n = 0; n < W; ++n) Bh[(size_t)m * W + n] = (float)h[std::abs(n - m)];
+                for (int n = 0; n < Wout; ++n) Bh[(size_t)m * Wout + n] = (float)hc((long)n + off - m);
             for (int e = 0; e < 3; ++e)
-                for (int n = 0; n < W; ++n) Bh[(size_t)(W + e) * W + n] = (float)(rr[n] / 3.0);
+                for (int n = 0; n < Wout; ++n) Bh[(size_t)(W + e) * Wout + n] = (float)(rr[n] / 3.0);
             p->B.alloc(sizeof(float) * Bh.size());
             JH_HIP(hipMemcpy(p->B.get(), Bh.data(), sizeof(float) * Bh.size(), hipMemcpyHostToDevice));
             if (mode != JH_FILTER_FP32_VALU) {
                 // operand packed per (k step s, column block cb, lane l): the 8
                 // values B[32 s + 8 (l / 16) + q][16 cb + l % 16], q = 0..7, as
                 // FP32, or as P BF16 parts (hi, [mid,] lo) split from the FP64 value
-                const int ncb8 = (W + 127) / 128 * 8, nst = p->Kp / 32;
+                const int ncb8 = (Wout + 127) / 128 * 8, nst = p->Kp / 32;
                 const int P = mode == JH_FILTER_MFMA_BF16X6 ? 3 : (mode == JH_FILTER_MFMA_BF16X3 ? 2 : 0);
                 auto bval = [&](int k, int col) -> double {
-                    if (col >= W) return 0.0;
-                    if (k < W) return h[std::abs(col - k)];
+                    if (col >= Wout) return 0.0;
+                    if (k < W) return hc((long)col + off - k);
                     if (k < W + 3) return rr[col] / 3.0;
                     return 0.0;
                 };
@@ -542,11 +532,12 @@ jh_status jh_filter_create(jh_filter_plan** plan, int device, int W, int V, int 
             pw_inverse_kernel<<<(p->nblocks + 63) / 64, 64>>>(dptm.as<double>(), dsrc.as<int>(), p->nblocks, (double)dso,
                                                               dinv.as<double>());
             check_launch("pw_inverse_kernel");
-            pw_min_kernel<<<dim3(256, p->nblocks), 256>>>(dinv.as<double>(), L, V, W, 0, dmin.as<unsigned long long>());
+            pw_min_kernel<<<dim3(256, p->nblocks), 256>>>(dinv.as<double>(), L, V, W, p->mirror,
+                                                          dmin.as<unsigned long long>());
             check_launch("pw_min_kernel");
             p->pw.alloc(sizeof(float) * (size_t)p->nblocks * V * W);
-            pw_write_kernel<<<dim3(256, p->nblocks), 256>>>(dinv.as<double>(), dmin.as<unsigned long long>(), W, V, 0,
-                                                             p->pw.as<float>());
+            pw_write_kernel<<<dim3(256, p->nblocks), 256>>>(dinv.as<double>(), dmin.as<unsigned long long>(), W, V,
+                                                             p->mirror, p->pw.as<float>());
             check_launch("pw_write_kernel");
             JH_HIP(hipDeviceSynchronize());
         } catch (...) {
@@ -554,6 +545,38 @@ jh_status jh_filter_create(jh_filter_plan** plan, int device, int W, int V, int 
             throw;
         }
         *plan = p;
+    });
+}
+
+}  // namespace
+
+extern "C" {
+
+jh_status jh_filter_create(jh_filter_plan** plan, int device, int W, int V, int nviews,
+                           const double* h_ptm, float dso, float halfSampleRate, float cutStart,
+                           float cutEnd, const float* adjPts, const float* adjDB, int nAdj,
+                           int window, int mode)
+{
+    return create_filter(plan, device, W, V, W, 0, nviews, h_ptm, dso, halfSampleRate, cutStart, cutEnd, adjPts,
+                         adjDB, nAdj, window, mode);
+}
+
+jh_status jh_filter_create_ex(jh_filter_plan** plan, int device, int Win, int V, int Wout, int yMirroring,
+                              int nviews, const double* h_ptm, float dso, float halfSampleRate, float cutStart,
+                              float cutEnd, const float* adjPts, const float* adjDB, int nAdj, int window,
+                              int mode)
+{
+    return create_filter(plan, device, Win, V, Wout, yMirroring, nviews, h_ptm, dso, halfSampleRate, cutStart,
+                         cutEnd, adjPts, adjDB, nAdj, window, mode);
+}
+
+jh_status jh_filter_shape(const jh_filter_plan* p, int* Win, int* Wout, int* yMirroring)
+{
+    return guard([&] {
+        require(p != nullptr, "jh_filter_shape: plan is NULL");
+        if (Win) *Win = p->W;
+        if (Wout) *Wout = p->Wout;
+        if (yMirroring) *yMirroring = p->mirror;
     });
 }
 
@@ -595,18 +618,20 @@ jh_status jh_filter_run(jh_filter_plan* p, const float* d_in, float* d_out, int 
         a.Bhl = p->Bhl.get();
         a.out = d_out;
         a.W = p->W; a.V = p->V;
+        a.Wout = p->Wout;
+        a.edge0 = p->mirror ? p->W - 3 : 0;  // FilterCore.cpp:205-215
         a.rows = p->V * nviews;
         a.Kp = p->Kp;
         a.view0 = view0;
         require((size_t)p->V * nviews < (1u << 31), "jh_filter_run: too many rows");
         hipStream_t st = as_stream(stream);
         if (p->mode == JH_FILTER_FP32_VALU) {
-            dim3 grid((p->W + 63) / 64, (a.rows + 63) / 64);
+            dim3 grid((p->Wout + 63) / 64, (a.rows + 63) / 64);
             require(grid.y <= 65535u, "jh_filter_run: too many rows for one launch");
             filter_fp32_kernel<<<grid, 256, 0, st>>>(a);
             check_launch("filter_fp32_kernel");
         } else {
-            dim3 grid((p->W + 127) / 128, (a.rows + 127) / 128);
+            dim3 grid((p->Wout + 127) / 128, (a.rows + 127) / 128);
             require(grid.y <= 65535u, "jh_filter_run: too many rows for one launch");
             if (p->mode == JH_FILTER_FP32) filter_mfma_kernel<kModeF32><<<grid, 256, 0, st>>>(a);
             else if (p->mode == JH_FILTER_MFMA_BF16X3) filter_mfma_kernel<kModeX3><<<grid, 256, 0, st>>>(a);

@@ -167,6 +167,38 @@ inline int fft_length(int W)
     return L;
 }
 
+// Inverse of the 4 x 4 transform matrix of a projection matrix, in FP64:
+// TM = PTM rows 0, 1, the pseudo-depth row (s20 P20, s20 P21, 0, s23 P23)
+// and PTM row 2 (TransformMatrix/MatrixConvertor.cpp:8-27), inverted by
+// Gauss-Jordan elimination with partial pivoting.  inv: row-major 4 x 4.
+__device__ inline void tm_inverse_fp64(const double* P, double dso, double* inv)
+{
+    const double far_ = (1.0 + 1.0 / dso) * dso, near_ = (1.0 - 1.0 / dso) * dso;
+    const double s20 = (far_ + near_) / (far_ - near_);
+    const double s23 = ((far_ + near_) - 2.0 * far_ * near_ / dso) / (far_ - near_) / dso;
+    double A[4][8];
+    for (int c = 0; c < 4; ++c) { A[0][c] = P[c]; A[1][c] = P[4 + c]; A[3][c] = P[8 + c]; }
+    A[2][0] = s20 * P[8]; A[2][1] = s20 * P[9]; A[2][2] = 0.0; A[2][3] = s23 * P[11];
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) A[r][4 + c] = (r == c) ? 1.0 : 0.0;
+    for (int k = 0; k < 4; ++k) {
+        int p = k;
+        for (int r = k + 1; r < 4; ++r)
+            if (fabs(A[r][k]) > fabs(A[p][k])) p = r;
+        if (p != k)
+            for (int c = 0; c < 8; ++c) { double t = A[k][c]; A[k][c] = A[p][c]; A[p][c] = t; }
+        const double d = A[k][k];
+        for (int c = 0; c < 8; ++c) A[k][c] /= d;
+        for (int r = 0; r < 4; ++r) {
+            if (r == k) continue;
+            const double f = A[r][k];
+            for (int c = 0; c < 8; ++c) A[r][c] -= f * A[k][c];
+        }
+    }
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) inv[4 * r + c] = A[r][4 + c];
+}
+
 }  // namespace jh
 
 // Opaque plan types of the C-ABI.

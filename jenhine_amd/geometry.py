@@ -60,17 +60,19 @@ class ReconParams:
     FilterWindow: int = 0  # 0 = reference cut-off semantics, 1 = Shepp-Logan (extension)
 
 
-def circular_ptms(n_views, sod, sdd, n_u, n_v, du, dv=None, start_deg=0.0):
+def circular_ptms(n_views, sod, sdd, n_u, n_v, du, dv=None, start_deg=0.0, u_shift=0.0):
     """FP64 projection matrices of a circular orbit about z.
 
     Source at angle b_k = start + 360 k / n_views on a circle of radius SOD in
     the xy-plane, flat detector at distance SDD from the source, u along the
     in-plane tangent and v along +z.  w = 1 at the iso-centre (P[11] = 1) and
-    the central ray samples the detector centre ((n-1)/2 in both axes)."""
+    the central ray samples the detector centre ((n-1)/2 in both axes), or
+    u_shift pixels left of it (an offset, half-fan detector that extends to
+    the right of the central ray when u_shift > 0)."""
     dv = du if dv is None else dv
     ku = sdd / (sod * du)
     kv = sdd / (sod * dv)
-    cu = (n_u - 1) / 2.0 - 0.5
+    cu = (n_u - 1) / 2.0 - 0.5 - u_shift
     cv = (n_v - 1) / 2.0 - 0.5
     P = np.zeros((n_views, 3, 4))
     for k in range(n_views):

@@ -6,7 +6,9 @@ The class names follow the reference cores they replace:
 * ``PreProcessingCore`` -- PreProcessingAgent/PreProcessingCore.cpp
 * ``FilterCore``        -- FilterAgent/FilterCore.cpp (+ PreFilterWeightGenerator)
 * ``BPCore``            -- BPAgent/BPCore.cpp and BPCUDAAgent/BPCUDACore.cpp
-* ``FDK``               -- the chain PreProcessingCore -> FilterCore -> BPCore
+* ``CompositeCore``     -- CompositeAgent/CompositeCore.cpp (offset detectors)
+* ``FDK``               -- the chain PreProcessingCore [-> CompositeCore] ->
+                           FilterCore -> BPCore
 
 All work is enqueued on torch's current stream.  Every method raises on a
 failed call; none falls back to a CPU path.
@@ -85,11 +87,18 @@ class PreProcessingCore:
 
 
 class FilterCore:
-    """Row ramp filter with pre-filter weight (FilterCore.cpp:13-222)."""
+    """Row ramp filter with pre-filter weight (FilterCore.cpp:13-222).
+
+    W is the input width (FilterCore's ``width``); w_out and y_mirroring are
+    its ``widthOut`` and ``yMirroring`` (defaults: W, False -- the plain
+    chain; the offset-detector chain uses 2 NumUsedDetsU in, NumUsedDetsU
+    out, yMirroring = detectorOnTheRight, JEngine.cpp:223-238)."""
 
     def __init__(self, W, V, ptm, dso, half_sample_rate, cut_start=0.0, cut_end=0.0, adj_pts=(),
-                 adj_db=(), window=0, mode="fp32", device="cuda:0"):
+                 adj_db=(), window=0, mode="fp32", device="cuda:0", w_out=None, y_mirroring=False):
         self.W, self.V = W, V
+        self.w_out = W if w_out is None else int(w_out)
+        self.y_mirroring = bool(y_mirroring)
         ptm = np.ascontiguousarray(ptm, np.float64).reshape(-1, 12)
         self.n_views = ptm.shape[0]
         m = {"fp32": _lib.JH_FILTER_FP32, "mfma": _lib.JH_FILTER_MFMA_BF16X3, "mfma6": _lib.JH_FILTER_MFMA_BF16X6,
@@ -98,10 +107,10 @@ class FilterCore:
         pts = (ctypes.c_float * max(n, 1))(*adj_pts)
         db = (ctypes.c_float * max(n, 1))(*adj_db)
         h = ctypes.c_void_p()
-        call("jh_filter_create", ctypes.byref(h), _dev_index(device), W, V, self.n_views,
-             ptm.ctypes.data_as(ctypes.c_void_p), ctypes.c_float(dso), ctypes.c_float(half_sample_rate),
-             ctypes.c_float(cut_start), ctypes.c_float(cut_end), pts if n else None, db if n else None, n,
-             window, m)
+        call("jh_filter_create_ex", ctypes.byref(h), _dev_index(device), W, V, self.w_out, int(self.y_mirroring),
+             self.n_views, ptm.ctypes.data_as(ctypes.c_void_p), ctypes.c_float(dso),
+             ctypes.c_float(half_sample_rate), ctypes.c_float(cut_start), ctypes.c_float(cut_end),
+             pts if n else None, db if n else None, n, window, m)
         self._h = h
 
     def response(self):
@@ -117,9 +126,11 @@ class FilterCore:
 
     def process(self, frames, view0=0, out=None):
         _require_cuda(frames, "frames", torch.float32)
+        if frames.dim() != 3 or frames.shape[1:] != (self.V, self.W):
+            raise ValueError(f"frames must be (N, {self.V}, {self.W})")
         n = frames.shape[0]
         if out is None:
-            out = torch.empty_like(frames)
+            out = torch.empty((n, self.V, self.w_out), dtype=torch.float32, device=frames.device)
         _require_cuda(out, "out", torch.float32)
         call("jh_filter_run", self._h, _ptr(frames), _ptr(out), view0, n, _stream())
         return out
@@ -130,6 +141,77 @@ class FilterCore:
             self._h = None
             try:
                 _lib.lib().jh_filter_destroy(h)
+            except Exception:  # interpreter shutdown: the library may be gone already
+                pass
+
+
+def detector_on_right(ptm0, n_used_u):
+    """JEngine.cpp:153-161 (through the library's jh_detector_on_right)."""
+    P = np.ascontiguousarray(ptm0, np.float64).reshape(12)
+    return bool(_lib.lib().jh_detector_on_right(P.ctypes.data_as(ctypes.c_void_p), int(n_used_u)))
+
+
+class CompositeCore:
+    """Offset-detector composite (CompositeAgent/CompositeCore.cpp:13-150):
+    each view (W x V) is widened to 2W columns with its conjugate view
+    (v + N/2) % N re-sampled through the iso-centre plane."""
+
+    def __init__(self, W, V, ptm, dso, detector_on_right=None, device="cuda:0"):
+        ptm = np.ascontiguousarray(ptm, np.float64).reshape(-1, 12)
+        self.W, self.V, self.n_views = W, V, ptm.shape[0]
+        side = -1 if detector_on_right is None else int(bool(detector_on_right))
+        h = ctypes.c_void_p()
+        call("jh_composite_create", ctypes.byref(h), _dev_index(device), W, V, self.n_views,
+             ptm.ctypes.data_as(ctypes.c_void_p), ctypes.c_float(dso), side)
+        self._h = h
+        s = ctypes.c_int()
+        call("jh_composite_side", h, ctypes.byref(s))
+        self.detector_on_right = bool(s.value)
+
+    def process_pairs(self, dirc, conj, view0=0, out=None):
+        """dirc, conj: (n, V, W) frames of views view0.. and their conjugates.
+        Each frame must be contiguous; the frames may be strided alike (e.g.
+        the two halves of a (n, 2, V, W) pipe of interleaved pairs)."""
+        for t, name in ((dirc, "dirc"), (conj, "conj")):
+            if t.dim() != 3 or t.shape[1:] != (self.V, self.W):
+                raise ValueError(f"{name} must be (n, {self.V}, {self.W})")
+            if not t.is_cuda or t.dtype != torch.float32:
+                raise ValueError(f"{name} must be a float32 device tensor")
+            if t.stride(2) != 1 or t.stride(1) != self.W or (t.shape[0] > 1 and t.stride(0) < self.V * self.W):
+                raise ValueError(f"{name}: every frame must be contiguous")
+        n = dirc.shape[0]
+        if conj.shape[0] != n:
+            raise ValueError("dirc and conj hold different frame counts")
+        stride = dirc.stride(0) if n > 1 else self.V * self.W
+        if n > 1 and conj.stride(0) != stride:
+            raise ValueError("dirc and conj frames must have the same stride")
+        if out is None:
+            out = torch.empty((n, self.V, 2 * self.W), dtype=torch.float32, device=dirc.device)
+        _require_cuda(out, "out", torch.float32)
+        call("jh_composite_run", self._h, _ptr(dirc), _ptr(conj), ctypes.c_longlong(stride), _ptr(out),
+             view0, n, _stream())
+        return out
+
+    def process(self, frames, out=None):
+        """frames: all N views (N, V, W) resident on the device -> (N, V, 2W)."""
+        _require_cuda(frames, "frames", torch.float32)
+        N = self.n_views
+        if frames.dim() != 3 or frames.shape != (N, self.V, self.W):
+            raise ValueError(f"frames must be ({N}, {self.V}, {self.W})")
+        if out is None:
+            out = torch.empty((N, self.V, 2 * self.W), dtype=torch.float32, device=frames.device)
+        half = N // 2
+        # views [0, N/2) pair with [N/2, N), views [N/2, N) with [0, N/2)
+        self.process_pairs(frames[:half], frames[half:], 0, out[:half])
+        self.process_pairs(frames[half:], frames[:half], half, out[half:])
+        return out
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._h = None
+            try:
+                _lib.lib().jh_composite_destroy(h)
             except Exception:  # interpreter shutdown: the library may be gone already
                 pass
 
@@ -292,21 +374,29 @@ def ctnum(x, norm0=1.0, norm1=1.0, mu_water=0.02):
 
 
 class FDK:
-    """PreProcessingCore -> FilterCore [-> ZFilterAgent] -> BPCore [->
-    CTNumAgent] on one device.  zfilter and ct (norm0, norm1, mu_water) are
-    fused into the backprojector's staging and normalisation."""
+    """PreProcessingCore [-> CompositeCore] -> FilterCore [-> ZFilterAgent]
+    -> BPCore [-> CTNumAgent] on one device.  zfilter and ct (norm0, norm1,
+    mu_water) are fused into the backprojector's staging and normalisation.
+    composite=True is JEngine's offset-detector chain (JEngine.cpp:152-238):
+    every view composed with its conjugate into 2W columns, filtered back to
+    W with yMirroring = detectorOnTheRight."""
 
     def __init__(self, scan, recon, ptm, filter_mode="fp32", storage="f32", device="cuda:0", zfilter=False,
-                 ct=None):
+                 ct=None, composite=False):
         self.scan, self.recon, self.storage = scan, recon, storage
         W, V = scan.NumUsedDetsU, scan.NumUsedDetsV
         self.pre = PreProcessingCore(scan.NumDetsU, scan.NumDetsV,
                                      (scan.BorderSizeUp, scan.BorderSizeDown, scan.BorderSizeLeft,
                                       scan.BorderSizeRight), W, V, scan.BrightField,
                                      scan.BeamHardeningParams, device)
-        self.filter = FilterCore(W, V, ptm, scan.DSO, scan.HalfSampleRate, recon.FilterCutOffStart,
+        self.composite = None
+        win, mirror = W, False
+        if composite:
+            self.composite = CompositeCore(W, V, ptm, scan.DSO, None, device)
+            win, mirror = 2 * W, self.composite.detector_on_right
+        self.filter = FilterCore(win, V, ptm, scan.DSO, scan.HalfSampleRate, recon.FilterCutOffStart,
                                  recon.FilterCutOffEnd, recon.FilterAdjustPoints, recon.FilterAdjustLevelInDB,
-                                 recon.FilterWindow, filter_mode, device)
+                                 recon.FilterWindow, filter_mode, device, w_out=W, y_mirroring=mirror)
         self.bp = BPCore(ptm, W, V, recon.NumPixelsX, recon.NumPixelsY, recon.NumPixelsZ, recon.PitchXY,
                          recon.PitchZ, device)
         if zfilter:
@@ -316,11 +406,12 @@ class FDK:
 
     def __call__(self, raw, dark=None, gain=None, return_stages=False):
         p = self.pre.process(raw, dark, gain)
-        f = self.filter.process(p)
+        c = self.composite.process(p) if self.composite is not None else p
+        f = self.filter.process(c)
         q = self.bp.stage(f, self.storage)
         vol = self.bp.run(q, self.storage)
         if return_stages:
-            return vol, dict(pre=p, filt=f, q=q)
+            return vol, dict(pre=p, comp=c if self.composite is not None else None, filt=f, q=q)
         return vol
 
 

@@ -121,43 +121,97 @@ int jo_preprocess(const uint16_t* raw, const uint16_t* dark, const float* gain,
 }
 
 typedef struct {
-    const void* in; const void* pw; const void* H; int W, V, nviews, fp64, nthreads, tid;
+    const void* in; const void* pw; const void* H; int Win, Wout, mirror, V, nviews, fp64, nthreads, tid;
     const int* pw_index; int L; void* out;
 } filt_args;
 
 static void* filt_worker(void* p)
 {
     filt_args* a = (filt_args*)p;
-    const size_t fr = (size_t)a->W * a->V, pwfr = (size_t)a->L * a->V;
+    const size_t fin = (size_t)a->Win * a->V, fout = (size_t)a->Wout * a->V, pwfr = (size_t)a->L * a->V;
     for (int v = a->tid; v < a->nviews; v += a->nthreads) {
         const int b = a->pw_index ? a->pw_index[v] : 0;
         if (a->fp64) {
             const double* pw = a->pw ? (const double*)a->pw + (size_t)b * pwfr : NULL;
-            jo_filter_rows_f64((const double*)a->in + v * fr, pw, (const double*)a->H, a->W, a->V,
-                               (double*)a->out + v * fr);
+            jo_filter_rows_ex_f64((const double*)a->in + v * fin, pw, (const double*)a->H, a->Win, a->Wout,
+                                  a->mirror, a->V, (double*)a->out + v * fout);
         } else {
             const float* pw = a->pw ? (const float*)a->pw + (size_t)b * pwfr : NULL;
-            jo_filter_rows_f32((const float*)a->in + v * fr, pw, (const float*)a->H, a->W, a->V,
-                               (float*)a->out + v * fr);
+            jo_filter_rows_ex_f32((const float*)a->in + v * fin, pw, (const float*)a->H, a->Win, a->Wout,
+                                  a->mirror, a->V, (float*)a->out + v * fout);
         }
     }
     return NULL;
 }
 
-/* Filter nviews frames (V rows x W).  pw holds nblocks pre-weight maps
- * (L x V each) and pw_index[view] picks the map for each view (NULL -> 0);
- * pw may be NULL (no pre-weight). */
-int jo_filter(const void* in, const void* pw, const int* pw_index, const void* H,
-              int W, int V, int nviews, void* out, int fp64, int nthreads)
+/* Filter nviews frames of V rows x Win into V rows x Wout (FilterCore with
+ * widthOut and yMirroring, FilterAgent/FilterCore.cpp:13-61,189-222).  pw
+ * holds nblocks pre-weight maps (L x V each) and pw_index[view] picks the
+ * map for each view (NULL -> 0); pw may be NULL (no pre-weight). */
+int jo_filter_ex(const void* in, const void* pw, const int* pw_index, const void* H,
+                 int Win, int Wout, int mirror, int V, int nviews, void* out, int fp64, int nthreads)
 {
-    if (W < 3 || V < 1) return -1;
+    if (Win < 3 || V < 1 || Wout < 1 || Wout * (mirror ? 2 : 1) > jo_fft_length(Win)) return -1;
     if (nthreads < 1) nthreads = 1;
     pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * nthreads);
     filt_args* args = (filt_args*)malloc(sizeof(filt_args) * nthreads);
     for (int t = 0; t < nthreads; ++t) {
-        filt_args a = {in, pw, H, W, V, nviews, fp64, nthreads, t, pw_index, jo_fft_length(W), out};
+        filt_args a = {in, pw, H, Win, Wout, mirror, V, nviews, fp64, nthreads, t, pw_index, jo_fft_length(Win), out};
         args[t] = a;
         pthread_create(&th[t], NULL, filt_worker, &args[t]);
+    }
+    for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
+    free(th); free(args);
+    return 0;
+}
+
+/* The oracle chain's filter (Win = Wout = W, yMirroring = false). */
+int jo_filter(const void* in, const void* pw, const int* pw_index, const void* H,
+              int W, int V, int nviews, void* out, int fp64, int nthreads)
+{
+    return jo_filter_ex(in, pw, pw_index, H, W, W, 0, V, nviews, out, fp64, nthreads);
+}
+
+typedef struct {
+    const void* pre; const void* ptm; int N, W, H, detRight, fp64, nthreads, tid; double dso; void* out;
+} comp_args;
+
+static void* comp_worker(void* p)
+{
+    comp_args* a = (comp_args*)p;
+    const size_t fin = (size_t)a->W * a->H, fout = 2 * fin;
+    for (int v = a->tid; v < a->N; v += a->nthreads) {
+        const int c = (v + a->N / 2) % a->N;
+        if (a->fp64) {
+            const double* pre = (const double*)a->pre;
+            const double* P = (const double*)a->ptm;
+            jo_composite_frame_f64(pre + v * fin, pre + c * fin, P + 12 * v, P + 12 * c, a->W, a->H, a->dso,
+                                   a->detRight, (double*)a->out + v * fout);
+        } else {
+            const float* pre = (const float*)a->pre;
+            const float* P = (const float*)a->ptm;
+            jo_composite_frame_f32(pre + v * fin, pre + c * fin, P + 12 * v, P + 12 * c, a->W, a->H,
+                                   (float)a->dso, a->detRight, (float*)a->out + v * fout);
+        }
+    }
+    return NULL;
+}
+
+/* Composite of every view of a full rotation (CompositeAgent.cpp:50-85 with
+ * the frame pairing of JEngine/InputAgent.cpp:47-85): view v is composed with
+ * its conjugate view (v + N/2) % N.  pre: N frames H x W; out: N frames
+ * H x 2W; ptm: N x 12 in the precision of the form. */
+int jo_composite(const void* pre, const void* ptm, int N, int W, int H, double dso, int detRight, void* out,
+                 int fp64, int nthreads)
+{
+    if (W < 40 || H < 2 || N < 2) return -1;
+    if (nthreads < 1) nthreads = 1;
+    pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * nthreads);
+    comp_args* args = (comp_args*)malloc(sizeof(comp_args) * nthreads);
+    for (int t = 0; t < nthreads; ++t) {
+        comp_args a = {pre, ptm, N, W, H, detRight, fp64, nthreads, t, dso, out};
+        args[t] = a;
+        pthread_create(&th[t], NULL, comp_worker, &args[t]);
     }
     for (int t = 0; t < nthreads; ++t) pthread_join(th[t], NULL);
     free(th); free(args);

@@ -422,34 +422,53 @@ void FN(jo_prefilter_weight)(const R* ptm, int W, int V, int detectorOnRight, R 
 /* Row filter                                                                */
 /* ------------------------------------------------------------------------ */
 
-/* Filters V rows of width W (output width W, yMirroring = false, the oracle
- * chain).  Follows FilterAgent/FilterCore.cpp:189-222: copy, zero-pad to L,
- * multiply by the pre-weight row (L values, may be NULL = all ones), write
- * the truncation ramp edge*i/R into the last R = (L-W)/2 samples with
- * edge = mean of the first three UNweighted inputs, rFFT, multiply each CCS
- * bin by H, irFFT (1/L), keep the first W samples. */
-void FN(jo_filter_rows)(const R* in, const R* pw, const R* H, int W, int V, R* out)
+/* Filters V rows of width Win into rows of width Wout.  Follows
+ * FilterAgent/FilterCore.cpp:13-61 (ramp) and :189-222 (ProcessRow with a
+ * pre-weight): copy, zero-pad to L = CalFFTLength(Win), multiply by the
+ * pre-weight row (L values, may be NULL = all ones), write the truncation
+ * ramp with the UNweighted edge mean, rFFT, multiply each CCS bin by H, irFFT
+ * (1/L), keep Wout samples from offset (yMirroring ? Wout : 0).
+ *   yMirroring = 0: ramp i/R into the last R = (L-Win)/2 samples, edge =
+ *                   (x[0] + x[1] + x[2]) / 3;
+ *   yMirroring = 1: ramp (R-1-i)/R right after the data, edge =
+ *                   (x[Win-1] + x[Win-2] + x[Win-3]) / 3.
+ * The oracle chain uses Win = Wout, yMirroring = 0; the offset-detector
+ * chain of JEngine/JEngine.cpp:223-238 uses Win = 2 Wout, yMirroring =
+ * detectorOnTheRight. */
+void FN(jo_filter_rows_ex)(const R* in, const R* pw, const R* H, int Win, int Wout, int mirror, int V, R* out)
 {
     int L = 1;
-    while (L < W * 2 - 1) L *= 2;
-    const int Rn = (L - W) / 2;
+    while (L < Win * 2 - 1) L *= 2;
+    const int Rn = (L - Win) / 2;
+    const int off = mirror ? Wout : 0;
     R* buf = (R*)malloc(sizeof(R) * L);
     R* ccs = (R*)malloc(sizeof(R) * (L + 2));
     R* ramp = (R*)malloc(sizeof(R) * (Rn > 0 ? Rn : 1));
-    for (int i = 0; i < Rn; ++i) ramp[i] = (R)i / (R)Rn;
+    for (int i = 0; i < Rn; ++i) ramp[i] = mirror ? (R)(Rn - 1 - i) / (R)Rn : (R)i / (R)Rn;
     for (int r = 0; r < V; ++r) {
-        const R* x = in + (size_t)r * W;
-        for (int i = 0; i < W; ++i) buf[i] = x[i];
-        for (int i = W; i < L; ++i) buf[i] = 0;
+        const R* x = in + (size_t)r * Win;
+        for (int i = 0; i < Win; ++i) buf[i] = x[i];
+        for (int i = Win; i < L; ++i) buf[i] = 0;
         if (pw) { const R* p = pw + (size_t)r * L; for (int i = 0; i < L; ++i) buf[i] = buf[i] * p[i]; }
-        R edge = (x[0] + x[1] + x[2]) / (R)3;
-        for (int i = 0; i < Rn; ++i) buf[L - Rn + i] = ramp[i] * edge;
+        if (mirror) {
+            R edge = (x[Win - 1] + x[Win - 2] + x[Win - 3]) / (R)3;
+            for (int i = 0; i < Rn; ++i) buf[Win + i] = ramp[i] * edge;
+        } else {
+            R edge = (x[0] + x[1] + x[2]) / (R)3;
+            for (int i = 0; i < Rn; ++i) buf[L - Rn + i] = ramp[i] * edge;
+        }
         FN(jo_rfft)(L, buf, ccs);
         for (int k = 0; k <= L / 2; ++k) { ccs[2 * k] *= H[k]; ccs[2 * k + 1] *= H[k]; }
         FN(jo_irfft)(L, ccs, buf);
-        memcpy(out + (size_t)r * W, buf, sizeof(R) * W);
+        memcpy(out + (size_t)r * Wout, buf + off, sizeof(R) * Wout);
     }
     free(buf); free(ccs); free(ramp);
+}
+
+/* The oracle chain's filter: Win = Wout = W, yMirroring = false. */
+void FN(jo_filter_rows)(const R* in, const R* pw, const R* H, int W, int V, R* out)
+{
+    FN(jo_filter_rows_ex)(in, pw, H, W, W, 0, V, out);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -705,6 +724,100 @@ void FN(jo_ctnum)(R* inout, size_t n, R norm0, R norm1, R muWater)
             x = y < (R)8191 ? y : (R)8191;
         }
         inout[t] = x;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Offset-detector composite                                                 */
+/* ------------------------------------------------------------------------ */
+
+/* Transition masks of CompositeCore (CompositeAgent/CompositeCore.cpp:
+ * 202-231) over transWidth = 40 columns: dirc[i] = 0.5 -/+ 0.5 cos(i pi / 40)
+ * (minus when the detector is on the right), conj[i] = 1 - dirc[i]. */
+void FN(jo_composite_masks)(int detRight, R* dirc, R* conj)
+{
+    const int tw = 40;
+    for (int i = 0; i < tw; ++i) {
+        const R c = COS((R)i / (R)tw * PI_R);
+        dirc[i] = detRight ? c * (R)-0.5 + (R)0.5 : c * (R)0.5 + (R)0.5;
+        conj[i] = (R)1 - dirc[i];
+    }
+}
+
+/* CompositeCore::Process (CompositeAgent/CompositeCore.cpp:13-150) for one
+ * view: out (H rows x 2W) = [conj half | direct half] (direct half on the
+ * right when detRight, else on the left).
+ *   - TM of the direct view (MatrixConvertor.cpp:8-27), inverted by 4 solves
+ *     against the identity (LAPACK sgesv, CompositeCore.cpp:194-199);
+ *   - every pixel (c, r) of the conj half is the detector point
+ *     (u1, v1, 0, 1) with u1 = c - W (detRight) or c + W, v1 = r
+ *     (GenerateUVW1Grid, :152-183); xyz = TM^-1 (u1, v1, 0, 1) and
+ *     (U, V, W) = P_conj xyz (the two sgemm calls, sums in k order);
+ *   - u2 = U / W, v2 = V / W clamped to [0, H-1]; bilinear remap of the
+ *     conjugate frame at (u2, v2), a sample outside the frame leaves 0;
+ *   - the 40 conj columns next to the direct half: out * maskConj +
+ *     edge * maskDirc, edge = mean of the 5 direct pixels at the shared
+ *     border (first 5 when detRight, else last 5);
+ *   - the direct frame is copied into its half. */
+void FN(jo_composite_frame)(const R* dirc, const R* conj, const R* ptm_d, const R* ptm_c, int W, int H,
+                            R DSO, int detRight, R* out)
+{
+    const int W2 = 2 * W, tw = 40, ew = 5;
+    for (size_t i = 0; i < (size_t)W2 * H; ++i) out[i] = 0;
+    const R far_ = ((R)1 + (R)1 / DSO) * DSO;
+    const R near_ = ((R)1 - (R)1 / DSO) * DSO;
+    const R s20 = (far_ + near_) / (far_ - near_);
+    const R s23 = ((far_ + near_) - (R)2 * far_ * near_ / DSO) / (far_ - near_) / DSO;
+    R tm[16];
+    memcpy(tm, ptm_d, 8 * sizeof(R));
+    tm[8] = s20 * ptm_d[8]; tm[9] = s20 * ptm_d[9]; tm[10] = 0; tm[11] = s23 * ptm_d[11];
+    memcpy(tm + 12, ptm_d + 8, 4 * sizeof(R));
+    R inv[16];
+    for (int c = 0; c < 4; ++c) {
+        R A[16], b[4] = {0, 0, 0, 0};
+        memcpy(A, tm, sizeof(A));
+        b[c] = 1;
+        FN(solve4)(A, b);
+        for (int r = 0; r < 4; ++r) inv[r * 4 + c] = b[r];
+    }
+    const int conjOff = detRight ? 0 : W;
+    const int dircOff = detRight ? W : 0;
+    const int transOff = detRight ? W - tw : W;
+    R md[40], mc[40];
+    FN(jo_composite_masks)(detRight, md, mc);
+    for (int r = 0; r < H; ++r) {
+        R* o = out + (size_t)r * W2;
+        for (int c = 0; c < W; ++c) {
+            const R g[4] = {detRight ? (R)c - (R)W : (R)(c + W), (R)r, (R)0, (R)1};
+            R xyz[4], uvw[3];
+            for (int i = 0; i < 4; ++i) {
+                R t = inv[i * 4] * g[0];
+                for (int k = 1; k < 4; ++k) t = t + inv[i * 4 + k] * g[k];
+                xyz[i] = t;
+            }
+            for (int i = 0; i < 3; ++i) {
+                R t = ptm_c[i * 4] * xyz[0];
+                for (int k = 1; k < 4; ++k) t = t + ptm_c[i * 4 + k] * xyz[k];
+                uvw[i] = t;
+            }
+            const R u2 = uvw[0] / uvw[2];
+            R v2 = uvw[1] / uvw[2];
+            if (v2 < (R)0) v2 = (R)0;
+            if (v2 > (R)(H - 1)) v2 = (R)(H - 1);
+            R val;
+            if (FN(remap_sample)(conj, W, H, u2, v2, &val)) o[conjOff + c] = val;
+        }
+        const R* d = dirc + (size_t)r * W;
+        const R* e = detRight ? d : d + W - ew;
+        R sum = e[0];
+        for (int i = 1; i < ew; ++i) sum = sum + e[i];
+        const R edge = sum / (R)ew;
+        for (int i = 0; i < tw; ++i) {
+            const R a = o[transOff + i] * mc[i];
+            const R b = edge * md[i];
+            o[transOff + i] = a + b;
+        }
+        for (int c = 0; c < W; ++c) o[dircOff + c] = d[c];
     }
 }
 

@@ -145,15 +145,47 @@ def preprocess(raw, out_hw=None, borders=(0, 0, 0, 0), bright_field=4000.0, bh=(
     return out
 
 
-def filter_rows(frames, H, preweights=None, pw_index=None, fp64=False, nthreads=8):
-    """frames: (N, V, W). preweights: (nblocks, V, L) or None."""
+def filter_rows(frames, H, preweights=None, pw_index=None, fp64=False, nthreads=8, w_out=None, mirror=False):
+    """frames: (N, V, Win). preweights: (nblocks, V, L) or None.  w_out and
+    mirror are FilterCore's widthOut and yMirroring (default: Win, False)."""
     frames = np.ascontiguousarray(frames, _dt(fp64))
     N, V, W = frames.shape
+    w_out = W if w_out is None else int(w_out)
     H = np.ascontiguousarray(H, _dt(fp64))
-    out = np.zeros_like(frames)
+    out = np.zeros((N, V, w_out), _dt(fp64))
     pw = None if preweights is None else np.ascontiguousarray(preweights, _dt(fp64))
     idx = None if pw_index is None else np.ascontiguousarray(pw_index, np.int32)
-    rc = lib().jo_filter(_p(frames), _p(pw), _p(idx), _p(H), W, V, N, _p(out), int(fp64), nthreads)
+    rc = lib().jo_filter_ex(_p(frames), _p(pw), _p(idx), _p(H), W, w_out, int(bool(mirror)), V, N, _p(out),
+                            int(fp64), nthreads)
+    assert rc == 0
+    return out
+
+
+def detector_on_right(ptm0, n_used_u):
+    """JEngine/JEngine.cpp:153-161: the detector is on the right when the
+    iso-centre projects left of the detector middle (float arithmetic)."""
+    P = np.asarray(ptm0, np.float32).reshape(12)
+    origin_u = np.float32(P[3] / P[11])
+    return bool(origin_u < np.float32(n_used_u) / np.float32(2.0))
+
+
+def composite_masks(det_right, fp64=False):
+    """Transition masks (dirc, conj) of CompositeCore over 40 columns."""
+    md = np.zeros(40, _dt(fp64))
+    mc = np.zeros(40, _dt(fp64))
+    getattr(lib(), "jo_composite_masks" + _sfx(fp64))(int(bool(det_right)), _p(md), _p(mc))
+    return md, mc
+
+
+def composite(pre, ptm, dso, det_right, fp64=False, nthreads=8):
+    """CompositeCore over a full rotation: (N, V, W) -> (N, V, 2W), view v
+    composed with its conjugate (v + N/2) % N."""
+    pre = np.ascontiguousarray(pre, _dt(fp64))
+    N, V, W = pre.shape
+    P = np.ascontiguousarray(np.asarray(ptm).reshape(N, 12), _dt(fp64))
+    out = np.zeros((N, V, 2 * W), _dt(fp64))
+    rc = lib().jo_composite(_p(pre), _p(P), N, W, V, ctypes.c_double(float(np.float32(dso)) if not fp64 else dso),
+                            int(bool(det_right)), _p(out), int(fp64), nthreads)
     assert rc == 0
     return out
 
@@ -182,12 +214,15 @@ def backproject(proj, ptm, vol_shape, pitch_xy, pitch_z, z0=0, nz=None, fp64=Fal
 
 
 def fdk(raw, ptm, scan, recon, fp64=False, nthreads=8, z0=0, nz=None, dark=None, gain=None,
-        return_stages=False, zfilter_v=False, ct=None):
-    """Full oracle chain PreProcessingCore -> FilterCore [-> ZFilterAgent] ->
-    BPCore [-> CTNumCore].
+        return_stages=False, zfilter_v=False, ct=None, composite_views=False):
+    """Full oracle chain PreProcessingCore [-> CompositeCore] -> FilterCore
+    [-> ZFilterAgent] -> BPCore [-> CTNumCore].
 
     scan: dict with W, V, bright_field, bh, hsr, dso, borders (optional).
     recon: dict with X, Y, Z, pxy, pz, cut_start, cut_end, window.
+    composite_views: the offset-detector chain of JEngine.cpp:152-238 (each
+    view composed with its conjugate into a 2W-wide frame, filtered with
+    widthOut = W and yMirroring = detectorOnTheRight);
     zfilter_v: apply the 1-2-1 v smoothing between filter and backprojection
     (JEngine.cpp:240-245); ct: (norm0, norm1, mu_water) for the CT-number map
     (JEngine.cpp:297-305)."""
@@ -195,13 +230,17 @@ def fdk(raw, ptm, scan, recon, fp64=False, nthreads=8, z0=0, nz=None, dark=None,
                    bh=scan.get("bh", (1.0, 0.0, 0.0)), dark=dark, gain=gain, fp64=fp64,
                    nthreads=nthreads)
     N, V, W = p.shape
-    H = filter_design(W, scan["hsr"], recon.get("cut_start", 0.0), recon.get("cut_end", 0.0),
+    ptm = np.asarray(ptm).reshape(N, 12)
+    det_right = detector_on_right(ptm[0], W) if composite_views else False
+    fin = composite(p, ptm, scan["dso"], det_right, fp64=fp64, nthreads=nthreads) if composite_views else p
+    Win = fin.shape[2]
+    H = filter_design(Win, scan["hsr"], recon.get("cut_start", 0.0), recon.get("cut_end", 0.0),
                       recon.get("adj_pts", ()), recon.get("adj_db", ()), recon.get("window", 0),
                       fp64=fp64)
     src, blk = preweight_schedule(N)
-    ptm = np.asarray(ptm).reshape(N, 12)
-    pws = np.stack([prefilter_weight(ptm[s], W, V, scan["dso"], fp64=fp64) for s in src])
-    f = filter_rows(p, H, pws, blk, fp64=fp64, nthreads=nthreads)
+    pws = np.stack([prefilter_weight(ptm[s], Win, V, scan["dso"], detector_on_right=det_right, fp64=fp64)
+                    for s in src])
+    f = filter_rows(fin, H, pws, blk, fp64=fp64, nthreads=nthreads, w_out=W, mirror=det_right)
     if zfilter_v:
         f = zfilter(f, fp64=fp64)
     vol = backproject(f, ptm, (recon["Z"], recon["Y"], recon["X"]), recon["pxy"], recon["pz"],
@@ -209,7 +248,8 @@ def fdk(raw, ptm, scan, recon, fp64=False, nthreads=8, z0=0, nz=None, dark=None,
     if ct is not None:
         vol = ctnum(vol, *ct, fp64=fp64)
     if return_stages:
-        return vol, dict(pre=p, H=H, pw=pws, pw_index=blk, filt=f)
+        return vol, dict(pre=p, comp=fin if composite_views else None, det_right=det_right, H=H, pw=pws,
+                         pw_index=blk, filt=f)
     return vol
 
 

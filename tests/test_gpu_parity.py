@@ -434,3 +434,99 @@ def small_data_cache():
     if not _SMALL_DATA:
         _SMALL_DATA.update(SMALL.data())
     return _SMALL_DATA
+
+
+# --------------------------------------------------------------------------
+# Offset-detector composite (CompositeAgent) and the mirrored filter
+# --------------------------------------------------------------------------
+
+def _offset_geometry(right, n_views=16, W=160, V=48):
+    return configs.OFFSET128.scaled(n_views=n_views, n_u=W, n_v=V, u_shift=48.0 if right else -48.0)
+
+
+@pytest.mark.parametrize("right", [True, False])
+def test_composite_kernel_vs_fp64_oracle(gpu, right):
+    """GPU composite (FP64-composed per-view homography, FP32 per pixel) vs
+    the FP64 oracle: direct half bit-exact; conjugate half and transition
+    within 1e-5 rel-RMSE (sample positions agree to ~1e-4 px)."""
+    from jenhine_amd.recon import CompositeCore
+    w = _offset_geometry(right)
+    P = w.ptms()
+    N, V, W = w.n_views, w.n_v, w.n_u
+    pre = (synth.normal(17, (N, V, W)) + 2.0).astype(np.float32)
+    comp = CompositeCore(W, V, P, w.sod, device=gpu)
+    assert comp.detector_on_right == right == o.detector_on_right(P[0], W)
+    got = comp.process(torch.from_numpy(pre).to(gpu)).cpu().numpy()
+    ref = o.composite(pre.astype(np.float64), P, w.sod, right, fp64=True)
+    dsl = slice(W, 2 * W) if right else slice(0, W)
+    csl = slice(0, W) if right else slice(W, 2 * W)
+    np.testing.assert_array_equal(got[:, :, dsl], pre)
+    e = rel_rmse(got[:, :, csl], ref[:, :, csl])
+    both = (got[:, :, csl] != 0) & (ref[:, :, csl] != 0)
+    d = np.abs(got[:, :, csl] - ref[:, :, csl])[both]
+    print("composite", "right" if right else "left", "rel-RMSE", e, "max-abs (both sampled)", d.max())
+    assert e < 1e-5 and d.max() < 1e-3
+    # the zero band (samples beyond the conjugate frame) agrees up to pixels
+    # whose source coordinate sits within rounding of the frame border
+    assert np.count_nonzero((got[:, :, csl] == 0) != (ref[:, :, csl] == 0)) <= N * V
+    # pipe layout: frames interleaved (direct, conjugate) as InputAgent emits them
+    pairs = np.empty((N, 2, V, W), np.float32)
+    for v in range(N):
+        pairs[v, 0] = pre[v]
+        pairs[v, 1] = pre[(v + N // 2) % N]
+    tp = torch.from_numpy(pairs).to(gpu)
+    out = comp.process_pairs(tp[:, 0], tp[:, 1], 0)
+    np.testing.assert_array_equal(out.cpu().numpy(), got)
+
+
+@pytest.mark.parametrize("right", [True, False])
+@pytest.mark.parametrize("mode,tol", [("fp32", 2e-6), ("mfma6", 2e-6), ("mfma", 3e-4), ("fp32valu", 2e-6)])
+def test_filter_widthout_and_mirroring_vs_fp64_oracle(gpu, right, mode, tol):
+    """FilterCore(width = 2W, widthOut = W, yMirroring = detectorOnTheRight)
+    of the offset-detector chain, against the FP64 oracle fed the oracle's own
+    pre-weight maps (shifted u axis)."""
+    from jenhine_amd.recon import FilterCore
+    w = _offset_geometry(right, n_views=12)
+    P = w.ptms()
+    N, V, W = w.n_views, w.n_v, w.n_u
+    scan = w.scan_params()
+    frames = np.ascontiguousarray(synth.uniform(19, (N, V, 2 * W), 0.0, 3.0))
+    H = o.filter_design(2 * W, scan.HalfSampleRate, window=1, fp64=True)
+    src, blk = o.preweight_schedule(N)
+    pws = np.stack([o.prefilter_weight(P[s], 2 * W, V, scan.DSO, detector_on_right=right, fp64=True) for s in src])
+    ref = o.filter_rows(frames.astype(np.float64), H, pws, blk, fp64=True, w_out=W, mirror=right)
+    flt = FilterCore(2 * W, V, P, scan.DSO, scan.HalfSampleRate, window=1, mode=mode, device=gpu, w_out=W,
+                     y_mirroring=right)
+    pw = flt.preweight(10).cpu().numpy()
+    np.testing.assert_allclose(pw, pws[10 // 9][:, :2 * W], rtol=2e-7)
+    out = flt.process(torch.from_numpy(frames).to(gpu)).cpu().numpy()
+    assert out.shape == (N, V, W)
+    e = rel_rmse(out, ref)
+    print("filter", mode, "mirror" if right else "plain", "2W->W rel-RMSE", e)
+    assert e <= tol
+
+
+@pytest.mark.parametrize("form,storage,tol", [("fp32", "f32", 1e-4), ("mfma6", "bf16", 1e-3), ("mfma6", "f16", 1e-3)])
+def test_fdk_offset_detector_chain_vs_oracle(gpu, form, storage, tol):
+    """The offset-detector chain Preprocess -> Composite -> Filter(2W -> W,
+    yMirroring) -> BP (JEngine.cpp:152-305) against the CPU oracle."""
+    from jenhine_amd.recon import FDK
+    w = configs.OFFSET128.scaled(n_views=64)
+    d = w.data()
+    sc, rc = oracle_params(w)
+    fdk = FDK(w.scan_params(), w.recon_params(), d["ptm"], form, storage, device=gpu, composite=True)
+    assert fdk.composite.detector_on_right
+    vol = fdk(_u16(d["raw"], gpu), _u16(d["dark"], gpu), torch.from_numpy(d["gain"]).to(gpu)).cpu().numpy()
+    z0, nz = w.Z // 2 - 4, 8
+    v64 = o.fdk(d["raw"], d["ptm"], sc, rc, fp64=True, dark=d["dark"], gain=d["gain"], composite_views=True,
+                z0=z0, nz=nz)
+    e64 = rel_rmse(vol[z0:z0 + nz], v64)
+    msg = f"offset chain {form}/{storage}: rel-RMSE vs FP64 {e64:.2e}"
+    if storage == "f32":
+        v32 = o.fdk(d["raw"], d["ptm"], sc, rc, fp64=False, dark=d["dark"], gain=d["gain"], composite_views=True,
+                    z0=z0, nz=nz)
+        e32 = rel_rmse(vol[z0:z0 + nz], v32)
+        msg += f", vs FP32-ref {e32:.2e}"
+        assert e32 <= tol
+    print(msg, "max-abs", max_abs(vol[z0:z0 + nz], v64))
+    assert e64 <= tol
