@@ -1,7 +1,8 @@
 // jh_fdk_pipeline: FDK reconstruction of one scan through the agent graph
 //
-//   MemoryInputAgent -> ProjectionUploadAgent -> PreProcessingHIPAgent ->
-//   FilterHIPAgent -> BPHIPAgent -> DeviceMemoryDownloadAgent -> SliceOutputAgent
+//   InputAgent | MemoryInputAgent -> ProjectionUploadAgent -> PreProcessingHIPAgent
+//   [-> CompositeHIPAgent] -> FilterHIPAgent -> BPHIPAgent -> DeviceMemoryDownloadAgent
+//   -> OutputAgent | SliceOutputAgent
 //
 // wired by pipe name ("Proj", "Slice") the way JEngine/JEngine.cpp:185-346
 // wires its agents, with the reference's retry on device shortage
@@ -9,18 +10,27 @@
 // out-of-memory prefix the graph is rebuilt with one more z-slab ("part") and
 // the input is streamed once per part.
 //
-// Input files (little-endian, raw): raw counts N x NumDetsV x NumDetsU u16,
-// matrices N x 12 f64, optional dark (u16) and gain (f32) frames.  Output: the
-// volume Z x Y x X f32.  Parameters are key=value arguments.
+// Input: either the reference's scan folder (input_folder=, input_template=
+// e.g. Proj_%d.prj, u16 frames; ptm_file= PTM.dat text) or raw files
+// (raw= N x NumDetsV x NumDetsU u16, ptm= N x 12 f64); optional dark (u16)
+// and gain (f32) frames.  composite=1 runs the offset-detector chain (each
+// view paired with its conjugate, JEngine.cpp:152-238).  Output: the volume
+// Z x Y x X f32 (out=) and/or slice files (output_folder=, output_template=
+// e.g. slice_%d.slc).  dump= / load= take comma-separated
+// <pipe>_<producer>_<consumer>:<path> items (the keys of the reference's
+// pipe_dump_load.xml, Config/PipeDumpLoadReader.cpp:14-55).  Parameters are
+// key=value arguments.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <iostream>
 #include <map>
+#include <set>
 #include <sstream>
 #include <string>
 
+#include "jh/FileAgents.h"
 #include "jh/HipAgents.h"
 
 using namespace jh;
@@ -39,11 +49,13 @@ std::vector<T> ReadFile(const std::string& path, size_t count)
 }
 
 // Streams the scan from memory, once per part, one shot per pass
-// (JEngine/InputAgent.cpp:39-91 streams files the same way).
+// (JEngine/InputAgent.cpp:39-91 streams files the same way), with each view
+// followed by its conjugate when pairs is set.
 class MemoryInputAgent : public SequentialAgentBase {
 public:
-    MemoryInputAgent(const std::vector<uint16_t>& raw, size_t frame, size_t views, size_t passes)
-        : SequentialAgentBase("MemoryInputAgent", 1), raw_(raw), frame_(frame), views_(views), passes_(passes)
+    MemoryInputAgent(const std::vector<uint16_t>& raw, size_t frame, size_t views, size_t passes, bool pairs)
+        : SequentialAgentBase("MemoryInputAgent", 1), raw_(raw), frame_(frame), views_(views), passes_(passes),
+          pairs_(pairs)
     {
     }
 
@@ -58,15 +70,18 @@ private:
     void WorkFlow0() override
     {
         for (size_t p = 0; p < passes_; ++p)
-            for (size_t v = 0; v < views_; ++v) {
-                auto wt = out_->GetWriteToken(1, v + 1 == views_);
-                ThreadOccupiedScope occ(this);
-                std::memcpy(wt.GetBuffer(0).data(), raw_.data() + v * frame_, frame_ * sizeof(uint16_t));
-            }
+            for (size_t v = 0; v < views_; ++v)
+                for (size_t k = 0; k < (pairs_ ? 2u : 1u); ++k) {
+                    const size_t view = k ? (v + views_ / 2) % views_ : v;
+                    auto wt = out_->GetWriteToken(1, v + 1 == views_ && k + 1 == (pairs_ ? 2u : 1u));
+                    ThreadOccupiedScope occ(this);
+                    std::memcpy(wt.GetBuffer(0).data(), raw_.data() + view * frame_, frame_ * sizeof(uint16_t));
+                }
         out_->Close();
     }
     const std::vector<uint16_t>& raw_;
     const size_t frame_, views_, passes_;
+    const bool pairs_;
     std::shared_ptr<Pipe<UINT16Vec>> out_;
 };
 
@@ -127,6 +142,21 @@ struct Args {
     long I(const std::string& k, const std::string& d = "") const { return std::atol(S(k, d).c_str()); }
 };
 
+// "key:path,key:path" -> map
+std::map<std::string, std::string> ParsePathMap(const std::string& s)
+{
+    std::map<std::string, std::string> m;
+    std::stringstream ss(s);
+    std::string item;
+    while (std::getline(ss, item, ',')) {
+        if (item.empty()) continue;
+        const size_t c = item.find(':');
+        if (c == std::string::npos || c == 0) ThrowException("dump/load item needs <pipe>_<producer>_<consumer>:<path>");
+        m[item.substr(0, c)] = item.substr(c + 1);
+    }
+    return m;
+}
+
 std::vector<float> ParseList(const std::string& s)
 {
     std::vector<float> v;
@@ -186,9 +216,18 @@ int main(int argc, char** argv)
             recon.MuWater = ct[2];
         }
         const int device = (int)a.I("device", "0");
+        const bool composite = a.I("composite", "0") != 0;
         const size_t frame = scan.NumDetsU * scan.NumDetsV;
-        std::vector<uint16_t> raw = ReadFile<uint16_t>(a.S("raw"), frame * scan.NumViews);
-        std::vector<double> ptm = ReadFile<double>(a.S("ptm"), 12 * scan.NumViews);
+        const bool fromFolder = a.kv.count("input_folder") != 0;
+        std::vector<uint16_t> raw;
+        if (!fromFolder) raw = ReadFile<uint16_t>(a.S("raw"), frame * scan.NumViews);
+        std::vector<double> ptm = a.kv.count("ptm_file") ? ReadPTMFile(a.S("ptm_file"), scan.NumViews)
+                                                         : ReadFile<double>(a.S("ptm"), 12 * scan.NumViews);
+        // JEngine.cpp:153-161
+        const bool detectorOnRight = jh_detector_on_right(ptm.data(), (int)scan.NumUsedDetsU) != 0;
+        const auto dumps = ParsePathMap(a.S("dump", ","));
+        const auto loads = ParsePathMap(a.S("load", ","));
+        const bool toFolder = a.kv.count("output_folder") != 0;
         std::vector<uint16_t> dark;
         std::vector<float> gain;
         if (a.kv.count("dark")) dark = ReadFile<uint16_t>(a.S("dark"), frame);
@@ -199,16 +238,33 @@ int main(int argc, char** argv)
         for (size_t parts = (size_t)a.I("parts", "1"); parts <= recon.NumPixelsZ; ++parts) {
             std::vector<std::shared_ptr<AgentBase>> agents;
             auto bp = std::make_shared<BPHIPAgent>(scan, recon, ptm, parts, device);
-            agents.push_back(std::make_shared<MemoryInputAgent>(raw, frame, scan.NumViews, parts));
+            if (fromFolder)
+                agents.push_back(std::make_shared<InputAgent>(scan.NumDetsU, scan.NumDetsV, scan.NumViews, parts,
+                                                              a.S("input_folder"), a.S("input_template"), composite));
+            else
+                agents.push_back(std::make_shared<MemoryInputAgent>(raw, frame, scan.NumViews, parts, composite));
             agents.push_back(std::make_shared<ProjectionUploadAgent>(scan.NumDetsU, scan.NumDetsV));
             agents.push_back(std::make_shared<PreProcessingHIPAgent>(scan, dark, gain, device));
-            agents.push_back(std::make_shared<FilterHIPAgent>(scan, recon, ptm, device));
+            if (composite) agents.push_back(std::make_shared<CompositeHIPAgent>(scan, ptm, detectorOnRight, device));
+            agents.push_back(std::make_shared<FilterHIPAgent>(scan, recon, ptm, device, composite, detectorOnRight));
             agents.push_back(bp);
             agents.push_back(std::make_shared<DeviceMemoryDownloadAgent>(plane));
-            agents.push_back(std::make_shared<SliceOutputAgent>(vol, plane, *bp, parts));
+            if (toFolder)
+                agents.push_back(std::make_shared<OutputAgent>(recon.NumPixelsX, recon.NumPixelsY,
+                                                               a.S("output_folder"), a.S("output_template")));
+            else
+                agents.push_back(std::make_shared<SliceOutputAgent>(vol, plane, *bp, parts));
             try {
                 std::vector<std::shared_ptr<PipeBase>> trunk;
                 for (auto& ag : agents) ag->SetPipes(trunk);
+                // dump / load files by <pipe>_<producer>_<consumer> (JEngine.cpp:349-400)
+                std::set<std::shared_ptr<PipeBase>> pipes;
+                for (auto& ag : agents) pipes.insert(ag->GetConnectedPipes().begin(), ag->GetConnectedPipes().end());
+                for (auto& p : pipes) {
+                    const std::string key = p->GetName() + "_" + p->GetProducer() + "_" + p->GetConsumer();
+                    if (dumps.count(key)) p->SetDump(dumps.at(key));
+                    if (loads.count(key)) p->SetLoad(loads.at(key));
+                }
                 for (auto& ag : agents) ag->GetReady();
             } catch (Exception& e) {
                 if (std::strncmp(e.what(), kDeviceOOMPrefix, std::strlen(kDeviceOOMPrefix)) == 0 ||
@@ -227,10 +283,14 @@ int main(int argc, char** argv)
             for (auto& ag : agents)
                 std::cerr << ag->GetAgentName() << ": occupied " << ag->GetOccupiedTime() << " s, async "
                           << ag->GetAsyncTime() << " s\n";
-            std::cout << "{\"parts\": " << parts << ", \"seconds\": " << span << "}" << std::endl;
-            std::ofstream os(a.S("out"), std::ios::binary);
-            os.write(reinterpret_cast<const char*>(vol.data()), (std::streamsize)(vol.size() * sizeof(float)));
-            if (!os) ThrowException("cannot write " + a.S("out"));
+            std::cout << "{\"parts\": " << parts << ", \"seconds\": " << span << ", \"composite\": "
+                      << (composite ? 1 : 0) << ", \"detector_on_right\": " << (detectorOnRight ? 1 : 0) << "}"
+                      << std::endl;
+            if (!toFolder) {
+                std::ofstream os(a.S("out"), std::ios::binary);
+                os.write(reinterpret_cast<const char*>(vol.data()), (std::streamsize)(vol.size() * sizeof(float)));
+                if (!os) ThrowException("cannot write " + a.S("out"));
+            }
             return 0;
         }
         ThrowException("device memory is short even with one slice per part");

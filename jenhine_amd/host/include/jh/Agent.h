@@ -48,6 +48,10 @@ public:
     virtual size_t GetNumThreads() const = 0;
 
     std::string GetAgentName() const { return name_; }
+    // Every pipe this agent reads, writes or creates (the main program
+    // collects them to list the graph and to set dump/load files,
+    // JEngine.cpp:320-400).
+    const std::set<std::shared_ptr<PipeBase>>& GetConnectedPipes() const { return connected_; }
     // True once a work flow of this agent ended with an exception.
     bool Failed() const { return failed_.load(); }
     double GetOccupiedTime() const { return occupied_.load(); }

@@ -7,6 +7,9 @@
 //                              go up once, so every later stage stays on the device)
 //   PreProcessingHIPAgent      DeviceMemory<u16> -> DeviceMemory<float>
 //                              (replaces PreProcessingAgent/PreProcessingAgent.cpp)
+//   CompositeHIPAgent          DeviceMemory<float> pairs (view, conjugate) ->
+//                              DeviceMemory<float> of 2W columns
+//                              (replaces CompositeAgent/CompositeAgent.cpp)
 //   FilterHIPAgent             DeviceMemory<float> -> DeviceMemory<float>
 //                              (replaces FilterAgent/FilterAgent.cpp)
 //   BPHIPAgent                 DeviceMemory<float> "Proj" -> DeviceMemory<float> "Slice"
@@ -103,6 +106,25 @@ void DeviceMemory<T>::Malloc(size_t n)
 
 }  // namespace hip
 
+// Dump / load payload of a device buffer: copied through host memory
+// (the reference's CUDA specialisation, CUDA/PipeDumpLoadImplCUDA.cpp:15-65).
+template <typename U>
+struct PipeElementIO<hip::DeviceMemory<U>, void> {
+    static constexpr bool kSupported = true;
+    static void Write(std::ostream& os, const hip::DeviceMemory<U>& d)
+    {
+        std::vector<U> h(d.Size());
+        if (!h.empty()) CheckHip(hipMemcpy(h.data(), d.Data(), d.Bytes(), hipMemcpyDeviceToHost), "dump copy");
+        os.write(reinterpret_cast<const char*>(h.data()), (std::streamsize)d.Bytes());
+    }
+    static void Read(std::istream& is, hip::DeviceMemory<U>& d)
+    {
+        std::vector<U> h(d.Size());
+        is.read(reinterpret_cast<char*>(h.data()), (std::streamsize)d.Bytes());
+        if (is && !h.empty()) CheckHip(hipMemcpy(d.Data(), h.data(), d.Bytes(), hipMemcpyHostToDevice), "load copy");
+    }
+};
+
 // Per-agent HIP stream, created on first use on the calling thread.
 class StreamHolder {
 public:
@@ -150,9 +172,34 @@ private:
     StreamHolder stream_;
 };
 
+// Composes every (view, conjugate) pair of the "Proj" pipe into one frame of
+// 2 NumUsedDetsU columns (read size 2, no overlap: CompositeAgent.cpp:22-34).
+// The view of a pair is its start index / 2 modulo the view count.
+class CompositeHIPAgent : public SequentialAgentBase {
+public:
+    CompositeHIPAgent(const ScanParams& scan, const std::vector<double>& ptms, bool detectorOnRight, int device = 0);
+    ~CompositeHIPAgent() override;
+    void GetReady() override;
+
+private:
+    void SetPipesImpl() override;
+    void WorkFlow0() override;
+    ScanParams scan_;
+    std::vector<double> ptms_;
+    bool right_;
+    int device_;
+    jh_composite_plan* plan_ = nullptr;
+    std::shared_ptr<Pipe<hip::DeviceMemory<float>>> in_, out_;
+    StreamHolder stream_;
+};
+
+// With composite = true the input frames are the composite's 2W columns and
+// the filter runs with widthOut = W and yMirroring = detectorOnRight (the
+// FilterAgent arguments of JEngine.cpp:223-238).
 class FilterHIPAgent : public SequentialAgentBase {
 public:
-    FilterHIPAgent(const ScanParams& scan, const ReconParams& recon, const std::vector<double>& ptms, int device = 0);
+    FilterHIPAgent(const ScanParams& scan, const ReconParams& recon, const std::vector<double>& ptms, int device = 0,
+                   bool composite = false, bool detectorOnRight = false);
     ~FilterHIPAgent() override;
     void GetReady() override;
 
@@ -163,6 +210,7 @@ private:
     ReconParams recon_;
     std::vector<double> ptms_;
     int device_;
+    bool composite_, right_;
     jh_filter_plan* plan_ = nullptr;
     std::shared_ptr<Pipe<hip::DeviceMemory<float>>> in_, out_;
     StreamHolder stream_;

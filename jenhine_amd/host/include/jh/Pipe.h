@@ -17,14 +17,24 @@
 // Differences: tokens notify the pipe from their shared control block
 // instead of being polled by use count; a closed pipe whose last shot was
 // never finished drains to PipeClosedAndEmptySignal instead of blocking.
+//
+// Dump and load (Pipeline/PipeBase.h:54-56, PipeBase.cpp:263-268,310-313,
+// PipeDumpLoadImpl.cpp:15-57): SetDump streams every element to a file as
+// its write finishes, in stream order; SetLoad overwrites the new (non-
+// overlap) elements of every read token with the next elements of a file
+// before the reader sees them.  The file holds the elements' payload bytes
+// back to back (PipeElementIO<T>), so a stream dumped from one run replays in
+// another.
 #pragma once
 
 #include <condition_variable>
 #include <cstdint>
 #include <deque>
+#include <fstream>
 #include <memory>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "Common.h"
@@ -32,6 +42,29 @@
 namespace jh {
 
 struct PipeClosedAndEmptySignal {};
+
+// Payload I/O of one pipe element for dump and load files.  Provided for
+// vectors of trivially copyable values (FloatVec, UINT16Vec, ...); other
+// element types specialise it (the device buffers do, in HipAgents.h).
+template <typename T, typename = void>
+struct PipeElementIO {
+    static constexpr bool kSupported = false;
+    static void Write(std::ostream&, const T&) {}
+    static void Read(std::istream&, T&) {}
+};
+
+template <typename U>
+struct PipeElementIO<std::vector<U>, std::enable_if_t<std::is_trivially_copyable<U>::value>> {
+    static constexpr bool kSupported = true;
+    static void Write(std::ostream& os, const std::vector<U>& v)
+    {
+        os.write(reinterpret_cast<const char*>(v.data()), (std::streamsize)(v.size() * sizeof(U)));
+    }
+    static void Read(std::istream& is, std::vector<U>& v)
+    {
+        is.read(reinterpret_cast<char*>(v.data()), (std::streamsize)(v.size() * sizeof(U)));
+    }
+};
 
 class PipeBase {
 public:
@@ -64,6 +97,11 @@ public:
     bool Closed();
     double GetProducerBlockedTime() const { return blocked_; }
 
+    // Dump every finished write to / load every read from a file (see above).
+    // Set before the graph starts; each at most once.
+    virtual void SetDump(const std::string& path) = 0;
+    virtual void SetLoad(const std::string& path) = 0;
+
     // Token bookkeeping shared by the typed tokens.
     struct Span {
         size_t start = 0, size = 0, overlap = 0;
@@ -84,8 +122,16 @@ protected:
     // Called (under the lock) once producer, consumer and template are known.
     virtual void InitBuffers() = 0;
     virtual bool HasTemplate() const = 0;
+    // Element payload of global positions [start, end) to the dump file /
+    // from the load file; called under the lock.
+    virtual void DumpElements(size_t start, size_t end) = 0;
+    virtual void LoadElements(size_t start, size_t end) = 0;
+    void OpenDump(const std::string& path);
+    void OpenLoad(const std::string& path);
     void TryInit();
     size_t bufferSize_ = 0;
+    std::unique_ptr<std::ofstream> dump_;
+    std::unique_ptr<std::ifstream> load_;
 
 private:
     friend struct TokenState;
@@ -115,6 +161,7 @@ private:
     std::deque<Slot> writes_, reads_;
     std::deque<Shot> shots_;
     double blocked_ = 0.0;
+    std::string ioError_;  // a failed dump (raised by the next token request)
     std::mutex m_;
     std::condition_variable cv_;
 };
@@ -211,8 +258,33 @@ public:
         return ReadToken(st, ptrs_.data() + st->span.start % bufferSize_);
     }
 
+    void SetDump(const std::string& path) override
+    {
+        if (!PipeElementIO<T>::kSupported) ThrowExceptionAndLog(GetName() + ": this element type cannot be dumped");
+        OpenDump(path);
+    }
+
+    void SetLoad(const std::string& path) override
+    {
+        if (!PipeElementIO<T>::kSupported) ThrowExceptionAndLog(GetName() + ": this element type cannot be loaded");
+        OpenLoad(path);
+    }
+
 protected:
     bool HasTemplate() const override { return hasTemplate_; }
+    void DumpElements(size_t start, size_t end) override
+    {
+        for (; start != end; ++start) PipeElementIO<T>::Write(*dump_, buffers_[start % bufferSize_]);
+        dump_->flush();
+        if (!*dump_) ThrowException(GetName() + ": writing the dump file failed");
+    }
+    void LoadElements(size_t start, size_t end) override
+    {
+        for (; start != end; ++start) {
+            PipeElementIO<T>::Read(*load_, buffers_[start % bufferSize_]);
+            if (!*load_) ThrowException(GetName() + ": the load file ended early");
+        }
+    }
     void InitBuffers() override
     {
         buffers_.assign(bufferSize_, template_[0]);

@@ -164,12 +164,70 @@ void PreProcessingHIPAgent::WorkFlow0()
 }
 
 // ---------------------------------------------------------------------------
+// CompositeHIPAgent
+// ---------------------------------------------------------------------------
+
+CompositeHIPAgent::CompositeHIPAgent(const ScanParams& scan, const std::vector<double>& ptms, bool detectorOnRight,
+                                     int device)
+    : SequentialAgentBase("CompositeAgent", 1), scan_(scan), ptms_(ptms), right_(detectorOnRight), device_(device)
+{
+    scan_.Derive();
+}
+
+CompositeHIPAgent::~CompositeHIPAgent()
+{
+    if (plan_) jh_composite_destroy(plan_);
+}
+
+void CompositeHIPAgent::SetPipesImpl()
+{
+    in_ = BranchPipeFromTrunk<hip::DeviceMemory<float>>("Proj");
+    in_->SetConsumer(GetAgentName(), 1, 2, 0);
+    out_ = CreateNewPipe<hip::DeviceMemory<float>>("Proj");
+    out_->SetProducer(GetAgentName(), 1, 1);
+    out_->SetTemplate(hip::DeviceMemory<float>(2 * scan_.NumUsedDetsU * scan_.NumUsedDetsV),
+                      {2 * scan_.NumUsedDetsU, scan_.NumUsedDetsV});
+    MergePipeToTrunk(out_);
+}
+
+void CompositeHIPAgent::GetReady()
+{
+    CheckHip(hipSetDevice(device_), "hipSetDevice");
+    CheckJh(jh_composite_create(&plan_, device_, (int)scan_.NumUsedDetsU, (int)scan_.NumUsedDetsV,
+                                (int)(ptms_.size() / 12), ptms_.data(), scan_.DSO, right_ ? 1 : 0),
+            "jh_composite_create");
+}
+
+void CompositeHIPAgent::WorkFlow0()
+{
+    CheckHip(hipSetDevice(device_), "hipSetDevice");
+    const size_t nv = ptms_.size() / 12;
+    try {
+        for (;;) {
+            auto rt = in_->GetReadToken();
+            if (rt.GetSize() != 2) ThrowExceptionAndLog(GetAgentName() + ": a shot holds an odd number of frames");
+            auto wt = out_->GetWriteToken(1, rt.IsShotEnd());
+            ThreadOccupiedScope occ(this);
+            const int view = (int)((rt.GetStartIndex() / 2) % nv);
+            CheckJh(jh_composite_run(plan_, rt.GetBuffer(0).Data(), rt.GetBuffer(1).Data(), 0, wt.GetBuffer(0).Data(),
+                                     view, 1, stream_.Get()),
+                    "jh_composite_run");
+            WaitAsyncScope wait(&occ);
+            stream_.Sync();
+        }
+    } catch (PipeClosedAndEmptySignal&) {
+    }
+    out_->Close();
+}
+
+// ---------------------------------------------------------------------------
 // FilterHIPAgent
 // ---------------------------------------------------------------------------
 
 FilterHIPAgent::FilterHIPAgent(const ScanParams& scan, const ReconParams& recon, const std::vector<double>& ptms,
-                               int device)
-    : SequentialAgentBase("FilterHIPAgent", 1), scan_(scan), recon_(recon), ptms_(ptms), device_(device)
+                               int device, bool composite, bool detectorOnRight)
+    : SequentialAgentBase("FilterHIPAgent", 1), scan_(scan), recon_(recon), ptms_(ptms), device_(device),
+      composite_(composite), right_(composite && detectorOnRight)
 {
     scan_.Derive();
 }
@@ -196,12 +254,13 @@ void FilterHIPAgent::GetReady()
     const int nv = (int)(ptms_.size() / 12);
     const int nAdj = (int)recon_.FilterAdjustPoints.size();
     if (recon_.FilterAdjustLevelInDB.size() != (size_t)nAdj) ThrowException("filter adjust points and levels differ");
-    CheckJh(jh_filter_create(&plan_, device_, (int)scan_.NumUsedDetsU, (int)scan_.NumUsedDetsV, nv, ptms_.data(),
-                             scan_.DSO, scan_.HalfSampleRate, recon_.FilterCutOffStart, recon_.FilterCutOffEnd,
-                             nAdj ? recon_.FilterAdjustPoints.data() : nullptr,
-                             nAdj ? recon_.FilterAdjustLevelInDB.data() : nullptr, nAdj, recon_.FilterWindow,
-                             recon_.FilterMode),
-            "jh_filter_create");
+    const int W = (int)scan_.NumUsedDetsU;
+    CheckJh(jh_filter_create_ex(&plan_, device_, composite_ ? 2 * W : W, (int)scan_.NumUsedDetsV, W, right_ ? 1 : 0,
+                                nv, ptms_.data(), scan_.DSO, scan_.HalfSampleRate, recon_.FilterCutOffStart,
+                                recon_.FilterCutOffEnd, nAdj ? recon_.FilterAdjustPoints.data() : nullptr,
+                                nAdj ? recon_.FilterAdjustLevelInDB.data() : nullptr, nAdj, recon_.FilterWindow,
+                                recon_.FilterMode),
+            "jh_filter_create_ex");
 }
 
 void FilterHIPAgent::WorkFlow0()

@@ -69,6 +69,28 @@ void PipeBase::TryInit()
     cv_.notify_all();
 }
 
+void PipeBase::OpenDump(const std::string& path)
+{
+    std::lock_guard<std::mutex> lk(m_);
+    if (dump_) ThrowExceptionAndLog(name_ + ": dump file is already set.");
+    dump_.reset(new std::ofstream(path, std::ios::binary));
+    if (!*dump_) {
+        dump_.reset();
+        ThrowExceptionAndLog(name_ + ": can not open dump file " + path);
+    }
+}
+
+void PipeBase::OpenLoad(const std::string& path)
+{
+    std::lock_guard<std::mutex> lk(m_);
+    if (load_) ThrowExceptionAndLog(name_ + ": load file is already set.");
+    load_.reset(new std::ifstream(path, std::ios::binary));
+    if (!*load_) {
+        load_.reset();
+        ThrowExceptionAndLog(name_ + ": can not open file: " + path);
+    }
+}
+
 size_t PipeBase::GetWritingPos() { std::lock_guard<std::mutex> lk(m_); return writingPos_; }
 size_t PipeBase::GetWriteDonePos() { std::lock_guard<std::mutex> lk(m_); return writeDonePos_; }
 size_t PipeBase::GetReadingPos() { std::lock_guard<std::mutex> lk(m_); return readingPos_; }
@@ -119,6 +141,7 @@ std::shared_ptr<PipeBase::TokenState> PipeBase::AcquireWrite(size_t size, bool i
     std::unique_lock<std::mutex> lk(m_);
     cv_.wait(lk, [&] { return ready_ || closed_; });
     if (closed_) ThrowExceptionAndLog(name_ + ": trying to get a write token when the pipe is closed");
+    if (!ioError_.empty()) ThrowExceptionAndLog(ioError_);
     if (size == 0 || size > bufferSize_) ThrowExceptionAndLog(name_ + ": write size is 0 or larger than the buffer");
     cv_.wait(lk, [&] { return closed_ || writingPos_ + size <= writeDonePos_ + bufferSize_; });
     if (!closed_ && !RoomToWrite(size)) {
@@ -159,8 +182,12 @@ std::shared_ptr<PipeBase::TokenState> PipeBase::AcquireRead()
         if (closed_ && writes_.empty() && reads_.empty()) throw PipeClosedAndEmptySignal();
         cv_.wait(lk);
     }
+    if (!ioError_.empty()) ThrowExceptionAndLog(ioError_);
     const size_t start = ReadStart();
     const size_t end = ReadEnd();
+    // replace the token's new elements before anyone can see them (the
+    // overlap was loaded with the previous token)
+    if (load_) LoadElements(readingPos_, end);
     auto st = std::make_shared<TokenState>();
     st->pipe = this;
     st->write = false;
@@ -183,10 +210,22 @@ void PipeBase::Release(bool write, uint64_t id)
     std::deque<Slot>& q = write ? writes_ : reads_;
     for (Slot& s : q)
         if (s.id == id) { s.released = true; break; }
+    const size_t doneBefore = writeDonePos_;
     while (!q.empty() && q.front().released) {
         if (write) writeDonePos_ += q.front().size;
         else readDonePos_ = q.front().start + q.front().size;
         q.pop_front();
+    }
+    if (write && dump_ && writeDonePos_ != doneBefore && ioError_.empty()) {
+        // runs from a token's destructor: a failure is kept and raised by the
+        // next token request, and the pipe is closed
+        try {
+            DumpElements(doneBefore, writeDonePos_);
+        } catch (std::exception& e) {
+            ioError_ = e.what();
+            closed_ = true;
+            GLog(ioError_);
+        }
     }
     cv_.notify_all();
 }

@@ -9,7 +9,12 @@
 #include <random>
 #include <thread>
 
+#include <cstdlib>
+#include <fstream>
+#include <unistd.h>
+
 #include "jh/Agent.h"
+#include "jh/FileAgents.h"
 
 using namespace jh;
 
@@ -311,6 +316,269 @@ static void MissingPipeIsReported()
     CHECK(threw);
 }
 
+// ----- dump / load and the on-disk formats ----------------------------------
+
+static std::string TempDir()
+{
+    char tmpl[] = "/tmp/jh_ut_XXXXXX";
+    char* d = mkdtemp(tmpl);
+    CHECK(d != nullptr);
+    return d;
+}
+
+static std::vector<char> Slurp(const std::string& path)
+{
+    std::ifstream is(path, std::ios::binary);
+    return std::vector<char>((std::istreambuf_iterator<char>(is)), std::istreambuf_iterator<char>());
+}
+
+// PipeBase.cpp:263-268,310-313: a dump holds every finished element in
+// stream order; a load replaces each read token's new elements, in order,
+// also when read tokens overlap.
+static void PipeDumpAndLoad()
+{
+    const std::string dir = TempDir();
+    const std::string path = dir + "/proj.dump";
+    const size_t n = 7, len = 5;
+    {
+        Pipe<FloatVec> p("Proj");
+        p.SetTemplate(FloatVec(len), {len});
+        p.SetProducer("w", 1, 1);
+        p.SetConsumer("r", 1, 3, 1);
+        p.SetDump(path);
+        std::thread reader([&] {
+            try {
+                for (;;) auto r = p.GetReadToken();
+            } catch (PipeClosedAndEmptySignal&) {
+            }
+        });
+        for (size_t i = 0; i < n; ++i) {
+            auto w = p.GetWriteToken(1, i + 1 == n);
+            for (size_t k = 0; k < len; ++k) w.GetBuffer(0)[k] = (float)(10 * i + k);
+        }
+        p.Close();
+        reader.join();
+    }
+    const std::vector<char> bytes = Slurp(path);
+    CHECK(bytes.size() == n * len * sizeof(float));
+    const float* f = reinterpret_cast<const float*>(bytes.data());
+    for (size_t i = 0; i < n * len; ++i) CHECK(f[i] == (float)(10 * (i / len) + i % len));
+    // replay: the producer writes zeros, the reader sees the dumped stream
+    Pipe<FloatVec> q("Proj");
+    q.SetTemplate(FloatVec(len), {len});
+    q.SetProducer("w", 1, 1);
+    q.SetConsumer("r", 1, 3, 1);
+    q.SetLoad(path);
+    std::vector<float> seen;
+    std::thread reader([&] {
+        try {
+            for (;;) {
+                auto r = q.GetReadToken();
+                for (size_t i = r.GetOverlapSize(); i < r.GetSize(); ++i) seen.push_back(r.GetBuffer(i)[0]);
+                if (r.GetOverlapSize()) CHECK(r.GetBuffer(0)[0] == seen[seen.size() - r.GetSize()]);
+            }
+        } catch (PipeClosedAndEmptySignal&) {
+        }
+    });
+    for (size_t i = 0; i < n; ++i) {
+        auto w = q.GetWriteToken(1, i + 1 == n);
+        std::fill(w.GetBuffer(0).begin(), w.GetBuffer(0).end(), 0.0f);
+    }
+    q.Close();
+    reader.join();
+    CHECK(seen.size() == n);
+    for (size_t i = 0; i < n; ++i) CHECK(seen[i] == (float)(10 * i));
+    unlink(path.c_str());
+    rmdir(dir.c_str());
+}
+
+static void LoadFileTooShortFailsTheReader()
+{
+    const std::string dir = TempDir();
+    const std::string path = dir + "/short.dump";
+    {
+        std::ofstream os(path, std::ios::binary);
+        const float v[3] = {1, 2, 3};
+        os.write(reinterpret_cast<const char*>(v), sizeof v);
+    }
+    Pipe<FloatVec> q("Proj");
+    q.SetTemplate(FloatVec(2), {2});
+    q.SetProducer("w", 1, 1);
+    q.SetConsumer("r", 1, 1, 0);
+    q.SetLoad(path);
+    for (int i = 0; i < 2; ++i) q.GetWriteToken(1, i == 1);
+    { auto r = q.GetReadToken(); CHECK(r.GetBuffer(0)[0] == 1.0f && r.GetBuffer(0)[1] == 2.0f); }
+    bool threw = false;
+    try {
+        auto r = q.GetReadToken();
+    } catch (Exception& e) {
+        threw = std::string(e.what()).find("ended early") != std::string::npos;
+    }
+    CHECK(threw);
+    unlink(path.c_str());
+    rmdir(dir.c_str());
+}
+
+// JEngine/PTMReader.h:17-23: 12 whitespace-separated numbers per view.
+static void PTMFileRoundTrip()
+{
+    const std::string dir = TempDir();
+    const std::string path = dir + "/PTM.dat";
+    std::vector<double> ptm(24);
+    for (size_t i = 0; i < ptm.size(); ++i) ptm[i] = (i % 5 ? 1.0 : -1.0) * (0.1 + i) / 3.0 * (i == 11 ? 1e-7 : 1.0);
+    WritePTMFile(path, ptm);
+    CHECK(ReadPTMFile(path, 2) == ptm);
+    bool threw = false;
+    try {
+        ReadPTMFile(path, 3);
+    } catch (Exception& e) {
+        threw = std::string(e.what()).find("numbers") != std::string::npos;
+    }
+    CHECK(threw);
+    {
+        std::ofstream os(path);
+        os << "1 2 3 4\n5 6 7 8 9 10 11 x12\n";
+    }
+    threw = false;
+    try {
+        ReadPTMFile(path, 1);
+    } catch (Exception& e) {
+        threw = std::string(e.what()).find("bad number") != std::string::npos;
+    }
+    CHECK(threw);
+    {
+        // the reference's test matrices are written as plain decimals
+        std::ofstream os(path);
+        os << "-1.33108 -7.78814 -0.01779 573.81219\n-0.96645 -0.02576 7.50237 344.55353\n-0.00225 0 0 1\n";
+    }
+    const std::vector<double> r = ReadPTMFile(path, 1);
+    CHECK(r[0] == -1.33108 && r[3] == 573.81219 && r[8] == -0.00225 && r[11] == 1.0);
+    unlink(path.c_str());
+    rmdir(dir.c_str());
+}
+
+static void IndexedFileNames()
+{
+    CHECK(IndexedFileName("/data/scan", "Proj_%d.prj", 7) == "/data/scan/Proj_7.prj");
+    CHECK(IndexedFileName("/data/scan/", "slice_%04d.slc", 12) == "/data/scan/slice_0012.slc");
+    CHECK(IndexedFileName("", "a%%b_%u", 3) == "a%b_3");
+    for (const char* bad : {"Proj.prj", "Proj_%s.prj", "P_%d_%d", "P_%-3d", "P_%"}) {
+        bool threw = false;
+        try {
+            IndexedFileName("d", bad, 1);
+        } catch (Exception&) {
+            threw = true;
+        }
+        CHECK(threw);
+    }
+}
+
+class U16Collector : public SequentialAgentBase {
+public:
+    U16Collector() : SequentialAgentBase("Collector", 1) {}
+    std::vector<unsigned> first;
+    std::vector<size_t> shotEnds;
+
+private:
+    void SetPipesImpl() override
+    {
+        in_ = BranchPipeFromTrunk<UINT16Vec>("Proj");
+        in_->SetConsumer(GetAgentName(), 1, 1, 0);
+    }
+    void WorkFlow0() override
+    {
+        try {
+            for (;;) {
+                auto r = in_->GetReadToken();
+                first.push_back(r.GetBuffer(0)[0]);
+                if (r.IsShotEnd()) shotEnds.push_back(first.size());
+            }
+        } catch (PipeClosedAndEmptySignal&) {
+        }
+    }
+    std::shared_ptr<Pipe<UINT16Vec>> in_;
+};
+
+// InputAgent.cpp:47-85: view v, then its conjugate (v + N/2) % N; one shot
+// per pass, the pipe closed after the last pass.
+static void InputAgentReadsFramesAndPairsConjugates()
+{
+    const std::string dir = TempDir();
+    const size_t N = 6, U = 4, V = 3;
+    for (size_t v = 0; v < N; ++v) {
+        std::vector<uint16_t> fr(U * V, (uint16_t)(100 + v));
+        std::ofstream os(IndexedFileName(dir, "Proj_%d.prj", v), std::ios::binary);
+        os.write(reinterpret_cast<const char*>(fr.data()), (std::streamsize)(fr.size() * 2));
+    }
+    for (bool pairs : {false, true}) {
+        auto in = std::make_shared<InputAgent>(U, V, N, 2, dir, "Proj_%d.prj", pairs);
+        auto col = std::make_shared<U16Collector>();
+        RunGraph({in, col});
+        CHECK(!in->Failed() && !col->Failed());
+        std::vector<unsigned> expect;
+        for (int pass = 0; pass < 2; ++pass)
+            for (size_t v = 0; v < N; ++v) {
+                expect.push_back(100 + (unsigned)v);
+                if (pairs) expect.push_back(100 + (unsigned)((v + N / 2) % N));
+            }
+        CHECK(col->first == expect);
+        const size_t per = pairs ? 2 * N : N;
+        CHECK(col->shotEnds == std::vector<size_t>({per, 2 * per}));
+    }
+    // a missing frame fails the agent and shuts the graph down
+    unlink(IndexedFileName(dir, "Proj_%d.prj", 4).c_str());
+    auto in = std::make_shared<InputAgent>(U, V, N, 1, dir, "Proj_%d.prj", false);
+    auto col = std::make_shared<U16Collector>();
+    RunGraph({in, col});
+    CHECK(in->Failed());
+    for (size_t v = 0; v < N; ++v) unlink(IndexedFileName(dir, "Proj_%d.prj", v).c_str());
+    rmdir(dir.c_str());
+}
+
+class SliceSource : public SequentialAgentBase {
+public:
+    SliceSource(size_t n, size_t len) : SequentialAgentBase("SliceSource", 1), n_(n), len_(len) {}
+
+private:
+    void SetPipesImpl() override
+    {
+        out_ = CreateNewPipe<FloatVec>("Slice");
+        out_->SetTemplate(FloatVec(len_), {len_});
+        out_->SetProducer(GetAgentName(), 1, 1);
+        MergePipeToTrunk(out_);
+    }
+    void WorkFlow0() override
+    {
+        for (size_t i = 0; i < n_; ++i) {
+            auto w = out_->GetWriteToken(1, i + 1 == n_);
+            for (size_t k = 0; k < len_; ++k) w.GetBuffer(0)[k] = (float)(i * 1000 + k);
+        }
+        out_->Close();
+    }
+    size_t n_, len_;
+    std::shared_ptr<Pipe<FloatVec>> out_;
+};
+
+// OutputAgent.cpp:53-76: slice i -> <folder>/<template % i>, X * Y floats.
+static void OutputAgentWritesSliceFiles()
+{
+    const std::string dir = TempDir() + "/out";
+    auto src = std::make_shared<SliceSource>(5, 6);
+    auto out = std::make_shared<OutputAgent>(3, 2, dir, "slice_%d.slc");
+    RunGraph({src, out});
+    CHECK(!out->Failed() && out->SlicesWritten() == 5);
+    for (size_t i = 0; i < 5; ++i) {
+        const std::string name = IndexedFileName(dir, "slice_%d.slc", i);
+        const std::vector<char> b = Slurp(name);
+        CHECK(b.size() == 6 * sizeof(float));
+        const float* f = reinterpret_cast<const float*>(b.data());
+        for (size_t k = 0; k < 6; ++k) CHECK(f[k] == (float)(i * 1000 + k));
+        unlink(name.c_str());
+    }
+    rmdir(dir.c_str());
+    rmdir(dir.substr(0, dir.size() - 4).c_str());
+}
+
 int main()
 {
     Run("TokenSizesAndShots", TokenSizesAndShots);
@@ -321,6 +589,12 @@ int main()
     Run("FailingSourceShutsDownGraph", FailingSourceShutsDownGraph);
     Run("FailingWorkerShutsDownGraph", FailingWorkerShutsDownGraph);
     Run("MissingPipeIsReported", MissingPipeIsReported);
+    Run("PipeDumpAndLoad", PipeDumpAndLoad);
+    Run("LoadFileTooShortFailsTheReader", LoadFileTooShortFailsTheReader);
+    Run("PTMFileRoundTrip", PTMFileRoundTrip);
+    Run("IndexedFileNames", IndexedFileNames);
+    Run("InputAgentReadsFramesAndPairsConjugates", InputAgentReadsFramesAndPairsConjugates);
+    Run("OutputAgentWritesSliceFiles", OutputAgentWritesSliceFiles);
     std::printf("%d failed\n", g_failed);
     return g_failed ? 1 : 0;
 }

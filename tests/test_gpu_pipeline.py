@@ -1,7 +1,9 @@
 """The C++ agent graph on the GPU (jh_fdk_pipeline): upload -> preprocess ->
-filter -> BPHIPAgent -> download, against the Python path over the same C-ABI
-and against the CPU oracle; and the retry with more z-slabs when set-up runs
-out of device memory (forced with the JH_MAX_DEVICE_BYTES test hook)."""
+[composite ->] filter -> BPHIPAgent -> download, against the Python path over
+the same C-ABI and against the CPU oracle; the retry with more z-slabs when
+set-up runs out of device memory (forced with the JH_MAX_DEVICE_BYTES test
+hook); the reference's scan-folder and slice-file formats; pipe dump and
+replay."""
 import json
 import os
 import subprocess
@@ -21,7 +23,8 @@ APP = os.path.join(ROOT, "jenhine_amd", "host", "build", "jh_fdk_pipeline")
 W = configs.CBCT128.scaled(n_views=24, X=64, Y=64, Z=48, pxy=0.5, pz=0.5)
 
 
-def _run(tmp_path, d, storage=0, filt=0, chunk=5, env=None, parts=1, extra=()):
+def _run(tmp_path, d, storage=0, filt=0, chunk=5, env=None, parts=1, extra=(), w=None, read_volume=True):
+    W = w or globals()["W"]
     paths = {}
     for k, dt in (("raw", np.uint16), ("ptm", np.float64), ("dark", np.uint16), ("gain", np.float32)):
         paths[k] = str(tmp_path / f"{k}.bin")
@@ -41,6 +44,8 @@ def _run(tmp_path, d, storage=0, filt=0, chunk=5, env=None, parts=1, extra=()):
     print(r.stdout, r.stderr[-2000:])
     assert r.returncode == 0, r.stderr[-2000:]
     info = json.loads(r.stdout.strip().splitlines()[-1])
+    if not read_volume:
+        return None, info
     return np.fromfile(out, np.float32).reshape(W.Z, W.Y, W.X), info
 
 
@@ -97,3 +102,68 @@ def test_agent_graph_production_chain_zfilter_ct(gpu, data, tmp_path):
     e = rel_rmse(vol, v32)
     print("agent graph zfilter+ct vs FP32-ref oracle", e)
     assert e <= 1e-4
+
+
+WOFF = configs.OFFSET128.scaled(n_views=24, X=64, Y=64, Z=32, pxy=0.5, pz=0.5)
+
+
+@pytest.fixture(scope="module")
+def offset_data():
+    return WOFF.data()
+
+
+def test_agent_graph_composite_chain(gpu, offset_data, tmp_path):
+    """InputAgent pairs -> CompositeAgent -> FilterAgent(2W -> W, yMirroring)
+    (JEngine.cpp:152-238) against the Python path and the oracle."""
+    vol, info = _run(tmp_path, offset_data, w=WOFF, extra=["composite=1"])
+    assert info["composite"] == 1 and info["detector_on_right"] == 1
+    from jenhine_amd.recon import FDK
+    fdk = FDK(WOFF.scan_params(), WOFF.recon_params(), offset_data["ptm"], "fp32", "f32", gpu, composite=True)
+    ref = fdk(torch.from_numpy(offset_data["raw"].view(np.int16)).to(gpu),
+              torch.from_numpy(offset_data["dark"].view(np.int16)).to(gpu),
+              torch.from_numpy(offset_data["gain"]).to(gpu)).cpu().numpy()
+    assert rel_rmse(vol, ref) <= 1e-6
+    sc, rc = oracle_params(WOFF)
+    v32 = o.fdk(offset_data["raw"], offset_data["ptm"], sc, rc, dark=offset_data["dark"], gain=offset_data["gain"],
+                composite_views=True)
+    e = rel_rmse(vol, v32)
+    print("agent graph composite chain vs FP32-ref oracle", e)
+    assert e <= 1e-4
+
+
+def test_agent_graph_reads_scan_folder_and_writes_slice_files(gpu, data, tmp_path):
+    """Proj_%d.prj frames + PTM.dat in, slice_%d.slc files out (InputAgent,
+    PTMReader, OutputAgent formats): the same volume as the in-memory run."""
+    from jenhine_amd import io as jio
+    ref, _ = _run(tmp_path, data)
+    folder = tmp_path / "scan"
+    jio.write_projection_folder(str(folder), data["raw"])
+    jio.write_ptm_dat(str(folder / "PTM.dat"), data["ptm"])
+    np.testing.assert_array_equal(jio.read_ptm_dat(str(folder / "PTM.dat"), W.n_views), data["ptm"])
+    outdir = tmp_path / "slices"
+    _, info = _run(tmp_path, data, read_volume=False,
+                   extra=[f"input_folder={folder}", "input_template=Proj_%d.prj", f"ptm_file={folder / 'PTM.dat'}",
+                          f"output_folder={outdir}", "output_template=slice_%d.slc"])
+    vol = jio.read_slices(str(outdir), W.Z, W.X, W.Y)
+    np.testing.assert_array_equal(vol, ref)
+
+
+def test_pipe_dump_and_replay(gpu, data, tmp_path):
+    """Dump the filtered-projection pipe of one run, replay it into another
+    whose raw input is blanked: the replay reconstructs the first run's volume
+    (PipeBase.cpp:263-268,310-313; keys of Config/PipeDumpLoadReader.cpp)."""
+    key = "Proj_FilterHIPAgent_BPHIPAgent"
+    dump = tmp_path / "filtered.dump"
+    ref, _ = _run(tmp_path, data, extra=[f"dump={key}:{dump}"])
+    assert dump.stat().st_size == W.n_views * W.n_v * W.n_u * 4
+    from jenhine_amd.recon import FDK
+    fdk = FDK(W.scan_params(), W.recon_params(), data["ptm"], "fp32", "f32", gpu)
+    _, st = fdk(torch.from_numpy(data["raw"].view(np.int16)).to(gpu),
+                torch.from_numpy(data["dark"].view(np.int16)).to(gpu), torch.from_numpy(data["gain"]).to(gpu),
+                return_stages=True)
+    filt = np.fromfile(dump, np.float32).reshape(W.n_views, W.n_v, W.n_u)
+    np.testing.assert_array_equal(filt, st["filt"].cpu().numpy())
+    blank = dict(data)
+    blank["raw"] = np.zeros_like(data["raw"])
+    vol, _ = _run(tmp_path, blank, extra=[f"load={key}:{dump}"])
+    np.testing.assert_array_equal(vol, ref)
