@@ -64,7 +64,7 @@ def main():
                     help="profiling run: a 1 GiB copy, then only the backprojection kernel")
     ap.add_argument("--kernel-form", choices=("fp32", "bf16", "f16"), default="bf16",
                     help="storage form of the --kernel-only backprojection")
-    ap.add_argument("--workload", choices=("dental512", "batch16"), default="dental512")
+    ap.add_argument("--workload", choices=("dental512", "batch16", "offset_dental512"), default="dental512")
     ap.add_argument("--branches", type=int, default=4, help="parallel graph branches (batch16)")
     ap.add_argument("--batch-storage", choices=("bf16", "f16", "f32"), default="bf16",
                     help="projection storage of the batch16 scans")
@@ -83,6 +83,8 @@ def main():
     dev = torch.device("cuda:0")
     if args.workload == "batch16":
         return run_batch(args, dev)
+    if args.workload == "offset_dental512":
+        return run_offset(args, dev)
     w = configs.DENTAL512
     if args.views:
         w = w.scaled(n_views=args.views)
@@ -253,6 +255,89 @@ def main():
                              "Measured limiter: gather addresses (texture addresser ~97% busy, "
                              "profiles/bp_dental512_pmc.md)"},
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def run_offset(args, dev):
+    """dental512 on a half-fan detector through the composite chain
+    (preprocess -> composite -> 2W -> W mirrored filter -> stage -> BP)."""
+    import torch
+    from jenhine_amd import configs
+    from jenhine_amd.recon import FDK
+
+    w = configs.OFFSET_DENTAL512
+    if args.views:
+        w = w.scaled(n_views=args.views)
+    t0 = time.time()
+    d = w.data()
+    log(f"synthetic half-fan data {w.n_views} x {w.n_v} x {w.n_u} in {time.time() - t0:.1f} s")
+    raw = torch.from_numpy(d["raw"].view(np.int16)).to(dev)
+    dark = torch.from_numpy(d["dark"].view(np.int16)).to(dev)
+    gain = torch.from_numpy(d["gain"]).to(dev)
+    scan, rec = w.scan_params(), w.recon_params()
+    units = w.gups_units
+    results, vols = {}, {}
+    for name, (mode, storage) in {"fp32": ("fp32", "f32"), "bf16": ("mfma6", "bf16")}.items():
+        fdk = FDK(scan, rec, d["ptm"], mode, storage, dev, composite=True)
+        _, st0 = fdk(raw, dark, gain, return_stages=True)
+        fdk.bp.autotune(st0["q"], storage, reps=1)
+        del st0
+        for _ in range(args.warmup):
+            fdk(raw, dark, gain)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            vol = fdk(raw, dark, gain)
+        torch.cuda.synchronize()
+        step = (time.perf_counter() - t) / args.steps
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
+        ev[0].record()
+        p = fdk.pre.process(raw, dark, gain)
+        ev[1].record()
+        c = fdk.composite.process(p)
+        ev[2].record()
+        f = fdk.filter.process(c)
+        ev[3].record()
+        q = fdk.bp.stage(f, storage)
+        ev[4].record()
+        vol = fdk.bp.run(q, storage)
+        ev[5].record()
+        torch.cuda.synchronize()
+        stt = [ev[i].elapsed_time(ev[i + 1]) for i in range(5)]
+        vols[name] = vol.cpu().numpy()
+        results[name] = dict(step_ms=step * 1e3, gups=units / step / 1e9,
+                             stage_ms=dict(pre=stt[0], composite=stt[1], filter=stt[2], stage=stt[3], bp=stt[4]))
+        log(name, json.dumps(results[name]))
+        del p, c, f, q, fdk
+    acc = {}
+    if not args.no_accuracy:
+        from oracle import oracle as o
+        nth = min(os.cpu_count() or 1, 16)
+        sc = dict(bright_field=scan.BrightField, hsr=scan.HalfSampleRate, dso=scan.DSO, bh=scan.BeamHardeningParams)
+        rc = dict(X=w.X, Y=w.Y, Z=w.Z, pxy=w.pxy, pz=w.pz, cut_start=rec.FilterCutOffStart,
+                  cut_end=rec.FilterCutOffEnd, window=rec.FilterWindow)
+        z0 = w.Z // 2 - args.acc_slices // 2
+        t = time.time()
+        v64 = o.fdk(d["raw"], d["ptm"], sc, rc, fp64=True, nthreads=nth, z0=z0, nz=args.acc_slices,
+                    dark=d["dark"], gain=d["gain"], composite_views=True)
+        log(f"FP64 oracle slices (composite chain) in {time.time() - t:.1f} s")
+        for name in vols:
+            g = vols[name][z0:z0 + args.acc_slices]
+            acc[name] = dict(rel_rmse_vs_fp64=rel_rmse(g, v64),
+                             max_abs_vs_fp64=float(np.nanmax(np.abs(g.astype(np.float64) - v64))),
+                             slices=[z0, z0 + args.acc_slices])
+        log("accuracy", json.dumps(acc))
+    line = {
+        "metric": "half-fan FDK GUPS at 512^3 x 600 views (composite chain)",
+        "value": results["bf16"]["gups"], "unit": "GUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": results["bf16"]["step_ms"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16-storage/f32-accumulate",
+        "data": "synthetic (seeded ellipsoid phantom, analytic line integrals, Poisson counts)",
+        "config": {"workload": f"offset_dental512: composite FDK {w.X}x{w.Y}x{w.Z} from {w.n_views} views of "
+                               f"{w.n_u}x{w.n_v} u16, central ray at column {w.n_u / 2 - 1 - w.u_shift:.0f}",
+                   "parallelism": "single-gpu"},
+        "forms": results, "accuracy": acc,
     }
     print(json.dumps(line), flush=True)
 

@@ -92,11 +92,15 @@ __global__ void bp_uweight_kernel(const double* __restrict__ ptm, int N, int W, 
 //  * FP32 (16 B): (q[v][u], q[v][u+1] - q[v][u], q[v+1][u], q[v+1][u+1] -
 //    q[v+1][u]).  Each row interpolation is one FMA, q + fu d, and d is the
 //    difference the interpolation would form anyway (same result to the bit).
-//  * BF16 / FP16 (8 B): (q[v][u], q[v][u+1], q[v+1][u], q[v+1][u+1]), each
-//    rounded once.  Storing differences instead makes the rounding error
-//    scale with |d|, which exceeds |q| on ramp-filtered noise, and a BF16
-//    weight for v_dot2 would move the sample by up to 2^-9 pixel; both
-//    measurably raise the error.
+//  * BF16 / FP16 (16 B, "strip"): the pairs (q[r][u], q[r][u+1]) of the four
+//    rows r = v .. v+3, one 32-bit word per row, each value rounded once.
+//    The first 8 bytes are the sample's 2 x 2 neighbourhood (one b64
+//    gather); a b128 gather serves two consecutive z samples of a column
+//    whose row step is below 2 pixels (the backprojector's fast path), which
+//    halves the gather addresses.  Storing differences instead makes the
+//    rounding error scale with |d|, which exceeds |q| on ramp-filtered
+//    noise, and a BF16 weight for v_dot2 would move the sample by up to 2^-9
+//    pixel; both measurably raise the error.
 // ---------------------------------------------------------------------------
 
 __device__ inline uint32_t f32_to_bf16_rne(float f)
@@ -160,45 +164,62 @@ __global__ void __launch_bounds__(256) zfilter_kernel(const float* __restrict__ 
     }
 }
 
+// One thread per staged pixel: blockIdx.x * 256 + threadIdx.x = u,
+// blockIdx.y = row, blockIdx.z = frame.
 template <int QT>
 __global__ void __launch_bounds__(256) bp_stage_kernel(const float* __restrict__ filt,
-                                                       const float* __restrict__ uw, int W, int V,
-                                                       int nviews, int zf, void* __restrict__ out)
+                                                       const float* __restrict__ uw, int W, int V, int zf,
+                                                       void* __restrict__ out)
 // uw == NULL packs the frames without weighting; zf != 0 applies the v
 // smoothing of ZFilterAgent to the frames first (before the uW weight, as in
 // the reference chain Filter -> ZFilter -> BPCore::InitView).
 {
+    const int u = blockIdx.x * 256 + threadIdx.x;
+    if (u >= W) return;
+    const int row = blockIdx.y;
+    const size_t view = blockIdx.z;
     const size_t frame = (size_t)W * V;
-    const size_t total = frame * nviews;
-    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
-         t += (size_t)gridDim.x * blockDim.x) {
-        const size_t view = t / frame;
-        const int pix = (int)(t - view * frame);
-        const int row = pix / W;
-        const int u = pix - row * W;
-        const float* fr = filt + view * frame;
-        const float* w = uw ? uw + view * W : nullptr;
-        const bool right = u + 1 < W, below = row + 1 < V;
-        const float wa = w ? w[u] : 1.0f;
-        const float wb = w && right ? w[u + 1] : 1.0f;
-        auto at = [&](int r, int c) { return zf ? zfilter_at(fr, W, V, r, c) : fr[(size_t)r * W + c]; };
-        const float a0 = at(row, u) * wa;
-        const float b0 = right ? at(row, u + 1) * wb : 0.0f;
-        const float a1 = below ? at(row + 1, u) * wa : 0.0f;
-        const float b1 = below && right ? at(row + 1, u + 1) * wb : 0.0f;
-        if (QT == JH_F32) {
-            reinterpret_cast<float4*>(out)[t] = make_float4(a0, b0 - a0, a1, b1 - a1);
-        } else if (QT == JH_BF16) {
-            u32x2_t o;
-            o.x = f32_to_bf16_rne(a0) | (f32_to_bf16_rne(b0) << 16);
-            o.y = f32_to_bf16_rne(a1) | (f32_to_bf16_rne(b1) << 16);
-            reinterpret_cast<u32x2_t*>(out)[t] = o;
-        } else {
-            u32x2_t o;
-            o.x = pack_f16x2(a0, b0);
-            o.y = pack_f16x2(a1, b1);
-            reinterpret_cast<u32x2_t*>(out)[t] = o;
+    const float* fr = filt + view * frame;
+    const float* w = uw ? uw + view * W : nullptr;
+    const bool right = u + 1 < W;
+    const int u1 = right ? u + 1 : u;
+    const float wa = w ? w[u] : 1.0f;
+    const float wb = w && right ? w[u + 1] : 1.0f;
+    constexpr int NR = QT == JH_F32 ? 2 : 4;  // rows of the quad or the strip
+    // rows row-1 .. row+NR of columns u, u+1, clamped into the frame
+    float xa[NR + 2], xb[NR + 2];
+#pragma unroll
+    for (int k = 0; k < NR + 2; ++k) {
+        const int r = min(max(row - 1 + k, 0), V - 1);
+        xa[k] = fr[(size_t)r * W + u];
+        xb[k] = fr[(size_t)r * W + u1];
+    }
+    // value of row row+k (k < NR) at column u or u+1, 0 beyond the frame
+    auto val = [&](const float* x, int k, float wgt, bool inside) {
+        if (row + k >= V || !inside) return 0.0f;
+        float v = x[k + 1];
+        if (zf) {
+#pragma clang fp contract(off)
+            // ZFilterAgent.cpp:49-71, as zfilter_at: the clamped neighbours
+            // are the clamped loads above
+            const float t = v * 0.5f + x[k] * 0.25f;
+            v = t + x[k + 2] * 0.25f;
         }
+        return v * wgt;
+    };
+    const size_t t = view * frame + (size_t)row * W + u;
+    if (QT == JH_F32) {
+        const float a0 = val(xa, 0, wa, true), b0 = val(xb, 0, wb, right);
+        const float a1 = val(xa, 1, wa, true), b1 = val(xb, 1, wb, right);
+        reinterpret_cast<float4*>(out)[t] = make_float4(a0, b0 - a0, a1, b1 - a1);
+    } else {
+        u32x4_t o;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float a = val(xa, k, wa, true), b = val(xb, k, wb, right);
+            o[k] = QT == JH_BF16 ? (f32_to_bf16_rne(a) | (f32_to_bf16_rne(b) << 16)) : pack_f16x2(a, b);
+        }
+        reinterpret_cast<u32x4_t*>(out)[t] = o;
     }
 }
 
@@ -227,7 +248,8 @@ struct Sampler;
 template <>
 struct Sampler<JH_F32> {
     typedef u32x4_t raw_t;
-    static constexpr int bytes = 16;
+    static constexpr int bytes = 16;   // bytes per sample load
+    static constexpr int stride = 16;  // bytes per staged pixel
     __device__ static raw_t load(__amdgpu_buffer_rsrc_t r, uint32_t off)
     {
         return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
@@ -250,16 +272,23 @@ template <>
 struct Sampler<JH_BF16> {
     typedef u32x2_t raw_t;
     static constexpr int bytes = 8;
+    static constexpr int stride = 16;
     __device__ static raw_t load(__amdgpu_buffer_rsrc_t r, uint32_t off)
     {
         return __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
     }
-    __device__ static float lerp(raw_t p, int r, float fu)
+    // the whole strip: rows v .. v+3
+    __device__ static u32x4_t load4(__amdgpu_buffer_rsrc_t r, uint32_t off)
     {
-        const uint32_t x = r == 0 ? p.x : p.y;
+        return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+    }
+    // one row's pair interpolated at fu
+    __device__ static float row(uint32_t x, float fu)
+    {
         const float a = __uint_as_float(x << 16), b = __uint_as_float(x & 0xffff0000u);
         return fmaf(fu, b - a, a);
     }
+    __device__ static float lerp(raw_t p, int r, float fu) { return row(r == 0 ? p.x : p.y, fu); }
     __device__ static float4 abcd(raw_t p)
     {
         const float q00 = __uint_as_float(p.x << 16), q01 = __uint_as_float(p.x & 0xffff0000u);
@@ -273,17 +302,26 @@ template <>
 struct Sampler<JH_F16> {
     typedef u32x2_t raw_t;
     static constexpr int bytes = 8;
+    static constexpr int stride = 16;
     __device__ static raw_t load(__amdgpu_buffer_rsrc_t r, uint32_t off)
     {
         return __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
     }
-    __device__ static float lerp(raw_t p, int r, float fu)
+    __device__ static u32x4_t load4(__amdgpu_buffer_rsrc_t r, uint32_t off)
+    {
+        return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+    }
+    __device__ static float row(uint32_t x, float fu)
     {
         // b fu + (a - a fu): two v_fma_mix, which convert the f16 halves inside
-        const uint32_t x = r == 0 ? p.x : p.y;
         const half2_t h = __builtin_bit_cast(half2_t, x);
         const float a = (float)h.x;
         return fmaf((float)h.y, fu, fmaf(-a, fu, a));
+    }
+    __device__ static float lerp(raw_t p, int r, float fu)
+    {
+        const uint32_t x = r == 0 ? p.x : p.y;
+        return row(x, fu);
     }
     __device__ static float4 abcd(raw_t p)
     {
@@ -310,6 +348,7 @@ struct BPArgs {
     uint32_t frame_bytes;
     int ct;              // apply the CT-number map to normalised output
     float ct_scale, ct_norm1;
+    int lds_px;          // LDS-staged fetch: pixels per coefficient buffer (two buffers)
 };
 
 constexpr int kZB = 8;  // samples whose loads are issued together
@@ -319,6 +358,43 @@ constexpr int kModeStaged = 3;  // geometry class 0 with the LDS-staged fetch
 constexpr int kOutNormalised = 0;  // vol = acc / count (one-shot)
 constexpr int kOutAccumulate = 1;  // acc += ..., count += ... (streaming)
 constexpr int kOutRaw = 2;         // vol = acc (the un-normalised operator, FP32 storage only)
+
+// The samples of one all-in-range column in pairs over BF16 / FP16 strips:
+// one b128 gather at row floor(sv_k) serves samples k and k+1; the second
+// sample's rows start HI or HI + 1 rows down.  The arithmetic of each sample
+// is that of the one-gather-per-sample path, so the results are identical.
+template <int ZC, int QT, int HI>
+__device__ __forceinline__ void bp_column_pairs(float* acc, __amdgpu_buffer_rsrc_t rs, uint32_t rowb, uint32_t ub,
+                                                float sv0, float dsv, float fu)
+{
+    typedef Sampler<QT> S;
+    constexpr int NP = ZC / 2;
+    uint32_t off[NP];
+    float fa[NP], fb[NP];
+    bool e[NP];
+#pragma unroll
+    for (int m = 0; m < NP; ++m) {
+        const float sa = fmaf((float)(2 * m), dsv, sv0);
+        const float sb = fmaf((float)(2 * m + 1), dsv, sv0);
+        const int ra = (int)sa, rb = (int)sb;
+        fa[m] = __builtin_amdgcn_fractf(sa);
+        fb[m] = __builtin_amdgcn_fractf(sb);
+        e[m] = (rb - ra) != HI;
+        off[m] = __umul24((uint32_t)ra, rowb) + ub;
+    }
+    u32x4_t p[NP];
+#pragma unroll
+    for (int m = 0; m < NP; ++m) p[m] = S::load4(rs, off[m]);
+#pragma unroll
+    for (int m = 0; m < NP; ++m) {
+        const float t0 = S::row(p[m].x, fu), b0 = S::row(p[m].y, fu);
+        acc[2 * m] += fmaf(fa[m], b0 - t0, t0);
+        const uint32_t xt = HI == 0 ? (e[m] ? p[m].y : p[m].x) : (e[m] ? p[m].z : p[m].y);
+        const uint32_t xb = HI == 0 ? (e[m] ? p[m].z : p[m].y) : (e[m] ? p[m].w : p[m].z);
+        const float t1 = S::row(xt, fu), b1 = S::row(xb, fu);
+        acc[2 * m + 1] += fmaf(fb[m], b1 - t1, t1);
+    }
+}
 
 // One tile: 64 blockDim.y voxel columns (8 x 8 per wave), ZC voxels deep,
 // all views of the launch.
@@ -344,7 +420,7 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
     const float fj = (float)min(j, a.Y - 1);
     const float fk0 = (float)k0;
     const float Wm1 = (float)(a.W - 1), Vm1 = (float)(a.V - 1);
-    const uint32_t rowb = (uint32_t)a.W * S::bytes;
+    const uint32_t rowb = (uint32_t)a.W * S::stride;
 
     float acc[ZC], cnt[ZC];
 #pragma unroll
@@ -371,7 +447,7 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
                 const float cw = fmaf(fu, uwq.y - uwq.x, uwq.x);
                 const float sv0 = fmaf(Vc, rw, 0.5f);
                 const float dsv = g.c[7] * rw;
-                const uint32_t ub = (uint32_t)iu * S::bytes;
+                const uint32_t ub = (uint32_t)iu * S::stride;
                 // sv is monotonic in k, so the column is inside the detector
                 // rows iff its first and last samples are.
                 const float svl = fmaf((float)(ZC - 1), dsv, sv0);
@@ -381,22 +457,39 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
                     // consumed: the kernel is bound by gather latency, and
                     // this keeps 2 ZC of them in flight per wave.
                     cfull += cw;
-                    uint32_t off[ZC];
-                    float fv[ZC];
-#pragma unroll
-                    for (int k = 0; k < ZC; ++k) {
-                        const float sv = fmaf((float)k, dsv, sv0);
-                        fv[k] = __builtin_amdgcn_fractf(sv);
-                        off[k] = __umul24((uint32_t)(int)sv, rowb) + ub;
+                    // BF16 / FP16 strips: one b128 gather per pair of samples
+                    // (k, k+1) when the row step lies clear of an integer
+                    // (margin 1/64 against the rounding of sv): sample k
+                    // reads rows 0-1 of its strip, sample k+1 rows d-(d+1),
+                    // d = floor(sv_k+1) - floor(sv_k), which is hi or hi+1
+                    bool paired = false;
+                    if constexpr (QT != JH_F32) {
+                        if (dsv >= 0.0f && dsv <= 0.984375f) {
+                            bp_column_pairs<ZC, QT, 0>(acc, rs, rowb, ub, sv0, dsv, fu);
+                            paired = true;
+                        } else if (dsv >= 1.015625f && dsv <= 1.984375f) {
+                            bp_column_pairs<ZC, QT, 1>(acc, rs, rowb, ub, sv0, dsv, fu);
+                            paired = true;
+                        }
                     }
-                    typename S::raw_t p[ZC];
+                    if (!paired) {
+                        uint32_t off[ZC];
+                        float fv[ZC];
 #pragma unroll
-                    for (int k = 0; k < ZC; ++k) p[k] = S::load(rs, off[k]);
+                        for (int k = 0; k < ZC; ++k) {
+                            const float sv = fmaf((float)k, dsv, sv0);
+                            fv[k] = __builtin_amdgcn_fractf(sv);
+                            off[k] = __umul24((uint32_t)(int)sv, rowb) + ub;
+                        }
+                        typename S::raw_t p[ZC];
 #pragma unroll
-                    for (int k = 0; k < ZC; ++k) {
-                        const float top = S::lerp(p[k], 0, fu);
-                        const float bot = S::lerp(p[k], 1, fu);
-                        acc[k] += fmaf(fv[k], bot - top, top);
+                        for (int k = 0; k < ZC; ++k) p[k] = S::load(rs, off[k]);
+#pragma unroll
+                        for (int k = 0; k < ZC; ++k) {
+                            const float top = S::lerp(p[k], 0, fu);
+                            const float bot = S::lerp(p[k], 1, fu);
+                            acc[k] += fmaf(fv[k], bot - top, top);
+                        }
                     }
                 } else {
 #pragma unroll
@@ -456,7 +549,7 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
                     const int iu = (int)tu;
                     fu[q] = __builtin_amdgcn_fractf(tu);
                     fv[q] = __builtin_amdgcn_fractf(t);
-                    off[q] = __umul24((uint32_t)(int)t, rowb) + (uint32_t)iu * S::bytes;
+                    off[q] = __umul24((uint32_t)(int)t, rowb) + (uint32_t)iu * S::stride;
                     uwq[q] = uwv[iu];
                 }
                 typename S::raw_t p[kZB];
@@ -509,19 +602,33 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
 //   value(fu, fv) = a + fu b + fv (c + fu d).
 // Every sample is then one ds_read_b128 and three FMAs; the per-sample
 // gathers no longer pass through the texture addresser, which bounds the
-// gather form.  A footprint larger than the LDS budget falls back to the
-// gather form for that view.  Sample positions are computed exactly as in the
-// gather form.
+// gather form.  Sample positions are computed exactly as in the gather form.
+//
+// The staging is software-pipelined over the views, with one barrier per
+// view: while the workgroup samples view v from one LDS buffer, its global
+// loads of view v+1's footprint are in flight (issued after the previous
+// barrier, held in registers), and they are converted into the other buffer
+// after the sampling.  Footprints are computed two views ahead by wave 0 into
+// a ring of three boxes.  A footprint larger than the buffer falls back to
+// gathers for that view.
 // ---------------------------------------------------------------------------
 
-constexpr int kLdsPixels = 2048;  // 32 KiB of FP32 coefficients per workgroup
+constexpr int kLdsPre = 8;    // footprint pixels per thread held in registers across a view
+constexpr int kLdsUw = 256;   // widest footprint whose count-weight pairs are staged too
+
+// Workgroup-shared LDS: two coefficient buffers of a.lds_px pixels each
+// (dynamic shared memory, sized by the host per tile shape).
+extern __shared__ float4 lds_dyn[];
 
 template <int ZC, int QT, int OUT>
 __device__ __forceinline__ void bp_tile_lds(const BPArgs& a, unsigned tile)
 {
     typedef Sampler<QT> S;
-    __shared__ float4 lds[kLdsPixels];
-    __shared__ int box[4];
+    __shared__ int box[3][4];  // u0, v0, width, pixels (pixels > lds_px: not staged)
+    // count-weight pairs uwp[u0 .. u0 + width) of the staged view, so that the
+    // sampling loop issues no vector-memory load (a wait on one would also
+    // wait for the footprint prefetch in flight)
+    __shared__ float2 uwl[2][kLdsUw];
     const int tx = tile % a.ntx;
     const unsigned rest = tile / a.ntx;
     const int tyi = rest % a.nty;
@@ -539,161 +646,221 @@ __device__ __forceinline__ void bp_tile_lds(const BPArgs& a, unsigned tile)
     const float cx0 = (float)min(tx * ex, a.X - 1), cx1 = (float)min(tx * ex + ex - 1, a.X - 1);
     const float cy0 = (float)min(tyi * ey, a.Y - 1), cy1 = (float)min(tyi * ey + ey - 1, a.Y - 1);
     const float Wm1 = (float)(a.W - 1), Vm1 = (float)(a.V - 1);
-    const uint32_t rowb = (uint32_t)a.W * S::bytes;
+    const uint32_t rowb = (uint32_t)a.W * S::stride;
     const int nthr = 64 * blockDim.y;
     const int tid = threadIdx.y * 64 + threadIdx.x;
+    const int cap = a.lds_px;
+    const int n = a.nviews;
+
+    auto rsrc = [&](int vv) {
+        return __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<char*>(reinterpret_cast<const char*>(a.q)) + (size_t)vv * a.frame_bytes, (short)0,
+            (int)a.frame_bytes, kBufferFlags);
+    };
+
+    // Footprint of the block in view vv into box[slot]: all of wave 0, one
+    // lane per corner column in every group of 4 (reciprocal instead of
+    // division: the one-pixel margin absorbs the difference).
+    auto footprint = [&](int vv, int slot) {
+        if (threadIdx.y != 0) return;
+        const Geo g = a.geo[a.view0 + vv];
+        const int c = threadIdx.x & 3;
+        const float cx = (c & 1) ? cx1 : cx0, cy = (c & 2) ? cy1 : cy0;
+        const float U = fmaf(g.c[3], fk0, fmaf(g.c[1], cx, fmaf(g.c[2], cy, g.c[0])));
+        const float Vv = fmaf(g.c[7], fk0, fmaf(g.c[5], cx, fmaf(g.c[6], cy, g.c[4])));
+        const float Ww = fmaf(g.c[11], fk0, fmaf(g.c[9], cx, fmaf(g.c[10], cy, g.c[8])));
+        const float r = __builtin_amdgcn_rcpf(Ww);
+        const float su = fmaf(U, r, 0.5f);
+        const float s0 = fmaf(Vv, r, 0.5f);
+        const float sl = fmaf((float)(ZC - 1), g.c[7] * r, s0);
+        float umin = su, umax = su, vmin = fminf(s0, sl), vmax = fmaxf(s0, sl);
+#pragma unroll
+        for (int m = 1; m <= 2; m <<= 1) {
+            umin = fminf(umin, __shfl_xor(umin, m));
+            umax = fmaxf(umax, __shfl_xor(umax, m));
+            vmin = fminf(vmin, __shfl_xor(vmin, m));
+            vmax = fmaxf(vmax, __shfl_xor(vmax, m));
+        }
+        if (threadIdx.x == 0) {
+            const int u0 = (int)fminf(fmaxf(floorf(umin) - 1.0f, 0.0f), Wm1);
+            const int u1 = (int)fminf(fmaxf(floorf(umax) + 1.0f, 0.0f), Wm1);
+            const int v0 = (int)fminf(fmaxf(floorf(vmin) - 1.0f, 0.0f), Vm1);
+            const int v1 = (int)fminf(fmaxf(floorf(vmax) + 1.0f, 0.0f), Vm1);
+            box[slot][0] = u0;
+            box[slot][1] = v0;
+            box[slot][2] = u1 - u0 + 1;
+            box[slot][3] = (u1 - u0 + 1) * (v1 - v0 + 1);
+        }
+    };
+
+    // Pixel b of a footprint box: row b / width (exact: (b + 0.5) / width is
+    // at least 0.5 / width away from an integer, far beyond float rounding).
+    auto pixel_offset = [&](int b, int u0, int v0, int wdt, float rinv) {
+        const int r = (int)(((float)b + 0.5f) * rinv);
+        const int c = b - r * wdt;
+        return (uint32_t)(v0 + r) * rowb + (uint32_t)(u0 + c) * S::stride;
+    };
+
+    // a view is staged when its footprint and count-weight row fit
+    auto staged = [&](int slot) { return box[slot][3] <= cap && box[slot][2] <= kLdsUw; };
+
+    typename S::raw_t pre[kLdsPre];
+    float2 preuw = make_float2(0.0f, 0.0f);
+    // issue the global loads of view vv's footprint (box slot) into pre[]
+    // (every lane issues every load, at a pixel clamped into the footprint:
+    // a load under a divergent branch makes the compiler wait for all loads
+    // in flight before it, which would serialise the prefetch)
+    auto issue = [&](int vv, int slot) {
+        if (!staged(slot)) return;
+        const int npx = box[slot][3];
+        const int u0 = box[slot][0], v0 = box[slot][1], wdt = box[slot][2];
+        preuw = a.uwp[(size_t)(a.view0 + vv) * a.W + u0 + min(tid, wdt - 1)];
+        const float rinv = 1.0f / (float)wdt;
+        const auto rs = rsrc(vv);
+#pragma unroll
+        for (int m = 0; m < kLdsPre; ++m) {
+            const int b = min(tid + m * nthr, npx - 1);
+            pre[m] = S::load(rs, pixel_offset(b, u0, v0, wdt, rinv));
+        }
+    };
+    // convert pre[] (and any pixels beyond kLdsPre per thread) into buffer buf
+    auto commit = [&](int vv, int slot, int buf) {
+        if (!staged(slot)) return;
+        const int npx = box[slot][3];
+        if (tid < box[slot][2]) uwl[buf][tid] = preuw;
+        if (box[slot][2] > nthr) {  // small workgroups: the rest of the row synchronously
+            for (int b = tid + nthr; b < box[slot][2]; b += nthr)
+                uwl[buf][b] = a.uwp[(size_t)(a.view0 + vv) * a.W + box[slot][0] + b];
+        }
+        float4* dst = lds_dyn + buf * cap;
+#pragma unroll
+        for (int m = 0; m < kLdsPre; ++m) {
+            const int b = tid + m * nthr;
+            if (b < npx) dst[b] = S::abcd(pre[m]);
+        }
+        if (npx > kLdsPre * nthr) {  // small workgroups: the rest synchronously
+            const int u0 = box[slot][0], v0 = box[slot][1], wdt = box[slot][2];
+            const float rinv = 1.0f / (float)wdt;
+            const auto rs = rsrc(vv);
+            for (int b = tid + kLdsPre * nthr; b < npx; b += nthr)
+                dst[b] = S::abcd(S::load(rs, pixel_offset(b, u0, v0, wdt, rinv)));
+        }
+    };
 
     float acc[ZC], cnt[ZC];
 #pragma unroll
     for (int kk = 0; kk < ZC; ++kk) { acc[kk] = 0.0f; cnt[kk] = 0.0f; }
     float cfull = 0.0f;
 
-    for (int v = 0; v < a.nviews; ++v) {
+    if (n > 0) {
+        footprint(0, 0);
+        if (n > 1) footprint(1, 1);
+        __syncthreads();
+        issue(0, 0);
+        commit(0, 0, 0);
+        if (n > 1) issue(1, 1);
+        __syncthreads();
+    }
+    for (int v = 0; v < n; ++v) {
+        const int slot = v % 3, buf = v & 1;
+        if (v + 2 < n) footprint(v + 2, (v + 2) % 3);
+
         const Geo g = a.geo[a.view0 + v];
-        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-            const_cast<char*>(reinterpret_cast<const char*>(a.q)) + (size_t)v * a.frame_bytes, (short)0,
-            (int)a.frame_bytes, kBufferFlags);
-
-        // Footprint of the block, computed by wave 0 with one lane per
-        // corner column (reciprocal instead of division: the one-pixel margin
-        // absorbs the difference) and handed to the workgroup through LDS.
-        if (threadIdx.y == 0) {
-            const int c = threadIdx.x & 3;
-            const float cx = (c & 1) ? cx1 : cx0, cy = (c & 2) ? cy1 : cy0;
-            const float U = fmaf(g.c[3], fk0, fmaf(g.c[1], cx, fmaf(g.c[2], cy, g.c[0])));
-            const float Vv = fmaf(g.c[7], fk0, fmaf(g.c[5], cx, fmaf(g.c[6], cy, g.c[4])));
-            const float Ww = fmaf(g.c[11], fk0, fmaf(g.c[9], cx, fmaf(g.c[10], cy, g.c[8])));
-            const float r = __builtin_amdgcn_rcpf(Ww);
-            const float su = fmaf(U, r, 0.5f);
-            const float s0 = fmaf(Vv, r, 0.5f);
-            const float sl = fmaf((float)(ZC - 1), g.c[7] * r, s0);
-            float umin = su, umax = su, vmin = fminf(s0, sl), vmax = fmaxf(s0, sl);
-#pragma unroll
-            for (int m = 1; m <= 2; m <<= 1) {
-                umin = fminf(umin, __shfl_xor(umin, m));
-                umax = fmaxf(umax, __shfl_xor(umax, m));
-                vmin = fminf(vmin, __shfl_xor(vmin, m));
-                vmax = fmaxf(vmax, __shfl_xor(vmax, m));
-            }
-            if (threadIdx.x == 0) {
-                box[0] = (int)fminf(fmaxf(floorf(umin) - 1.0f, 0.0f), Wm1);
-                box[1] = (int)fminf(fmaxf(floorf(umax) + 1.0f, 0.0f), Wm1);
-                box[2] = (int)fminf(fmaxf(floorf(vmin) - 1.0f, 0.0f), Vm1);
-                box[3] = (int)fminf(fmaxf(floorf(vmax) + 1.0f, 0.0f), Vm1);
-            }
-        }
-        __syncthreads();  // the box is known and every read of the previous view's tile is done
-        const int u0 = box[0], u1 = box[1], v0 = box[2], v1 = box[3];
-        const int wdt = u1 - u0 + 1;
-        const int npx = wdt * (v1 - v0 + 1);
-        const bool staged = npx <= kLdsPixels;
-        if (staged) {
-            if (wdt <= 64) {
-                // rows of up to 64 pixels: lane group g of 2^lg lanes takes a
-                // row, no division
-                const int lg = wdt <= 16 ? 4 : (wdt <= 32 ? 5 : 6);
-                const int c = tid & ((1 << lg) - 1);
-                const int rstep = nthr >> lg;
-                if (c < wdt) {
-                    int r = tid >> lg;
-                    uint32_t goff = (uint32_t)(v0 + r) * rowb + (uint32_t)(u0 + c) * S::bytes;
-                    const uint32_t gstep = (uint32_t)rstep * rowb;
-                    int li = r * wdt + c;
-                    const int lstep = rstep * wdt;
-                    const int hgt = v1 - v0 + 1;
-                    for (; r + rstep < hgt; r += 2 * rstep) {
-                        const typename S::raw_t p0 = S::load(rs, goff);
-                        const typename S::raw_t p1 = S::load(rs, goff + gstep);
-                        lds[li] = S::abcd(p0);
-                        lds[li + lstep] = S::abcd(p1);
-                        goff += 2 * gstep;
-                        li += 2 * lstep;
-                    }
-                    if (r < hgt) lds[li] = S::abcd(S::load(rs, goff));
-                }
-            } else {
-                const float rinv = 1.0f / (float)wdt;
-                for (int b = tid; b < npx; b += nthr) {
-                    // exact row index: (b + 0.5) / wdt is at least 0.5 / wdt
-                    // away from an integer, far beyond float rounding here
-                    const int r = (int)(((float)b + 0.5f) * rinv);
-                    const int c = b - r * wdt;
-                    lds[b] = S::abcd(S::load(rs, (uint32_t)(v0 + r) * rowb + (uint32_t)(u0 + c) * S::bytes));
-                }
-            }
-        }
-        __syncthreads();  // the tile is in LDS
-
         const float Uc = fmaf(g.c[3], fk0, fmaf(g.c[1], fi, fmaf(g.c[2], fj, g.c[0])));
         const float Vc = fmaf(g.c[7], fk0, fmaf(g.c[5], fi, fmaf(g.c[6], fj, g.c[4])));
         const float Wc = fmaf(g.c[11], fk0, fmaf(g.c[9], fi, fmaf(g.c[10], fj, g.c[8])));
         const float rw = 1.0f / Wc;
         const float su = fmaf(Uc, rw, 0.5f);
-        if (!(su >= 0.0f && su <= Wm1)) continue;
-        const int iu = (int)su;
-        const float fu = __builtin_amdgcn_fractf(su);
-        const float2 uwq = a.uwp[(size_t)(a.view0 + v) * a.W + iu];
-        const float cw = fmaf(fu, uwq.y - uwq.x, uwq.x);
-        const float sv0 = fmaf(Vc, rw, 0.5f);
-        const float dsv = g.c[7] * rw;
-        const float svl = fmaf((float)(ZC - 1), dsv, sv0);
-        const bool all_in = fminf(sv0, svl) >= 0.0f && fmaxf(sv0, svl) <= Vm1;
-        if (staged) {
-            // byte offset of sample row iv: iv * wdt * 16 + lb (mod 2^32)
-            const uint32_t lrow = (uint32_t)wdt * 16u;
-            const uint32_t lb = (uint32_t)(iu - u0) * 16u - (uint32_t)v0 * lrow;
-            const char* lbase = reinterpret_cast<const char*>(lds);
-            if (all_in) {
-                cfull += cw;
+        if (su >= 0.0f && su <= Wm1) {
+            const int iu = (int)su;
+            const float fu = __builtin_amdgcn_fractf(su);
+            const float sv0 = fmaf(Vc, rw, 0.5f);
+            const float dsv = g.c[7] * rw;
+            const float svl = fmaf((float)(ZC - 1), dsv, sv0);
+            const bool all_in = fminf(sv0, svl) >= 0.0f && fmaxf(sv0, svl) <= Vm1;
+            const int u0 = box[slot][0], v0 = box[slot][1], wdt = box[slot][2];
+            if (staged(slot)) {
+                const float2 uwq = uwl[buf][iu - u0];
+                const float cw = fmaf(fu, uwq.y - uwq.x, uwq.x);
+                // byte offset of sample row iv: iv * wdt * 16 + lb (mod 2^32)
+                const uint32_t lrow = (uint32_t)wdt * 16u;
+                const uint32_t lb = (uint32_t)(iu - u0) * 16u - (uint32_t)v0 * lrow;
+                const char* lbase = reinterpret_cast<const char*>(lds_dyn + buf * cap);
+                if (all_in) {
+                    cfull += cw;
+                    // kZB reads in flight per wave before any is consumed: at
+                    // the occupancy the LDS budget allows, LDS latency is
+                    // otherwise exposed
 #pragma unroll
-                for (int k = 0; k < ZC; ++k) {
-                    const float sv = fmaf((float)k, dsv, sv0);
-                    const float fv = __builtin_amdgcn_fractf(sv);
-                    const float4 c4 = *reinterpret_cast<const float4*>(lbase + __umul24((uint32_t)(int)sv, lrow) + lb);
-                    acc[k] += fmaf(fv, fmaf(fu, c4.w, c4.z), fmaf(fu, c4.y, c4.x));
+                    for (int kb = 0; kb < ZC; kb += kZB) {
+                        float4 c4[kZB];
+                        float fv[kZB];
+#pragma unroll
+                        for (int q = 0; q < kZB; ++q) {
+                            const float sv = fmaf((float)(kb + q), dsv, sv0);
+                            fv[q] = __builtin_amdgcn_fractf(sv);
+                            c4[q] = *reinterpret_cast<const float4*>(lbase + __umul24((uint32_t)(int)sv, lrow) + lb);
+                        }
+                        // keep the scheduler from interleaving the reads with
+                        // their uses (it pairs them under register pressure)
+                        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                        for (int q = 0; q < kZB; ++q)
+                            acc[kb + q] += fmaf(fv[q], fmaf(fu, c4[q].w, c4[q].z), fmaf(fu, c4[q].y, c4[q].x));
+                    }
+                } else {
+#pragma unroll
+                    for (int k = 0; k < ZC; ++k) {
+                        const float sv = fmaf((float)k, dsv, sv0);
+                        const float t = __builtin_amdgcn_fmed3f(sv, 0.0f, Vm1);
+                        const bool in = (t == sv);
+                        const float fv = __builtin_amdgcn_fractf(t);
+                        const float4 c4 =
+                            *reinterpret_cast<const float4*>(lbase + __umul24((uint32_t)(int)t, lrow) + lb);
+                        const float val = fmaf(fv, fmaf(fu, c4.w, c4.z), fmaf(fu, c4.y, c4.x));
+                        acc[k] += in ? val : 0.0f;
+                        cnt[k] += in ? cw : 0.0f;
+                    }
                 }
             } else {
+                // footprint larger than the LDS buffer: gathers for this view
+                const float2 uwq = a.uwp[(size_t)(a.view0 + v) * a.W + iu];
+                const float cw = fmaf(fu, uwq.y - uwq.x, uwq.x);
+                const auto rs = rsrc(v);
+                const uint32_t ub = (uint32_t)iu * S::stride;
 #pragma unroll
-                for (int k = 0; k < ZC; ++k) {
-                    const float sv = fmaf((float)k, dsv, sv0);
-                    const float t = __builtin_amdgcn_fmed3f(sv, 0.0f, Vm1);
-                    const bool in = (t == sv);
-                    const float fv = __builtin_amdgcn_fractf(t);
-                    const float4 c4 = *reinterpret_cast<const float4*>(lbase + __umul24((uint32_t)(int)t, lrow) + lb);
-                    const float val = fmaf(fv, fmaf(fu, c4.w, c4.z), fmaf(fu, c4.y, c4.x));
-                    acc[k] += in ? val : 0.0f;
-                    cnt[k] += in ? cw : 0.0f;
-                }
-            }
-        } else {
-            // footprint larger than the LDS budget: gathers for this view
-            const uint32_t ub = (uint32_t)iu * S::bytes;
+                for (int kb = 0; kb < ZC; kb += kZB) {
+                    uint32_t off[kZB];
+                    float fv[kZB];
+                    bool in[kZB];
 #pragma unroll
-            for (int kb = 0; kb < ZC; kb += kZB) {
-                uint32_t off[kZB];
-                float fv[kZB];
-                bool in[kZB];
+                    for (int q = 0; q < kZB; ++q) {
+                        const float sv = fmaf((float)(kb + q), dsv, sv0);
+                        const float t = __builtin_amdgcn_fmed3f(sv, 0.0f, Vm1);
+                        in[q] = (t == sv);
+                        fv[q] = __builtin_amdgcn_fractf(t);
+                        off[q] = __umul24((uint32_t)(int)t, rowb) + ub;
+                    }
+                    typename S::raw_t p[kZB];
 #pragma unroll
-                for (int q = 0; q < kZB; ++q) {
-                    const float sv = fmaf((float)(kb + q), dsv, sv0);
-                    const float t = __builtin_amdgcn_fmed3f(sv, 0.0f, Vm1);
-                    in[q] = (t == sv);
-                    fv[q] = __builtin_amdgcn_fractf(t);
-                    off[q] = __umul24((uint32_t)(int)t, rowb) + ub;
-                }
-                typename S::raw_t p[kZB];
+                    for (int q = 0; q < kZB; ++q) p[q] = S::load(rs, off[q]);
 #pragma unroll
-                for (int q = 0; q < kZB; ++q) p[q] = S::load(rs, off[q]);
-#pragma unroll
-                for (int q = 0; q < kZB; ++q) {
-                    const float top = S::lerp(p[q], 0, fu);
-                    const float bot = S::lerp(p[q], 1, fu);
-                    const float val = fmaf(fv[q], bot - top, top);
-                    acc[kb + q] += in[q] ? val : 0.0f;
-                    cnt[kb + q] += in[q] ? cw : 0.0f;
+                    for (int q = 0; q < kZB; ++q) {
+                        const float top = S::lerp(p[q], 0, fu);
+                        const float bot = S::lerp(p[q], 1, fu);
+                        const float val = fmaf(fv[q], bot - top, top);
+                        acc[kb + q] += in[q] ? val : 0.0f;
+                        cnt[kb + q] += in[q] ? cw : 0.0f;
+                    }
                 }
             }
         }
+        // view v+1's footprint into the other buffer (its last reader was view
+        // v-1, before the previous barrier), then one barrier: the buffer and
+        // box v+2 are ready and every read of this view's buffer is done
+        if (v + 1 < n) commit(v + 1, (v + 1) % 3, buf ^ 1);
+        __syncthreads();
+        if (v + 2 < n) issue(v + 2, (v + 2) % 3);
     }
 
     if (i >= a.X || j >= a.Y) return;
@@ -823,7 +990,14 @@ static void launch_bp(jh_bp_plan* p, const void* q, int dtype, int view0, int nv
     bp_kernel_t k = out == kOutNormalised   ? pick_kernel<kOutNormalised>(p->mode, p->zc, dtype, p->fetch)
                     : out == kOutAccumulate ? pick_kernel<kOutAccumulate>(p->mode, p->zc, dtype, p->fetch)
                                             : pick_kernel<kOutRaw>(p->mode, p->zc, dtype, p->fetch);
-    hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(64, p->ty), 0, stream, a);
+    // LDS-staged fetch: two buffers sized for the block footprints of this
+    // tile depth (a larger footprint gathers), 64 zc pixels each, capped so
+    // that both, with the kernel's static box ring and count-weight rows
+    // (4 KiB + 48 B), stay within 64 KiB
+    const bool staged = p->mode == 0 && p->fetch == JH_FETCH_LDS;
+    a.lds_px = staged ? std::min(64 * p->zc, (65536 - 4096 - 64) / 32) : 0;
+    const size_t lds_bytes = staged ? 2 * (size_t)a.lds_px * sizeof(float4) : 0;
+    hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(64, p->ty), lds_bytes, stream, a);
     check_launch("bp_kernel");
 }
 
@@ -917,14 +1091,20 @@ static void stage(jh_bp_plan* p, const float* d_filtered, int view0, int nviews,
         require(dtype == JH_F32 || dtype == JH_BF16 || dtype == JH_F16, "jh_bp_stage: unknown dtype");
         if (nviews == 0) return;
         DeviceScope ds(p->device);
-        const size_t total = (size_t)p->W * p->V * nviews;
-        const float* uw = weighted ? p->uw.as<float>() + (size_t)view0 * p->W : nullptr;
-        const unsigned grid = grid_for(total, 256);
+        require(p->V <= 65535, "jh_bp_stage: more than 65535 detector rows");
         hipStream_t s = as_stream(stream);
         const int zf = weighted ? p->zfilter : 0;
-        if (dtype == JH_F32) bp_stage_kernel<JH_F32><<<grid, 256, 0, s>>>(d_filtered, uw, p->W, p->V, nviews, zf, d_q);
-        else if (dtype == JH_BF16) bp_stage_kernel<JH_BF16><<<grid, 256, 0, s>>>(d_filtered, uw, p->W, p->V, nviews, zf, d_q);
-        else bp_stage_kernel<JH_F16><<<grid, 256, 0, s>>>(d_filtered, uw, p->W, p->V, nviews, zf, d_q);
+        const size_t fin = (size_t)p->W * p->V, fout = fin * dtype_bytes_quad(dtype);
+        for (int c0 = 0; c0 < nviews; c0 += 65535) {  // grid z: frames
+            const int n = std::min(nviews - c0, 65535);
+            const float* uw = weighted ? p->uw.as<float>() + (size_t)(view0 + c0) * p->W : nullptr;
+            const float* in = d_filtered + (size_t)c0 * fin;
+            void* out = static_cast<char*>(d_q) + (size_t)c0 * fout;
+            const dim3 grid((unsigned)((p->W + 255) / 256), (unsigned)p->V, (unsigned)n);
+            if (dtype == JH_F32) bp_stage_kernel<JH_F32><<<grid, 256, 0, s>>>(in, uw, p->W, p->V, zf, out);
+            else if (dtype == JH_BF16) bp_stage_kernel<JH_BF16><<<grid, 256, 0, s>>>(in, uw, p->W, p->V, zf, out);
+            else bp_stage_kernel<JH_F16><<<grid, 256, 0, s>>>(in, uw, p->W, p->V, zf, out);
+        }
         check_launch("bp_stage_kernel");
     }
 }
