@@ -24,7 +24,7 @@ rel-RMSE against the FP64 oracle on the centre slices.
 The roofline object covers the dominant kernel (bp_kernel) timed with events
 on the stream it runs on; its algorithmic bytes are 8 B per voxel-update (an
 FP32 accumulator read + write per update, the reference's per-view
-formulation).  cpu_baseline times the CPU oracle's FP32 backprojection on a
+formulation); its traffic is the measured HBM bytes per launch.  cpu_baseline times the CPU oracle's FP32 backprojection on a
 bounded sample of slices on the host cores.
 """
 import argparse
@@ -64,6 +64,8 @@ def main():
                     help="profiling run: a 1 GiB copy, then only the backprojection kernel")
     ap.add_argument("--kernel-form", choices=("fp32", "bf16", "f16"), default="bf16",
                     help="storage form of the --kernel-only backprojection")
+    ap.add_argument("--kernel-tile", default=None,
+                    help="zc,ty tile of the --kernel-only backprojection (default: the autotuner's choice)")
     ap.add_argument("--workload", choices=("dental512", "batch16", "offset_dental512"), default="dental512")
     ap.add_argument("--branches", type=int, default=4, help="parallel graph branches (batch16)")
     ap.add_argument("--batch-storage", choices=("bf16", "f16", "f32"), default="bf16",
@@ -107,6 +109,10 @@ def main():
         torch.cuda.synchronize()
         fdk = forms[args.kernel_form]
         q = fdk.bp.stage(fdk.filter.process(fdk.pre.process(raw, dark, gain)), fdk.storage)
+        if args.kernel_tile:
+            fdk.bp.set_tile(*(int(x) for x in args.kernel_tile.split(",")))
+        else:
+            fdk.bp.autotune(q, fdk.storage, reps=1)
         vol = fdk.bp.run(q, fdk.storage)
         torch.cuda.synchronize()
         t = time.perf_counter()
@@ -249,11 +255,12 @@ def main():
         "accuracy": acc,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
                      "frac": achieved / 8000.0, "traffic": traffic,
-                     "kernel": "bp_kernel (bf16 quad storage)", "bytes_per_update": 8,
+                     "kernel": "bp_kernel (bf16 strip storage)", "bytes_per_update": 8,
                      "note": "achieved counts the per-view formulation's 8 B/voxel-update (FP32 accumulator "
                              "read + write); frac > 1 means the register-accumulating kernel beats that roof. "
-                             "Measured limiter: gather addresses (texture addresser ~97% busy, "
-                             "profiles/bp_dental512_pmc.md)"},
+                             "traffic: measured HBM bytes per launch (FETCH_SIZE + WRITE_SIZE). Measured "
+                             "limiter: the vector-memory path's cache-line rate (22 L1 accesses per b128 "
+                             "gather, ~25 cycles per gather per CU, profiles/bp_dental512_pmc.md)"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

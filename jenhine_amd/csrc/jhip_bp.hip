@@ -169,7 +169,7 @@ __global__ void __launch_bounds__(256) zfilter_kernel(const float* __restrict__ 
 template <int QT>
 __global__ void __launch_bounds__(256) bp_stage_kernel(const float* __restrict__ filt,
                                                        const float* __restrict__ uw, int W, int V, int zf,
-                                                       void* __restrict__ out)
+                                                       int strip, void* __restrict__ out)
 // uw == NULL packs the frames without weighting; zf != 0 applies the v
 // smoothing of ZFilterAgent to the frames first (before the uW weight, as in
 // the reference chain Filter -> ZFilter -> BPCore::InitView).
@@ -185,7 +185,7 @@ __global__ void __launch_bounds__(256) bp_stage_kernel(const float* __restrict__
     const int u1 = right ? u + 1 : u;
     const float wa = w ? w[u] : 1.0f;
     const float wb = w && right ? w[u + 1] : 1.0f;
-    constexpr int NR = QT == JH_F32 ? 2 : 4;  // rows of the quad or the strip
+    constexpr int NR = QT == JH_F32 ? 2 : 4;  // rows of the quad, or at most of the strip
     // rows row-1 .. row+NR of columns u, u+1, clamped into the frame
     float xa[NR + 2], xb[NR + 2];
 #pragma unroll
@@ -216,10 +216,18 @@ __global__ void __launch_bounds__(256) bp_stage_kernel(const float* __restrict__
         u32x4_t o;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
+            if (k >= 2 && !strip) break;
             const float a = val(xa, k, wa, true), b = val(xb, k, wb, right);
             o[k] = QT == JH_BF16 ? (f32_to_bf16_rne(a) | (f32_to_bf16_rne(b) << 16)) : pack_f16x2(a, b);
         }
-        reinterpret_cast<u32x4_t*>(out)[t] = o;
+        if (strip) {
+            reinterpret_cast<u32x4_t*>(out)[t] = o;
+        } else {
+            u32x2_t q;
+            q.x = o.x;
+            q.y = o.y;
+            reinterpret_cast<u32x2_t*>(out)[t] = q;
+        }
     }
 }
 
@@ -249,7 +257,6 @@ template <>
 struct Sampler<JH_F32> {
     typedef u32x4_t raw_t;
     static constexpr int bytes = 16;   // bytes per sample load
-    static constexpr int stride = 16;  // bytes per staged pixel
     __device__ static raw_t load(__amdgpu_buffer_rsrc_t r, uint32_t off)
     {
         return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
@@ -272,7 +279,6 @@ template <>
 struct Sampler<JH_BF16> {
     typedef u32x2_t raw_t;
     static constexpr int bytes = 8;
-    static constexpr int stride = 16;
     __device__ static raw_t load(__amdgpu_buffer_rsrc_t r, uint32_t off)
     {
         return __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
@@ -302,7 +308,6 @@ template <>
 struct Sampler<JH_F16> {
     typedef u32x2_t raw_t;
     static constexpr int bytes = 8;
-    static constexpr int stride = 16;
     __device__ static raw_t load(__amdgpu_buffer_rsrc_t r, uint32_t off)
     {
         return __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
@@ -349,6 +354,8 @@ struct BPArgs {
     int ct;              // apply the CT-number map to normalised output
     float ct_scale, ct_norm1;
     int lds_px;          // LDS-staged fetch: pixels per coefficient buffer (two buffers)
+    int pxb;             // bytes per staged pixel (16: FP32 quads, BF16/FP16 strips; 8: BF16/FP16 quads)
+    int strip;           // BF16/FP16 staged as strips
 };
 
 constexpr int kZB = 8;  // samples whose loads are issued together
@@ -420,7 +427,7 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
     const float fj = (float)min(j, a.Y - 1);
     const float fk0 = (float)k0;
     const float Wm1 = (float)(a.W - 1), Vm1 = (float)(a.V - 1);
-    const uint32_t rowb = (uint32_t)a.W * S::stride;
+    const uint32_t rowb = (uint32_t)a.W * (uint32_t)a.pxb;
 
     float acc[ZC], cnt[ZC];
 #pragma unroll
@@ -447,7 +454,7 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
                 const float cw = fmaf(fu, uwq.y - uwq.x, uwq.x);
                 const float sv0 = fmaf(Vc, rw, 0.5f);
                 const float dsv = g.c[7] * rw;
-                const uint32_t ub = (uint32_t)iu * S::stride;
+                const uint32_t ub = (uint32_t)iu * (uint32_t)a.pxb;
                 // sv is monotonic in k, so the column is inside the detector
                 // rows iff its first and last samples are.
                 const float svl = fmaf((float)(ZC - 1), dsv, sv0);
@@ -464,7 +471,8 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
                     // d = floor(sv_k+1) - floor(sv_k), which is hi or hi+1
                     bool paired = false;
                     if constexpr (QT != JH_F32) {
-                        if (dsv >= 0.0f && dsv <= 0.984375f) {
+                        if (!a.strip) {
+                        } else if (dsv >= 0.0f && dsv <= 0.984375f) {
                             bp_column_pairs<ZC, QT, 0>(acc, rs, rowb, ub, sv0, dsv, fu);
                             paired = true;
                         } else if (dsv >= 1.015625f && dsv <= 1.984375f) {
@@ -549,7 +557,7 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
                     const int iu = (int)tu;
                     fu[q] = __builtin_amdgcn_fractf(tu);
                     fv[q] = __builtin_amdgcn_fractf(t);
-                    off[q] = __umul24((uint32_t)(int)t, rowb) + (uint32_t)iu * S::stride;
+                    off[q] = __umul24((uint32_t)(int)t, rowb) + (uint32_t)iu * (uint32_t)a.pxb;
                     uwq[q] = uwv[iu];
                 }
                 typename S::raw_t p[kZB];
@@ -646,7 +654,7 @@ __device__ __forceinline__ void bp_tile_lds(const BPArgs& a, unsigned tile)
     const float cx0 = (float)min(tx * ex, a.X - 1), cx1 = (float)min(tx * ex + ex - 1, a.X - 1);
     const float cy0 = (float)min(tyi * ey, a.Y - 1), cy1 = (float)min(tyi * ey + ey - 1, a.Y - 1);
     const float Wm1 = (float)(a.W - 1), Vm1 = (float)(a.V - 1);
-    const uint32_t rowb = (uint32_t)a.W * S::stride;
+    const uint32_t rowb = (uint32_t)a.W * (uint32_t)a.pxb;
     const int nthr = 64 * blockDim.y;
     const int tid = threadIdx.y * 64 + threadIdx.x;
     const int cap = a.lds_px;
@@ -698,7 +706,7 @@ __device__ __forceinline__ void bp_tile_lds(const BPArgs& a, unsigned tile)
     auto pixel_offset = [&](int b, int u0, int v0, int wdt, float rinv) {
         const int r = (int)(((float)b + 0.5f) * rinv);
         const int c = b - r * wdt;
-        return (uint32_t)(v0 + r) * rowb + (uint32_t)(u0 + c) * S::stride;
+        return (uint32_t)(v0 + r) * rowb + (uint32_t)(u0 + c) * (uint32_t)a.pxb;
     };
 
     // a view is staged when its footprint and count-weight row fit
@@ -827,7 +835,7 @@ __device__ __forceinline__ void bp_tile_lds(const BPArgs& a, unsigned tile)
                 const float2 uwq = a.uwp[(size_t)(a.view0 + v) * a.W + iu];
                 const float cw = fmaf(fu, uwq.y - uwq.x, uwq.x);
                 const auto rs = rsrc(v);
-                const uint32_t ub = (uint32_t)iu * S::stride;
+                const uint32_t ub = (uint32_t)iu * (uint32_t)a.pxb;
 #pragma unroll
                 for (int kb = 0; kb < ZC; kb += kZB) {
                     uint32_t off[kZB];
@@ -961,6 +969,13 @@ static bp_kernel_t pick_kernel(int mode, int zc, int dtype, int fetch)
     }
 }
 
+// Bytes per staged detector pixel: the FP32 quad (16), the BF16 / FP16 quad
+// (8) or the BF16 / FP16 strip (16).
+static int stage_pixel_bytes(const jh_bp_plan* p, int dtype)
+{
+    return dtype == JH_F32 ? 16 : (p->strip ? 16 : 8);
+}
+
 static void launch_bp(jh_bp_plan* p, const void* q, int dtype, int view0, int nviews, float* vol,
                       int out, hipStream_t stream)
 {
@@ -981,7 +996,9 @@ static void launch_bp(jh_bp_plan* p, const void* q, int dtype, int view0, int nv
     a.ntx = (p->X + ex - 1) / ex;
     a.nty = (p->Y + ey - 1) / ey;
     a.ntz = (p->Z + p->zc - 1) / p->zc;
-    const size_t fb = (size_t)p->W * p->V * dtype_bytes_quad(dtype);
+    const size_t fb = (size_t)p->W * p->V * stage_pixel_bytes(p, dtype);
+    a.pxb = stage_pixel_bytes(p, dtype);
+    a.strip = dtype != JH_F32 && p->strip;
     require(fb < (1ull << 31), "backprojection frame too large");
     a.frame_bytes = (uint32_t)fb;
     const size_t ntiles = (size_t)a.ntx * a.nty * a.ntz;
@@ -1030,6 +1047,7 @@ jh_status jh_bp_create(jh_bp_plan** plan, int device, const double* h_ptm, int n
             p->zc = 32;
             p->ty = 4;
             p->fetch = JH_FETCH_GATHER;
+            p->strip = 0;
             p->zfilter = 0;
             p->ct = 0;
             p->ct_scale = 1.0f;
@@ -1056,6 +1074,18 @@ jh_status jh_bp_create(jh_bp_plan** plan, int device, const double* h_ptm, int n
                 if (e.c[11] != 0.0f) zinv_w = false;
             }
             p->mode = (zinv_u && zinv_w) ? 0 : (zinv_w ? 1 : 2);
+            // BF16 / FP16 layout: strips pay off when the backprojector's pair
+            // path applies, i.e. when the row step per z voxel at the volume
+            // centre lies in [0, 0.984] or [1.016, 1.984] for most views
+            // (class 0 only); otherwise quads (a strip's stride would halve
+            // the pixels per cache line of the one-gather-per-sample path)
+            int pairable = 0;
+            const float ic = 0.5f * (float)(X - 1), jc = 0.5f * (float)(Y - 1);
+            for (const Geo& e : g) {
+                const float dsv = e.c[7] / (e.c[8] + e.c[9] * ic + e.c[10] * jc);
+                if ((dsv >= 0.0f && dsv <= 0.984375f) || (dsv >= 1.015625f && dsv <= 1.984375f)) ++pairable;
+            }
+            p->strip = p->mode == 0 && 2 * pairable >= nviews ? 1 : 0;
         } catch (...) {
             delete p;
             throw;
@@ -1078,7 +1108,7 @@ jh_status jh_bp_stage_bytes(jh_bp_plan* p, int dtype, int nviews, size_t* bytes)
     return guard([&] {
         require(p != nullptr && bytes != nullptr && nviews >= 0, "jh_bp_stage_bytes: bad arguments");
         require(dtype == JH_F32 || dtype == JH_BF16 || dtype == JH_F16, "jh_bp_stage_bytes: unknown dtype");
-        *bytes = (size_t)p->W * p->V * nviews * dtype_bytes_quad(dtype);
+        *bytes = (size_t)p->W * p->V * nviews * stage_pixel_bytes(p, dtype);
     });
 }
 
@@ -1094,16 +1124,17 @@ static void stage(jh_bp_plan* p, const float* d_filtered, int view0, int nviews,
         require(p->V <= 65535, "jh_bp_stage: more than 65535 detector rows");
         hipStream_t s = as_stream(stream);
         const int zf = weighted ? p->zfilter : 0;
-        const size_t fin = (size_t)p->W * p->V, fout = fin * dtype_bytes_quad(dtype);
+        const size_t fin = (size_t)p->W * p->V, fout = fin * stage_pixel_bytes(p, dtype);
         for (int c0 = 0; c0 < nviews; c0 += 65535) {  // grid z: frames
             const int n = std::min(nviews - c0, 65535);
             const float* uw = weighted ? p->uw.as<float>() + (size_t)(view0 + c0) * p->W : nullptr;
             const float* in = d_filtered + (size_t)c0 * fin;
             void* out = static_cast<char*>(d_q) + (size_t)c0 * fout;
             const dim3 grid((unsigned)((p->W + 255) / 256), (unsigned)p->V, (unsigned)n);
-            if (dtype == JH_F32) bp_stage_kernel<JH_F32><<<grid, 256, 0, s>>>(in, uw, p->W, p->V, zf, out);
-            else if (dtype == JH_BF16) bp_stage_kernel<JH_BF16><<<grid, 256, 0, s>>>(in, uw, p->W, p->V, zf, out);
-            else bp_stage_kernel<JH_F16><<<grid, 256, 0, s>>>(in, uw, p->W, p->V, zf, out);
+            const int st = p->strip;
+            if (dtype == JH_F32) bp_stage_kernel<JH_F32><<<grid, 256, 0, s>>>(in, uw, p->W, p->V, zf, 0, out);
+            else if (dtype == JH_BF16) bp_stage_kernel<JH_BF16><<<grid, 256, 0, s>>>(in, uw, p->W, p->V, zf, st, out);
+            else bp_stage_kernel<JH_F16><<<grid, 256, 0, s>>>(in, uw, p->W, p->V, zf, st, out);
         }
         check_launch("bp_stage_kernel");
     }
@@ -1203,6 +1234,23 @@ jh_status jh_bp_set_tile(jh_bp_plan* p, int zc, int ty)
         require(ty == 0 || ty == 1 || ty == 2 || ty == 4, "jh_bp_set_tile: ty must be 1, 2 or 4");
         if (zc) p->zc = zc;
         if (ty) p->ty = ty;
+    });
+}
+
+jh_status jh_bp_set_layout(jh_bp_plan* p, int layout)
+{
+    return guard([&] {
+        require(p != nullptr, "jh_bp_set_layout: plan is NULL");
+        require(layout == JH_LAYOUT_QUAD || layout == JH_LAYOUT_STRIP, "jh_bp_set_layout: unknown layout");
+        p->strip = layout == JH_LAYOUT_STRIP ? 1 : 0;
+    });
+}
+
+jh_status jh_bp_get_layout(jh_bp_plan* p, int* layout)
+{
+    return guard([&] {
+        require(p != nullptr && layout != nullptr, "jh_bp_get_layout: bad arguments");
+        *layout = p->strip ? JH_LAYOUT_STRIP : JH_LAYOUT_QUAD;
     });
 }
 

@@ -21,6 +21,7 @@ struct jh_bp_plan {
     int mode;        // 0: u and w constant along z, 1: w constant along z, 2: general
     int zc, ty;      // backprojector tile: z voxels per thread, waves per workgroup
     int fetch;       // JH_FETCH_GATHER or JH_FETCH_LDS (geometry class 0 only)
+    int strip;       // BF16/FP16 staged as four-row strips (JH_LAYOUT_STRIP) or quads
     int zfilter;     // staging applies the 1-2-1 v smoothing first
     int ct;          // normalised output goes through the CT-number map
     float ct_scale, ct_norm1;

@@ -145,14 +145,6 @@ inline void check_launch(const char* what)
     if (e != hipSuccess) fail(JH_ERR_HIP, std::string("launch of ") + what + ": " + hipGetErrorString(e));
 }
 
-inline int dtype_bytes_quad(int dtype)
-{
-    // Storage bytes per detector pixel in the backprojector's layout: the
-    // FP32 quad (the 2 x 2 neighbourhood of a bilinear sample) or the
-    // BF16 / FP16 strip (column pairs of four rows).
-    (void)dtype;
-    return 16;
-}
 
 // Host filter design (FP64): H[k], k = 0..L/2, incl. the 2*pi*f_N gain.
 std::vector<double> design_filter(int W, double hsr, double cutStart, double cutEnd,

@@ -266,6 +266,17 @@ class BPCore:
     def set_fetch(self, fetch):
         call("jh_bp_set_fetch", self._h, self.FETCH[fetch])
 
+    LAYOUT = {"quad": 0, "strip": 1}
+
+    def set_layout(self, layout):
+        """Staged BF16/FP16 layout ("quad" or "strip"); stage again after a change."""
+        call("jh_bp_set_layout", self._h, self.LAYOUT[layout])
+
+    def layout(self):
+        v = ctypes.c_int()
+        call("jh_bp_get_layout", self._h, ctypes.byref(v))
+        return {v2: k for k, v2 in self.LAYOUT.items()}[v.value]
+
     def fetch(self):
         f = ctypes.c_int()
         call("jh_bp_get_fetch", self._h, ctypes.byref(f))
