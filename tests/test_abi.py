@@ -38,3 +38,25 @@ def test_error_path_without_device():
     st = L.jh_bp_create(ctypes.byref(h), 0, None, 0, 0, 0, 0, 0, 0, ctypes.c_float(0), ctypes.c_float(0))
     assert st == _lib.JH_ERR_INVALID
     assert b"projection" in L.jh_last_error()
+
+
+def test_detector_side_and_argument_checks_without_device():
+    import numpy as np
+    from jenhine_amd import configs
+    from oracle import oracle as o
+    L = _lib.lib()
+    for shift in (48.0, -48.0, 0.0):
+        P = np.ascontiguousarray(configs.OFFSET128.scaled(u_shift=shift).ptms()[0], np.float64)
+        got = L.jh_detector_on_right(P.ctypes.data_as(ctypes.c_void_p), 160)
+        assert bool(got) == o.detector_on_right(P, 160) == (shift >= 0.0)
+    h = ctypes.c_void_p()
+    P = np.zeros((15, 12))
+    # an odd view count has no conjugate pairs
+    st = L.jh_composite_create(ctypes.byref(h), 0, 64, 32, 15, P.ctypes.data_as(ctypes.c_void_p),
+                               ctypes.c_float(300.0), -1)
+    assert st == _lib.JH_ERR_INVALID and b"even" in L.jh_last_error()
+    # yMirroring keeps output columns [Wout, 2 Wout), which must lie inside the FFT length
+    st = L.jh_filter_create_ex(ctypes.byref(h), 0, 64, 8, 80, 1, 4, P.ctypes.data_as(ctypes.c_void_p),
+                               ctypes.c_float(300.0), ctypes.c_float(1.0), ctypes.c_float(0), ctypes.c_float(0),
+                               None, None, 0, 0, 0)
+    assert st == _lib.JH_ERR_INVALID and b"FFT length" in L.jh_last_error()
