@@ -45,7 +45,7 @@ typedef int jh_status;
 #define JH_FETCH_LDS 1
 
 #define JH_LAYOUT_QUAD 0  /* staged BF16/FP16 pixel: its 2 x 2 neighbourhood (8 B) */
-#define JH_LAYOUT_STRIP 1 /* staged BF16/FP16 pixel: column pairs of four rows (16 B) */
+#define JH_LAYOUT_STRIP 1 /* staged BF16/FP16 pixel: four rows of its column (8 B) */
 
 #define JH_WINDOW_REFERENCE 0   /* FilterCore.cpp:96-134 cut-off semantics */
 #define JH_WINDOW_SHEPP_LOGAN 1 /* extension: sinc(f / (2 f_N)), zero above f_N */
@@ -271,15 +271,16 @@ jh_status jh_bp_set_tile(jh_bp_plan* plan, int zc, int ty);
 jh_status jh_bp_set_fetch(jh_bp_plan* plan, int fetch);
 jh_status jh_bp_get_fetch(jh_bp_plan* plan, int* fetch);
 
-/* Staged layout of BF16/FP16 projections: JH_LAYOUT_QUAD (a pixel holds the
- * 2 x 2 neighbourhood of a bilinear sample) or JH_LAYOUT_STRIP (the column
- * pairs of four rows, so one 16-byte gather serves two consecutive z samples
- * of a voxel column).  jh_bp_create picks strips when, at the volume centre,
- * the detector row step per z voxel of most views lies in [0, 0.984] or
- * [1.016, 1.984] (geometry class 0), else quads.  Changing it changes
- * jh_bp_stage_bytes and the format jh_bp_stage writes: stage again after a
- * change.  FP32 always uses its 16-byte quad.  No counterpart in the
- * reference (textures, BPCUDACore.cu:62). */
+/* Staged layout of BF16/FP16 projections, 8 bytes per pixel either way:
+ * JH_LAYOUT_QUAD (a pixel holds the 2 x 2 neighbourhood of a bilinear
+ * sample) or JH_LAYOUT_STRIP (a pixel holds four rows v .. v+3 of its
+ * column, rows padded with one zero pixel; a 16-byte gather then reads
+ * columns u and u+1 of four rows and serves two consecutive z samples of a
+ * voxel column).  jh_bp_create picks strips for geometry class 0 and quads
+ * otherwise.  Changing it changes the format jh_bp_stage writes (and
+ * jh_bp_stage_bytes): stage again after a change.  FP32 always uses its
+ * 16-byte quad.  No counterpart in the reference (textures,
+ * BPCUDACore.cu:62). */
 jh_status jh_bp_set_layout(jh_bp_plan* plan, int layout);
 jh_status jh_bp_get_layout(jh_bp_plan* plan, int* layout);
 

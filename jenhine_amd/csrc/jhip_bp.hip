@@ -80,27 +80,29 @@ __global__ void bp_uweight_kernel(const double* __restrict__ ptm, int N, int W, 
 }
 
 // ---------------------------------------------------------------------------
-// Staging: q = filtered * uW in the quad layout.
+// Staging: q = filtered * uW in the backprojector's layout.
 //
-// Frame layout: V rows of W pixels; pixel (u, v) holds the 2 x 2 neighbourhood
-// of a bilinear sample whose integer part is (u, v), with q[v][W] = 0 and
+// Frame layout: V rows of pixels; the data a bilinear sample whose integer
+// part is (u, v) needs is one aligned gather, with q[v][W] = 0 and
 // q[V][u] = 0 (so a sample at u <= W-1, v <= V-1 needs no clamping: at the
-// far edges the weight of the padding is exactly 0), and one sample is one
-// aligned gather.  The gathers are limited by the cache lines each wave
-// instruction touches, not by bytes, so the 2x-redundant layout halves
-// their cost.
+// far edges the weight of the padding is exactly 0).  The gathers are
+// limited by the cache lines each wave instruction touches, not by bytes.
 //  * FP32 (16 B): (q[v][u], q[v][u+1] - q[v][u], q[v+1][u], q[v+1][u+1] -
 //    q[v+1][u]).  Each row interpolation is one FMA, q + fu d, and d is the
 //    difference the interpolation would form anyway (same result to the bit).
-//  * BF16 / FP16 (16 B, "strip"): the pairs (q[r][u], q[r][u+1]) of the four
-//    rows r = v .. v+3, one 32-bit word per row, each value rounded once.
-//    The first 8 bytes are the sample's 2 x 2 neighbourhood (one b64
-//    gather); a b128 gather serves two consecutive z samples of a column
-//    whose row step is below 2 pixels (the backprojector's fast path), which
-//    halves the gather addresses.  Storing differences instead makes the
-//    rounding error scale with |d|, which exceeds |q| on ramp-filtered
-//    noise, and a BF16 weight for v_dot2 would move the sample by up to 2^-9
-//    pixel; both measurably raise the error.
+//  * BF16 / FP16 quad (8 B): (q[v][u], q[v][u+1], q[v+1][u], q[v+1][u+1]),
+//    each value rounded once.
+//  * BF16 / FP16 column strip (8 B, the default for geometry class 0): the
+//    four rows q[v..v+3][u] of column u, rows padded to W + 1 pixels.  A
+//    16-byte gather at (u, v) reads columns u and u+1 of four rows: the
+//    sample's 2 x 2 neighbourhood, and that of the next z sample of the
+//    column when the row step is below 2 pixels (the backprojector's pair
+//    path), which halves the gathers; neighbouring lanes' windows overlap,
+//    so a wave touches fewer cache lines than with 16-byte-per-pixel data.
+//    Storing differences instead makes the rounding error scale with |d|,
+//    which exceeds |q| on ramp-filtered noise, and a BF16 weight for v_dot2
+//    would move the sample by up to 2^-9 pixel; both measurably raise the
+//    error.
 // ---------------------------------------------------------------------------
 
 __device__ inline uint32_t f32_to_bf16_rne(float f)
@@ -165,34 +167,43 @@ __global__ void __launch_bounds__(256) zfilter_kernel(const float* __restrict__ 
 }
 
 // One thread per staged pixel: blockIdx.x * 256 + threadIdx.x = u,
-// blockIdx.y = row, blockIdx.z = frame.
-template <int QT>
+// blockIdx.y = row, blockIdx.z = frame.  Layouts: FP32 quads (16 B), BF16 /
+// FP16 quads (8 B) or, with strip, BF16 / FP16 column strips (8 B, rows
+// padded to W + 1 pixels; thread u = W writes the zero padding).
+template <int QT, bool strip>
 __global__ void __launch_bounds__(256) bp_stage_kernel(const float* __restrict__ filt,
                                                        const float* __restrict__ uw, int W, int V, int zf,
-                                                       int strip, void* __restrict__ out)
+                                                       void* __restrict__ out)
 // uw == NULL packs the frames without weighting; zf != 0 applies the v
 // smoothing of ZFilterAgent to the frames first (before the uW weight, as in
 // the reference chain Filter -> ZFilter -> BPCore::InitView).
 {
     const int u = blockIdx.x * 256 + threadIdx.x;
-    if (u >= W) return;
+    const int rowpx = W + (strip ? 1 : 0);
+    static_assert(!(strip && QT == JH_F32), "FP32 is staged as quads");
+    if (u >= rowpx) return;
     const int row = blockIdx.y;
     const size_t view = blockIdx.z;
     const size_t frame = (size_t)W * V;
+    const size_t t = (view * V + row) * (size_t)rowpx + u;
+    if (u == W) {  // strip padding column
+        reinterpret_cast<u32x2_t*>(out)[t] = u32x2_t{0u, 0u};
+        return;
+    }
     const float* fr = filt + view * frame;
     const float* w = uw ? uw + view * W : nullptr;
     const bool right = u + 1 < W;
     const int u1 = right ? u + 1 : u;
     const float wa = w ? w[u] : 1.0f;
     const float wb = w && right ? w[u + 1] : 1.0f;
-    constexpr int NR = QT == JH_F32 ? 2 : 4;  // rows of the quad, or at most of the strip
-    // rows row-1 .. row+NR of columns u, u+1, clamped into the frame
+    constexpr int NR = strip ? 4 : 2;  // rows of the quad or the strip
+    // rows row-1 .. row+NR of columns u (and u+1 for quads), clamped into the frame
     float xa[NR + 2], xb[NR + 2];
 #pragma unroll
     for (int k = 0; k < NR + 2; ++k) {
         const int r = min(max(row - 1 + k, 0), V - 1);
         xa[k] = fr[(size_t)r * W + u];
-        xb[k] = fr[(size_t)r * W + u1];
+        xb[k] = strip ? 0.0f : fr[(size_t)r * W + u1];
     }
     // value of row row+k (k < NR) at column u or u+1, 0 beyond the frame
     auto val = [&](const float* x, int k, float wgt, bool inside) {
@@ -202,32 +213,30 @@ __global__ void __launch_bounds__(256) bp_stage_kernel(const float* __restrict__
 #pragma clang fp contract(off)
             // ZFilterAgent.cpp:49-71, as zfilter_at: the clamped neighbours
             // are the clamped loads above
-            const float t = v * 0.5f + x[k] * 0.25f;
-            v = t + x[k + 2] * 0.25f;
+            const float tt = v * 0.5f + x[k] * 0.25f;
+            v = tt + x[k + 2] * 0.25f;
         }
         return v * wgt;
     };
-    const size_t t = view * frame + (size_t)row * W + u;
-    if (QT == JH_F32) {
+    auto pack = [&](float lo, float hi) {
+        return QT == JH_BF16 ? (f32_to_bf16_rne(lo) | (f32_to_bf16_rne(hi) << 16)) : pack_f16x2(lo, hi);
+    };
+    if constexpr (QT == JH_F32) {
         const float a0 = val(xa, 0, wa, true), b0 = val(xb, 0, wb, right);
         const float a1 = val(xa, 1, wa, true), b1 = val(xb, 1, wb, right);
         reinterpret_cast<float4*>(out)[t] = make_float4(a0, b0 - a0, a1, b1 - a1);
+    } else if constexpr (strip) {
+        // column u, rows row .. row+3
+        u32x2_t o;
+        o.x = pack(val(xa, 0, wa, true), val(xa, 1, wa, true));
+        o.y = pack(val(xa, 2, wa, true), val(xa, 3, wa, true));
+        reinterpret_cast<u32x2_t*>(out)[t] = o;
     } else {
-        u32x4_t o;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if (k >= 2 && !strip) break;
-            const float a = val(xa, k, wa, true), b = val(xb, k, wb, right);
-            o[k] = QT == JH_BF16 ? (f32_to_bf16_rne(a) | (f32_to_bf16_rne(b) << 16)) : pack_f16x2(a, b);
-        }
-        if (strip) {
-            reinterpret_cast<u32x4_t*>(out)[t] = o;
-        } else {
-            u32x2_t q;
-            q.x = o.x;
-            q.y = o.y;
-            reinterpret_cast<u32x2_t*>(out)[t] = q;
-        }
+        // rows row, row+1 of columns u, u+1
+        u32x2_t o;
+        o.x = pack(val(xa, 0, wa, true), val(xb, 0, wb, right));
+        o.y = pack(val(xa, 1, wa, true), val(xb, 1, wb, right));
+        reinterpret_cast<u32x2_t*>(out)[t] = o;
     }
 }
 
@@ -283,12 +292,7 @@ struct Sampler<JH_BF16> {
     {
         return __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
     }
-    // the whole strip: rows v .. v+3
-    __device__ static u32x4_t load4(__amdgpu_buffer_rsrc_t r, uint32_t off)
-    {
-        return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
-    }
-    // one row's pair interpolated at fu
+    // one row's pair (q[u], q[u+1]) interpolated at fu
     __device__ static float row(uint32_t x, float fu)
     {
         const float a = __uint_as_float(x << 16), b = __uint_as_float(x & 0xffff0000u);
@@ -311,10 +315,6 @@ struct Sampler<JH_F16> {
     __device__ static raw_t load(__amdgpu_buffer_rsrc_t r, uint32_t off)
     {
         return __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
-    }
-    __device__ static u32x4_t load4(__amdgpu_buffer_rsrc_t r, uint32_t off)
-    {
-        return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
     }
     __device__ static float row(uint32_t x, float fu)
     {
@@ -340,6 +340,66 @@ struct Sampler<JH_F16> {
     }
 };
 
+// Column strips (BF16 / FP16): staged pixel (u, v) holds the four rows
+// v .. v+3 of column u as two 32-bit words, w0 = (q[v][u], q[v+1][u]) and
+// w1 = (q[v+2][u], q[v+3][u]) (low half first), 8 bytes; rows are padded to
+// W + 1 pixels (the padding column is 0).  One 16-byte gather at (u, v)
+// returns p = (w0(u), w1(u), w0(u+1), w1(u+1)): columns u and u+1 of four
+// rows.  A word pair (wu, wu1) holding rows (t, t+1) of columns u and u+1
+// gives the sample's two row values; row t+1..t+2 of the strip is the word
+// alignbit(w1, w0, 16).
+constexpr int kBF16Strip = 4;  // sampler ids of the strip layout (kernel template argument QT)
+constexpr int kF16Strip = 5;
+
+template <int QT>
+struct StripSampler {
+    typedef u32x4_t raw_t;
+    static constexpr int bytes = 16;
+    __device__ static raw_t load(__amdgpu_buffer_rsrc_t r, uint32_t off)
+    {
+        return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+    }
+    __device__ static float lo(uint32_t w)
+    {
+        if constexpr (QT == kBF16Strip) return __uint_as_float(w << 16);
+        else return (float)__builtin_bit_cast(half2_t, w).x;
+    }
+    __device__ static float hi(uint32_t w)
+    {
+        if constexpr (QT == kBF16Strip) return __uint_as_float(w & 0xffff0000u);
+        else return (float)__builtin_bit_cast(half2_t, w).y;
+    }
+    // a + fu (b - a) in the operation order of the quad samplers
+    __device__ static float mix(float a, float b, float fu)
+    {
+        if constexpr (QT == kBF16Strip) return fmaf(fu, b - a, a);
+        else return fmaf(b, fu, fmaf(-a, fu, a));
+    }
+    // rows t (low halves) and t+1 (high halves) of columns u, u+1
+    __device__ static void rows(uint32_t wu, uint32_t wu1, float fu, float& top, float& bot)
+    {
+        top = mix(lo(wu), lo(wu1), fu);
+        bot = mix(hi(wu), hi(wu1), fu);
+    }
+    __device__ static float lerp(raw_t p, int r, float fu)
+    {
+        return r == 0 ? mix(lo(p.x), lo(p.z), fu) : mix(hi(p.x), hi(p.z), fu);
+    }
+    __device__ static float4 abcd(raw_t p)
+    {
+        const float q00 = lo(p.x), q01 = lo(p.z), q10 = hi(p.x), q11 = hi(p.z);
+        const float b = q01 - q00;
+        return make_float4(q00, b, q10 - q00, (q11 - q10) - b);
+    }
+};
+template <>
+struct Sampler<kBF16Strip> : StripSampler<kBF16Strip> {};
+template <>
+struct Sampler<kF16Strip> : StripSampler<kF16Strip> {};
+
+template <int QT>
+constexpr bool is_strip() { return QT == kBF16Strip || QT == kF16Strip; }
+
 struct BPArgs {
     const void* q;       // views view0..view0+nviews-1, quad layout
     const Geo* geo;      // all views
@@ -354,8 +414,8 @@ struct BPArgs {
     int ct;              // apply the CT-number map to normalised output
     float ct_scale, ct_norm1;
     int lds_px;          // LDS-staged fetch: pixels per coefficient buffer (two buffers)
-    int pxb;             // bytes per staged pixel (16: FP32 quads, BF16/FP16 strips; 8: BF16/FP16 quads)
-    int strip;           // BF16/FP16 staged as strips
+    int pxb;             // bytes per staged pixel (16: FP32 quads; 8: BF16/FP16 quads and strips)
+    int row_px;          // staged pixels per detector row (W, or W + 1 for strips)
 };
 
 constexpr int kZB = 8;  // samples whose loads are issued together
@@ -366,10 +426,11 @@ constexpr int kOutNormalised = 0;  // vol = acc / count (one-shot)
 constexpr int kOutAccumulate = 1;  // acc += ..., count += ... (streaming)
 constexpr int kOutRaw = 2;         // vol = acc (the un-normalised operator, FP32 storage only)
 
-// The samples of one all-in-range column in pairs over BF16 / FP16 strips:
-// one b128 gather at row floor(sv_k) serves samples k and k+1; the second
-// sample's rows start HI or HI + 1 rows down.  The arithmetic of each sample
-// is that of the one-gather-per-sample path, so the results are identical.
+// The samples of one all-in-range column in pairs over column strips: one
+// 16-byte gather at row floor(sv_k) serves samples k and k+1; the second
+// sample's rows start HI or HI + 1 rows down.  Each sample interpolates the
+// same four values with the same operations as the quad layout's one gather
+// per sample, so the results are identical.
 template <int ZC, int QT, int HI>
 __device__ __forceinline__ void bp_column_pairs(float* acc, __amdgpu_buffer_rsrc_t rs, uint32_t rowb, uint32_t ub,
                                                 float sv0, float dsv, float fu)
@@ -391,14 +452,19 @@ __device__ __forceinline__ void bp_column_pairs(float* acc, __amdgpu_buffer_rsrc
     }
     u32x4_t p[NP];
 #pragma unroll
-    for (int m = 0; m < NP; ++m) p[m] = S::load4(rs, off[m]);
+    for (int m = 0; m < NP; ++m) p[m] = S::load(rs, off[m]);
 #pragma unroll
     for (int m = 0; m < NP; ++m) {
-        const float t0 = S::row(p[m].x, fu), b0 = S::row(p[m].y, fu);
+        float t0, b0;
+        S::rows(p[m].x, p[m].z, fu, t0, b0);
         acc[2 * m] += fmaf(fa[m], b0 - t0, t0);
-        const uint32_t xt = HI == 0 ? (e[m] ? p[m].y : p[m].x) : (e[m] ? p[m].z : p[m].y);
-        const uint32_t xb = HI == 0 ? (e[m] ? p[m].z : p[m].y) : (e[m] ? p[m].w : p[m].z);
-        const float t1 = S::row(xt, fu), b1 = S::row(xb, fu);
+        // rows 1-2 of the strip: the high half of w0 and the low half of w1
+        const uint32_t m12u = __builtin_amdgcn_alignbit(p[m].y, p[m].x, 16);
+        const uint32_t m12v = __builtin_amdgcn_alignbit(p[m].w, p[m].z, 16);
+        const uint32_t wu = HI == 0 ? (e[m] ? m12u : p[m].x) : (e[m] ? p[m].y : m12u);
+        const uint32_t wv = HI == 0 ? (e[m] ? m12v : p[m].z) : (e[m] ? p[m].w : m12v);
+        float t1, b1;
+        S::rows(wu, wv, fu, t1, b1);
         acc[2 * m + 1] += fmaf(fb[m], b1 - t1, t1);
     }
 }
@@ -427,7 +493,7 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
     const float fj = (float)min(j, a.Y - 1);
     const float fk0 = (float)k0;
     const float Wm1 = (float)(a.W - 1), Vm1 = (float)(a.V - 1);
-    const uint32_t rowb = (uint32_t)a.W * (uint32_t)a.pxb;
+    const uint32_t rowb = (uint32_t)a.row_px * (uint32_t)a.pxb;
 
     float acc[ZC], cnt[ZC];
 #pragma unroll
@@ -470,9 +536,8 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
                     // reads rows 0-1 of its strip, sample k+1 rows d-(d+1),
                     // d = floor(sv_k+1) - floor(sv_k), which is hi or hi+1
                     bool paired = false;
-                    if constexpr (QT != JH_F32) {
-                        if (!a.strip) {
-                        } else if (dsv >= 0.0f && dsv <= 0.984375f) {
+                    if constexpr (is_strip<QT>()) {
+                        if (dsv >= 0.0f && dsv <= 0.984375f) {
                             bp_column_pairs<ZC, QT, 0>(acc, rs, rowb, ub, sv0, dsv, fu);
                             paired = true;
                         } else if (dsv >= 1.015625f && dsv <= 1.984375f) {
@@ -654,7 +719,7 @@ __device__ __forceinline__ void bp_tile_lds(const BPArgs& a, unsigned tile)
     const float cx0 = (float)min(tx * ex, a.X - 1), cx1 = (float)min(tx * ex + ex - 1, a.X - 1);
     const float cy0 = (float)min(tyi * ey, a.Y - 1), cy1 = (float)min(tyi * ey + ey - 1, a.Y - 1);
     const float Wm1 = (float)(a.W - 1), Vm1 = (float)(a.V - 1);
-    const uint32_t rowb = (uint32_t)a.W * (uint32_t)a.pxb;
+    const uint32_t rowb = (uint32_t)a.row_px * (uint32_t)a.pxb;
     const int nthr = 64 * blockDim.y;
     const int tid = threadIdx.y * 64 + threadIdx.x;
     const int cap = a.lds_px;
@@ -939,41 +1004,52 @@ static unsigned grid_for(size_t n, unsigned block)
 typedef void (*bp_kernel_t)(BPArgs);
 
 template <int MODE, int ZC, int OUT>
-static bp_kernel_t pick_q(int dtype)
+static bp_kernel_t pick_q(int dtype, int strip)
 {
     if (OUT == kOutRaw) return bp_kernel<MODE, ZC, JH_F32, kOutRaw>;
     switch (dtype) {
     case JH_F32: return bp_kernel<MODE, ZC, JH_F32, OUT>;
-    case JH_BF16: return bp_kernel<MODE, ZC, JH_BF16, OUT>;
-    default: return bp_kernel<MODE, ZC, JH_F16, OUT>;
+    case JH_BF16: return strip ? bp_kernel<MODE, ZC, kBF16Strip, OUT> : bp_kernel<MODE, ZC, JH_BF16, OUT>;
+    default: return strip ? bp_kernel<MODE, ZC, kF16Strip, OUT> : bp_kernel<MODE, ZC, JH_F16, OUT>;
     }
 }
 
 template <int MODE, int OUT>
-static bp_kernel_t pick_zc(int zc, int dtype)
+static bp_kernel_t pick_zc(int zc, int dtype, int strip)
 {
     switch (zc) {
-    case 8: return pick_q<MODE, 8, OUT>(dtype);
-    case 16: return pick_q<MODE, 16, OUT>(dtype);
-    default: return pick_q<MODE, 32, OUT>(dtype);
+    case 8: return pick_q<MODE, 8, OUT>(dtype, strip);
+    case 16: return pick_q<MODE, 16, OUT>(dtype, strip);
+    default: return pick_q<MODE, 32, OUT>(dtype, strip);
     }
 }
 
 template <int OUT>
-static bp_kernel_t pick_kernel(int mode, int zc, int dtype, int fetch)
+static bp_kernel_t pick_kernel(int mode, int zc, int dtype, int fetch, int strip)
 {
     switch (mode) {
-    case 0: return fetch == JH_FETCH_LDS ? pick_zc<kModeStaged, OUT>(zc, dtype) : pick_zc<0, OUT>(zc, dtype);
-    case 1: return pick_zc<1, OUT>(zc, dtype);
-    default: return pick_zc<2, OUT>(zc, dtype);
+    case 0:
+        return fetch == JH_FETCH_LDS ? pick_zc<kModeStaged, OUT>(zc, dtype, strip)
+                                     : pick_zc<0, OUT>(zc, dtype, strip);
+    case 1: return pick_zc<1, OUT>(zc, dtype, strip);
+    default: return pick_zc<2, OUT>(zc, dtype, strip);
     }
 }
 
-// Bytes per staged detector pixel: the FP32 quad (16), the BF16 / FP16 quad
-// (8) or the BF16 / FP16 strip (16).
+// Staged layout of one view: bytes per pixel (FP32 quad 16, BF16 / FP16 quad
+// or column strip 8) and pixels per row (column strips carry a zero padding
+// column).
 static int stage_pixel_bytes(const jh_bp_plan* p, int dtype)
 {
-    return dtype == JH_F32 ? 16 : (p->strip ? 16 : 8);
+    (void)p;
+    return dtype == JH_F32 ? 16 : 8;
+}
+
+static bool stage_strip(const jh_bp_plan* p, int dtype) { return dtype != JH_F32 && p->strip; }
+
+static size_t stage_frame_bytes(const jh_bp_plan* p, int dtype)
+{
+    return (size_t)(p->W + (stage_strip(p, dtype) ? 1 : 0)) * p->V * stage_pixel_bytes(p, dtype);
 }
 
 static void launch_bp(jh_bp_plan* p, const void* q, int dtype, int view0, int nviews, float* vol,
@@ -996,17 +1072,18 @@ static void launch_bp(jh_bp_plan* p, const void* q, int dtype, int view0, int nv
     a.ntx = (p->X + ex - 1) / ex;
     a.nty = (p->Y + ey - 1) / ey;
     a.ntz = (p->Z + p->zc - 1) / p->zc;
-    const size_t fb = (size_t)p->W * p->V * stage_pixel_bytes(p, dtype);
+    const size_t fb = stage_frame_bytes(p, dtype);
     a.pxb = stage_pixel_bytes(p, dtype);
-    a.strip = dtype != JH_F32 && p->strip;
+    a.row_px = p->W + (stage_strip(p, dtype) ? 1 : 0);
     require(fb < (1ull << 31), "backprojection frame too large");
     a.frame_bytes = (uint32_t)fb;
     const size_t ntiles = (size_t)a.ntx * a.nty * a.ntz;
     require(ntiles < (1ull << 31), "backprojection grid too large");
     const size_t grid = (ntiles + 7) / 8 * 8;
-    bp_kernel_t k = out == kOutNormalised   ? pick_kernel<kOutNormalised>(p->mode, p->zc, dtype, p->fetch)
-                    : out == kOutAccumulate ? pick_kernel<kOutAccumulate>(p->mode, p->zc, dtype, p->fetch)
-                                            : pick_kernel<kOutRaw>(p->mode, p->zc, dtype, p->fetch);
+    const int st = stage_strip(p, dtype) ? 1 : 0;
+    bp_kernel_t k = out == kOutNormalised   ? pick_kernel<kOutNormalised>(p->mode, p->zc, dtype, p->fetch, st)
+                    : out == kOutAccumulate ? pick_kernel<kOutAccumulate>(p->mode, p->zc, dtype, p->fetch, st)
+                                            : pick_kernel<kOutRaw>(p->mode, p->zc, dtype, p->fetch, st);
     // LDS-staged fetch: two buffers sized for the block footprints of this
     // tile depth (a larger footprint gathers), 64 zc pixels each, capped so
     // that both, with the kernel's static box ring and count-weight rows
@@ -1074,18 +1151,10 @@ jh_status jh_bp_create(jh_bp_plan** plan, int device, const double* h_ptm, int n
                 if (e.c[11] != 0.0f) zinv_w = false;
             }
             p->mode = (zinv_u && zinv_w) ? 0 : (zinv_w ? 1 : 2);
-            // BF16 / FP16 layout: strips pay off when the backprojector's pair
-            // path applies, i.e. when the row step per z voxel at the volume
-            // centre lies in [0, 0.984] or [1.016, 1.984] for most views
-            // (class 0 only); otherwise quads (a strip's stride would halve
-            // the pixels per cache line of the one-gather-per-sample path)
-            int pairable = 0;
-            const float ic = 0.5f * (float)(X - 1), jc = 0.5f * (float)(Y - 1);
-            for (const Geo& e : g) {
-                const float dsv = e.c[7] / (e.c[8] + e.c[9] * ic + e.c[10] * jc);
-                if ((dsv >= 0.0f && dsv <= 0.984375f) || (dsv >= 1.015625f && dsv <= 1.984375f)) ++pairable;
-            }
-            p->strip = p->mode == 0 && 2 * pairable >= nviews ? 1 : 0;
+            // BF16 / FP16 layout: column strips for geometry class 0 (the
+            // backprojector's pair path needs them; where the row step rules
+            // pairs out they measure as fast as quads), quads otherwise
+            p->strip = p->mode == 0 ? 1 : 0;
         } catch (...) {
             delete p;
             throw;
@@ -1108,7 +1177,7 @@ jh_status jh_bp_stage_bytes(jh_bp_plan* p, int dtype, int nviews, size_t* bytes)
     return guard([&] {
         require(p != nullptr && bytes != nullptr && nviews >= 0, "jh_bp_stage_bytes: bad arguments");
         require(dtype == JH_F32 || dtype == JH_BF16 || dtype == JH_F16, "jh_bp_stage_bytes: unknown dtype");
-        *bytes = (size_t)p->W * p->V * nviews * stage_pixel_bytes(p, dtype);
+        *bytes = stage_frame_bytes(p, dtype) * nviews;
     });
 }
 
@@ -1124,17 +1193,20 @@ static void stage(jh_bp_plan* p, const float* d_filtered, int view0, int nviews,
         require(p->V <= 65535, "jh_bp_stage: more than 65535 detector rows");
         hipStream_t s = as_stream(stream);
         const int zf = weighted ? p->zfilter : 0;
-        const size_t fin = (size_t)p->W * p->V, fout = fin * stage_pixel_bytes(p, dtype);
+        const size_t fin = (size_t)p->W * p->V, fout = stage_frame_bytes(p, dtype);
+        const int st = stage_strip(p, dtype) ? 1 : 0;
         for (int c0 = 0; c0 < nviews; c0 += 65535) {  // grid z: frames
             const int n = std::min(nviews - c0, 65535);
             const float* uw = weighted ? p->uw.as<float>() + (size_t)(view0 + c0) * p->W : nullptr;
             const float* in = d_filtered + (size_t)c0 * fin;
             void* out = static_cast<char*>(d_q) + (size_t)c0 * fout;
-            const dim3 grid((unsigned)((p->W + 255) / 256), (unsigned)p->V, (unsigned)n);
-            const int st = p->strip;
-            if (dtype == JH_F32) bp_stage_kernel<JH_F32><<<grid, 256, 0, s>>>(in, uw, p->W, p->V, zf, 0, out);
-            else if (dtype == JH_BF16) bp_stage_kernel<JH_BF16><<<grid, 256, 0, s>>>(in, uw, p->W, p->V, zf, st, out);
-            else bp_stage_kernel<JH_F16><<<grid, 256, 0, s>>>(in, uw, p->W, p->V, zf, st, out);
+            const dim3 grid((unsigned)((p->W + st + 255) / 256), (unsigned)p->V, (unsigned)n);
+            if (dtype == JH_F32) bp_stage_kernel<JH_F32, false><<<grid, 256, 0, s>>>(in, uw, p->W, p->V, zf, out);
+            else if (dtype == JH_BF16 && st)
+                bp_stage_kernel<JH_BF16, true><<<grid, 256, 0, s>>>(in, uw, p->W, p->V, zf, out);
+            else if (dtype == JH_BF16) bp_stage_kernel<JH_BF16, false><<<grid, 256, 0, s>>>(in, uw, p->W, p->V, zf, out);
+            else if (st) bp_stage_kernel<JH_F16, true><<<grid, 256, 0, s>>>(in, uw, p->W, p->V, zf, out);
+            else bp_stage_kernel<JH_F16, false><<<grid, 256, 0, s>>>(in, uw, p->W, p->V, zf, out);
         }
         check_launch("bp_stage_kernel");
     }

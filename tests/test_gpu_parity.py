@@ -539,10 +539,11 @@ def test_fdk_offset_detector_chain_vs_oracle(gpu, form, storage, tol):
 @pytest.mark.parametrize("dtype", ["bf16", "f16"])
 @pytest.mark.parametrize("tile", [(8, 4), (16, 2), (32, 1)])
 def test_bp_strip_layout_equals_quad_layout_bit_exact(gpu, dtype, tile):
-    """The strip layout's paired gathers read the same four values per sample
-    and interpolate them with the same operations as the quad layout's one
-    gather per sample: the volumes agree to the bit (one-shot and streamed),
-    including the columns that leave the detector rows (partial path)."""
+    """The column-strip layout's paired gathers read the same four values per
+    sample and interpolate them with the same operations as the quad layout's
+    one gather per sample: the volumes agree to the bit (one-shot and
+    streamed), including the columns that leave the detector rows (partial
+    path) and the LDS-staged fetch."""
     from jenhine_amd.recon import BPCore
     w = SMALL
     filt = torch.from_numpy(synth.normal(31, (w.n_views, w.n_v, w.n_u))).to(gpu)
@@ -553,27 +554,36 @@ def test_bp_strip_layout_equals_quad_layout_bit_exact(gpu, dtype, tile):
     for layout in ("quad", "strip"):
         bp.set_layout(layout)
         q = bp.stage(filt, dtype)
-        assert q.numel() == w.n_views * w.n_v * w.n_u * (16 if layout == "strip" else 8)
+        # 8 B per pixel; strips pad each row with one zero pixel
+        assert q.numel() == w.n_views * w.n_v * (w.n_u + (layout == "strip")) * 8
         vol = bp.run(q, dtype).cpu().numpy()
         bp.reset()
         h = q.numel() // 2
         bp.accumulate(q[:h], dtype, 0, w.n_views // 2)
         bp.accumulate(q[h:], dtype, w.n_views // 2, w.n_views - w.n_views // 2)
         out[layout] = (vol, bp.finalize().cpu().numpy())
+        if layout == "strip":
+            bp.set_fetch("lds")
+            out["lds"] = (bp.run(q, dtype).cpu().numpy(),)
+            bp.set_fetch("gather")
     for k in range(2):
         np.testing.assert_array_equal(out["strip"][k], out["quad"][k])
+    # LDS staging reads the same neighbourhoods from strips (its own bilinear
+    # association, as against quads: test_bp_lds_fetch_matches_gather)
+    assert rel_rmse(out["lds"][0], out["quad"][0]) <= 1e-6
 
 
-def test_bp_layout_choice_follows_the_row_step(gpu):
-    """Quads for a fine detector (row step above 2 pixels per z voxel, where a
-    strip would only widen the stride of single gathers), strips when the
-    row step allows pairs; both reconstruct the same volume."""
+def test_bp_layouts_agree_on_a_fine_detector(gpu):
+    """A fine detector (3.5 rows per z voxel) rules the paired gathers out:
+    every sample takes the one-gather path, from quads or from column strips
+    (plan default for geometry class 0), with bit-identical volumes."""
     from jenhine_amd.recon import BPCore
     w = SMALL.scaled(du=0.12, n_u=200, n_v=200, name="fine-detector")
     filt = torch.from_numpy(synth.normal(32, (w.n_views, w.n_v, w.n_u))).to(gpu)
     bp = BPCore(w.ptms(), w.n_u, w.n_v, w.X, w.Y, w.Z, w.pxy, w.pz, device=gpu)
-    assert bp.layout() == "quad"  # 0.25 * 500/300 / 0.12 = 3.5 rows per z voxel
-    quad = bp.run(bp.stage(filt, "bf16"), "bf16").cpu().numpy()
-    bp.set_layout("strip")
+    assert bp.geometry_mode() == 0 and bp.layout() == "strip"
     strip = bp.run(bp.stage(filt, "bf16"), "bf16").cpu().numpy()
+    bp.set_layout("quad")
+    quad = bp.run(bp.stage(filt, "bf16"), "bf16").cpu().numpy()
+    assert np.isfinite(quad).sum() > 1000
     np.testing.assert_array_equal(strip, quad)
