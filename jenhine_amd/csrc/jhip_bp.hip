@@ -114,6 +114,7 @@ __device__ inline uint32_t f32_to_bf16_rne(float f)
 }
 
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
 typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
 typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
 
@@ -257,24 +258,46 @@ __global__ void bp_uwpair_kernel(const float* __restrict__ uw, int N, int W, flo
 
 constexpr int kBufferFlags = 0x00020000;  // raw buffer, 32-bit dwords (gfx9 family)
 
-// One quad load per sample; row r (0 = top, 1 = bottom) interpolated at
-// fractional offset fu.
+// One quad load per sample; row r (0 = top, 1 = bottom) interpolated along u
+// with the weight w of the sample's column (weight()).
+//
+// The BF16 and FP16 samplers interpolate a row with one v_dot2_f32_bf16 /
+// v_dot2_f32_f16 of its packed pair (q[u], q[u+1]) and the packed weights
+// (1 - w, w), accumulated in FP32.  w is the fraction fu rounded to a
+// multiple of 1/256 (BF16) or 1/2048 (FP16): both weights are then exact in
+// the 16-bit type and sum to one, and the position along u is off by at most
+// 1/512 (1/4096) pixel -- the reference's hardware bilinear texture fetch
+// (cudaFilterModeLinear) quantises both fractions to 1/256.  The FP32 sampler
+// uses fu itself.  weight() also returns the fraction it applies, for the
+// count weight, so that numerator and count interpolate alike.
 template <int QT>
 struct Sampler;
 
 template <>
 struct Sampler<JH_F32> {
     typedef u32x4_t raw_t;
+    typedef float wt_t;
     static constexpr int bytes = 16;   // bytes per sample load
     __device__ static raw_t load(__amdgpu_buffer_rsrc_t r, uint32_t off)
     {
         return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
     }
+    __device__ static wt_t weight(float fu, float& fe)
+    {
+        fe = fu;
+        return fu;
+    }
     // (q, d): q + fu d
-    __device__ static float lerp(raw_t p, int r, float fu)
+    __device__ static float lerp(raw_t p, int r, wt_t fu)
     {
         return r == 0 ? fmaf(fu, __uint_as_float(p.y), __uint_as_float(p.x))
                       : fmaf(fu, __uint_as_float(p.w), __uint_as_float(p.z));
+    }
+    // acc + the sample at row fraction fv
+    __device__ static float accum(float acc, raw_t p, float fv, wt_t fu)
+    {
+        const float top = lerp(p, 0, fu), bot = lerp(p, 1, fu);
+        return acc + fmaf(fv, bot - top, top);
     }
     // bilinear coefficients (q00, q01 - q00, q10 - q00, (q11 - q10) - (q01 - q00))
     __device__ static float4 abcd(raw_t p)
@@ -284,110 +307,159 @@ struct Sampler<JH_F32> {
     }
 };
 
+// Packed-pair arithmetic of the 16-bit storage types.
+// Row interpolation of the 16-bit storage types, one v_dot2 per row pair
+// (q[u], q[u+1]), accumulated in FP32.  Half16<QT>::Wt is the weight of a
+// sample's column, formed once per view; accum() adds one sample interpolated
+// from its top and bottom row pairs.
+template <int QT>
+struct Half16;
+
+// BF16: the row is interpolated from its nearer column,
+//   q[u] + g (q[u+1] - q[u])   (fu <= 1/2, g = fu), or
+//   q[u+1] + g (q[u] - q[u+1]) (fu > 1/2, g = 1 - fu),
+// as one dot of the pair with (-g, g) (or (g, -g)) accumulated onto the nearer
+// value, which is exact in FP32.  g is rounded to BF16 (relative error
+// 2^-9, g <= 1/2): the position along u is off by at most 2^-10 pixel.
 template <>
-struct Sampler<JH_BF16> {
-    typedef u32x2_t raw_t;
-    static constexpr int bytes = 8;
-    __device__ static raw_t load(__amdgpu_buffer_rsrc_t r, uint32_t off)
+struct Half16<JH_BF16> {
+    struct Wt {
+        uint32_t w;    // (weight of q[u], weight of q[u+1]), BF16 pair
+        uint32_t sel;  // v_perm selector moving the nearer value's BF16 bits into an FP32 word
+    };
+    // x.lo w.lo + x.hi w.hi + c
+    __device__ static float dot(uint32_t x, uint32_t w, float c)
     {
-        return __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+        return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2_t, x), __builtin_bit_cast(bf16x2_t, w), c,
+                                               false);
     }
-    // one row's pair (q[u], q[u+1]) interpolated at fu
-    __device__ static float row(uint32_t x, float fu)
+    __device__ static Wt weight(float fu, float& fe)
     {
-        const float a = __uint_as_float(x << 16), b = __uint_as_float(x & 0xffff0000u);
-        return fmaf(fu, b - a, a);
+        const bool far = fu > 0.5f;
+        const float g = far ? 1.0f - fu : fu;  // exact
+        const uint32_t gb = f32_to_bf16_rne(g);
+        const float ge = __uint_as_float(gb << 16);
+        fe = far ? 1.0f - ge : ge;
+        const uint32_t neg = gb | 0x8000u;
+        Wt r;
+        r.w = far ? (gb | (neg << 16)) : (neg | (gb << 16));
+        r.sel = far ? 0x03020c0cu : 0x01000c0cu;
+        return r;
     }
-    __device__ static float lerp(raw_t p, int r, float fu) { return row(r == 0 ? p.x : p.y, fu); }
-    __device__ static float4 abcd(raw_t p)
+    __device__ static float row(uint32_t x, Wt w)
     {
-        const float q00 = __uint_as_float(p.x << 16), q01 = __uint_as_float(p.x & 0xffff0000u);
-        const float q10 = __uint_as_float(p.y << 16), q11 = __uint_as_float(p.y & 0xffff0000u);
-        const float b = q01 - q00;
-        return make_float4(q00, b, q10 - q00, (q11 - q10) - b);
+        return dot(x, w.w, __uint_as_float(__builtin_amdgcn_perm(x, x, w.sel)));
     }
+    // acc + (top + fv (bot - top)) for the row pairs rt (top) and rb (bottom)
+    __device__ static float accum(float acc, uint32_t rt, uint32_t rb, Wt w, float fv)
+    {
+        const float top = row(rt, w), bot = row(rb, w);
+        return acc + fmaf(fv, bot - top, top);
+    }
+    __device__ static float lo(uint32_t w) { return __uint_as_float(w << 16); }
+    __device__ static float hi(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
 };
 
+// FP16: the pair's weights are (1 - w, w), w the fraction rounded to a
+// multiple of 1/2048, so both are exact in FP16 and sum to one; the position
+// along u is off by at most 1/4096 pixel.  bot - top is one dot of the bottom
+// pair with (1 - w, w) accumulated into one of the top pair with -(1 - w, w),
+// and top is accumulated into acc directly (the accumulator is rounded twice
+// per sample).
 template <>
-struct Sampler<JH_F16> {
+struct Half16<JH_F16> {
+    struct Wt {
+        uint32_t w, nw;  // (1 - w, w) and its negation, FP16 pairs
+    };
+    __device__ static float dot(uint32_t x, uint32_t w, float c)
+    {
+        return __builtin_amdgcn_fdot2(__builtin_bit_cast(half2_t, x), __builtin_bit_cast(half2_t, w), c, false);
+    }
+    __device__ static Wt weight(float fu, float& fe)
+    {
+        const float w = __builtin_rintf(fu * 2048.0f) * (1.0f / 2048.0f);
+        fe = w;
+        const uint32_t p = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(1.0f - w, w));
+        return Wt{p, p ^ 0x80008000u};
+    }
+    __device__ static float accum(float acc, uint32_t rt, uint32_t rb, Wt w, float fv)
+    {
+        const float d = dot(rt, w.nw, dot(rb, w.w, 0.0f));
+        return fmaf(fv, d, dot(rt, w.w, acc));
+    }
+    __device__ static float lo(uint32_t w) { return (float)__builtin_bit_cast(half2_t, w).x; }
+    __device__ static float hi(uint32_t w) { return (float)__builtin_bit_cast(half2_t, w).y; }
+};
+
+// BF16 / FP16 quads: staged pixel (u, v) holds the row pairs
+// (q[v][u], q[v][u+1]) and (q[v+1][u], q[v+1][u+1]), low half first.
+template <int QT>
+struct QuadSampler {
+    typedef Half16<QT> H;
     typedef u32x2_t raw_t;
+    typedef typename H::Wt wt_t;
     static constexpr int bytes = 8;
     __device__ static raw_t load(__amdgpu_buffer_rsrc_t r, uint32_t off)
     {
         return __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
     }
-    __device__ static float row(uint32_t x, float fu)
+    __device__ static wt_t weight(float fu, float& fe) { return H::weight(fu, fe); }
+    __device__ static float accum(float acc, raw_t p, float fv, wt_t w)
     {
-        // b fu + (a - a fu): two v_fma_mix, which convert the f16 halves inside
-        const half2_t h = __builtin_bit_cast(half2_t, x);
-        const float a = (float)h.x;
-        return fmaf((float)h.y, fu, fmaf(-a, fu, a));
-    }
-    __device__ static float lerp(raw_t p, int r, float fu)
-    {
-        const uint32_t x = r == 0 ? p.x : p.y;
-        return row(x, fu);
+        // the words are copied out of the vector first: a bit cast taken
+        // directly on the element p.y was compiled as one of p.x
+        const uint32_t px = p.x, py = p.y;
+        return H::accum(acc, px, py, w, fv);
     }
     __device__ static float4 abcd(raw_t p)
     {
-        // the halves are copied out of the vector first: a bit cast taken
-        // directly on the element p.y was compiled as one of p.x
         const uint32_t px = p.x, py = p.y;
-        const half2_t t = __builtin_bit_cast(half2_t, px), u = __builtin_bit_cast(half2_t, py);
-        const float q00 = (float)t.x, q01 = (float)t.y, q10 = (float)u.x, q11 = (float)u.y;
+        const float q00 = H::lo(px), q01 = H::hi(px), q10 = H::lo(py), q11 = H::hi(py);
         const float b = q01 - q00;
         return make_float4(q00, b, q10 - q00, (q11 - q10) - b);
     }
 };
+template <>
+struct Sampler<JH_BF16> : QuadSampler<JH_BF16> {};
+template <>
+struct Sampler<JH_F16> : QuadSampler<JH_F16> {};
 
 // Column strips (BF16 / FP16): staged pixel (u, v) holds the four rows
 // v .. v+3 of column u as two 32-bit words, w0 = (q[v][u], q[v+1][u]) and
 // w1 = (q[v+2][u], q[v+3][u]) (low half first), 8 bytes; rows are padded to
 // W + 1 pixels (the padding column is 0).  One 16-byte gather at (u, v)
 // returns p = (w0(u), w1(u), w0(u+1), w1(u+1)): columns u and u+1 of four
-// rows.  A word pair (wu, wu1) holding rows (t, t+1) of columns u and u+1
-// gives the sample's two row values; row t+1..t+2 of the strip is the word
-// alignbit(w1, w0, 16).
+// rows.  Row r's pair (q[v+r][u], q[v+r][u+1]) is one v_perm_b32 of a word
+// of each column; it is the word the quad layout stores for that row, so the
+// two layouts interpolate identical operands.
 constexpr int kBF16Strip = 4;  // sampler ids of the strip layout (kernel template argument QT)
 constexpr int kF16Strip = 5;
 
 template <int QT>
 struct StripSampler {
+    typedef Half16<QT == kBF16Strip ? JH_BF16 : JH_F16> H;
     typedef u32x4_t raw_t;
+    typedef typename H::Wt wt_t;
     static constexpr int bytes = 16;
     __device__ static raw_t load(__amdgpu_buffer_rsrc_t r, uint32_t off)
     {
         return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
     }
-    __device__ static float lo(uint32_t w)
+    __device__ static wt_t weight(float fu, float& fe) { return H::weight(fu, fe); }
+    // row r (0..3) of the window as the pair (column u, column u+1)
+    template <int R>
+    __device__ static uint32_t rowpair(raw_t p)
     {
-        if constexpr (QT == kBF16Strip) return __uint_as_float(w << 16);
-        else return (float)__builtin_bit_cast(half2_t, w).x;
+        constexpr uint32_t sel = (R & 1) ? 0x07060302u : 0x05040100u;
+        return R < 2 ? __builtin_amdgcn_perm(p.z, p.x, sel) : __builtin_amdgcn_perm(p.w, p.y, sel);
     }
-    __device__ static float hi(uint32_t w)
+    __device__ static float accum(float acc, raw_t p, float fv, wt_t w)
     {
-        if constexpr (QT == kBF16Strip) return __uint_as_float(w & 0xffff0000u);
-        else return (float)__builtin_bit_cast(half2_t, w).y;
-    }
-    // a + fu (b - a) in the operation order of the quad samplers
-    __device__ static float mix(float a, float b, float fu)
-    {
-        if constexpr (QT == kBF16Strip) return fmaf(fu, b - a, a);
-        else return fmaf(b, fu, fmaf(-a, fu, a));
-    }
-    // rows t (low halves) and t+1 (high halves) of columns u, u+1
-    __device__ static void rows(uint32_t wu, uint32_t wu1, float fu, float& top, float& bot)
-    {
-        top = mix(lo(wu), lo(wu1), fu);
-        bot = mix(hi(wu), hi(wu1), fu);
-    }
-    __device__ static float lerp(raw_t p, int r, float fu)
-    {
-        return r == 0 ? mix(lo(p.x), lo(p.z), fu) : mix(hi(p.x), hi(p.z), fu);
+        return H::accum(acc, rowpair<0>(p), rowpair<1>(p), w, fv);
     }
     __device__ static float4 abcd(raw_t p)
     {
-        const float q00 = lo(p.x), q01 = lo(p.z), q10 = hi(p.x), q11 = hi(p.z);
+        const float q00 = H::lo(p.x), q01 = H::lo(p.z), q10 = H::hi(p.x), q11 = H::hi(p.z);
         const float b = q01 - q00;
         return make_float4(q00, b, q10 - q00, (q11 - q10) - b);
     }
@@ -429,11 +501,11 @@ constexpr int kOutRaw = 2;         // vol = acc (the un-normalised operator, FP3
 // The samples of one all-in-range column in pairs over column strips: one
 // 16-byte gather at row floor(sv_k) serves samples k and k+1; the second
 // sample's rows start HI or HI + 1 rows down.  Each sample interpolates the
-// same four values with the same operations as the quad layout's one gather
-// per sample, so the results are identical.
+// same row pairs with the same operations as the quad layout's one gather per
+// sample, so the results are identical.
 template <int ZC, int QT, int HI>
 __device__ __forceinline__ void bp_column_pairs(float* acc, __amdgpu_buffer_rsrc_t rs, uint32_t rowb, uint32_t ub,
-                                                float sv0, float dsv, float fu)
+                                                float sv0, float dsv, typename Sampler<QT>::wt_t wq)
 {
     typedef Sampler<QT> S;
     constexpr int NP = ZC / 2;
@@ -455,17 +527,20 @@ __device__ __forceinline__ void bp_column_pairs(float* acc, __amdgpu_buffer_rsrc
     for (int m = 0; m < NP; ++m) p[m] = S::load(rs, off[m]);
 #pragma unroll
     for (int m = 0; m < NP; ++m) {
-        float t0, b0;
-        S::rows(p[m].x, p[m].z, fu, t0, b0);
-        acc[2 * m] += fmaf(fa[m], b0 - t0, t0);
-        // rows 1-2 of the strip: the high half of w0 and the low half of w1
-        const uint32_t m12u = __builtin_amdgcn_alignbit(p[m].y, p[m].x, 16);
-        const uint32_t m12v = __builtin_amdgcn_alignbit(p[m].w, p[m].z, 16);
-        const uint32_t wu = HI == 0 ? (e[m] ? m12u : p[m].x) : (e[m] ? p[m].y : m12u);
-        const uint32_t wv = HI == 0 ? (e[m] ? m12v : p[m].z) : (e[m] ? p[m].w : m12v);
-        float t1, b1;
-        S::rows(wu, wv, fu, t1, b1);
-        acc[2 * m + 1] += fmaf(fb[m], b1 - t1, t1);
+        const uint32_t r0 = S::template rowpair<0>(p[m]);
+        const uint32_t r1 = S::template rowpair<1>(p[m]);
+        const uint32_t r2 = S::template rowpair<2>(p[m]);
+        acc[2 * m] = S::H::accum(acc[2 * m], r0, r1, wq, fa[m]);
+        uint32_t rt, rb;
+        if constexpr (HI == 0) {
+            rt = e[m] ? r1 : r0;
+            rb = e[m] ? r2 : r1;
+        } else {
+            const uint32_t r3 = S::template rowpair<3>(p[m]);
+            rt = e[m] ? r2 : r1;
+            rb = e[m] ? r3 : r2;
+        }
+        acc[2 * m + 1] = S::H::accum(acc[2 * m + 1], rt, rb, wq, fb[m]);
     }
 }
 
@@ -499,6 +574,16 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
 #pragma unroll
     for (int kk = 0; kk < ZC; ++kk) { acc[kk] = 0.0f; cnt[kk] = 0.0f; }
     float cfull = 0.0f;  // count weight of the views whose samples all count
+    // MODE 0: the per-sample count weights of the views whose column is only
+    // partly inside the detector rows are kept in LDS, one slot per thread and
+    // sample (touched by that thread only, so no barrier): such views are rare,
+    // and ZC registers held across the view loop would cost occupancy.
+    __shared__ float cnt_part[MODE == 0 ? ZC * 256 : 1];
+    const int tid = threadIdx.y * 64 + threadIdx.x;
+    if constexpr (MODE == 0) {
+#pragma unroll
+        for (int kk = 0; kk < ZC; ++kk) cnt_part[kk * 256 + tid] = 0.0f;
+    }
 
     for (int v = 0; v < a.nviews; ++v) {
         const Geo g = a.geo[a.view0 + v];
@@ -510,17 +595,21 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
         const float Vc = fmaf(g.c[7], fk0, fmaf(g.c[5], fi, fmaf(g.c[6], fj, g.c[4])));
         const float Wc = fmaf(g.c[11], fk0, fmaf(g.c[9], fi, fmaf(g.c[10], fj, g.c[8])));
         if (MODE == 0) {
-            // u and w are constant along the column: hoist the u side
-            const float rw = 1.0f / Wc;
+            // u and w are constant along the column: hoist the u side.
+            // 1/w by v_rcp_f32 (1 ulp; the forward projector takes the same
+            // reciprocal, so the pair stays adjoint)
+            const float rw = __builtin_amdgcn_rcpf(Wc);
             const float su = fmaf(Uc, rw, 0.5f);
             if (su >= 0.0f && su <= Wm1) {
                 const int iu = (int)su;
-                const float fu = __builtin_amdgcn_fractf(su);
+                float fe;
+                const typename S::wt_t wq = S::weight(__builtin_amdgcn_fractf(su), fe);
+                // the count-weight pair is loaded ahead of the gathers and
+                // used after them (a wait for it alone would stall the view)
                 const float2 uwq = uwv[iu];
-                const float cw = fmaf(fu, uwq.y - uwq.x, uwq.x);
                 const float sv0 = fmaf(Vc, rw, 0.5f);
                 const float dsv = g.c[7] * rw;
-                const uint32_t ub = (uint32_t)iu * (uint32_t)a.pxb;
+                const uint32_t ub = __umul24((uint32_t)iu, (uint32_t)a.pxb);
                 // sv is monotonic in k, so the column is inside the detector
                 // rows iff its first and last samples are.
                 const float svl = fmaf((float)(ZC - 1), dsv, sv0);
@@ -529,7 +618,6 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
                     // All gathers of the column are issued before any is
                     // consumed: the kernel is bound by gather latency, and
                     // this keeps 2 ZC of them in flight per wave.
-                    cfull += cw;
                     // BF16 / FP16 strips: one b128 gather per pair of samples
                     // (k, k+1) when the row step lies clear of an integer
                     // (margin 1/64 against the rounding of sv): sample k
@@ -538,10 +626,10 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
                     bool paired = false;
                     if constexpr (is_strip<QT>()) {
                         if (dsv >= 0.0f && dsv <= 0.984375f) {
-                            bp_column_pairs<ZC, QT, 0>(acc, rs, rowb, ub, sv0, dsv, fu);
+                            bp_column_pairs<ZC, QT, 0>(acc, rs, rowb, ub, sv0, dsv, wq);
                             paired = true;
                         } else if (dsv >= 1.015625f && dsv <= 1.984375f) {
-                            bp_column_pairs<ZC, QT, 1>(acc, rs, rowb, ub, sv0, dsv, fu);
+                            bp_column_pairs<ZC, QT, 1>(acc, rs, rowb, ub, sv0, dsv, wq);
                             paired = true;
                         }
                     }
@@ -558,13 +646,11 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
 #pragma unroll
                         for (int k = 0; k < ZC; ++k) p[k] = S::load(rs, off[k]);
 #pragma unroll
-                        for (int k = 0; k < ZC; ++k) {
-                            const float top = S::lerp(p[k], 0, fu);
-                            const float bot = S::lerp(p[k], 1, fu);
-                            acc[k] += fmaf(fv[k], bot - top, top);
-                        }
+                        for (int k = 0; k < ZC; ++k) acc[k] = S::accum(acc[k], p[k], fv[k], wq);
                     }
+                    cfull += fmaf(fe, uwq.y - uwq.x, uwq.x);
                 } else {
+                    const float cw = fmaf(fe, uwq.y - uwq.x, uwq.x);
 #pragma unroll
                     for (int kb = 0; kb < ZC; kb += kZB) {
                         uint32_t off[kZB];
@@ -583,11 +669,9 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
                         for (int q = 0; q < kZB; ++q) p[q] = S::load(rs, off[q]);
 #pragma unroll
                         for (int q = 0; q < kZB; ++q) {
-                            const float top = S::lerp(p[q], 0, fu);
-                            const float bot = S::lerp(p[q], 1, fu);
-                            const float val = fmaf(fv[q], bot - top, top);
-                            acc[kb + q] += in[q] ? val : 0.0f;
-                            cnt[kb + q] += in[q] ? cw : 0.0f;
+                            const float sum = S::accum(acc[kb + q], p[q], fv[q], wq);
+                            acc[kb + q] = in[q] ? sum : acc[kb + q];
+                            cnt_part[(kb + q) * 256 + tid] += in[q] ? cw : 0.0f;
                         }
                     }
                 }
@@ -595,13 +679,14 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
         } else {
             // MODE 1: w constant along the column, u and v linear in z;
             // MODE 2: general, one reciprocal per sample.
-            const float rw = 1.0f / Wc;
+            const float rw = __builtin_amdgcn_rcpf(Wc);
             const float su0 = fmaf(Uc, rw, 0.5f), dsu = g.c[3] * rw;
             const float sv0 = fmaf(Vc, rw, 0.5f), dsv = g.c[7] * rw;
 #pragma unroll
             for (int kb = 0; kb < ZC; kb += kZB) {
                 uint32_t off[kZB];
-                float fu[kZB], fv[kZB];
+                float fe[kZB], fv[kZB];
+                typename S::wt_t wq[kZB];
                 bool in[kZB];
                 float2 uwq[kZB];
 #pragma unroll
@@ -620,9 +705,9 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
                     const float t = __builtin_amdgcn_fmed3f(sv, 0.0f, Vm1);
                     in[q] = (t == sv) && (tu == su);
                     const int iu = (int)tu;
-                    fu[q] = __builtin_amdgcn_fractf(tu);
+                    wq[q] = S::weight(__builtin_amdgcn_fractf(tu), fe[q]);
                     fv[q] = __builtin_amdgcn_fractf(t);
-                    off[q] = __umul24((uint32_t)(int)t, rowb) + (uint32_t)iu * (uint32_t)a.pxb;
+                    off[q] = __umul24((uint32_t)(int)t, rowb) + __umul24((uint32_t)iu, (uint32_t)a.pxb);
                     uwq[q] = uwv[iu];
                 }
                 typename S::raw_t p[kZB];
@@ -630,11 +715,9 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
                 for (int q = 0; q < kZB; ++q) p[q] = S::load(rs, off[q]);
 #pragma unroll
                 for (int q = 0; q < kZB; ++q) {
-                    const float top = S::lerp(p[q], 0, fu[q]);
-                    const float bot = S::lerp(p[q], 1, fu[q]);
-                    const float val = fmaf(fv[q], bot - top, top);
-                    const float cw = fmaf(fu[q], uwq[q].y - uwq[q].x, uwq[q].x);
-                    acc[kb + q] += in[q] ? val : 0.0f;
+                    const float sum = S::accum(acc[kb + q], p[q], fv[q], wq[q]);
+                    const float cw = fmaf(fe[q], uwq[q].y - uwq[q].x, uwq[q].x);
+                    acc[kb + q] = in[q] ? sum : acc[kb + q];
                     cnt[kb + q] += in[q] ? cw : 0.0f;
                 }
             }
@@ -643,7 +726,7 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
 
     if (i >= a.X || j >= a.Y) return;
 #pragma unroll
-    for (int kk = 0; kk < ZC; ++kk) cnt[kk] += cfull;
+    for (int kk = 0; kk < ZC; ++kk) cnt[kk] = (MODE == 0 ? cnt_part[kk * 256 + tid] : cnt[kk]) + cfull;
     const size_t plane = (size_t)a.X * a.Y;
     size_t idx = ((size_t)k0 * a.Y + j) * a.X + i;
 #pragma unroll
@@ -687,6 +770,17 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
 // ---------------------------------------------------------------------------
 
 constexpr int kLdsPre = 8;    // footprint pixels per thread held in registers across a view
+
+// acc + the sample with bilinear coefficients c = (a, b, c, d) at (fu, fv),
+// rounded at the accumulator in the same places as the gather form's
+// Sampler<QT>::accum: once, or twice (top, then fv (bot - top)) for FP16.
+template <int QT>
+__device__ __forceinline__ float lds_accum(float acc, float4 c, float fu, float fv)
+{
+    const float top = fmaf(fu, c.y, c.x), d = fmaf(fu, c.w, c.z);
+    if constexpr (QT == JH_F16 || QT == kF16Strip) return fmaf(fv, d, acc + top);
+    else return acc + fmaf(fv, d, top);
+}
 constexpr int kLdsUw = 256;   // widest footprint whose count-weight pairs are staged too
 
 // Workgroup-shared LDS: two coefficient buffers of a.lds_px pixels each
@@ -842,11 +936,13 @@ __device__ __forceinline__ void bp_tile_lds(const BPArgs& a, unsigned tile)
         const float Uc = fmaf(g.c[3], fk0, fmaf(g.c[1], fi, fmaf(g.c[2], fj, g.c[0])));
         const float Vc = fmaf(g.c[7], fk0, fmaf(g.c[5], fi, fmaf(g.c[6], fj, g.c[4])));
         const float Wc = fmaf(g.c[11], fk0, fmaf(g.c[9], fi, fmaf(g.c[10], fj, g.c[8])));
-        const float rw = 1.0f / Wc;
+        const float rw = __builtin_amdgcn_rcpf(Wc);
         const float su = fmaf(Uc, rw, 0.5f);
         if (su >= 0.0f && su <= Wm1) {
             const int iu = (int)su;
-            const float fu = __builtin_amdgcn_fractf(su);
+            // fu: the u fraction the gather form applies for this storage type
+            float fu;
+            const typename S::wt_t wq = S::weight(__builtin_amdgcn_fractf(su), fu);
             const float sv0 = fmaf(Vc, rw, 0.5f);
             const float dsv = g.c[7] * rw;
             const float svl = fmaf((float)(ZC - 1), dsv, sv0);
@@ -878,8 +974,7 @@ __device__ __forceinline__ void bp_tile_lds(const BPArgs& a, unsigned tile)
                         // their uses (it pairs them under register pressure)
                         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                        for (int q = 0; q < kZB; ++q)
-                            acc[kb + q] += fmaf(fv[q], fmaf(fu, c4[q].w, c4[q].z), fmaf(fu, c4[q].y, c4[q].x));
+                        for (int q = 0; q < kZB; ++q) acc[kb + q] = lds_accum<QT>(acc[kb + q], c4[q], fu, fv[q]);
                     }
                 } else {
 #pragma unroll
@@ -890,8 +985,8 @@ __device__ __forceinline__ void bp_tile_lds(const BPArgs& a, unsigned tile)
                         const float fv = __builtin_amdgcn_fractf(t);
                         const float4 c4 =
                             *reinterpret_cast<const float4*>(lbase + __umul24((uint32_t)(int)t, lrow) + lb);
-                        const float val = fmaf(fv, fmaf(fu, c4.w, c4.z), fmaf(fu, c4.y, c4.x));
-                        acc[k] += in ? val : 0.0f;
+                        const float sum = lds_accum<QT>(acc[k], c4, fu, fv);
+                        acc[k] = in ? sum : acc[k];
                         cnt[k] += in ? cw : 0.0f;
                     }
                 }
@@ -900,7 +995,7 @@ __device__ __forceinline__ void bp_tile_lds(const BPArgs& a, unsigned tile)
                 const float2 uwq = a.uwp[(size_t)(a.view0 + v) * a.W + iu];
                 const float cw = fmaf(fu, uwq.y - uwq.x, uwq.x);
                 const auto rs = rsrc(v);
-                const uint32_t ub = (uint32_t)iu * (uint32_t)a.pxb;
+                const uint32_t ub = __umul24((uint32_t)iu, (uint32_t)a.pxb);
 #pragma unroll
                 for (int kb = 0; kb < ZC; kb += kZB) {
                     uint32_t off[kZB];
@@ -919,10 +1014,8 @@ __device__ __forceinline__ void bp_tile_lds(const BPArgs& a, unsigned tile)
                     for (int q = 0; q < kZB; ++q) p[q] = S::load(rs, off[q]);
 #pragma unroll
                     for (int q = 0; q < kZB; ++q) {
-                        const float top = S::lerp(p[q], 0, fu);
-                        const float bot = S::lerp(p[q], 1, fu);
-                        const float val = fmaf(fv[q], bot - top, top);
-                        acc[kb + q] += in[q] ? val : 0.0f;
+                        const float sum = S::accum(acc[kb + q], p[q], fv[q], wq);
+                        acc[kb + q] = in[q] ? sum : acc[kb + q];
                         cnt[kb + q] += in[q] ? cw : 0.0f;
                     }
                 }
@@ -964,8 +1057,12 @@ __device__ __forceinline__ void bp_tile_lds(const BPArgs& a, unsigned tile)
 // (A persistent grid that sweeps the tiles in rounds was measured slower in
 // every tile shape on dental512 and made the compiler emit ~45 % more vector
 // instructions, so each workgroup runs exactly one tile.)
+// ZC = 16 in geometry class 0 is held to five waves per SIMD (at most 96
+// VGPRs; the BF16 strip kernel otherwise takes 97 and runs four): measured
+// 2 % faster on dental512, with no spills.  Six waves (80 VGPRs) was slower.
 template <int MODE, int ZC, int QT, int OUT>
-__global__ void __launch_bounds__(256) bp_kernel(BPArgs a)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ZC == 16 && MODE == 0 ? 5 : 1)))
+bp_kernel(BPArgs a)
 {
     const unsigned ntiles = (unsigned)a.ntx * a.nty * a.ntz;
     const unsigned t8 = gridDim.x >> 3;  // gridDim.x = 8 * ceil(ntiles / 8)

@@ -54,7 +54,7 @@ __device__ inline float bp_su(const Geo& g, float fi, float fj, float& rw)
 {
     const float Uc = fmaf(g.c[3], 0.0f, fmaf(g.c[1], fi, fmaf(g.c[2], fj, g.c[0])));
     const float Wc = fmaf(g.c[11], 0.0f, fmaf(g.c[9], fi, fmaf(g.c[10], fj, g.c[8])));
-    rw = 1.0f / Wc;
+    rw = __builtin_amdgcn_rcpf(Wc);
     return fmaf(Uc, rw, 0.5f);
 }
 
@@ -156,7 +156,7 @@ __global__ void __launch_bounds__(256) fp_scatter_kernel(FPArgs a, int nviews)
         const float Uc = fmaf(g.c[3], fk0, fmaf(g.c[1], fi, fmaf(g.c[2], fj, g.c[0])));
         const float Vc = fmaf(g.c[7], fk0, fmaf(g.c[5], fi, fmaf(g.c[6], fj, g.c[4])));
         const float Wc = fmaf(g.c[11], fk0, fmaf(g.c[9], fi, fmaf(g.c[10], fj, g.c[8])));
-        const float rw = 1.0f / Wc;
+        const float rw = __builtin_amdgcn_rcpf(Wc);
         const float su0 = fmaf(Uc, rw, 0.5f), dsu = g.c[3] * rw;
         const float sv0 = fmaf(Vc, rw, 0.5f), dsv = g.c[7] * rw;
         for (int kk = 0; kk < kn; ++kk) {
