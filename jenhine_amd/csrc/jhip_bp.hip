@@ -120,6 +120,10 @@ typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
 
 __device__ inline uint32_t pack_f16x2(float a, float b)
 {
+    // a and b are FP32 values, each rounded once more here: the empty asm
+    // keeps the compiler from folding the product that made them into a
+    // mixed-precision multiply that rounds straight to FP16
+    asm volatile("" : "+v"(a), "+v"(b));
     half2_t h;
     h.x = (_Float16)a;
     h.y = (_Float16)b;
@@ -167,10 +171,17 @@ __global__ void __launch_bounds__(256) zfilter_kernel(const float* __restrict__ 
     }
 }
 
-// One thread per staged pixel: blockIdx.x * 256 + threadIdx.x = u,
-// blockIdx.y = row, blockIdx.z = frame.  Layouts: FP32 quads (16 B), BF16 /
-// FP16 quads (8 B) or, with strip, BF16 / FP16 column strips (8 B, rows
-// padded to W + 1 pixels; thread u = W writes the zero padding).
+// One thread per column and run of kStageRun staged rows:
+// blockIdx.x * 64 + threadIdx.x = u, (blockIdx.y * 4 + threadIdx.y) *
+// kStageRun = first row, blockIdx.z = frame.  Layouts: FP32 quads (16 B),
+// BF16 / FP16 quads (8 B) or, with strip, BF16 / FP16 column strips (8 B, rows
+// padded to W + 1 pixels; thread u = W - 1 also writes the zero padding).
+// The thread slides a window of input rows down its run in registers, so an
+// input row is loaded (kStageRun + NR + 1) / kStageRun times (NR = 2 for
+// quads, 4 for strips) rather than NR + 2 times; every staged value is the
+// same function of the same inputs as with one thread per staged pixel.
+constexpr int kStageRun = 16;
+
 template <int QT, bool strip>
 __global__ void __launch_bounds__(256) bp_stage_kernel(const float* __restrict__ filt,
                                                        const float* __restrict__ uw, int W, int V, int zf,
@@ -179,65 +190,71 @@ __global__ void __launch_bounds__(256) bp_stage_kernel(const float* __restrict__
 // smoothing of ZFilterAgent to the frames first (before the uW weight, as in
 // the reference chain Filter -> ZFilter -> BPCore::InitView).
 {
-    const int u = blockIdx.x * 256 + threadIdx.x;
-    const int rowpx = W + (strip ? 1 : 0);
+    // every staged value is rounded as written: q = RN(x uW), and the FP32
+    // quad's differences RN(q[u+1] - q[u]) are not fused with the products
+#pragma clang fp contract(off)
     static_assert(!(strip && QT == JH_F32), "FP32 is staged as quads");
-    if (u >= rowpx) return;
-    const int row = blockIdx.y;
+    constexpr int NR = strip ? 4 : 2;          // rows of the quad or the strip
+    constexpr int NX = kStageRun + NR + 1;     // window: rows row0-1 .. row0+kStageRun+NR-1
+    const int u = blockIdx.x * 64 + threadIdx.x;
+    const int row0 = (blockIdx.y * 4 + threadIdx.y) * kStageRun;
+    if (u >= W || row0 >= V) return;
+    const int rowpx = W + (strip ? 1 : 0);
     const size_t view = blockIdx.z;
-    const size_t frame = (size_t)W * V;
-    const size_t t = (view * V + row) * (size_t)rowpx + u;
-    if (u == W) {  // strip padding column
-        reinterpret_cast<u32x2_t*>(out)[t] = u32x2_t{0u, 0u};
-        return;
-    }
-    const float* fr = filt + view * frame;
+    const float* fr = filt + view * (size_t)W * V;
     const float* w = uw ? uw + view * W : nullptr;
     const bool right = u + 1 < W;
     const int u1 = right ? u + 1 : u;
     const float wa = w ? w[u] : 1.0f;
     const float wb = w && right ? w[u + 1] : 1.0f;
-    constexpr int NR = strip ? 4 : 2;  // rows of the quad or the strip
-    // rows row-1 .. row+NR of columns u (and u+1 for quads), clamped into the frame
-    float xa[NR + 2], xb[NR + 2];
+    // rows row0-1+j of columns u (and u+1 for quads), clamped into the frame
+    float xa[NX], xb[NX];
 #pragma unroll
-    for (int k = 0; k < NR + 2; ++k) {
-        const int r = min(max(row - 1 + k, 0), V - 1);
-        xa[k] = fr[(size_t)r * W + u];
-        xb[k] = strip ? 0.0f : fr[(size_t)r * W + u1];
+    for (int j = 0; j < NX; ++j) {
+        const int r = min(max(row0 - 1 + j, 0), V - 1);
+        xa[j] = fr[(size_t)r * W + u];
+        xb[j] = strip ? 0.0f : fr[(size_t)r * W + u1];
     }
-    // value of row row+k (k < NR) at column u or u+1, 0 beyond the frame
-    auto val = [&](const float* x, int k, float wgt, bool inside) {
-        if (row + k >= V || !inside) return 0.0f;
-        float v = x[k + 1];
+    // value of row row0+i at column u or u+1, 0 beyond the frame
+    auto val = [&](const float* x, int i, float wgt, bool inside) {
+        if (row0 + i >= V || !inside) return 0.0f;
+        float v = x[i + 1];
         if (zf) {
 #pragma clang fp contract(off)
             // ZFilterAgent.cpp:49-71, as zfilter_at: the clamped neighbours
             // are the clamped loads above
-            const float tt = v * 0.5f + x[k] * 0.25f;
-            v = tt + x[k + 2] * 0.25f;
+            const float tt = v * 0.5f + x[i] * 0.25f;
+            v = tt + x[i + 2] * 0.25f;
         }
         return v * wgt;
     };
     auto pack = [&](float lo, float hi) {
         return QT == JH_BF16 ? (f32_to_bf16_rne(lo) | (f32_to_bf16_rne(hi) << 16)) : pack_f16x2(lo, hi);
     };
-    if constexpr (QT == JH_F32) {
-        const float a0 = val(xa, 0, wa, true), b0 = val(xb, 0, wb, right);
-        const float a1 = val(xa, 1, wa, true), b1 = val(xb, 1, wb, right);
-        reinterpret_cast<float4*>(out)[t] = make_float4(a0, b0 - a0, a1, b1 - a1);
-    } else if constexpr (strip) {
-        // column u, rows row .. row+3
-        u32x2_t o;
-        o.x = pack(val(xa, 0, wa, true), val(xa, 1, wa, true));
-        o.y = pack(val(xa, 2, wa, true), val(xa, 3, wa, true));
-        reinterpret_cast<u32x2_t*>(out)[t] = o;
-    } else {
-        // rows row, row+1 of columns u, u+1
-        u32x2_t o;
-        o.x = pack(val(xa, 0, wa, true), val(xb, 0, wb, right));
-        o.y = pack(val(xa, 1, wa, true), val(xb, 1, wb, right));
-        reinterpret_cast<u32x2_t*>(out)[t] = o;
+    size_t t = (view * V + row0) * (size_t)rowpx + u;
+#pragma unroll
+    for (int r = 0; r < kStageRun; ++r) {
+        if (row0 + r < V) {
+            if constexpr (QT == JH_F32) {
+                const float a0 = val(xa, r, wa, true), b0 = val(xb, r, wb, right);
+                const float a1 = val(xa, r + 1, wa, true), b1 = val(xb, r + 1, wb, right);
+                reinterpret_cast<float4*>(out)[t] = make_float4(a0, b0 - a0, a1, b1 - a1);
+            } else if constexpr (strip) {
+                // column u, rows row .. row+3
+                u32x2_t o;
+                o.x = pack(val(xa, r, wa, true), val(xa, r + 1, wa, true));
+                o.y = pack(val(xa, r + 2, wa, true), val(xa, r + 3, wa, true));
+                reinterpret_cast<u32x2_t*>(out)[t] = o;
+                if (u == W - 1) reinterpret_cast<u32x2_t*>(out)[t + 1] = u32x2_t{0u, 0u};
+            } else {
+                // rows row, row+1 of columns u, u+1
+                u32x2_t o;
+                o.x = pack(val(xa, r, wa, true), val(xb, r, wb, right));
+                o.y = pack(val(xa, r + 1, wa, true), val(xb, r + 1, wb, right));
+                reinterpret_cast<u32x2_t*>(out)[t] = o;
+            }
+        }
+        t += rowpx;
     }
 }
 
@@ -1297,13 +1314,16 @@ static void stage(jh_bp_plan* p, const float* d_filtered, int view0, int nviews,
             const float* uw = weighted ? p->uw.as<float>() + (size_t)(view0 + c0) * p->W : nullptr;
             const float* in = d_filtered + (size_t)c0 * fin;
             void* out = static_cast<char*>(d_q) + (size_t)c0 * fout;
-            const dim3 grid((unsigned)((p->W + st + 255) / 256), (unsigned)p->V, (unsigned)n);
-            if (dtype == JH_F32) bp_stage_kernel<JH_F32, false><<<grid, 256, 0, s>>>(in, uw, p->W, p->V, zf, out);
+            const dim3 grid((unsigned)((p->W + 63) / 64), (unsigned)((p->V + 4 * kStageRun - 1) / (4 * kStageRun)),
+                            (unsigned)n);
+            const dim3 block(64, 4);
+            if (dtype == JH_F32) bp_stage_kernel<JH_F32, false><<<grid, block, 0, s>>>(in, uw, p->W, p->V, zf, out);
             else if (dtype == JH_BF16 && st)
-                bp_stage_kernel<JH_BF16, true><<<grid, 256, 0, s>>>(in, uw, p->W, p->V, zf, out);
-            else if (dtype == JH_BF16) bp_stage_kernel<JH_BF16, false><<<grid, 256, 0, s>>>(in, uw, p->W, p->V, zf, out);
-            else if (st) bp_stage_kernel<JH_F16, true><<<grid, 256, 0, s>>>(in, uw, p->W, p->V, zf, out);
-            else bp_stage_kernel<JH_F16, false><<<grid, 256, 0, s>>>(in, uw, p->W, p->V, zf, out);
+                bp_stage_kernel<JH_BF16, true><<<grid, block, 0, s>>>(in, uw, p->W, p->V, zf, out);
+            else if (dtype == JH_BF16)
+                bp_stage_kernel<JH_BF16, false><<<grid, block, 0, s>>>(in, uw, p->W, p->V, zf, out);
+            else if (st) bp_stage_kernel<JH_F16, true><<<grid, block, 0, s>>>(in, uw, p->W, p->V, zf, out);
+            else bp_stage_kernel<JH_F16, false><<<grid, block, 0, s>>>(in, uw, p->W, p->V, zf, out);
         }
         check_launch("bp_stage_kernel");
     }

@@ -587,3 +587,59 @@ def test_bp_layouts_agree_on_a_fine_detector(gpu):
     quad = bp.run(bp.stage(filt, "bf16"), "bf16").cpu().numpy()
     assert np.isfinite(quad).sum() > 1000
     np.testing.assert_array_equal(strip, quad)
+
+
+def _h16_bits(x, dtype):
+    """16-bit patterns of float32 values, rounded to nearest-even."""
+    x = np.ascontiguousarray(x, np.float32)
+    if dtype == "f16":
+        return x.astype(np.float16).view(np.uint16).astype(np.uint32)
+    b = x.view(np.uint32).astype(np.uint64)
+    return (((b + 0x7FFF + ((b >> 16) & 1)) >> 16) & 0xFFFF).astype(np.uint32)
+
+
+def _staged_reference(filt, uw, dtype, layout, zf):
+    """The staged layouts built from their definitions (jhip.h): q = frame * uW
+    (after the 1-2-1 v smoothing with clamped neighbours when zf), values of
+    rows beyond the frame and of the column right of the last one are 0."""
+    N, V, W = filt.shape
+    x = filt.astype(np.float32)
+    if zf:
+        xp = np.concatenate([x[:, :1], x, x[:, -1:]], axis=1)
+        x = (xp[:, 1:-1] * np.float32(0.5) + xp[:, :-2] * np.float32(0.25)) + xp[:, 2:] * np.float32(0.25)
+    q = np.zeros((N, V + 4, W + 1), np.float32)
+    q[:, :V, :W] = x * uw[:, None, :].astype(np.float32)
+    if dtype == "f32":
+        a0, b0, a1, b1 = q[:, :V, :W], q[:, :V, 1:], q[:, 1:V + 1, :W], q[:, 1:V + 1, 1:]
+        return np.stack([a0, b0 - a0, a1, b1 - a1], axis=-1).view(np.uint32)
+    h = lambda a: _h16_bits(a, dtype)
+    if layout == "strip":
+        w0 = h(q[:, 0:V]) | (h(q[:, 1:V + 1]) << 16)
+        w1 = h(q[:, 2:V + 2]) | (h(q[:, 3:V + 3]) << 16)
+        out = np.stack([w0, w1], axis=-1)
+        out[:, :, W] = 0
+        return out
+    ox = h(q[:, 0:V, :W]) | (h(q[:, 0:V, 1:]) << 16)
+    oy = h(q[:, 1:V + 1, :W]) | (h(q[:, 1:V + 1, 1:]) << 16)
+    return np.stack([ox, oy], axis=-1)
+
+
+@pytest.mark.parametrize("dtype,layout", [("f32", "quad"), ("bf16", "quad"), ("bf16", "strip"), ("f16", "quad"),
+                                          ("f16", "strip")])
+@pytest.mark.parametrize("zf", [False, True])
+def test_stage_layouts_bit_exact(gpu, dtype, layout, zf):
+    """jh_bp_stage against the layouts' definitions, to the bit, on a frame
+    whose width and height are not multiples of the kernel's column block
+    (64) and row run (16): edge columns, the strip padding column and the
+    rows beyond the frame."""
+    from jenhine_amd.recon import BPCore
+    w = SMALL.scaled(n_views=5, n_u=50, n_v=70, name="ragged-frame")
+    bp = BPCore(w.ptms(), w.n_u, w.n_v, w.X, w.Y, w.Z, w.pxy, w.pz, device=gpu)
+    if dtype != "f32":
+        bp.set_layout(layout)
+    bp.set_zfilter(zf)
+    filt = synth.normal(33, (w.n_views, w.n_v, w.n_u)).astype(np.float32)
+    got = bp.stage(torch.from_numpy(filt).to(gpu), dtype).cpu().numpy().view(np.uint32)
+    ref = _staged_reference(filt, bp.uweights(), dtype, layout, zf)
+    assert got.size == ref.size
+    np.testing.assert_array_equal(got, ref.reshape(-1))
