@@ -164,22 +164,27 @@ __device__ inline float load_corrected(const uint16_t* raw, const uint16_t* dark
     return f < 1.0f ? 1.0f : f;
 }
 
+// Grid: blockIdx.z = frame, (blockIdx.y * 4 + threadIdx.y) = output row,
+// blockIdx.x * 256 + i * 64 + threadIdx.x (i < 4) = output columns: each
+// load and store instruction of a wave covers 64 consecutive pixels, and no
+// thread divides an index.
 __global__ void __launch_bounds__(256) preprocess_kernel(
     const uint16_t* __restrict__ raw, const uint16_t* __restrict__ dark,
-    const float* __restrict__ gain, float* __restrict__ out, int nviews, int inW, int inH, int bUp,
+    const float* __restrict__ gain, float* __restrict__ out, int inW, int inH, int bUp,
     int bLeft, int roiW, int roiH, int outW, int outH, int resize, float bf, float a0, float a1,
     float a2, int doBH)
 {
 #pragma clang fp contract(off)
-    const size_t frame_out = (size_t)outW * outH;
-    const size_t total = frame_out * nviews;
-    const size_t frame_in = (size_t)inW * inH;
-    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
-         t += (size_t)gridDim.x * blockDim.x) {
-        const size_t view = t / frame_out;
-        const int pix = (int)(t - view * frame_out);
-        const int y = pix / outW, x = pix - y * outW;
-        const uint16_t* r = raw + view * frame_in;
+    const int y = blockIdx.y * 4 + threadIdx.y;
+    if (y >= outH) return;
+    const size_t view = blockIdx.z;
+    const uint16_t* r = raw + view * (size_t)inW * inH;
+    float* o = out + (view * outH + y) * (size_t)outW;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int x = blockIdx.x * 256 + i * 64 + threadIdx.x;
+        if (x >= outW) break;
+        const int pix = y * outW + x;
         float f;
         if (!resize) {
             f = load_corrected(r, dark, (size_t)(y + bUp) * inW + (x + bLeft));
@@ -215,7 +220,7 @@ __global__ void __launch_bounds__(256) preprocess_kernel(
             b = b + a0;
             p = p * b;
         }
-        out[t] = p;
+        o[x] = p;
     }
 }
 
@@ -288,11 +293,15 @@ jh_status jh_pre_run(jh_pre_plan* p, const uint16_t* d_raw, const uint16_t* d_da
         DeviceScope ds(p->device);
         const int roiW = p->inW - p->bLeft - p->bRight, roiH = p->inH - p->bUp - p->bDown;
         const int resize = (roiW != p->outW || roiH != p->outH) ? 1 : 0;
-        const size_t total = (size_t)p->outW * p->outH * nviews;
-        const size_t blocks = std::min<size_t>((total + 255) / 256, (size_t)256 * 1024);
-        preprocess_kernel<<<(unsigned)blocks, 256, 0, as_stream(stream)>>>(
-            d_raw, d_dark, d_gain, d_out, nviews, p->inW, p->inH, p->bUp, p->bLeft, roiW, roiH,
-            p->outW, p->outH, resize, p->brightField, p->bh[0], p->bh[1], p->bh[2], p->doBH ? 1 : 0);
+        const size_t fin = (size_t)p->inW * p->inH, fout = (size_t)p->outW * p->outH;
+        for (int c0 = 0; c0 < nviews; c0 += 65535) {  // grid z: frames
+            const int n = std::min(nviews - c0, 65535);
+            const dim3 grid((unsigned)((p->outW + 255) / 256), (unsigned)((p->outH + 3) / 4), (unsigned)n);
+            preprocess_kernel<<<grid, dim3(64, 4), 0, as_stream(stream)>>>(
+                d_raw + (size_t)c0 * fin, d_dark, d_gain, d_out + (size_t)c0 * fout, p->inW, p->inH, p->bUp,
+                p->bLeft, roiW, roiH, p->outW, p->outH, resize, p->brightField, p->bh[0], p->bh[1], p->bh[2],
+                p->doBH ? 1 : 0);
+        }
         check_launch("preprocess_kernel");
     });
 }
