@@ -259,8 +259,9 @@ def main():
                      "note": "achieved counts the per-view formulation's 8 B/voxel-update (FP32 accumulator "
                              "read + write); frac > 1 means the register-accumulating kernel beats that roof. "
                              "traffic: measured HBM bytes per launch (FETCH_SIZE + WRITE_SIZE). Measured "
-                             "limiter: the vector-memory path's cache-line rate (22 L1 accesses per b128 "
-                             "gather, ~25 cycles per gather per CU, profiles/bp_dental512_pmc.md)"},
+                             "limiter: the vector-memory path's per-instruction rate (21 L1 accesses per "
+                             "16-byte gather, ~27 cycles per gather per CU; VALU 58 % busy, "
+                             "profiles/bp_dental512_pmc.md)"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
