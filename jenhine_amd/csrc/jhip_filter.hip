@@ -320,14 +320,37 @@ __global__ void __launch_bounds__(256) filter_mfma_kernel(FiltArgs a)
             for (int rb = 0; rb < 2; ++rb) {
                 float e[8];
                 load_a8(a, xr[rb], pwr[rb], kb, e);
+                if constexpr (P == 3) {
+                    // hi / mid / lo by truncation: each part is the top 16
+                    // bits of the remainder (exact in BF16, same sign), the
+                    // remainders are exact, and the three parts sum to the
+                    // FP32 value exactly; one v_perm packs a part of two k
+                    u32x4_t w[3];
 #pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    float r = e[q];
+                    for (int h = 0; h < 4; ++h) {
+                        float r0 = e[2 * h], r1 = e[2 * h + 1];
 #pragma unroll
-                    for (int p = 0; p < P; ++p) {
-                        const float h = bf16_round(r);
-                        ap[rb][p][q] = to_bf16(h);
-                        r = r - h;
+                        for (int p = 0; p < 3; ++p) {
+                            const uint32_t b0 = __float_as_uint(r0), b1 = __float_as_uint(r1);
+                            w[p][h] = __builtin_amdgcn_perm(b1, b0, 0x07060302u);
+                            if (p < 2) {
+                                r0 = r0 - __uint_as_float(b0 & 0xffff0000u);
+                                r1 = r1 - __uint_as_float(b1 & 0xffff0000u);
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int p = 0; p < 3; ++p) ap[rb][p] = __builtin_bit_cast(bf16x8_t, w[p]);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        float r = e[q];
+#pragma unroll
+                        for (int p = 0; p < P; ++p) {
+                            const float h = bf16_round(r);
+                            ap[rb][p][q] = to_bf16(h);
+                            r = r - h;
+                        }
                     }
                 }
             }
