@@ -224,6 +224,29 @@ def test_bp_general_geometry_modes(gpu):
         assert e <= 3e-4
 
 
+@pytest.mark.parametrize("dtype,tol", [("bf16", 5e-3), ("f16", 1e-3)])
+def test_bp_general_geometry_modes_16bit_storage(gpu, dtype, tol):
+    # the per-sample weight packing of the 16-bit samplers (modes 1 and 2
+    # form the u weights sample by sample); white noise, 2 views, no averaging:
+    # BF16 rounds each value by up to 2^-9 relative and the near-column weight
+    # moves u by up to 2^-10 pixel, FP16 by 2^-12 and 2^-12 pixel
+    P = np.array([-1.33897, -7.81194, 0.01236, 575.79443, -0.97515, -0.03862, 7.51861, 346.26642,
+                  -0.00226, -0.00004, 0.0, 1.0], np.float64)
+    tilt = P.copy()
+    tilt[10] = 0.0003
+    for ptm, mode in ((np.stack([P, P * 1.0001]), 1), (np.stack([tilt, P]), 2)):
+        ptm = ptm.reshape(-1, 12)
+        filt = synth.normal(13, (2, 686, 644))
+        shape = (96, 80, 20, 0.25, 1.0)
+        vol, bp = _bp_gpu(gpu, ptm, filt, shape, dtype=dtype)
+        assert bp.geometry_mode() == mode
+        ref = o.backproject(filt, ptm.astype(np.float32), (20, 80, 96), 0.25, 1.0)
+        assert np.mean(np.isnan(vol) != np.isnan(ref)) < 1e-3
+        e = rel_rmse(vol, ref)
+        print("mode", mode, dtype, "rel-RMSE vs FP32-ref", e)
+        assert e <= tol
+
+
 def test_bp_reference_gpu_case(gpu):
     # UTBPCUDACore.cpp:16-201: one view, 644 x 686 detector, 500 x 500 x 50
     # volume; projection 0.5 on its first half and 1.0 on the second; at most
