@@ -151,13 +151,20 @@ def main():
         torch.cuda.synchronize()
         step = (time.perf_counter() - t) / args.steps
         # stages of one more pass, timed with events
+        # (filter and staging are one kernel when the plans fuse)
+        fused = fdk.fused()
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
         ev[0].record()
         p = fdk.pre.process(raw, dark, gain)
         ev[1].record()
-        f = fdk.filter.process(p)
-        ev[2].record()
-        q = fdk.bp.stage(f, fdk.storage)
+        if fused:
+            f = None
+            q = fdk.filter.process_staged(p, fdk.bp, fdk.storage)
+            ev[2].record()
+        else:
+            f = fdk.filter.process(p)
+            ev[2].record()
+            q = fdk.bp.stage(f, fdk.storage)
         ev[3].record()
         vol = fdk.bp.run(q, fdk.storage)
         ev[4].record()
@@ -174,7 +181,8 @@ def main():
         vols[name] = vol.cpu().numpy()
         results[name] = dict(step_ms=step * 1e3, gups=units / step / 1e9, bp_ms=bp_ms,
                              bp_gups=units / (bp_ms * 1e-3) / 1e9,
-                             stage_ms=dict(pre=st[0], filter=st[1], stage=st[2], bp=st[3]))
+                             stage_ms=(dict(pre=st[0], filter_stage=st[1], bp=st[3]) if fused else
+                                       dict(pre=st[0], filter=st[1], stage=st[2], bp=st[3])))
         log(name, json.dumps(results[name]))
         del p, f, q
 
@@ -299,15 +307,21 @@ def run_offset(args, dev):
             vol = fdk(raw, dark, gain)
         torch.cuda.synchronize()
         step = (time.perf_counter() - t) / args.steps
+        fused = fdk.fused()
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
         ev[0].record()
         p = fdk.pre.process(raw, dark, gain)
         ev[1].record()
         c = fdk.composite.process(p)
         ev[2].record()
-        f = fdk.filter.process(c)
-        ev[3].record()
-        q = fdk.bp.stage(f, storage)
+        if fused:
+            f = None
+            q = fdk.filter.process_staged(c, fdk.bp, storage)
+            ev[3].record()
+        else:
+            f = fdk.filter.process(c)
+            ev[3].record()
+            q = fdk.bp.stage(f, storage)
         ev[4].record()
         vol = fdk.bp.run(q, storage)
         ev[5].record()
@@ -315,7 +329,9 @@ def run_offset(args, dev):
         stt = [ev[i].elapsed_time(ev[i + 1]) for i in range(5)]
         vols[name] = vol.cpu().numpy()
         results[name] = dict(step_ms=step * 1e3, gups=units / step / 1e9,
-                             stage_ms=dict(pre=stt[0], composite=stt[1], filter=stt[2], stage=stt[3], bp=stt[4]))
+                             stage_ms=(dict(pre=stt[0], composite=stt[1], filter_stage=stt[2], bp=stt[4]) if fused
+                                       else dict(pre=stt[0], composite=stt[1], filter=stt[2], stage=stt[3],
+                                                 bp=stt[4])))
         log(name, json.dumps(results[name]))
         del p, c, f, q, fdk
     acc = {}

@@ -154,6 +154,21 @@ jh_status jh_filter_preweight(jh_filter_plan* plan, int view, float* d_out, jh_s
 jh_status jh_filter_run(jh_filter_plan* plan, const float* d_in, float* d_out, int view0,
                         int nviews, jh_stream_t stream);
 
+/* Whether jh_filter_run_staged can take the plan pair: an MFMA filter whose
+ * Wout is the backprojector's W (same V and view count, same device), and a
+ * backprojector staging `dtype` (JH_BF16 / JH_F16) as column strips without
+ * the ZFilter.  *fusable = 1 or 0. */
+jh_status jh_filter_stage_fusable(const jh_filter_plan* plan, const jh_bp_plan* bp, int dtype, int* fusable);
+
+/* jh_filter_run followed by jh_bp_stage in one pass, with bit-identical
+ * output: views view0..view0+nviews-1 are filtered (FilterCore::ProcessRow,
+ * FilterCore.cpp:189-222), weighted by uW and written in the backprojector's
+ * staged layout (BPCore::InitView, BPCore.cpp:147-151) to d_q
+ * (jh_bp_stage_bytes bytes); the filtered FP32 frames are not written.
+ * Requires jh_filter_stage_fusable. */
+jh_status jh_filter_run_staged(jh_filter_plan* plan, jh_bp_plan* bp, const float* d_in, int view0, int nviews,
+                               int dtype, void* d_q, jh_stream_t stream);
+
 jh_status jh_filter_destroy(jh_filter_plan* plan);
 
 /* ---------------------------------------------------------------------- */
@@ -214,7 +229,11 @@ jh_status jh_bp_stage(jh_bp_plan* plan, const float* d_filtered, int view0, int 
                       void* d_q, jh_stream_t stream);
 
 /* One-shot normalised backprojection of all views: d_vol (Z x Y x X f32) =
- * sum_v bilinear(q_v) / sum_v bilinear(uW_v).  Replaces the whole
+ * sum_v bilinear(q_v) / sum_v bilinear(uW_v), accumulated in FP32.  The u
+ * weight of a BF16 sample is taken from the nearer column and rounded to BF16
+ * (u off by at most 2^-10 pixel), of an FP16 sample rounded to a multiple of
+ * 1/2048 (2^-12 pixel); FP32 samples use the exact fraction.  The count
+ * weights use the same u weight as the samples.  Replaces the whole
  * BPCore InitShot/InitView/ProcessSlice/DoneSlice sequence
  * (BPCore.cpp:108-205) and BPCUDACore's per-view loop
  * (BPCUDAAgent/BPCUDAAgent.cpp:231-287). */

@@ -29,6 +29,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "jhip_bp_plan.h"
 #include "jhip_internal.h"
 
 struct jh_filter_plan {
@@ -126,6 +127,11 @@ struct FiltArgs {
     int W, V, rows, Kp, view0;
     int Wout;            // output width
     int edge0;           // first of the three edge columns (0, or W - 3 with yMirroring)
+    // fused staging (out unused): BF16/FP16 column strips of the launch's
+    // views, q = filtered * uW, rows padded to W + 1 pixels
+    void* q;             // NULL: write the filtered FP32 rows to out
+    const float* uw;     // uW of the launch's first view onwards, Wout per view
+    int qtype;           // JH_BF16 or JH_F16
 };
 
 // Element (r, m) of the A operand.
@@ -250,6 +256,118 @@ __device__ inline void load_a8(const FiltArgs& a, const float* xr, const float* 
     }
 }
 
+// 16-bit pattern of x rounded to nearest-even: BF16 (as the backprojector's
+// staging) or FP16 (the empty asm keeps the product that made x from being
+// folded into a multiply that rounds straight to FP16).
+__device__ inline uint32_t round16(float x, int qtype)
+{
+    if (qtype == JH_BF16) {
+        uint32_t u = __float_as_uint(x);
+        if ((u & 0x7fffffffu) > 0x7f800000u) return ((u >> 16) | 0x40u) & 0xffffu;  // quiet NaN
+        u += 0x7fffu + ((u >> 16) & 1u);
+        return u >> 16;
+    }
+    asm volatile("" : "+v"(x));
+    return __builtin_bit_cast(unsigned short, (_Float16)x);
+}
+
+constexpr int kStagePitch = 136;  // LDS row pitch (16-bit values) of the staged tile: 4 rows apart = 16 banks
+
+// Fused staging epilogue of filter_mfma_kernel: the workgroup's 128 x 128
+// tile of filtered values, times the backprojector's uW and rounded to 16
+// bits, goes through LDS and is written as column strips -- pixel (v, u)
+// holds rows v .. v+3 of column u, 0 beyond the view's last row, rows padded
+// to W + 1 pixels (the padding column is 0) -- bit-identical to
+// jh_filter_run followed by jh_bp_stage.  A strip whose rows reach into the
+// next row tile is written in 16-bit slots, each by the workgroup that owns
+// the slot's row; a slot whose row lies beyond the view or the launch is
+// written as 0 by the strip's owner.
+__device__ void stage_strips_epilogue(const FiltArgs& a, const f32x4_t (&acc)[2][8], unsigned char* lds, int lane,
+                                      int wave)
+{
+    const int T0 = blockIdx.y * 128, C0 = blockIdx.x * 128;
+    const int vt = T0 / a.V, vrow0 = T0 - vt * a.V;  // view and row (in its view) of the tile's first row
+    uint16_t* t16 = reinterpret_cast<uint16_t*>(lds);
+    __syncthreads();  // every wave is done with the B tile
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int rl = wave * 32 + rb * 16 + 4 * (lane >> 4) + q, row = T0 + rl;
+            int view = vt, vrow = vrow0 + rl;
+            while (vrow >= a.V) { vrow -= a.V; ++view; }
+            const float* uwv = a.uw + (size_t)view * a.Wout;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const int cl = c * 16 + (lane & 15), col = C0 + cl;
+                uint32_t h = 0;
+                if (row < a.rows && col < a.Wout) h = round16(acc[rb][c][q] * uwv[col], a.qtype);
+                t16[rl * kStagePitch + cl] = (uint16_t)h;
+            }
+        }
+    __syncthreads();
+    const int cl = threadIdx.x & 127, half = threadIdx.x >> 7;
+    const int col = C0 + cl;
+    if (col >= a.Wout) return;
+    const bool padcol = col == a.Wout - 1;  // also writes the zero padding column
+    const size_t rowpx = (size_t)a.Wout + 1;
+    // strip pixel (v, col) of launch row v (frames are contiguous: v = view V + row)
+    char* const q = static_cast<char*>(a.q) + (size_t)col * 8;
+    int vl = half * 64;
+    int vend = (vt + 1) * a.V;  // first row of the view after row T0 + vl
+    {
+        int vrow = vrow0 + vl;
+        while (vrow >= a.V) { vrow -= a.V; vend += a.V; }
+    }
+    // window of the staged values of rows vl .. vl+3 (0 where not this tile's)
+    uint32_t w1 = t16[vl * kStagePitch + cl], w2 = t16[(vl + 1) * kStagePitch + cl];
+    uint32_t w3 = t16[(vl + 2) * kStagePitch + cl];
+    for (int k = 0; k < 64; ++k, ++vl) {
+        const int R = T0 + vl;
+        if (R >= a.rows) break;
+        if (R >= vend) vend += a.V;
+        const uint32_t w4 = vl + 3 < 128 ? t16[(vl + 3) * kStagePitch + cl] : 0u;
+        const uint32_t hv[4] = {w1, w2, w3, w4};
+        bool all = true;
+        uint32_t mine = 0;  // bit s: this workgroup writes slot s
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int r = R + s;
+            const bool beyond = r >= vend || r >= a.rows;  // written as 0 here
+            const bool here = vl + s < 128;                 // else the next tile's owner writes it
+            if (beyond || here) mine |= 1u << s;
+            else all = false;
+        }
+        uint32_t v4[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) v4[s] = (R + s >= vend || R + s >= a.rows) ? 0u : hv[s];
+        char* px = q + (size_t)R * rowpx * 8;
+        if (all) {
+            *reinterpret_cast<uint2*>(px) = make_uint2(v4[0] | (v4[1] << 16), v4[2] | (v4[3] << 16));
+        } else {
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+                if (mine & (1u << s)) *reinterpret_cast<uint16_t*>(px + 2 * s) = (uint16_t)v4[s];
+        }
+        if (padcol) *reinterpret_cast<uint2*>(px + 8) = make_uint2(0u, 0u);
+        w1 = w2; w2 = w3; w3 = w4;
+    }
+    // this tile's first rows in the last strips of the previous tile (same view only)
+    if (half == 0 && T0 > 0) {
+#pragma unroll
+        for (int rl = 0; rl < 3; ++rl) {
+            const int r = T0 + rl;
+            if (r >= a.rows) break;
+            int vrow = vrow0 + rl;
+            while (vrow >= a.V) vrow -= a.V;
+#pragma unroll
+            for (int s = rl + 1; s < 4; ++s)
+                if (vrow >= s)  // row r - s is in r's view
+                    *reinterpret_cast<uint16_t*>(q + (size_t)(r - s) * rowpx * 8 + 2 * s) = t16[rl * kStagePitch + cl];
+        }
+    }
+}
+
 template <int MODE>
 __global__ void __launch_bounds__(256) filter_mfma_kernel(FiltArgs a)
 {
@@ -258,7 +376,10 @@ __global__ void __launch_bounds__(256) filter_mfma_kernel(FiltArgs a)
     constexpr int LB = M::lane_bytes;                 // B bytes per lane per column block
     constexpr int TILE_BYTES = 8 * 64 * LB;           // B bytes per k step per workgroup
     constexpr int CHUNKS = TILE_BYTES / 16 / 256;     // 16-byte chunks per thread
-    __shared__ __attribute__((aligned(16))) unsigned char lds[TILE_BYTES];
+    // the B tile of a k step; after the last step, the staged tile of the
+    // fused epilogue (128 rows at kStagePitch 16-bit values)
+    constexpr int LDS_BYTES = TILE_BYTES > 128 * kStagePitch * 2 ? TILE_BYTES : 128 * kStagePitch * 2;
+    __shared__ __attribute__((aligned(16))) unsigned char lds[LDS_BYTES];
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -379,6 +500,10 @@ __global__ void __launch_bounds__(256) filter_mfma_kernel(FiltArgs a)
             for (int c = 0; c < CHUNKS; ++c) reinterpret_cast<u32x4_t*>(lds)[threadIdx.x + c * 256] = pre[c];
         }
         __syncthreads();
+    }
+    if (a.q != nullptr) {
+        stage_strips_epilogue(a, acc, lds, lane, wave);
+        return;
     }
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb)
@@ -571,6 +696,56 @@ jh_status create_filter(jh_filter_plan** plan, int device, int W, int V, int Wou
     });
 }
 
+
+bool stage_fusable(const jh_filter_plan* p, const jh_bp_plan* bp, int dtype)
+{
+    return p != nullptr && bp != nullptr && p->mode != JH_FILTER_FP32_VALU && (dtype == JH_BF16 || dtype == JH_F16) &&
+           bp->strip && !bp->zfilter && bp->device == p->device && p->Wout == bp->W && p->V == bp->V &&
+           p->N == bp->N;
+}
+
+// One launch of the filter over views view0 .. view0+nviews-1 of the scan:
+// FP32 rows to d_out, or (d_q != NULL) fused staging into column strips.
+void run_filter(jh_filter_plan* p, const float* d_in, float* d_out, int view0, int nviews, jh_stream_t stream,
+                void* d_q, const float* uw, int qtype)
+{
+    require(view0 >= 0 && nviews >= 0 && view0 + nviews <= p->N, "jh_filter_run: view range outside the scan");
+    if (nviews == 0) return;
+    DeviceScope ds(p->device);
+    FiltArgs a;
+    a.x = d_in;
+    a.pw = p->pw.as<float>();
+    a.blk = p->blk.as<int>();
+    a.B = p->B.as<float>();
+    a.Bhl = p->Bhl.get();
+    a.out = d_out;
+    a.W = p->W; a.V = p->V;
+    a.Wout = p->Wout;
+    a.edge0 = p->mirror ? p->W - 3 : 0;  // FilterCore.cpp:205-215
+    a.rows = p->V * nviews;
+    a.Kp = p->Kp;
+    a.view0 = view0;
+    a.q = d_q;
+    a.uw = uw;
+    a.qtype = qtype;
+    require((size_t)p->V * nviews < (1u << 31), "jh_filter_run: too many rows");
+    hipStream_t st = as_stream(stream);
+    if (p->mode == JH_FILTER_FP32_VALU) {
+        require(d_q == nullptr, "jh_filter_run: the vector-ALU filter does not stage");
+        dim3 grid((p->Wout + 63) / 64, (a.rows + 63) / 64);
+        require(grid.y <= 65535u, "jh_filter_run: too many rows for one launch");
+        filter_fp32_kernel<<<grid, 256, 0, st>>>(a);
+        check_launch("filter_fp32_kernel");
+    } else {
+        dim3 grid((p->Wout + 127) / 128, (a.rows + 127) / 128);
+        require(grid.y <= 65535u, "jh_filter_run: too many rows for one launch");
+        if (p->mode == JH_FILTER_FP32) filter_mfma_kernel<kModeF32><<<grid, 256, 0, st>>>(a);
+        else if (p->mode == JH_FILTER_MFMA_BF16X3) filter_mfma_kernel<kModeX3><<<grid, 256, 0, st>>>(a);
+        else filter_mfma_kernel<kModeX6><<<grid, 256, 0, st>>>(a);
+        check_launch("filter_mfma_kernel");
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -629,38 +804,29 @@ jh_status jh_filter_run(jh_filter_plan* p, const float* d_in, float* d_out, int 
 {
     return guard([&] {
         require(p != nullptr && d_in != nullptr && d_out != nullptr, "jh_filter_run: bad arguments");
-        require(view0 >= 0 && nviews >= 0 && view0 + nviews <= p->N, "jh_filter_run: view range outside the scan");
         require(d_in != d_out, "jh_filter_run: in-place filtering is not supported");
-        if (nviews == 0) return;
-        DeviceScope ds(p->device);
-        FiltArgs a;
-        a.x = d_in;
-        a.pw = p->pw.as<float>();
-        a.blk = p->blk.as<int>();
-        a.B = p->B.as<float>();
-        a.Bhl = p->Bhl.get();
-        a.out = d_out;
-        a.W = p->W; a.V = p->V;
-        a.Wout = p->Wout;
-        a.edge0 = p->mirror ? p->W - 3 : 0;  // FilterCore.cpp:205-215
-        a.rows = p->V * nviews;
-        a.Kp = p->Kp;
-        a.view0 = view0;
-        require((size_t)p->V * nviews < (1u << 31), "jh_filter_run: too many rows");
-        hipStream_t st = as_stream(stream);
-        if (p->mode == JH_FILTER_FP32_VALU) {
-            dim3 grid((p->Wout + 63) / 64, (a.rows + 63) / 64);
-            require(grid.y <= 65535u, "jh_filter_run: too many rows for one launch");
-            filter_fp32_kernel<<<grid, 256, 0, st>>>(a);
-            check_launch("filter_fp32_kernel");
-        } else {
-            dim3 grid((p->Wout + 127) / 128, (a.rows + 127) / 128);
-            require(grid.y <= 65535u, "jh_filter_run: too many rows for one launch");
-            if (p->mode == JH_FILTER_FP32) filter_mfma_kernel<kModeF32><<<grid, 256, 0, st>>>(a);
-            else if (p->mode == JH_FILTER_MFMA_BF16X3) filter_mfma_kernel<kModeX3><<<grid, 256, 0, st>>>(a);
-            else filter_mfma_kernel<kModeX6><<<grid, 256, 0, st>>>(a);
-            check_launch("filter_mfma_kernel");
-        }
+        run_filter(p, d_in, d_out, view0, nviews, stream, nullptr, nullptr, 0);
+    });
+}
+
+jh_status jh_filter_stage_fusable(const jh_filter_plan* p, const jh_bp_plan* bp, int dtype, int* fusable)
+{
+    return guard([&] {
+        require(fusable != nullptr, "jh_filter_stage_fusable: bad arguments");
+        *fusable = stage_fusable(p, bp, dtype) ? 1 : 0;
+    });
+}
+
+jh_status jh_filter_run_staged(jh_filter_plan* p, jh_bp_plan* bp, const float* d_in, int view0, int nviews,
+                               int dtype, void* d_q, jh_stream_t stream)
+{
+    return guard([&] {
+        require(p != nullptr && bp != nullptr && d_in != nullptr && d_q != nullptr,
+                "jh_filter_run_staged: bad arguments");
+        require(stage_fusable(p, bp, dtype),
+                "jh_filter_run_staged: the plans do not fuse (MFMA filter writing the backprojector's W columns, "
+                "BF16/FP16 column strips, no ZFilter, same device, frame size and view count)");
+        run_filter(p, d_in, nullptr, view0, nviews, stream, d_q, bp->uw.as<float>() + (size_t)view0 * bp->W, dtype);
     });
 }
 

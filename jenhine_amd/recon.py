@@ -135,6 +135,24 @@ class FilterCore:
         call("jh_filter_run", self._h, _ptr(frames), _ptr(out), view0, n, _stream())
         return out
 
+    def stage_fusable(self, bp, dtype):
+        """Whether process_staged can feed `bp` in `dtype` (see jh_filter_stage_fusable)."""
+        ok = ctypes.c_int()
+        call("jh_filter_stage_fusable", self._h, bp._h, DTYPES[dtype], ctypes.byref(ok))
+        return bool(ok.value)
+
+    def process_staged(self, frames, bp, dtype, view0=0, out=None):
+        """process() followed by bp.stage() in one pass (bit-identical); the
+        filtered FP32 frames are not written."""
+        _require_cuda(frames, "frames", torch.float32)
+        if frames.dim() != 3 or frames.shape[1:] != (self.V, self.W):
+            raise ValueError(f"frames must be (N, {self.V}, {self.W})")
+        n = frames.shape[0]
+        if out is None:
+            out = torch.empty(bp.stage_bytes(dtype, n), dtype=torch.uint8, device=frames.device)
+        call("jh_filter_run_staged", self._h, bp._h, _ptr(frames), view0, n, DTYPES[dtype], _ptr(out), _stream())
+        return out
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
@@ -415,9 +433,17 @@ class FDK:
         if ct is not None:
             self.bp.set_ctnum(True, *ct)
 
+    def fused(self):
+        """Whether the filter writes the staged strips directly (filter and
+        staging in one kernel, FilterCore.process_staged)."""
+        return self.filter.stage_fusable(self.bp, self.storage)
+
     def __call__(self, raw, dark=None, gain=None, return_stages=False):
         p = self.pre.process(raw, dark, gain)
         c = self.composite.process(p) if self.composite is not None else p
+        if not return_stages and self.fused():
+            q = self.filter.process_staged(c, self.bp, self.storage)
+            return self.bp.run(q, self.storage)
         f = self.filter.process(c)
         q = self.bp.stage(f, self.storage)
         vol = self.bp.run(q, self.storage)

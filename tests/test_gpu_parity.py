@@ -666,3 +666,59 @@ def test_stage_layouts_bit_exact(gpu, dtype, layout, zf):
     ref = _staged_reference(filt, bp.uweights(), dtype, layout, zf)
     assert got.size == ref.size
     np.testing.assert_array_equal(got, ref.reshape(-1))
+
+
+@pytest.mark.parametrize("dtype", ["bf16", "f16"])
+@pytest.mark.parametrize("case", ["views-in-one-row-tile", "multiple-of-128-columns", "composite-2W-to-W"])
+def test_filter_run_staged_equals_filter_then_stage(gpu, dtype, case):
+    """jh_filter_run_staged writes, to the bit, what jh_filter_run followed by
+    jh_bp_stage writes: with frames shorter than the 128-row tile (tiles and
+    strips across view boundaries), with a width that is a multiple of the
+    128-column tile (the padding column comes from the last column), through
+    the composite chain's 2W -> W mirrored filter, and for a view range other
+    than the first (pre-weight blocks and uW of those views)."""
+    from jenhine_amd.recon import BPCore, FilterCore
+    mirror = False
+    if case == "views-in-one-row-tile":
+        w = SMALL.scaled(n_views=5, n_u=200, n_v=70, name="short-frames")
+    elif case == "multiple-of-128-columns":
+        w = SMALL.scaled(n_views=5, n_u=256, n_v=150, name="256-columns")
+    else:
+        w = _offset_geometry(True, n_views=6)
+        mirror = True
+    N, V, W = w.n_views, w.n_v, w.n_u
+    win = 2 * W if mirror else W
+    ptm = w.ptms()
+    scan = w.scan_params()
+    filt = FilterCore(win, V, ptm, scan.DSO, scan.HalfSampleRate, mode="mfma6", device=gpu, w_out=W,
+                      y_mirroring=mirror)
+    bp = BPCore(ptm, W, V, w.X, w.Y, w.Z, w.pxy, w.pz, device=gpu)
+    assert bp.layout() == "strip" and filt.stage_fusable(bp, dtype)
+    x = torch.from_numpy(synth.normal(41, (N, V, win))).to(gpu)
+    ref = bp.stage(filt.process(x), dtype)
+    got = filt.process_staged(x, bp, dtype)
+    assert got.numel() == ref.numel() == N * V * (W + 1) * 8
+    assert torch.equal(got, ref)
+    ref2 = bp.stage(filt.process(x[1:4], view0=2), dtype, view0=2)
+    got2 = filt.process_staged(x[1:4], bp, dtype, view0=2)
+    assert torch.equal(got2, ref2)
+
+
+def test_filter_run_staged_refuses_plans_that_do_not_fuse(gpu):
+    from jenhine_amd.recon import BPCore, FilterCore
+    w = SMALL.scaled(n_views=4, n_u=64, n_v=40, name="small-frames")
+    ptm = w.ptms()
+    scan = w.scan_params()
+    bp = BPCore(ptm, w.n_u, w.n_v, w.X, w.Y, w.Z, w.pxy, w.pz, device=gpu)
+    mfma = FilterCore(w.n_u, w.n_v, ptm, scan.DSO, scan.HalfSampleRate, mode="mfma6", device=gpu)
+    valu = FilterCore(w.n_u, w.n_v, ptm, scan.DSO, scan.HalfSampleRate, mode="fp32valu", device=gpu)
+    assert mfma.stage_fusable(bp, "bf16") and not mfma.stage_fusable(bp, "f32")
+    assert not valu.stage_fusable(bp, "bf16")
+    bp.set_zfilter(True)
+    assert not mfma.stage_fusable(bp, "bf16")
+    bp.set_zfilter(False)
+    bp.set_layout("quad")
+    assert not mfma.stage_fusable(bp, "bf16")
+    x = torch.zeros((w.n_views, w.n_v, w.n_u), dtype=torch.float32, device=gpu)
+    with pytest.raises(RuntimeError, match="do not fuse"):
+        mfma.process_staged(x, bp, "bf16")
