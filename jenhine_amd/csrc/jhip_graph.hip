@@ -260,9 +260,18 @@ jh_status jh_graph_instantiate(jh_graph* g, int nbranches)
                         const int n = std::min(s.chunk, s.bp->N - v0);
                         check_status(jh_pre_run(s.pre, s.raw + inFrame * v0, s.dark, s.gain, br.pre.as<float>(), n, st),
                                      "preprocess");
-                        check_status(jh_filter_run(s.filt, br.pre.as<float>(), br.filt.as<float>(), v0, n, st),
-                                     "filter");
-                        check_status(jh_bp_stage(s.bp, br.filt.as<float>(), v0, n, s.dtype, br.q.get(), st), "stage");
+                        int fused = 0;
+                        check_status(jh_filter_stage_fusable(s.filt, s.bp, s.dtype, &fused), "fusable");
+                        if (fused) {  // filter and staging in one kernel
+                            check_status(jh_filter_run_staged(s.filt, s.bp, br.pre.as<float>(), v0, n, s.dtype,
+                                                              br.q.get(), st),
+                                         "filter and stage");
+                        } else {
+                            check_status(jh_filter_run(s.filt, br.pre.as<float>(), br.filt.as<float>(), v0, n, st),
+                                         "filter");
+                            check_status(jh_bp_stage(s.bp, br.filt.as<float>(), v0, n, s.dtype, br.q.get(), st),
+                                         "stage");
+                        }
                         check_status(jh_bp_accumulate(s.bp, br.q.get(), s.dtype, v0, n, st), "accumulate");
                     }
                     check_status(jh_bp_finalize(s.bp, s.vol, 0, s.bp->Z, st), "finalize");
