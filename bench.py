@@ -151,13 +151,19 @@ def main():
         torch.cuda.synchronize()
         step = (time.perf_counter() - t) / args.steps
         # stages of one more pass, timed with events
-        # (filter and staging are one kernel when the plans fuse)
+        # (filter and staging are one kernel when the plans fuse; then the
+        # preprocessing writes the filter's pre-weighted operand)
         fused = fdk.fused()
+        operand = fdk.operand_path()
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
         ev[0].record()
-        p = fdk.pre.process(raw, dark, gain)
+        p = fdk.pre.process_operand(raw, fdk.filter, dark, gain) if operand else fdk.pre.process(raw, dark, gain)
         ev[1].record()
-        if fused:
+        if operand:
+            f = None
+            q = fdk.filter.process_operand_staged(p, raw.shape[0], fdk.bp, fdk.storage)
+            ev[2].record()
+        elif fused:
             f = None
             q = fdk.filter.process_staged(p, fdk.bp, fdk.storage)
             ev[2].record()
@@ -181,7 +187,8 @@ def main():
         vols[name] = vol.cpu().numpy()
         results[name] = dict(step_ms=step * 1e3, gups=units / step / 1e9, bp_ms=bp_ms,
                              bp_gups=units / (bp_ms * 1e-3) / 1e9,
-                             stage_ms=(dict(pre=st[0], filter_stage=st[1], bp=st[3]) if fused else
+                             stage_ms=(dict(pre_operand=st[0], filter_stage=st[1], bp=st[3]) if operand else
+                                       dict(pre=st[0], filter_stage=st[1], bp=st[3]) if fused else
                                        dict(pre=st[0], filter=st[1], stage=st[2], bp=st[3])))
         log(name, json.dumps(results[name]))
         del p, f, q

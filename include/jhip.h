@@ -169,6 +169,22 @@ jh_status jh_filter_stage_fusable(const jh_filter_plan* plan, const jh_bp_plan* 
 jh_status jh_filter_run_staged(jh_filter_plan* plan, jh_bp_plan* bp, const float* d_in, int view0, int nviews,
                                int dtype, void* d_q, jh_stream_t stream);
 
+/* The pre-weighted filter operand: preprocessing (PreProcessingCore::Process,
+ * PreProcessingCore.cpp:67-86, without resizing) written straight into the
+ * MFMA filter's A operand -- each pixel times its pre-weight (FilterCore's
+ * ProcessRow input, FilterCore.cpp:189-222, with FilterAgent's weighting),
+ * plus the three unweighted edge columns -- in the order the MFMA kernel
+ * loads it; jh_filter_run_operand_staged then filters and stages it.  The
+ * pair is bit-identical to jh_pre_run, jh_filter_run_staged.  Operand size:
+ * jh_filter_operand_bytes (16-row blocks of Kp floats).  Requires a
+ * non-resizing preprocessing plan whose output frame is the filter's input
+ * frame, an MFMA filter, and V * nviews <= 1048560. */
+jh_status jh_filter_operand_bytes(const jh_filter_plan* plan, int nviews, size_t* bytes);
+jh_status jh_pre_run_operand(jh_pre_plan* pre, jh_filter_plan* plan, const uint16_t* d_raw, const uint16_t* d_dark,
+                             const float* d_gain, int view0, int nviews, float* d_a, jh_stream_t stream);
+jh_status jh_filter_run_operand_staged(jh_filter_plan* plan, jh_bp_plan* bp, const float* d_a, int view0,
+                                       int nviews, int dtype, void* d_q, jh_stream_t stream);
+
 jh_status jh_filter_destroy(jh_filter_plan* plan);
 
 /* ---------------------------------------------------------------------- */

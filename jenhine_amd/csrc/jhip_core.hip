@@ -147,22 +147,9 @@ __global__ void ptm_modify_kernel(double* ptm, int n, int op, double val)
 }
 
 // ---------------------------------------------------------------------------
-// Fused preprocessing.  One thread per output pixel.  Restates
-// PreProcessingAgent/PreProcessingCore.cpp:67-86 with the steps of
-// GainCorr.cpp:10-13, NLog.cpp:9-13 and BeamHardeningCorr.cpp:17-27, in the
-// reference's float operation order (no contraction):
-//   v = raw - dark (integers, extension) ; f = max(float(v), 1)
-//   crop/resize ; f /= BrightField*gain ; p = -ln f ;
-//   BH: b = p*a2; b += a1; b *= p; b += a0; p *= b
+// Fused preprocessing (load_corrected, pre_log): one thread per output
+// pixel.
 // ---------------------------------------------------------------------------
-
-__device__ inline float load_corrected(const uint16_t* raw, const uint16_t* dark, size_t idx)
-{
-    int v = (int)raw[idx];
-    if (dark) v -= (int)dark[idx];
-    float f = (float)v;
-    return f < 1.0f ? 1.0f : f;
-}
 
 // Grid: blockIdx.z = frame, (blockIdx.y * 4 + threadIdx.y) = output row,
 // blockIdx.x * 256 + i * 64 + threadIdx.x (i < 4) = output columns: each
@@ -208,19 +195,7 @@ __global__ void __launch_bounds__(256) preprocess_kernel(
             const float b = p10 + tx * (p11 - p10);
             f = a + ty * (b - a);
         }
-        float d = bf;
-        if (gain) d = d * gain[pix];
-        f = f / d;
-        float p = logf(f);
-        p = p * -1.0f;
-        if (doBH) {
-            float b = p * a2;
-            b = b + a1;
-            b = b * p;
-            b = b + a0;
-            p = p * b;
-        }
-        o[x] = p;
+        o[x] = pre_log(f, bf, gain, (size_t)pix, a0, a1, a2, doBH);
     }
 }
 

@@ -132,6 +132,7 @@ struct FiltArgs {
     void* q;             // NULL: write the filtered FP32 rows to out
     const float* uw;     // uW of the launch's first view onwards, Wout per view
     int qtype;           // JH_BF16 or JH_F16
+    int tiled;           // x is the pre-weighted A operand in the MFMA tile order (pre_operand_kernel)
 };
 
 // Element (r, m) of the A operand.
@@ -189,6 +190,112 @@ __global__ void __launch_bounds__(256) filter_fp32_kernel(FiltArgs a)
             if (cc < a.Wout) a.out[(size_t)rr * a.Wout + cc] = c[p][q];
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Preprocessing written straight into the MFMA kernel's A operand.
+//
+// Element (r, k) of the operand, r a row of the launch (view r / V): the
+// preprocessed pixel times the pre-weight for k < W, the unweighted pixel
+// edge0 + k - W for W <= k < W + 3 (the truncation ramp's edge columns), 0
+// beyond -- the values the MFMA kernel forms from the preprocessed frames and
+// the pre-weight maps (load_a8), computed by the same operations.  Layout: the
+// floats (r, k) of 16-row block r16 and k step s = k / 32 that lane
+// l = 16 g + (r % 16), g = (k % 32) / 8, supplies, in two 16-byte halves h =
+// (k % 8) / 4: float4 index ((r16 nsteps + s) 2 + h) 64 + l.  The MFMA kernel
+// then reads 1 KB contiguous per wave load and no pre-weight.  Rows beyond
+// the launch are 0.
+// ---------------------------------------------------------------------------
+
+struct PreOpArgs {
+    const uint16_t* raw;  // nviews frames of inW x inH
+    const uint16_t* dark;
+    const float* gain;    // W x V (NULL: none)
+    const float* pw;      // nblocks x V x W pre-weight maps
+    const int* blk;       // block of every view of the scan
+    float4* out;
+    int inW, inH, bUp, bLeft;
+    int W, V, rows, nsteps, view0, edge0;
+    float bf, a0, a1, a2;
+    int doBH;
+    int vec;              // rows of counts, dark, gain and pre-weights allow 16-byte loads
+};
+
+constexpr int kOpPitch = 132;  // LDS row pitch (floats) of the operand tile
+
+// Workgroup: 16 rows x 128 k (four k steps) of the operand.  Loads: thread t
+// takes row t / 16 and the 8 columns 8 (t % 16) ..: a wave reads four rows of
+// 256 contiguous bytes of raw counts per instruction.  The values go through
+// LDS and leave in the operand order, 1 KB contiguous per wave store.
+__global__ void __launch_bounds__(256) pre_operand_kernel(PreOpArgs a)
+{
+    __shared__ float tile[16 * kOpPitch];
+    const int t = threadIdx.x;
+    const int i = t >> 4, cg = t & 15;
+    const int r = blockIdx.y * 16 + i;
+    const int kb = blockIdx.x * 128 + 8 * cg;
+    float e[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) e[j] = 0.0f;
+    if (r < a.rows && kb < a.W + 3) {
+        const int view = r / a.V, vrow = r - view * a.V;
+        const uint16_t* raw = a.raw + (size_t)view * a.inW * a.inH;
+        const size_t rowin = (size_t)(vrow + a.bUp) * a.inW + a.bLeft;
+        const size_t rowout = (size_t)vrow * a.W;
+        const float* pw = a.pw + ((size_t)a.blk[a.view0 + view] * a.V + vrow) * a.W;
+        if (kb + 8 <= a.W && a.vec) {
+            // 16-byte loads: 8 counts, 8 dark values, 8 gains, 8 pre-weights
+            const uint4 rv = *reinterpret_cast<const uint4*>(raw + rowin + kb);
+            uint4 dv = make_uint4(0u, 0u, 0u, 0u);
+            if (a.dark) dv = *reinterpret_cast<const uint4*>(a.dark + rowin + kb);
+            float g[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            if (a.gain) {
+                const float4 g0 = *reinterpret_cast<const float4*>(a.gain + rowout + kb);
+                const float4 g1 = *reinterpret_cast<const float4*>(a.gain + rowout + kb + 4);
+                g[0] = g0.x; g[1] = g0.y; g[2] = g0.z; g[3] = g0.w; g[4] = g1.x; g[5] = g1.y; g[6] = g1.z; g[7] = g1.w;
+            }
+            const float4 w0 = *reinterpret_cast<const float4*>(pw + kb);
+            const float4 w1 = *reinterpret_cast<const float4*>(pw + kb + 4);
+            const float w[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+            const uint32_t rw[4] = {rv.x, rv.y, rv.z, rv.w}, dw[4] = {dv.x, dv.y, dv.z, dv.w};
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const uint16_t rc = (uint16_t)(rw[j >> 1] >> (16 * (j & 1)));
+                const uint16_t dc = (uint16_t)(dw[j >> 1] >> (16 * (j & 1)));
+                // the steps of load_corrected and pre_log, on the loaded values
+                int v = (int)rc;
+                if (a.dark) v -= (int)dc;
+                float f = (float)v;
+                f = f < 1.0f ? 1.0f : f;
+                float d = a.bf;
+                if (a.gain) d = d * g[j];
+                e[j] = pre_log_div(f, d, a.a0, a.a1, a.a2, a.doBH) * w[j];
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int k = kb + j;
+                const int c = k < a.W ? k : a.edge0 + k - a.W;
+                if (k < a.W + 3) {
+                    const float p = pre_log(load_corrected(raw, a.dark, rowin + c), a.bf, a.gain, rowout + c, a.a0,
+                                            a.a1, a.a2, a.doBH);
+                    e[j] = k < a.W ? p * pw[k] : p;
+                }
+            }
+        }
+    }
+    float* dstt = tile + i * kOpPitch + 8 * cg;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) dstt[j] = e[j];
+    __syncthreads();
+    // operand order: k step s = 4 blockIdx.x + t / 64, lane l = t % 64, halves h
+    const int sl = t >> 6, l = t & 63;
+    const int s = blockIdx.x * 4 + sl;
+    if (s >= a.nsteps) return;
+    const float* src = tile + (l & 15) * kOpPitch + 32 * sl + 8 * (l >> 4);
+    float4* dst = a.out + ((size_t)blockIdx.y * a.nsteps + s) * 128 + l;
+    dst[0] = make_float4(src[0], src[1], src[2], src[3]);
+    dst[64] = make_float4(src[4], src[5], src[6], src[7]);
 }
 
 // ---------------------------------------------------------------------------
@@ -414,6 +521,26 @@ __global__ void __launch_bounds__(256) filter_mfma_kernel(FiltArgs a)
     for (int c = 0; c < CHUNKS; ++c) reinterpret_cast<u32x4_t*>(lds)[threadIdx.x + c * 256] = pre[c];
     __syncthreads();
 
+    // the 8 A values lane supplies for row block rb at k step s: formed from
+    // the preprocessed frame and the pre-weight map, or read from the
+    // pre-weighted operand (two 16-byte loads; the 16-row block exists in the
+    // operand iff it starts inside the launch)
+    auto load_a = [&](int rb, int s, int kb, float (&e)[8]) {
+        if (a.tiled) {
+            const int rs = blockIdx.y * 128 + wave * 32 + rb * 16;
+            float4 x0 = make_float4(0.f, 0.f, 0.f, 0.f), x1 = x0;
+            if (rs < a.rows) {
+                const float4* src = reinterpret_cast<const float4*>(a.x) + ((size_t)(rs / 16) * nsteps + s) * 128 + lane;
+                x0 = src[0];
+                x1 = src[64];
+            }
+            e[0] = x0.x; e[1] = x0.y; e[2] = x0.z; e[3] = x0.w;
+            e[4] = x1.x; e[5] = x1.y; e[6] = x1.z; e[7] = x1.w;
+        } else {
+            load_a8(a, xr[rb], pwr[rb], kb, e);
+        }
+    };
+
     for (int s = 0; s < nsteps; ++s) {
         if (s + 1 < nsteps) {
 #pragma unroll
@@ -423,7 +550,7 @@ __global__ void __launch_bounds__(256) filter_mfma_kernel(FiltArgs a)
         if (MODE == kModeF32) {
             float ea[2][8];
 #pragma unroll
-            for (int rb = 0; rb < 2; ++rb) load_a8(a, xr[rb], pwr[rb], kb, ea[rb]);
+            for (int rb = 0; rb < 2; ++rb) load_a(rb, s, kb, ea[rb]);
 #pragma unroll
             for (int c = 0; c < 8; ++c) {
                 const float4* bp = reinterpret_cast<const float4*>(lds + (c * 64 + lane) * LB);
@@ -440,7 +567,7 @@ __global__ void __launch_bounds__(256) filter_mfma_kernel(FiltArgs a)
 #pragma unroll
             for (int rb = 0; rb < 2; ++rb) {
                 float e[8];
-                load_a8(a, xr[rb], pwr[rb], kb, e);
+                load_a(rb, s, kb, e);
                 if constexpr (P == 3) {
                     // hi / mid / lo by truncation: each part is the top 16
                     // bits of the remainder (exact in BF16, same sign), the
@@ -707,7 +834,7 @@ bool stage_fusable(const jh_filter_plan* p, const jh_bp_plan* bp, int dtype)
 // One launch of the filter over views view0 .. view0+nviews-1 of the scan:
 // FP32 rows to d_out, or (d_q != NULL) fused staging into column strips.
 void run_filter(jh_filter_plan* p, const float* d_in, float* d_out, int view0, int nviews, jh_stream_t stream,
-                void* d_q, const float* uw, int qtype)
+                void* d_q, const float* uw, int qtype, int tiled = 0)
 {
     require(view0 >= 0 && nviews >= 0 && view0 + nviews <= p->N, "jh_filter_run: view range outside the scan");
     if (nviews == 0) return;
@@ -728,10 +855,11 @@ void run_filter(jh_filter_plan* p, const float* d_in, float* d_out, int view0, i
     a.q = d_q;
     a.uw = uw;
     a.qtype = qtype;
+    a.tiled = tiled;
     require((size_t)p->V * nviews < (1u << 31), "jh_filter_run: too many rows");
     hipStream_t st = as_stream(stream);
     if (p->mode == JH_FILTER_FP32_VALU) {
-        require(d_q == nullptr, "jh_filter_run: the vector-ALU filter does not stage");
+        require(d_q == nullptr && !tiled, "jh_filter_run: the vector-ALU filter does not stage");
         dim3 grid((p->Wout + 63) / 64, (a.rows + 63) / 64);
         require(grid.y <= 65535u, "jh_filter_run: too many rows for one launch");
         filter_fp32_kernel<<<grid, 256, 0, st>>>(a);
@@ -744,6 +872,13 @@ void run_filter(jh_filter_plan* p, const float* d_in, float* d_out, int view0, i
         else filter_mfma_kernel<kModeX6><<<grid, 256, 0, st>>>(a);
         check_launch("filter_mfma_kernel");
     }
+}
+
+// Bytes of the pre-weighted A operand of nviews views: 16-row blocks x Kp.
+size_t operand_bytes(const jh_filter_plan* p, int nviews)
+{
+    const size_t rows = (size_t)p->V * nviews;
+    return (rows + 15) / 16 * 16 * (size_t)p->Kp * sizeof(float);
 }
 
 }  // namespace
@@ -827,6 +962,68 @@ jh_status jh_filter_run_staged(jh_filter_plan* p, jh_bp_plan* bp, const float* d
                 "jh_filter_run_staged: the plans do not fuse (MFMA filter writing the backprojector's W columns, "
                 "BF16/FP16 column strips, no ZFilter, same device, frame size and view count)");
         run_filter(p, d_in, nullptr, view0, nviews, stream, d_q, bp->uw.as<float>() + (size_t)view0 * bp->W, dtype);
+    });
+}
+
+jh_status jh_filter_operand_bytes(const jh_filter_plan* p, int nviews, size_t* bytes)
+{
+    return guard([&] {
+        require(p != nullptr && bytes != nullptr && nviews >= 0, "jh_filter_operand_bytes: bad arguments");
+        require(p->mode != JH_FILTER_FP32_VALU, "jh_filter_operand_bytes: the vector-ALU filter takes no operand");
+        *bytes = operand_bytes(p, nviews);
+    });
+}
+
+jh_status jh_pre_run_operand(jh_pre_plan* pre, jh_filter_plan* p, const uint16_t* d_raw, const uint16_t* d_dark,
+                             const float* d_gain, int view0, int nviews, float* d_a, jh_stream_t stream)
+{
+    return guard([&] {
+        require(pre != nullptr && p != nullptr && d_raw != nullptr && d_a != nullptr,
+                "jh_pre_run_operand: bad arguments");
+        require(view0 >= 0 && nviews >= 0 && view0 + nviews <= p->N, "jh_pre_run_operand: view range outside the scan");
+        require(p->mode != JH_FILTER_FP32_VALU, "jh_pre_run_operand: the vector-ALU filter takes no operand");
+        require(pre->device == p->device, "jh_pre_run_operand: plans on different devices");
+        const int roiW = pre->inW - pre->bLeft - pre->bRight, roiH = pre->inH - pre->bUp - pre->bDown;
+        require(roiW == pre->outW && roiH == pre->outH, "jh_pre_run_operand: resizing preprocessing takes jh_pre_run");
+        require(pre->outW == p->W && pre->outH == p->V, "jh_pre_run_operand: frame size differs from the filter's input");
+        if (nviews == 0) return;
+        const size_t rows = (size_t)p->V * nviews;
+        require((rows + 15) / 16 <= 65535u, "jh_pre_run_operand: more than 1048560 rows in one call");
+        DeviceScope ds(p->device);
+        PreOpArgs a;
+        a.raw = d_raw;
+        a.dark = d_dark;
+        a.gain = d_gain;
+        a.pw = p->pw.as<float>();
+        a.blk = p->blk.as<int>();
+        a.out = reinterpret_cast<float4*>(d_a);
+        a.inW = pre->inW; a.inH = pre->inH; a.bUp = pre->bUp; a.bLeft = pre->bLeft;
+        a.W = p->W; a.V = p->V; a.rows = (int)rows; a.nsteps = p->Kp / 32; a.view0 = view0;
+        a.edge0 = p->mirror ? p->W - 3 : 0;
+        a.bf = pre->brightField; a.a0 = pre->bh[0]; a.a1 = pre->bh[1]; a.a2 = pre->bh[2];
+        a.doBH = pre->doBH ? 1 : 0;
+        // 16-byte loads need 8-count alignment of every row start (and of the frames and tables)
+        a.vec = (pre->inW % 8 == 0 && pre->bLeft % 8 == 0 && p->W % 8 == 0 &&
+                 reinterpret_cast<uintptr_t>(d_raw) % 16 == 0 &&
+                 (d_dark == nullptr || reinterpret_cast<uintptr_t>(d_dark) % 16 == 0) &&
+                 (d_gain == nullptr || reinterpret_cast<uintptr_t>(d_gain) % 16 == 0)) ? 1 : 0;
+        const dim3 grid((unsigned)((p->Kp + 127) / 128), (unsigned)((rows + 15) / 16));
+        pre_operand_kernel<<<grid, 256, 0, as_stream(stream)>>>(a);
+        check_launch("pre_operand_kernel");
+    });
+}
+
+jh_status jh_filter_run_operand_staged(jh_filter_plan* p, jh_bp_plan* bp, const float* d_a, int view0, int nviews,
+                                       int dtype, void* d_q, jh_stream_t stream)
+{
+    return guard([&] {
+        require(p != nullptr && bp != nullptr && d_a != nullptr && d_q != nullptr,
+                "jh_filter_run_operand_staged: bad arguments");
+        require(stage_fusable(p, bp, dtype),
+                "jh_filter_run_operand_staged: the plans do not fuse (MFMA filter writing the backprojector's W "
+                "columns, BF16/FP16 column strips, no ZFilter, same device, frame size and view count)");
+        run_filter(p, d_a, nullptr, view0, nviews, stream, d_q, bp->uw.as<float>() + (size_t)view0 * bp->W, dtype,
+                   1);
     });
 }
 

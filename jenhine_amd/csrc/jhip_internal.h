@@ -193,6 +193,50 @@ __device__ inline void tm_inverse_fp64(const double* P, double dso, double* inv)
         for (int c = 0; c < 4; ++c) inv[4 * r + c] = A[r][4 + c];
 }
 
+// Fused preprocessing of one pixel.  Restates
+// PreProcessingAgent/PreProcessingCore.cpp:67-86 with the steps of
+// GainCorr.cpp:10-13, NLog.cpp:9-13 and BeamHardeningCorr.cpp:17-27, in the
+// reference's float operation order (no contraction):
+//   v = raw - dark (integers, extension) ; f = max(float(v), 1)
+//   crop/resize ; f /= BrightField*gain ; p = -ln f ;
+//   BH: b = p*a2; b += a1; b *= p; b += a0; p *= b
+// Shared by the preprocessing kernel and the kernel that writes the
+// preprocessed frames straight into the filter's operand.
+__device__ inline float load_corrected(const uint16_t* raw, const uint16_t* dark, size_t idx)
+{
+    int v = (int)raw[idx];
+    if (dark) v -= (int)dark[idx];
+    float f = (float)v;
+    return f < 1.0f ? 1.0f : f;
+}
+
+// -ln(f / d) and the beam-hardening polynomial, d = BrightField * gain
+__device__ inline float pre_log_div(float f, float d, float a0, float a1, float a2, int doBH)
+{
+#pragma clang fp contract(off)
+    f = f / d;
+    float p = logf(f);
+    p = p * -1.0f;
+    if (doBH) {
+        float b = p * a2;
+        b = b + a1;
+        b = b * p;
+        b = b + a0;
+        p = p * b;
+    }
+    return p;
+}
+
+// pre_log_div with d = bf * gain[pix] (bf alone when gain is NULL)
+__device__ inline float pre_log(float f, float bf, const float* gain, size_t pix, float a0, float a1, float a2,
+                                int doBH)
+{
+#pragma clang fp contract(off)
+    float d = bf;
+    if (gain) d = d * gain[pix];
+    return pre_log_div(f, d, a0, a1, a2, doBH);
+}
+
 }  // namespace jh
 
 // Opaque plan types of the C-ABI.

@@ -52,6 +52,7 @@ class PreProcessingCore:
     def __init__(self, in_w, in_h, borders=(0, 0, 0, 0), out_w=None, out_h=None, bright_field=4000.0,
                  bh=(1.0, 0.0, 0.0), device="cuda:0"):
         bu, bd, bl, br = borders
+        self.borders = tuple(borders)
         self.out_w = out_w if out_w is not None else in_w - bl - br
         self.out_h = out_h if out_h is not None else in_h - bu - bd
         self.in_w, self.in_h = in_w, in_h
@@ -74,6 +75,25 @@ class PreProcessingCore:
         _require_cuda(dark, "dark")
         _require_cuda(gain, "gain", torch.float32)
         call("jh_pre_run", self._h, _ptr(raw), _ptr(dark), _ptr(gain), _ptr(out), n, _stream())
+        return out
+
+    def resizes(self):
+        bu, bd, bl, br = self.borders
+        return (self.in_w - bl - br, self.in_h - bu - bd) != (self.out_w, self.out_h)
+
+    def process_operand(self, raw, filt, dark=None, gain=None, view0=0, out=None):
+        """Preprocessing written straight into `filt`'s pre-weighted MFMA
+        operand (jh_pre_run_operand): feed it to FilterCore.process_operand_staged."""
+        _require_cuda(raw, "raw")
+        if raw.element_size() != 2:
+            raise ValueError("raw must hold 16-bit counts")
+        n = raw.shape[0]
+        if out is None:
+            out = torch.empty(filt.operand_bytes(n) // 4, dtype=torch.float32, device=raw.device)
+        _require_cuda(dark, "dark")
+        _require_cuda(gain, "gain", torch.float32)
+        call("jh_pre_run_operand", self._h, filt._h, _ptr(raw), _ptr(dark), _ptr(gain), view0, n, _ptr(out),
+             _stream())
         return out
 
     def __del__(self):
@@ -101,6 +121,7 @@ class FilterCore:
         self.y_mirroring = bool(y_mirroring)
         ptm = np.ascontiguousarray(ptm, np.float64).reshape(-1, 12)
         self.n_views = ptm.shape[0]
+        self.mode = mode
         m = {"fp32": _lib.JH_FILTER_FP32, "mfma": _lib.JH_FILTER_MFMA_BF16X3, "mfma6": _lib.JH_FILTER_MFMA_BF16X6,
              "fp32valu": _lib.JH_FILTER_FP32_VALU}[mode]
         n = len(adj_pts)
@@ -140,6 +161,23 @@ class FilterCore:
         ok = ctypes.c_int()
         call("jh_filter_stage_fusable", self._h, bp._h, DTYPES[dtype], ctypes.byref(ok))
         return bool(ok.value)
+
+    def operand_bytes(self, n_views):
+        b = ctypes.c_size_t()
+        call("jh_filter_operand_bytes", self._h, n_views, ctypes.byref(b))
+        return b.value
+
+    def process_operand_staged(self, operand, n_views, bp, dtype, view0=0, out=None):
+        """process_staged() from the pre-weighted operand of
+        PreProcessingCore.process_operand."""
+        _require_cuda(operand, "operand", torch.float32)
+        if operand.numel() * 4 < self.operand_bytes(n_views):
+            raise ValueError("operand is smaller than operand_bytes(n_views)")
+        if out is None:
+            out = torch.empty(bp.stage_bytes(dtype, n_views), dtype=torch.uint8, device=operand.device)
+        call("jh_filter_run_operand_staged", self._h, bp._h, _ptr(operand), view0, n_views, DTYPES[dtype], _ptr(out),
+             _stream())
+        return out
 
     def process_staged(self, frames, bp, dtype, view0=0, out=None):
         """process() followed by bp.stage() in one pass (bit-identical); the
@@ -438,7 +476,17 @@ class FDK:
         staging in one kernel, FilterCore.process_staged)."""
         return self.filter.stage_fusable(self.bp, self.storage)
 
+    def operand_path(self):
+        """Whether preprocessing writes the filter's pre-weighted operand
+        (fused chain without composite or resizing): pre -> filter+stage -> bp."""
+        return (self.composite is None and not self.pre.resizes() and self.filter.mode != "fp32valu"
+                and self.fused())
+
     def __call__(self, raw, dark=None, gain=None, return_stages=False):
+        if not return_stages and self.operand_path() and raw.shape[0] * self.filter.V <= 1048560:
+            a = self.pre.process_operand(raw, self.filter, dark, gain)
+            q = self.filter.process_operand_staged(a, raw.shape[0], self.bp, self.storage)
+            return self.bp.run(q, self.storage)
         p = self.pre.process(raw, dark, gain)
         c = self.composite.process(p) if self.composite is not None else p
         if not return_stages and self.fused():

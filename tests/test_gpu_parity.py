@@ -722,3 +722,62 @@ def test_filter_run_staged_refuses_plans_that_do_not_fuse(gpu):
     x = torch.zeros((w.n_views, w.n_v, w.n_u), dtype=torch.float32, device=gpu)
     with pytest.raises(RuntimeError, match="do not fuse"):
         mfma.process_staged(x, bp, "bf16")
+
+
+@pytest.mark.parametrize("dtype", ["bf16", "f16"])
+def test_pre_operand_path_equals_two_kernel_chain(gpu, dtype):
+    """jh_pre_run_operand + jh_filter_run_operand_staged against jh_pre_run +
+    jh_filter_run_staged, to the bit: a frame width that is not a multiple of
+    the 32-wide k step, a row count that is not a multiple of the 16-row
+    block, dark and gain correction and beam hardening, a view range other
+    than the first; and the operand itself against its definition (pixel
+    times pre-weight, the three unweighted edge columns, zeros beyond, in the
+    MFMA tile order)."""
+    from jenhine_amd.recon import BPCore, FilterCore, PreProcessingCore
+    w = SMALL.scaled(n_views=5, n_u=200, n_v=70, name="ragged-operand")
+    N, V, W = w.n_views, w.n_v, w.n_u
+    d = w.data()
+    ptm = w.ptms()
+    scan = w.scan_params()
+    pre = PreProcessingCore(W, V, bright_field=4000.0, bh=(1.0, 0.03, 0.004), device=gpu)
+    filt = FilterCore(W, V, ptm, scan.DSO, scan.HalfSampleRate, mode="mfma6", device=gpu)
+    bp = BPCore(ptm, W, V, w.X, w.Y, w.Z, w.pxy, w.pz, device=gpu)
+    raw, dark = _u16(d["raw"], gpu), _u16(d["dark"], gpu)
+    gain = torch.from_numpy(d["gain"]).to(gpu)
+    p = pre.process(raw, dark, gain)
+    ref = filt.process_staged(p, bp, dtype)
+    a = pre.process_operand(raw, filt, dark, gain)
+    got = filt.process_operand_staged(a, N, bp, dtype)
+    assert torch.equal(got, ref)
+    ref2 = filt.process_staged(p[1:4], bp, dtype, view0=2)
+    got2 = filt.process_operand_staged(pre.process_operand(raw[1:4], filt, dark, gain, view0=2), 3, bp, dtype,
+                                       view0=2)
+    assert torch.equal(got2, ref2)
+    # the operand against its definition
+    nsteps = (filt.operand_bytes(16) // 4) // (16 * V) // 32  # 16 views: V*16 rows, a multiple of 16
+    Kp = nsteps * 32
+    pn = p.cpu().numpy().reshape(N * V, W)
+    pw = np.stack([filt.preweight(v).cpu().numpy() for v in range(N)]).reshape(N * V, W)
+    rows = N * V
+    full = np.zeros(((rows + 15) // 16 * 16, Kp), np.float32)
+    full[:rows, :W] = pn * pw
+    full[:rows, W:W + 3] = pn[:, 0:3]
+    # tile order: ((r16 * nsteps + s) * 2 + h) * 64 + 16 g + i, 4 floats
+    t = full.reshape(-1, 16, nsteps, 4, 2, 4)           # r16, i, s, g, h, e
+    t = t.transpose(0, 2, 4, 3, 1, 5).reshape(-1)       # r16, s, h, g, i, e
+    np.testing.assert_array_equal(a.cpu().numpy()[: t.size], t)
+
+
+def test_fdk_operand_path_equals_stage_by_stage(gpu):
+    """The production chain (pre -> operand -> filter with fused staging ->
+    bp) gives the volume of the stage-by-stage chain to the bit."""
+    from jenhine_amd.recon import FDK
+    w = SMALL
+    d = small_data_cache()
+    sc, rc = w.scan_params(), w.recon_params()
+    fdk = FDK(sc, rc, d["ptm"], "mfma6", "bf16", gpu)
+    assert fdk.operand_path()
+    raw = _u16(d["raw"], gpu)
+    vol = fdk(raw).cpu().numpy()
+    ref, _ = fdk(raw, return_stages=True)
+    np.testing.assert_array_equal(vol, ref.cpu().numpy())
