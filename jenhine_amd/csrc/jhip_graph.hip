@@ -46,6 +46,18 @@ static void check_status(jh_status s, const char* what)
     }
 }
 
+// Whether a chunk of n views of the scan takes the operand path: filter and
+// staging fuse, and the preprocessing neither resizes nor changes the frame.
+static bool operand_path(const GraphScan& s, int n)
+{
+    int fused = 0;
+    check_status(jh_filter_stage_fusable(s.filt, s.bp, s.dtype, &fused), "fusable");
+    const jh_pre_plan* p = s.pre;
+    const int roiW = p->inW - p->bLeft - p->bRight, roiH = p->inH - p->bUp - p->bDown;
+    return fused && roiW == p->outW && roiH == p->outH && p->outW == s.bp->W && p->outH == s.bp->V &&
+           (size_t)s.bp->V * n <= 1048560u;
+}
+
 static double scan_cost(const GraphScan& s)
 {
     // voxel-updates dominate; the per-pixel chain adds a term per detector pixel
@@ -230,6 +242,11 @@ jh_status jh_graph_instantiate(jh_graph* g, int nbranches)
                     if (s.branch != b) continue;
                     const size_t frame = (size_t)s.bp->W * s.bp->V * sizeof(float);
                     pb = std::max(pb, frame * s.chunk);
+                    if (operand_path(s, s.chunk)) {  // the pre buffer holds the filter's operand
+                        size_t ob = 0;
+                        check_status(jh_filter_operand_bytes(s.filt, s.chunk, &ob), "operand bytes");
+                        pb = std::max(pb, ob);
+                    }
                     fb = std::max(fb, frame * s.chunk);
                     size_t q = 0;
                     check_status(jh_bp_stage_bytes(s.bp, s.dtype, s.chunk, &q), "stage bytes");
@@ -258,10 +275,21 @@ jh_status jh_graph_instantiate(jh_graph* g, int nbranches)
                     check_status(jh_bp_reset(s.bp, st), "reset");
                     for (int v0 = 0; v0 < s.bp->N; v0 += s.chunk) {
                         const int n = std::min(s.chunk, s.bp->N - v0);
-                        check_status(jh_pre_run(s.pre, s.raw + inFrame * v0, s.dark, s.gain, br.pre.as<float>(), n, st),
-                                     "preprocess");
                         int fused = 0;
                         check_status(jh_filter_stage_fusable(s.filt, s.bp, s.dtype, &fused), "fusable");
+                        if (operand_path(s, n)) {
+                            // preprocessing into the filter's pre-weighted operand, then filter and staging
+                            check_status(jh_pre_run_operand(s.pre, s.filt, s.raw + inFrame * v0, s.dark, s.gain, v0, n,
+                                                            br.pre.as<float>(), st),
+                                         "preprocess into the operand");
+                            check_status(jh_filter_run_operand_staged(s.filt, s.bp, br.pre.as<float>(), v0, n, s.dtype,
+                                                                      br.q.get(), st),
+                                         "filter and stage");
+                            check_status(jh_bp_accumulate(s.bp, br.q.get(), s.dtype, v0, n, st), "accumulate");
+                            continue;
+                        }
+                        check_status(jh_pre_run(s.pre, s.raw + inFrame * v0, s.dark, s.gain, br.pre.as<float>(), n, st),
+                                     "preprocess");
                         if (fused) {  // filter and staging in one kernel
                             check_status(jh_filter_run_staged(s.filt, s.bp, br.pre.as<float>(), v0, n, s.dtype,
                                                               br.q.get(), st),
