@@ -72,7 +72,8 @@ def main():
                     help="projection storage of the batch16 scans")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+        print(json.dumps({"error": "the benchmark measures one GPU (the path shards by z-slabs, one process "
+                                   "per GPU, not built here); --gpus must be 1",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 

@@ -8,7 +8,8 @@ Tolerances:
     (each operand keeps ~17 significant bits, and the ramp filter's
     cancellation, sum|terms| / |result| ~ 20, amplifies that);
   * backprojection FP32: rel-RMSE <= 1e-4 vs the FP32-ref oracle volume (the
-    north-star bar) -- observed ~1e-6; BF16 / FP16 storage: <= 1e-3 vs FP64.
+    FP32 path's accuracy bar) -- observed ~1e-6; BF16 / FP16 storage: <= 1e-3
+    vs FP64.
 """
 import numpy as np
 import pytest
