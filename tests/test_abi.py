@@ -60,3 +60,20 @@ def test_detector_side_and_argument_checks_without_device():
                                ctypes.c_float(300.0), ctypes.c_float(1.0), ctypes.c_float(0), ctypes.c_float(0),
                                None, None, 0, 0, 0)
     assert st == _lib.JH_ERR_INVALID and b"FFT length" in L.jh_last_error()
+
+
+def test_fused_path_entry_points_reject_bad_arguments_without_device():
+    L = _lib.lib()
+    ok = ctypes.c_int(7)
+    # no plans: not fusable, and the runs refuse before any device work
+    assert L.jh_filter_stage_fusable(None, None, _lib.JH_BF16, ctypes.byref(ok)) == _lib.JH_OK
+    assert ok.value == 0
+    assert L.jh_filter_stage_fusable(None, None, _lib.JH_BF16, None) == _lib.JH_ERR_INVALID
+    st = L.jh_filter_run_staged(None, None, None, 0, 1, _lib.JH_BF16, None, None)
+    assert st == _lib.JH_ERR_INVALID and b"bad arguments" in L.jh_last_error()
+    st = L.jh_pre_run_operand(None, None, None, None, None, 0, 1, None, None)
+    assert st == _lib.JH_ERR_INVALID and b"bad arguments" in L.jh_last_error()
+    st = L.jh_filter_run_operand_staged(None, None, None, 0, 1, _lib.JH_BF16, None, None)
+    assert st == _lib.JH_ERR_INVALID and b"bad arguments" in L.jh_last_error()
+    b = ctypes.c_size_t()
+    assert L.jh_filter_operand_bytes(None, 1, ctypes.byref(b)) == _lib.JH_ERR_INVALID
