@@ -396,6 +396,9 @@ __device__ void stage_strips_epilogue(const FiltArgs& a, const f32x4_t (&acc)[2]
     const int vt = T0 / a.V, vrow0 = T0 - vt * a.V;  // view and row (in its view) of the tile's first row
     uint16_t* t16 = reinterpret_cast<uint16_t*>(lds);
     __syncthreads();  // every wave is done with the B tile
+    // the lane's 8 columns' uW, reloaded when its row enters another view
+    int cview = -1;
+    float uwc[8];
 #pragma unroll
     for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
@@ -403,12 +406,21 @@ __device__ void stage_strips_epilogue(const FiltArgs& a, const f32x4_t (&acc)[2]
             const int rl = wave * 32 + rb * 16 + 4 * (lane >> 4) + q, row = T0 + rl;
             int view = vt, vrow = vrow0 + rl;
             while (vrow >= a.V) { vrow -= a.V; ++view; }
-            const float* uwv = a.uw + (size_t)view * a.Wout;
+            const bool live = row < a.rows;
+            if (live && view != cview) {
+                cview = view;
+                const float* uwv = a.uw + (size_t)view * a.Wout;
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    const int col = C0 + c * 16 + (lane & 15);
+                    uwc[c] = col < a.Wout ? uwv[col] : 0.0f;
+                }
+            }
 #pragma unroll
             for (int c = 0; c < 8; ++c) {
-                const int cl = c * 16 + (lane & 15), col = C0 + cl;
+                const int cl = c * 16 + (lane & 15);
                 uint32_t h = 0;
-                if (row < a.rows && col < a.Wout) h = round16(acc[rb][c][q] * uwv[col], a.qtype);
+                if (live && C0 + cl < a.Wout) h = round16(acc[rb][c][q] * uwc[c], a.qtype);
                 t16[rl * kStagePitch + cl] = (uint16_t)h;
             }
         }
@@ -429,7 +441,19 @@ __device__ void stage_strips_epilogue(const FiltArgs& a, const f32x4_t (&acc)[2]
     // window of the staged values of rows vl .. vl+3 (0 where not this tile's)
     uint32_t w1 = t16[vl * kStagePitch + cl], w2 = t16[(vl + 1) * kStagePitch + cl];
     uint32_t w3 = t16[(vl + 2) * kStagePitch + cl];
-    for (int k = 0; k < 64; ++k, ++vl) {
+    // strips whose four rows lie in this tile, in the view of the run's first
+    // row and in the launch: whole 8-byte pixels, no per-slot decisions
+    const int R0 = T0 + vl;
+    const int nfast = max(0, min(min(64, 125 - vl), min(vend - 3 - R0, a.rows - 3 - R0)));
+    int k = 0;
+    for (; k < nfast; ++k, ++vl) {
+        const uint32_t w4 = t16[(vl + 3) * kStagePitch + cl];
+        char* px = q + (size_t)(T0 + vl) * rowpx * 8;
+        *reinterpret_cast<uint2*>(px) = make_uint2(w1 | (w2 << 16), w3 | (w4 << 16));
+        if (padcol) *reinterpret_cast<uint2*>(px + 8) = make_uint2(0u, 0u);
+        w1 = w2; w2 = w3; w3 = w4;
+    }
+    for (; k < 64; ++k, ++vl) {
         const int R = T0 + vl;
         if (R >= a.rows) break;
         if (R >= vend) vend += a.V;
