@@ -65,6 +65,21 @@ def test_preprocess_dark_gain_bh_borders(gpu, small_data):
     assert ulp_diff_f32(out, ref).max() <= 4
 
 
+def test_preprocess_aligned_rows_dark_gain_bh(gpu, small_data):
+    """Rows aligned to 8 pixels take the 16-byte-load kernel (dark, gain,
+    beam hardening): same bar against the oracle as the per-pixel kernel."""
+    from jenhine_amd.recon import PreProcessingCore
+    d = small_data
+    raw = d["raw"][:6]
+    bh = (1.0, 0.03, 0.004)
+    H, W = raw.shape[1:]
+    assert W % 8 == 0
+    pre = PreProcessingCore(W, H, bright_field=4000.0, bh=bh, device=gpu)
+    out = pre.process(_u16(raw, gpu), _u16(d["dark"], gpu), torch.from_numpy(d["gain"]).to(gpu)).cpu().numpy()
+    ref = o.preprocess(raw, bright_field=4000.0, bh=bh, dark=d["dark"], gain=d["gain"])
+    assert ulp_diff_f32(out, ref).max() <= 4
+
+
 def test_preprocess_dark_below_raw_clamps(gpu):
     from jenhine_amd.recon import PreProcessingCore
     raw = np.full((1, 8, 16), 100, np.uint16)
@@ -726,7 +741,8 @@ def test_filter_run_staged_refuses_plans_that_do_not_fuse(gpu):
 
 
 @pytest.mark.parametrize("dtype", ["bf16", "f16"])
-def test_pre_operand_path_equals_two_kernel_chain(gpu, dtype):
+@pytest.mark.parametrize("width", [200, 196])  # 16-byte loads, and the per-pixel path (rows not 8-aligned)
+def test_pre_operand_path_equals_two_kernel_chain(gpu, dtype, width):
     """jh_pre_run_operand + jh_filter_run_operand_staged against jh_pre_run +
     jh_filter_run_staged, to the bit: a frame width that is not a multiple of
     the 32-wide k step, a row count that is not a multiple of the 16-row
@@ -735,7 +751,7 @@ def test_pre_operand_path_equals_two_kernel_chain(gpu, dtype):
     times pre-weight, the three unweighted edge columns, zeros beyond, in the
     MFMA tile order)."""
     from jenhine_amd.recon import BPCore, FilterCore, PreProcessingCore
-    w = SMALL.scaled(n_views=5, n_u=200, n_v=70, name="ragged-operand")
+    w = SMALL.scaled(n_views=5, n_u=width, n_v=70, name="ragged-operand")
     N, V, W = w.n_views, w.n_v, w.n_u
     d = w.data()
     ptm = w.ptms()
