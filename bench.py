@@ -66,6 +66,9 @@ def main():
                     help="storage form of the --kernel-only backprojection")
     ap.add_argument("--kernel-tile", default=None,
                     help="zc,ty tile of the --kernel-only backprojection (default: the autotuner's choice)")
+    ap.add_argument("--kernel-filter", action="store_true",
+                    help="with --kernel-only: time the headline chain's filter (pre-weighted operand in, "
+                         "staged strips out) instead of the backprojection")
     ap.add_argument("--workload", choices=("dental512", "batch16", "offset_dental512"), default="dental512")
     ap.add_argument("--branches", type=int, default=4, help="parallel graph branches (batch16)")
     ap.add_argument("--batch-storage", choices=("bf16", "f16", "f32"), default="bf16",
@@ -109,6 +112,21 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         fdk = forms[args.kernel_form]
+        if args.kernel_filter:
+            if not fdk.operand_path():
+                raise SystemExit("--kernel-filter needs a form whose filter fuses (bf16 or f16)")
+            n = raw.shape[0]
+            a = fdk.pre.process_operand(raw, fdk.filter, dark, gain)
+            q = fdk.filter.process_operand_staged(a, n, fdk.bp, fdk.storage)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            for _ in range(args.steps):
+                fdk.filter.process_operand_staged(a, n, fdk.bp, fdk.storage, out=q)
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t) / args.steps * 1e3
+            log(f"filter_mfma_kernel (operand in, {fdk.storage} strips out): {ms:.2f} ms/launch")
+            print(json.dumps({"kernel_only": True, "filter_ms": ms, "copy_bytes": 2 * src.numel() * 4}))
+            return
         q = fdk.bp.stage(fdk.filter.process(fdk.pre.process(raw, dark, gain)), fdk.storage)
         if args.kernel_tile:
             fdk.bp.set_tile(*(int(x) for x in args.kernel_tile.split(",")))
