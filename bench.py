@@ -334,13 +334,21 @@ def run_offset(args, dev):
         torch.cuda.synchronize()
         step = (time.perf_counter() - t) / args.steps
         fused = fdk.fused()
+        operand = fdk.composite_operand_path()
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
         ev[0].record()
         p = fdk.pre.process(raw, dark, gain)
         ev[1].record()
-        c = fdk.composite.process(p)
+        if operand:
+            c = fdk.composite.process_operand(p, fdk.filter)
+        else:
+            c = fdk.composite.process(p)
         ev[2].record()
-        if fused:
+        if operand:
+            f = None
+            q = fdk.filter.process_operand_staged(c, w.n_views, fdk.bp, storage)
+            ev[3].record()
+        elif fused:
             f = None
             q = fdk.filter.process_staged(c, fdk.bp, storage)
             ev[3].record()
@@ -355,7 +363,9 @@ def run_offset(args, dev):
         stt = [ev[i].elapsed_time(ev[i + 1]) for i in range(5)]
         vols[name] = vol.cpu().numpy()
         results[name] = dict(step_ms=step * 1e3, gups=units / step / 1e9,
-                             stage_ms=(dict(pre=stt[0], composite=stt[1], filter_stage=stt[2], bp=stt[4]) if fused
+                             stage_ms=(dict(pre=stt[0], composite_operand=stt[1], filter_stage=stt[2], bp=stt[4])
+                                       if operand else
+                                       dict(pre=stt[0], composite=stt[1], filter_stage=stt[2], bp=stt[4]) if fused
                                        else dict(pre=stt[0], composite=stt[1], filter=stt[2], stage=stt[3],
                                                  bp=stt[4])))
         log(name, json.dumps(results[name]))

@@ -217,6 +217,19 @@ jh_status jh_composite_side(const jh_composite_plan* plan, int* detectorOnRight)
 jh_status jh_composite_run(jh_composite_plan* plan, const float* d_dirc, const float* d_conj, long long inStride,
                            float* d_out, int view0, int nviews, jh_stream_t stream);
 
+/* The composite (as jh_composite_run) written straight into the MFMA filter's
+ * pre-weighted operand, the way jh_pre_run_operand writes preprocessing:
+ * views view0 .. view0 + nviews - 1 land at operand rows
+ * (view0 - aview0) * V .. of d_a, an operand of the filter launch that starts
+ * at view aview0, so the two halves of a scan (different conjugate buffers)
+ * fill one operand with two calls.  The filter takes frames of 2W columns
+ * (plan->W == 2 * composite W, same V and view count, same device, an MFMA
+ * mode).  jh_filter_run_operand_staged(plan, bp, d_a, aview0, ...) then gives
+ * bit-identically what jh_composite_run, jh_filter_run_staged give. */
+jh_status jh_composite_run_operand(jh_composite_plan* plan, jh_filter_plan* filter, const float* d_dirc,
+                                   const float* d_conj, long long inStride, int view0, int nviews, float* d_a,
+                                   int aview0, jh_stream_t stream);
+
 jh_status jh_composite_destroy(jh_composite_plan* plan);
 
 /* ---------------------------------------------------------------------- */

@@ -26,7 +26,8 @@ EXPORTS = [
     "jh_filter_create", "jh_filter_create_ex", "jh_filter_shape", "jh_filter_response", "jh_filter_preweight",
     "jh_filter_run", "jh_filter_stage_fusable", "jh_filter_run_staged", "jh_filter_operand_bytes",
     "jh_pre_run_operand", "jh_filter_run_operand_staged", "jh_filter_destroy",
-    "jh_detector_on_right", "jh_composite_create", "jh_composite_side", "jh_composite_run", "jh_composite_destroy",
+    "jh_detector_on_right", "jh_composite_create", "jh_composite_side", "jh_composite_run", "jh_composite_run_operand",
+    "jh_composite_destroy",
     "jh_bp_create", "jh_bp_uweights", "jh_bp_stage_bytes", "jh_bp_stage", "jh_bp_run", "jh_bp_reset",
     "jh_bp_accumulate", "jh_bp_finalize", "jh_bp_set_tile", "jh_bp_get_tile", "jh_bp_geometry_mode",
     "jh_bp_destroy", "jh_bp_pack", "jh_bp_backproject", "jh_fp_project", "jh_bp_autotune",
@@ -80,6 +81,7 @@ _SIGS = {
     "jh_composite_create": (c_int, [P, c_int, c_int, c_int, c_int, P, c_float, c_int]),
     "jh_composite_side": (c_int, [P, P]),
     "jh_composite_run": (c_int, [P, P, P, ctypes.c_longlong, P, c_int, c_int, P]),
+    "jh_composite_run_operand": (c_int, [P, P, P, P, ctypes.c_longlong, c_int, c_int, P, c_int, P]),
     "jh_composite_destroy": (c_int, [P]),
     "jh_bp_create": (c_int, [P, c_int, P, c_int, c_int, c_int, c_int, c_int, c_int, c_float, c_float]),
     "jh_bp_uweights": (c_int, [P, P]),

@@ -23,6 +23,7 @@
 #include <cmath>
 #include <vector>
 
+#include "jhip_filter_plan.h"
 #include "jhip_internal.h"
 
 struct jh_composite_plan {
@@ -87,6 +88,51 @@ __device__ __forceinline__ bool remap_at(const float* __restrict__ img, int W, i
     return true;
 }
 
+// Pixel (r, c) of frame f's conjugate half (output column conjOff + c).
+__device__ __forceinline__ float conj_value(const CompArgs& a, int f, int r, int c)
+{
+    const int W = a.W, V = a.V;
+    const float* dirc = a.dirc + (size_t)f * a.inStride;
+    const float* conj = a.conj + (size_t)f * a.inStride;
+    const float* g = a.G + 9 * (size_t)(a.view0 + f);
+    const float u1 = a.detRight ? (float)c - (float)W : (float)(c + W);
+    const float v1 = (float)r;
+    const float U = fmaf(g[0], u1, fmaf(g[1], v1, g[2]));
+    const float Vv = fmaf(g[3], u1, fmaf(g[4], v1, g[5]));
+    const float S = fmaf(g[6], u1, fmaf(g[7], v1, g[8]));
+    const float u2 = U / S;
+    float v2 = Vv / S;
+    if (v2 < 0.0f) v2 = 0.0f;
+    if (v2 > (float)(V - 1)) v2 = (float)(V - 1);
+    float val = 0.0f;
+    remap_at(conj, W, V, u2, v2, val);
+    const int conjOff = a.detRight ? 0 : W;
+    const int transOff = a.detRight ? W - kTransWidth : W;
+    const int oc = conjOff + c;
+    const float* drow = dirc + (size_t)r * W;
+    if (oc >= transOff && oc < transOff + kTransWidth) {
+        const int i = oc - transOff;
+        const float* e = a.detRight ? drow : drow + W - kEdgeWidth;
+        float sum = e[0];
+#pragma unroll
+        for (int k = 1; k < kEdgeWidth; ++k) sum = sum + e[k];
+        const float edge = sum / (float)kEdgeWidth;
+        const float x = val * a.mc[i];
+        const float y = edge * a.md[i];
+        val = x + y;
+    }
+    return val;
+}
+
+// Output column oc (0 .. 2W-1) of row r of composite frame f.
+__device__ __forceinline__ float composite_value(const CompArgs& a, int f, int r, int oc)
+{
+    const int W = a.W;
+    const int dOff = a.detRight ? W : 0;
+    if (oc >= dOff && oc < dOff + W) return a.dirc[(size_t)f * a.inStride + (size_t)r * W + (oc - dOff)];
+    return conj_value(a, f, r, oc - (a.detRight ? 0 : W));
+}
+
 // One thread per pixel of the conjugate half; the direct half is copied by
 // the same thread (column c of the direct frame).
 __global__ void __launch_bounds__(256) composite_kernel(CompArgs a)
@@ -97,39 +143,66 @@ __global__ void __launch_bounds__(256) composite_kernel(CompArgs a)
         const int f = (int)(t / ((size_t)V * W));
         const int rem = (int)(t - (size_t)f * V * W);
         const int r = rem / W, c = rem - r * W;
-        const float* dirc = a.dirc + (size_t)f * a.inStride;
-        const float* conj = a.conj + (size_t)f * a.inStride;
         float* o = a.out + ((size_t)f * V + r) * W2;
-        const float* g = a.G + 9 * (size_t)(a.view0 + f);
-        const float u1 = a.detRight ? (float)c - (float)W : (float)(c + W);
-        const float v1 = (float)r;
-        const float U = fmaf(g[0], u1, fmaf(g[1], v1, g[2]));
-        const float Vv = fmaf(g[3], u1, fmaf(g[4], v1, g[5]));
-        const float S = fmaf(g[6], u1, fmaf(g[7], v1, g[8]));
-        const float u2 = U / S;
-        float v2 = Vv / S;
-        if (v2 < 0.0f) v2 = 0.0f;
-        if (v2 > (float)(V - 1)) v2 = (float)(V - 1);
-        float val = 0.0f;
-        remap_at(conj, W, V, u2, v2, val);
-        const int conjOff = a.detRight ? 0 : W;
-        const int transOff = a.detRight ? W - kTransWidth : W;
-        const int oc = conjOff + c;
-        const float* drow = dirc + (size_t)r * W;
-        if (oc >= transOff && oc < transOff + kTransWidth) {
-            const int i = oc - transOff;
-            const float* e = a.detRight ? drow : drow + W - kEdgeWidth;
-            float sum = e[0];
-#pragma unroll
-            for (int k = 1; k < kEdgeWidth; ++k) sum = sum + e[k];
-            const float edge = sum / (float)kEdgeWidth;
-            const float x = val * a.mc[i];
-            const float y = edge * a.md[i];
-            val = x + y;
-        }
-        o[oc] = val;
-        o[(a.detRight ? W : 0) + c] = drow[c];
+        o[(a.detRight ? 0 : W) + c] = conj_value(a, f, r, c);
+        o[(a.detRight ? W : 0) + c] = a.dirc[(size_t)f * a.inStride + (size_t)r * W + c];
     }
+}
+
+// The composite written straight into the filter's pre-weighted operand (the
+// layout of pre_operand_kernel, for a filter whose input is the 2W-wide
+// composite): element (R, k) of operand row R = (view0 - aview0 + f) V + r is
+// the composite pixel times the pre-weight for k < 2W, the unweighted
+// composite pixel edge0 + k - 2W for the three edge columns, 0 beyond.  The
+// launch writes the rows of its views only (each float4 of the operand holds
+// one row), so the two halves of a scan can be written by two launches.
+// Workgroup: 16 operand rows x 128 k, through LDS as in pre_operand_kernel.
+struct CompOpArgs {
+    CompArgs c;
+    float4* out;
+    const float* pw;  // the filter's pre-weight maps, nblocks x V x 2W
+    const int* blk;   // block of every view of the scan
+    int R0, R1;       // operand rows of this launch's views
+    int blockRow0;    // first 16-row block of the launch
+    int nsteps, aview0, edge0;
+};
+
+constexpr int kCompOpPitch = 132;
+
+__global__ void __launch_bounds__(256) composite_operand_kernel(CompOpArgs a)
+{
+    __shared__ float tile[16 * kCompOpPitch];
+    const int W2 = 2 * a.c.W, V = a.c.V;
+    const int t = threadIdx.x;
+    const int rb = a.blockRow0 + blockIdx.y;
+    // compute phase: consecutive lanes take consecutive columns of one row
+    // (coalesced frame reads); thread t holds rows t / 128 + 2 j, j < 8
+    const int kc = t & 127, k = blockIdx.x * 128 + kc;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int i = (t >> 7) + 2 * j;
+        const int R = rb * 16 + i;
+        float e = 0.0f;
+        if (R >= a.R0 && R < a.R1 && k < W2 + 3) {
+            const int rel = R - a.R0;
+            const int f = rel / V, r = rel - f * V;
+            const int view = a.R0 / V + f;  // view of the filter launch (operand rows start at view aview0)
+            if (k < W2)
+                e = composite_value(a.c, f, r, k) * a.pw[((size_t)a.blk[a.aview0 + view] * V + r) * W2 + k];
+            else
+                e = composite_value(a.c, f, r, a.edge0 + k - W2);
+        }
+        tile[i * kCompOpPitch + kc] = e;
+    }
+    __syncthreads();
+    const int sl = t >> 6, l = t & 63;
+    const int s = blockIdx.x * 4 + sl;
+    const int Rl = rb * 16 + (l & 15);
+    if (s >= a.nsteps || Rl < a.R0 || Rl >= a.R1) return;
+    const float* src = tile + (l & 15) * kCompOpPitch + 32 * sl + 8 * (l >> 4);
+    float4* dst = a.out + ((size_t)rb * a.nsteps + s) * 128 + l;
+    dst[0] = make_float4(src[0], src[1], src[2], src[3]);
+    dst[64] = make_float4(src[4], src[5], src[6], src[7]);
 }
 #pragma clang fp contract(on)
 
@@ -227,6 +300,51 @@ jh_status jh_composite_run(jh_composite_plan* p, const float* d_dirc, const floa
         const size_t total = (size_t)nviews * p->V * p->W;
         composite_kernel<<<grid_1d(total), 256, 0, as_stream(stream)>>>(a);
         check_launch("composite_kernel");
+    });
+}
+
+jh_status jh_composite_run_operand(jh_composite_plan* p, jh_filter_plan* fp, const float* d_dirc, const float* d_conj,
+                                   long long inStride, int view0, int nviews, float* d_a, int aview0,
+                                   jh_stream_t stream)
+{
+    return guard([&] {
+        require(p != nullptr && fp != nullptr && d_dirc != nullptr && d_conj != nullptr && d_a != nullptr,
+                "jh_composite_run_operand: bad arguments");
+        require(view0 >= 0 && nviews >= 0 && view0 + nviews <= p->N,
+                "jh_composite_run_operand: view range outside the scan");
+        require(aview0 >= 0 && aview0 <= view0, "jh_composite_run_operand: the operand starts after the views");
+        require(nviews <= 1 || inStride >= (long long)p->W * p->V, "jh_composite_run_operand: input frames overlap");
+        require(fp->mode != JH_FILTER_FP32_VALU, "jh_composite_run_operand: the vector-ALU filter takes no operand");
+        require(fp->W == 2 * p->W && fp->V == p->V && fp->N == p->N && fp->device == p->device,
+                "jh_composite_run_operand: the filter does not take this composite (2W columns, same views)");
+        if (nviews == 0) return;
+        const size_t R0 = (size_t)(view0 - aview0) * p->V, R1 = R0 + (size_t)nviews * p->V;
+        require(R1 < (1u << 31), "jh_composite_run_operand: too many rows");
+        const size_t b0 = R0 / 16, b1 = (R1 - 1) / 16;
+        require(b1 - b0 + 1 <= 65535u, "jh_composite_run_operand: more than 65535 row blocks in one call");
+        DeviceScope ds(p->device);
+        CompOpArgs a;
+        a.c.dirc = d_dirc;
+        a.c.conj = d_conj;
+        a.c.inStride = inStride;
+        a.c.out = nullptr;
+        a.c.G = p->G.as<float>();
+        a.c.W = p->W; a.c.V = p->V;
+        a.c.view0 = view0;
+        a.c.nviews = nviews;
+        a.c.detRight = p->detRight;
+        for (int i = 0; i < kTransWidth; ++i) { a.c.md[i] = p->maskDirc[i]; a.c.mc[i] = p->maskConj[i]; }
+        a.out = reinterpret_cast<float4*>(d_a);
+        a.pw = fp->pw.as<float>();
+        a.blk = fp->blk.as<int>();
+        a.R0 = (int)R0; a.R1 = (int)R1;
+        a.blockRow0 = (int)b0;
+        a.nsteps = fp->Kp / 32;
+        a.aview0 = aview0;
+        a.edge0 = fp->mirror ? fp->W - 3 : 0;
+        const dim3 grid((unsigned)((fp->Kp + 127) / 128), (unsigned)(b1 - b0 + 1));
+        composite_operand_kernel<<<grid, 256, 0, as_stream(stream)>>>(a);
+        check_launch("composite_operand_kernel");
     });
 }
 

@@ -30,21 +30,9 @@
 #include <cstring>
 
 #include "jhip_bp_plan.h"
+#include "jhip_filter_plan.h"
 #include "jhip_internal.h"
 
-struct jh_filter_plan {
-    int device;
-    int W, V, N, L, K, Kp;  // W: input width
-    int Wout, mirror;       // output width, yMirroring
-    int mode;
-    int nblocks;
-    std::vector<int> view_block;  // block index of every view
-    std::vector<double> H;        // L/2 + 1
-    jh::DevBuf B;                 // Kp x Wout float (vector-ALU form)
-    jh::DevBuf Bhl;               // MFMA forms: packed FP32 or BF16-part operand
-    jh::DevBuf pw;                // nblocks x V x W float
-    jh::DevBuf blk;               // N int: block of every view (device copy)
-};
 
 namespace jh {
 

@@ -1,0 +1,21 @@
+// Row-filter plan shared by the filter and the kernels that write its
+// pre-weighted operand.
+#pragma once
+
+#include <vector>
+
+#include "jhip_internal.h"
+
+struct jh_filter_plan {
+    int device;
+    int W, V, N, L, K, Kp;  // W: input width
+    int Wout, mirror;       // output width, yMirroring
+    int mode;
+    int nblocks;
+    std::vector<int> view_block;  // block index of every view
+    std::vector<double> H;        // L/2 + 1
+    jh::DevBuf B;                 // Kp x Wout float (vector-ALU form)
+    jh::DevBuf Bhl;               // MFMA forms: packed FP32 or BF16-part operand
+    jh::DevBuf pw;                // nblocks x V x W float
+    jh::DevBuf blk;               // N int: block of every view (device copy)
+};

@@ -262,6 +262,46 @@ class CompositeCore:
         self.process_pairs(frames[half:], frames[:half], half, out[half:])
         return out
 
+    def process_pairs_operand(self, dirc, conj, filt, view0=0, out=None, aview0=0):
+        """process_pairs() written straight into `filt`'s pre-weighted MFMA
+        operand (jh_composite_run_operand): views view0.. land at the rows of
+        an operand whose filter launch starts at view aview0."""
+        for t, name in ((dirc, "dirc"), (conj, "conj")):
+            if t.dim() != 3 or t.shape[1:] != (self.V, self.W):
+                raise ValueError(f"{name} must be (n, {self.V}, {self.W})")
+            if not t.is_cuda or t.dtype != torch.float32:
+                raise ValueError(f"{name} must be a float32 device tensor")
+            if t.stride(2) != 1 or t.stride(1) != self.W or (t.shape[0] > 1 and t.stride(0) < self.V * self.W):
+                raise ValueError(f"{name}: every frame must be contiguous")
+        n = dirc.shape[0]
+        if conj.shape[0] != n:
+            raise ValueError("dirc and conj hold different frame counts")
+        stride = dirc.stride(0) if n > 1 else self.V * self.W
+        if n > 1 and conj.stride(0) != stride:
+            raise ValueError("dirc and conj frames must have the same stride")
+        if out is None:
+            out = torch.empty(filt.operand_bytes(view0 + n - aview0) // 4, dtype=torch.float32, device=dirc.device)
+        _require_cuda(out, "out", torch.float32)
+        if out.numel() * 4 < filt.operand_bytes(view0 + n - aview0):
+            raise ValueError("out is smaller than the operand of these views")
+        call("jh_composite_run_operand", self._h, filt._h, _ptr(dirc), _ptr(conj), ctypes.c_longlong(stride),
+             view0, n, _ptr(out), aview0, _stream())
+        return out
+
+    def process_operand(self, frames, filt, out=None):
+        """process() written straight into `filt`'s pre-weighted operand of
+        all N views: feed it to FilterCore.process_operand_staged(out, N, ...)."""
+        _require_cuda(frames, "frames", torch.float32)
+        N = self.n_views
+        if frames.dim() != 3 or frames.shape != (N, self.V, self.W):
+            raise ValueError(f"frames must be ({N}, {self.V}, {self.W})")
+        if out is None:
+            out = torch.empty(filt.operand_bytes(N) // 4, dtype=torch.float32, device=frames.device)
+        half = N // 2
+        self.process_pairs_operand(frames[:half], frames[half:], filt, 0, out)
+        self.process_pairs_operand(frames[half:], frames[:half], filt, half, out)
+        return out
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
@@ -482,12 +522,21 @@ class FDK:
         return (self.composite is None and not self.pre.resizes() and self.filter.mode != "fp32valu"
                 and self.fused())
 
+    def composite_operand_path(self):
+        """Whether the composite writes the filter's pre-weighted operand
+        (fused offset-detector chain): pre -> composite -> filter+stage -> bp."""
+        return self.composite is not None and self.filter.mode != "fp32valu" and self.fused()
+
     def __call__(self, raw, dark=None, gain=None, return_stages=False):
         if not return_stages and self.operand_path() and raw.shape[0] * self.filter.V <= 1048560:
             a = self.pre.process_operand(raw, self.filter, dark, gain)
             q = self.filter.process_operand_staged(a, raw.shape[0], self.bp, self.storage)
             return self.bp.run(q, self.storage)
         p = self.pre.process(raw, dark, gain)
+        if not return_stages and self.composite_operand_path() and raw.shape[0] * self.filter.V <= 1048560:
+            a = self.composite.process_operand(p, self.filter)
+            q = self.filter.process_operand_staged(a, raw.shape[0], self.bp, self.storage)
+            return self.bp.run(q, self.storage)
         c = self.composite.process(p) if self.composite is not None else p
         if not return_stages and self.fused():
             q = self.filter.process_staged(c, self.bp, self.storage)

@@ -785,6 +785,55 @@ def test_pre_operand_path_equals_two_kernel_chain(gpu, dtype, width):
     np.testing.assert_array_equal(a.cpu().numpy()[: t.size], t)
 
 
+@pytest.mark.parametrize("dtype", ["bf16", "f16"])
+@pytest.mark.parametrize("right", [True, False])
+def test_composite_operand_path_equals_composite_then_filter(gpu, dtype, right):
+    """jh_composite_run_operand + jh_filter_run_operand_staged against
+    jh_composite_run + jh_filter_run_staged, to the bit, through the 2W -> W
+    filter with and without yMirroring: each half of the scan is one call
+    and a half ends inside a 16-row block (3 views of 42 rows), so the two
+    calls share a block; and a view range other than the first."""
+    from jenhine_amd.recon import BPCore, CompositeCore, FilterCore
+    w = _offset_geometry(right, n_views=6, W=160, V=42)
+    N, V, W = w.n_views, w.n_v, w.n_u
+    ptm = w.ptms()
+    scan = w.scan_params()
+    comp = CompositeCore(W, V, ptm, w.sod, device=gpu)
+    assert comp.detector_on_right == right
+    filt = FilterCore(2 * W, V, ptm, scan.DSO, scan.HalfSampleRate, mode="mfma6", device=gpu, w_out=W,
+                      y_mirroring=right)
+    bp = BPCore(ptm, W, V, w.X, w.Y, w.Z, w.pxy, w.pz, device=gpu)
+    assert filt.stage_fusable(bp, dtype)
+    frames = torch.from_numpy((synth.normal(43, (N, V, W)) + 2.0).astype(np.float32)).to(gpu)
+    c = comp.process(frames)
+    ref = filt.process_staged(c, bp, dtype)
+    a = comp.process_operand(frames, filt)
+    got = filt.process_operand_staged(a, N, bp, dtype)
+    assert torch.equal(got, ref)
+    # views 1 and 2 (conjugates 4 and 5) alone: an operand that starts at view 1
+    half = N // 2
+    a2 = comp.process_pairs_operand(frames[1:half], frames[half + 1:], filt, view0=1, aview0=1)
+    ref2 = filt.process_staged(c[1:half], bp, dtype, view0=1)
+    got2 = filt.process_operand_staged(a2, half - 1, bp, dtype, view0=1)
+    assert torch.equal(got2, ref2)
+
+
+def test_fdk_composite_operand_path_equals_stage_by_stage(gpu):
+    """The offset-detector production chain (pre -> composite into the
+    operand -> filter with fused staging -> bp) gives the volume of the
+    stage-by-stage chain to the bit."""
+    from jenhine_amd.recon import FDK
+    w = configs.OFFSET128.scaled(n_views=16)
+    d = w.data()
+    fdk = FDK(w.scan_params(), w.recon_params(), d["ptm"], "mfma6", "bf16", device=gpu, composite=True)
+    assert fdk.composite_operand_path()
+    raw, dark = _u16(d["raw"], gpu), _u16(d["dark"], gpu)
+    gain = torch.from_numpy(d["gain"]).to(gpu)
+    vol = fdk(raw, dark, gain).cpu().numpy()
+    ref, _ = fdk(raw, dark, gain, return_stages=True)
+    np.testing.assert_array_equal(vol, ref.cpu().numpy())
+
+
 def test_fdk_operand_path_equals_stage_by_stage(gpu):
     """The production chain (pre -> operand -> filter with fused staging ->
     bp) gives the volume of the stage-by-stage chain to the bit."""
