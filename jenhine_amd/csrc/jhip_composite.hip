@@ -181,7 +181,8 @@ __global__ void __launch_bounds__(256) composite_operand_kernel(CompOpArgs a)
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
         const int i = (t >> 7) + 2 * j;
-        const int R = rb * 16 + i;
+        // one row per wave: scalar geometry, pre-weight row and view block
+        const int R = __builtin_amdgcn_readfirstlane(rb * 16 + i);
         float e = 0.0f;
         if (R >= a.R0 && R < a.R1 && k < W2 + 3) {
             const int rel = R - a.R0;
