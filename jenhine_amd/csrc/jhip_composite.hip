@@ -178,22 +178,29 @@ __global__ void __launch_bounds__(256) composite_operand_kernel(CompOpArgs a)
     // compute phase: consecutive lanes take consecutive columns of one row
     // (coalesced frame reads); thread t holds rows t / 128 + 2 j, j < 8
     const int kc = t & 127, k = blockIdx.x * 128 + kc;
+    // one row per wave: scalar geometry, pre-weight row and view block.  The
+    // wave's rows step by 2; (f, r) of the first row at or after R0 comes from
+    // one division and is advanced from there.
+    const int rows = a.R1 - a.R0;
+    const int vbase = a.aview0 + a.R0 / V;  // scan view of launch row R0
+    int rel = __builtin_amdgcn_readfirstlane(rb * 16 + (t >> 7) - a.R0);
+    const int relS = rel < 0 ? (rel & 1) : rel;
+    int f = relS / V, r = relS - f * V;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-        const int i = (t >> 7) + 2 * j;
-        // one row per wave: scalar geometry, pre-weight row and view block
-        const int R = __builtin_amdgcn_readfirstlane(rb * 16 + i);
         float e = 0.0f;
-        if (R >= a.R0 && R < a.R1 && k < W2 + 3) {
-            const int rel = R - a.R0;
-            const int f = rel / V, r = rel - f * V;
-            const int view = a.R0 / V + f;  // view of the filter launch (operand rows start at view aview0)
+        if (rel >= 0 && rel < rows && k < W2 + 3) {
             if (k < W2)
-                e = composite_value(a.c, f, r, k) * a.pw[((size_t)a.blk[a.aview0 + view] * V + r) * W2 + k];
+                e = composite_value(a.c, f, r, k) * a.pw[((size_t)a.blk[vbase + f] * V + r) * W2 + k];
             else
                 e = composite_value(a.c, f, r, a.edge0 + k - W2);
         }
-        tile[i * kCompOpPitch + kc] = e;
+        tile[((t >> 7) + 2 * j) * kCompOpPitch + kc] = e;
+        if (rel >= 0) {
+            r += 2;
+            if (r >= V) { r -= V; ++f; }
+        }
+        rel += 2;
     }
     __syncthreads();
     const int sl = t >> 6, l = t & 63;
