@@ -75,5 +75,7 @@ def test_fused_path_entry_points_reject_bad_arguments_without_device():
     assert st == _lib.JH_ERR_INVALID and b"bad arguments" in L.jh_last_error()
     st = L.jh_filter_run_operand_staged(None, None, None, 0, 1, _lib.JH_BF16, None, None)
     assert st == _lib.JH_ERR_INVALID and b"bad arguments" in L.jh_last_error()
+    st = L.jh_composite_run_operand(None, None, None, None, ctypes.c_longlong(0), 0, 1, None, 0, None)
+    assert st == _lib.JH_ERR_INVALID and b"bad arguments" in L.jh_last_error()
     b = ctypes.c_size_t()
     assert L.jh_filter_operand_bytes(None, 1, ctypes.byref(b)) == _lib.JH_ERR_INVALID
