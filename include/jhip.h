@@ -224,8 +224,11 @@ jh_status jh_composite_run(jh_composite_plan* plan, const float* d_dirc, const f
  * at view aview0, so the two halves of a scan (different conjugate buffers)
  * fill one operand with two calls.  The filter takes frames of 2W columns
  * (plan->W == 2 * composite W, same V and view count, same device, an MFMA
- * mode).  jh_filter_run_operand_staged(plan, bp, d_a, aview0, ...) then gives
- * bit-identically what jh_composite_run, jh_filter_run_staged give. */
+ * mode).  d_a holds at least jh_filter_operand_bytes(plan, view0 + nviews -
+ * aview0) bytes; a call writes only the float4s of its own rows, so calls for
+ * disjoint views may share a 16-row block.  jh_filter_run_operand_staged(plan,
+ * bp, d_a, aview0, ...) then gives bit-identically what jh_composite_run,
+ * jh_filter_run_staged give. */
 jh_status jh_composite_run_operand(jh_composite_plan* plan, jh_filter_plan* filter, const float* d_dirc,
                                    const float* d_conj, long long inStride, int view0, int nviews, float* d_a,
                                    int aview0, jh_stream_t stream);
