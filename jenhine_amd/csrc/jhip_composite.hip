@@ -88,13 +88,14 @@ __device__ __forceinline__ bool remap_at(const float* __restrict__ img, int W, i
     return true;
 }
 
-// Pixel (r, c) of frame f's conjugate half (output column conjOff + c).
-__device__ __forceinline__ float conj_value(const CompArgs& a, int f, int r, int c)
+// Pixel (r, c) of a frame's conjugate half (output column conjOff + c):
+// dirc and conj are the frame's direct and conjugate frames, g its
+// homography.
+__device__ __forceinline__ float conj_value_at(const CompArgs& a, const float* __restrict__ dirc,
+                                               const float* __restrict__ conj, const float* __restrict__ g, int r,
+                                               int c)
 {
     const int W = a.W, V = a.V;
-    const float* dirc = a.dirc + (size_t)f * a.inStride;
-    const float* conj = a.conj + (size_t)f * a.inStride;
-    const float* g = a.G + 9 * (size_t)(a.view0 + f);
     const float u1 = a.detRight ? (float)c - (float)W : (float)(c + W);
     const float v1 = (float)r;
     const float U = fmaf(g[0], u1, fmaf(g[1], v1, g[2]));
@@ -122,6 +123,13 @@ __device__ __forceinline__ float conj_value(const CompArgs& a, int f, int r, int
         val = x + y;
     }
     return val;
+}
+
+// Pixel (r, c) of frame f's conjugate half.
+__device__ __forceinline__ float conj_value(const CompArgs& a, int f, int r, int c)
+{
+    return conj_value_at(a, a.dirc + (size_t)f * a.inStride, a.conj + (size_t)f * a.inStride,
+                         a.G + 9 * (size_t)(a.view0 + f), r, c);
 }
 
 // Output column oc (0 .. 2W-1) of row r of composite frame f.
@@ -168,49 +176,71 @@ struct CompOpArgs {
 };
 
 constexpr int kCompOpPitch = 132;
+constexpr int kCompOpThreads = 128;  // 16 rows x 128 columns: 16 rows per thread
 
-__global__ void __launch_bounds__(256) composite_operand_kernel(CompOpArgs a)
+__global__ void __launch_bounds__(kCompOpThreads) composite_operand_kernel(CompOpArgs a)
 {
     __shared__ float tile[16 * kCompOpPitch];
     const int W2 = 2 * a.c.W, V = a.c.V;
     const int t = threadIdx.x;
     const int rb = a.blockRow0 + blockIdx.y;
     // compute phase: consecutive lanes take consecutive columns of one row
-    // (coalesced frame reads); thread t holds rows t / 128 + 2 j, j < 8
+    // (coalesced frame reads); thread t holds rows t / 128 + kRowStep j
+    constexpr int kRowStep = kCompOpThreads / 128;
     const int kc = t & 127, k = blockIdx.x * 128 + kc;
     // one row per wave: scalar geometry, pre-weight row and view block.  The
-    // wave's rows step by 2; (f, r) of the first row at or after R0 comes from
-    // one division and is advanced from there.
+    // wave's rows step by kRowStep; (f, r) of the first row at or after R0
+    // comes from one division and is advanced from there.
     const int rows = a.R1 - a.R0;
     const int vbase = a.aview0 + a.R0 / V;  // scan view of launch row R0
     int rel = __builtin_amdgcn_readfirstlane(rb * 16 + (t >> 7) - a.R0);
-    const int relS = rel < 0 ? (rel & 1) : rel;
+    const int relS = rel < 0 ? rel + (kRowStep - 1 - rel) / kRowStep * kRowStep : rel;  // first row >= R0
     int f = relS / V, r = relS - f * V;
+    // the frame's pointers, advanced per view; the column this lane reads
+    const int W = a.c.W;
+    const int dOff = a.c.detRight ? W : 0, cOff = a.c.detRight ? 0 : W;
+    const int kk = k < W2 ? k : a.edge0 + k - W2;  // composite column of operand column k
+    const bool direct = kk >= dOff && kk < dOff + W;
+    const int fc = min(f, a.c.nviews - 1);  // rows past the launch's views are not read
+    const float* dircF = a.c.dirc + (size_t)fc * a.c.inStride;
+    const float* conjF = a.c.conj + (size_t)fc * a.c.inStride;
+    const float* g = a.c.G + 9 * (size_t)(a.c.view0 + fc);
+    const float* pwF = a.pw + (size_t)a.blk[vbase + fc] * V * W2;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < 16 / kRowStep; ++j) {
         float e = 0.0f;
         if (rel >= 0 && rel < rows && k < W2 + 3) {
-            if (k < W2)
-                e = composite_value(a.c, f, r, k) * a.pw[((size_t)a.blk[vbase + f] * V + r) * W2 + k];
-            else
-                e = composite_value(a.c, f, r, a.edge0 + k - W2);
+            e = direct ? dircF[(size_t)r * W + (kk - dOff)] : conj_value_at(a.c, dircF, conjF, g, r, kk - cOff);
+            if (k < W2) e = e * pwF[(size_t)r * W2 + k];
         }
-        tile[((t >> 7) + 2 * j) * kCompOpPitch + kc] = e;
+        tile[((t >> 7) + kRowStep * j) * kCompOpPitch + kc] = e;
         if (rel >= 0) {
-            r += 2;
-            if (r >= V) { r -= V; ++f; }
+            r += kRowStep;
+            if (r >= V) {
+                while (r >= V) { r -= V; ++f; }
+                if (f < a.c.nviews) {
+                    dircF = a.c.dirc + (size_t)f * a.c.inStride;
+                    conjF = a.c.conj + (size_t)f * a.c.inStride;
+                    g = a.c.G + 9 * (size_t)(a.c.view0 + f);
+                    pwF = a.pw + (size_t)a.blk[vbase + f] * V * W2;
+                }
+            }
         }
-        rel += 2;
+        rel += kRowStep;
     }
     __syncthreads();
-    const int sl = t >> 6, l = t & 63;
-    const int s = blockIdx.x * 4 + sl;
-    const int Rl = rb * 16 + (l & 15);
-    if (s >= a.nsteps || Rl < a.R0 || Rl >= a.R1) return;
-    const float* src = tile + (l & 15) * kCompOpPitch + 32 * sl + 8 * (l >> 4);
-    float4* dst = a.out + ((size_t)rb * a.nsteps + s) * 128 + l;
-    dst[0] = make_float4(src[0], src[1], src[2], src[3]);
-    dst[64] = make_float4(src[4], src[5], src[6], src[7]);
+    // write phase: the 512 float4s of the block (4 k steps x 2 halves x 64
+    // lanes of the MFMA operand order), 512 / kCompOpThreads per thread
+#pragma unroll
+    for (int i = 0; i < 512 / kCompOpThreads; ++i) {
+        const int idx = t + i * kCompOpThreads;
+        const int sl = idx >> 7, h = (idx >> 6) & 1, l = idx & 63;
+        const int s = blockIdx.x * 4 + sl;
+        const int Rl = rb * 16 + (l & 15);
+        if (s >= a.nsteps || Rl < a.R0 || Rl >= a.R1) continue;
+        const float* src = tile + (l & 15) * kCompOpPitch + 32 * sl + 8 * (l >> 4) + 4 * h;
+        a.out[(((size_t)rb * a.nsteps + s) * 2 + h) * 64 + l] = make_float4(src[0], src[1], src[2], src[3]);
+    }
 }
 #pragma clang fp contract(on)
 
@@ -351,7 +381,7 @@ jh_status jh_composite_run_operand(jh_composite_plan* p, jh_filter_plan* fp, con
         a.aview0 = aview0;
         a.edge0 = fp->mirror ? fp->W - 3 : 0;
         const dim3 grid((unsigned)((fp->Kp + 127) / 128), (unsigned)(b1 - b0 + 1));
-        composite_operand_kernel<<<grid, 256, 0, as_stream(stream)>>>(a);
+        composite_operand_kernel<<<grid, kCompOpThreads, 0, as_stream(stream)>>>(a);
         check_launch("composite_operand_kernel");
     });
 }
