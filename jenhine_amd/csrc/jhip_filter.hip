@@ -487,17 +487,32 @@ __device__ void stage_strips_epilogue(const FiltArgs& a, const f32x4_t (&acc)[2]
     }
 }
 
-template <int MODE>
-__global__ void __launch_bounds__(256) filter_mfma_kernel(FiltArgs a)
+// 16 bytes per lane from global memory straight into LDS (global_load_lds):
+// lane i's bytes land at wave_base + 16 i; wave_base is the wave's LDS base.
+__device__ __forceinline__ void copy16_to_lds(const void* src, unsigned char* wave_base)
+{
+    __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)wave_base, 16, 0, 0);
+}
+
+// s_waitcnt vmcnt(0) (expcnt and lgkmcnt left at their maxima): every copy
+// into LDS has landed.
+__device__ __forceinline__ void wait_lds_copies() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// TILED: A is the pre-weighted operand (a.tiled); the split-BF16 forms then
+// load the next k step's A while the current one computes.
+template <int MODE, bool TILED>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) filter_mfma_kernel(FiltArgs a)
 {
     typedef FiltMode<MODE> M;
     constexpr int P = M::parts;
     constexpr int LB = M::lane_bytes;                 // B bytes per lane per column block
     constexpr int TILE_BYTES = 8 * 64 * LB;           // B bytes per k step per workgroup
     constexpr int CHUNKS = TILE_BYTES / 16 / 256;     // 16-byte chunks per thread
-    // the B tile of a k step; after the last step, the staged tile of the
-    // fused epilogue (128 rows at kStagePitch 16-bit values)
-    constexpr int LDS_BYTES = TILE_BYTES > 128 * kStagePitch * 2 ? TILE_BYTES : 128 * kStagePitch * 2;
+    // the B tiles of two k steps: step s reads buffer s & 1 while the B tile
+    // of step s + 1 is copied from memory straight into the other
+    // (global_load_lds, no registers); after the last step, the staged tile
+    // of the fused epilogue (128 rows at kStagePitch 16-bit values)
+    constexpr int LDS_BYTES = 2 * TILE_BYTES > 128 * kStagePitch * 2 ? 2 * TILE_BYTES : 128 * kStagePitch * 2;
     __shared__ __attribute__((aligned(16))) unsigned char lds[LDS_BYTES];
 
     const int lane = threadIdx.x & 63;
@@ -526,11 +541,16 @@ __global__ void __launch_bounds__(256) filter_mfma_kernel(FiltArgs a)
 
     const u32x4_t* gB = reinterpret_cast<const u32x4_t*>(a.Bhl);
     auto gsrc = [&](int s) { return gB + ((size_t)s * ncb8 + cb0) * 64 * LB / 16; };
-    u32x4_t pre[CHUNKS];
+    // B tile of step s into LDS buffer buf: 16-byte chunk threadIdx.x + 256 c
+    // of the tile; each wave's 64 chunks land contiguously at its LDS base
+    auto copy_b = [&](int s, int buf) {
+        const u32x4_t* src = gsrc(s) + threadIdx.x;
 #pragma unroll
-    for (int c = 0; c < CHUNKS; ++c) pre[c] = gsrc(0)[threadIdx.x + c * 256];
-#pragma unroll
-    for (int c = 0; c < CHUNKS; ++c) reinterpret_cast<u32x4_t*>(lds)[threadIdx.x + c * 256] = pre[c];
+        for (int c = 0; c < CHUNKS; ++c)
+            copy16_to_lds(src + c * 256, lds + buf * TILE_BYTES + (c * 256 + (threadIdx.x & ~63u)) * 16);
+    };
+    copy_b(0, 0);
+    wait_lds_copies();
     __syncthreads();
 
     // the 8 A values lane supplies for row block rb at k step s: formed from
@@ -553,19 +573,34 @@ __global__ void __launch_bounds__(256) filter_mfma_kernel(FiltArgs a)
         }
     };
 
-    for (int s = 0; s < nsteps; ++s) {
-        if (s + 1 < nsteps) {
+    // the operand's A of step s for row block rb; rows past the launch read
+    // the last block (their results are never stored), so the loads carry
+    // no condition and the wait for one step's A does not wait for the next
+    auto load_tiled = [&](int rb, int s, float4 (&x)[2]) {
+        const int rs = min(blockIdx.y * 128 + wave * 32 + rb * 16, (a.rows - 1) / 16 * 16);
+        const float4* src = reinterpret_cast<const float4*>(a.x) + ((size_t)(rs / 16) * nsteps + s) * 128 + lane;
+        x[0] = src[0];
+        x[1] = src[64];
+    };
+    float4 acur[2][2];
+    if constexpr (TILED && MODE != kModeF32) {
 #pragma unroll
-            for (int c = 0; c < CHUNKS; ++c) pre[c] = gsrc(s + 1)[threadIdx.x + c * 256];
-        }
+        for (int rb = 0; rb < 2; ++rb) load_tiled(rb, 0, acur[rb]);
+    }
+
+    for (int s = 0; s < nsteps; ++s) {
+        const unsigned char* bt = lds + (s & 1) * TILE_BYTES;
         const int kb = 32 * s + kq;
         if (MODE == kModeF32) {
             float ea[2][8];
 #pragma unroll
             for (int rb = 0; rb < 2; ++rb) load_a(rb, s, kb, ea[rb]);
+            // unconditional (the last step recopies its own tile into the idle
+            // buffer), so that the waits for A do not also wait for the copy
+            copy_b(min(s + 1, nsteps - 1), (s + 1) & 1);
 #pragma unroll
             for (int c = 0; c < 8; ++c) {
-                const float4* bp = reinterpret_cast<const float4*>(lds + (c * 64 + lane) * LB);
+                const float4* bp = reinterpret_cast<const float4*>(bt + (c * 64 + lane) * LB);
                 const float4 b0 = bp[0], b1 = bp[1];
                 const float eb[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
 #pragma unroll
@@ -576,10 +611,21 @@ __global__ void __launch_bounds__(256) filter_mfma_kernel(FiltArgs a)
             }
         } else {
             bf16x8_t ap[2][P];
+            float4 anext[2][2];
+            if constexpr (TILED) {
+#pragma unroll
+                for (int rb = 0; rb < 2; ++rb) load_tiled(rb, min(s + 1, nsteps - 1), anext[rb]);
+            }
 #pragma unroll
             for (int rb = 0; rb < 2; ++rb) {
                 float e[8];
-                load_a(rb, s, kb, e);
+                if constexpr (TILED) {
+                    const float4 x0 = acur[rb][0], x1 = acur[rb][1];
+                    e[0] = x0.x; e[1] = x0.y; e[2] = x0.z; e[3] = x0.w;
+                    e[4] = x1.x; e[5] = x1.y; e[6] = x1.z; e[7] = x1.w;
+                } else {
+                    load_a(rb, s, kb, e);
+                }
                 if constexpr (P == 3) {
                     // hi / mid / lo by truncation: each part is the top 16
                     // bits of the remainder (exact in BF16, same sign), the
@@ -614,9 +660,12 @@ __global__ void __launch_bounds__(256) filter_mfma_kernel(FiltArgs a)
                     }
                 }
             }
+            // unconditional (the last step recopies its own tile into the idle
+            // buffer), so that the waits for A do not also wait for the copy
+            copy_b(min(s + 1, nsteps - 1), (s + 1) & 1);
 #pragma unroll
             for (int c = 0; c < 8; ++c) {
-                const bf16x8_t* bp = reinterpret_cast<const bf16x8_t*>(lds + (c * 64 + lane) * LB);
+                const bf16x8_t* bp = reinterpret_cast<const bf16x8_t*>(bt + (c * 64 + lane) * LB);
                 bf16x8_t bq[P];
 #pragma unroll
                 for (int p = 0; p < P; ++p) bq[p] = bp[p];
@@ -632,12 +681,12 @@ __global__ void __launch_bounds__(256) filter_mfma_kernel(FiltArgs a)
                     acc[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ap[rb][0], bq[0], acc[rb][c], 0, 0, 0);
                 }
             }
-        }
-        __syncthreads();
-        if (s + 1 < nsteps) {
+            if constexpr (TILED) {
 #pragma unroll
-            for (int c = 0; c < CHUNKS; ++c) reinterpret_cast<u32x4_t*>(lds)[threadIdx.x + c * 256] = pre[c];
+                for (int rb = 0; rb < 2; ++rb) { acur[rb][0] = anext[rb][0]; acur[rb][1] = anext[rb][1]; }
+            }
         }
+        wait_lds_copies();
         __syncthreads();
     }
     if (a.q != nullptr) {
@@ -879,9 +928,16 @@ void run_filter(jh_filter_plan* p, const float* d_in, float* d_out, int view0, i
     } else {
         dim3 grid((p->Wout + 127) / 128, (a.rows + 127) / 128);
         require(grid.y <= 65535u, "jh_filter_run: too many rows for one launch");
-        if (p->mode == JH_FILTER_FP32) filter_mfma_kernel<kModeF32><<<grid, 256, 0, st>>>(a);
-        else if (p->mode == JH_FILTER_MFMA_BF16X3) filter_mfma_kernel<kModeX3><<<grid, 256, 0, st>>>(a);
-        else filter_mfma_kernel<kModeX6><<<grid, 256, 0, st>>>(a);
+        if (p->mode == JH_FILTER_FP32) {
+            if (tiled) filter_mfma_kernel<kModeF32, true><<<grid, 256, 0, st>>>(a);
+            else filter_mfma_kernel<kModeF32, false><<<grid, 256, 0, st>>>(a);
+        } else if (p->mode == JH_FILTER_MFMA_BF16X3) {
+            if (tiled) filter_mfma_kernel<kModeX3, true><<<grid, 256, 0, st>>>(a);
+            else filter_mfma_kernel<kModeX3, false><<<grid, 256, 0, st>>>(a);
+        } else {
+            if (tiled) filter_mfma_kernel<kModeX6, true><<<grid, 256, 0, st>>>(a);
+            else filter_mfma_kernel<kModeX6, false><<<grid, 256, 0, st>>>(a);
+        }
         check_launch("filter_mfma_kernel");
     }
 }
