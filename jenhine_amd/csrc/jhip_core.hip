@@ -124,6 +124,9 @@ std::vector<double> design_filter(int W, double hsr, double cutStart, double cut
 
 __global__ void ptm_modify_kernel(double* ptm, int n, int op, double val)
 {
+    // each element is one product and one sum, rounded separately (no
+    // contraction), so the device and the FP64 host restatement agree to the bit
+#pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double* P = ptm + 12 * (size_t)i;

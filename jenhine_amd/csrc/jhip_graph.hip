@@ -30,6 +30,7 @@ struct GraphScan {
     int chunk;  // views per chunk (0 = not chosen yet)
     float* vol;
     int branch;
+    bool operand;  // every chunk takes the operand path (decided once, at instantiation)
 };
 
 struct GraphBranch {
@@ -46,8 +47,13 @@ static void check_status(jh_status s, const char* what)
     }
 }
 
-// Whether a chunk of n views of the scan takes the operand path: filter and
-// staging fuse, and the preprocessing neither resizes nor changes the frame.
+// Whether a chunk of n views of the scan can take the operand path: filter
+// and staging fuse, and the preprocessing neither resizes nor changes the
+// frame.  The row limit is monotonic in n, so when the scan's full chunk
+// qualifies every shorter chunk (the last, N % chunk) does too; the graph
+// decides once per scan from the full chunk and sizes the branch's operand
+// buffer from that decision, so no chunk writes an operand the buffer was
+// not sized for.
 static bool operand_path(const GraphScan& s, int n)
 {
     int fused = 0;
@@ -132,7 +138,7 @@ jh_status jh_graph_add_scan(jh_graph* g, jh_pre_plan* pre, jh_filter_plan* filt,
         require(fW == bp->W && fV == bp->V && fN == bp->N, "jh_graph_add_scan: filter plan does not match");
         for (const auto& s : g->scans)
             require(s.bp != bp && s.vol != d_vol, "jh_graph_add_scan: plan or volume used twice");
-        GraphScan s{pre, filt, bp, d_raw, d_dark, d_gain, dtype, std::min(viewsPerChunk, bp->N), d_vol, -1};
+        GraphScan s{pre, filt, bp, d_raw, d_dark, d_gain, dtype, std::min(viewsPerChunk, bp->N), d_vol, -1, false};
         g->scans.push_back(s);
         if (index) *index = (int)g->scans.size() - 1;
     });
@@ -216,8 +222,10 @@ jh_status jh_graph_instantiate(jh_graph* g, int nbranches)
         require(nbranches >= 1 && nbranches <= 16, "jh_graph_instantiate: branches must be 1..16");
         DeviceScope ds(g->device);
         const int nb = std::min<int>(nbranches, (int)g->scans.size());
-        for (auto& s : g->scans)
+        for (auto& s : g->scans) {
             if (s.chunk <= 0) s.chunk = std::min(s.bp->N, 128);
+            s.operand = operand_path(s, s.chunk);
+        }
 
         // Longest-first greedy assignment of scans to branches.
         std::vector<int> order(g->scans.size());
@@ -242,7 +250,7 @@ jh_status jh_graph_instantiate(jh_graph* g, int nbranches)
                     if (s.branch != b) continue;
                     const size_t frame = (size_t)s.bp->W * s.bp->V * sizeof(float);
                     pb = std::max(pb, frame * s.chunk);
-                    if (operand_path(s, s.chunk)) {  // the pre buffer holds the filter's operand
+                    if (s.operand) {  // the pre buffer holds the filter's operand of a full chunk
                         size_t ob = 0;
                         check_status(jh_filter_operand_bytes(s.filt, s.chunk, &ob), "operand bytes");
                         pb = std::max(pb, ob);
@@ -277,7 +285,7 @@ jh_status jh_graph_instantiate(jh_graph* g, int nbranches)
                         const int n = std::min(s.chunk, s.bp->N - v0);
                         int fused = 0;
                         check_status(jh_filter_stage_fusable(s.filt, s.bp, s.dtype, &fused), "fusable");
-                        if (operand_path(s, n)) {
+                        if (s.operand) {
                             // preprocessing into the filter's pre-weighted operand, then filter and staging
                             check_status(jh_pre_run_operand(s.pre, s.filt, s.raw + inFrame * v0, s.dark, s.gain, v0, n,
                                                             br.pre.as<float>(), st),

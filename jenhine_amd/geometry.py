@@ -118,6 +118,29 @@ def move_z(ptm, z):
     return P.reshape(np.shape(ptm))
 
 
+def adjust_ptm(ptm, border_left=0, border_up=0, mirroring_y=False, center_z=0.0):
+    """JEngine's adjustment of the matrices read from PTM.dat
+    (JEngine/JEngine.cpp:139-150), in FP64: MoveUAxis(BorderSizeLeft),
+    MoveVAxis(BorderSizeUp), ReverseXAxis, ReverseYAxis when MirroringY, then
+    MoveZAxis(-CenterZ) (CenterZ = PitchZ * Volume_Translation_Z,
+    Config/ConfigHelper.cpp:85-86)."""
+    P = move_v(move_u(ptm, float(np.float32(border_left))), float(np.float32(border_up)))
+    P = reverse_x(P)
+    if mirroring_y:
+        P = reverse_y(P)
+    return move_z(P, -float(center_z))
+
+
+def unadjust_ptm(ptm, border_left=0, border_up=0, mirroring_y=False, center_z=0.0):
+    """The inverse of adjust_ptm (each step undone in reverse order): the
+    scanner matrices whose adjustment gives ptm."""
+    P = move_z(ptm, float(center_z))
+    if mirroring_y:
+        P = reverse_y(P)
+    P = reverse_x(P)
+    return move_u(move_v(P, -float(np.float32(border_up))), -float(np.float32(border_left)))
+
+
 def project(ptm, xyz):
     """(u, v) of world points xyz (n x 3) through one 3x4 matrix."""
     P = np.asarray(ptm, np.float64).reshape(3, 4)

@@ -13,7 +13,10 @@
 // Input: either the reference's scan folder (input_folder=, input_template=
 // e.g. Proj_%d.prj, u16 frames; ptm_file= PTM.dat text) or raw files
 // (raw= N x NumDetsV x NumDetsU u16, ptm= N x 12 f64); optional dark (u16)
-// and gain (f32) frames.  composite=1 runs the offset-detector chain (each
+// and gain (f32) frames.  Matrices from PTM.dat get JEngine's adjustment
+// (JEngine.cpp:139-150: borders, ReverseX, ReverseY with mirror_y=1,
+// MoveZ(-PitchZ * translation_z)); adjust_ptm=0/1 overrides the default
+// (1 for ptm_file=, 0 for ptm=).  composite=1 runs the offset-detector chain (each
 // view paired with its conjugate, JEngine.cpp:152-238).  Output: the volume
 // Z x Y x X f32 (out=) and/or slice files (output_folder=, output_template=
 // e.g. slice_%d.slc).  dump= / load= take comma-separated
@@ -221,9 +224,17 @@ int main(int argc, char** argv)
         const bool fromFolder = a.kv.count("input_folder") != 0;
         std::vector<uint16_t> raw;
         if (!fromFolder) raw = ReadFile<uint16_t>(a.S("raw"), frame * scan.NumViews);
-        std::vector<double> ptm = a.kv.count("ptm_file") ? ReadPTMFile(a.S("ptm_file"), scan.NumViews)
-                                                         : ReadFile<double>(a.S("ptm"), 12 * scan.NumViews);
-        // JEngine.cpp:153-161
+        recon.MirroringY = a.I("mirror_y", "0") != 0;
+        recon.CenterZ = recon.PitchZ * (float)a.D("translation_z", "0");  // ConfigHelper.cpp:85-86
+        const bool ptmFile = a.kv.count("ptm_file") != 0;
+        std::vector<double> ptm = ptmFile ? ReadPTMFile(a.S("ptm_file"), scan.NumViews)
+                                          : ReadFile<double>(a.S("ptm"), 12 * scan.NumViews);
+        // PTM.dat holds the scanner's matrices: JEngine adjusts them for the
+        // borders, the axis directions and the z translation before use
+        // (JEngine.cpp:139-150); raw FP64 matrices (ptm=) are taken as the
+        // adjusted ones unless adjust_ptm=1
+        if (a.I("adjust_ptm", ptmFile ? "1" : "0") != 0) AdjustProjectionMatrices(ptm, scan, recon, device);
+        // JEngine.cpp:153-161, on the adjusted matrices
         const bool detectorOnRight = jh_detector_on_right(ptm.data(), (int)scan.NumUsedDetsU) != 0;
         const auto dumps = ParsePathMap(a.S("dump", ","));
         const auto loads = ParsePathMap(a.S("load", ","));

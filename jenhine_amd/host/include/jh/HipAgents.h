@@ -43,6 +43,14 @@ extern const char* const kDeviceOOMPrefix;
 void CheckJh(jh_status s, const char* what);
 void CheckHip(hipError_t e, const char* what);
 
+// JEngine's adjustment of the matrices read from PTM.dat, applied in FP64 on
+// `device` through jh_ptm_modify (JEngine/JEngine.cpp:139-150): per view
+// MoveUAxis(BorderSizeLeft), MoveVAxis(BorderSizeUp), ReverseXAxis, then
+// ReverseYAxis when MirroringY, and MoveZAxis(-CenterZ).  The detector-side
+// decision (JEngine.cpp:153-161) is taken on the adjusted matrices.
+void AdjustProjectionMatrices(std::vector<double>& ptm, const ScanParams& scan, const ReconParams& recon,
+                              int device);
+
 namespace hip {
 
 // Throws with kDeviceOOMPrefix when the allocation fails.  (The library's

@@ -60,6 +60,9 @@ struct ReconParams {
     float CTNumNorm0 = 1.0f;  // ReconParams::CTNumNorm0 (ConfigHelper.cpp:91)
     float CTNumNorm1 = 1.0f;  // ReconParams::CTNumNorm1 (ConfigHelper.cpp:92)
     float MuWater = 0.02f;    // ReconParams::MuWater (ConfigHelper.cpp:70)
+    // the projection-matrix adjustment of JEngine.cpp:139-150
+    bool MirroringY = false;  // ReconParams::MirroringY (ConfigHelper.cpp:82)
+    float CenterZ = 0.0f;     // ReconParams::CenterZ = PitchZ * Volume_Translation_Z (ConfigHelper.cpp:85-86)
 };
 
 }  // namespace jh

@@ -21,6 +21,50 @@ void CheckHip(hipError_t e, const char* what)
     if (e != hipSuccess) throw Exception(std::string(what) + ": " + hipGetErrorString(e));
 }
 
+void AdjustProjectionMatrices(std::vector<double>& ptm, const ScanParams& scan, const ReconParams& recon,
+                              int device)
+{
+    if (ptm.size() % 12) ThrowException("AdjustProjectionMatrices: not a whole number of 3x4 matrices");
+    const int nv = (int)(ptm.size() / 12);
+    if (nv == 0) return;
+    CheckHip(hipSetDevice(device), "hipSetDevice");
+    double* d = nullptr;
+    CheckHip(hipMalloc(&d, ptm.size() * sizeof(double)), "hipMalloc ptm");
+    jh_status s = JH_OK;
+    const char* what = "ptm upload";
+    if (hipMemcpy(d, ptm.data(), ptm.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(d);
+        CheckHip(hipErrorInvalidValue, what);
+    }
+    // the reference's order (JEngine.cpp:141-149); values as the reference
+    // forms them (float(BorderSize*), -CenterZ)
+    struct Op {
+        int op;
+        double value;
+        bool on;
+    };
+    const Op ops[] = {
+        {JH_PTM_MOVE_U, (double)(float)scan.BorderSizeLeft, true},
+        {JH_PTM_MOVE_V, (double)(float)scan.BorderSizeUp, true},
+        {JH_PTM_REVERSE_X, 0.0, true},
+        {JH_PTM_REVERSE_Y, 0.0, recon.MirroringY},
+        {JH_PTM_MOVE_Z, -(double)recon.CenterZ, true},
+    };
+    for (const Op& o : ops) {
+        if (!o.on) continue;
+        s = jh_ptm_modify(d, nv, o.op, o.value, nullptr);
+        if (s != JH_OK) {
+            what = "jh_ptm_modify";
+            break;
+        }
+    }
+    const hipError_t e = s == JH_OK ? hipMemcpy(ptm.data(), d, ptm.size() * sizeof(double), hipMemcpyDeviceToHost)
+                                    : hipSuccess;
+    (void)hipFree(d);
+    CheckJh(s, what);
+    CheckHip(e, "ptm download");
+}
+
 namespace hip {
 
 static std::atomic<size_t> g_allocated{0};

@@ -134,12 +134,16 @@ def test_agent_graph_composite_chain(gpu, offset_data, tmp_path):
 def test_agent_graph_reads_scan_folder_and_writes_slice_files(gpu, data, tmp_path):
     """Proj_%d.prj frames + PTM.dat in, slice_%d.slc files out (InputAgent,
     PTMReader, OutputAgent formats): the same volume as the in-memory run."""
-    from jenhine_amd import io as jio
+    from jenhine_amd import geometry, io as jio
     ref, _ = _run(tmp_path, data)
     folder = tmp_path / "scan"
     jio.write_projection_folder(str(folder), data["raw"])
-    jio.write_ptm_dat(str(folder / "PTM.dat"), data["ptm"])
-    np.testing.assert_array_equal(jio.read_ptm_dat(str(folder / "PTM.dat"), W.n_views), data["ptm"])
+    # PTM.dat holds the scanner's matrices, which JEngine adjusts on reading
+    # (JEngine.cpp:139-150; with no borders and no translation: ReverseX)
+    scanner = geometry.unadjust_ptm(data["ptm"])
+    jio.write_ptm_dat(str(folder / "PTM.dat"), scanner)
+    np.testing.assert_array_equal(jio.read_ptm_dat(str(folder / "PTM.dat"), W.n_views), scanner)
+    np.testing.assert_array_equal(geometry.adjust_ptm(scanner), data["ptm"])
     outdir = tmp_path / "slices"
     _, info = _run(tmp_path, data, read_volume=False,
                    extra=[f"input_folder={folder}", "input_template=Proj_%d.prj", f"ptm_file={folder / 'PTM.dat'}",
@@ -167,3 +171,64 @@ def test_pipe_dump_and_replay(gpu, data, tmp_path):
     blank["raw"] = np.zeros_like(data["raw"])
     vol, _ = _run(tmp_path, blank, extra=[f"load={key}:{dump}"])
     np.testing.assert_array_equal(vol, ref)
+
+
+def _bordered(data, bl, br, bu, bd):
+    """Raw and dark frames with borders around the used detector area (the
+    borders hold values that must be cropped away)."""
+    n, V, U = data["raw"].shape
+    raw = np.full((n, V + bu + bd, U + bl + br), 60000, np.uint16)
+    raw[:, bu:bu + V, bl:bl + U] = data["raw"]
+    dark = np.full((V + bu + bd, U + bl + br), 7, np.uint16)
+    dark[bu:bu + V, bl:bl + U] = data["dark"]
+    out = dict(data)
+    out["raw"], out["dark"] = raw, dark
+    return out
+
+
+def test_agent_graph_adjusts_scanner_matrices(gpu, data, tmp_path):
+    """A reference scan folder with borders, Volume_Mirroring_Y and
+    Volume_Translation_Z: the pipeline applies MoveU(borderLeft),
+    MoveV(borderUp), ReverseX, ReverseY and MoveZ(-CenterZ) to the PTM.dat
+    matrices (JEngine.cpp:139-150) on the device.  The result equals, to the
+    bit, a run given the matrices adjusted by the FP64 host restatement, and
+    reconstructs the phantom of the original geometry."""
+    from jenhine_amd import geometry, io as jio
+    bl, br, bu, bd, tz = 5, 3, 4, 2, 6.0
+    cz = float(np.float32(np.float32(W.pz) * np.float32(tz)))  # CenterZ = PitchZ * translation, in float
+    scanner = geometry.unadjust_ptm(data["ptm"], bl, bu, True, cz)
+    bd_data = _bordered(data, bl, br, bu, bd)
+    borders = [f"border_left={bl}", f"border_right={br}", f"border_up={bu}", f"border_down={bd}",
+               f"detu={W.n_u + bl + br}", f"detv={W.n_v + bu + bd}"]
+    folder = tmp_path / "scan"
+    jio.write_projection_folder(str(folder), bd_data["raw"])
+    jio.write_ptm_dat(str(folder / "PTM.dat"), scanner)
+    vol, _ = _run(tmp_path, bd_data, extra=borders + [f"input_folder={folder}", "input_template=Proj_%d.prj",
+                                                      f"ptm_file={folder / 'PTM.dat'}", "mirror_y=1",
+                                                      f"translation_z={tz}"])
+    host = dict(bd_data)
+    host["ptm"] = geometry.adjust_ptm(scanner, bl, bu, True, cz)
+    ref, _ = _run(tmp_path, host, extra=borders)
+    np.testing.assert_array_equal(vol, ref)
+    # the adjusted matrices are the original geometry up to FP64 rounding:
+    # the volume is the plain run's (no borders, original matrices)
+    plain, _ = _run(tmp_path, data)
+    e = rel_rmse(vol, plain)
+    print("adjusted scanner matrices vs original geometry", e)
+    assert e <= 1e-5
+
+
+def test_agent_graph_mirroring_y_flips_the_volume(gpu, data, tmp_path):
+    """ReverseYAxis (Volume_Mirroring_Y) mirrors the volume in y: with the
+    centred axes of BPCore.cpp:47-66, y -> -y maps row j to row Y-1-j."""
+    from jenhine_amd import geometry, io as jio
+    folder = tmp_path / "scan"
+    jio.write_projection_folder(str(folder), data["raw"])
+    jio.write_ptm_dat(str(folder / "PTM.dat"), geometry.unadjust_ptm(data["ptm"], mirroring_y=True))
+    args = [f"input_folder={folder}", "input_template=Proj_%d.prj", f"ptm_file={folder / 'PTM.dat'}"]
+    a, _ = _run(tmp_path, data, extra=args + ["mirror_y=1"])
+    b, _ = _run(tmp_path, data, extra=args + ["mirror_y=0"])
+    e = rel_rmse(b, a[:, ::-1, :])
+    print("mirror_y=0 vs mirror_y=1 flipped in y", e)
+    assert e <= 1e-5
+    assert rel_rmse(b, a) > 1e-2  # the phantom is not symmetric in y

@@ -226,6 +226,27 @@ def test_ptm_modifiers_invariants():
     np.testing.assert_allclose(v0, v1, rtol=1e-12)
 
 
+def test_ptm_adjust_sequence_closed_forms():
+    """JEngine's adjustment (JEngine.cpp:139-150) of the reference test matrix:
+    the borders shift (u, v) by (-borderLeft, -borderUp), ReverseX / ReverseY
+    mirror x / y, MoveZ(-CenterZ) moves the volume origin to z = -CenterZ;
+    unadjust_ptm inverts it."""
+    P = REF_PTM.astype(np.float64)
+    xyz = np.array([[x, y, z] for x in (-3, 2) for y in (-1, 5) for z in (-2, 1)], np.float64)
+    u0, v0 = geometry.project(P, xyz)
+    A = geometry.adjust_ptm(P, border_left=7, border_up=3, mirroring_y=True, center_z=0.45)
+    u1, v1 = geometry.project(A, xyz * [-1, -1, 1] + [0, 0, 0.45])
+    np.testing.assert_allclose(u1, u0 - 7.0, rtol=0, atol=1e-9)
+    np.testing.assert_allclose(v1, v0 - 3.0, rtol=0, atol=1e-9)
+    A = geometry.adjust_ptm(P)
+    u1, v1 = geometry.project(A, xyz * [-1, 1, 1])
+    np.testing.assert_allclose(u1, u0, rtol=1e-12)
+    np.testing.assert_allclose(v1, v0, rtol=1e-12)
+    for mir in (False, True):
+        B = geometry.unadjust_ptm(geometry.adjust_ptm(P, 7, 3, mir, 0.45), 7, 3, mir, 0.45)
+        np.testing.assert_allclose(B, P, rtol=1e-13, atol=1e-13 * np.abs(P).max())
+
+
 def test_circular_ptm_central_ray_and_magnification():
     ptm = geometry.circular_ptms(8, 350.0, 600.0, 768, 768, 0.2)
     for P in ptm:
