@@ -47,6 +47,9 @@ typedef int jh_status;
 #define JH_LAYOUT_QUAD 0  /* staged BF16/FP16 pixel: its 2 x 2 neighbourhood (8 B) */
 #define JH_LAYOUT_STRIP 1 /* staged BF16/FP16 pixel: four rows of its column (8 B) */
 
+#define JH_WEIGHT_COUNT 0 /* backprojection normalised by the back-projected uW (the reference) */
+#define JH_WEIGHT_FDK 1   /* FDK distance weight 1/w^2, no count (jh_bp_set_weight) */
+
 #define JH_WINDOW_REFERENCE 0   /* FilterCore.cpp:96-134 cut-off semantics */
 #define JH_WINDOW_SHEPP_LOGAN 1 /* extension: sinc(f / (2 f_N)), zero above f_N */
 
@@ -348,6 +351,22 @@ jh_status jh_bp_set_zfilter(jh_bp_plan* plan, int enable);
  * min(8191, (x - 300) * norm1 + 300) (JEngine.cpp:297-305; MuWater defaults to
  * 0.02, Config/ConfigHelper.cpp:70). */
 jh_status jh_bp_set_ctnum(jh_bp_plan* plan, int enable, float norm0, float norm1, float muWater);
+
+/* Voxel weighting of the backprojection outputs (jh_bp_run,
+ * jh_bp_accumulate + jh_bp_finalize).  JH_WEIGHT_COUNT (the default) is the
+ * reference's: vol = sum_v bilinear(q_v) / sum_v bilinear(uW_v)
+ * (BPAgent/BPCore.cpp:147-151,198-205).  JH_WEIGHT_FDK is the textbook FDK
+ * distance weight that normalisation replaces: vol = scale * sum_v
+ * bilinear(q_v) / w_v^2, w_v = P_v[2] . (x, y, z, 1) the voxel's third
+ * homogeneous coordinate (1 at the iso-centre for the reference's
+ * normalised matrices), a sample off the detector adding nothing (no count;
+ * BPCUDAAgent/BPCUDACore.cu:62 accumulates unweighted samples).  scale = 0
+ * selects 2 / nviews, which matches the count form at the iso-centre of a
+ * full orbit, where the conjugate views' uW sum to one.  The weighted form
+ * always gathers (JH_FETCH_LDS applies to the count form only).  No
+ * counterpart in the reference as an option. */
+jh_status jh_bp_set_weight(jh_bp_plan* plan, int weight, float scale);
+jh_status jh_bp_get_weight(jh_bp_plan* plan, int* weight, float* scale);
 
 /* Autotuner: times the normalised backprojection of views 0..nviews-1 of
  * d_q (reps launches each, after one warm-up) for every fetch form x tile

@@ -15,6 +15,7 @@ JH_F32, JH_BF16, JH_F16 = 0, 1, 2
 JH_FILTER_FP32, JH_FILTER_MFMA_BF16X3, JH_FILTER_MFMA_BF16X6, JH_FILTER_FP32_VALU = 0, 1, 2, 3
 JH_WINDOW_REFERENCE, JH_WINDOW_SHEPP_LOGAN = 0, 1
 JH_LAYOUT_QUAD, JH_LAYOUT_STRIP = 0, 1
+JH_WEIGHT_COUNT, JH_WEIGHT_FDK = 0, 1
 JH_PTM_MOVE_U, JH_PTM_MOVE_V, JH_PTM_REVERSE_X, JH_PTM_REVERSE_Y, JH_PTM_MOVE_Z = range(5)
 
 DTYPES = {"f32": JH_F32, "bf16": JH_BF16, "f16": JH_F16}
@@ -32,7 +33,7 @@ EXPORTS = [
     "jh_bp_accumulate", "jh_bp_finalize", "jh_bp_set_tile", "jh_bp_get_tile", "jh_bp_geometry_mode",
     "jh_bp_destroy", "jh_bp_pack", "jh_bp_backproject", "jh_fp_project", "jh_bp_autotune",
     "jh_bp_set_fetch", "jh_bp_get_fetch", "jh_bp_set_layout", "jh_bp_get_layout", "jh_bp_set_zfilter",
-    "jh_bp_set_ctnum", "jh_zfilter_run",
+    "jh_bp_set_ctnum", "jh_bp_set_weight", "jh_bp_get_weight", "jh_zfilter_run",
     "jh_ctnum_run",
     "jh_graph_create", "jh_graph_add_scan", "jh_graph_autotune", "jh_graph_scan_tuning", "jh_graph_instantiate",
     "jh_graph_launch", "jh_graph_destroy",
@@ -102,6 +103,8 @@ _SIGS = {
     "jh_bp_set_fetch": (c_int, [P, c_int]),
     "jh_bp_set_zfilter": (c_int, [P, c_int]),
     "jh_bp_set_ctnum": (c_int, [P, c_int, c_float, c_float, c_float]),
+    "jh_bp_set_weight": (c_int, [P, c_int, c_float]),
+    "jh_bp_get_weight": (c_int, [P, P, P]),
     "jh_zfilter_run": (c_int, [P, P, c_int, c_int, c_int, P]),
     "jh_ctnum_run": (c_int, [P, c_size_t, c_float, c_float, c_float, P]),
     "jh_bp_get_fetch": (c_int, [P, P]),

@@ -316,6 +316,12 @@ struct Sampler<JH_F32> {
         const float top = lerp(p, 0, fu), bot = lerp(p, 1, fu);
         return acc + fmaf(fv, bot - top, top);
     }
+    // the sample alone (the distance-weighted form scales it before adding)
+    __device__ static float sample(raw_t p, float fv, wt_t fu)
+    {
+        const float top = lerp(p, 0, fu), bot = lerp(p, 1, fu);
+        return fmaf(fv, bot - top, top);
+    }
     // bilinear coefficients (q00, q01 - q00, q10 - q00, (q11 - q10) - (q01 - q00))
     __device__ static float4 abcd(raw_t p)
     {
@@ -373,6 +379,11 @@ struct Half16<JH_BF16> {
         const float top = row(rt, w), bot = row(rb, w);
         return acc + fmaf(fv, bot - top, top);
     }
+    __device__ static float sample(uint32_t rt, uint32_t rb, Wt w, float fv)
+    {
+        const float top = row(rt, w), bot = row(rb, w);
+        return fmaf(fv, bot - top, top);
+    }
     __device__ static float lo(uint32_t w) { return __uint_as_float(w << 16); }
     __device__ static float hi(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
 };
@@ -404,6 +415,11 @@ struct Half16<JH_F16> {
         const float d = dot(rt, w.nw, dot(rb, w.w, 0.0f));
         return fmaf(fv, d, dot(rt, w.w, acc));
     }
+    __device__ static float sample(uint32_t rt, uint32_t rb, Wt w, float fv)
+    {
+        const float d = dot(rt, w.nw, dot(rb, w.w, 0.0f));
+        return fmaf(fv, d, dot(rt, w.w, 0.0f));
+    }
     __device__ static float lo(uint32_t w) { return (float)__builtin_bit_cast(half2_t, w).x; }
     __device__ static float hi(uint32_t w) { return (float)__builtin_bit_cast(half2_t, w).y; }
 };
@@ -427,6 +443,11 @@ struct QuadSampler {
         // directly on the element p.y was compiled as one of p.x
         const uint32_t px = p.x, py = p.y;
         return H::accum(acc, px, py, w, fv);
+    }
+    __device__ static float sample(raw_t p, float fv, wt_t w)
+    {
+        const uint32_t px = p.x, py = p.y;
+        return H::sample(px, py, w, fv);
     }
     __device__ static float4 abcd(raw_t p)
     {
@@ -474,6 +495,10 @@ struct StripSampler {
     {
         return H::accum(acc, rowpair<0>(p), rowpair<1>(p), w, fv);
     }
+    __device__ static float sample(raw_t p, float fv, wt_t w)
+    {
+        return H::sample(rowpair<0>(p), rowpair<1>(p), w, fv);
+    }
     __device__ static float4 abcd(raw_t p)
     {
         const float q00 = H::lo(p.x), q01 = H::lo(p.z), q10 = H::hi(p.x), q11 = H::hi(p.z);
@@ -505,6 +530,7 @@ struct BPArgs {
     int lds_px;          // LDS-staged fetch: pixels per coefficient buffer (two buffers)
     int pxb;             // bytes per staged pixel (16: FP32 quads; 8: BF16/FP16 quads and strips)
     int row_px;          // staged pixels per detector row (W, or W + 1 for strips)
+    float wscale;        // distance-weighted outputs: vol = wscale * acc
 };
 
 constexpr int kZB = 8;  // samples whose loads are issued together
@@ -514,15 +540,30 @@ constexpr int kModeStaged = 3;  // geometry class 0 with the LDS-staged fetch
 constexpr int kOutNormalised = 0;  // vol = acc / count (one-shot)
 constexpr int kOutAccumulate = 1;  // acc += ..., count += ... (streaming)
 constexpr int kOutRaw = 2;         // vol = acc (the un-normalised operator, FP32 storage only)
+// The FDK distance-weighted forms: every sample is scaled by 1/w^2 (w the
+// third homogeneous coordinate of the voxel in the view) and there is no
+// count: vol = wscale * acc (one-shot) or acc += ... (streaming).
+constexpr int kOutFdk = 3;
+constexpr int kOutFdkAccumulate = 4;
+
+constexpr bool distance_weighted(int out) { return out == kOutFdk || out == kOutFdkAccumulate; }
+
+// acc + s (count-normalised forms) or acc + s / w^2 (iw2 = 1/w^2)
+template <bool WT>
+__device__ __forceinline__ float add_sample(float acc, float s, float iw2)
+{
+    if constexpr (WT) return fmaf(s, iw2, acc);
+    else return acc + s;
+}
 
 // The samples of one all-in-range column in pairs over column strips: one
 // 16-byte gather at row floor(sv_k) serves samples k and k+1; the second
 // sample's rows start HI or HI + 1 rows down.  Each sample interpolates the
 // same row pairs with the same operations as the quad layout's one gather per
 // sample, so the results are identical.
-template <int ZC, int QT, int HI>
+template <int ZC, int QT, int HI, bool WT>
 __device__ __forceinline__ void bp_column_pairs(float* acc, __amdgpu_buffer_rsrc_t rs, uint32_t rowb, uint32_t ub,
-                                                float sv0, float dsv, typename Sampler<QT>::wt_t wq)
+                                                float sv0, float dsv, typename Sampler<QT>::wt_t wq, float iw2)
 {
     typedef Sampler<QT> S;
     constexpr int NP = ZC / 2;
@@ -547,7 +588,8 @@ __device__ __forceinline__ void bp_column_pairs(float* acc, __amdgpu_buffer_rsrc
         const uint32_t r0 = S::template rowpair<0>(p[m]);
         const uint32_t r1 = S::template rowpair<1>(p[m]);
         const uint32_t r2 = S::template rowpair<2>(p[m]);
-        acc[2 * m] = S::H::accum(acc[2 * m], r0, r1, wq, fa[m]);
+        if constexpr (WT) acc[2 * m] = fmaf(S::H::sample(r0, r1, wq, fa[m]), iw2, acc[2 * m]);
+        else acc[2 * m] = S::H::accum(acc[2 * m], r0, r1, wq, fa[m]);
         uint32_t rt, rb;
         if constexpr (HI == 0) {
             rt = e[m] ? r1 : r0;
@@ -557,7 +599,8 @@ __device__ __forceinline__ void bp_column_pairs(float* acc, __amdgpu_buffer_rsrc
             rt = e[m] ? r2 : r1;
             rb = e[m] ? r3 : r2;
         }
-        acc[2 * m + 1] = S::H::accum(acc[2 * m + 1], rt, rb, wq, fb[m]);
+        if constexpr (WT) acc[2 * m + 1] = fmaf(S::H::sample(rt, rb, wq, fb[m]), iw2, acc[2 * m + 1]);
+        else acc[2 * m + 1] = S::H::accum(acc[2 * m + 1], rt, rb, wq, fb[m]);
     }
 }
 
@@ -567,6 +610,12 @@ template <int MODE, int ZC, int QT, int OUT>
 __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
 {
     typedef Sampler<QT> S;
+    constexpr bool WT = distance_weighted(OUT);
+    // acc + the sample of raw p (scaled by iw2 = 1/w^2 in the weighted forms)
+    auto add = [](float acc, typename S::raw_t p, float fv, typename S::wt_t wq, float iw2) {
+        if constexpr (WT) return fmaf(S::sample(p, fv, wq), iw2, acc);
+        else return S::accum(acc, p, fv, wq);
+    };
     const int tx = tile % a.ntx;
     const unsigned rest = tile / a.ntx;
     const int tyi = rest % a.nty;
@@ -595,9 +644,9 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
     // partly inside the detector rows are kept in LDS, one slot per thread and
     // sample (touched by that thread only, so no barrier): such views are rare,
     // and ZC registers held across the view loop would cost occupancy.
-    __shared__ float cnt_part[MODE == 0 ? ZC * 256 : 1];
+    __shared__ float cnt_part[MODE == 0 && !WT ? ZC * 256 : 1];
     const int tid = threadIdx.y * 64 + threadIdx.x;
-    if constexpr (MODE == 0) {
+    if constexpr (MODE == 0 && !WT) {
 #pragma unroll
         for (int kk = 0; kk < ZC; ++kk) cnt_part[kk * 256 + tid] = 0.0f;
     }
@@ -617,6 +666,7 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
             // reciprocal, so the pair stays adjoint)
             const float rw = __builtin_amdgcn_rcpf(Wc);
             const float su = fmaf(Uc, rw, 0.5f);
+            const float iw2 = rw * rw;  // distance weight (weighted forms)
             if (su >= 0.0f && su <= Wm1) {
                 const int iu = (int)su;
                 float fe;
@@ -643,10 +693,10 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
                     bool paired = false;
                     if constexpr (is_strip<QT>()) {
                         if (dsv >= 0.0f && dsv <= 0.984375f) {
-                            bp_column_pairs<ZC, QT, 0>(acc, rs, rowb, ub, sv0, dsv, wq);
+                            bp_column_pairs<ZC, QT, 0, WT>(acc, rs, rowb, ub, sv0, dsv, wq, iw2);
                             paired = true;
                         } else if (dsv >= 1.015625f && dsv <= 1.984375f) {
-                            bp_column_pairs<ZC, QT, 1>(acc, rs, rowb, ub, sv0, dsv, wq);
+                            bp_column_pairs<ZC, QT, 1, WT>(acc, rs, rowb, ub, sv0, dsv, wq, iw2);
                             paired = true;
                         }
                     }
@@ -663,9 +713,9 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
 #pragma unroll
                         for (int k = 0; k < ZC; ++k) p[k] = S::load(rs, off[k]);
 #pragma unroll
-                        for (int k = 0; k < ZC; ++k) acc[k] = S::accum(acc[k], p[k], fv[k], wq);
+                        for (int k = 0; k < ZC; ++k) acc[k] = add(acc[k], p[k], fv[k], wq, iw2);
                     }
-                    cfull += fmaf(fe, uwq.y - uwq.x, uwq.x);
+                    if constexpr (!WT) cfull += fmaf(fe, uwq.y - uwq.x, uwq.x);
                 } else {
                     const float cw = fmaf(fe, uwq.y - uwq.x, uwq.x);
 #pragma unroll
@@ -686,9 +736,9 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
                         for (int q = 0; q < kZB; ++q) p[q] = S::load(rs, off[q]);
 #pragma unroll
                         for (int q = 0; q < kZB; ++q) {
-                            const float sum = S::accum(acc[kb + q], p[q], fv[q], wq);
+                            const float sum = add(acc[kb + q], p[q], fv[q], wq, iw2);
                             acc[kb + q] = in[q] ? sum : acc[kb + q];
-                            cnt_part[(kb + q) * 256 + tid] += in[q] ? cw : 0.0f;
+                            if constexpr (!WT) cnt_part[(kb + q) * 256 + tid] += in[q] ? cw : 0.0f;
                         }
                     }
                 }
@@ -702,7 +752,7 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
 #pragma unroll
             for (int kb = 0; kb < ZC; kb += kZB) {
                 uint32_t off[kZB];
-                float fe[kZB], fv[kZB];
+                float fe[kZB], fv[kZB], iw2[kZB];
                 typename S::wt_t wq[kZB];
                 bool in[kZB];
                 float2 uwq[kZB];
@@ -713,10 +763,12 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
                     if (MODE == 1) {
                         su = fmaf(fk, dsu, su0);
                         sv = fmaf(fk, dsv, sv0);
+                        iw2[q] = rw * rw;
                     } else {
                         const float r = __builtin_amdgcn_rcpf(fmaf(fk, g.c[11], Wc));
                         su = fmaf(fmaf(fk, g.c[3], Uc), r, 0.5f);
                         sv = fmaf(fmaf(fk, g.c[7], Vc), r, 0.5f);
+                        iw2[q] = r * r;
                     }
                     const float tu = __builtin_amdgcn_fmed3f(su, 0.0f, Wm1);
                     const float t = __builtin_amdgcn_fmed3f(sv, 0.0f, Vm1);
@@ -732,29 +784,35 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
                 for (int q = 0; q < kZB; ++q) p[q] = S::load(rs, off[q]);
 #pragma unroll
                 for (int q = 0; q < kZB; ++q) {
-                    const float sum = S::accum(acc[kb + q], p[q], fv[q], wq[q]);
-                    const float cw = fmaf(fe[q], uwq[q].y - uwq[q].x, uwq[q].x);
+                    const float sum = add(acc[kb + q], p[q], fv[q], wq[q], iw2[q]);
                     acc[kb + q] = in[q] ? sum : acc[kb + q];
-                    cnt[kb + q] += in[q] ? cw : 0.0f;
+                    if constexpr (!WT) {
+                        const float cw = fmaf(fe[q], uwq[q].y - uwq[q].x, uwq[q].x);
+                        cnt[kb + q] += in[q] ? cw : 0.0f;
+                    }
                 }
             }
         }
     }
 
     if (i >= a.X || j >= a.Y) return;
+    if constexpr (!WT) {
 #pragma unroll
-    for (int kk = 0; kk < ZC; ++kk) cnt[kk] = (MODE == 0 ? cnt_part[kk * 256 + tid] : cnt[kk]) + cfull;
+        for (int kk = 0; kk < ZC; ++kk) cnt[kk] = (MODE == 0 ? cnt_part[kk * 256 + tid] : cnt[kk]) + cfull;
+    }
     const size_t plane = (size_t)a.X * a.Y;
     size_t idx = ((size_t)k0 * a.Y + j) * a.X + i;
 #pragma unroll
     for (int kk = 0; kk < ZC; ++kk) {
         if (k0 + kk < a.Z) {
-            if (OUT == kOutNormalised) {
-                const float v = acc[kk] / cnt[kk];
+            if constexpr (OUT == kOutNormalised || OUT == kOutFdk) {
+                const float v = OUT == kOutFdk ? a.wscale * acc[kk] : acc[kk] / cnt[kk];
                 a.vol[idx] = a.ct ? ct_number(v, a.ct_scale, a.ct_norm1) : v;
-            } else if (OUT == kOutAccumulate) {
+            } else if constexpr (OUT == kOutAccumulate) {
                 a.acc[idx] += acc[kk];
                 a.cnt[idx] += cnt[kk];
+            } else if constexpr (OUT == kOutFdkAccumulate) {
+                a.acc[idx] += acc[kk];
             } else {
                 a.vol[idx] = acc[kk];
             }
@@ -807,6 +865,7 @@ extern __shared__ float4 lds_dyn[];
 template <int ZC, int QT, int OUT>
 __device__ __forceinline__ void bp_tile_lds(const BPArgs& a, unsigned tile)
 {
+    static_assert(!distance_weighted(OUT), "the distance-weighted forms gather");
     typedef Sampler<QT> S;
     __shared__ int box[3][4];  // u0, v0, width, pixels (pixels > lds_px: not staged)
     // count-weight pairs uwp[u0 .. u0 + width) of the staged view, so that the
@@ -1085,8 +1144,11 @@ bp_kernel(BPArgs a)
     const unsigned t8 = gridDim.x >> 3;  // gridDim.x = 8 * ceil(ntiles / 8)
     const unsigned tile = (blockIdx.x & 7u) * t8 + (blockIdx.x >> 3);
     if (tile >= ntiles) return;  // uniform per workgroup
-    if constexpr (MODE == kModeStaged) bp_tile_lds<ZC, QT, OUT>(a, tile);
-    else bp_tile<MODE, ZC, QT, OUT>(a, tile);
+    if constexpr (MODE == kModeStaged) {
+        if constexpr (!distance_weighted(OUT)) bp_tile_lds<ZC, QT, OUT>(a, tile);
+    } else {
+        bp_tile<MODE, ZC, QT, OUT>(a, tile);
+    }
 }
 
 __global__ void ctnum_kernel(float* __restrict__ x, size_t n, float scale, float norm1)
@@ -1096,10 +1158,11 @@ __global__ void ctnum_kernel(float* __restrict__ x, size_t n, float scale, float
 }
 
 __global__ void bp_finalize_kernel(const float* __restrict__ acc, const float* __restrict__ cnt,
-                                   float* __restrict__ vol, size_t n, int ct, float ct_scale, float ct_norm1)
+                                   float* __restrict__ vol, size_t n, int ct, float ct_scale, float ct_norm1,
+                                   int weighted, float wscale)
 {
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
-        const float v = acc[t] / cnt[t];
+        const float v = weighted ? wscale * acc[t] : acc[t] / cnt[t];
         vol[t] = ct ? ct_number(v, ct_scale, ct_norm1) : v;
     }
 }
@@ -1143,8 +1206,10 @@ static bp_kernel_t pick_kernel(int mode, int zc, int dtype, int fetch, int strip
 {
     switch (mode) {
     case 0:
-        return fetch == JH_FETCH_LDS ? pick_zc<kModeStaged, OUT>(zc, dtype, strip)
-                                     : pick_zc<0, OUT>(zc, dtype, strip);
+        // the distance-weighted forms have no LDS-staged fetch: they gather
+        if constexpr (!distance_weighted(OUT))
+            if (fetch == JH_FETCH_LDS) return pick_zc<kModeStaged, OUT>(zc, dtype, strip);
+        return pick_zc<0, OUT>(zc, dtype, strip);
     case 1: return pick_zc<1, OUT>(zc, dtype, strip);
     default: return pick_zc<2, OUT>(zc, dtype, strip);
     }
@@ -1181,6 +1246,7 @@ static void launch_bp(jh_bp_plan* p, const void* q, int dtype, int view0, int nv
     a.ct = p->ct;
     a.ct_scale = p->ct_scale;
     a.ct_norm1 = p->ct_norm1;
+    a.wscale = p->wscale;
     // workgroup extent: 8 x 8 voxel columns per wave (see bp_tile)
     const int ex = p->ty >= 2 ? 16 : 8, ey = p->ty >= 4 ? 16 : 8;
     a.ntx = (p->X + ex - 1) / ex;
@@ -1195,14 +1261,16 @@ static void launch_bp(jh_bp_plan* p, const void* q, int dtype, int view0, int nv
     require(ntiles < (1ull << 31), "backprojection grid too large");
     const size_t grid = (ntiles + 7) / 8 * 8;
     const int st = stage_strip(p, dtype) ? 1 : 0;
-    bp_kernel_t k = out == kOutNormalised   ? pick_kernel<kOutNormalised>(p->mode, p->zc, dtype, p->fetch, st)
-                    : out == kOutAccumulate ? pick_kernel<kOutAccumulate>(p->mode, p->zc, dtype, p->fetch, st)
-                                            : pick_kernel<kOutRaw>(p->mode, p->zc, dtype, p->fetch, st);
+    bp_kernel_t k = out == kOutNormalised      ? pick_kernel<kOutNormalised>(p->mode, p->zc, dtype, p->fetch, st)
+                    : out == kOutAccumulate    ? pick_kernel<kOutAccumulate>(p->mode, p->zc, dtype, p->fetch, st)
+                    : out == kOutFdk           ? pick_kernel<kOutFdk>(p->mode, p->zc, dtype, p->fetch, st)
+                    : out == kOutFdkAccumulate ? pick_kernel<kOutFdkAccumulate>(p->mode, p->zc, dtype, p->fetch, st)
+                                               : pick_kernel<kOutRaw>(p->mode, p->zc, dtype, p->fetch, st);
     // LDS-staged fetch: two buffers sized for the block footprints of this
     // tile depth (a larger footprint gathers), 64 zc pixels each, capped so
     // that both, with the kernel's static box ring and count-weight rows
     // (4 KiB + 48 B), stay within 64 KiB
-    const bool staged = p->mode == 0 && p->fetch == JH_FETCH_LDS;
+    const bool staged = p->mode == 0 && p->fetch == JH_FETCH_LDS && !distance_weighted(out);
     a.lds_px = staged ? std::min(64 * p->zc, (65536 - 4096 - 64) / 32) : 0;
     const size_t lds_bytes = staged ? 2 * (size_t)a.lds_px * sizeof(float4) : 0;
     hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(64, p->ty), lds_bytes, stream, a);
@@ -1243,6 +1311,8 @@ jh_status jh_bp_create(jh_bp_plan** plan, int device, const double* h_ptm, int n
             p->ct = 0;
             p->ct_scale = 1.0f;
             p->ct_norm1 = 1.0f;
+            p->weight = JH_WEIGHT_COUNT;
+            p->wscale = 2.0f / (float)nviews;
             p->ptm.alloc(sizeof(double) * 12 * nviews);
             p->geo.alloc(sizeof(Geo) * nviews);
             p->uw.alloc(sizeof(float) * (size_t)nviews * W);
@@ -1359,7 +1429,8 @@ jh_status jh_bp_run(jh_bp_plan* p, const void* d_q, int dtype, float* d_vol, jh_
         require(p != nullptr && d_q != nullptr && d_vol != nullptr, "jh_bp_run: bad arguments");
         require(dtype == JH_F32 || dtype == JH_BF16 || dtype == JH_F16, "jh_bp_run: unknown dtype");
         DeviceScope ds(p->device);
-        launch_bp(p, d_q, dtype, 0, p->N, d_vol, kOutNormalised, as_stream(stream));
+        launch_bp(p, d_q, dtype, 0, p->N, d_vol, p->weight == JH_WEIGHT_FDK ? kOutFdk : kOutNormalised,
+                  as_stream(stream));
     });
 }
 
@@ -1394,7 +1465,8 @@ jh_status jh_bp_accumulate(jh_bp_plan* p, const void* d_q, int dtype, int view0,
         require(p->acc.get() != nullptr, "jh_bp_accumulate: call jh_bp_reset first");
         if (nviews == 0) return;
         DeviceScope ds(p->device);
-        launch_bp(p, d_q, dtype, view0, nviews, nullptr, kOutAccumulate, as_stream(stream));
+        launch_bp(p, d_q, dtype, view0, nviews, nullptr,
+                  p->weight == JH_WEIGHT_FDK ? kOutFdkAccumulate : kOutAccumulate, as_stream(stream));
     });
 }
 
@@ -1410,7 +1482,7 @@ jh_status jh_bp_finalize(jh_bp_plan* p, float* d_vol, int z0, int nz, jh_stream_
         const size_t n = plane * nz;
         bp_finalize_kernel<<<grid_for(n, 256), 256, 0, as_stream(stream)>>>(
             p->acc.as<float>() + plane * z0, p->cnt.as<float>() + plane * z0, d_vol, n, p->ct, p->ct_scale,
-            p->ct_norm1);
+            p->ct_norm1, p->weight == JH_WEIGHT_FDK ? 1 : 0, p->wscale);
         check_launch("bp_finalize_kernel");
     });
 }
@@ -1465,6 +1537,26 @@ jh_status jh_bp_set_zfilter(jh_bp_plan* p, int enable)
     return guard([&] {
         require(p != nullptr, "jh_bp_set_zfilter: plan is NULL");
         p->zfilter = enable ? 1 : 0;
+    });
+}
+
+jh_status jh_bp_set_weight(jh_bp_plan* p, int weight, float scale)
+{
+    return guard([&] {
+        require(p != nullptr, "jh_bp_set_weight: plan is NULL");
+        require(weight == JH_WEIGHT_COUNT || weight == JH_WEIGHT_FDK, "jh_bp_set_weight: unknown weighting");
+        require(std::isfinite(scale) && scale >= 0.0f, "jh_bp_set_weight: scale must be finite and >= 0");
+        p->weight = weight;
+        p->wscale = scale > 0.0f ? scale : 2.0f / (float)p->N;
+    });
+}
+
+jh_status jh_bp_get_weight(jh_bp_plan* p, int* weight, float* scale)
+{
+    return guard([&] {
+        require(p != nullptr, "jh_bp_get_weight: plan is NULL");
+        if (weight) *weight = p->weight;
+        if (scale) *scale = p->wscale;
     });
 }
 
@@ -1531,9 +1623,10 @@ jh_status jh_bp_autotune(jh_bp_plan* p, const void* d_q, int dtype, int nviews, 
                         p->fetch = f;
                         p->zc = zcs[zi];
                         p->ty = tys[ti];
-                        launch_bp(p, d_q, dtype, 0, nviews, d_vol, kOutNormalised, s);  // warm-up
+                        const int out = p->weight == JH_WEIGHT_FDK ? kOutFdk : kOutNormalised;
+                        launch_bp(p, d_q, dtype, 0, nviews, d_vol, out, s);  // warm-up
                         JH_HIP(hipEventRecord(e0, s));
-                        for (int r = 0; r < reps; ++r) launch_bp(p, d_q, dtype, 0, nviews, d_vol, kOutNormalised, s);
+                        for (int r = 0; r < reps; ++r) launch_bp(p, d_q, dtype, 0, nviews, d_vol, out, s);
                         JH_HIP(hipEventRecord(e1, s));
                         JH_HIP(hipEventSynchronize(e1));
                         float ms = 0.0f;

@@ -25,6 +25,8 @@ struct jh_bp_plan {
     int zfilter;     // staging applies the 1-2-1 v smoothing first
     int ct;          // normalised output goes through the CT-number map
     float ct_scale, ct_norm1;
+    int weight;      // JH_WEIGHT_COUNT (the reference's normalisation) or JH_WEIGHT_FDK
+    float wscale;    // JH_WEIGHT_FDK: vol = wscale * sum of samples / w^2
     jh::DevBuf ptm;  // N x 12 double
     jh::DevBuf geo;  // N x Geo
     jh::DevBuf uw;   // N x W float

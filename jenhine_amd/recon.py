@@ -362,6 +362,19 @@ class BPCore:
     def set_fetch(self, fetch):
         call("jh_bp_set_fetch", self._h, self.FETCH[fetch])
 
+    WEIGHT = {"count": 0, "fdk": 1}
+
+    def set_weight(self, weight="count", scale=0.0):
+        """"count": the reference's normalisation by the back-projected uW;
+        "fdk": the distance weight, vol = scale * sum_v bilinear(q_v) / w_v^2
+        (scale 0 selects 2 / n_views)."""
+        call("jh_bp_set_weight", self._h, self.WEIGHT[weight], ctypes.c_float(scale))
+
+    def weight(self):
+        m, sc = ctypes.c_int(), ctypes.c_float()
+        call("jh_bp_get_weight", self._h, ctypes.byref(m), ctypes.byref(sc))
+        return {v: k for k, v in self.WEIGHT.items()}[m.value], sc.value
+
     LAYOUT = {"quad": 0, "strip": 1}
 
     def set_layout(self, layout):
@@ -489,7 +502,7 @@ class FDK:
     W with yMirroring = detectorOnTheRight."""
 
     def __init__(self, scan, recon, ptm, filter_mode="fp32", storage="f32", device="cuda:0", zfilter=False,
-                 ct=None, composite=False):
+                 ct=None, composite=False, weight="count", weight_scale=0.0):
         self.scan, self.recon, self.storage = scan, recon, storage
         W, V = scan.NumUsedDetsU, scan.NumUsedDetsV
         self.pre = PreProcessingCore(scan.NumDetsU, scan.NumDetsV,
@@ -510,6 +523,8 @@ class FDK:
             self.bp.set_zfilter(True)
         if ct is not None:
             self.bp.set_ctnum(True, *ct)
+        if weight != "count":
+            self.bp.set_weight(weight, weight_scale)
 
     def fused(self):
         """Whether the filter writes the staged strips directly (filter and

@@ -510,6 +510,7 @@ typedef struct {
     const R* proj; const R* ptm; int N, W, V, X, Y, Z; R pxy, pz; int z0, nz;
     R* vol; R* acc; R* cnt; const R* uw; const R *ax, *ay, *az;
     int nthreads, tid;
+    int fdkw; R scale;  /* distance-weighted form (jo_bp_fdkw) */
 } FN(bp_args);
 
 static void* FN(bp_worker)(void* p)
@@ -558,6 +559,14 @@ static void* FN(bp_worker)(void* p)
                     uu = uu / ww; vv = vv / ww;
                     uu = uu + (R)0.5; vv = vv + (R)0.5;
                     R s = 0, c = 0;
+                    if (a->fdkw) {
+                        /* distance weight 1/w^2 of the sample, no count */
+                        if (FN(remap_sample)(q, W, V, uu, vv, &s)) {
+                            const R iw = (R)1 / ww;
+                            acc[i] = acc[i] + s * (iw * iw);
+                        }
+                        continue;
+                    }
                     if (FN(remap_sample)(q, W, V, uu, vv, &s)) FN(remap_sample)(ones, W, V, uu, vv, &c);
                     acc[i] = acc[i] + s;
                     cnt[i] = cnt[i] + c;
@@ -566,14 +575,15 @@ static void* FN(bp_worker)(void* p)
         }
         for (int k = t0; k < t1; ++k)
             for (size_t i = 0; i < XY; ++i)
-                a->vol[(size_t)k * XY + i] = a->acc[(size_t)k * XY + i] / a->cnt[(size_t)k * XY + i];
+                a->vol[(size_t)k * XY + i] = a->fdkw ? a->scale * a->acc[(size_t)k * XY + i]
+                                                     : a->acc[(size_t)k * XY + i] / a->cnt[(size_t)k * XY + i];
     }
     free(U); free(Vv); free(Wd); free(q); free(ones); free(xs); free(ys);
     return NULL;
 }
 
-int FN(jo_bp)(const R* proj, const R* ptm, int N, int W, int V, int X, int Y, int Z,
-              R pxy, R pz, int z0, int nz, R* vol, R* acc, R* cnt, int nthreads)
+static int FN(bp_run)(const R* proj, const R* ptm, int N, int W, int V, int X, int Y, int Z,
+                      R pxy, R pz, int z0, int nz, R* vol, R* acc, R* cnt, int nthreads, int fdkw, R scale)
 {
     if (z0 < 0 || nz < 0 || z0 + nz > Z || W < 2 || V < 2) return -1;
     const size_t XY = (size_t)X * Y;
@@ -589,7 +599,8 @@ int FN(jo_bp)(const R* proj, const R* ptm, int N, int W, int V, int X, int Y, in
     pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * nthreads);
     FN(bp_args)* args = (FN(bp_args)*)malloc(sizeof(FN(bp_args)) * nthreads);
     for (int t = 0; t < nthreads; ++t) {
-        FN(bp_args) a = {proj, ptm, N, W, V, X, Y, Z, pxy, pz, z0, nz, vol, accb, cntb, uw, ax, ay, az, nthreads, t};
+        FN(bp_args) a = {proj, ptm, N, W, V, X, Y, Z, pxy, pz, z0, nz, vol, accb, cntb, uw, ax, ay, az, nthreads, t,
+                         fdkw, scale};
         args[t] = a;
         pthread_create(&th[t], NULL, FN(bp_worker), &args[t]);
     }
@@ -598,6 +609,29 @@ int FN(jo_bp)(const R* proj, const R* ptm, int N, int W, int V, int X, int Y, in
     if (!acc) free(accb);
     if (!cnt) free(cntb);
     return 0;
+}
+
+int FN(jo_bp)(const R* proj, const R* ptm, int N, int W, int V, int X, int Y, int Z,
+              R pxy, R pz, int z0, int nz, R* vol, R* acc, R* cnt, int nthreads)
+{
+    return FN(bp_run)(proj, ptm, N, W, V, X, Y, Z, pxy, pz, z0, nz, vol, acc, cnt, nthreads, 0, (R)0);
+}
+
+/* The FDK distance-weighted form of the same backprojection, the weighting
+ * the reference replaces by the count normalisation (BPCore.cpp:147-151,
+ * 198-205; BPCUDACore.cu:62 accumulates unweighted samples):
+ *   acc += remap(q) * (1/w)^2 with w = Wd + Z*P10 the sample's third
+ *   homogeneous coordinate (1 at the iso-centre for the reference's
+ *   normalised matrices), a sample outside the detector adding nothing;
+ *   vol = scale * acc.  No count volume (cnt stays 0). */
+int FN(jo_bp_fdkw)(const R* proj, const R* ptm, int N, int W, int V, int X, int Y, int Z,
+                   R pxy, R pz, int z0, int nz, R scale, R* vol, R* acc, int nthreads)
+{
+    const size_t n = (size_t)X * Y * (nz > 0 ? nz : 1);
+    R* cnt = (R*)calloc(n, sizeof(R));
+    const int rc = FN(bp_run)(proj, ptm, N, W, V, X, Y, Z, pxy, pz, z0, nz, vol, acc, cnt, nthreads, 1, scale);
+    free(cnt);
+    return rc;
 }
 
 

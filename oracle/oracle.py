@@ -191,9 +191,11 @@ def composite(pre, ptm, dso, det_right, fp64=False, nthreads=8):
 
 
 def backproject(proj, ptm, vol_shape, pitch_xy, pitch_z, z0=0, nz=None, fp64=False, nthreads=8,
-                return_parts=False):
+                return_parts=False, fdk_scale=None):
     """proj: (N, V, W) filtered projections; vol_shape = (Z, Y, X).
-    Returns vol[z0:z0+nz] (and acc, count when return_parts)."""
+    Returns vol[z0:z0+nz] (and acc, count when return_parts).  With
+    fdk_scale the distance-weighted form: vol = fdk_scale * sum_v
+    remap(q_v) / w^2 (jo_bp_fdkw; count returned as zeros)."""
     proj = np.ascontiguousarray(proj, _dt(fp64))
     N, V, W = proj.shape
     ptm = np.ascontiguousarray(ptm, _dt(fp64)).reshape(N, 12)
@@ -204,9 +206,14 @@ def backproject(proj, ptm, vol_shape, pitch_xy, pitch_z, z0=0, nz=None, fp64=Fal
     acc = np.zeros_like(vol)
     cnt = np.zeros_like(vol)
     R = _real(fp64)
-    rc = getattr(lib(), "jo_bp" + _sfx(fp64))(
-        _p(proj), _p(ptm), N, W, V, X, Y, Z, R(pitch_xy), R(pitch_z), z0, nz,
-        _p(vol), _p(acc), _p(cnt), nthreads)
+    if fdk_scale is not None:
+        rc = getattr(lib(), "jo_bp_fdkw" + _sfx(fp64))(
+            _p(proj), _p(ptm), N, W, V, X, Y, Z, R(pitch_xy), R(pitch_z), z0, nz, R(fdk_scale),
+            _p(vol), _p(acc), nthreads)
+    else:
+        rc = getattr(lib(), "jo_bp" + _sfx(fp64))(
+            _p(proj), _p(ptm), N, W, V, X, Y, Z, R(pitch_xy), R(pitch_z), z0, nz,
+            _p(vol), _p(acc), _p(cnt), nthreads)
     assert rc == 0
     if return_parts:
         return vol, acc, cnt
@@ -214,7 +221,7 @@ def backproject(proj, ptm, vol_shape, pitch_xy, pitch_z, z0=0, nz=None, fp64=Fal
 
 
 def fdk(raw, ptm, scan, recon, fp64=False, nthreads=8, z0=0, nz=None, dark=None, gain=None,
-        return_stages=False, zfilter_v=False, ct=None, composite_views=False):
+        return_stages=False, zfilter_v=False, ct=None, composite_views=False, fdk_scale=None):
     """Full oracle chain PreProcessingCore [-> CompositeCore] -> FilterCore
     [-> ZFilterAgent] -> BPCore [-> CTNumCore].
 
@@ -244,7 +251,7 @@ def fdk(raw, ptm, scan, recon, fp64=False, nthreads=8, z0=0, nz=None, dark=None,
     if zfilter_v:
         f = zfilter(f, fp64=fp64)
     vol = backproject(f, ptm, (recon["Z"], recon["Y"], recon["X"]), recon["pxy"], recon["pz"],
-                      z0=z0, nz=nz, fp64=fp64, nthreads=nthreads)
+                      z0=z0, nz=nz, fp64=fp64, nthreads=nthreads, fdk_scale=fdk_scale)
     if ct is not None:
         vol = ctnum(vol, *ct, fp64=fp64)
     if return_stages:

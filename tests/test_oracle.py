@@ -282,6 +282,47 @@ def test_bp_linear_projection_single_view_is_exact():
     np.testing.assert_allclose(vol, (a + b * su + c * sv) / cnt, rtol=1e-12)
 
 
+def test_bp_fdk_weight_single_view_closed_form():
+    """The distance-weighted form on one view whose weighted image is affine:
+    every voxel holds scale * q(su, sv) / w^2 exactly, with w the third
+    homogeneous coordinate P[2] . (x, y, z, 1) of the voxel centre, computed
+    here from the matrix alone; voxels projecting off the detector hold 0
+    (no renormalisation).  A dropped or squared-twice weight, or w taken at
+    the iso-centre instead of the voxel, fails."""
+    W, V = 64, 48
+    ptm = geometry.circular_ptms(1, 300.0, 500.0, W, V, 0.5, start_deg=17.0)
+    a, b, c = 0.3, 0.01, -0.02
+    uw = o.uweights(ptm, W, fp64=True)[0]
+    uu, vv = np.meshgrid(np.arange(W), np.arange(V))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        proj = np.where(uu >= 1, (a + b * uu + c * vv) / uw[uu], 0.0)[None]
+    shape = (6, 60, 10)   # y extends past both detector edges
+    vol = o.backproject(proj, ptm, shape, 0.4, 0.4, fp64=True, fdk_scale=0.7)
+    ax = lambda n, p: (np.arange(n) - (n - 1) / 2) * p
+    zz, yy, xx = np.meshgrid(ax(6, 0.4), ax(60, 0.4), ax(10, 0.4), indexing="ij")
+    X4 = np.c_[xx.ravel(), yy.ravel(), zz.ravel(), np.ones(xx.size)]
+    uvw = X4 @ ptm[0].reshape(3, 4).T
+    su = (uvw[:, 0] / uvw[:, 2] + 0.5).reshape(shape)
+    sv = (uvw[:, 1] / uvw[:, 2] + 0.5).reshape(shape)
+    w = uvw[:, 2].reshape(shape)
+    inside = (su >= 0) & (su <= W - 1) & (sv >= 0) & (sv <= V - 1)
+    affine = inside & (su >= 1.0)          # column 0 of the image is 0, not affine
+    assert affine.sum() > 500 and (~inside).sum() > 100
+    assert np.abs(w - 1).max() > 1e-3   # the weight is not trivially 1
+    np.testing.assert_allclose(vol[affine], (0.7 * (a + b * su + c * sv) / w ** 2)[affine], rtol=1e-12, atol=1e-15)
+    assert (vol[~inside] == 0).all()
+
+
+def test_bp_fdk_weight_fp32_matches_fp64():
+    w = configs.CBCT128.scaled(n_views=12, X=24, Y=24, Z=8)
+    rng = np.random.default_rng(5)
+    proj = rng.standard_normal((12, w.n_v, w.n_u))
+    v64 = o.backproject(proj, w.ptms(), (8, 24, 24), w.pxy, w.pz, fp64=True, fdk_scale=2.0 / 12)
+    v32 = o.backproject(proj.astype(np.float32), w.ptms(), (8, 24, 24), w.pxy, w.pz, fdk_scale=2.0 / 12)
+    # zero-mean random data (as the count form's test): the sums cancel
+    assert rel_rmse(v32, v64) < 5e-5
+
+
 def test_bp_constant_projection_gives_constant_volume():
     w = configs.CBCT128.scaled(n_views=16, X=32, Y=32, Z=16)
     proj = np.full((16, w.n_v, w.n_u), 2.5, np.float32)
