@@ -108,6 +108,13 @@ jh_status jh_pre_create(jh_pre_plan** plan, int device, int inW, int inH, int bU
 jh_status jh_pre_run(jh_pre_plan* plan, const uint16_t* d_raw, const uint16_t* d_dark,
                      const float* d_gain, float* d_out, int nviews, jh_stream_t stream);
 
+/* An upper bound of |preprocessed value| for gain maps within [gainMin,
+ * gainMax] (no gain map: 1, 1): the corrected count lies in [1, 65535], so
+ * -ln(f / (BrightField g)) and its beam-hardening polynomial are bounded
+ * over that range.  Used to choose the FP16 storage exponent
+ * (jh_bp_set_f16_exponent).  No counterpart in the reference. */
+jh_status jh_pre_output_bound(const jh_pre_plan* plan, double gainMin, double gainMax, double* bound);
+
 jh_status jh_pre_destroy(jh_pre_plan* plan);
 
 /* ---------------------------------------------------------------------- */
@@ -187,6 +194,12 @@ jh_status jh_pre_run_operand(jh_pre_plan* pre, jh_filter_plan* plan, const uint1
                              const float* d_gain, int view0, int nviews, float* d_a, jh_stream_t stream);
 jh_status jh_filter_run_operand_staged(jh_filter_plan* plan, jh_bp_plan* bp, const float* d_a, int view0,
                                        int nviews, int dtype, void* d_q, jh_stream_t stream);
+
+/* outBound = inBound * max_n sum_k |B[k][n]|: a bound of the filtered
+ * values for inputs bounded by inBound (the pre-weight is at most 1; B is
+ * the Toeplitz operand with the truncation-ramp rows, in FP64).  No
+ * counterpart in the reference. */
+jh_status jh_filter_output_bound(const jh_filter_plan* plan, double inBound, double* outBound);
 
 jh_status jh_filter_destroy(jh_filter_plan* plan);
 
@@ -351,6 +364,19 @@ jh_status jh_bp_set_zfilter(jh_bp_plan* plan, int enable);
  * min(8191, (x - 300) * norm1 + 300) (JEngine.cpp:297-305; MuWater defaults to
  * 0.02, Config/ConfigHelper.cpp:70). */
 jh_status jh_bp_set_ctnum(jh_bp_plan* plan, int enable, float norm0, float norm1, float muWater);
+
+/* FP16 storage range: jh_bp_stage and the fused filter staging store
+ * q * 2^-exponent in FP16, and the outputs of FP16 storage (jh_bp_run,
+ * jh_bp_finalize of FP16 chunks) are multiplied back by 2^exponent.  Both
+ * scalings are exact, so a result whose staged values are within FP16's
+ * normal range is bit-identical to exponent 0; a host picks the exponent so
+ * that its bound of |q| (jh_pre_output_bound, jh_filter_output_bound; uW <=
+ * 1) maps to at most 2^15, which rules out overflow (FP16's largest finite
+ * value is 65504).  Set it before staging: chunks accumulated since the last
+ * jh_bp_reset must share one exponent.  Range [-64, 64]; default 0.  No
+ * counterpart in the reference (it stores FP32 textures). */
+jh_status jh_bp_set_f16_exponent(jh_bp_plan* plan, int exponent);
+jh_status jh_bp_get_f16_exponent(jh_bp_plan* plan, int* exponent);
 
 /* Voxel weighting of the backprojection outputs (jh_bp_run,
  * jh_bp_accumulate + jh_bp_finalize).  JH_WEIGHT_COUNT (the default) is the

@@ -23,17 +23,18 @@ DTYPES = {"f32": JH_F32, "bf16": JH_BF16, "f16": JH_F16}
 # Every symbol include/jhip.h declares (tests check the library exports all).
 EXPORTS = [
     "jh_last_error", "jh_version", "jh_device_count", "jh_ptm_modify",
-    "jh_pre_create", "jh_pre_run", "jh_pre_destroy",
+    "jh_pre_create", "jh_pre_run", "jh_pre_output_bound", "jh_pre_destroy",
     "jh_filter_create", "jh_filter_create_ex", "jh_filter_shape", "jh_filter_response", "jh_filter_preweight",
     "jh_filter_run", "jh_filter_stage_fusable", "jh_filter_run_staged", "jh_filter_operand_bytes",
-    "jh_pre_run_operand", "jh_filter_run_operand_staged", "jh_filter_destroy",
+    "jh_pre_run_operand", "jh_filter_run_operand_staged", "jh_filter_output_bound", "jh_filter_destroy",
     "jh_detector_on_right", "jh_composite_create", "jh_composite_side", "jh_composite_run", "jh_composite_run_operand",
     "jh_composite_destroy",
     "jh_bp_create", "jh_bp_uweights", "jh_bp_stage_bytes", "jh_bp_stage", "jh_bp_run", "jh_bp_reset",
     "jh_bp_accumulate", "jh_bp_finalize", "jh_bp_set_tile", "jh_bp_get_tile", "jh_bp_geometry_mode",
     "jh_bp_destroy", "jh_bp_pack", "jh_bp_backproject", "jh_fp_project", "jh_bp_autotune",
     "jh_bp_set_fetch", "jh_bp_get_fetch", "jh_bp_set_layout", "jh_bp_get_layout", "jh_bp_set_zfilter",
-    "jh_bp_set_ctnum", "jh_bp_set_weight", "jh_bp_get_weight", "jh_zfilter_run",
+    "jh_bp_set_ctnum", "jh_bp_set_weight", "jh_bp_get_weight", "jh_bp_set_f16_exponent", "jh_bp_get_f16_exponent",
+    "jh_zfilter_run",
     "jh_ctnum_run",
     "jh_graph_create", "jh_graph_add_scan", "jh_graph_autotune", "jh_graph_scan_tuning", "jh_graph_instantiate",
     "jh_graph_launch", "jh_graph_destroy",
@@ -104,6 +105,10 @@ _SIGS = {
     "jh_bp_set_zfilter": (c_int, [P, c_int]),
     "jh_bp_set_ctnum": (c_int, [P, c_int, c_float, c_float, c_float]),
     "jh_bp_set_weight": (c_int, [P, c_int, c_float]),
+    "jh_bp_set_f16_exponent": (c_int, [P, c_int]),
+    "jh_bp_get_f16_exponent": (c_int, [P, P]),
+    "jh_pre_output_bound": (c_int, [P, c_double, c_double, P]),
+    "jh_filter_output_bound": (c_int, [P, c_double, P]),
     "jh_bp_get_weight": (c_int, [P, P, P]),
     "jh_zfilter_run": (c_int, [P, P, c_int, c_int, c_int, P]),
     "jh_ctnum_run": (c_int, [P, c_size_t, c_float, c_float, c_float, P]),

@@ -111,6 +111,16 @@ OFFSET_DENTAL512 = Workload("offset_dental512", 350.0, 600.0, 600, 768, 768, 0.2
 WORKLOADS = {w.name: w for w in (CBCT128, DENTAL512, ADJOINT256, OFFSET128, OFFSET_DENTAL512)}
 
 
+def batch_scan(name, n, views, det, seed):
+    """One scan of the batched form: n^3 at 0.25 mm, SOD 300, SDD 500,
+    Ram-Lak, 10 ellipsoids; the detector pitch is chosen so the volume fits
+    the field of view."""
+    fov = n * 0.25 * 1.25
+    du = fov * (500.0 / 300.0) / det
+    return Workload(name, 300.0, 500.0, views, det, det, float(du), n, n, n, 0.25, 0.25,
+                    n_ellipsoids=10, window=0, ramlak=True, seed=seed)
+
+
 def batch16(seed=4):
     """16 scans of uneven sizes: volumes from {128..400}^3, views from
     {180, 360, 540, 720}, detectors from {256..1024}^2."""
@@ -123,9 +133,5 @@ def batch16(seed=4):
         n = int(rng.choice(vols))
         d = int(rng.choice(dets))
         v = int(rng.choice(views))
-        # detector pitch chosen so the volume fits the field of view
-        fov = n * 0.25 * 1.25
-        du = fov * (500.0 / 300.0) / d
-        out.append(Workload(f"batch{s}", 300.0, 500.0, v, d, d, float(du), n, n, n, 0.25, 0.25,
-                            n_ellipsoids=10, window=0, ramlak=True, seed=seed * 100 + s))
+        out.append(batch_scan(f"batch{s}", n, v, d, seed * 100 + s))
     return out

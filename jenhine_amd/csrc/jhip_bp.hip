@@ -185,10 +185,12 @@ constexpr int kStageRun = 16;
 template <int QT, bool strip>
 __global__ void __launch_bounds__(256) bp_stage_kernel(const float* __restrict__ filt,
                                                        const float* __restrict__ uw, int W, int V, int zf,
-                                                       void* __restrict__ out)
+                                                       float qs, void* __restrict__ out)
 // uw == NULL packs the frames without weighting; zf != 0 applies the v
 // smoothing of ZFilterAgent to the frames first (before the uW weight, as in
-// the reference chain Filter -> ZFilter -> BPCore::InitView).
+// the reference chain Filter -> ZFilter -> BPCore::InitView).  qs (a power of
+// two) scales the staged values; uW * qs and x * (uW qs) = (x uW) qs are
+// exact in the normal range, so the values differ from qs = 1 by qs alone.
 {
     // every staged value is rounded as written: q = RN(x uW), and the FP32
     // quad's differences RN(q[u+1] - q[u]) are not fused with the products
@@ -205,8 +207,8 @@ __global__ void __launch_bounds__(256) bp_stage_kernel(const float* __restrict__
     const float* w = uw ? uw + view * W : nullptr;
     const bool right = u + 1 < W;
     const int u1 = right ? u + 1 : u;
-    const float wa = w ? w[u] : 1.0f;
-    const float wb = w && right ? w[u + 1] : 1.0f;
+    const float wa = (w ? w[u] : 1.0f) * qs;
+    const float wb = (w && right ? w[u + 1] : 1.0f) * qs;
     // rows row0-1+j of columns u (and u+1 for quads), clamped into the frame
     float xa[NX], xb[NX];
 #pragma unroll
@@ -531,6 +533,7 @@ struct BPArgs {
     int pxb;             // bytes per staged pixel (16: FP32 quads; 8: BF16/FP16 quads and strips)
     int row_px;          // staged pixels per detector row (W, or W + 1 for strips)
     float wscale;        // distance-weighted outputs: vol = wscale * acc
+    float qscale;        // normalised outputs are multiplied by it (2^f16exp for FP16 storage)
 };
 
 constexpr int kZB = 8;  // samples whose loads are issued together
@@ -806,7 +809,7 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
     for (int kk = 0; kk < ZC; ++kk) {
         if (k0 + kk < a.Z) {
             if constexpr (OUT == kOutNormalised || OUT == kOutFdk) {
-                const float v = OUT == kOutFdk ? a.wscale * acc[kk] : acc[kk] / cnt[kk];
+                const float v = (OUT == kOutFdk ? a.wscale * acc[kk] : acc[kk] / cnt[kk]) * a.qscale;
                 a.vol[idx] = a.ct ? ct_number(v, a.ct_scale, a.ct_norm1) : v;
             } else if constexpr (OUT == kOutAccumulate) {
                 a.acc[idx] += acc[kk];
@@ -1114,7 +1117,7 @@ __device__ __forceinline__ void bp_tile_lds(const BPArgs& a, unsigned tile)
     for (int kk = 0; kk < ZC; ++kk) {
         if (k0 + kk < a.Z) {
             if (OUT == kOutNormalised) {
-                const float v = acc[kk] / cnt[kk];
+                const float v = acc[kk] / cnt[kk] * a.qscale;
                 a.vol[idx] = a.ct ? ct_number(v, a.ct_scale, a.ct_norm1) : v;
             } else if (OUT == kOutAccumulate) {
                 a.acc[idx] += acc[kk];
@@ -1159,10 +1162,10 @@ __global__ void ctnum_kernel(float* __restrict__ x, size_t n, float scale, float
 
 __global__ void bp_finalize_kernel(const float* __restrict__ acc, const float* __restrict__ cnt,
                                    float* __restrict__ vol, size_t n, int ct, float ct_scale, float ct_norm1,
-                                   int weighted, float wscale)
+                                   int weighted, float wscale, float qscale)
 {
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
-        const float v = weighted ? wscale * acc[t] : acc[t] / cnt[t];
+        const float v = (weighted ? wscale * acc[t] : acc[t] / cnt[t]) * qscale;
         vol[t] = ct ? ct_number(v, ct_scale, ct_norm1) : v;
     }
 }
@@ -1171,6 +1174,11 @@ __global__ void fill_zero_kernel(float* __restrict__ p, size_t n)
 {
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x)
         p[t] = 0.0f;
+}
+
+float bp_stage_scale(const jh_bp_plan* p, int dtype)
+{
+    return dtype == JH_F16 ? std::ldexp(1.0f, -p->f16exp) : 1.0f;
 }
 
 static unsigned grid_for(size_t n, unsigned block)
@@ -1247,6 +1255,7 @@ static void launch_bp(jh_bp_plan* p, const void* q, int dtype, int view0, int nv
     a.ct_scale = p->ct_scale;
     a.ct_norm1 = p->ct_norm1;
     a.wscale = p->wscale;
+    a.qscale = dtype == JH_F16 ? std::ldexp(1.0f, p->f16exp) : 1.0f;
     // workgroup extent: 8 x 8 voxel columns per wave (see bp_tile)
     const int ex = p->ty >= 2 ? 16 : 8, ey = p->ty >= 4 ? 16 : 8;
     a.ntx = (p->X + ex - 1) / ex;
@@ -1313,6 +1322,9 @@ jh_status jh_bp_create(jh_bp_plan** plan, int device, const double* h_ptm, int n
             p->ct_norm1 = 1.0f;
             p->weight = JH_WEIGHT_COUNT;
             p->wscale = 2.0f / (float)nviews;
+            p->f16exp = 0;
+            p->acc_scale = 1.0f;
+            p->acc_dtype = -1;
             p->ptm.alloc(sizeof(double) * 12 * nviews);
             p->geo.alloc(sizeof(Geo) * nviews);
             p->uw.alloc(sizeof(float) * (size_t)nviews * W);
@@ -1377,6 +1389,7 @@ static void stage(jh_bp_plan* p, const float* d_filtered, int view0, int nviews,
         require(p->V <= 65535, "jh_bp_stage: more than 65535 detector rows");
         hipStream_t s = as_stream(stream);
         const int zf = weighted ? p->zfilter : 0;
+        const float qs = bp_stage_scale(p, dtype);
         const size_t fin = (size_t)p->W * p->V, fout = stage_frame_bytes(p, dtype);
         const int st = stage_strip(p, dtype) ? 1 : 0;
         for (int c0 = 0; c0 < nviews; c0 += 65535) {  // grid z: frames
@@ -1387,13 +1400,13 @@ static void stage(jh_bp_plan* p, const float* d_filtered, int view0, int nviews,
             const dim3 grid((unsigned)((p->W + 63) / 64), (unsigned)((p->V + 4 * kStageRun - 1) / (4 * kStageRun)),
                             (unsigned)n);
             const dim3 block(64, 4);
-            if (dtype == JH_F32) bp_stage_kernel<JH_F32, false><<<grid, block, 0, s>>>(in, uw, p->W, p->V, zf, out);
+            if (dtype == JH_F32) bp_stage_kernel<JH_F32, false><<<grid, block, 0, s>>>(in, uw, p->W, p->V, zf, qs, out);
             else if (dtype == JH_BF16 && st)
-                bp_stage_kernel<JH_BF16, true><<<grid, block, 0, s>>>(in, uw, p->W, p->V, zf, out);
+                bp_stage_kernel<JH_BF16, true><<<grid, block, 0, s>>>(in, uw, p->W, p->V, zf, qs, out);
             else if (dtype == JH_BF16)
-                bp_stage_kernel<JH_BF16, false><<<grid, block, 0, s>>>(in, uw, p->W, p->V, zf, out);
-            else if (st) bp_stage_kernel<JH_F16, true><<<grid, block, 0, s>>>(in, uw, p->W, p->V, zf, out);
-            else bp_stage_kernel<JH_F16, false><<<grid, block, 0, s>>>(in, uw, p->W, p->V, zf, out);
+                bp_stage_kernel<JH_BF16, false><<<grid, block, 0, s>>>(in, uw, p->W, p->V, zf, qs, out);
+            else if (st) bp_stage_kernel<JH_F16, true><<<grid, block, 0, s>>>(in, uw, p->W, p->V, zf, qs, out);
+            else bp_stage_kernel<JH_F16, false><<<grid, block, 0, s>>>(in, uw, p->W, p->V, zf, qs, out);
         }
         check_launch("bp_stage_kernel");
     }
@@ -1452,6 +1465,8 @@ jh_status jh_bp_reset(jh_bp_plan* p, jh_stream_t stream)
         fill_zero_kernel<<<grid_for(n, 256), 256, 0, s>>>(p->acc.as<float>(), n);
         fill_zero_kernel<<<grid_for(n, 256), 256, 0, s>>>(p->cnt.as<float>(), n);
         check_launch("fill_zero_kernel");
+        p->acc_scale = 1.0f;
+        p->acc_dtype = -1;
     });
 }
 
@@ -1464,6 +1479,13 @@ jh_status jh_bp_accumulate(jh_bp_plan* p, const void* d_q, int dtype, int view0,
         require(view0 >= 0 && nviews >= 0 && view0 + nviews <= p->N, "jh_bp_accumulate: view range outside the scan");
         require(p->acc.get() != nullptr, "jh_bp_accumulate: call jh_bp_reset first");
         if (nviews == 0) return;
+        // the accumulators hold one output scale: every chunk since the
+        // reset must be staged alike
+        const float sc = dtype == JH_F16 ? std::ldexp(1.0f, p->f16exp) : 1.0f;
+        require(p->acc_dtype < 0 || sc == p->acc_scale,
+                "jh_bp_accumulate: chunks of differently scaled storage since the last reset");
+        p->acc_scale = sc;
+        p->acc_dtype = dtype;
         DeviceScope ds(p->device);
         launch_bp(p, d_q, dtype, view0, nviews, nullptr,
                   p->weight == JH_WEIGHT_FDK ? kOutFdkAccumulate : kOutAccumulate, as_stream(stream));
@@ -1482,7 +1504,7 @@ jh_status jh_bp_finalize(jh_bp_plan* p, float* d_vol, int z0, int nz, jh_stream_
         const size_t n = plane * nz;
         bp_finalize_kernel<<<grid_for(n, 256), 256, 0, as_stream(stream)>>>(
             p->acc.as<float>() + plane * z0, p->cnt.as<float>() + plane * z0, d_vol, n, p->ct, p->ct_scale,
-            p->ct_norm1, p->weight == JH_WEIGHT_FDK ? 1 : 0, p->wscale);
+            p->ct_norm1, p->weight == JH_WEIGHT_FDK ? 1 : 0, p->wscale, p->acc_scale);
         check_launch("bp_finalize_kernel");
     });
 }
@@ -1537,6 +1559,23 @@ jh_status jh_bp_set_zfilter(jh_bp_plan* p, int enable)
     return guard([&] {
         require(p != nullptr, "jh_bp_set_zfilter: plan is NULL");
         p->zfilter = enable ? 1 : 0;
+    });
+}
+
+jh_status jh_bp_set_f16_exponent(jh_bp_plan* p, int exponent)
+{
+    return guard([&] {
+        require(p != nullptr, "jh_bp_set_f16_exponent: plan is NULL");
+        require(exponent >= -64 && exponent <= 64, "jh_bp_set_f16_exponent: exponent outside [-64, 64]");
+        p->f16exp = exponent;
+    });
+}
+
+jh_status jh_bp_get_f16_exponent(jh_bp_plan* p, int* exponent)
+{
+    return guard([&] {
+        require(p != nullptr && exponent != nullptr, "jh_bp_get_f16_exponent: bad arguments");
+        *exponent = p->f16exp;
     });
 }
 

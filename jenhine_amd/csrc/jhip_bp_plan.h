@@ -27,6 +27,9 @@ struct jh_bp_plan {
     float ct_scale, ct_norm1;
     int weight;      // JH_WEIGHT_COUNT (the reference's normalisation) or JH_WEIGHT_FDK
     float wscale;    // JH_WEIGHT_FDK: vol = wscale * sum of samples / w^2
+    int f16exp;      // FP16 storage holds q * 2^-f16exp; outputs are scaled back by 2^f16exp
+    float acc_scale; // output scale of what the accumulators hold (2^f16exp of FP16 chunks, else 1)
+    int acc_dtype;   // storage type of the chunks accumulated since the last reset (-1: none)
     jh::DevBuf ptm;  // N x 12 double
     jh::DevBuf geo;  // N x Geo
     jh::DevBuf uw;   // N x W float

@@ -274,6 +274,39 @@ jh_status jh_ptm_modify(double* d_ptm, int n, int op, double value, jh_stream_t 
     });
 }
 
+jh_status jh_pre_output_bound(const jh_pre_plan* p, double gainMin, double gainMax, double* bound)
+{
+    return guard([&] {
+        require(p != nullptr && bound != nullptr, "jh_pre_output_bound: bad arguments");
+        require(gainMin > 0.0 && gainMax >= gainMin && std::isfinite(gainMax), "jh_pre_output_bound: bad gain range");
+        // the corrected count f lies in [1, 65535] (raw - dark clamped below
+        // at 1; a resize interpolates between such values), so
+        // p = -ln(f / (BF g)) lies in [ln(BF gmin / 65535), ln(BF gmax)]
+        const double bf = (double)p->brightField;
+        const double lo = std::log(bf * gainMin / 65535.0), hi = std::log(bf * gainMax);
+        double m = std::max(std::fabs(lo), std::fabs(hi));
+        if (p->doBH) {
+            // |p (a0 + a1 p + a2 p^2)| over [lo, hi]: the cubic's extremes are
+            // at the ends or where 3 a2 p^2 + 2 a1 p + a0 = 0
+            const double a0 = p->bh[0], a1 = p->bh[1], a2 = p->bh[2];
+            auto c = [&](double x) { return std::fabs(x * (a0 + a1 * x + a2 * x * x)); };
+            m = std::max(c(lo), c(hi));
+            const double A = 3.0 * a2, B = 2.0 * a1, C = a0;
+            if (A != 0.0) {
+                const double d = B * B - 4.0 * A * C;
+                if (d >= 0.0) {
+                    for (double x : {(-B + std::sqrt(d)) / (2.0 * A), (-B - std::sqrt(d)) / (2.0 * A)})
+                        if (x > lo && x < hi) m = std::max(m, c(x));
+                }
+            } else if (B != 0.0) {
+                const double x = -C / B;
+                if (x > lo && x < hi) m = std::max(m, c(x));
+            }
+        }
+        *bound = m * (1.0 + 1e-5);  // float evaluation of the chain
+    });
+}
+
 jh_status jh_pre_create(jh_pre_plan** plan, int device, int inW, int inH, int bUp, int bDown,
                         int bLeft, int bRight, int outW, int outH, float brightField,
                         const float* bh)

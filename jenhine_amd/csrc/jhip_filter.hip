@@ -120,6 +120,7 @@ struct FiltArgs {
     void* q;             // NULL: write the filtered FP32 rows to out
     const float* uw;     // uW of the launch's first view onwards, Wout per view
     int qtype;           // JH_BF16 or JH_F16
+    float qs;            // staged values are q * qs (qs = 2^-f16exp for FP16, exact; else 1)
     int tiled;           // x is the pre-weighted A operand in the MFMA tile order (pre_operand_kernel)
 };
 
@@ -401,7 +402,9 @@ __device__ void stage_strips_epilogue(const FiltArgs& a, const f32x4_t (&acc)[2]
 #pragma unroll
                 for (int c = 0; c < 8; ++c) {
                     const int col = C0 + c * 16 + (lane & 15);
-                    uwc[c] = col < a.Wout ? uwv[col] : 0.0f;
+                    // uW * qs is exact (qs a power of two), and so is
+                    // acc * (uW qs) = (acc uW) qs in the normal range
+                    uwc[c] = col < a.Wout ? uwv[col] * a.qs : 0.0f;
                 }
             }
 #pragma unroll
@@ -806,6 +809,14 @@ jh_status create_filter(jh_filter_plan** plan, int device, int W, int V, int Wou
                 for (int n = 0; n < Wout; ++n) Bh[(size_t)m * Wout + n] = (float)hc((long)n + off - m);
             for (int e = 0; e < 3; ++e)
                 for (int n = 0; n < Wout; ++n) Bh[(size_t)(W + e) * Wout + n] = (float)(rr[n] / 3.0);
+            // the operator's infinity norm over its inputs (the pre-weight is
+            // at most 1): |out[n]| <= gain_bound * max|in|
+            p->gain_bound = 0.0;
+            for (int n = 0; n < Wout; ++n) {
+                double s = std::fabs(rr[n]);
+                for (int m = 0; m < W; ++m) s += std::fabs(hc((long)n + off - m));
+                p->gain_bound = std::max(p->gain_bound, s);
+            }
             p->B.alloc(sizeof(float) * Bh.size());
             JH_HIP(hipMemcpy(p->B.get(), Bh.data(), sizeof(float) * Bh.size(), hipMemcpyHostToDevice));
             if (mode != JH_FILTER_FP32_VALU) {
@@ -895,7 +906,7 @@ bool stage_fusable(const jh_filter_plan* p, const jh_bp_plan* bp, int dtype)
 // One launch of the filter over views view0 .. view0+nviews-1 of the scan:
 // FP32 rows to d_out, or (d_q != NULL) fused staging into column strips.
 void run_filter(jh_filter_plan* p, const float* d_in, float* d_out, int view0, int nviews, jh_stream_t stream,
-                void* d_q, const float* uw, int qtype, int tiled = 0)
+                void* d_q, const float* uw, int qtype, int tiled = 0, float qs = 1.0f)
 {
     require(view0 >= 0 && nviews >= 0 && view0 + nviews <= p->N, "jh_filter_run: view range outside the scan");
     if (nviews == 0) return;
@@ -916,6 +927,7 @@ void run_filter(jh_filter_plan* p, const float* d_in, float* d_out, int view0, i
     a.q = d_q;
     a.uw = uw;
     a.qtype = qtype;
+    a.qs = qs;
     a.tiled = tiled;
     require((size_t)p->V * nviews < (1u << 31), "jh_filter_run: too many rows");
     hipStream_t st = as_stream(stream);
@@ -981,6 +993,15 @@ jh_status jh_filter_shape(const jh_filter_plan* p, int* Win, int* Wout, int* yMi
     });
 }
 
+jh_status jh_filter_output_bound(const jh_filter_plan* p, double inBound, double* outBound)
+{
+    return guard([&] {
+        require(p != nullptr && outBound != nullptr, "jh_filter_output_bound: bad arguments");
+        require(inBound >= 0.0 && std::isfinite(inBound), "jh_filter_output_bound: bad input bound");
+        *outBound = inBound * p->gain_bound;
+    });
+}
+
 jh_status jh_filter_response(jh_filter_plan* p, double* h_H, int* L)
 {
     return guard([&] {
@@ -1029,7 +1050,8 @@ jh_status jh_filter_run_staged(jh_filter_plan* p, jh_bp_plan* bp, const float* d
         require(stage_fusable(p, bp, dtype),
                 "jh_filter_run_staged: the plans do not fuse (MFMA filter writing the backprojector's W columns, "
                 "BF16/FP16 column strips, no ZFilter, same device, frame size and view count)");
-        run_filter(p, d_in, nullptr, view0, nviews, stream, d_q, bp->uw.as<float>() + (size_t)view0 * bp->W, dtype);
+        run_filter(p, d_in, nullptr, view0, nviews, stream, d_q, bp->uw.as<float>() + (size_t)view0 * bp->W, dtype, 0,
+                   bp_stage_scale(bp, dtype));
     });
 }
 
@@ -1091,7 +1113,7 @@ jh_status jh_filter_run_operand_staged(jh_filter_plan* p, jh_bp_plan* bp, const 
                 "jh_filter_run_operand_staged: the plans do not fuse (MFMA filter writing the backprojector's W "
                 "columns, BF16/FP16 column strips, no ZFilter, same device, frame size and view count)");
         run_filter(p, d_a, nullptr, view0, nviews, stream, d_q, bp->uw.as<float>() + (size_t)view0 * bp->W, dtype,
-                   1);
+                   1, bp_stage_scale(bp, dtype));
     });
 }
 

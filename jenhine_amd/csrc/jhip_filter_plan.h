@@ -12,6 +12,7 @@ struct jh_filter_plan {
     int Wout, mirror;       // output width, yMirroring
     int mode;
     int nblocks;
+    double gain_bound;            // max over output columns of sum_k |B[k][n]| (FP64): |out| <= gain_bound * max|in|
     std::vector<int> view_block;  // block index of every view
     std::vector<double> H;        // L/2 + 1
     jh::DevBuf B;                 // Kp x Wout float (vector-ALU form)

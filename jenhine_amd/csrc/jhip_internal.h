@@ -151,6 +151,10 @@ std::vector<double> design_filter(int W, double hsr, double cutStart, double cut
                                   const std::vector<double>& adjPts,
                                   const std::vector<double>& adjDB, int window);
 
+// Factor of the values a backprojection plan stages in `dtype`: 2^-f16exp for
+// FP16 (exact), 1 otherwise.
+float bp_stage_scale(const jh_bp_plan* p, int dtype);
+
 // Device and frame shape of a filter plan (W x V frames, nviews views).
 void filter_plan_shape(const jh_filter_plan* p, int* device, int* W, int* V, int* nviews);
 

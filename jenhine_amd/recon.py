@@ -77,6 +77,12 @@ class PreProcessingCore:
         call("jh_pre_run", self._h, _ptr(raw), _ptr(dark), _ptr(gain), _ptr(out), n, _stream())
         return out
 
+    def output_bound(self, gain_min=1.0, gain_max=1.0):
+        """Upper bound of |preprocessed value| for gains in [gain_min, gain_max]."""
+        b = ctypes.c_double()
+        call("jh_pre_output_bound", self._h, ctypes.c_double(gain_min), ctypes.c_double(gain_max), ctypes.byref(b))
+        return b.value
+
     def resizes(self):
         bu, bd, bl, br = self.borders
         return (self.in_w - bl - br, self.in_h - bu - bd) != (self.out_w, self.out_h)
@@ -133,6 +139,12 @@ class FilterCore:
              ctypes.c_float(half_sample_rate), ctypes.c_float(cut_start), ctypes.c_float(cut_end),
              pts if n else None, db if n else None, n, window, m)
         self._h = h
+
+    def output_bound(self, in_bound):
+        """Upper bound of |filtered value| for inputs bounded by in_bound."""
+        b = ctypes.c_double()
+        call("jh_filter_output_bound", self._h, ctypes.c_double(in_bound), ctypes.byref(b))
+        return b.value
 
     def response(self):
         L = ctypes.c_int()
@@ -362,6 +374,15 @@ class BPCore:
     def set_fetch(self, fetch):
         call("jh_bp_set_fetch", self._h, self.FETCH[fetch])
 
+    def set_f16_exponent(self, exponent):
+        """FP16 storage holds q * 2^-exponent; outputs are scaled back."""
+        call("jh_bp_set_f16_exponent", self._h, int(exponent))
+
+    def f16_exponent(self):
+        k = ctypes.c_int()
+        call("jh_bp_get_f16_exponent", self._h, ctypes.byref(k))
+        return k.value
+
     WEIGHT = {"count": 0, "fdk": 1}
 
     def set_weight(self, weight="count", scale=0.0):
@@ -469,6 +490,15 @@ class BPCore:
                 pass
 
 
+def f16_exponent_for(bound):
+    """The FP16 storage exponent k that maps values bounded by `bound` into
+    [0, 2^15]: FP16's largest finite value is 65504."""
+    if not bound > 0.0:
+        return 0
+    m, e = np.frexp(bound)  # bound = m 2^e, 0.5 <= m < 1: bound <= 2^e
+    return int(min(64, max(-64, e - 15)))
+
+
 def zfilter(frames, out=None):
     """ZFilterAgent (FilterAgent/ZFilterAgent.cpp:49-71): 1-2-1 smoothing
     along v with clamped edges, frames (N, V, W) f32 on the device."""
@@ -525,6 +555,23 @@ class FDK:
             self.bp.set_ctnum(True, *ct)
         if weight != "count":
             self.bp.set_weight(weight, weight_scale)
+        self._f16_gain = ()  # the gain tensor the FP16 exponent was chosen for
+
+    def prepare(self, gain=None):
+        """FP16 storage: chooses the storage exponent from an a-priori bound
+        of |q| (preprocessing bound for this gain map's range, times the
+        filter's gain bound; uW <= 1), so no staged value can overflow.  Reads
+        the gain map's range once per gain tensor."""
+        if self.storage != "f16" or (self._f16_gain and self._f16_gain[0] is gain):
+            return
+        if gain is None:
+            gmin = gmax = 1.0
+        else:
+            lo, hi = torch.aminmax(gain)
+            gmin, gmax = float(lo), float(hi)
+        bound = self.filter.output_bound(self.pre.output_bound(gmin, gmax))
+        self.bp.set_f16_exponent(f16_exponent_for(bound))
+        self._f16_gain = (gain,)
 
     def fused(self):
         """Whether the filter writes the staged strips directly (filter and
@@ -543,6 +590,7 @@ class FDK:
         return self.composite is not None and self.filter.mode != "fp32valu" and self.fused()
 
     def __call__(self, raw, dark=None, gain=None, return_stages=False):
+        self.prepare(gain)
         if not return_stages and self.operand_path() and raw.shape[0] * self.filter.V <= 1048560:
             a = self.pre.process_operand(raw, self.filter, dark, gain)
             q = self.filter.process_operand_staged(a, raw.shape[0], self.bp, self.storage)
@@ -584,6 +632,7 @@ class BatchGraph:
         _require_cuda(vol, "vol", torch.float32)
         _require_cuda(dark, "dark")
         _require_cuda(gain, "gain", torch.float32)
+        fdk.prepare(gain)
         idx = ctypes.c_int()
         call("jh_graph_add_scan", self._h, fdk.pre._h, fdk.filter._h, fdk.bp._h, _ptr(raw), _ptr(dark),
              _ptr(gain), DTYPES[fdk.storage], views_per_chunk, _ptr(vol), ctypes.byref(idx))

@@ -1,0 +1,98 @@
+"""The dental workload at full size (512^3 at 0.15 mm from 600 views of
+768^2 u16 at 0.2 mm, SOD 350, SDD 600, Shepp-Logan window, cubic beam
+hardening, 40 ellipsoids plus a 0.3/mm tooth) through the whole chain in the
+three forms the benchmark times, against the CPU oracle on 8 slices: 4 at
+the centre and 2 at each z edge, where the columns leave the detector rows
+within a tile (the partial-view path of the backprojector).
+
+Row step of this geometry: 0.15 mm * (600 / 350) / 0.2 mm = 1.29 detector
+rows per z voxel, so the BF16 / FP16 column-strip kernel takes its paired
+path with the second sample one or two rows down (the HI = 1 branch).
+
+Bars (BPAgent/BPCore.cpp:157-205 is the restated semantics): FP32 <= 1e-4
+rel-RMSE vs the FP32-ref oracle; BF16 and FP16 storage <= 1e-3 vs FP64."""
+import numpy as np
+import pytest
+import torch
+
+from jenhine_amd import configs
+from oracle import oracle as o
+from tests.helpers import max_abs, oracle_params, rel_rmse
+
+pytestmark = pytest.mark.gpu
+
+W = configs.DENTAL512
+SLABS = ((0, 2), (254, 4), (510, 2))   # (z0, nz): bottom edge, centre, top edge
+
+
+@pytest.fixture(scope="module")
+def data():
+    return W.data()
+
+
+def _oracle_slices(d, fp64):
+    sc, rc = oracle_params(W)
+    # one oracle pass filters every view; the slabs are back-projected from it
+    z0, nz = SLABS[1]
+    mid, st = o.fdk(d["raw"], d["ptm"], sc, rc, fp64=fp64, nthreads=16, z0=z0, nz=nz, dark=d["dark"],
+                    gain=d["gain"], return_stages=True)
+    out = {}
+    for z0, nz in SLABS:
+        out[z0] = mid if z0 == SLABS[1][0] else o.backproject(st["filt"], d["ptm"], (W.Z, W.Y, W.X), W.pxy, W.pz,
+                                                               z0=z0, nz=nz, fp64=fp64, nthreads=16)
+    return out
+
+
+@pytest.fixture(scope="module")
+def oracle64(data):
+    return _oracle_slices(data, True)
+
+
+@pytest.fixture(scope="module")
+def oracle32(data):
+    return _oracle_slices(data, False)
+
+
+def test_dental512_row_step_takes_the_hi1_pair_branch():
+    P = W.ptms()[0].reshape(3, 4)
+    step = P[1, 2] * W.pz / P[2, 3]   # dv/dz at the iso-centre (w = 1 there)
+    assert 1.015625 <= step <= 1.984375, step
+
+
+@pytest.mark.parametrize("filter_mode,storage", [("fp32", "f32"), ("mfma6", "bf16"), ("mfma6", "f16")])
+def test_dental512_fdk_vs_oracle(gpu, data, oracle64, oracle32, filter_mode, storage):
+    from jenhine_amd.recon import FDK
+    fdk = FDK(W.scan_params(), W.recon_params(), data["ptm"], filter_mode, storage, device=gpu)
+    if storage != "f32":
+        assert fdk.bp.layout() == "strip" and fdk.operand_path()
+    vol = fdk(torch.from_numpy(data["raw"].view(np.int16)).to(gpu),
+              torch.from_numpy(data["dark"].view(np.int16)).to(gpu),
+              torch.from_numpy(data["gain"]).to(gpu)).cpu().numpy()
+    got = np.concatenate([vol[z0:z0 + nz] for z0, nz in SLABS])
+    ref64 = np.concatenate([oracle64[z0] for z0, _ in SLABS])
+    e64 = rel_rmse(got, ref64)
+    edge = np.concatenate([vol[z0:z0 + nz] for z0, nz in (SLABS[0], SLABS[2])])
+    edge64 = np.concatenate([oracle64[z0] for z0, _ in (SLABS[0], SLABS[2])])
+    nan_mismatch = np.mean(np.isnan(got) != np.isnan(ref64))
+    # the edge slices lie outside the phantom (ellipsoids in the inner 60 % of
+    # the field), so their own RMS is small: their error is also measured
+    # against the RMS of all 8 reference slices
+    m = np.isfinite(edge) & np.isfinite(edge64)
+    full = np.isfinite(ref64)
+    e_edge = float(np.sqrt(np.mean((edge[m].astype(np.float64) - edge64[m]) ** 2)) /
+                   np.sqrt(np.mean(ref64[full] ** 2)))
+    print(filter_mode, storage, "rel-RMSE vs FP64", e64, "max-abs", max_abs(got, ref64),
+          "edge slices: own rel-RMSE", rel_rmse(edge, edge64), "error RMS / reference RMS", e_edge,
+          "NaN mismatch", nan_mismatch)
+    assert nan_mismatch < 1e-4
+    assert np.isfinite(got).mean() > 0.5
+    if storage == "f32":
+        ref32 = np.concatenate([oracle32[z0] for z0, _ in SLABS])
+        e32 = rel_rmse(got, ref32)
+        print("rel-RMSE vs FP32-ref", e32)
+        assert e32 <= 1e-4
+        assert e64 <= 1e-4
+        assert e_edge <= 1e-4
+    else:
+        assert e64 <= 1e-3
+        assert e_edge <= 1e-3

@@ -575,19 +575,29 @@ def test_fdk_offset_detector_chain_vs_oracle(gpu, form, storage, tol):
 # Staged layouts of BF16 / FP16 projections
 # --------------------------------------------------------------------------
 
+# the dental geometry scaled down: 0.6 mm * (600 / 350) / 0.8 mm = 1.29 rows
+# per z voxel, the headline workload's row step (the HI = 1 pair branch)
+DENTAL_STEP = configs.DENTAL512.scaled(n_views=60, n_u=192, n_v=192, du=0.8, X=128, Y=128, Z=128, pxy=0.6,
+                                       pz=0.6, name="dental-step-1.29")
+
+
 @pytest.mark.parametrize("dtype", ["bf16", "f16"])
 @pytest.mark.parametrize("tile", [(8, 4), (16, 2), (32, 1)])
-def test_bp_strip_layout_equals_quad_layout_bit_exact(gpu, dtype, tile):
+@pytest.mark.parametrize("geom", ["cbct-step-0.83", "dental-step-1.29"])
+def test_bp_strip_layout_equals_quad_layout_bit_exact(gpu, dtype, tile, geom):
     """The column-strip layout's paired gathers read the same four values per
     sample and interpolate them with the same operations as the quad layout's
     one gather per sample: the volumes agree to the bit (one-shot and
     streamed), including the columns that leave the detector rows (partial
-    path) and the LDS-staged fetch."""
+    path) and the LDS-staged fetch.  Row steps 0.83 (second sample zero or
+    one row down) and 1.29 (one or two rows down)."""
     from jenhine_amd.recon import BPCore
-    w = SMALL
+    w = SMALL if geom.startswith("cbct") else DENTAL_STEP
+    step = w.ptms()[0][6] * w.pz / w.ptms()[0][11]
+    assert (step < 1.0) == geom.startswith("cbct") and 0.5 < step < 1.984375
     filt = torch.from_numpy(synth.normal(31, (w.n_views, w.n_v, w.n_u))).to(gpu)
     bp = BPCore(w.ptms(), w.n_u, w.n_v, w.X, w.Y, w.Z, w.pxy, w.pz, device=gpu)
-    assert bp.layout() == "strip"  # row step 0.83 pixel per z voxel
+    assert bp.layout() == "strip"
     bp.set_tile(*tile)
     out = {}
     for layout in ("quad", "strip"):

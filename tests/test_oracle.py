@@ -247,6 +247,15 @@ def test_ptm_adjust_sequence_closed_forms():
         np.testing.assert_allclose(B, P, rtol=1e-13, atol=1e-13 * np.abs(P).max())
 
 
+def test_f16_exponent_for_maps_the_bound_into_fp16_range():
+    from jenhine_amd.recon import f16_exponent_for
+    for b in (1e-9, 3e-3, 0.7, 1.0, 1000.0, 32768.0, 32769.0, 5e8):
+        k = f16_exponent_for(b)
+        s = b * 2.0 ** -k
+        assert 2 ** 14 <= s < 2 ** 15, (b, k, s)
+    assert f16_exponent_for(0.0) == 0
+
+
 def test_circular_ptm_central_ray_and_magnification():
     ptm = geometry.circular_ptms(8, 350.0, 600.0, 768, 768, 0.2)
     for P in ptm:
