@@ -323,6 +323,29 @@ jh_status jh_bp_backproject(jh_bp_plan* plan, const void* d_q, int view0, int nv
 jh_status jh_fp_project(jh_bp_plan* plan, const float* d_vol, float* d_proj, int view0, int nviews,
                         int weighted, jh_stream_t stream);
 
+/* Allocation-free streaming accumulation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, hosts that forbid hipMalloc).  It is
+ * jh_bp_reset / jh_bp_accumulate / jh_bp_finalize in one entry point whose
+ * accumulator and count volumes live in a static device array of the
+ * library (two volumes of at most 8 Mi = 8388608 voxels, e.g. 256 x 256 x
+ * 128; 64 MiB held by every device the library runs on) instead of the
+ * plan's buffers; it calls no allocating routine.  flags:
+ *   JH_NOALLOC_FIRST  clear the static accumulators, then accumulate;
+ *   JH_NOALLOC_LAST   after accumulating, write the finished volume (as
+ *                     jh_bp_finalize, Z x Y x X f32) to d_vol.
+ * views view0 .. view0+nviews-1 of d_q (staged as for jh_bp_accumulate) are
+ * accumulated; nviews may be 0.  Calls between FIRST and LAST continue one
+ * sequence and must keep the volume size, weighting and storage scale.
+ * Every call on a device shares that device's array and the array keeps the
+ * last call's contents: sequences that overlap on one device (from several
+ * threads or streams) race and give wrong volumes.  A volume above the limit
+ * is refused with JH_ERR_INVALID. */
+#define JH_NOALLOC_FIRST 1
+#define JH_NOALLOC_LAST 2
+jh_status jh_bp_accumulate_noalloc(jh_bp_plan* plan, const void* d_q, int dtype, int view0, int nviews, int flags,
+                                   float* d_vol, jh_stream_t stream);
+
 /* Tile shape of the backprojector: z voxels per thread (8, 16 or 32) and
  * waves per workgroup (1, 2 or 4; each wave covers 8 x 8 voxel columns, so a
  * workgroup covers 8 x 8, 16 x 8 or 16 x 16).  0 keeps the current value.  No

@@ -16,6 +16,7 @@ JH_FILTER_FP32, JH_FILTER_MFMA_BF16X3, JH_FILTER_MFMA_BF16X6, JH_FILTER_FP32_VAL
 JH_WINDOW_REFERENCE, JH_WINDOW_SHEPP_LOGAN = 0, 1
 JH_LAYOUT_QUAD, JH_LAYOUT_STRIP = 0, 1
 JH_WEIGHT_COUNT, JH_WEIGHT_FDK = 0, 1
+JH_NOALLOC_FIRST, JH_NOALLOC_LAST = 1, 2
 JH_PTM_MOVE_U, JH_PTM_MOVE_V, JH_PTM_REVERSE_X, JH_PTM_REVERSE_Y, JH_PTM_MOVE_Z = range(5)
 
 DTYPES = {"f32": JH_F32, "bf16": JH_BF16, "f16": JH_F16}
@@ -30,7 +31,7 @@ EXPORTS = [
     "jh_detector_on_right", "jh_composite_create", "jh_composite_side", "jh_composite_run", "jh_composite_run_operand",
     "jh_composite_destroy",
     "jh_bp_create", "jh_bp_uweights", "jh_bp_stage_bytes", "jh_bp_stage", "jh_bp_run", "jh_bp_reset",
-    "jh_bp_accumulate", "jh_bp_finalize", "jh_bp_set_tile", "jh_bp_get_tile", "jh_bp_geometry_mode",
+    "jh_bp_accumulate", "jh_bp_finalize", "jh_bp_accumulate_noalloc", "jh_bp_set_tile", "jh_bp_get_tile", "jh_bp_geometry_mode",
     "jh_bp_destroy", "jh_bp_pack", "jh_bp_backproject", "jh_fp_project", "jh_bp_autotune",
     "jh_bp_set_fetch", "jh_bp_get_fetch", "jh_bp_set_layout", "jh_bp_get_layout", "jh_bp_set_zfilter",
     "jh_bp_set_ctnum", "jh_bp_set_weight", "jh_bp_get_weight", "jh_bp_set_f16_exponent", "jh_bp_get_f16_exponent",
@@ -106,6 +107,7 @@ _SIGS = {
     "jh_bp_set_ctnum": (c_int, [P, c_int, c_float, c_float, c_float]),
     "jh_bp_set_weight": (c_int, [P, c_int, c_float]),
     "jh_bp_set_f16_exponent": (c_int, [P, c_int]),
+    "jh_bp_accumulate_noalloc": (c_int, [P, P, c_int, c_int, c_int, c_int, P, P]),
     "jh_bp_get_f16_exponent": (c_int, [P, P]),
     "jh_pre_output_bound": (c_int, [P, c_double, c_double, P]),
     "jh_filter_output_bound": (c_int, [P, c_double, P]),

@@ -23,6 +23,8 @@
 //    detector rows of that z-range).
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
 
 #include "jhip_internal.h"
 
@@ -1240,15 +1242,15 @@ static size_t stage_frame_bytes(const jh_bp_plan* p, int dtype)
 }
 
 static void launch_bp(jh_bp_plan* p, const void* q, int dtype, int view0, int nviews, float* vol,
-                      int out, hipStream_t stream)
+                      int out, hipStream_t stream, float* acc = nullptr, float* cnt = nullptr)
 {
     BPArgs a;
     a.q = q;
     a.geo = p->geo.as<Geo>();
     a.uwp = p->uwp.as<float2>();
     a.vol = vol;
-    a.acc = p->acc.as<float>();
-    a.cnt = p->cnt.as<float>();
+    a.acc = acc ? acc : p->acc.as<float>();
+    a.cnt = cnt ? cnt : p->cnt.as<float>();
     a.W = p->W; a.V = p->V; a.X = p->X; a.Y = p->Y; a.Z = p->Z;
     a.view0 = view0; a.nviews = nviews;
     a.ct = p->ct;
@@ -1506,6 +1508,112 @@ jh_status jh_bp_finalize(jh_bp_plan* p, float* d_vol, int z0, int nz, jh_stream_
             p->acc.as<float>() + plane * z0, p->cnt.as<float>() + plane * z0, d_vol, n, p->ct, p->ct_scale,
             p->ct_norm1, p->weight == JH_WEIGHT_FDK ? 1 : 0, p->wscale, p->acc_scale);
         check_launch("bp_finalize_kernel");
+    });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free streaming accumulation (jh_bp_accumulate_noalloc).
+//
+// The accumulator and count volumes live in one file-scope device array,
+// `scratch`, of kNoAllocVoxels voxels each, instead of plan-owned buffers:
+// every GPU the library runs on holds its own copy (64 MiB) until the process
+// ends.  Host code never passes the array's name to a kernel (in host code it
+// is a stand-in, not a device address): the device address is looked up once
+// per device with hipGetSymbolAddress, with that device current, and kept.
+// ---------------------------------------------------------------------------
+
+namespace jh {
+
+constexpr size_t kNoAllocVoxels = size_t(1) << 23;  // 8 Mi voxels: e.g. 256 x 256 x 128
+
+static __device__ __align__(256) float scratch[2 * kNoAllocVoxels];
+
+constexpr int kMaxDevices = 64;
+
+struct NoAllocState {
+    std::mutex m;
+    float* ptr[kMaxDevices] = {};    // device address of `scratch`, per device
+    // the open accumulation sequence of each device (host-side bookkeeping)
+    bool open[kMaxDevices] = {};
+    size_t nvox[kMaxDevices] = {};
+    float scale[kMaxDevices] = {};
+    int weight[kMaxDevices] = {};
+};
+
+static NoAllocState& noalloc_state()
+{
+    static NoAllocState s;
+    return s;
+}
+
+// Device address of `scratch` on the current device `dev` (looked up once).
+static float* scratch_on(int dev)
+{
+    NoAllocState& s = noalloc_state();
+    require(dev >= 0 && dev < kMaxDevices, "jh_bp_accumulate_noalloc: device index beyond the supported range");
+    if (!s.ptr[dev]) {
+        void* ptr = nullptr;
+        JH_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        require(ptr != nullptr, "jh_bp_accumulate_noalloc: no device address for the scratch array");
+        s.ptr[dev] = static_cast<float*>(ptr);
+    }
+    return s.ptr[dev];
+}
+
+}  // namespace jh
+
+extern "C" {
+
+jh_status jh_bp_accumulate_noalloc(jh_bp_plan* p, const void* d_q, int dtype, int view0, int nviews, int flags,
+                                   float* d_vol, jh_stream_t stream)
+{
+    return guard([&] {
+        require(p != nullptr, "jh_bp_accumulate_noalloc: plan is NULL");
+        require(dtype == JH_F32 || dtype == JH_BF16 || dtype == JH_F16, "jh_bp_accumulate_noalloc: unknown dtype");
+        require((flags & ~(JH_NOALLOC_FIRST | JH_NOALLOC_LAST)) == 0, "jh_bp_accumulate_noalloc: unknown flags");
+        require(view0 >= 0 && nviews >= 0 && view0 + nviews <= p->N,
+                "jh_bp_accumulate_noalloc: view range outside the scan");
+        require(nviews == 0 || d_q != nullptr, "jh_bp_accumulate_noalloc: no staged views");
+        require(!(flags & JH_NOALLOC_LAST) || d_vol != nullptr, "jh_bp_accumulate_noalloc: no output volume");
+        const size_t nvox = (size_t)p->X * p->Y * p->Z;
+        require(nvox <= kNoAllocVoxels,
+                "jh_bp_accumulate_noalloc: volume of " + std::to_string(nvox) + " voxels exceeds the " +
+                    std::to_string(kNoAllocVoxels) + "-voxel scratch accumulators");
+        DeviceScope ds(p->device);
+        NoAllocState& st = noalloc_state();
+        std::lock_guard<std::mutex> lock(st.m);
+        float* const acc = scratch_on(p->device);
+        float* const cnt = acc + kNoAllocVoxels;
+        const int dev = p->device;
+        const float sc = dtype == JH_F16 ? std::ldexp(1.0f, p->f16exp) : 1.0f;
+        const bool wt = p->weight == JH_WEIGHT_FDK;
+        hipStream_t s = as_stream(stream);
+        if (flags & JH_NOALLOC_FIRST) {
+            // the array keeps the previous call's contents: clear it here
+            // (a fill kernel, so that a captured graph holds no memset node)
+            fill_zero_kernel<<<grid_for(nvox, 256), 256, 0, s>>>(acc, nvox);
+            if (!wt) fill_zero_kernel<<<grid_for(nvox, 256), 256, 0, s>>>(cnt, nvox);
+            check_launch("fill_zero_kernel");
+            st.open[dev] = true;
+            st.nvox[dev] = nvox;
+            st.scale[dev] = sc;
+            st.weight[dev] = p->weight;
+        } else {
+            require(st.open[dev], "jh_bp_accumulate_noalloc: no open sequence on this device (pass JH_NOALLOC_FIRST)");
+            require(st.nvox[dev] == nvox && st.weight[dev] == p->weight && (nviews == 0 || st.scale[dev] == sc),
+                    "jh_bp_accumulate_noalloc: the call does not continue the open sequence (volume, weighting or "
+                    "storage scale differ)");
+        }
+        if (nviews > 0)
+            launch_bp(p, d_q, dtype, view0, nviews, nullptr, wt ? kOutFdkAccumulate : kOutAccumulate, s, acc, cnt);
+        if (flags & JH_NOALLOC_LAST) {
+            bp_finalize_kernel<<<grid_for(nvox, 256), 256, 0, s>>>(acc, cnt, d_vol, nvox, p->ct, p->ct_scale,
+                                                                   p->ct_norm1, wt ? 1 : 0, p->wscale, st.scale[dev]);
+            check_launch("bp_finalize_kernel");
+            st.open[dev] = false;
+        }
     });
 }
 
