@@ -363,8 +363,9 @@ struct Half16<JH_BF16> {
     __device__ static Wt weight(float fu, float& fe)
     {
         const bool far = fu > 0.5f;
-        const float g = far ? 1.0f - fu : fu;  // exact
-        const uint32_t gb = f32_to_bf16_rne(g);
+        const float g = far ? 1.0f - fu : fu;  // exact, in [0, 1/2]: finite
+        const uint32_t gu = __float_as_uint(g);
+        const uint32_t gb = (gu + 0x7fffu + ((gu >> 16) & 1u)) >> 16;  // round to nearest even
         const float ge = __uint_as_float(gb << 16);
         fe = far ? 1.0f - ge : ge;
         const uint32_t neg = gb | 0x8000u;
@@ -522,6 +523,7 @@ struct BPArgs {
     const void* q;       // views view0..view0+nviews-1, quad layout
     const Geo* geo;      // all views
     const float2* uwp;   // all views, N x W count-weight pairs
+    const float* colcnt; // geometry class 0: X x Y count weights of the launch's views (bp_colcount_kernel)
     float* vol;          // one-shot output
     float* acc;          // streaming accumulators
     float* cnt;
@@ -644,7 +646,6 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
     float acc[ZC], cnt[ZC];
 #pragma unroll
     for (int kk = 0; kk < ZC; ++kk) { acc[kk] = 0.0f; cnt[kk] = 0.0f; }
-    float cfull = 0.0f;  // count weight of the views whose samples all count
     // MODE 0: the per-sample count weights of the views whose column is only
     // partly inside the detector rows are kept in LDS, one slot per thread and
     // sample (touched by that thread only, so no barrier): such views are rare,
@@ -676,9 +677,6 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
                 const int iu = (int)su;
                 float fe;
                 const typename S::wt_t wq = S::weight(__builtin_amdgcn_fractf(su), fe);
-                // the count-weight pair is loaded ahead of the gathers and
-                // used after them (a wait for it alone would stall the view)
-                const float2 uwq = uwv[iu];
                 const float sv0 = fmaf(Vc, rw, 0.5f);
                 const float dsv = g.c[7] * rw;
                 const uint32_t ub = __umul24((uint32_t)iu, (uint32_t)a.pxb);
@@ -720,9 +718,16 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
 #pragma unroll
                         for (int k = 0; k < ZC; ++k) acc[k] = add(acc[k], p[k], fv[k], wq, iw2);
                     }
-                    if constexpr (!WT) cfull += fmaf(fe, uwq.y - uwq.x, uwq.x);
+                    // every sample counts: the column's count weight of this
+                    // view is in a.colcnt already
                 } else {
-                    const float cw = fmaf(fe, uwq.y - uwq.x, uwq.x);
+                    // the count weight of the samples outside the detector
+                    // rows is taken off a.colcnt at the end
+                    float cw = 0.0f;
+                    if constexpr (!WT) {
+                        const float2 uwq = uwv[iu];
+                        cw = fmaf(fe, uwq.y - uwq.x, uwq.x);
+                    }
 #pragma unroll
                     for (int kb = 0; kb < ZC; kb += kZB) {
                         uint32_t off[kZB];
@@ -743,7 +748,7 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
                         for (int q = 0; q < kZB; ++q) {
                             const float sum = add(acc[kb + q], p[q], fv[q], wq, iw2);
                             acc[kb + q] = in[q] ? sum : acc[kb + q];
-                            if constexpr (!WT) cnt_part[(kb + q) * 256 + tid] += in[q] ? cw : 0.0f;
+                            if constexpr (!WT) cnt_part[(kb + q) * 256 + tid] += in[q] ? 0.0f : cw;
                         }
                     }
                 }
@@ -802,8 +807,11 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
 
     if (i >= a.X || j >= a.Y) return;
     if constexpr (!WT) {
+        if constexpr (MODE == 0) {
+            const float c = a.colcnt[(size_t)j * a.X + i];
 #pragma unroll
-        for (int kk = 0; kk < ZC; ++kk) cnt[kk] = (MODE == 0 ? cnt_part[kk * 256 + tid] : cnt[kk]) + cfull;
+            for (int kk = 0; kk < ZC; ++kk) cnt[kk] = c - cnt_part[kk * 256 + tid];
+        }
     }
     const size_t plane = (size_t)a.X * a.Y;
     size_t idx = ((size_t)k0 * a.Y + j) * a.X + i;
@@ -1132,6 +1140,40 @@ __device__ __forceinline__ void bp_tile_lds(const BPArgs& a, unsigned tile)
     }
 }
 
+// Count weights of geometry class 0 per voxel column (u and w do not depend
+// on z): the sum over the launch's views of the interpolated uW at the
+// column's u, in view order and with the sampler's u weight -- the same
+// operations as the backprojector's own sample positions.  The gather form
+// takes off the samples that fall outside the detector rows; every other
+// sample of the column counts.  One thread per column, shared by all the
+// column's z tiles (so the backprojector loads no count weight per view).
+template <int QT>
+__global__ void __launch_bounds__(256) bp_colcount_kernel(const Geo* __restrict__ geo, const float2* __restrict__ uwp,
+                                                          int view0, int nviews, int X, int Y, int W,
+                                                          float* __restrict__ colcnt)
+{
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    if (i >= X || j >= Y) return;
+    const float fi = (float)i, fj = (float)j, Wm1 = (float)(W - 1);
+    float c = 0.0f;
+    for (int v = 0; v < nviews; ++v) {
+        const Geo g = geo[view0 + v];
+        // as bp_tile with c[3] = c[11] = 0 (fmaf(0, k0, x) = x)
+        const float Uc = fmaf(g.c[1], fi, fmaf(g.c[2], fj, g.c[0]));
+        const float Wc = fmaf(g.c[9], fi, fmaf(g.c[10], fj, g.c[8]));
+        const float rw = __builtin_amdgcn_rcpf(Wc);
+        const float su = fmaf(Uc, rw, 0.5f);
+        if (su >= 0.0f && su <= Wm1) {
+            float fe;
+            (void)Sampler<QT>::weight(__builtin_amdgcn_fractf(su), fe);
+            const float2 uwq = uwp[(size_t)(view0 + v) * W + (int)su];
+            c += fmaf(fe, uwq.y - uwq.x, uwq.x);
+        }
+    }
+    colcnt[(size_t)j * X + i] = c;
+}
+
 // Tile schedule.  Workgroups are dispatched round-robin over the 8 XCDs, so
 // workgroup b runs on XCD b % 8.  XCD x takes the contiguous tile range
 // [x T8, (x+1) T8): a z-slab, whose detector rows then stay in that XCD's L2.
@@ -1248,6 +1290,7 @@ static void launch_bp(jh_bp_plan* p, const void* q, int dtype, int view0, int nv
     a.q = q;
     a.geo = p->geo.as<Geo>();
     a.uwp = p->uwp.as<float2>();
+    a.colcnt = p->colcnt.as<float>();
     a.vol = vol;
     a.acc = acc ? acc : p->acc.as<float>();
     a.cnt = cnt ? cnt : p->cnt.as<float>();
@@ -1284,6 +1327,18 @@ static void launch_bp(jh_bp_plan* p, const void* q, int dtype, int view0, int nv
     const bool staged = p->mode == 0 && p->fetch == JH_FETCH_LDS && !distance_weighted(out);
     a.lds_px = staged ? std::min(64 * p->zc, (65536 - 4096 - 64) / 32) : 0;
     const size_t lds_bytes = staged ? 2 * (size_t)a.lds_px * sizeof(float4) : 0;
+    if (p->mode == 0 && !staged && (out == kOutNormalised || out == kOutAccumulate)) {
+        // the count weights of the launch's views per voxel column
+        const dim3 cg((unsigned)((p->X + 63) / 64), (unsigned)((p->Y + 3) / 4));
+        const dim3 cb(64, 4);
+        if (dtype == JH_F32)
+            bp_colcount_kernel<JH_F32><<<cg, cb, 0, stream>>>(a.geo, a.uwp, view0, nviews, p->X, p->Y, p->W, p->colcnt.as<float>());
+        else if (dtype == JH_BF16)
+            bp_colcount_kernel<JH_BF16><<<cg, cb, 0, stream>>>(a.geo, a.uwp, view0, nviews, p->X, p->Y, p->W, p->colcnt.as<float>());
+        else
+            bp_colcount_kernel<JH_F16><<<cg, cb, 0, stream>>>(a.geo, a.uwp, view0, nviews, p->X, p->Y, p->W, p->colcnt.as<float>());
+        check_launch("bp_colcount_kernel");
+    }
     hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(64, p->ty), lds_bytes, stream, a);
     check_launch("bp_kernel");
 }
@@ -1349,6 +1404,7 @@ jh_status jh_bp_create(jh_bp_plan** plan, int device, const double* h_ptm, int n
                 if (e.c[11] != 0.0f) zinv_w = false;
             }
             p->mode = (zinv_u && zinv_w) ? 0 : (zinv_w ? 1 : 2);
+            if (p->mode == 0) p->colcnt.alloc(sizeof(float) * (size_t)X * Y);
             // BF16 / FP16 layout: column strips for geometry class 0 (the
             // backprojector's pair path needs them; where the row step rules
             // pairs out they measure as fast as quads), quads otherwise

@@ -34,6 +34,7 @@ struct jh_bp_plan {
     jh::DevBuf geo;  // N x Geo
     jh::DevBuf uw;   // N x W float
     jh::DevBuf uwp;  // N x W float2 count-weight pairs
+    jh::DevBuf colcnt;  // geometry class 0: X x Y count weights of a launch's views
     jh::DevBuf acc;  // X*Y*Z float (streaming form, allocated on first use)
     jh::DevBuf cnt;  // X*Y*Z float
     jh::DevBuf xt;   // X*Y*Z float, z-fastest copy of the volume for the projector
