@@ -21,11 +21,26 @@ by one hipGraph launch (BF16 storage, 6-product filter), tiles and view chunks
 from the batch autotuner; value = aggregate GUPS of one launch, with per-scan
 rel-RMSE against the FP64 oracle on the centre slices.
 
-The roofline object covers the dominant kernel (bp_kernel) timed with events
-on the stream it runs on; its algorithmic bytes are 8 B per voxel-update (an
-FP32 accumulator read + write per update, the reference's per-view
-formulation); its traffic is the measured HBM bytes per launch.  cpu_baseline times the CPU oracle's FP32 backprojection on a
-bounded sample of slices on the host cores.
+--workload adjoint256 times the differentiation form: the forward projector
+A and the un-normalised backprojector A^T on 256^3 from 360 views of 512^2,
+with the dot-product test of the pair in the same run.
+
+The roofline object covers the dominant kernel (bp_kernel), timed with events
+on the stream it runs on.  Its roof is the lowest of three, each from the
+kernel's own schedule and the counters committed under profiles/ (rocprofv3
+--pmc of the same kernel):
+  * HBM: the bytes the schedule must move at least once (the volume written
+    once, every staged view read once) at 8 TB/s;
+  * VALU: the measured VALU instructions per voxel-update at the published
+    FP32 vector rate (157.3 TFLOP/s = 1.23e12 wave instructions/s);
+  * vector-memory issue: the bytes each update requests through the texture
+    path at 64 B per clock per CU (a 64-lane 16-byte gather issues in 16
+    clocks, measured by jenhine_amd/gather_rate);
+frac = achieved GUPS / the lowest roof.  The reference's per-view formulation
+(an FP32 accumulator read + write per update, 8 B) is kept as a separate
+north-star field.  cpu_baseline times the CPU oracle (the reference
+algorithm in portable C, FP32) on the host cores: preprocessing and filtering
+of every view and the backprojection of a bounded slab of slices.
 """
 import argparse
 import json
@@ -69,11 +84,19 @@ def main():
     ap.add_argument("--kernel-filter", action="store_true",
                     help="with --kernel-only: time the headline chain's filter (pre-weighted operand in, "
                          "staged strips out) instead of the backprojection")
-    ap.add_argument("--workload", choices=("dental512", "batch16", "offset_dental512"), default="dental512")
+    ap.add_argument("--workload", choices=("dental512", "batch16", "offset_dental512", "adjoint256"),
+                    default="dental512")
     ap.add_argument("--branches", type=int, default=4, help="parallel graph branches (batch16)")
-    ap.add_argument("--batch-storage", choices=("bf16", "f16", "f32"), default="bf16",
-                    help="projection storage of the batch16 scans")
+    ap.add_argument("--summarize-counters", metavar="DIR", default=None,
+                    help="no GPU work: read the rocprofv3 --pmc CSV files under DIR (kernel-only runs of one "
+                         "backprojection tile) and print the counter summary the roofline reads")
+    ap.add_argument("--tile", default=None, help="zc,ty of the profiled kernel (with --summarize-counters)")
+    ap.add_argument("--batch-storage", choices=("bf16", "f16", "f32"), default="f16",
+                    help="projection storage of the batch16 scans (FP16 with its range exponent: BF16 storage "
+                         "misses 1e-3 on noise-dominated Ram-Lak scans)")
     args = ap.parse_args()
+    if args.summarize_counters:
+        return summarize_counters(args.summarize_counters, args.tile)
     if args.gpus != 1:
         print(json.dumps({"error": "the benchmark measures one GPU (the path shards by z-slabs, one process "
                                    "per GPU, not built here); --gpus must be 1",
@@ -91,6 +114,8 @@ def main():
         return run_batch(args, dev)
     if args.workload == "offset_dental512":
         return run_offset(args, dev)
+    if args.workload == "adjoint256":
+        return run_adjoint(args, dev)
     w = configs.DENTAL512
     if args.views:
         w = w.scaled(n_views=args.views)
@@ -235,39 +260,13 @@ def main():
                                  slices=[z0, z0 + args.acc_slices])
             log("accuracy", json.dumps(acc))
         if not args.no_cpu:
-            # FP32-ref oracle: preprocess + filter all views, then time its
-            # backprojection of cpu_slices centre slices
-            p32 = o.preprocess(d["raw"], bright_field=sc["bright_field"], bh=sc["bh"], dark=d["dark"],
-                               gain=d["gain"], nthreads=nth)
-            H = o.filter_design(w.n_u, sc["hsr"], rc["cut_start"], rc["cut_end"], window=rc["window"])
-            src, blk = o.preweight_schedule(w.n_views)
-            pws = np.stack([o.prefilter_weight(d["ptm"][s], w.n_u, w.n_v, sc["dso"]) for s in src])
-            f32 = o.filter_rows(p32, H, pws, blk, nthreads=nth)
-            del p32
-            z0 = w.Z // 2 - args.cpu_slices // 2
-            t = time.perf_counter()
-            v32 = o.backproject(f32, d["ptm"], (w.Z, w.Y, w.X), w.pxy, w.pz, z0=z0, nz=args.cpu_slices,
-                                nthreads=nth)
-            dt = time.perf_counter() - t
-            n_upd = w.X * w.Y * args.cpu_slices * w.n_views
-            cpu = dict(value=n_upd / dt / 1e9, unit="GUPS", cores=nth, kind="port",
-                       sample=f"CPU oracle FP32-ref backprojection of {args.cpu_slices} centre slices "
-                              f"(512 x 512 x {args.cpu_slices}) x {w.n_views} views, {nth} threads, "
-                              f"{dt:.1f} s")
+            cpu, v32, z0 = cpu_baseline(o, d, w, sc, rc, nth, args.cpu_slices)
             acc["fp32_vs_fp32ref_rel_rmse"] = rel_rmse(vols["fp32"][z0:z0 + args.cpu_slices], v32)
             log("cpu", json.dumps(cpu), "fp32 vs FP32-ref", acc["fp32_vs_fp32ref_rel_rmse"])
 
     head = results["bf16"]
-    bp_bytes = 8.0 * units
-    achieved = bp_bytes / (head["bp_ms"] * 1e-3) / 1e9
-    # HBM bytes per launch of this kernel from the committed rocprofv3 --pmc
-    # summary (FETCH_SIZE and WRITE_SIZE, each scaled by what the counter
-    # reads for a 1 GiB device copy); null when the summary is absent
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "bp_dental512_bf16_traffic.json")
-    if os.path.exists(tf):
-        with open(tf) as fh:
-            traffic = json.load(fh)["hbm_bytes"]
+    stage_all = forms["bf16"].bp.stage_bytes("bf16", w.n_views)
+    roof = bp_roofline(head["bp_gups"], units, w, stage_all, tuned["bf16"])
     line = {
         "metric": "backprojection GUPS at 512^3 x 600 views; FP32 vs BF16/MFMA path at <=1e-3 rel-RMSE",
         "value": head["gups"],
@@ -287,16 +286,217 @@ def main():
         "forms": results,
         "autotune": tuned,
         "accuracy": acc,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
-                     "frac": achieved / 8000.0, "traffic": traffic,
-                     "kernel": "bp_kernel (bf16 strip storage)", "bytes_per_update": 8,
-                     "note": "achieved counts the per-view formulation's 8 B/voxel-update (FP32 accumulator "
-                             "read + write); frac > 1 means the register-accumulating kernel beats that roof. "
-                             "traffic: measured HBM bytes per launch (FETCH_SIZE + WRITE_SIZE). Measured "
-                             "limiter: the vector-memory path's per-instruction rate (21 L1 accesses per "
-                             "16-byte gather, ~27 cycles per gather per CU; VALU 58 % busy, "
-                             "profiles/bp_dental512_pmc.md)"},
+        "roofline": roof,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+# Published MI355X peaks used by the roofline: HBM 8 TB/s; FP32 vector 157.3
+# TFLOP/s = 256 CUs x 4 SIMDs x 2.4 GHz x 32 FMA lanes per clock, i.e. one
+# wave64 VALU instruction per SIMD every 2 clocks (1.229e12 per second).
+HBM_PEAK = 8.0e12
+CLOCK = 2.4e9
+CUS = 256
+VALU_WAVE_INSTR_PER_S = 1024 * CLOCK / 2.0
+TA_BYTES_PER_CLOCK = 64.0   # per CU: four lanes of a 16-byte gather per clock
+
+
+def bp_roofline(gups, units, w, stage_bytes, tuned):
+    """The backprojector's roofline from its schedule and the committed
+    counters of the same kernel (profiles/bp_dental512_bf16_counters.json)."""
+    nvox = w.X * w.Y * w.Z
+    # compulsory bytes of the schedule: one launch writes the volume once and
+    # reads every view's staged frame once (register accumulation over all
+    # views of the launch: no accumulator traffic)
+    b_sched = 4.0 * nvox + stage_bytes
+    gups_hbm = HBM_PEAK / (b_sched / units) / 1e9
+    # bytes requested through the texture path per update: one 16-byte gather
+    # per pair of z samples (column strips) -> 8 B
+    req_per_update = 8.0
+    gups_ta = CUS * CLOCK * TA_BYTES_PER_CLOCK / req_per_update / 1e9
+    roof = {"kernel": f"bp_kernel (BF16 column strips, zc {tuned['zc']}, {tuned['ty']} waves per workgroup)",
+            "achieved": gups, "unit": "GUPS",
+            "compulsory_bytes": b_sched, "gups_hbm_sched": gups_hbm,
+            "ta_bytes_per_update": req_per_update, "gups_vmem_issue": gups_ta,
+            "north_star_hbm_8B": {"bytes_per_update": 8, "achieved_GBps": 8.0 * gups,
+                                  "peak_GBps": HBM_PEAK / 1e9, "frac": 8.0 * gups * 1e9 / HBM_PEAK,
+                                  "note": "the reference's per-view formulation (FP32 accumulator read + write "
+                                          "per update); this kernel accumulates in registers and never moves "
+                                          "those bytes, so this is not a roof it runs against"}}
+    cf = os.path.join(ROOT, "profiles", "bp_dental512_bf16_counters.json")
+    c = None
+    if os.path.exists(cf):
+        with open(cf) as fh:
+            c = json.load(fh)
+    c = (c or {}).get("tiles", {}).get(f"{tuned['zc']},{tuned['ty']}")
+    if c is not None:
+        valu_per_update = c["valu_insts"] * 64.0 / units
+        gups_valu = VALU_WAVE_INSTR_PER_S * 64.0 / valu_per_update / 1e9
+        roof.update(valu_lane_instr_per_update=valu_per_update, gups_valu=gups_valu,
+                    traffic=c["hbm_bytes"], traffic_over_compulsory=c["hbm_bytes"] / b_sched,
+                    l2_hit=c["l2_hit"], valu_busy=c["valu_busy"], ta_busy=c["ta_busy"],
+                    counters="profiles/bp_dental512_bf16_counters.json")
+    else:
+        gups_valu = None
+        roof.update(traffic=None, counters="absent for this tile: VALU roof not evaluated")
+    roofs = {"hbm": gups_hbm, "vmem_issue": gups_ta}
+    if gups_valu:
+        roofs["valu"] = gups_valu
+    bound = min(roofs, key=roofs.get)
+    roof.update(bound=bound, peak=roofs[bound], frac=gups / roofs[bound])
+    return roof
+
+
+def summarize_counters(folder, tile):
+    """Per-launch counters of the backprojection kernel from rocprofv3 --pmc
+    CSV output (one counter set per run), with the HBM byte counters scaled by
+    what they read for the 1 GiB device copy that --kernel-only runs first."""
+    import csv
+    import glob
+    vals = {}   # (kernel, counter) -> list of per-dispatch values
+    for path in glob.glob(os.path.join(folder, "**", "*counter_collection.csv"), recursive=True):
+        with open(path) as fh:
+            for row in csv.DictReader(fh):
+                k = (row["Kernel_Name"], row["Counter_Name"])
+                vals.setdefault(k, {}).setdefault(row.get("Dispatch_Id", len(vals)), 0.0)
+                vals[k][row.get("Dispatch_Id")] += float(row["Counter_Value"])
+    def per_dispatch(pred, counter):
+        xs = [v for (kn, cn), d in vals.items() if pred(kn) and cn == counter for v in d.values()]
+        return sum(xs) / len(xs) if xs else None
+    is_bp = lambda kn: "bp_kernel" in kn
+    is_copy = lambda kn: "copy" in kn.lower() and "bp_" not in kn
+    gib = float(1 << 30)
+    fetch_copy, write_copy = per_dispatch(is_copy, "FETCH_SIZE"), per_dispatch(is_copy, "WRITE_SIZE")
+    fetch_scale = gib / (fetch_copy * 1024.0) if fetch_copy else None
+    write_scale = gib / (write_copy * 1024.0) if write_copy else None
+    fetch, write = per_dispatch(is_bp, "FETCH_SIZE"), per_dispatch(is_bp, "WRITE_SIZE")
+    hit, miss = per_dispatch(is_bp, "TCC_HIT_sum"), per_dispatch(is_bp, "TCC_MISS_sum")
+    gui = per_dispatch(is_bp, "GRBM_GUI_ACTIVE")
+    zc, ty = (int(x) for x in tile.split(",")) if tile else (None, None)
+    out = {"kernel": "bp_kernel (BF16 column strips, dental512)", "zc": zc, "ty": ty,
+           "fetch_kib": fetch, "write_kib": write, "fetch_scale": fetch_scale, "write_scale": write_scale,
+           "hbm_bytes": (fetch * 1024.0 * fetch_scale + write * 1024.0 * write_scale)
+           if None not in (fetch, write, fetch_scale, write_scale) else None,
+           "valu_insts": per_dispatch(is_bp, "SQ_INSTS_VALU"), "vmem_rd_insts": per_dispatch(is_bp, "SQ_INSTS_VMEM_RD"),
+           "lds_insts": per_dispatch(is_bp, "SQ_INSTS_LDS"), "waves": per_dispatch(is_bp, "SQ_WAVES"),
+           "l2_hit": hit / (hit + miss) if hit is not None and miss else None,
+           "gui_active_cycles_per_xcd": gui / 8.0 if gui else None}
+    busy = per_dispatch(is_bp, "SQ_BUSY_CYCLES")
+    valu_act = per_dispatch(is_bp, "SQ_ACTIVE_INST_VALU")
+    ta = per_dispatch(is_bp, "TA_TA_BUSY_sum")
+    if out["gui_active_cycles_per_xcd"]:
+        cyc = out["gui_active_cycles_per_xcd"]
+        # two clocks per wave-level VALU instruction on each of 1024 SIMDs
+        out["valu_busy"] = out["valu_insts"] * 2.0 / (1024 * cyc) if out["valu_insts"] else None
+        out["ta_busy"] = ta / (CUS * cyc) if ta else None
+    else:
+        out["valu_busy"] = out["ta_busy"] = None
+    out["sq_busy_cycles"], out["sq_active_inst_valu"] = busy, valu_act
+    out["tcp_accesses"] = per_dispatch(is_bp, "TCP_TOTAL_CACHE_ACCESSES_sum")
+    print(json.dumps(out, indent=1))
+
+
+def host_cpu():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return model, os.cpu_count(), avail
+
+
+def cpu_baseline(o, d, w, sc, rc, nth, nslices):
+    """The CPU oracle (FP32-ref, the reference's op order in portable C,
+    threaded like the reference: filter rows in tasks, slices in tasks of 10)
+    on the host: preprocessing and filtering of every view, then the
+    backprojection of nslices centre slices; the full-volume end-to-end time
+    is extrapolated from the slab (backprojection cost is linear in slices)."""
+    model, ncpu, avail = host_cpu()
+    t0 = time.perf_counter()
+    p32 = o.preprocess(d["raw"], bright_field=sc["bright_field"], bh=sc["bh"], dark=d["dark"], gain=d["gain"],
+                       nthreads=nth)
+    t1 = time.perf_counter()
+    H = o.filter_design(w.n_u, sc["hsr"], rc["cut_start"], rc["cut_end"], window=rc["window"])
+    src, blk = o.preweight_schedule(w.n_views)
+    pws = np.stack([o.prefilter_weight(d["ptm"][s], w.n_u, w.n_v, sc["dso"]) for s in src])
+    f32 = o.filter_rows(p32, H, pws, blk, nthreads=nth)
+    t2 = time.perf_counter()
+    del p32
+    z0 = w.Z // 2 - nslices // 2
+    v32 = o.backproject(f32, d["ptm"], (w.Z, w.Y, w.X), w.pxy, w.pz, z0=z0, nz=nslices, nthreads=nth)
+    t3 = time.perf_counter()
+    t_pre, t_filt, t_bp = t1 - t0, t2 - t1, t3 - t2
+    e2e_full = t_pre + t_filt + t_bp * (w.Z / nslices)
+    units = w.X * w.Y * w.Z * w.n_views
+    cpu = dict(value=units / e2e_full / 1e9, unit="GUPS", cores=nth, kind="port",
+               host={"cpu_model": model, "logical_cpus": ncpu, "cpus_available": avail},
+               seconds={"preprocess_all_views": t_pre, "filter_all_views": t_filt,
+                        f"backproject_{nslices}_slices": t_bp, "end_to_end_extrapolated": e2e_full},
+               bp_only_gups=w.X * w.Y * nslices * w.n_views / t_bp / 1e9,
+               sample=f"CPU oracle FP32-ref (portable C, not IPP), {nth} threads: preprocessing and filtering of all "
+                      f"{w.n_views} views of {w.n_u}x{w.n_v}, backprojection of {nslices} centre slices "
+                      f"({w.X}x{w.Y}x{nslices}); value = whole-volume FDK GUPS with the backprojection "
+                      f"extrapolated linearly from the slab ({t_pre + t_filt + t_bp:.1f} s measured)")
+    return cpu, v32, z0
+
+
+def run_adjoint(args, dev):
+    """The differentiation form at 256^3 from 360 views of 512^2: forward
+    projector A (jh_fp_project) and backprojector A^T (jh_bp_backproject of
+    packed FP32 frames), timed with events; dot-product test in the same run."""
+    import torch
+    from jenhine_amd import configs, synth
+    from jenhine_amd.recon import BPCore
+
+    w = configs.ADJOINT256
+    if args.views:
+        w = w.scaled(n_views=args.views)
+    bp = BPCore(w.ptms(), w.n_u, w.n_v, w.X, w.Y, w.Z, w.pxy, w.pz, device=dev)
+    x = torch.from_numpy(synth.normal(21, (w.Z, w.Y, w.X))).to(dev)
+    y = torch.from_numpy(synth.normal(22, (w.n_views, w.n_v, w.n_u))).to(dev)
+    proj = torch.empty((w.n_views, w.n_v, w.n_u), dtype=torch.float32, device=dev)
+    back = torch.empty((w.Z, w.Y, w.X), dtype=torch.float32, device=dev)
+    q = bp.pack(y, "f32")
+    for _ in range(max(1, args.warmup)):
+        bp.project(x, out=proj)
+        bp.backproject(q, 0, w.n_views, out=back)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    ev[0].record()
+    for _ in range(args.steps):
+        bp.project(x, out=proj)
+    ev[1].record()
+    for _ in range(args.steps):
+        bp.backproject(q, 0, w.n_views, out=back)
+    ev[2].record()
+    torch.cuda.synchronize()
+    fp_ms = ev[0].elapsed_time(ev[1]) / args.steps
+    bp_ms = ev[1].elapsed_time(ev[2]) / args.steps
+    units = w.gups_units
+    lhs = float((proj.double() * y.double()).sum())
+    rhs = float((x.double() * back.double()).sum())
+    gap = abs(lhs - rhs) / max(abs(lhs), abs(rhs))
+    log(f"forward {fp_ms:.2f} ms, back {bp_ms:.2f} ms, dot-test gap {gap:.2e}")
+    line = {
+        "metric": "adjoint pair GUPS at 256^3 x 360 views of 512^2 (forward projector A and backprojector A^T)",
+        "value": 2 * units / ((fp_ms + bp_ms) * 1e-3) / 1e9, "unit": "GUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": fp_ms + bp_ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (x, y ~ N(0, 1))",
+        "config": {"workload": f"adjoint256: {w.X}x{w.Y}x{w.Z} at {w.pxy} mm, {w.n_views} views of "
+                               f"{w.n_u}x{w.n_v} at {w.du} mm", "parallelism": "single-gpu"},
+        "forward": {"ms": fp_ms, "gups": units / (fp_ms * 1e-3) / 1e9},
+        "back": {"ms": bp_ms, "gups": units / (bp_ms * 1e-3) / 1e9},
+        "dot_test_rel_gap": gap,
     }
     print(json.dumps(line), flush=True)
 

@@ -45,6 +45,18 @@ def build_hip(verbose=True):
     return out
 
 
+def build_probe():
+    """gather_rate: the vector-memory issue-rate probe the roofline cites."""
+    outdir = os.path.join(HERE, "build")
+    os.makedirs(outdir, exist_ok=True)
+    out = os.path.join(outdir, "gather_rate")
+    _run(["/opt/rocm/bin/hipcc", "--offload-arch=gfx950", "-O3", "-std=c++17",
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument",
+          os.path.join(CSRC, "gather_rate.hip"), "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build_synth():
     out = os.path.join(HERE, "libjhsynth.so")
     tmp = out + ".tmp"
@@ -64,9 +76,10 @@ def build_all():
     build_synth()
     build_hip()
     build_host()
+    build_probe()
 
 
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"all": build_all, "hip": build_hip, "synth": build_synth, "host": build_host}[w]()
+        {"all": build_all, "hip": build_hip, "synth": build_synth, "host": build_host, "probe": build_probe}[w]()
