@@ -9,13 +9,12 @@
 // optionally multiplied by uW_v(iu) (the transpose of the weighted staging).
 //
 // For geometries whose u and w do not depend on z (mode 0, circular orbits)
-// it is a deterministic gather: one wave covers 64 detector rows of one
-// detector column iu and walks the voxel columns (i, j) of the wedge whose
-// samples can fall in columns iu-1 .. iu; for each candidate it recomputes su
-// and sv with the backprojector's own float expressions (including the z
-// tiling), so membership and weights are those of the backprojector.  Other
-// geometries use a scatter with float atomics (same weights, summation order
-// not fixed).
+// it is a deterministic gather (fp_block_kernel): a workgroup covers a block
+// of detector columns and walks the wedge of voxel columns whose samples can
+// fall in them, recomputing su and sv with the backprojector's own float
+// expressions (including the z tiling), so membership and weights are those
+// of the backprojector.  Other geometries use a scatter with float atomics
+// (same weights, summation order not fixed).
 #include <cmath>
 
 #include "jhip_bp_plan.h"
@@ -58,83 +57,116 @@ __device__ inline float bp_su(const Geo& g, float fi, float fj, float& rw)
     return fmaf(Uc, rw, 0.5f);
 }
 
-__global__ void __launch_bounds__(256) fp_gather_kernel(FPArgs a)
+// Block form of the mode-0 gather: one workgroup per view and block of kFpCols
+// detector columns [u0, u0 + kFpCols), rows in passes of 64 x blockDim.y.
+// The workgroup walks the wedge of voxel columns whose su lies in
+// [u0 - 1, u0 + kFpCols) once (each voxel column feeds detector columns
+// floor(su) and floor(su) + 1); every lane owns one detector row and adds,
+// per voxel column, the column's vertical profile at its row -- the samples
+// k whose rows floor(sv_k), floor(sv_k) + 1 include it, with the
+// backprojector's own sample expressions (z tiling included) -- times
+// 1 - fu and fu into its LDS row.  Lanes own their rows, so the LDS needs no
+// atomics and the summation order is fixed.
+constexpr int kFpCols = 16;
+
+__global__ void __launch_bounds__(512) fp_block_kernel(FPArgs a)
 {
-    const int v = blockIdx.z;
-    const int iv = blockIdx.x * 64 + threadIdx.x;
-    const int iu = blockIdx.y * blockDim.y + threadIdx.y;  // uniform across the wave
-    if (iu >= a.W) return;
+    __shared__ float accl[512 * (kFpCols + 1)];  // row r of the pass: accl[r * 17 + c], c < kFpCols
+    const int v = blockIdx.y;
+    const int u0 = blockIdx.x * kFpCols;
+    const int lane_row = threadIdx.y * 64 + threadIdx.x;
+    const int rows_per_pass = 64 * blockDim.y;
     const Geo g = a.geo[a.view0 + v];
     const float Wm1 = (float)(a.W - 1), Vm1 = (float)(a.V - 1);
-    // wedge of voxel columns whose su lies in [iu-1, iu+1): boundaries
-    // (c1 - t c9) i + (c2 - t c10) j + (c0 - t c8) = 0 with t = s - 0.5
-    const double t0 = (double)iu - 1.5, t1 = (double)iu + 0.5;
+    const int zmask = ~(a.zc - 1);  // the backprojector's z tile start (zc is a power of two)
+    // wedge boundaries (c1 - t c9) i + (c2 - t c10) j + (c0 - t c8) = 0 at
+    // su = t + 0.5 for t = u0 - 1.5 and t = u0 + kFpCols - 0.5
+    const double t0 = (double)u0 - 1.5, t1 = (double)(u0 + kFpCols) - 0.5;
     const double ai0 = g.c[1] - t0 * g.c[9], aj0 = g.c[2] - t0 * g.c[10], a00 = g.c[0] - t0 * g.c[8];
     const double ai1 = g.c[1] - t1 * g.c[9], aj1 = g.c[2] - t1 * g.c[10], a01 = g.c[0] - t1 * g.c[8];
-    const double tc = (double)iu - 0.5;
+    const double tc = (double)u0 + 0.5 * kFpCols - 0.5;
     const bool major_i = fabs(g.c[2] - tc * g.c[10]) >= fabs(g.c[1] - tc * g.c[9]);
     const int nmaj = major_i ? a.X : a.Y, nmin = major_i ? a.Y : a.X;
-    float acc = 0.0f;
-    for (int m = 0; m < nmaj; ++m) {
-        // minor-axis range on the two boundary lines
-        double b0, b1;
-        if (major_i) {
-            b0 = aj0 != 0.0 ? -(ai0 * m + a00) / aj0 : (double)-1e30;
-            b1 = aj1 != 0.0 ? -(ai1 * m + a01) / aj1 : (double)1e30;
-        } else {
-            b0 = ai0 != 0.0 ? -(aj0 * m + a00) / ai0 : (double)-1e30;
-            b1 = ai1 != 0.0 ? -(aj1 * m + a01) / ai1 : (double)1e30;
-        }
-        const double lo = fmin(b0, b1), hi = fmax(b0, b1);
-        const int n0 = (int)fmax(0.0, floor(lo) - 2.0);
-        const int n1 = (int)fmin((double)(nmin - 1), ceil(hi) + 2.0);
-        for (int n = n0; n <= n1; ++n) {
-            const int i = major_i ? m : n, j = major_i ? n : m;
-            const float fi = (float)i, fj = (float)j;
-            float rw;
-            const float su = bp_su(g, fi, fj, rw);
-            if (!(su >= 0.0f && su <= Wm1)) continue;
-            const int iu0 = (int)su;
-            const float fu = __builtin_amdgcn_fractf(su);
-            float wu;
-            if (iu0 == iu) wu = 1.0f - fu;
-            else if (iu0 == iu - 1) wu = fu;
-            else continue;
-            // z candidates for this lane's row: sv(k) in [iv-1, iv+1)
-            const float dsv = g.c[7] * rw;
-            const float Vc0 = fmaf(g.c[5], fi, fmaf(g.c[6], fj, g.c[4]));
-            const float svA = fmaf(Vc0, rw, 0.5f);
-            int k0, k1;
-            if (fabsf(dsv) > 1e-6f) {
-                const float ka = ((float)iv - 1.0f - svA) / dsv, kb = ((float)iv + 1.0f - svA) / dsv;
-                k0 = (int)fmaxf(0.0f, floorf(fminf(ka, kb)) - 1.0f);
-                k1 = (int)fminf((float)(a.Z - 1), ceilf(fmaxf(ka, kb)) + 1.0f);
-            } else {
-                k0 = 0;
-                k1 = a.Z - 1;
+    // minor-axis bound on line b: -(a_maj m + a_0) / a_min, linear in m
+    const double am0 = major_i ? aj0 : ai0, ak0 = major_i ? ai0 : aj0;
+    const double am1 = major_i ? aj1 : ai1, ak1 = major_i ? ai1 : aj1;
+    const double s0 = am0 != 0.0 ? -ak0 / am0 : 0.0, c0 = am0 != 0.0 ? -a00 / am0 : -1e30;
+    const double s1 = am1 != 0.0 ? -ak1 / am1 : 0.0, c1 = am1 != 0.0 ? -a01 / am1 : 1e30;
+    for (int rp = 0; rp < a.V; rp += rows_per_pass) {
+        const int iv = rp + lane_row;
+        float* const mine = accl + lane_row * (kFpCols + 1);
+#pragma unroll
+        for (int c = 0; c < kFpCols; ++c) mine[c] = 0.0f;
+        const float fiv = (float)iv;
+        for (int m = 0; m < nmaj; ++m) {
+            const double b0 = am0 != 0.0 ? fma((double)m, s0, c0) : -1e30;
+            const double b1 = am1 != 0.0 ? fma((double)m, s1, c1) : 1e30;
+            const int n0 = (int)fmax(0.0, floor(fmin(b0, b1)) - 2.0);
+            const int n1 = (int)fmin((double)(nmin - 1), ceil(fmax(b0, b1)) + 2.0);
+            for (int n = n0; n <= n1; ++n) {
+                const int i = major_i ? m : n, j = major_i ? n : m;
+                const float fi = (float)i, fj = (float)j;
+                float rw;
+                const float su = bp_su(g, fi, fj, rw);
+                if (!(su >= 0.0f && su <= Wm1)) continue;
+                const int iu0 = (int)su;
+                if (iu0 < u0 - 1 || iu0 >= u0 + kFpCols) continue;
+                const float fu = __builtin_amdgcn_fractf(su);
+                const float dsv = g.c[7] * rw;
+                const float Vc0 = fmaf(g.c[5], fi, fmaf(g.c[6], fj, g.c[4]));
+                // samples k whose rows floor(sv_k) .. floor(sv_k) + 1 may
+                // include this lane's row: sv_k in [iv - 1, iv + 1), with a
+                // margin of one sample against the rounding of sv
+                const float svA = fmaf(Vc0, rw, 0.5f);
+                int k0 = 0, k1 = a.Z - 1;
+                if (fabsf(dsv) > 1e-6f) {
+                    // k with sv_k in [iv - 1, iv + 1): sv_k = svA + k dsv up to
+                    // the rounding of the backprojector's per-tile expressions
+                    // (a few 1e-6 of a row on these sizes); the margin of
+                    // 1/64 sample also absorbs the reciprocal's ulp
+                    const float r = __builtin_amdgcn_rcpf(dsv);
+                    const float ka = (fiv - 1.0f - svA) * r, kb = (fiv + 1.0f - svA) * r;
+                    k0 = (int)fmaxf(0.0f, ceilf(fminf(ka, kb) - 0.015625f));
+                    k1 = (int)fminf((float)(a.Z - 1), floorf(fmaxf(ka, kb) + 0.015625f));
+                }
+                if (iv >= a.V) k1 = -1;
+                const float* xc = a.xt + ((size_t)j * a.X + i) * a.Z;
+                float prof = 0.0f;
+                // candidates in pairs: the loads of a pair (at addresses
+                // clamped into the column) are in flight together, then the
+                // exact membership test picks the samples that count
+                for (int kb = k0; kb <= k1; kb += 2) {
+                    float xv[2];
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) xv[q] = xc[min(kb + q, a.Z - 1)];
+#pragma unroll
+                    for (int q = 0; q < 2; ++q) {
+                        const int k = kb + q;
+                        const int kt = k & zmask;
+                        const float Vc = fmaf(g.c[7], (float)kt, Vc0);
+                        const float sv0 = fmaf(Vc, rw, 0.5f);
+                        const float sv = fmaf((float)(k - kt), dsv, sv0);
+                        const float tt = __builtin_amdgcn_fmed3f(sv, 0.0f, Vm1);
+                        const int ivk = (int)tt;
+                        const float fv = __builtin_amdgcn_fractf(tt);
+                        const bool take = k <= k1 && tt == sv && (ivk == iv || ivk == iv - 1);
+                        const float wv = ivk == iv ? 1.0f - fv : fv;
+                        prof = take ? fmaf(wv, xv[q], prof) : prof;
+                    }
+                }
+                // detector columns iu0 (weight 1 - fu) and iu0 + 1 (fu) of the block
+                if (iu0 >= u0) mine[iu0 - u0] = fmaf(1.0f - fu, prof, mine[iu0 - u0]);
+                if (iu0 + 1 < u0 + kFpCols) mine[iu0 + 1 - u0] = fmaf(fu, prof, mine[iu0 + 1 - u0]);
             }
-            const float* xc = a.xt + ((size_t)j * a.X + i) * a.Z;
-            for (int k = k0; k <= k1; ++k) {
-                // the backprojector's sv: tile start kt, offset kk
-                const int kt = (k / a.zc) * a.zc;
-                const float Vc = fmaf(g.c[7], (float)kt, Vc0);
-                const float sv0 = fmaf(Vc, rw, 0.5f);
-                const float sv = fmaf((float)(k - kt), dsv, sv0);
-                const float tt = __builtin_amdgcn_fmed3f(sv, 0.0f, Vm1);
-                if (tt != sv) continue;
-                const int ivk = (int)tt;
-                const float fv = __builtin_amdgcn_fractf(tt);
-                float wv;
-                if (ivk == iv) wv = 1.0f - fv;
-                else if (ivk == iv - 1) wv = fv;
-                else continue;
-                acc = fmaf(wu * wv, xc[k], acc);
+        }
+        if (iv < a.V) {
+            float* yr = a.y + ((size_t)v * a.V + iv) * a.W;
+#pragma unroll
+            for (int c = 0; c < kFpCols; ++c) {
+                const int u = u0 + c;
+                if (u < a.W) yr[u] = mine[c] * (a.uw ? a.uw[(size_t)(a.view0 + v) * a.W + u] : 1.0f);
             }
         }
-    }
-    if (iv < a.V) {
-        const float w = a.uw ? a.uw[(size_t)(a.view0 + v) * a.W + iu] : 1.0f;
-        a.y[((size_t)v * a.V + iv) * a.W + iu] = acc * w;
     }
 }
 
@@ -227,8 +259,9 @@ jh_status jh_fp_project(jh_bp_plan* p, const float* d_vol, float* d_proj, int vi
             fp_transpose_kernel<<<dim3((p->X + 31) / 32, (p->Z + 31) / 32, p->Y), dim3(32, 8), 0, s>>>(
                 d_vol, p->xt.as<float>(), p->X, p->Y, p->Z);
             check_launch("fp_transpose_kernel");
-            fp_gather_kernel<<<dim3((p->V + 63) / 64, (p->W + 3) / 4, nviews), dim3(64, 4), 0, s>>>(a);
-            check_launch("fp_gather_kernel");
+            // rows in passes of 64 x 8
+            fp_block_kernel<<<dim3((p->W + kFpCols - 1) / kFpCols, nviews), dim3(64, 8), 0, s>>>(a);
+            check_launch("fp_block_kernel");
         } else {
             a.xt = nullptr;
             const size_t n = (size_t)nviews * p->V * p->W;

@@ -86,6 +86,32 @@ def test_projector_matches_cpu_oracle_and_general_modes(gpu):
             assert err <= 1e-5
 
 
+@pytest.mark.parametrize("du,zc", [(1.0, 8), (0.25, 16), (2.2, 32)])
+def test_projector_row_steps_and_z_tiles(gpu, du, zc):
+    """The block projector's sample range per detector row is taken from the
+    row step: 0.38 rows per z voxel (du 2.2), 0.83 (du 1.0) and 3.3 (du
+    0.25), each with another
+    z tile of the backprojector (the sample positions follow its tiling):
+    against the CPU oracle's transpose and by the dot test with the
+    backprojector."""
+    from jenhine_amd.recon import BPCore
+    w = configs.CBCT128.scaled(n_views=7, n_u=64, n_v=96, X=28, Y=24, Z=40, pxy=0.5, pz=0.5, du=du)
+    x = synth.normal(24, (w.Z, w.Y, w.X))
+    y = synth.normal(25, (w.n_views, w.n_v, w.n_u))
+    bp = BPCore(w.ptms(), w.n_u, w.n_v, w.X, w.Y, w.Z, w.pxy, w.pz, device=gpu)
+    bp.set_tile(zc, 4)
+    assert bp.geometry_mode() == 0
+    g = bp.project(torch.from_numpy(x).to(gpu)).cpu().numpy()
+    c = o.forward_project(x, w.ptms(), (w.n_v, w.n_u), w.pxy, w.pz, weighted=False)
+    err = np.sqrt(np.mean((g - c) ** 2)) / np.sqrt(np.mean(c ** 2))
+    yt = torch.from_numpy(y).to(gpu)
+    bty = bp.backproject(bp.pack(yt, "f32"), 0, w.n_views).cpu().numpy()
+    lhs, rhs = float((g.astype(np.float64) * y).sum()), float((x.astype(np.float64) * bty).sum())
+    gap = abs(lhs - rhs) / max(abs(lhs), abs(rhs))
+    print("du", du, "zc", zc, "projector vs CPU", err, "dot gap", gap)
+    assert err <= 1e-5 and gap <= 1e-5
+
+
 def _loss_setup(w, dev, seed=31, noise=0.01):
     d = w.data(noise=False)
     p = d["lineint"]
