@@ -23,12 +23,14 @@
 // <pipe>_<producer>_<consumer>:<path> items (the keys of the reference's
 // pipe_dump_load.xml, Config/PipeDumpLoadReader.cpp:14-55).  Parameters are
 // key=value arguments.
+#include <atomic>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <iostream>
 #include <map>
+#include <mutex>
 #include <set>
 #include <sstream>
 #include <string>
@@ -53,12 +55,16 @@ std::vector<T> ReadFile(const std::string& path, size_t count)
 
 // Streams the scan from memory, once per part, one shot per pass
 // (JEngine/InputAgent.cpp:39-91 streams files the same way), with each view
-// followed by its conjugate when pairs is set.
+// followed by its conjugate when pairs is set.  Four threads copy frames;
+// each takes the next frame and its write token together, so the order is
+// kept.
+constexpr int kHostThreads = 4;
+
 class MemoryInputAgent : public SequentialAgentBase {
 public:
     MemoryInputAgent(const std::vector<uint16_t>& raw, size_t frame, size_t views, size_t passes, bool pairs)
-        : SequentialAgentBase("MemoryInputAgent", 1), raw_(raw), frame_(frame), views_(views), passes_(passes),
-          pairs_(pairs)
+        : SequentialAgentBase("MemoryInputAgent", kHostThreads), raw_(raw), frame_(frame), views_(views),
+          passes_(passes), pairs_(pairs)
     {
     }
 
@@ -67,33 +73,48 @@ private:
     {
         out_ = CreateNewPipe<UINT16Vec>("Proj");
         out_->SetTemplate(UINT16Vec(frame_), {frame_});
-        out_->SetProducer(GetAgentName(), 1, 1);
+        out_->SetProducer(GetAgentName(), kHostThreads, 1);
         MergePipeToTrunk(out_);
     }
-    void WorkFlow0() override
+    void WorkFlow0() override { Copy(); }
+    void WorkFlow1() override { Copy(); }
+    void WorkFlow2() override { Copy(); }
+    void WorkFlow3() override { Copy(); }
+    void Copy()
     {
-        for (size_t p = 0; p < passes_; ++p)
-            for (size_t v = 0; v < views_; ++v)
-                for (size_t k = 0; k < (pairs_ ? 2u : 1u); ++k) {
-                    const size_t view = k ? (v + views_ / 2) % views_ : v;
-                    auto wt = out_->GetWriteToken(1, v + 1 == views_ && k + 1 == (pairs_ ? 2u : 1u));
-                    ThreadOccupiedScope occ(this);
-                    std::memcpy(wt.GetBuffer(0).data(), raw_.data() + view * frame_, frame_ * sizeof(uint16_t));
-                }
-        out_->Close();
+        const size_t per = pairs_ ? 2u : 1u, total = passes_ * views_ * per;
+        for (;;) {
+            size_t view;
+            Pipe<UINT16Vec>::WriteToken wt;
+            {
+                std::lock_guard<std::mutex> lock(m_);
+                if (next_ == total) break;
+                const size_t e = next_++;
+                const size_t k = e % per, v = (e / per) % views_;
+                view = k ? (v + views_ / 2) % views_ : v;
+                wt = out_->GetWriteToken(1, v + 1 == views_ && k + 1 == per);
+            }
+            ThreadOccupiedScope occ(this);
+            std::memcpy(wt.GetBuffer(0).data(), raw_.data() + view * frame_, frame_ * sizeof(uint16_t));
+        }
+        if (--running_ == 0) out_->Close();
     }
     const std::vector<uint16_t>& raw_;
     const size_t frame_, views_, passes_;
     const bool pairs_;
+    std::mutex m_;
+    size_t next_ = 0;
+    std::atomic<int> running_{kHostThreads};
     std::shared_ptr<Pipe<UINT16Vec>> out_;
 };
 
 // Collects slices into the output volume (JEngine/OutputAgent.cpp:37-82
-// writes them to files instead).
+// writes them to files instead).  Four threads; each takes a slice token and
+// its index together.
 class SliceOutputAgent : public SequentialAgentBase {
 public:
     SliceOutputAgent(std::vector<float>& vol, size_t plane, const BPHIPAgent& bp, size_t parts)
-        : SequentialAgentBase("SliceOutputAgent", 1), vol_(vol), plane_(plane), bp_(bp), parts_(parts)
+        : SequentialAgentBase("SliceOutputAgent", kHostThreads), vol_(vol), plane_(plane), bp_(bp), parts_(parts)
     {
     }
 
@@ -101,32 +122,44 @@ private:
     void SetPipesImpl() override
     {
         in_ = BranchPipeFromTrunk<FloatVec>("Slice");
-        in_->SetConsumer(GetAgentName(), 1, 1, 0);
+        in_->SetConsumer(GetAgentName(), kHostThreads, 1, 0);
     }
-    void WorkFlow0() override
+    void WorkFlow0() override { Collect(); }
+    void WorkFlow1() override { Collect(); }
+    void WorkFlow2() override { Collect(); }
+    void WorkFlow3() override { Collect(); }
+    void Collect()
     {
-        size_t part = 0, z = 0;
         try {
             for (;;) {
-                auto rt = in_->GetReadToken();
-                ThreadOccupiedScope occ(this);
-                const size_t slice = bp_.PartStart(part) + z;
-                std::memcpy(vol_.data() + slice * plane_, rt.GetBuffer(0).data(), plane_ * sizeof(float));
-                ++z;
-                if (rt.IsShotEnd()) {
-                    ++part;
-                    z = 0;
+                Pipe<FloatVec>::ReadToken rt;
+                size_t slice;
+                {
+                    std::lock_guard<std::mutex> lock(m_);
+                    rt = in_->GetReadToken();
+                    slice = bp_.PartStart(part_) + z_;
+                    ++z_;
+                    if (rt.IsShotEnd()) {
+                        ++part_;
+                        z_ = 0;
+                    }
                 }
+                ThreadOccupiedScope occ(this);
+                std::memcpy(vol_.data() + slice * plane_, rt.GetBuffer(0).data(), plane_ * sizeof(float));
             }
         } catch (PipeClosedAndEmptySignal&) {
         }
-        if (part != parts_) ThrowExceptionAndLog("SliceOutputAgent: received " + std::to_string(part) + " of " +
-                                                 std::to_string(parts_) + " parts");
+        if (--running_ == 0 && part_ != parts_)
+            ThrowExceptionAndLog("SliceOutputAgent: received " + std::to_string(part_) + " of " +
+                                 std::to_string(parts_) + " parts");
     }
     std::vector<float>& vol_;
     const size_t plane_;
     const BPHIPAgent& bp_;
     const size_t parts_;
+    std::mutex m_;
+    size_t part_ = 0, z_ = 0;
+    std::atomic<int> running_{kHostThreads};
     std::shared_ptr<Pipe<FloatVec>> in_;
 };
 
@@ -254,8 +287,8 @@ int main(int argc, char** argv)
                                                               a.S("input_folder"), a.S("input_template"), composite));
             else
                 agents.push_back(std::make_shared<MemoryInputAgent>(raw, frame, scan.NumViews, parts, composite));
-            agents.push_back(std::make_shared<ProjectionUploadAgent>(scan.NumDetsU, scan.NumDetsV));
-            agents.push_back(std::make_shared<PreProcessingHIPAgent>(scan, dark, gain, device));
+            agents.push_back(std::make_shared<ProjectionUploadAgent>(scan.NumDetsU, scan.NumDetsV, recon.ViewsPerChunk));
+            agents.push_back(std::make_shared<PreProcessingHIPAgent>(scan, dark, gain, device, recon.ViewsPerChunk));
             if (composite) agents.push_back(std::make_shared<CompositeHIPAgent>(scan, ptm, detectorOnRight, device));
             agents.push_back(std::make_shared<FilterHIPAgent>(scan, recon, ptm, device, composite, detectorOnRight));
             agents.push_back(bp);

@@ -17,14 +17,22 @@
 //   DeviceMemoryDownloadAgent  DeviceMemory<float> "Slice" -> FloatVec "Slice"
 //                              (replaces CUDA/DeviceMemoryDownloadAgent.cpp)
 //
-// Each agent owns one HIP stream created on its own thread and synchronises
-// it before releasing a write token, so the next agent may use the data
-// (the reference's ordering rule, BPCUDAAgent.cpp:240-261).
+// Each agent thread owns one HIP stream created on that thread and
+// synchronises it before releasing a write token, so the next agent may use
+// the data (the reference's ordering rule, BPCUDAAgent.cpp:240-261).  The
+// device agents take chunks of views (ReconParams::ViewsPerChunk): device
+// frame pipes allocate their ring as one buffer (PipeRing), so an aligned
+// chunk token is one contiguous run of frames and the chunk is one launch and
+// one stream synchronisation.  The copy agents (upload, download) run
+// kCopyThreads threads, each with its own stream; a thread takes its read and write tokens
+// together under the agent's lock, so the stream order is kept.
 #pragma once
 
 #include <hip/hip_runtime_api.h>
 
+#include <atomic>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -59,20 +67,37 @@ namespace hip {
 void* DeviceMalloc(size_t bytes);
 void DeviceFree(void* p, size_t bytes);
 
-// Owned device buffer of n elements.  Copyable (a pipe copies its template
-// into every ring slot): a copy allocates and copies the contents device to
-// device.  Replaces CUDA/DeviceMemory.cpp:19-79.
+// One device allocation shared by several DeviceMemory views (a pipe's ring).
+std::shared_ptr<void> DeviceArena(size_t bytes);
+
+// Owned device buffer of n elements, or a view of n elements of an arena.
+// Copyable (a pipe copies its template into every ring slot): a copy
+// allocates and copies the contents device to device.  Replaces
+// CUDA/DeviceMemory.cpp:19-79.
 template <typename T>
 class DeviceMemory {
 public:
     DeviceMemory() = default;
     explicit DeviceMemory(size_t n) { Malloc(n); }
+    // n elements at element offset `offset` of an arena (kept alive by the view)
+    static DeviceMemory View(const std::shared_ptr<void>& arena, size_t offset, size_t n)
+    {
+        DeviceMemory d;
+        d.arena_ = arena;
+        d.p_ = static_cast<T*>(arena.get()) + offset;
+        d.n_ = n;
+        return d;
+    }
     DeviceMemory(const DeviceMemory& o)
     {
         Malloc(o.n_);
         if (n_) CheckHip(hipMemcpy(p_, o.p_, n_ * sizeof(T), hipMemcpyDeviceToDevice), "DeviceMemory copy");
     }
-    DeviceMemory(DeviceMemory&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+    DeviceMemory(DeviceMemory&& o) noexcept : p_(o.p_), n_(o.n_), arena_(std::move(o.arena_))
+    {
+        o.p_ = nullptr;
+        o.n_ = 0;
+    }
     DeviceMemory& operator=(const DeviceMemory& o)
     {
         if (this != &o) {
@@ -84,7 +109,14 @@ public:
     }
     DeviceMemory& operator=(DeviceMemory&& o) noexcept
     {
-        if (this != &o) { Free(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
+        if (this != &o) {
+            Free();
+            p_ = o.p_;
+            n_ = o.n_;
+            arena_ = std::move(o.arena_);
+            o.p_ = nullptr;
+            o.n_ = 0;
+        }
         return *this;
     }
     ~DeviceMemory() { Free(); }
@@ -97,12 +129,14 @@ private:
     void Malloc(size_t n);
     void Free()
     {
-        DeviceFree(p_, n_ * sizeof(T));
+        if (arena_) arena_.reset();
+        else DeviceFree(p_, n_ * sizeof(T));
         p_ = nullptr;
         n_ = 0;
     }
     T* p_ = nullptr;
     size_t n_ = 0;
+    std::shared_ptr<void> arena_;  // set for a view: p_ points into it
 };
 
 template <typename T>
@@ -113,6 +147,30 @@ void DeviceMemory<T>::Malloc(size_t n)
 }
 
 }  // namespace hip
+
+// A ring of device frames is one allocation: slot i follows slot i-1, so the
+// frames of a token that does not wrap around the ring are contiguous and a
+// chunk of views can be processed by one launch.
+template <typename U>
+struct PipeRing<hip::DeviceMemory<U>> {
+    static void Fill(std::vector<hip::DeviceMemory<U>>& buffers, size_t n, const hip::DeviceMemory<U>& tmpl)
+    {
+        const size_t e = tmpl.Size();
+        auto arena = hip::DeviceArena(n * e * sizeof(U));
+        buffers.clear();
+        buffers.reserve(n);
+        for (size_t i = 0; i < n; ++i) buffers.push_back(hip::DeviceMemory<U>::View(arena, i * e, e));
+    }
+};
+
+// Whether the frames of a token lie back to back (one contiguous buffer).
+template <typename U>
+bool Contiguous(const hip::DeviceMemory<U>* const* frames, size_t n)
+{
+    for (size_t i = 1; i < n; ++i)
+        if (frames[i]->Data() != frames[0]->Data() + i * frames[0]->Size()) return false;
+    return true;
+}
 
 // Dump / load payload of a device buffer: copied through host memory
 // (the reference's CUDA specialisation, CUDA/PipeDumpLoadImplCUDA.cpp:15-65).
@@ -144,24 +202,53 @@ private:
     hipStream_t s_ = nullptr;
 };
 
+// Threads of the copy agents.  Each thread stages its frames through a pinned
+// host buffer of its own (hipHostMalloc): the host memcpy between the pipe's
+// pageable frame and that buffer runs on the copy threads in parallel, and
+// the device DMA runs from / to pinned memory.
+constexpr int kCopyThreads = 4;
+
+// A pinned host buffer, allocated on first use.
+class PinnedBuffer {
+public:
+    PinnedBuffer() = default;
+    PinnedBuffer(const PinnedBuffer&) = delete;
+    PinnedBuffer& operator=(const PinnedBuffer&) = delete;
+    ~PinnedBuffer();
+    void* Get(size_t bytes);
+
+private:
+    void* p_ = nullptr;
+    size_t n_ = 0;
+};
+
+// Copies the raw frames up; viewsPerChunk aligns the device ring to the
+// chunks its consumer reads.
 class ProjectionUploadAgent : public SequentialAgentBase {
 public:
-    ProjectionUploadAgent(size_t width, size_t height);
+    ProjectionUploadAgent(size_t width, size_t height, size_t viewsPerChunk = 1);
 
 private:
     void SetPipesImpl() override;
-    void WorkFlow0() override;
-    const size_t width_, height_;
+    void WorkFlow0() override { Copy(0); }
+    void WorkFlow1() override { Copy(1); }
+    void WorkFlow2() override { Copy(2); }
+    void WorkFlow3() override { Copy(3); }
+    void Copy(int thread);
+    const size_t width_, height_, chunk_;
     std::shared_ptr<Pipe<UINT16Vec>> in_;
     std::shared_ptr<Pipe<hip::DeviceMemory<uint16_t>>> out_;
-    StreamHolder stream_;
+    std::mutex tokens_;
+    std::atomic<int> running_{kCopyThreads};
+    StreamHolder stream_[kCopyThreads];
+    PinnedBuffer pinned_[kCopyThreads];
 };
 
 class PreProcessingHIPAgent : public SequentialAgentBase {
 public:
     // dark (inH x inW) and gain (outH x outW) are optional extensions
     PreProcessingHIPAgent(const ScanParams& scan, const std::vector<uint16_t>& dark = {},
-                          const std::vector<float>& gain = {}, int device = 0);
+                          const std::vector<float>& gain = {}, int device = 0, size_t viewsPerChunk = 1);
     ~PreProcessingHIPAgent() override;
     void GetReady() override;
 
@@ -172,6 +259,7 @@ private:
     std::vector<uint16_t> darkHost_;
     std::vector<float> gainHost_;
     int device_;
+    size_t chunk_;
     jh_pre_plan* plan_ = nullptr;
     hip::DeviceMemory<uint16_t> dark_;
     hip::DeviceMemory<float> gain_;
@@ -219,6 +307,7 @@ private:
     std::vector<double> ptms_;
     int device_;
     bool composite_, right_;
+    size_t chunk_;
     jh_filter_plan* plan_ = nullptr;
     std::shared_ptr<Pipe<hip::DeviceMemory<float>>> in_, out_;
     StreamHolder stream_;
@@ -261,11 +350,18 @@ public:
 
 private:
     void SetPipesImpl() override;
-    void WorkFlow0() override;
+    void WorkFlow0() override { Copy(0); }
+    void WorkFlow1() override { Copy(1); }
+    void WorkFlow2() override { Copy(2); }
+    void WorkFlow3() override { Copy(3); }
+    void Copy(int thread);
     const size_t n_;
     std::shared_ptr<Pipe<hip::DeviceMemory<float>>> in_;
     std::shared_ptr<Pipe<FloatVec>> out_;
-    StreamHolder stream_;
+    std::mutex tokens_;
+    std::atomic<int> running_{kCopyThreads};
+    StreamHolder stream_[kCopyThreads];
+    PinnedBuffer pinned_[kCopyThreads];
 };
 
 }  // namespace jh

@@ -66,6 +66,14 @@ struct PipeElementIO<std::vector<U>, std::enable_if_t<std::is_trivially_copyable
     }
 };
 
+// How a pipe fills its ring from the element template: copies by default.
+// Element types may specialise it (the device buffers allocate the ring's
+// slots from one contiguous allocation, HipAgents.h).
+template <typename T, typename = void>
+struct PipeRing {
+    static void Fill(std::vector<T>& buffers, size_t n, const T& tmpl) { buffers.assign(n, tmpl); }
+};
+
 class PipeBase {
 public:
     explicit PipeBase(std::string name);
@@ -76,6 +84,10 @@ public:
     void SetConsumer(const std::string& name, size_t number, size_t sizeIncludingOverlap, size_t overlapSize);
     void SetProducer(const std::string& name, size_t number, size_t size);
     void SetBufferSize(size_t size);
+    // The default ring size is rounded up to a multiple of `granule` (an
+    // extension: consumers that process whole chunks of a contiguous ring
+    // need chunk-aligned tokens never to wrap).
+    void SetBufferGranule(size_t granule);
 
     const std::string& GetName() const { return name_; }
     const std::string& GetConsumer() const { return consumer_; }
@@ -130,6 +142,7 @@ protected:
     void OpenLoad(const std::string& path);
     void TryInit();
     size_t bufferSize_ = 0;
+    size_t granule_ = 1;
     std::unique_ptr<std::ofstream> dump_;
     std::unique_ptr<std::ifstream> load_;
 
@@ -287,7 +300,7 @@ protected:
     }
     void InitBuffers() override
     {
-        buffers_.assign(bufferSize_, template_[0]);
+        PipeRing<T>::Fill(buffers_, bufferSize_, template_[0]);
         // pointers repeated once so that a token crossing the end of the
         // ring still sees contiguous element pointers
         ptrs_.resize(2 * bufferSize_ - 1);

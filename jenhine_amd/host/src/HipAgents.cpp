@@ -88,6 +88,12 @@ void DeviceFree(void* p, size_t bytes)
     }
 }
 
+std::shared_ptr<void> DeviceArena(size_t bytes)
+{
+    void* p = bytes ? DeviceMalloc(bytes) : nullptr;
+    return std::shared_ptr<void>(p, [bytes](void* q) { DeviceFree(q, bytes); });
+}
+
 }  // namespace hip
 
 StreamHolder::~StreamHolder()
@@ -103,43 +109,75 @@ hipStream_t StreamHolder::Get()
 
 void StreamHolder::Sync() { CheckHip(hipStreamSynchronize(Get()), "hipStreamSynchronize"); }
 
+PinnedBuffer::~PinnedBuffer()
+{
+    if (p_) (void)hipHostFree(p_);
+}
+
+void* PinnedBuffer::Get(size_t bytes)
+{
+    if (bytes > n_) {
+        if (p_) CheckHip(hipHostFree(p_), "hipHostFree");
+        p_ = nullptr;
+        n_ = 0;
+        if (hipHostMalloc(&p_, bytes, hipHostMallocDefault) != hipSuccess || !p_) {
+            (void)hipGetLastError();
+            p_ = nullptr;
+            throw Exception(std::string("hipHostMalloc of ") + std::to_string(bytes) + " bytes failed");
+        }
+        n_ = bytes;
+    }
+    return p_;
+}
+
 // ---------------------------------------------------------------------------
 // ProjectionUploadAgent
 // ---------------------------------------------------------------------------
 
-ProjectionUploadAgent::ProjectionUploadAgent(size_t width, size_t height)
-    : SequentialAgentBase("ProjectionUploadAgent", 1), width_(width), height_(height)
+ProjectionUploadAgent::ProjectionUploadAgent(size_t width, size_t height, size_t viewsPerChunk)
+    : SequentialAgentBase("ProjectionUploadAgent", kCopyThreads), width_(width), height_(height),
+      chunk_(std::max<size_t>(1, viewsPerChunk))
 {
 }
 
 void ProjectionUploadAgent::SetPipesImpl()
 {
     in_ = BranchPipeFromTrunk<UINT16Vec>("Proj");
-    in_->SetConsumer(GetAgentName(), 1, 1, 0);
+    in_->SetConsumer(GetAgentName(), kCopyThreads, 1, 0);
     out_ = CreateNewPipe<hip::DeviceMemory<uint16_t>>("Proj");
     out_->SetTemplate(hip::DeviceMemory<uint16_t>(width_ * height_), {width_, height_});
-    out_->SetProducer(GetAgentName(), 1, 1);
+    out_->SetBufferGranule(chunk_);
+    out_->SetProducer(GetAgentName(), kCopyThreads, 1);
     MergePipeToTrunk(out_);
 }
 
-void ProjectionUploadAgent::WorkFlow0()
+// Several threads, each with its own stream; a thread takes a read token and the
+// matching write token together, so frames keep their stream order.
+void ProjectionUploadAgent::Copy(int thread)
 {
     try {
         for (;;) {
-            auto rt = in_->GetReadToken();
-            auto wt = out_->GetWriteToken(1, rt.IsShotEnd());
+            Pipe<UINT16Vec>::ReadToken rt;
+            Pipe<hip::DeviceMemory<uint16_t>>::WriteToken wt;
+            {
+                std::lock_guard<std::mutex> lock(tokens_);
+                rt = in_->GetReadToken();
+                wt = out_->GetWriteToken(1, rt.IsShotEnd());
+            }
             ThreadOccupiedScope occ(this);
             const UINT16Vec& src = rt.GetBuffer(0);
             if (src.size() != width_ * height_) ThrowExceptionAndLog(GetAgentName() + ": frame size mismatch");
-            CheckHip(hipMemcpyAsync(wt.GetBuffer(0).Data(), src.data(), src.size() * sizeof(uint16_t),
-                                    hipMemcpyHostToDevice, stream_.Get()),
+            const size_t bytes = src.size() * sizeof(uint16_t);
+            void* stage = pinned_[thread].Get(bytes);
+            std::memcpy(stage, src.data(), bytes);
+            CheckHip(hipMemcpyAsync(wt.GetBuffer(0).Data(), stage, bytes, hipMemcpyHostToDevice, stream_[thread].Get()),
                      "upload");
             WaitAsyncScope wait(&occ);
-            stream_.Sync();
+            stream_[thread].Sync();
         }
     } catch (PipeClosedAndEmptySignal&) {
     }
-    out_->Close();
+    if (--running_ == 0) out_->Close();
 }
 
 // ---------------------------------------------------------------------------
@@ -147,8 +185,9 @@ void ProjectionUploadAgent::WorkFlow0()
 // ---------------------------------------------------------------------------
 
 PreProcessingHIPAgent::PreProcessingHIPAgent(const ScanParams& scan, const std::vector<uint16_t>& dark,
-                                             const std::vector<float>& gain, int device)
-    : SequentialAgentBase("PreProcessingHIPAgent", 1), scan_(scan), darkHost_(dark), gainHost_(gain), device_(device)
+                                             const std::vector<float>& gain, int device, size_t viewsPerChunk)
+    : SequentialAgentBase("PreProcessingHIPAgent", 1), scan_(scan), darkHost_(dark), gainHost_(gain), device_(device),
+      chunk_(std::max<size_t>(1, viewsPerChunk))
 {
     scan_.Derive();
 }
@@ -161,11 +200,13 @@ PreProcessingHIPAgent::~PreProcessingHIPAgent()
 void PreProcessingHIPAgent::SetPipesImpl()
 {
     in_ = BranchPipeFromTrunk<hip::DeviceMemory<uint16_t>>("Proj");
-    in_->SetConsumer(GetAgentName(), 1, 1, 0);
+    in_->SetBufferGranule(chunk_);
+    in_->SetConsumer(GetAgentName(), 1, chunk_, 0);
     out_ = CreateNewPipe<hip::DeviceMemory<float>>("Proj");
     out_->SetTemplate(hip::DeviceMemory<float>(scan_.NumUsedDetsU * scan_.NumUsedDetsV),
                       {scan_.NumUsedDetsU, scan_.NumUsedDetsV});
-    out_->SetProducer(GetAgentName(), 1, 1);
+    out_->SetBufferGranule(chunk_);
+    out_->SetProducer(GetAgentName(), 1, chunk_);
     MergePipeToTrunk(out_);
 }
 
@@ -194,11 +235,20 @@ void PreProcessingHIPAgent::WorkFlow0()
     try {
         for (;;) {
             auto rt = in_->GetReadToken();
-            auto wt = out_->GetWriteToken(1, rt.IsShotEnd());
+            const size_t n = rt.GetSize();
+            auto wt = out_->GetWriteToken(n, rt.IsShotEnd());
             ThreadOccupiedScope occ(this);
-            CheckJh(jh_pre_run(plan_, rt.GetBuffer(0).Data(), dark_.Data(), gain_.Data(), wt.GetBuffer(0).Data(), 1,
-                               stream_.Get()),
-                    "jh_pre_run");
+            // one launch over the chunk when both runs of frames are contiguous
+            if (Contiguous(rt.GetBufferPtrs(), n) && Contiguous(wt.GetBufferPtrs(), n)) {
+                CheckJh(jh_pre_run(plan_, rt.GetBuffer(0).Data(), dark_.Data(), gain_.Data(), wt.GetBuffer(0).Data(),
+                                   (int)n, stream_.Get()),
+                        "jh_pre_run");
+            } else {
+                for (size_t i = 0; i < n; ++i)
+                    CheckJh(jh_pre_run(plan_, rt.GetBuffer(i).Data(), dark_.Data(), gain_.Data(),
+                                       wt.GetBuffer(i).Data(), 1, stream_.Get()),
+                            "jh_pre_run");
+            }
             WaitAsyncScope wait(&occ);
             stream_.Sync();
         }
@@ -271,7 +321,7 @@ void CompositeHIPAgent::WorkFlow0()
 FilterHIPAgent::FilterHIPAgent(const ScanParams& scan, const ReconParams& recon, const std::vector<double>& ptms,
                                int device, bool composite, bool detectorOnRight)
     : SequentialAgentBase("FilterHIPAgent", 1), scan_(scan), recon_(recon), ptms_(ptms), device_(device),
-      composite_(composite), right_(composite && detectorOnRight)
+      composite_(composite), right_(composite && detectorOnRight), chunk_(std::max<size_t>(1, recon.ViewsPerChunk))
 {
     scan_.Derive();
 }
@@ -284,11 +334,13 @@ FilterHIPAgent::~FilterHIPAgent()
 void FilterHIPAgent::SetPipesImpl()
 {
     in_ = BranchPipeFromTrunk<hip::DeviceMemory<float>>("Proj");
-    in_->SetConsumer(GetAgentName(), 1, 1, 0);
+    in_->SetBufferGranule(chunk_);
+    in_->SetConsumer(GetAgentName(), 1, chunk_, 0);
     out_ = CreateNewPipe<hip::DeviceMemory<float>>("Proj");
     out_->SetTemplate(hip::DeviceMemory<float>(scan_.NumUsedDetsU * scan_.NumUsedDetsV),
                       {scan_.NumUsedDetsU, scan_.NumUsedDetsV});
-    out_->SetProducer(GetAgentName(), 1, 1);
+    out_->SetBufferGranule(chunk_);
+    out_->SetProducer(GetAgentName(), 1, chunk_);
     MergePipeToTrunk(out_);
 }
 
@@ -314,11 +366,22 @@ void FilterHIPAgent::WorkFlow0()
     try {
         for (;;) {
             auto rt = in_->GetReadToken();
-            auto wt = out_->GetWriteToken(1, rt.IsShotEnd());
+            const size_t n = rt.GetSize();
+            auto wt = out_->GetWriteToken(n, rt.IsShotEnd());
             ThreadOccupiedScope occ(this);
             const int view = (int)(rt.GetStartIndex() % nv);
-            CheckJh(jh_filter_run(plan_, rt.GetBuffer(0).Data(), wt.GetBuffer(0).Data(), view, 1, stream_.Get()),
-                    "jh_filter_run");
+            // one launch over the chunk when its frames are contiguous and its
+            // views consecutive (a chunk never spans two shots)
+            if (view + n <= nv && Contiguous(rt.GetBufferPtrs(), n) && Contiguous(wt.GetBufferPtrs(), n)) {
+                CheckJh(jh_filter_run(plan_, rt.GetBuffer(0).Data(), wt.GetBuffer(0).Data(), view, (int)n,
+                                      stream_.Get()),
+                        "jh_filter_run");
+            } else {
+                for (size_t i = 0; i < n; ++i)
+                    CheckJh(jh_filter_run(plan_, rt.GetBuffer(i).Data(), wt.GetBuffer(i).Data(),
+                                          (int)((rt.GetStartIndex() + i) % nv), 1, stream_.Get()),
+                            "jh_filter_run");
+            }
             WaitAsyncScope wait(&occ);
             stream_.Sync();
         }
@@ -358,14 +421,19 @@ size_t BPHIPAgent::PartSlices(size_t p) const
     return std::min(per, recon_.NumPixelsZ - PartStart(p));
 }
 
+// slices finalised per launch (one contiguous write token)
+constexpr size_t kSliceChunk = 16;
+
 void BPHIPAgent::SetPipesImpl()
 {
     in_ = BranchPipeFromTrunk<hip::DeviceMemory<float>>("Proj");
+    in_->SetBufferGranule(recon_.ViewsPerChunk);
     in_->SetConsumer(GetAgentName(), 1, recon_.ViewsPerChunk, 0);
     out_ = CreateNewPipe<hip::DeviceMemory<float>>("Slice");
     out_->SetTemplate(hip::DeviceMemory<float>(recon_.NumPixelsX * recon_.NumPixelsY),
                       {recon_.NumPixelsX, recon_.NumPixelsY});
-    out_->SetProducer(GetAgentName(), 1, 1);
+    out_->SetBufferGranule(kSliceChunk);
+    out_->SetProducer(GetAgentName(), 1, kSliceChunk);
     MergePipeToTrunk(out_);
 }
 
@@ -428,15 +496,23 @@ void BPHIPAgent::WorkFlow0()
             jh_bp_plan* plan = PlanFor(shot % numParts_);
             if (!frameBytes) CheckJh(jh_bp_stage_bytes(plan, recon_.StorageType, 1, &frameBytes), "stage bytes");
             if (rt.IsShotStart()) CheckJh(jh_bp_reset(plan, stream_.Get()), "jh_bp_reset");
-            // stage the chunk's views contiguously, then accumulate runs of
-            // consecutive view indices
+            // stage the chunk's views contiguously (one launch when the
+            // frames are contiguous and their views consecutive), then
+            // accumulate runs of consecutive view indices
             const size_t n = rt.GetSize();
+            const int view0 = (int)(rt.GetStartIndex() % nv);
+            const bool whole = view0 + n <= nv && Contiguous(rt.GetBufferPtrs(), n);
+            if (whole)
+                CheckJh(jh_bp_stage(plan, rt.GetBuffer(0).Data(), view0, (int)n, recon_.StorageType, q_.Data(),
+                                    stream_.Get()),
+                        "jh_bp_stage");
             size_t runStart = 0;
             for (size_t i = 0; i < n; ++i) {
                 const int view = (int)((rt.GetStartIndex() + i) % nv);
-                CheckJh(jh_bp_stage(plan, rt.GetBuffer(i).Data(), view, 1, recon_.StorageType,
-                                    q_.Data() + i * frameBytes, stream_.Get()),
-                        "jh_bp_stage");
+                if (!whole)
+                    CheckJh(jh_bp_stage(plan, rt.GetBuffer(i).Data(), view, 1, recon_.StorageType,
+                                        q_.Data() + i * frameBytes, stream_.Get()),
+                            "jh_bp_stage");
                 const bool last = i + 1 == n;
                 const bool wraps = !last && (int)((rt.GetStartIndex() + i + 1) % nv) != view + 1;
                 if (last || wraps) {
@@ -454,9 +530,17 @@ void BPHIPAgent::WorkFlow0()
             if (rt.IsShotEnd()) {
                 rt.Reset();  // views are no longer needed
                 const size_t nz = PartSlices(shot % numParts_);
-                for (size_t z = 0; z < nz; ++z) {
-                    auto wt = out_->GetWriteToken(1, z + 1 == nz);
-                    CheckJh(jh_bp_finalize(plan, wt.GetBuffer(0).Data(), (int)z, 1, stream_.Get()), "jh_bp_finalize");
+                for (size_t z = 0; z < nz; z += kSliceChunk) {
+                    const size_t m = std::min(kSliceChunk, nz - z);
+                    auto wt = out_->GetWriteToken(m, z + m == nz);
+                    if (Contiguous(wt.GetBufferPtrs(), m)) {
+                        CheckJh(jh_bp_finalize(plan, wt.GetBuffer(0).Data(), (int)z, (int)m, stream_.Get()),
+                                "jh_bp_finalize");
+                    } else {
+                        for (size_t i = 0; i < m; ++i)
+                            CheckJh(jh_bp_finalize(plan, wt.GetBuffer(i).Data(), (int)(z + i), 1, stream_.Get()),
+                                    "jh_bp_finalize");
+                    }
                     stream_.Sync();
                 }
                 ++shot;
@@ -472,36 +556,47 @@ void BPHIPAgent::WorkFlow0()
 // ---------------------------------------------------------------------------
 
 DeviceMemoryDownloadAgent::DeviceMemoryDownloadAgent(size_t elements)
-    : SequentialAgentBase("DeviceMemoryDownloadAgent", 1), n_(elements)
+    : SequentialAgentBase("DeviceMemoryDownloadAgent", kCopyThreads), n_(elements)
 {
 }
 
 void DeviceMemoryDownloadAgent::SetPipesImpl()
 {
     in_ = BranchPipeFromTrunk<hip::DeviceMemory<float>>("Slice");
-    in_->SetConsumer(GetAgentName(), 1, 1, 0);
+    in_->SetConsumer(GetAgentName(), kCopyThreads, 1, 0);
     out_ = CreateNewPipe<FloatVec>("Slice");
     out_->SetTemplate(FloatVec(n_), {n_});
-    out_->SetProducer(GetAgentName(), 1, 1);
+    out_->SetProducer(GetAgentName(), kCopyThreads, 1);
     MergePipeToTrunk(out_);
 }
 
-void DeviceMemoryDownloadAgent::WorkFlow0()
+// Several threads, each with its own stream; read and write tokens are taken
+// together, so slices keep their order.
+void DeviceMemoryDownloadAgent::Copy(int thread)
 {
     try {
         for (;;) {
-            auto rt = in_->GetReadToken();
-            auto wt = out_->GetWriteToken(1, rt.IsShotEnd());
+            Pipe<hip::DeviceMemory<float>>::ReadToken rt;
+            Pipe<FloatVec>::WriteToken wt;
+            {
+                std::lock_guard<std::mutex> lock(tokens_);
+                rt = in_->GetReadToken();
+                wt = out_->GetWriteToken(1, rt.IsShotEnd());
+            }
             ThreadOccupiedScope occ(this);
-            CheckHip(hipMemcpyAsync(wt.GetBuffer(0).data(), rt.GetBuffer(0).Data(), n_ * sizeof(float),
-                                    hipMemcpyDeviceToHost, stream_.Get()),
+            void* stage = pinned_[thread].Get(n_ * sizeof(float));
+            CheckHip(hipMemcpyAsync(stage, rt.GetBuffer(0).Data(), n_ * sizeof(float), hipMemcpyDeviceToHost,
+                                    stream_[thread].Get()),
                      "download");
-            WaitAsyncScope wait(&occ);
-            stream_.Sync();
+            {
+                WaitAsyncScope wait(&occ);
+                stream_[thread].Sync();
+            }
+            std::memcpy(wt.GetBuffer(0).data(), stage, n_ * sizeof(float));
         }
     } catch (PipeClosedAndEmptySignal&) {
     }
-    out_->Close();
+    if (--running_ == 0) out_->Close();
 }
 
 }  // namespace jh

@@ -57,13 +57,27 @@ void PipeBase::SetBufferSize(size_t size)
     TryInit();
 }
 
+void PipeBase::SetBufferGranule(size_t granule)
+{
+    {
+        std::lock_guard<std::mutex> lk(m_);
+        if (ready_) ThrowExceptionAndLog(name_ + ": pipe is already initialized.");
+        if (granule == 0) ThrowExceptionAndLog(name_ + ": granule can not be 0.");
+        granule_ = granule;
+    }
+    TryInit();
+}
+
 void PipeBase::TryInit()
 {
     std::lock_guard<std::mutex> lk(m_);
     if (ready_ || numReaders_ == 0 || numWriters_ == 0 || !HasTemplate()) return;
     // default: every writer and reader can hold a token at once
     // (Pipeline/PipeImpl.h:84-89)
-    if (bufferSize_ == 0) bufferSize_ = numWriters_ * writeSize_ + numReaders_ * (readSize_ - overlap_) + overlap_;
+    if (bufferSize_ == 0) {
+        bufferSize_ = numWriters_ * writeSize_ + numReaders_ * (readSize_ - overlap_) + overlap_;
+        bufferSize_ = (bufferSize_ + granule_ - 1) / granule_ * granule_;
+    }
     InitBuffers();
     ready_ = true;
     cv_.notify_all();
