@@ -84,6 +84,9 @@ def main():
     ap.add_argument("--kernel-filter", action="store_true",
                     help="with --kernel-only: time the headline chain's filter (pre-weighted operand in, "
                          "staged strips out) instead of the backprojection")
+    ap.add_argument("--filter-form", choices=("mfma", "mfma6", "mfma16"), default="mfma16",
+                    help="MFMA filter of the bf16 and f16 chains: hi/lo BF16 (3 products), hi/mid/lo BF16 "
+                         "(6 products) or row-scaled hi/lo FP16 (3 products)")
     ap.add_argument("--workload", choices=("dental512", "batch16", "offset_dental512", "adjoint256"),
                     default="dental512")
     ap.add_argument("--branches", type=int, default=4, help="parallel graph branches (batch16)")
@@ -127,8 +130,8 @@ def main():
     gain = torch.from_numpy(d["gain"]).to(dev)
     scan, rec = w.scan_params(), w.recon_params()
     forms = {"fp32": FDK(scan, rec, d["ptm"], "fp32", "f32", dev),
-             "bf16": FDK(scan, rec, d["ptm"], "mfma6", "bf16", dev),
-             "f16": FDK(scan, rec, d["ptm"], "mfma6", "f16", dev)}
+             "bf16": FDK(scan, rec, d["ptm"], args.filter_form, "bf16", dev),
+             "f16": FDK(scan, rec, d["ptm"], args.filter_form, "f16", dev)}
     if args.kernel_only:
         # profiling run: a plain copy of known size (scales the HBM counters),
         # then the backprojection kernel of one form `steps` times
@@ -520,7 +523,7 @@ def run_offset(args, dev):
     scan, rec = w.scan_params(), w.recon_params()
     units = w.gups_units
     results, vols = {}, {}
-    for name, (mode, storage) in {"fp32": ("fp32", "f32"), "bf16": ("mfma6", "bf16")}.items():
+    for name, (mode, storage) in {"fp32": ("fp32", "f32"), "bf16": (args.filter_form, "bf16")}.items():
         fdk = FDK(scan, rec, d["ptm"], mode, storage, dev, composite=True)
         _, st0 = fdk(raw, dark, gain, return_stages=True)
         fdk.bp.autotune(st0["q"], storage, reps=1)
@@ -616,8 +619,8 @@ def run_batch(args, dev):
         e = dict(w=w, d=d, raw=torch.from_numpy(d["raw"].view(np.int16)).to(dev),
                  dark=torch.from_numpy(d["dark"].view(np.int16)).to(dev), gain=torch.from_numpy(d["gain"]).to(dev),
                  vol=torch.empty((w.Z, w.Y, w.X), dtype=torch.float32, device=dev))
-        e["fdk"] = FDK(w.scan_params(), w.recon_params(), d["ptm"], "fp32" if args.batch_storage == "f32" else "mfma6",
-                       args.batch_storage, dev)
+        e["fdk"] = FDK(w.scan_params(), w.recon_params(), d["ptm"],
+                       "fp32" if args.batch_storage == "f32" else args.filter_form, args.batch_storage, dev)
         entries.append(e)
     units = sum(e["w"].gups_units for e in entries)
     log(f"16 scans generated in {time.time() - t0:.1f} s; {units / 1e9:.1f} G voxel-updates per launch")

@@ -39,6 +39,9 @@ typedef int jh_status;
 #define JH_FILTER_MFMA_BF16X3 1 /* hi/lo split BF16 on MFMA, 3 products */
 #define JH_FILTER_MFMA_BF16X6 2 /* hi/mid/lo split BF16 on MFMA, 6 products */
 #define JH_FILTER_FP32_VALU 3   /* FP32 FMA on the vector ALUs (cross-check) */
+#define JH_FILTER_MFMA_F16X3 4  /* hi/lo split FP16 on MFMA, 3 products: every row of the
+                                   pre-weighted input and the filter matrix scaled by a power
+                                   of two so that hi + lo keep 22 significant bits */
 
 /* Filter windows. */
 #define JH_FETCH_GATHER 0 /* backprojector sample fetch forms (jh_bp_set_fetch) */

@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(HERE, "libjhip.so")
 
 JH_OK, JH_ERR_INVALID, JH_ERR_OOM, JH_ERR_HIP = 0, 1, 2, 3
 JH_F32, JH_BF16, JH_F16 = 0, 1, 2
-JH_FILTER_FP32, JH_FILTER_MFMA_BF16X3, JH_FILTER_MFMA_BF16X6, JH_FILTER_FP32_VALU = 0, 1, 2, 3
+JH_FILTER_FP32, JH_FILTER_MFMA_BF16X3, JH_FILTER_MFMA_BF16X6, JH_FILTER_FP32_VALU, JH_FILTER_MFMA_F16X3 = 0, 1, 2, 3, 4
 JH_WINDOW_REFERENCE, JH_WINDOW_SHEPP_LOGAN = 0, 1
 JH_LAYOUT_QUAD, JH_LAYOUT_STRIP = 0, 1
 JH_WEIGHT_COUNT, JH_WEIGHT_FDK = 0, 1

@@ -13,9 +13,12 @@
 //    keeping ZC accumulators and ZC counts in registers across all views of
 //    a launch, so the volume is written once per launch (one-shot) or
 //    read-modify-written once per view chunk (streaming);
-//  * projections are stored in a "pair" layout, pixel (u, v) holding
-//    (q[v][u], q[v][u+1] - q[v][u]) in FP32 and (q[v][u], q[v][u+1]) in BF16
-//    or FP16, so one bilinear sample is two aligned gathers that hit L1/L2;
+//  * projections are staged so that one bilinear sample is one aligned
+//    gather: FP32 quads (16 B, a pixel's 2 x 2 neighbourhood), BF16 / FP16
+//    quads (8 B) or, for geometry class 0, BF16 / FP16 column strips (8 B =
+//    four rows of one column), where one 16-byte gather serves two
+//    consecutive z samples when the row step is below 2 pixels (the staging
+//    section below gives the layouts);
 //  * when u and w do not depend on z (circular orbits) the u-side of the
 //    bilinear weights and the count weight are hoisted out of the z loop;
 //  * 64 x TY voxel columns per workgroup, workgroups remapped so that each of

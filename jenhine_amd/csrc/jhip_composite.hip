@@ -173,6 +173,8 @@ struct CompOpArgs {
     int R0, R1;       // operand rows of this launch's views
     int blockRow0;    // first 16-row block of the launch
     int nsteps, aview0, edge0;
+    int bstride;      // float4s per 16-row block of the operand
+    int rowmax;       // split-FP16 filter: write the row maxima of the k tile
 };
 
 constexpr int kCompOpPitch = 132;
@@ -229,6 +231,24 @@ __global__ void __launch_bounds__(kCompOpThreads) composite_operand_kernel(CompO
         rel += kRowStep;
     }
     __syncthreads();
+    if (a.rowmax) {
+        // the k tile's largest magnitude of each of the launch's rows: thread
+        // t scans 16 columns of row t % 16, then the 8 threads of a row meet
+        static_assert(kCompOpThreads == 128, "two waves of 16 rows x 8 column groups");
+        __shared__ float wmax[2][16];
+        const int i = t & 15, c0 = (t >> 4) * 16;
+        float m = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) m = fmaxf(m, fabsf(tile[i * kCompOpPitch + c0 + c]));
+        m = fmaxf(m, __shfl_xor(m, 16));
+        m = fmaxf(m, __shfl_xor(m, 32));
+        if ((t & 63) < 16) wmax[t >> 6][i] = m;
+        __syncthreads();
+        const int Rl = rb * 16 + i;
+        if (t < 16 && Rl >= a.R0 && Rl < a.R1)
+            reinterpret_cast<float*>(a.out + (size_t)rb * a.bstride + (size_t)a.nsteps * 128)[blockIdx.x * 16 + i] =
+                fmaxf(wmax[0][i], wmax[1][i]);
+    }
     // write phase: the 512 float4s of the block (4 k steps x 2 halves x 64
     // lanes of the MFMA operand order), 512 / kCompOpThreads per thread
 #pragma unroll
@@ -239,7 +259,7 @@ __global__ void __launch_bounds__(kCompOpThreads) composite_operand_kernel(CompO
         const int Rl = rb * 16 + (l & 15);
         if (s >= a.nsteps || Rl < a.R0 || Rl >= a.R1) continue;
         const float* src = tile + (l & 15) * kCompOpPitch + 32 * sl + 8 * (l >> 4) + 4 * h;
-        a.out[(((size_t)rb * a.nsteps + s) * 2 + h) * 64 + l] = make_float4(src[0], src[1], src[2], src[3]);
+        a.out[(size_t)rb * a.bstride + ((size_t)s * 2 + h) * 64 + l] = make_float4(src[0], src[1], src[2], src[3]);
     }
 }
 #pragma clang fp contract(on)
@@ -380,6 +400,8 @@ jh_status jh_composite_run_operand(jh_composite_plan* p, jh_filter_plan* fp, con
         a.nsteps = fp->Kp / 32;
         a.aview0 = aview0;
         a.edge0 = fp->mirror ? fp->W - 3 : 0;
+        a.bstride = fp->bstride;
+        a.rowmax = fp->mode == JH_FILTER_MFMA_F16X3 ? 1 : 0;
         const dim3 grid((unsigned)((fp->Kp + 127) / 128), (unsigned)(b1 - b0 + 1));
         composite_operand_kernel<<<grid, kCompOpThreads, 0, as_stream(stream)>>>(a);
         check_launch("composite_operand_kernel");

@@ -122,6 +122,8 @@ struct FiltArgs {
     int qtype;           // JH_BF16 or JH_F16
     float qs;            // staged values are q * qs (qs = 2^-f16exp for FP16, exact; else 1)
     int tiled;           // x is the pre-weighted A operand in the MFMA tile order (pre_operand_kernel)
+    int bstride;         // float4s per 16-row block of that operand
+    float bscale;        // split-FP16 form: factor of the packed filter matrix (a power of two)
 };
 
 // Element (r, m) of the A operand.
@@ -191,9 +193,15 @@ __global__ void __launch_bounds__(256) filter_fp32_kernel(FiltArgs a)
 // the pre-weight maps (load_a8), computed by the same operations.  Layout: the
 // floats (r, k) of 16-row block r16 and k step s = k / 32 that lane
 // l = 16 g + (r % 16), g = (k % 32) / 8, supplies, in two 16-byte halves h =
-// (k % 8) / 4: float4 index ((r16 nsteps + s) 2 + h) 64 + l.  The MFMA kernel
-// then reads 1 KB contiguous per wave load and no pre-weight.  Rows beyond
-// the launch are 0.
+// (k % 8) / 4: float4 index r16 bstride + (2 s + h) 64 + l, bstride = 128
+// nsteps.  The MFMA kernel then reads 1 KB contiguous per wave load and no
+// pre-weight.  Rows beyond the launch are 0.
+//
+// Split-FP16 form: the filter scales every row r by 2^e_r, chosen from the
+// row's largest magnitude, before splitting it into FP16 parts; each block
+// then also holds the row maxima of its 128-column k tiles, tile-major (16
+// floats per tile: ntk = ceil(Kp / 128) tiles, 4 ntk more float4s, so
+// bstride = 128 nsteps + 4 ntk).
 // ---------------------------------------------------------------------------
 
 struct PreOpArgs {
@@ -205,28 +213,19 @@ struct PreOpArgs {
     float4* out;
     int inW, inH, bUp, bLeft;
     int W, V, rows, nsteps, view0, edge0;
+    int bstride;          // float4s per 16-row block of the operand
+    int rowmax;           // split-FP16 form: write the row maxima of each k tile
     float bf, a0, a1, a2;
     int doBH;
     int vec;              // rows of counts, dark, gain and pre-weights allow 16-byte loads
 };
 
-constexpr int kOpPitch = 132;  // LDS row pitch (floats) of the operand tile
-
-// Workgroup: 16 rows x 128 k (four k steps) of the operand.  Loads: thread t
-// takes row t / 16 and the 8 columns 8 (t % 16) ..: a wave reads four rows of
-// 256 contiguous bytes of raw counts per instruction.  The values go through
-// LDS and leave in the operand order, 1 KB contiguous per wave store.
-__global__ void __launch_bounds__(256) pre_operand_kernel(PreOpArgs a)
+// Operand elements (r, kb .. kb+7) of a launch row r < rows (0 beyond K).
+__device__ inline void pre_operand_elems(const PreOpArgs& a, int r, int kb, float (&e)[8])
 {
-    __shared__ float tile[16 * kOpPitch];
-    const int t = threadIdx.x;
-    const int i = t >> 4, cg = t & 15;
-    const int r = blockIdx.y * 16 + i;
-    const int kb = blockIdx.x * 128 + 8 * cg;
-    float e[8];
 #pragma unroll
     for (int j = 0; j < 8; ++j) e[j] = 0.0f;
-    if (r < a.rows && kb < a.W + 3) {
+    if (kb < a.W + 3) {
         const int view = r / a.V, vrow = r - view * a.V;
         const uint16_t* raw = a.raw + (size_t)view * a.inW * a.inH;
         const size_t rowin = (size_t)(vrow + a.bUp) * a.inW + a.bLeft;
@@ -273,18 +272,87 @@ __global__ void __launch_bounds__(256) pre_operand_kernel(PreOpArgs a)
             }
         }
     }
+}
+
+constexpr int kOpPitch = 132;  // LDS row pitch (floats) of the operand tile
+
+// Workgroup: 16 rows x 128 k (four k steps) of the operand.  Loads: thread t
+// takes row t / 16 and the 8 columns 8 (t % 16) ..: a wave reads four rows of
+// 256 contiguous bytes of raw counts per instruction.  The values go through
+// LDS and leave in the operand order, 1 KB contiguous per wave store.
+__global__ void __launch_bounds__(256) pre_operand_kernel(PreOpArgs a)
+{
+    __shared__ float tile[16 * kOpPitch];
+    const int t = threadIdx.x;
+    const int i = t >> 4, cg = t & 15;
+    const int r = blockIdx.y * 16 + i;
+    const int kb = blockIdx.x * 128 + 8 * cg;
+    float e[8];
+    if (r < a.rows) {
+        pre_operand_elems(a, r, kb, e);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) e[j] = 0.0f;
+    }
     float* dstt = tile + i * kOpPitch + 8 * cg;
 #pragma unroll
     for (int j = 0; j < 8; ++j) dstt[j] = e[j];
+    float4* blk = a.out + (size_t)blockIdx.y * a.bstride;
+    if (a.rowmax) {
+        // the row's largest magnitude in this k tile (the 16 lanes of row i)
+        float m = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) m = fmaxf(m, fabsf(e[j]));
+#pragma unroll
+        for (int x = 1; x < 16; x <<= 1) m = fmaxf(m, __shfl_xor(m, x));
+        if (cg == 0) reinterpret_cast<float*>(blk + (size_t)a.nsteps * 128)[blockIdx.x * 16 + i] = m;
+    }
     __syncthreads();
     // operand order: k step s = 4 blockIdx.x + t / 64, lane l = t % 64, halves h
     const int sl = t >> 6, l = t & 63;
     const int s = blockIdx.x * 4 + sl;
     if (s >= a.nsteps) return;
     const float* src = tile + (l & 15) * kOpPitch + 32 * sl + 8 * (l >> 4);
-    float4* dst = a.out + ((size_t)blockIdx.y * a.nsteps + s) * 128 + l;
+    float4* dst = blk + (size_t)s * 128 + l;
     dst[0] = make_float4(src[0], src[1], src[2], src[3]);
     dst[64] = make_float4(src[4], src[5], src[6], src[7]);
+}
+
+// ---------------------------------------------------------------------------
+// Split-FP16 operand rows: every row r of the A operand is scaled by 2^e_r,
+// e_r chosen from the row's largest magnitude m so that m 2^e_r lies in
+// [2^14, 2^15) (FP16's largest finite value is 65504), and each scaled value
+// v is split into hi = RN16(v) and lo = RN16(v - hi); v - hi is exact in
+// FP32, so hi + lo keep 22 significant bits.  The results are multiplied by
+// 2^-e_r and by the filter matrix's own power-of-two factor, exactly.
+// ---------------------------------------------------------------------------
+
+// Exponent e_r of a row's scale 2^e_r from its largest magnitude m (m >= 0,
+// not NaN): m 2^e_r in [2^14, 2^15); 100 for m < 2^-85 (and 0); -114 for an
+// infinite m (the row's results are then infinite or NaN in every form).
+__device__ inline int row_scale_exp(float m)
+{
+    const int E = (int)((__float_as_uint(m) >> 23) & 0xffu);  // m = f 2^(E - 126), f in [0.5, 1)
+    return E == 0 ? 100 : min(141 - E, 100);
+}
+
+// 2^e for -126 <= e <= 127.
+__device__ inline float exp2i(int e) { return __uint_as_float((uint32_t)(e + 127) << 23); }
+
+// Eight operand values of one row times the row's scale s, as FP16 hi and lo
+// parts, packed two per word in k order.  The scaled value v = x s is exact
+// (s a power of two); hi = RN16(v), and v - hi is exact in FP32.
+__device__ inline void split_f16(const float (&e)[8], float s, uint32_t (&hi)[4], uint32_t (&lo)[4])
+{
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+        const float v0 = e[2 * h] * s, v1 = e[2 * h + 1] * s;
+        const _Float16 h0 = (_Float16)v0, h1 = (_Float16)v1;
+        const _Float16 l0 = (_Float16)(v0 - (float)h0), l1 = (_Float16)(v1 - (float)h1);
+        hi[h] = (uint32_t)__builtin_bit_cast(unsigned short, h0) | ((uint32_t)__builtin_bit_cast(unsigned short, h1) << 16);
+        lo[h] = (uint32_t)__builtin_bit_cast(unsigned short, l0) | ((uint32_t)__builtin_bit_cast(unsigned short, l1) << 16);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -298,21 +366,27 @@ __global__ void __launch_bounds__(256) pre_operand_kernel(PreOpArgs a)
 //    that; the operands are split into P = 2 (hi, lo) or P = 3 (hi, mid, lo)
 //    BF16 parts and every product of parts whose orders sum to <= P-1 is
 //    accumulated in FP32 (3 or 6 products), smallest first;
+//  * the split-FP16 form uses v_mfma_f32_16x16x32_f16 on hi / lo FP16 parts
+//    of the row-scaled operand and of the scaled filter matrix (3 products,
+//    smallest first); the exact power-of-two factors are applied to the FP32
+//    results;
 //  * the FP32 form issues 8 v_mfma_f32_16x16x4_f32, sub-step q taking
 //    k = 32 s + 8 (l / 16) + q (a permutation of k applied to both operands).
 // D layout (16x16 family): lane l holds D[4 (l / 16) + i][l % 16], i = 0..3.
-// The B operand is packed on the host per (k step, column block, lane) and
-// staged through LDS once per workgroup and k step, with the next step's
-// tile prefetched into registers during the MFMAs.
+// The B operand is packed on the host per (k step, column block, lane) --
+// part-major per (k step, column block) in the split-FP16 form -- and copied
+// into LDS once per workgroup and k step, one step ahead into the other of
+// two buffers (global_load_lds, no registers).
 // ---------------------------------------------------------------------------
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
 // Native 16-byte vector for the B prefetch: HIP's uint4 is a struct whose
 // copies go through memory and would keep the prefetch in scratch.
 typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
 
-constexpr int kModeF32 = 0, kModeX3 = 2, kModeX6 = 3;
+constexpr int kModeF32 = 0, kModeX3 = 2, kModeX6 = 3, kModeH3 = 4;
 
 template <int MODE>
 struct FiltMode;
@@ -322,6 +396,8 @@ template <>
 struct FiltMode<kModeX3> { static constexpr int parts = 2; static constexpr int lane_bytes = 32; };
 template <>
 struct FiltMode<kModeX6> { static constexpr int parts = 3; static constexpr int lane_bytes = 48; };
+template <>
+struct FiltMode<kModeH3> { static constexpr int parts = 2; static constexpr int lane_bytes = 32; };
 
 __device__ inline float bf16_round(float f)
 {
@@ -379,9 +455,9 @@ constexpr int kStagePitch = 136;  // LDS row pitch (16-bit values) of the staged
 // the slot's row; a slot whose row lies beyond the view or the launch is
 // written as 0 by the strip's owner.
 __device__ void stage_strips_epilogue(const FiltArgs& a, const f32x4_t (&acc)[2][8], unsigned char* lds, int lane,
-                                      int wave)
+                                      int wave, int ctile, int rtile)
 {
-    const int T0 = blockIdx.y * 128, C0 = blockIdx.x * 128;
+    const int T0 = rtile * 128, C0 = ctile * 128;
     const int vt = T0 / a.V, vrow0 = T0 - vt * a.V;  // view and row (in its view) of the tile's first row
     uint16_t* t16 = reinterpret_cast<uint16_t*>(lds);
     __syncthreads();  // every wave is done with the B tile
@@ -517,12 +593,27 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) f
     // of the fused epilogue (128 rows at kStagePitch 16-bit values)
     constexpr int LDS_BYTES = 2 * TILE_BYTES > 128 * kStagePitch * 2 ? 2 * TILE_BYTES : 128 * kStagePitch * 2;
     __shared__ __attribute__((aligned(16))) unsigned char lds[LDS_BYTES];
+    constexpr bool H16 = MODE == kModeH3;
+    __shared__ float rowfac[H16 ? 128 : 1];  // split-FP16: the result factor of each row
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int ncb8 = (a.Wout + 127) / 128 * 8;         // packed column blocks (multiple of 8)
-    const int cb0 = blockIdx.x * 8;
-    const int rowTile = blockIdx.y * 128 + wave * 32;
+    // tile of this workgroup: the launch is 1-D and the hardware hands
+    // workgroup L to XCD L % 8, so XCD x takes a contiguous run of tiles in
+    // column-fastest order -- the column tiles of a row tile share one L2
+    // and read its operand rows from memory once
+    int ctile, rtile;
+    {
+        const int ntiles = gridDim.x, q = ntiles / 8, r = ntiles % 8;
+        const int L = blockIdx.x, x = L % 8, i = L / 8;
+        const int t = (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + i;
+        const int nct = (a.Wout + 127) / 128;
+        rtile = t / nct;
+        ctile = t - rtile * nct;
+    }
+    const int cb0 = ctile * 8;
+    const int rowTile = rtile * 128 + wave * 32;
     const int kq = 8 * (lane >> 4);
     const int nsteps = a.Kp / 32;
 
@@ -541,6 +632,36 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) f
     for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
         for (int c = 0; c < 8; ++c) acc[rb][c] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
+
+    // split-FP16: each row's scale from its largest magnitude -- from the
+    // operand's k-tile maxima, or over all of K from the FP32 rows (the four
+    // lanes l, l ^ 16, l ^ 32, l ^ 48 of a row share its k groups)
+    float rscale[2] = {1.0f, 1.0f};
+    if constexpr (H16) {
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb) {
+            float m = 0.0f;
+            if constexpr (TILED) {
+                const int rs = min(rowTile + rb * 16, (a.rows - 1) / 16 * 16);
+                const float* tm = reinterpret_cast<const float*>(reinterpret_cast<const float4*>(a.x) +
+                                                                 (size_t)(rs / 16) * a.bstride + nsteps * 128);
+                const int ntk = (a.Kp + 127) / 128;
+                for (int kt = lane >> 4; kt < ntk; kt += 4) m = fmaxf(m, tm[kt * 16 + (lane & 15)]);
+            } else {
+                for (int s = 0; s < nsteps; ++s) {
+                    float e[8];
+                    load_a8(a, xr[rb], pwr[rb], 32 * s + kq, e);
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) m = fmaxf(m, fabsf(e[q]));
+                }
+            }
+            m = fmaxf(m, __shfl_xor(m, 16));
+            m = fmaxf(m, __shfl_xor(m, 32));
+            const int se = row_scale_exp(m);
+            rscale[rb] = exp2i(se);
+            if (lane < 16) rowfac[wave * 32 + rb * 16 + lane] = exp2i(-se) * a.bscale;
+        }
+    }
 
     const u32x4_t* gB = reinterpret_cast<const u32x4_t*>(a.Bhl);
     auto gsrc = [&](int s) { return gB + ((size_t)s * ncb8 + cb0) * 64 * LB / 16; };
@@ -562,10 +683,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) f
     // operand iff it starts inside the launch)
     auto load_a = [&](int rb, int s, int kb, float (&e)[8]) {
         if (a.tiled) {
-            const int rs = blockIdx.y * 128 + wave * 32 + rb * 16;
+            const int rs = rtile * 128 + wave * 32 + rb * 16;
             float4 x0 = make_float4(0.f, 0.f, 0.f, 0.f), x1 = x0;
             if (rs < a.rows) {
-                const float4* src = reinterpret_cast<const float4*>(a.x) + ((size_t)(rs / 16) * nsteps + s) * 128 + lane;
+                const float4* src = reinterpret_cast<const float4*>(a.x) + (size_t)(rs / 16) * a.bstride + s * 128 + lane;
                 x0 = src[0];
                 x1 = src[64];
             }
@@ -580,8 +701,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) f
     // the last block (their results are never stored), so the loads carry
     // no condition and the wait for one step's A does not wait for the next
     auto load_tiled = [&](int rb, int s, float4 (&x)[2]) {
-        const int rs = min(blockIdx.y * 128 + wave * 32 + rb * 16, (a.rows - 1) / 16 * 16);
-        const float4* src = reinterpret_cast<const float4*>(a.x) + ((size_t)(rs / 16) * nsteps + s) * 128 + lane;
+        const int rs = min(rtile * 128 + wave * 32 + rb * 16, (a.rows - 1) / 16 * 16);
+        const float4* src = reinterpret_cast<const float4*>(a.x) + (size_t)(rs / 16) * a.bstride + s * 128 + lane;
         x[0] = src[0];
         x[1] = src[64];
     };
@@ -611,6 +732,48 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) f
 #pragma unroll
                     for (int q = 0; q < 8; ++q)
                         acc[rb][c] = __builtin_amdgcn_mfma_f32_16x16x4f32(ea[rb][q], eb[q], acc[rb][c], 0, 0, 0);
+            }
+        } else if constexpr (H16) {
+            f16x8_t ap[2][2];
+            float4 anext[2][2];
+            if constexpr (TILED) {
+#pragma unroll
+                for (int rb = 0; rb < 2; ++rb) load_tiled(rb, min(s + 1, nsteps - 1), anext[rb]);
+#pragma unroll
+                for (int rb = 0; rb < 2; ++rb) {
+                    const float e[8] = {acur[rb][0].x, acur[rb][0].y, acur[rb][0].z, acur[rb][0].w,
+                                        acur[rb][1].x, acur[rb][1].y, acur[rb][1].z, acur[rb][1].w};
+                    uint32_t hi[4], lo[4];
+                    split_f16(e, rscale[rb], hi, lo);
+                    ap[rb][0] = __builtin_bit_cast(f16x8_t, (u32x4_t){hi[0], hi[1], hi[2], hi[3]});
+                    ap[rb][1] = __builtin_bit_cast(f16x8_t, (u32x4_t){lo[0], lo[1], lo[2], lo[3]});
+                }
+            } else {
+#pragma unroll
+                for (int rb = 0; rb < 2; ++rb) {
+                    float e[8];
+                    load_a8(a, xr[rb], pwr[rb], kb, e);
+                    uint32_t hi[4], lo[4];
+                    split_f16(e, rscale[rb], hi, lo);
+                    ap[rb][0] = __builtin_bit_cast(f16x8_t, (u32x4_t){hi[0], hi[1], hi[2], hi[3]});
+                    ap[rb][1] = __builtin_bit_cast(f16x8_t, (u32x4_t){lo[0], lo[1], lo[2], lo[3]});
+                }
+            }
+            copy_b(min(s + 1, nsteps - 1), (s + 1) & 1);
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const f16x8_t* bp = reinterpret_cast<const f16x8_t*>(bt + c * 64 * LB) + lane;
+                const f16x8_t b0 = bp[0], b1 = bp[64];
+#pragma unroll
+                for (int rb = 0; rb < 2; ++rb) {
+                    acc[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ap[rb][1], b0, acc[rb][c], 0, 0, 0);
+                    acc[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ap[rb][0], b1, acc[rb][c], 0, 0, 0);
+                    acc[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ap[rb][0], b0, acc[rb][c], 0, 0, 0);
+                }
+            }
+            if constexpr (TILED) {
+#pragma unroll
+                for (int rb = 0; rb < 2; ++rb) { acur[rb][0] = anext[rb][0]; acur[rb][1] = anext[rb][1]; }
             }
         } else {
             bf16x8_t ap[2][P];
@@ -692,8 +855,20 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) f
         wait_lds_copies();
         __syncthreads();
     }
+    if constexpr (H16) {
+        // the row factors 2^-e_r * bscale (exact powers of two) of rows
+        // 4 (l / 16) + q of each row block (written by this wave)
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float rf = rowfac[wave * 32 + rb * 16 + 4 * (lane >> 4) + q];
+#pragma unroll
+                for (int c = 0; c < 8; ++c) acc[rb][c][q] *= rf;
+            }
+    }
     if (a.q != nullptr) {
-        stage_strips_epilogue(a, acc, lds, lane, wave);
+        stage_strips_epilogue(a, acc, lds, lane, wave, ctile, rtile);
         return;
     }
 #pragma unroll
@@ -724,6 +899,33 @@ static float bf16_to_float(unsigned short b)
     float f;
     std::memcpy(&f, &u, 4);
     return f;
+}
+
+// x (finite, |x| < 65520) rounded to the nearest FP16 value, ties to even,
+// subnormals included: the value, and its bit pattern in *bits.
+static double f16_rne(double x, unsigned short* bits)
+{
+    const unsigned short sign = std::signbit(x) ? 0x8000u : 0u;
+    const double ax = std::fabs(x);
+    if (ax == 0.0) {
+        *bits = sign;
+        return x;
+    }
+    int e;
+    std::frexp(ax, &e);                   // ax = f 2^e, f in [0.5, 1)
+    const int qe = std::max(e - 11, -24);  // quantum: 11 significant bits, or the subnormal step 2^-24
+    const double n = std::nearbyint(std::ldexp(ax, -qe));  // ties to even (default rounding mode)
+    const double v = std::ldexp(n, qe);
+    unsigned short b;
+    if (v < 0x1p-14) {
+        b = (unsigned short)n;  // subnormal: v = n 2^-24
+    } else {
+        int ev;
+        const double f = std::frexp(v, &ev);  // v = f 2^ev; FP16 exponent field ev + 14
+        b = (unsigned short)(((ev + 14) << 10) | (int)std::ldexp(2.0 * f - 1.0, 10));
+    }
+    *bits = (unsigned short)(sign | b);
+    return sign ? -v : v;
 }
 
 }  // namespace jh
@@ -762,7 +964,7 @@ jh_status create_filter(jh_filter_plan** plan, int device, int W, int V, int Wou
         require(nAdj >= 0 && (nAdj == 0 || (adjPts && adjDB)), "jh_filter_create: bad adjust points");
         require(window == JH_WINDOW_REFERENCE || window == JH_WINDOW_SHEPP_LOGAN, "jh_filter_create: unknown window");
         require(mode == JH_FILTER_FP32 || mode == JH_FILTER_MFMA_BF16X3 || mode == JH_FILTER_MFMA_BF16X6 ||
-                    mode == JH_FILTER_FP32_VALU,
+                    mode == JH_FILTER_FP32_VALU || mode == JH_FILTER_MFMA_F16X3,
                 "jh_filter_create: unknown mode");
         require(W % 4 == 0 || mode == JH_FILTER_FP32_VALU, "jh_filter_create: the MFMA forms need W % 4 == 0");
         int ndev = 0;
@@ -778,6 +980,8 @@ jh_status create_filter(jh_filter_plan** plan, int device, int W, int V, int Wou
             p->L = fft_length(W);
             p->K = W + 3;
             p->Kp = (p->K + 31) / 32 * 32;
+            p->bscale = 1.0f;
+            p->bstride = p->Kp * 4 + (mode == JH_FILTER_MFMA_F16X3 ? 4 * operand_ktiles(p) : 0);
             std::vector<double> pts(adjPts, adjPts + nAdj), dbs(adjDB, adjDB + nAdj);
             p->H = design_filter(W, halfSampleRate, cutStart, cutEnd, pts, dbs, window);
             // convolution kernel h = irFFT(H) / L (real, even)
@@ -832,7 +1036,34 @@ jh_status create_filter(jh_filter_plan** plan, int device, int W, int V, int Wou
                     return 0.0;
                 };
                 const size_t lanes = (size_t)nst * ncb8 * 64;
-                if (P == 0) {
+                if (mode == JH_FILTER_MFMA_F16X3) {
+                    // hi / lo FP16 parts of B 2^eb, eb such that max|B| 2^eb
+                    // lies in [2^13, 2^14): hi + lo keep 22 significant bits
+                    // (lo may be subnormal: absolute error <= 2^-25)
+                    double bmax = 0.0;
+                    for (int k = 0; k < p->K; ++k)
+                        for (int col = 0; col < Wout; ++col) bmax = std::max(bmax, std::fabs(bval(k, col)));
+                    int e = 0;
+                    if (bmax > 0.0) std::frexp(bmax, &e);
+                    const int eb = 14 - e;
+                    p->bscale = (float)std::ldexp(1.0, -eb);
+                    // part-major per (k step, column block): the 64 lanes' hi
+                    // parts, then their lo parts (each LDS read of a part is
+                    // 64 lanes x 16 contiguous bytes)
+                    std::vector<unsigned short> pk(lanes * 16);
+                    for (int s = 0; s < nst; ++s)
+                        for (int cb = 0; cb < ncb8; ++cb)
+                            for (int l = 0; l < 64; ++l) {
+                                unsigned short* dst = &pk[((size_t)s * ncb8 + cb) * 64 * 16 + l * 8];
+                                for (int q = 0; q < 8; ++q) {
+                                    const double v = std::ldexp(bval(32 * s + 8 * (l >> 4) + q, 16 * cb + (l & 15)), eb);
+                                    const double hi = f16_rne(v, &dst[q]);
+                                    f16_rne(v - hi, &dst[64 * 8 + q]);
+                                }
+                            }
+                    p->Bhl.alloc(sizeof(unsigned short) * pk.size());
+                    JH_HIP(hipMemcpy(p->Bhl.get(), pk.data(), sizeof(unsigned short) * pk.size(), hipMemcpyHostToDevice));
+                } else if (P == 0) {
                     std::vector<float> pk(lanes * 8);
                     for (int s = 0; s < nst; ++s)
                         for (int cb = 0; cb < ncb8; ++cb)
@@ -929,6 +1160,8 @@ void run_filter(jh_filter_plan* p, const float* d_in, float* d_out, int view0, i
     a.qtype = qtype;
     a.qs = qs;
     a.tiled = tiled;
+    a.bstride = p->bstride;
+    a.bscale = p->bscale;
     require((size_t)p->V * nviews < (1u << 31), "jh_filter_run: too many rows");
     hipStream_t st = as_stream(stream);
     if (p->mode == JH_FILTER_FP32_VALU) {
@@ -938,14 +1171,20 @@ void run_filter(jh_filter_plan* p, const float* d_in, float* d_out, int view0, i
         filter_fp32_kernel<<<grid, 256, 0, st>>>(a);
         check_launch("filter_fp32_kernel");
     } else {
-        dim3 grid((p->Wout + 127) / 128, (a.rows + 127) / 128);
-        require(grid.y <= 65535u, "jh_filter_run: too many rows for one launch");
+        // one workgroup per 128 x 128 output tile, 1-D (the kernel maps it to
+        // tiles XCD by XCD)
+        const size_t ntiles = (size_t)((p->Wout + 127) / 128) * ((a.rows + 127) / 128);
+        require(ntiles <= 2147483647u, "jh_filter_run: too many rows for one launch");
+        const dim3 grid((unsigned)ntiles);
         if (p->mode == JH_FILTER_FP32) {
             if (tiled) filter_mfma_kernel<kModeF32, true><<<grid, 256, 0, st>>>(a);
             else filter_mfma_kernel<kModeF32, false><<<grid, 256, 0, st>>>(a);
         } else if (p->mode == JH_FILTER_MFMA_BF16X3) {
             if (tiled) filter_mfma_kernel<kModeX3, true><<<grid, 256, 0, st>>>(a);
             else filter_mfma_kernel<kModeX3, false><<<grid, 256, 0, st>>>(a);
+        } else if (p->mode == JH_FILTER_MFMA_F16X3) {
+            if (tiled) filter_mfma_kernel<kModeH3, true><<<grid, 256, 0, st>>>(a);
+            else filter_mfma_kernel<kModeH3, false><<<grid, 256, 0, st>>>(a);
         } else {
             if (tiled) filter_mfma_kernel<kModeX6, true><<<grid, 256, 0, st>>>(a);
             else filter_mfma_kernel<kModeX6, false><<<grid, 256, 0, st>>>(a);
@@ -954,12 +1193,14 @@ void run_filter(jh_filter_plan* p, const float* d_in, float* d_out, int view0, i
     }
 }
 
-// Bytes of the pre-weighted A operand of nviews views: 16-row blocks x Kp.
+// Bytes of the pre-weighted A operand of nviews views: 16-row blocks of Kp
+// floats (plus the k-tile row maxima in the split-FP16 form).
 size_t operand_bytes(const jh_filter_plan* p, int nviews)
 {
     const size_t rows = (size_t)p->V * nviews;
-    return (rows + 15) / 16 * 16 * (size_t)p->Kp * sizeof(float);
+    return (rows + 15) / 16 * (size_t)p->bstride * 16;
 }
+
 
 }  // namespace
 
@@ -1090,6 +1331,8 @@ jh_status jh_pre_run_operand(jh_pre_plan* pre, jh_filter_plan* p, const uint16_t
         a.inW = pre->inW; a.inH = pre->inH; a.bUp = pre->bUp; a.bLeft = pre->bLeft;
         a.W = p->W; a.V = p->V; a.rows = (int)rows; a.nsteps = p->Kp / 32; a.view0 = view0;
         a.edge0 = p->mirror ? p->W - 3 : 0;
+        a.bstride = p->bstride;
+        a.rowmax = p->mode == JH_FILTER_MFMA_F16X3 ? 1 : 0;
         a.bf = pre->brightField; a.a0 = pre->bh[0]; a.a1 = pre->bh[1]; a.a2 = pre->bh[2];
         a.doBH = pre->doBH ? 1 : 0;
         // 16-byte loads need 8-count alignment of every row start (and of the frames and tables)
@@ -1097,7 +1340,8 @@ jh_status jh_pre_run_operand(jh_pre_plan* pre, jh_filter_plan* p, const uint16_t
                  reinterpret_cast<uintptr_t>(d_raw) % 16 == 0 &&
                  (d_dark == nullptr || reinterpret_cast<uintptr_t>(d_dark) % 16 == 0) &&
                  (d_gain == nullptr || reinterpret_cast<uintptr_t>(d_gain) % 16 == 0)) ? 1 : 0;
-        const dim3 grid((unsigned)((p->Kp + 127) / 128), (unsigned)((rows + 15) / 16));
+        const unsigned nblk = (unsigned)((rows + 15) / 16);
+        const dim3 grid((unsigned)((p->Kp + 127) / 128), nblk);
         pre_operand_kernel<<<grid, 256, 0, as_stream(stream)>>>(a);
         check_launch("pre_operand_kernel");
     });

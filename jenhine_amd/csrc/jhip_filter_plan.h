@@ -13,10 +13,16 @@ struct jh_filter_plan {
     int mode;
     int nblocks;
     double gain_bound;            // max over output columns of sum_k |B[k][n]| (FP64): |out| <= gain_bound * max|in|
+    float bscale;                 // split-FP16 form: the packed matrix is B / bscale (bscale a power of two), else 1
+    int bstride;                  // float4s per 16-row block of the A operand (Kp * 4, plus 4 per k tile in FP16)
     std::vector<int> view_block;  // block index of every view
     std::vector<double> H;        // L/2 + 1
     jh::DevBuf B;                 // Kp x Wout float (vector-ALU form)
-    jh::DevBuf Bhl;               // MFMA forms: packed FP32 or BF16-part operand
+    jh::DevBuf Bhl;               // MFMA forms: packed FP32, BF16-part or FP16-part operand
     jh::DevBuf pw;                // nblocks x V x W float
     jh::DevBuf blk;               // N int: block of every view (device copy)
 };
+
+// 128-column k tiles of the A operand (the split-FP16 form keeps the row
+// maxima of each).
+inline int operand_ktiles(const jh_filter_plan* p) { return (p->Kp + 127) / 128; }

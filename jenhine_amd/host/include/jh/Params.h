@@ -49,7 +49,7 @@ struct ReconParams {
     std::vector<float> FilterAdjustLevelInDB;
     // extensions of this back end
     int FilterWindow = 0;    // JH_WINDOW_REFERENCE or JH_WINDOW_SHEPP_LOGAN
-    int FilterMode = 0;      // JH_FILTER_FP32, JH_FILTER_MFMA_BF16X3, JH_FILTER_MFMA_BF16X6 or JH_FILTER_FP32_VALU
+    int FilterMode = 0;      // JH_FILTER_FP32, _MFMA_BF16X3, _MFMA_BF16X6, _FP32_VALU or _MFMA_F16X3
     int StorageType = 0;     // JH_F32, JH_BF16 or JH_F16
     size_t ViewsPerChunk = 8;
     // the production chain's two stages around the backprojection, fused

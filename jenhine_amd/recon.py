@@ -129,7 +129,7 @@ class FilterCore:
         self.n_views = ptm.shape[0]
         self.mode = mode
         m = {"fp32": _lib.JH_FILTER_FP32, "mfma": _lib.JH_FILTER_MFMA_BF16X3, "mfma6": _lib.JH_FILTER_MFMA_BF16X6,
-             "fp32valu": _lib.JH_FILTER_FP32_VALU}[mode]
+             "fp32valu": _lib.JH_FILTER_FP32_VALU, "mfma16": _lib.JH_FILTER_MFMA_F16X3}[mode]
         n = len(adj_pts)
         pts = (ctypes.c_float * max(n, 1))(*adj_pts)
         db = (ctypes.c_float * max(n, 1))(*adj_db)

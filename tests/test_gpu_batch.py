@@ -30,7 +30,7 @@ def _workload(i, n, views, det):
 
 # (volume edge, views, detector edge, storage, filter form, views per chunk)
 SPECS = [(48, 60, 96, "bf16", "mfma6", 16), (64, 90, 128, "bf16", "mfma6", 25), (40, 45, 80, "f32", "fp32", 45),
-         (56, 72, 112, "f16", "mfma6", 32), (32, 30, 64, "bf16", "mfma", 7)]
+         (56, 72, 112, "f16", "mfma16", 32), (32, 30, 64, "bf16", "mfma", 7), (48, 40, 96, "bf16", "mfma16", 9)]
 
 
 @pytest.fixture(scope="module")
@@ -219,7 +219,7 @@ def test_graph_config4_sizes_vs_fp64(gpu, config4_scans, storage, tol):
     keep = []
     for c in config4_scans:
         w, d = c["w"], c["d"]
-        fdk = FDK(w.scan_params(), w.recon_params(), d["ptm"], "mfma6", storage, device=gpu)
+        fdk = FDK(w.scan_params(), w.recon_params(), d["ptm"], "mfma16", storage, device=gpu)
         raw, dark, gain = _u16(d["raw"], gpu), _u16(d["dark"], gpu), torch.from_numpy(d["gain"]).to(gpu)
         vol = torch.empty((w.Z, w.Y, w.X), dtype=torch.float32, device=gpu)
         g.add(fdk, raw, vol, dark, gain)
@@ -249,7 +249,7 @@ def test_graph_sizes_the_operand_buffer_for_every_chunk(gpu):
     w = configs.Workload("tall", 300.0, 500.0, 600, 64, 2048, 0.15, 16, 16, 16, 0.25, 0.25, n_ellipsoids=3,
                          window=0, ramlak=True, seed=77)
     d = w.data()
-    fdk = FDK(w.scan_params(), w.recon_params(), d["ptm"], "mfma6", "bf16", device=gpu)
+    fdk = FDK(w.scan_params(), w.recon_params(), d["ptm"], "mfma16", "bf16", device=gpu)
     assert fdk.fused()
     s = dict(fdk=fdk, raw=_u16(d["raw"], gpu), dark=_u16(d["dark"], gpu), gain=torch.from_numpy(d["gain"]).to(gpu),
              chunk=513)

@@ -59,7 +59,8 @@ def test_dental512_row_step_takes_the_hi1_pair_branch():
     assert 1.015625 <= step <= 1.984375, step
 
 
-@pytest.mark.parametrize("filter_mode,storage", [("fp32", "f32"), ("mfma6", "bf16"), ("mfma6", "f16")])
+@pytest.mark.parametrize("filter_mode,storage", [("fp32", "f32"), ("mfma16", "bf16"), ("mfma16", "f16"),
+                                                 ("mfma6", "bf16")])
 def test_dental512_fdk_vs_oracle(gpu, data, oracle64, oracle32, filter_mode, storage):
     from jenhine_amd.recon import FDK
     fdk = FDK(W.scan_params(), W.recon_params(), data["ptm"], filter_mode, storage, device=gpu)

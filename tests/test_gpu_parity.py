@@ -4,7 +4,9 @@ Tolerances:
   * preprocessing: integer steps bit-exact; float result within 2 ulp of the
     FP32-ref oracle (the only non-exact step is ln: OCML logf vs glibc logf);
   * filter FP32 (MFMA and vector ALU) and hi/mid/lo BF16 (6 products):
-    rel-RMSE <= 2e-6 vs the FP64 oracle; hi/lo BF16 (3 products): <= 3e-4
+    rel-RMSE <= 2e-6 vs the FP64 oracle; row-scaled hi/lo FP16 (3 products):
+    <= 3e-6 (22 significant bits per operand against FP32's 24, so about
+    twice the FP32 forms' ~1e-6); hi/lo BF16 (3 products): <= 3e-4
     (each operand keeps ~17 significant bits, and the ramp filter's
     cancellation, sum|terms| / |result| ~ 20, amplifies that);
   * backprojection FP32: rel-RMSE <= 1e-4 vs the FP32-ref oracle volume (the
@@ -136,7 +138,7 @@ def test_filter_fp32_vs_fp64_oracle(gpu, small_data):
     assert e <= 2e-6
 
 
-@pytest.mark.parametrize("mode,tol", [("mfma", 3e-4), ("mfma6", 2e-6), ("fp32valu", 2e-6)])
+@pytest.mark.parametrize("mode,tol", [("mfma", 3e-4), ("mfma6", 2e-6), ("fp32valu", 2e-6), ("mfma16", 3e-6)])
 def test_filter_other_forms_vs_fp64_oracle(gpu, small_data, mode, tol):
     flt, out, ref, H, pws = _filter_case(gpu, small_data, mode)
     e = rel_rmse(out, ref)
@@ -155,10 +157,43 @@ def test_filter_mfma_partial_column_tile(gpu):
     src, blk = o.preweight_schedule(3)
     pws = np.stack([o.prefilter_weight(ptm[s], 200, 40, scan.DSO, fp64=True) for s in src])
     ref = o.filter_rows(frames.astype(np.float64), H, pws, blk, fp64=True)
-    for mode, tol in (("fp32", 2e-6), ("mfma6", 2e-6), ("mfma", 3e-4)):
+    for mode, tol in (("fp32", 2e-6), ("mfma6", 2e-6), ("mfma", 3e-4), ("mfma16", 3e-6)):
         flt = FilterCore(200, 40, ptm, scan.DSO, scan.HalfSampleRate, window=1, mode=mode, device=gpu)
         out = flt.process(torch.from_numpy(frames).to(gpu)).cpu().numpy()
         assert rel_rmse(out, ref) <= tol, mode
+
+
+def test_filter_preweight_schedule_on_a_non_circular_orbit(gpu):
+    """Source distance, detector distance and the principal point change from
+    view to view, so every 9-view block has its own pre-weight map
+    (FilterAgent/FilterAgent.cpp:129-145): the device filter must pick the
+    same source view per block as the oracle's schedule."""
+    from jenhine_amd.recon import FilterCore
+    n, W, V, dso = 25, 96, 40, 300.0
+
+    def view(k):
+        P = geometry.circular_ptms(1, dso + 80.0 * np.sin(0.7 * k), 450.0 + 60.0 * np.cos(0.3 * k), W, V, 0.6,
+                                   start_deg=360.0 * k / n, u_shift=25.0 * np.sin(k))
+        return geometry.move_v(P, 12.0 * np.cos(1.3 * k))
+
+    P = np.concatenate([view(k) for k in range(n)])
+    frames = np.ascontiguousarray(synth.uniform(11, (n, V, W), 0.0, 3.0))
+    H = o.filter_design(W, 0.5, window=1, fp64=True)
+    src, blk = o.preweight_schedule(n)
+    assert len(src) == 3
+    pws = np.stack([o.prefilter_weight(P[s], W, V, dso, fp64=True) for s in src])
+    ref = o.filter_rows(frames.astype(np.float64), H, pws, blk, fp64=True)
+    # the same rows filtered with another view of each block as the source
+    # differ by far more than the bar, so a schedule mistake cannot pass
+    for shift in (-4, -3, 1):
+        other = np.stack([o.prefilter_weight(P[s + shift], W, V, dso, fp64=True) for s in src])
+        assert rel_rmse(o.filter_rows(frames.astype(np.float64), H, other, blk, fp64=True), ref) > 1e-4
+    for mode, tol in (("fp32", 2e-6), ("mfma6", 2e-6), ("mfma16", 3e-6)):
+        flt = FilterCore(W, V, P, dso, 0.5, window=1, mode=mode, device=gpu)
+        out = flt.process(torch.from_numpy(frames).to(gpu)).cpu().numpy()
+        e = rel_rmse(out, ref)
+        print("non-circular filter", mode, e)
+        assert e <= tol, mode
 
 
 # --------------------------------------------------------------------------
@@ -750,16 +785,18 @@ def test_filter_run_staged_refuses_plans_that_do_not_fuse(gpu):
         mfma.process_staged(x, bp, "bf16")
 
 
+@pytest.mark.parametrize("mode", ["mfma6", "mfma16"])
 @pytest.mark.parametrize("dtype", ["bf16", "f16"])
 @pytest.mark.parametrize("width", [200, 196])  # 16-byte loads, and the per-pixel path (rows not 8-aligned)
-def test_pre_operand_path_equals_two_kernel_chain(gpu, dtype, width):
+def test_pre_operand_path_equals_two_kernel_chain(gpu, dtype, width, mode):
     """jh_pre_run_operand + jh_filter_run_operand_staged against jh_pre_run +
     jh_filter_run_staged, to the bit: a frame width that is not a multiple of
     the 32-wide k step, a row count that is not a multiple of the 16-row
     block, dark and gain correction and beam hardening, a view range other
     than the first; and the operand itself against its definition (pixel
     times pre-weight, the three unweighted edge columns, zeros beyond, in the
-    MFMA tile order)."""
+    MFMA tile order; the split-FP16 form also keeps each row's largest
+    magnitude per 128-column k tile)."""
     from jenhine_amd.recon import BPCore, FilterCore, PreProcessingCore
     w = SMALL.scaled(n_views=5, n_u=width, n_v=70, name="ragged-operand")
     N, V, W = w.n_views, w.n_v, w.n_u
@@ -767,7 +804,7 @@ def test_pre_operand_path_equals_two_kernel_chain(gpu, dtype, width):
     ptm = w.ptms()
     scan = w.scan_params()
     pre = PreProcessingCore(W, V, bright_field=4000.0, bh=(1.0, 0.03, 0.004), device=gpu)
-    filt = FilterCore(W, V, ptm, scan.DSO, scan.HalfSampleRate, mode="mfma6", device=gpu)
+    filt = FilterCore(W, V, ptm, scan.DSO, scan.HalfSampleRate, mode=mode, device=gpu)
     bp = BPCore(ptm, W, V, w.X, w.Y, w.Z, w.pxy, w.pz, device=gpu)
     raw, dark = _u16(d["raw"], gpu), _u16(d["dark"], gpu)
     gain = torch.from_numpy(d["gain"]).to(gpu)
@@ -781,23 +818,34 @@ def test_pre_operand_path_equals_two_kernel_chain(gpu, dtype, width):
                                        view0=2)
     assert torch.equal(got2, ref2)
     # the operand against its definition
-    nsteps = (filt.operand_bytes(16) // 4) // (16 * V) // 32  # 16 views: V*16 rows, a multiple of 16
-    Kp = nsteps * 32
+    Kp = (W + 3 + 31) // 32 * 32
+    nsteps, ntk = Kp // 32, (Kp + 127) // 128
+    extra = 16 * ntk if mode == "mfma16" else 0   # the split-FP16 form's k-tile row maxima
+    assert filt.operand_bytes(16) == 4 * V * (16 * Kp + extra)  # 16 views: V blocks of 16 rows
     pn = p.cpu().numpy().reshape(N * V, W)
     pw = np.stack([filt.preweight(v).cpu().numpy() for v in range(N)]).reshape(N * V, W)
     rows = N * V
-    full = np.zeros(((rows + 15) // 16 * 16, Kp), np.float32)
+    nb = (rows + 15) // 16
+    full = np.zeros((nb * 16, Kp), np.float32)
     full[:rows, :W] = pn * pw
     full[:rows, W:W + 3] = pn[:, 0:3]
+    ab = a.cpu().numpy()[: nb * (16 * Kp + extra)].reshape(nb, 16 * Kp + extra)
     # tile order: ((r16 * nsteps + s) * 2 + h) * 64 + 16 g + i, 4 floats
     t = full.reshape(-1, 16, nsteps, 4, 2, 4)           # r16, i, s, g, h, e
-    t = t.transpose(0, 2, 4, 3, 1, 5).reshape(-1)       # r16, s, h, g, i, e
-    np.testing.assert_array_equal(a.cpu().numpy()[: t.size], t)
+    t = t.transpose(0, 2, 4, 3, 1, 5).reshape(nb, -1)   # r16, s, h, g, i, e
+    np.testing.assert_array_equal(ab[:, :16 * Kp], t)
+    if extra:
+        # per block: tile kt's maxima of |value| over columns 128 kt .. 128 kt + 127, 16 rows each
+        fp = np.zeros((nb * 16, ntk * 128), np.float32)
+        fp[:, :Kp] = np.abs(full)
+        want = fp.reshape(nb, 16, ntk, 128).max(axis=3).transpose(0, 2, 1).reshape(nb, -1)
+        np.testing.assert_array_equal(ab[:, 16 * Kp:], want)
 
 
+@pytest.mark.parametrize("mode", ["mfma6", "mfma16"])
 @pytest.mark.parametrize("dtype", ["bf16", "f16"])
 @pytest.mark.parametrize("right", [True, False])
-def test_composite_operand_path_equals_composite_then_filter(gpu, dtype, right):
+def test_composite_operand_path_equals_composite_then_filter(gpu, dtype, right, mode):
     """jh_composite_run_operand + jh_filter_run_operand_staged against
     jh_composite_run + jh_filter_run_staged, to the bit, through the 2W -> W
     filter with and without yMirroring: each half of the scan is one call
@@ -810,7 +858,7 @@ def test_composite_operand_path_equals_composite_then_filter(gpu, dtype, right):
     scan = w.scan_params()
     comp = CompositeCore(W, V, ptm, w.sod, device=gpu)
     assert comp.detector_on_right == right
-    filt = FilterCore(2 * W, V, ptm, scan.DSO, scan.HalfSampleRate, mode="mfma6", device=gpu, w_out=W,
+    filt = FilterCore(2 * W, V, ptm, scan.DSO, scan.HalfSampleRate, mode=mode, device=gpu, w_out=W,
                       y_mirroring=right)
     bp = BPCore(ptm, W, V, w.X, w.Y, w.Z, w.pxy, w.pz, device=gpu)
     assert filt.stage_fusable(bp, dtype)
@@ -828,14 +876,15 @@ def test_composite_operand_path_equals_composite_then_filter(gpu, dtype, right):
     assert torch.equal(got2, ref2)
 
 
-def test_fdk_composite_operand_path_equals_stage_by_stage(gpu):
+@pytest.mark.parametrize("mode", ["mfma6", "mfma16"])
+def test_fdk_composite_operand_path_equals_stage_by_stage(gpu, mode):
     """The offset-detector production chain (pre -> composite into the
     operand -> filter with fused staging -> bp) gives the volume of the
     stage-by-stage chain to the bit."""
     from jenhine_amd.recon import FDK
     w = configs.OFFSET128.scaled(n_views=16)
     d = w.data()
-    fdk = FDK(w.scan_params(), w.recon_params(), d["ptm"], "mfma6", "bf16", device=gpu, composite=True)
+    fdk = FDK(w.scan_params(), w.recon_params(), d["ptm"], mode, "bf16", device=gpu, composite=True)
     assert fdk.composite_operand_path()
     raw, dark = _u16(d["raw"], gpu), _u16(d["dark"], gpu)
     gain = torch.from_numpy(d["gain"]).to(gpu)
@@ -844,14 +893,15 @@ def test_fdk_composite_operand_path_equals_stage_by_stage(gpu):
     np.testing.assert_array_equal(vol, ref.cpu().numpy())
 
 
-def test_fdk_operand_path_equals_stage_by_stage(gpu):
+@pytest.mark.parametrize("mode", ["mfma6", "mfma16"])
+def test_fdk_operand_path_equals_stage_by_stage(gpu, mode):
     """The production chain (pre -> operand -> filter with fused staging ->
     bp) gives the volume of the stage-by-stage chain to the bit."""
     from jenhine_amd.recon import FDK
     w = SMALL
     d = small_data_cache()
     sc, rc = w.scan_params(), w.recon_params()
-    fdk = FDK(sc, rc, d["ptm"], "mfma6", "bf16", gpu)
+    fdk = FDK(sc, rc, d["ptm"], mode, "bf16", gpu)
     assert fdk.operand_path()
     raw = _u16(d["raw"], gpu)
     vol = fdk(raw).cpu().numpy()
