@@ -81,6 +81,8 @@ def main():
                     help="storage form of the --kernel-only backprojection")
     ap.add_argument("--kernel-tile", default=None,
                     help="zc,ty tile of the --kernel-only backprojection (default: the autotuner's choice)")
+    ap.add_argument("--kernel-fetch", choices=("gather", "lds"), default=None,
+                    help="sample fetch of the --kernel-only backprojection (default: the autotuner's choice)")
     ap.add_argument("--kernel-filter", action="store_true",
                     help="with --kernel-only: time the headline chain's filter (pre-weighted operand in, "
                          "staged strips out) instead of the backprojection")
@@ -158,6 +160,8 @@ def main():
         q = fdk.bp.stage(fdk.filter.process(fdk.pre.process(raw, dark, gain)), fdk.storage)
         if args.kernel_tile:
             fdk.bp.set_tile(*(int(x) for x in args.kernel_tile.split(",")))
+            if args.kernel_fetch:
+                fdk.bp.set_fetch(args.kernel_fetch)
         else:
             fdk.bp.autotune(q, fdk.storage, reps=1)
         vol = fdk.bp.run(q, fdk.storage)
@@ -167,7 +171,7 @@ def main():
             fdk.bp.run(q, fdk.storage, out=vol)
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t) / args.steps * 1e3
-        log(f"bp_kernel {fdk.storage}: {ms:.2f} ms/launch (tile {fdk.bp.tile()}), "
+        log(f"bp_kernel {fdk.storage}: {ms:.2f} ms/launch (tile {fdk.bp.tile()}, {fdk.bp.fetch()}), "
             f"copy kernel moved {2 * src.numel() * 4} bytes")
         print(json.dumps({"kernel_only": True, "bp_ms": ms, "copy_bytes": 2 * src.numel() * 4}))
         return

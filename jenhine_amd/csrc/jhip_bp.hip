@@ -536,7 +536,7 @@ struct BPArgs {
     uint32_t frame_bytes;
     int ct;              // apply the CT-number map to normalised output
     float ct_scale, ct_norm1;
-    int lds_px;          // LDS-staged fetch: pixels per coefficient buffer (two buffers)
+    int lds_px;          // LDS-staged fetch: pixels per coefficient buffer (two buffers); strip kernel: copy rounds per view
     int pxb;             // bytes per staged pixel (16: FP32 quads; 8: BF16/FP16 quads and strips)
     int row_px;          // staged pixels per detector row (W, or W + 1 for strips)
     float wscale;        // distance-weighted outputs: vol = wscale * acc
@@ -570,10 +570,11 @@ __device__ __forceinline__ float add_sample(float acc, float s, float iw2)
 // 16-byte gather at row floor(sv_k) serves samples k and k+1; the second
 // sample's rows start HI or HI + 1 rows down.  Each sample interpolates the
 // same row pairs with the same operations as the quad layout's one gather per
-// sample, so the results are identical.
-template <int ZC, int QT, int HI, bool WT>
-__device__ __forceinline__ void bp_column_pairs(float* acc, __amdgpu_buffer_rsrc_t rs, uint32_t rowb, uint32_t ub,
-                                                float sv0, float dsv, typename Sampler<QT>::wt_t wq, float iw2)
+// sample, so the results are identical.  fetch(off) returns the 16-byte
+// window at byte offset off = row * rowb + ub (from memory, or from LDS).
+template <int ZC, int QT, int HI, bool WT, class F>
+__device__ __forceinline__ void bp_column_pairs(float* acc, const F& fetch, uint32_t rowb, uint32_t ub, float sv0,
+                                                float dsv, typename Sampler<QT>::wt_t wq, float iw2)
 {
     typedef Sampler<QT> S;
     constexpr int NP = ZC / 2;
@@ -592,7 +593,7 @@ __device__ __forceinline__ void bp_column_pairs(float* acc, __amdgpu_buffer_rsrc
     }
     u32x4_t p[NP];
 #pragma unroll
-    for (int m = 0; m < NP; ++m) p[m] = S::load(rs, off[m]);
+    for (int m = 0; m < NP; ++m) p[m] = fetch(off[m]);
 #pragma unroll
     for (int m = 0; m < NP; ++m) {
         const uint32_t r0 = S::template rowpair<0>(p[m]);
@@ -698,11 +699,12 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
                     // d = floor(sv_k+1) - floor(sv_k), which is hi or hi+1
                     bool paired = false;
                     if constexpr (is_strip<QT>()) {
+                        const auto gather = [&](uint32_t o) { return S::load(rs, o); };
                         if (dsv >= 0.0f && dsv <= 0.984375f) {
-                            bp_column_pairs<ZC, QT, 0, WT>(acc, rs, rowb, ub, sv0, dsv, wq, iw2);
+                            bp_column_pairs<ZC, QT, 0, WT>(acc, gather, rowb, ub, sv0, dsv, wq, iw2);
                             paired = true;
                         } else if (dsv >= 1.015625f && dsv <= 1.984375f) {
-                            bp_column_pairs<ZC, QT, 1, WT>(acc, rs, rowb, ub, sv0, dsv, wq, iw2);
+                            bp_column_pairs<ZC, QT, 1, WT>(acc, gather, rowb, ub, sv0, dsv, wq, iw2);
                             paired = true;
                         }
                     }
@@ -1177,6 +1179,329 @@ __global__ void __launch_bounds__(256) bp_colcount_kernel(const Geo* __restrict_
     colcnt[(size_t)j * X + i] = c;
 }
 
+// ---------------------------------------------------------------------------
+// LDS-staged column strips (geometry class 0, BF16 / FP16 column strips,
+// count-normalised forms).
+//
+// Per view the workgroup copies the strip pixels of its block's detector
+// footprint -- strip rows [v0, v1] (a strip pixel holds its row and the three
+// below), pixels [u0, u1] (a sample's window is pixels u, u+1; the padding
+// column W included) -- from the staged frame into LDS unchanged, 16 bytes
+// per lane and instruction with buffer_load ... lds (no registers; a chunk
+// reaching past the frame reads 0), and samples there with the operations of
+// the gather form: a sample's 16-byte window is one ds_read2_b64, and one
+// window serves a pair of z samples where the gather form pairs them, so the
+// volume is the gather form's to the bit.  A dedicated copy wave keeps the
+// next views' copies in flight while the other waves sample (see
+// bp_tile_strip_lds).  What this buys: each footprint pixel passes the
+// texture path once per workgroup instead of once per sampling lane, and
+// the window reads (8 B per update) run at the LDS rate, twice the texture
+// path's 64 B per clock and CU.
+// ---------------------------------------------------------------------------
+
+// The strip rectangle a block of voxel columns [cx0, cx1] x [cy0, cy1], z
+// voxels fk0 .. fk0 + ZC - 1, samples in a view: first pixel u0, first strip
+// row v0, 16-byte chunks per row and in all.  Evaluated by the 4 lanes of
+// every lane group of 4 (one per corner column; the result is the group's):
+// u/w and v/w are linear-fractional in the voxel coordinates, so their
+// extremes over the block are at its corners; the reciprocal is the one the
+// samples take, and one pixel of margin on each side covers the difference.
+// A window is pixels u, u+1 (the padding column W included) of one strip row.
+struct StripBox {
+    int u0, v0, cpr, nch;
+};
+
+template <int ZC>
+__device__ __forceinline__ StripBox strip_footprint(const Geo& g, float cx0, float cx1, float cy0, float cy1, float fk0,
+                                                    int W, int V)
+{
+    const int c = threadIdx.x & 3;
+    const float cx = (c & 1) ? cx1 : cx0, cy = (c & 2) ? cy1 : cy0;
+    const float U = fmaf(g.c[3], fk0, fmaf(g.c[1], cx, fmaf(g.c[2], cy, g.c[0])));
+    const float Vv = fmaf(g.c[7], fk0, fmaf(g.c[5], cx, fmaf(g.c[6], cy, g.c[4])));
+    const float Ww = fmaf(g.c[11], fk0, fmaf(g.c[9], cx, fmaf(g.c[10], cy, g.c[8])));
+    const float r = __builtin_amdgcn_rcpf(Ww);
+    const float su = fmaf(U, r, 0.5f);
+    const float s0 = fmaf(Vv, r, 0.5f);
+    const float sl = fmaf((float)(ZC - 1), g.c[7] * r, s0);
+    float umin = su, umax = su, vmin = fminf(s0, sl), vmax = fmaxf(s0, sl);
+#pragma unroll
+    for (int m = 1; m <= 2; m <<= 1) {
+        umin = fminf(umin, __shfl_xor(umin, m));
+        umax = fmaxf(umax, __shfl_xor(umax, m));
+        vmin = fminf(vmin, __shfl_xor(vmin, m));
+        vmax = fmaxf(vmax, __shfl_xor(vmax, m));
+    }
+    const float Wm1 = (float)(W - 1), Vm1 = (float)(V - 1);
+    StripBox b;
+    b.u0 = (int)fminf(fmaxf(floorf(umin) - 1.0f, 0.0f), Wm1);
+    const int u1 = (int)fminf(fmaxf(floorf(umax) + 2.0f, 0.0f), (float)W);
+    b.v0 = (int)fminf(fmaxf(floorf(vmin) - 1.0f, 0.0f), Vm1);
+    const int v1 = (int)fminf(fmaxf(floorf(vmax) + 1.0f, 0.0f), Vm1);
+    b.cpr = (u1 - b.u0 + 2) >> 1;
+    b.nch = b.cpr * (v1 - b.v0 + 1);
+    return b;
+}
+
+// The largest strip rectangle (in chunks) over the tiles of a tile shape and
+// views view0 .. view0 + nviews - 1, into *out (atomic max): lane group g of
+// the grid takes tile g / nviews, view g % nviews (workgroup and tile
+// extents as the backprojection kernel's).
+template <int ZC>
+__global__ void strip_footprint_max_kernel(const Geo* __restrict__ geo, int view0, int nviews, int ntx, int nty, int ntz,
+                                           int ex, int ey, int X, int Y, int W, int V, int* __restrict__ out)
+{
+    const size_t grp = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 2;
+    const size_t total = (size_t)ntx * nty * ntz * nviews;
+    const size_t gi = grp < total ? grp : total - 1;  // the lanes of a group stay together
+    const int vv = (int)(gi % nviews);
+    const size_t tile = gi / nviews;
+    const int tx = (int)(tile % ntx), tyi = (int)((tile / ntx) % nty), tz = (int)(tile / ((size_t)ntx * nty));
+    const float cx0 = (float)min(tx * ex, X - 1), cx1 = (float)min(tx * ex + ex - 1, X - 1);
+    const float cy0 = (float)min(tyi * ey, Y - 1), cy1 = (float)min(tyi * ey + ey - 1, Y - 1);
+    const StripBox b = strip_footprint<ZC>(geo[view0 + vv], cx0, cx1, cy0, cy1, (float)(tz * ZC), W, V);
+    if ((threadIdx.x & 3) == 0 && grp < total) atomicMax(out, b.nch);
+}
+
+// s_waitcnt vmcnt(k), 0 <= k <= 63 (expcnt and lgkmcnt at their maxima): at
+// most k vector-memory operations of this wave are outstanding (they complete
+// in issue order).  The count is an immediate of the instruction.
+__device__ __forceinline__ void wait_vmcnt(int k)
+{
+#define JH_WAIT_CASE(n) \
+    case n: __builtin_amdgcn_s_waitcnt(((n) & 15) | (((n) >> 4) << 14) | 0x0F70); break;
+#define JH_WAIT_CASE8(n) \
+    JH_WAIT_CASE(n) JH_WAIT_CASE(n + 1) JH_WAIT_CASE(n + 2) JH_WAIT_CASE(n + 3) JH_WAIT_CASE(n + 4) \
+    JH_WAIT_CASE(n + 5) JH_WAIT_CASE(n + 6) JH_WAIT_CASE(n + 7)
+    switch (k) {
+        JH_WAIT_CASE8(0) JH_WAIT_CASE8(8) JH_WAIT_CASE8(16) JH_WAIT_CASE8(24) JH_WAIT_CASE8(32)
+        JH_WAIT_CASE8(40) JH_WAIT_CASE8(48) JH_WAIT_CASE8(56)
+    default: __builtin_amdgcn_s_waitcnt(0x0F70); break;
+    }
+#undef JH_WAIT_CASE8
+#undef JH_WAIT_CASE
+}
+
+// A view's geometry through the constant address space: scalar loads, which
+// the copies into LDS (vector-memory operations that write memory) do not
+// hold up -- a vector load of it would make the wave wait for every copy in
+// flight before its own result.
+__device__ __forceinline__ Geo geo_of(const Geo* geo, int view)
+{
+    const __attribute__((address_space(4))) float* p = (const __attribute__((address_space(4))) float*)(geo + view);
+    Geo g;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) g.c[k] = p[k];
+    return g;
+}
+
+constexpr int kStripAhead = 2;               // views in flight beyond the one sampled
+constexpr int kStripBufs = kStripAhead + 1;  // footprint buffers (and boxes) per workgroup
+
+// The workgroup: blockDim.y - 1 sampling waves (the voxel columns of
+// bp_tile) and one copy wave, the last.  Per view v the copy wave computes
+// the footprint of view v + kStripAhead, issues its copy (a.lds_px rounds of
+// 64 chunks of 16 bytes, whatever the footprint's size: the rounds past it
+// copy its last chunk again into the buffer's slack), then waits until view
+// v + 1 has landed (the copies complete in issue order), while the sampling
+// waves sample view v; one barrier per view.  Only the copy wave has copies
+// in flight, so the waits the compiler places before the sampling waves'
+// LDS reads (for copies that might write what they read) do not stall them.
+template <int ZC, int QT, int OUT>
+__device__ __forceinline__ void bp_tile_strip_lds(const BPArgs& a, unsigned tile)
+{
+    static_assert(is_strip<QT>() && (OUT == kOutNormalised || OUT == kOutAccumulate),
+                  "column strips, count-normalised forms");
+    typedef Sampler<QT> S;
+    __shared__ int box[kStripBufs][4];  // u0, v0, chunks per row, chunks
+    __shared__ float cnt_part[ZC * 256];
+    const int nsw = blockDim.y - 1;  // sampling waves
+    const bool copier = threadIdx.y == nsw;
+    const int tx = tile % a.ntx;
+    const unsigned rest = tile / a.ntx;
+    const int tyi = rest % a.nty;
+    const int tz = rest / a.nty;
+    // the voxel columns of bp_tile with nsw waves
+    const int px = nsw >= 2 ? 2 : 1;
+    const int wx = threadIdx.y % px, wy = threadIdx.y / px;
+    const int ex = px * 8, ey = (nsw / px) * 8;
+    const int i = tx * ex + wx * 8 + (threadIdx.x & 7);
+    const int j = tyi * ey + wy * 8 + (threadIdx.x >> 3);
+    const int k0 = tz * ZC;
+    const float fi = (float)min(i, a.X - 1);
+    const float fj = (float)min(j, a.Y - 1);
+    const float fk0 = (float)k0;
+    const float cx0 = (float)min(tx * ex, a.X - 1), cx1 = (float)min(tx * ex + ex - 1, a.X - 1);
+    const float cy0 = (float)min(tyi * ey, a.Y - 1), cy1 = (float)min(tyi * ey + ey - 1, a.Y - 1);
+    const float Wm1 = (float)(a.W - 1), Vm1 = (float)(a.V - 1);
+    const uint32_t rowb = (uint32_t)a.row_px * 8u;  // strip pixels are 8 bytes
+    const int tid = threadIdx.y * 64 + threadIdx.x;
+    const int rounds = a.lds_px;      // copy rounds per view (64 chunks each)
+    const int cap = rounds * 64;      // 16-byte chunks per buffer
+    const int n = a.nviews;
+    unsigned char* const lbuf = reinterpret_cast<unsigned char*>(lds_dyn);
+
+    // copy wave: footprint of view vv into box slot and buffer slot (both vv % kStripBufs)
+    auto stage = [&](int vv) {
+        const int slot = vv % kStripBufs;
+        const StripBox bx = strip_footprint<ZC>(geo_of(a.geo, a.view0 + vv), cx0, cx1, cy0, cy1, fk0, a.W, a.V);
+        if (threadIdx.x == 0) {
+            box[slot][0] = bx.u0;
+            box[slot][1] = bx.v0;
+            box[slot][2] = bx.cpr;
+            box[slot][3] = bx.nch;
+        }
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<char*>(reinterpret_cast<const char*>(a.q)) + (size_t)vv * a.frame_bytes, (short)0,
+            (int)a.frame_bytes, kBufferFlags);
+        const float rinv = 1.0f / (float)bx.cpr;
+        const uint32_t gbase = (uint32_t)bx.v0 * rowb + (uint32_t)bx.u0 * 8u;
+        const int last = min(bx.nch, cap) - 1;  // the host sizes cap for the largest footprint
+        unsigned char* const dst = lbuf + (size_t)slot * cap * 16;
+        for (int r = 0; r < rounds; ++r) {
+            const int c = min(r * 64 + (int)threadIdx.x, last);
+            const int row = (int)(((float)c + 0.5f) * rinv);  // exact: (c + 0.5) / cpr is 0.5 / cpr from an integer
+            const uint32_t off = gbase + (uint32_t)row * rowb + (uint32_t)(c - row * bx.cpr) * 16u;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (__attribute__((address_space(3))) void*)(dst + r * 64 * 16),
+                                                     16, off, 0, 0, 0);
+        }
+    };
+
+    float acc[ZC];
+#pragma unroll
+    for (int kk = 0; kk < ZC; ++kk) acc[kk] = 0.0f;
+    // the partial-column count corrections, one slot per thread and sample
+    // (touched by that thread only), as in bp_tile
+    if (!copier) {
+#pragma unroll
+        for (int kk = 0; kk < ZC; ++kk) cnt_part[kk * 256 + tid] = 0.0f;
+    }
+
+    if (copier && n > 0) {
+        for (int w = 0; w < kStripAhead && w < n; ++w) stage(w);
+        wait_vmcnt((min(kStripAhead, n) - 1) * rounds);  // view 0 has landed
+    }
+    for (int v = 0; v < n; ++v) {
+        // view v has landed (the copy wave waited for it), its box is
+        // visible, and every sampling wave is done with view v - 1
+        __syncthreads();
+        if (copier) {
+            if (v + kStripAhead < n) stage(v + kStripAhead);
+            // views in flight after view v + 1: v + 2 .. min(v + kStripAhead, n - 1)
+            const int after = min(v + kStripAhead, n - 1) - (v + 1);
+            if (v + 1 < n) wait_vmcnt(max(after, 0) * rounds);
+            continue;
+        }
+        const int slot = v % kStripBufs;
+        const Geo g = geo_of(a.geo, a.view0 + v);
+        const float Uc = fmaf(g.c[3], fk0, fmaf(g.c[1], fi, fmaf(g.c[2], fj, g.c[0])));
+        const float Vc = fmaf(g.c[7], fk0, fmaf(g.c[5], fi, fmaf(g.c[6], fj, g.c[4])));
+        const float Wc = fmaf(g.c[11], fk0, fmaf(g.c[9], fi, fmaf(g.c[10], fj, g.c[8])));
+        const float rw = __builtin_amdgcn_rcpf(Wc);
+        const float su = fmaf(Uc, rw, 0.5f);
+        if (!(su >= 0.0f && su <= Wm1)) continue;
+        const int iu = (int)su;
+        float fe;
+        const typename S::wt_t wq = S::weight(__builtin_amdgcn_fractf(su), fe);
+        const float sv0 = fmaf(Vc, rw, 0.5f);
+        const float dsv = g.c[7] * rw;
+        const float svl = fmaf((float)(ZC - 1), dsv, sv0);
+        const bool all_in = fminf(sv0, svl) >= 0.0f && fmaxf(sv0, svl) <= Vm1;
+        // (a footprint over the buffer, which the host rules out, samples nothing)
+        if (box[slot][3] > cap) continue;
+        const unsigned char* lbase = lbuf + (size_t)slot * cap * 16;
+        const uint32_t lrow = (uint32_t)box[slot][2] * 16u;
+        const uint32_t lb = (uint32_t)(iu - box[slot][0]) * 8u - (uint32_t)box[slot][1] * lrow;
+        // the window at byte offset o = row * lrow + lb of the footprint
+        const auto fetch = [&](uint32_t o) -> u32x4_t {
+            const uint2* q = reinterpret_cast<const uint2*>(lbase + o);
+            const uint2 x = q[0], y = q[1];
+            return (u32x4_t){x.x, x.y, y.x, y.y};
+        };
+        if (all_in) {
+            if (dsv >= 0.0f && dsv <= 0.984375f) {
+                bp_column_pairs<ZC, QT, 0, false>(acc, fetch, lrow, lb, sv0, dsv, wq, 0.0f);
+            } else if (dsv >= 1.015625f && dsv <= 1.984375f) {
+                bp_column_pairs<ZC, QT, 1, false>(acc, fetch, lrow, lb, sv0, dsv, wq, 0.0f);
+            } else {
+                uint32_t off[ZC];
+                float fv[ZC];
+#pragma unroll
+                for (int k = 0; k < ZC; ++k) {
+                    const float sv = fmaf((float)k, dsv, sv0);
+                    fv[k] = __builtin_amdgcn_fractf(sv);
+                    off[k] = __umul24((uint32_t)(int)sv, lrow) + lb;
+                }
+                typename S::raw_t p[ZC];
+#pragma unroll
+                for (int k = 0; k < ZC; ++k) p[k] = fetch(off[k]);
+#pragma unroll
+                for (int k = 0; k < ZC; ++k) acc[k] = S::accum(acc[k], p[k], fv[k], wq);
+            }
+            // every sample counts: the column's count weight is in a.colcnt
+        } else {
+            // the count weight of the samples outside the detector rows is
+            // taken off a.colcnt at the end
+            const float2 uwq = a.uwp[(size_t)(a.view0 + v) * a.W + iu];
+            const float cw = fmaf(fe, uwq.y - uwq.x, uwq.x);
+#pragma unroll
+            for (int kb = 0; kb < ZC; kb += kZB) {
+                uint32_t off[kZB];
+                float fv[kZB];
+                bool in[kZB];
+#pragma unroll
+                for (int q = 0; q < kZB; ++q) {
+                    const float sv = fmaf((float)(kb + q), dsv, sv0);
+                    const float t = __builtin_amdgcn_fmed3f(sv, 0.0f, Vm1);
+                    in[q] = (t == sv);
+                    fv[q] = __builtin_amdgcn_fractf(t);
+                    off[q] = __umul24((uint32_t)(int)t, lrow) + lb;
+                }
+                typename S::raw_t p[kZB];
+#pragma unroll
+                for (int q = 0; q < kZB; ++q) p[q] = fetch(off[q]);
+#pragma unroll
+                for (int q = 0; q < kZB; ++q) {
+                    const float sum = S::accum(acc[kb + q], p[q], fv[q], wq);
+                    acc[kb + q] = in[q] ? sum : acc[kb + q];
+                    cnt_part[(kb + q) * 256 + tid] += in[q] ? 0.0f : cw;
+                }
+            }
+        }
+    }
+
+    if (copier || i >= a.X || j >= a.Y) return;
+    const float c = a.colcnt[(size_t)j * a.X + i];
+    const size_t plane = (size_t)a.X * a.Y;
+    size_t idx = ((size_t)k0 * a.Y + j) * a.X + i;
+#pragma unroll
+    for (int kk = 0; kk < ZC; ++kk) {
+        if (k0 + kk < a.Z) {
+            const float cnt = c - cnt_part[kk * 256 + tid];
+            if constexpr (OUT == kOutNormalised) {
+                const float v = acc[kk] / cnt * a.qscale;
+                a.vol[idx] = a.ct ? ct_number(v, a.ct_scale, a.ct_norm1) : v;
+            } else {
+                a.acc[idx] += acc[kk];
+                a.cnt[idx] += cnt;
+            }
+        }
+        idx += plane;
+    }
+}
+
+// The LDS-staged strip kernel: bp_kernel's tile schedule with one more wave
+// per workgroup (the copy wave).
+template <int ZC, int QT, int OUT>
+__global__ void __launch_bounds__(320) bp_strip_lds_kernel(BPArgs a)
+{
+    const unsigned ntiles = (unsigned)a.ntx * a.nty * a.ntz;
+    const unsigned t8 = gridDim.x >> 3;  // gridDim.x = 8 * ceil(ntiles / 8)
+    const unsigned tile = (blockIdx.x & 7u) * t8 + (blockIdx.x >> 3);
+    if (tile >= ntiles) return;  // uniform per workgroup
+    bp_tile_strip_lds<ZC, QT, OUT>(a, tile);
+}
+
 // Tile schedule.  Workgroups are dispatched round-robin over the 8 XCDs, so
 // workgroup b runs on XCD b % 8.  XCD x takes the contiguous tile range
 // [x T8, (x+1) T8): a z-slab, whose detector rows then stay in that XCD's L2.
@@ -1256,6 +1581,69 @@ static bp_kernel_t pick_zc(int zc, int dtype, int strip)
     }
 }
 
+// LDS-staged column strips (count-normalised forms only; nullptr otherwise).
+template <int OUT>
+static bp_kernel_t pick_strip_lds(int zc, int dtype)
+{
+    if constexpr (OUT == kOutNormalised || OUT == kOutAccumulate) {
+        const bool bf = dtype == JH_BF16;
+        switch (zc) {
+        case 8: return bf ? bp_strip_lds_kernel<8, kBF16Strip, OUT> : bp_strip_lds_kernel<8, kF16Strip, OUT>;
+        case 16: return bf ? bp_strip_lds_kernel<16, kBF16Strip, OUT> : bp_strip_lds_kernel<16, kF16Strip, OUT>;
+        default: return bf ? bp_strip_lds_kernel<32, kBF16Strip, OUT> : bp_strip_lds_kernel<32, kF16Strip, OUT>;
+        }
+    }
+    return nullptr;
+}
+
+// The largest strip footprint (16-byte chunks) of the plan's tile shape over
+// all its views, measured once per shape on the device with the kernel's own
+// arithmetic; -1 where it cannot be measured now (the stream is capturing a
+// graph: the launch then gathers).
+static int strip_footprint_chunks(jh_bp_plan* p, int ntx, int nty, int ntz, int ex, int ey, hipStream_t stream)
+{
+    const int zi = p->zc >= 32 ? 2 : p->zc >= 16 ? 1 : 0, ti = p->ty >= 4 ? 2 : p->ty >= 2 ? 1 : 0;
+    int& c = p->strip_chunks[zi][ti];
+    if (c >= 0) return c;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    JH_HIP(hipStreamIsCapturing(stream, &cs));
+    if (cs != hipStreamCaptureStatusNone) return -1;
+    const size_t lanes = (size_t)ntx * nty * ntz * p->N * 4;
+    require(lanes / 256 < (1ull << 31), "backprojection footprint grid too large");
+    DevBuf d(sizeof(int));
+    const int zero = 0;
+    JH_HIP(hipMemcpyAsync(d.get(), &zero, sizeof(int), hipMemcpyHostToDevice, stream));
+    const unsigned grid = (unsigned)((lanes + 255) / 256);
+    const Geo* geo = p->geo.as<Geo>();
+    if (p->zc >= 32)
+        strip_footprint_max_kernel<32><<<grid, 256, 0, stream>>>(geo, 0, p->N, ntx, nty, ntz, ex, ey, p->X, p->Y, p->W,
+                                                                 p->V, d.as<int>());
+    else if (p->zc >= 16)
+        strip_footprint_max_kernel<16><<<grid, 256, 0, stream>>>(geo, 0, p->N, ntx, nty, ntz, ex, ey, p->X, p->Y, p->W,
+                                                                 p->V, d.as<int>());
+    else
+        strip_footprint_max_kernel<8><<<grid, 256, 0, stream>>>(geo, 0, p->N, ntx, nty, ntz, ex, ey, p->X, p->Y, p->W,
+                                                                p->V, d.as<int>());
+    check_launch("strip_footprint_max_kernel");
+    int h = 0;
+    JH_HIP(hipMemcpyAsync(&h, d.get(), sizeof(int), hipMemcpyDeviceToHost, stream));
+    JH_HIP(hipStreamSynchronize(stream));
+    c = h;
+    return c;
+}
+
+// Copy rounds (64 chunks of 16 bytes) per view of the strip kernel for a
+// largest footprint of foot chunks, or 0 where its buffers (kStripBufs of
+// them, beside its static footprint boxes and the partial-column count
+// slots, ZC per thread of the 256 sampling threads) would exceed 64 KiB or
+// its waits the 63 outstanding operations of the counter.
+static int strip_lds_rounds(int zc, int foot)
+{
+    const int rounds = (foot + 63) / 64;
+    const size_t bytes = (size_t)kStripBufs * rounds * 64 * 16 + 64 + (size_t)zc * 256 * 4;
+    return (rounds >= 1 && bytes <= 65536 && (kStripAhead - 1) * rounds <= 63) ? rounds : 0;
+}
+
 template <int OUT>
 static bp_kernel_t pick_kernel(int mode, int zc, int dtype, int fetch, int strip)
 {
@@ -1263,7 +1651,11 @@ static bp_kernel_t pick_kernel(int mode, int zc, int dtype, int fetch, int strip
     case 0:
         // the distance-weighted forms have no LDS-staged fetch: they gather
         if constexpr (!distance_weighted(OUT))
-            if (fetch == JH_FETCH_LDS) return pick_zc<kModeStaged, OUT>(zc, dtype, strip);
+            if (fetch == JH_FETCH_LDS) {
+                if (strip && dtype != JH_F32)
+                    if (bp_kernel_t k = pick_strip_lds<OUT>(zc, dtype)) return k;
+                return pick_zc<kModeStaged, OUT>(zc, dtype, strip);
+            }
         return pick_zc<0, OUT>(zc, dtype, strip);
     case 1: return pick_zc<1, OUT>(zc, dtype, strip);
     default: return pick_zc<2, OUT>(zc, dtype, strip);
@@ -1318,18 +1710,33 @@ static void launch_bp(jh_bp_plan* p, const void* q, int dtype, int view0, int nv
     require(ntiles < (1ull << 31), "backprojection grid too large");
     const size_t grid = (ntiles + 7) / 8 * 8;
     const int st = stage_strip(p, dtype) ? 1 : 0;
-    bp_kernel_t k = out == kOutNormalised      ? pick_kernel<kOutNormalised>(p->mode, p->zc, dtype, p->fetch, st)
-                    : out == kOutAccumulate    ? pick_kernel<kOutAccumulate>(p->mode, p->zc, dtype, p->fetch, st)
-                    : out == kOutFdk           ? pick_kernel<kOutFdk>(p->mode, p->zc, dtype, p->fetch, st)
-                    : out == kOutFdkAccumulate ? pick_kernel<kOutFdkAccumulate>(p->mode, p->zc, dtype, p->fetch, st)
-                                               : pick_kernel<kOutRaw>(p->mode, p->zc, dtype, p->fetch, st);
-    // LDS-staged fetch: two buffers sized for the block footprints of this
-    // tile depth (a larger footprint gathers), 64 zc pixels each, capped so
-    // that both, with the kernel's static box ring and count-weight rows
-    // (4 KiB + 48 B), stay within 64 KiB
-    const bool staged = p->mode == 0 && p->fetch == JH_FETCH_LDS && !distance_weighted(out);
-    a.lds_px = staged ? std::min(64 * p->zc, (65536 - 4096 - 64) / 32) : 0;
-    const size_t lds_bytes = staged ? 2 * (size_t)a.lds_px * sizeof(float4) : 0;
+    // LDS-staged fetch.  Column strips take the strip kernel, whose two
+    // buffers hold the largest footprint of the tile shape (rounded up to the
+    // copy's 256-chunk rounds); where that exceeds the LDS the launch gathers.
+    // Other layouts take the coefficient kernel: two buffers sized for the
+    // block footprints of this tile depth (a larger footprint gathers), 64 zc
+    // pixels each, capped so that both, with the kernel's static box ring and
+    // count-weight rows (4 KiB + 48 B), stay within 64 KiB.
+    int fetch = p->fetch;
+    bool strip_lds = p->mode == 0 && fetch == JH_FETCH_LDS && st && (out == kOutNormalised || out == kOutAccumulate);
+    int strip_rounds = 0;
+    if (strip_lds) {
+        const int foot = strip_footprint_chunks(p, a.ntx, a.nty, a.ntz, ex, ey, stream);
+        strip_rounds = foot < 0 ? 0 : strip_lds_rounds(p->zc, foot);
+        if (strip_rounds == 0) {
+            strip_lds = false;
+            fetch = JH_FETCH_GATHER;
+        }
+    }
+    const bool staged = p->mode == 0 && fetch == JH_FETCH_LDS && !distance_weighted(out) && !strip_lds;
+    bp_kernel_t k = out == kOutNormalised      ? pick_kernel<kOutNormalised>(p->mode, p->zc, dtype, fetch, st)
+                    : out == kOutAccumulate    ? pick_kernel<kOutAccumulate>(p->mode, p->zc, dtype, fetch, st)
+                    : out == kOutFdk           ? pick_kernel<kOutFdk>(p->mode, p->zc, dtype, fetch, st)
+                    : out == kOutFdkAccumulate ? pick_kernel<kOutFdkAccumulate>(p->mode, p->zc, dtype, fetch, st)
+                                               : pick_kernel<kOutRaw>(p->mode, p->zc, dtype, fetch, st);
+    a.lds_px = strip_lds ? strip_rounds : staged ? std::min(64 * p->zc, (65536 - 4096 - 64) / 32) : 0;
+    const size_t lds_bytes = strip_lds ? (size_t)kStripBufs * strip_rounds * 64 * 16
+                             : staged  ? 2 * (size_t)a.lds_px * sizeof(float4) : 0;
     if (p->mode == 0 && !staged && (out == kOutNormalised || out == kOutAccumulate)) {
         // the count weights of the launch's views per voxel column
         const dim3 cg((unsigned)((p->X + 63) / 64), (unsigned)((p->Y + 3) / 4));
@@ -1342,7 +1749,8 @@ static void launch_bp(jh_bp_plan* p, const void* q, int dtype, int view0, int nv
             bp_colcount_kernel<JH_F16><<<cg, cb, 0, stream>>>(a.geo, a.uwp, view0, nviews, p->X, p->Y, p->W, p->colcnt.as<float>());
         check_launch("bp_colcount_kernel");
     }
-    hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(64, p->ty), lds_bytes, stream, a);
+    // (the strip kernel's workgroups carry one more wave, its copy wave)
+    hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(64, p->ty + (strip_lds ? 1 : 0)), lds_bytes, stream, a);
     check_launch("bp_kernel");
 }
 
@@ -1385,6 +1793,8 @@ jh_status jh_bp_create(jh_bp_plan** plan, int device, const double* h_ptm, int n
             p->f16exp = 0;
             p->acc_scale = 1.0f;
             p->acc_dtype = -1;
+            for (auto& r : p->strip_chunks)
+                for (int& c : r) c = -1;
             p->ptm.alloc(sizeof(double) * 12 * nviews);
             p->geo.alloc(sizeof(Geo) * nviews);
             p->uw.alloc(sizeof(float) * (size_t)nviews * W);

@@ -30,6 +30,8 @@ struct jh_bp_plan {
     int f16exp;      // FP16 storage holds q * 2^-f16exp; outputs are scaled back by 2^f16exp
     float acc_scale; // output scale of what the accumulators hold (2^f16exp of FP16 chunks, else 1)
     int acc_dtype;   // storage type of the chunks accumulated since the last reset (-1: none)
+    int strip_chunks[3][3];  // LDS-staged strips: largest footprint (16-byte chunks) over all views per
+                             // tile shape (zc 8 / 16 / 32 x ty 1 / 2 / 4); -1 until measured
     jh::DevBuf ptm;  // N x 12 double
     jh::DevBuf geo;  // N x Geo
     jh::DevBuf uw;   // N x W float

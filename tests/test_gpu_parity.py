@@ -385,7 +385,9 @@ def test_bp_autotune_keeps_results(gpu):
 @pytest.mark.parametrize("tile", [(32, 4), (16, 2), (8, 1)])
 def test_bp_lds_fetch_matches_gather(gpu, dtype, tile):
     """The LDS-staged fetch samples the same positions from the same values:
-    only the association of the bilinear terms differs."""
+    FP32 quads are staged as bilinear coefficients (only the association of
+    the bilinear terms differs); BF16 / FP16 column strips are staged as they
+    are and sampled with the gather form's operations (equal to the bit)."""
     w = SMALL
     ptm = w.ptms()
     filt = torch.from_numpy(synth.normal(21, (w.n_views, w.n_v, w.n_u))).to(gpu)
@@ -410,6 +412,8 @@ def test_bp_lds_fetch_matches_gather(gpu, dtype, tile):
         e = rel_rmse(b, a)
         print(dtype, tile, ["one-shot", "streamed"][k], "LDS vs gather rel-RMSE", e)
         assert e <= 1e-6
+        if dtype != "f32":
+            np.testing.assert_array_equal(b, a)
 
 
 def test_bp_lds_fetch_falls_back_beyond_budget(gpu):
@@ -624,7 +628,7 @@ def test_bp_strip_layout_equals_quad_layout_bit_exact(gpu, dtype, tile, geom):
     sample and interpolate them with the same operations as the quad layout's
     one gather per sample: the volumes agree to the bit (one-shot and
     streamed), including the columns that leave the detector rows (partial
-    path) and the LDS-staged fetch.  Row steps 0.83 (second sample zero or
+    path), and so does the LDS-staged strip fetch.  Row steps 0.83 (second sample zero or
     one row down) and 1.29 (one or two rows down)."""
     from jenhine_amd.recon import BPCore
     w = SMALL if geom.startswith("cbct") else DENTAL_STEP
@@ -648,13 +652,16 @@ def test_bp_strip_layout_equals_quad_layout_bit_exact(gpu, dtype, tile, geom):
         out[layout] = (vol, bp.finalize().cpu().numpy())
         if layout == "strip":
             bp.set_fetch("lds")
-            out["lds"] = (bp.run(q, dtype).cpu().numpy(),)
+            vol = bp.run(q, dtype).cpu().numpy()
+            bp.reset()
+            bp.accumulate(q[:h], dtype, 0, w.n_views // 2)
+            bp.accumulate(q[h:], dtype, w.n_views // 2, w.n_views - w.n_views // 2)
+            out["lds"] = (vol, bp.finalize().cpu().numpy())
             bp.set_fetch("gather")
     for k in range(2):
         np.testing.assert_array_equal(out["strip"][k], out["quad"][k])
-    # LDS staging reads the same neighbourhoods from strips (its own bilinear
-    # association, as against quads: test_bp_lds_fetch_matches_gather)
-    assert rel_rmse(out["lds"][0], out["quad"][0]) <= 1e-6
+        # the LDS-staged strips: the same windows, the same operations
+        np.testing.assert_array_equal(out["lds"][k], out["quad"][k])
 
 
 def test_bp_layouts_agree_on_a_fine_detector(gpu):
