@@ -96,12 +96,19 @@ def main():
                     help="no GPU work: read the rocprofv3 --pmc CSV files under DIR (kernel-only runs of one "
                          "backprojection tile) and print the counter summary the roofline reads")
     ap.add_argument("--tile", default=None, help="zc,ty of the profiled kernel (with --summarize-counters)")
+    ap.add_argument("--fetch", choices=("gather", "lds"), default="gather",
+                    help="fetch form of the profiled kernel (with --summarize-counters)")
+    ap.add_argument("--form", choices=("fp32", "bf16", "f16"), default="bf16",
+                    help="storage form of the profiled kernel (with --summarize-counters)")
+    ap.add_argument("--counters-out", default=None,
+                    help="with --summarize-counters: merge the summary into this JSON file (e.g. "
+                         "profiles/bp_dental512_counters.json)")
     ap.add_argument("--batch-storage", choices=("bf16", "f16", "f32"), default="f16",
                     help="projection storage of the batch16 scans (FP16 with its range exponent: BF16 storage "
                          "misses 1e-3 on noise-dominated Ram-Lak scans)")
     args = ap.parse_args()
     if args.summarize_counters:
-        return summarize_counters(args.summarize_counters, args.tile)
+        return summarize_counters(args.summarize_counters, args.tile, args.fetch, args.form, args.counters_out)
     if args.gpus != 1:
         print(json.dumps({"error": "the benchmark measures one GPU (the path shards by z-slabs, one process "
                                    "per GPU, not built here); --gpus must be 1",
@@ -306,102 +313,146 @@ HBM_PEAK = 8.0e12
 CLOCK = 2.4e9
 CUS = 256
 VALU_WAVE_INSTR_PER_S = 1024 * CLOCK / 2.0
-TA_BYTES_PER_CLOCK = 64.0   # per CU: four lanes of a 16-byte gather per clock
+TA_BYTES_PER_CLOCK = 64.0    # per CU: four lanes of a 16-byte gather per clock
+LDS_BYTES_PER_CLOCK = 128.0  # per CU: LDS read bandwidth
+COUNTERS_FILE = os.path.join("profiles", "bp_dental512_counters.json")
 
 
-def bp_roofline(gups, units, w, stage_bytes, tuned):
+def counter_key(form, fetch, zc, ty):
+    return f"{form},{fetch},{zc},{ty}"
+
+
+def bp_roofline(gups, units, w, stage_bytes, tuned, form="bf16"):
     """The backprojector's roofline from its schedule and the committed
-    counters of the same kernel (profiles/bp_dental512_bf16_counters.json)."""
+    counters of the same kernel (profiles/bp_dental512_counters.json, keyed by
+    storage form, fetch form and tile)."""
     nvox = w.X * w.Y * w.Z
+    fetch = tuned["fetch"]
     # compulsory bytes of the schedule: one launch writes the volume once and
     # reads every view's staged frame once (register accumulation over all
     # views of the launch: no accumulator traffic)
     b_sched = 4.0 * nvox + stage_bytes
     gups_hbm = HBM_PEAK / (b_sched / units) / 1e9
-    # bytes requested through the texture path per update: one 16-byte gather
-    # per pair of z samples (column strips) -> 8 B
+    # bytes each update reads as a sample window: one 16-byte window per pair
+    # of z samples (column strips) -> 8 B, through the texture path (gather
+    # form) or from LDS (LDS-staged form, whose copies into LDS pass the
+    # texture path once per footprint pixel and workgroup)
     req_per_update = 8.0
-    gups_ta = CUS * CLOCK * TA_BYTES_PER_CLOCK / req_per_update / 1e9
-    roof = {"kernel": f"bp_kernel (BF16 column strips, zc {tuned['zc']}, {tuned['ty']} waves per workgroup)",
+    rate = TA_BYTES_PER_CLOCK if fetch == "gather" else LDS_BYTES_PER_CLOCK
+    gups_win = CUS * CLOCK * rate / req_per_update / 1e9
+    win = "vmem_issue" if fetch == "gather" else "lds_read"
+    roof = {"kernel": f"{'bp_kernel' if fetch == 'gather' else 'bp_strip_lds_kernel'} ({form.upper()} column strips, "
+                      f"{fetch} fetch, zc {tuned['zc']}, {tuned['ty']} sampling waves per workgroup)",
             "achieved": gups, "unit": "GUPS",
             "compulsory_bytes": b_sched, "gups_hbm_sched": gups_hbm,
-            "ta_bytes_per_update": req_per_update, "gups_vmem_issue": gups_ta,
+            "window_bytes_per_update": req_per_update, f"gups_{win}": gups_win,
             "north_star_hbm_8B": {"bytes_per_update": 8, "achieved_GBps": 8.0 * gups,
                                   "peak_GBps": HBM_PEAK / 1e9, "frac": 8.0 * gups * 1e9 / HBM_PEAK,
                                   "note": "the reference's per-view formulation (FP32 accumulator read + write "
                                           "per update); this kernel accumulates in registers and never moves "
                                           "those bytes, so this is not a roof it runs against"}}
-    cf = os.path.join(ROOT, "profiles", "bp_dental512_bf16_counters.json")
+    cf = os.path.join(ROOT, COUNTERS_FILE)
     c = None
     if os.path.exists(cf):
         with open(cf) as fh:
             c = json.load(fh)
-    c = (c or {}).get("tiles", {}).get(f"{tuned['zc']},{tuned['ty']}")
-    if c is not None:
+    key = counter_key(form, fetch, tuned["zc"], tuned["ty"])
+    c = (c or {}).get("kernels", {}).get(key)
+    gups_valu = None
+    if c is not None and c.get("valu_insts"):
         valu_per_update = c["valu_insts"] * 64.0 / units
         gups_valu = VALU_WAVE_INSTR_PER_S * 64.0 / valu_per_update / 1e9
         roof.update(valu_lane_instr_per_update=valu_per_update, gups_valu=gups_valu,
                     traffic=c["hbm_bytes"], traffic_over_compulsory=c["hbm_bytes"] / b_sched,
                     l2_hit=c["l2_hit"], valu_busy=c["valu_busy"], ta_busy=c["ta_busy"],
-                    counters="profiles/bp_dental512_bf16_counters.json")
+                    counters=f"{COUNTERS_FILE} [{key}]")
     else:
-        gups_valu = None
-        roof.update(traffic=None, counters="absent for this tile: VALU roof not evaluated")
-    roofs = {"hbm": gups_hbm, "vmem_issue": gups_ta}
+        roof.update(traffic=None, counters=f"no counters for {key}: VALU roof not evaluated")
+    roofs = {"hbm": gups_hbm, win: gups_win}
     if gups_valu:
         roofs["valu"] = gups_valu
     bound = min(roofs, key=roofs.get)
-    roof.update(bound=bound, peak=roofs[bound], frac=gups / roofs[bound])
+    roof.update(bound=bound, peak=roofs[bound], frac=gups / roofs[bound], roofs=roofs)
     return roof
 
 
-def summarize_counters(folder, tile):
+def summarize_counters(folder, tile, fetch, form, out_file):
     """Per-launch counters of the backprojection kernel from rocprofv3 --pmc
     CSV output (one counter set per run), with the HBM byte counters scaled by
-    what they read for the 1 GiB device copy that --kernel-only runs first."""
+    what they read for the 1 GiB device copy that --kernel-only runs first;
+    merged into out_file under "form,fetch,zc,ty"."""
     import csv
     import glob
-    vals = {}   # (kernel, counter) -> list of per-dispatch values
+    vals = {}   # (kernel, counter) -> {dispatch: value}
+    span = {}   # kernel -> {dispatch: ns}
     for path in glob.glob(os.path.join(folder, "**", "*counter_collection.csv"), recursive=True):
         with open(path) as fh:
             for row in csv.DictReader(fh):
+                d = (path, row.get("Dispatch_Id"))
                 k = (row["Kernel_Name"], row["Counter_Name"])
-                vals.setdefault(k, {}).setdefault(row.get("Dispatch_Id", len(vals)), 0.0)
-                vals[k][row.get("Dispatch_Id")] += float(row["Counter_Value"])
+                vals.setdefault(k, {}).setdefault(d, 0.0)
+                vals[k][d] += float(row["Counter_Value"])
+                span.setdefault(row["Kernel_Name"], {})[d] = int(row["End_Timestamp"]) - int(row["Start_Timestamp"])
+
     def per_dispatch(pred, counter):
-        xs = [v for (kn, cn), d in vals.items() if pred(kn) and cn == counter for v in d.values()]
+        xs = [v for (kn, cn), dd in vals.items() if pred(kn) and cn == counter for v in dd.values()]
         return sum(xs) / len(xs) if xs else None
-    is_bp = lambda kn: "bp_kernel" in kn
+    is_bp = lambda kn: "bp_kernel" in kn or "bp_strip_lds_kernel" in kn
     is_copy = lambda kn: "copy" in kn.lower() and "bp_" not in kn
     gib = float(1 << 30)
-    fetch_copy, write_copy = per_dispatch(is_copy, "FETCH_SIZE"), per_dispatch(is_copy, "WRITE_SIZE")
+    def largest(pred, counter):  # the 1 GiB copy, not the small uploads of the same copy kernel
+        xs = [v for (kn, cn), dd in vals.items() if pred(kn) and cn == counter for v in dd.values()]
+        return max(xs) if xs else None
+    fetch_copy, write_copy = largest(is_copy, "FETCH_SIZE"), largest(is_copy, "WRITE_SIZE")
     fetch_scale = gib / (fetch_copy * 1024.0) if fetch_copy else None
     write_scale = gib / (write_copy * 1024.0) if write_copy else None
-    fetch, write = per_dispatch(is_bp, "FETCH_SIZE"), per_dispatch(is_bp, "WRITE_SIZE")
+    fetch_kib, write_kib = per_dispatch(is_bp, "FETCH_SIZE"), per_dispatch(is_bp, "WRITE_SIZE")
     hit, miss = per_dispatch(is_bp, "TCC_HIT_sum"), per_dispatch(is_bp, "TCC_MISS_sum")
     gui = per_dispatch(is_bp, "GRBM_GUI_ACTIVE")
-    zc, ty = (int(x) for x in tile.split(",")) if tile else (None, None)
-    out = {"kernel": "bp_kernel (BF16 column strips, dental512)", "zc": zc, "ty": ty,
-           "fetch_kib": fetch, "write_kib": write, "fetch_scale": fetch_scale, "write_scale": write_scale,
-           "hbm_bytes": (fetch * 1024.0 * fetch_scale + write * 1024.0 * write_scale)
-           if None not in (fetch, write, fetch_scale, write_scale) else None,
-           "valu_insts": per_dispatch(is_bp, "SQ_INSTS_VALU"), "vmem_rd_insts": per_dispatch(is_bp, "SQ_INSTS_VMEM_RD"),
-           "lds_insts": per_dispatch(is_bp, "SQ_INSTS_LDS"), "waves": per_dispatch(is_bp, "SQ_WAVES"),
+    names = sorted({kn for kn in span if is_bp(kn)})
+    ns = [t for kn in names for t in span[kn].values()]
+    zc, ty = (int(x) for x in tile.split(","))
+    out = {"kernel": names, "form": form, "fetch": fetch, "zc": zc, "ty": ty,
+           "ms": sum(ns) / len(ns) / 1e6 if ns else None,
+           "fetch_kib": fetch_kib, "write_kib": write_kib, "fetch_scale": fetch_scale, "write_scale": write_scale,
+           "hbm_bytes": (fetch_kib * 1024.0 * fetch_scale + write_kib * 1024.0 * write_scale)
+           if None not in (fetch_kib, write_kib, fetch_scale, write_scale) else None,
            "l2_hit": hit / (hit + miss) if hit is not None and miss else None,
            "gui_active_cycles_per_xcd": gui / 8.0 if gui else None}
-    busy = per_dispatch(is_bp, "SQ_BUSY_CYCLES")
-    valu_act = per_dispatch(is_bp, "SQ_ACTIVE_INST_VALU")
-    ta = per_dispatch(is_bp, "TA_TA_BUSY_sum")
-    if out["gui_active_cycles_per_xcd"]:
-        cyc = out["gui_active_cycles_per_xcd"]
-        # two clocks per wave-level VALU instruction on each of 1024 SIMDs
+    for name, counter in (("valu_insts", "SQ_INSTS_VALU"), ("vmem_rd_insts", "SQ_INSTS_VMEM_RD"),
+                          ("lds_insts", "SQ_INSTS_LDS"), ("waves", "SQ_WAVES"), ("sq_busy_cycles", "SQ_BUSY_CYCLES"),
+                          ("sq_wave_cycles", "SQ_WAVE_CYCLES"), ("sq_active_inst_valu", "SQ_ACTIVE_INST_VALU"),
+                          ("sq_wait_inst_any", "SQ_WAIT_INST_ANY"), ("sq_active_inst_lds", "SQ_ACTIVE_INST_LDS"),
+                          ("sq_wait_inst_lds", "SQ_WAIT_INST_LDS"), ("lds_bank_conflict", "SQ_LDS_BANK_CONFLICT"),
+                          ("ta_busy_sum", "TA_TA_BUSY_sum"), ("ta_buffer_read_wavefronts", "TA_BUFFER_READ_WAVEFRONTS_sum"),
+                          ("td_busy_sum", "TD_TD_BUSY_sum"), ("tcp_accesses", "TCP_TOTAL_CACHE_ACCESSES_sum"),
+                          ("tcp_tcc_read_req", "TCP_TCC_READ_REQ_sum")):
+        out[name] = per_dispatch(is_bp, counter)
+    cyc = out["gui_active_cycles_per_xcd"]
+    if cyc:
+        out["clock_ghz"] = cyc / (out["ms"] * 1e6) if out["ms"] else None
+        # two clocks per wave-level VALU instruction on each of the 1024 SIMDs
         out["valu_busy"] = out["valu_insts"] * 2.0 / (1024 * cyc) if out["valu_insts"] else None
-        out["ta_busy"] = ta / (CUS * cyc) if ta else None
+        out["ta_busy"] = out["ta_busy_sum"] / (CUS * cyc) if out["ta_busy_sum"] else None
+        out["lds_bank_conflict_share"] = (out["lds_bank_conflict"] / (CUS * cyc)
+                                          if out["lds_bank_conflict"] is not None else None)
+        out["vmem_cycles_per_insn_per_cu"] = (CUS * cyc / out["vmem_rd_insts"] if out["vmem_rd_insts"] else None)
     else:
-        out["valu_busy"] = out["ta_busy"] = None
-    out["sq_busy_cycles"], out["sq_active_inst_valu"] = busy, valu_act
-    out["tcp_accesses"] = per_dispatch(is_bp, "TCP_TOTAL_CACHE_ACCESSES_sum")
+        out["clock_ghz"] = out["valu_busy"] = out["ta_busy"] = None
+    if out["vmem_rd_insts"] and out["tcp_accesses"]:
+        out["tcp_accesses_per_vmem"] = out["tcp_accesses"] / out["vmem_rd_insts"]
     print(json.dumps(out, indent=1))
+    if out_file:
+        path = out_file if os.path.isabs(out_file) else os.path.join(ROOT, out_file)
+        allc = {}
+        if os.path.exists(path):
+            with open(path) as fh:
+                allc = json.load(fh)
+        allc.setdefault("source", "rocprofv3 --pmc passes of bench.py --kernel-only (profiles/bp_dental512_pmc.md)")
+        allc.setdefault("kernels", {})[counter_key(form, fetch, zc, ty)] = out
+        with open(path, "w") as fh:
+            json.dump(allc, fh, indent=1)
+            fh.write("\n")
 
 
 def host_cpu():

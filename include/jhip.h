@@ -425,7 +425,9 @@ jh_status jh_bp_get_weight(jh_bp_plan* plan, int* weight, float* scale);
  * shape zc in {8, 16, 32} x ty in {1, 2, 4}, and keeps the fastest in the
  * plan.  h_ms (18 floats, may be NULL) receives the milliseconds per launch
  * in the order fetch, zc, ty (outer to inner); -1 marks a combination that
- * does not apply (the LDS fetch outside geometry class 0).  The per-view cost
+ * does not apply (the LDS fetch outside geometry class 0, or column strips
+ * whose footprint at that tile exceeds the LDS, where the launch would
+ * gather).  The per-view cost
  * depends on the view angle, so nviews should cover the whole orbit.
  * Writes d_vol; synchronises the stream.  No counterpart in the reference. */
 jh_status jh_bp_autotune(jh_bp_plan* plan, const void* d_q, int dtype, int nviews, float* d_vol, int reps,

@@ -1260,7 +1260,12 @@ __global__ void strip_footprint_max_kernel(const Geo* __restrict__ geo, int view
     const float cx0 = (float)min(tx * ex, X - 1), cx1 = (float)min(tx * ex + ex - 1, X - 1);
     const float cy0 = (float)min(tyi * ey, Y - 1), cy1 = (float)min(tyi * ey + ey - 1, Y - 1);
     const StripBox b = strip_footprint<ZC>(geo[view0 + vv], cx0, cx1, cy0, cy1, (float)(tz * ZC), W, V);
-    if ((threadIdx.x & 3) == 0 && grp < total) atomicMax(out, b.nch);
+    // the wave's largest footprint, then one atomic per wave (one per lane
+    // group made the 2^25 groups of a 512^3 x 600 shape queue on one word)
+    int m = grp < total ? b.nch : 0;
+#pragma unroll
+    for (int s = 4; s < 64; s <<= 1) m = max(m, __shfl_xor(m, s));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, m);
 }
 
 // s_waitcnt vmcnt(k), 0 <= k <= 63 (expcnt and lgkmcnt at their maxima): at
@@ -1644,6 +1649,17 @@ static int strip_lds_rounds(int zc, int foot)
     return (rounds >= 1 && bytes <= 65536 && (kStripAhead - 1) * rounds <= 63) ? rounds : 0;
 }
 
+// Copy rounds per view of the strip kernel at the plan's tile shape, or 0
+// where its buffers do not fit (or the footprint cannot be measured now): the
+// launch then gathers.
+static int strip_rounds_of(jh_bp_plan* p, hipStream_t stream)
+{
+    const int ex = p->ty >= 2 ? 16 : 8, ey = p->ty >= 4 ? 16 : 8;
+    const int ntx = (p->X + ex - 1) / ex, nty = (p->Y + ey - 1) / ey, ntz = (p->Z + p->zc - 1) / p->zc;
+    const int foot = strip_footprint_chunks(p, ntx, nty, ntz, ex, ey, stream);
+    return foot < 0 ? 0 : strip_lds_rounds(p->zc, foot);
+}
+
 template <int OUT>
 static bp_kernel_t pick_kernel(int mode, int zc, int dtype, int fetch, int strip)
 {
@@ -1721,8 +1737,7 @@ static void launch_bp(jh_bp_plan* p, const void* q, int dtype, int view0, int nv
     bool strip_lds = p->mode == 0 && fetch == JH_FETCH_LDS && st && (out == kOutNormalised || out == kOutAccumulate);
     int strip_rounds = 0;
     if (strip_lds) {
-        const int foot = strip_footprint_chunks(p, a.ntx, a.nty, a.ntz, ex, ey, stream);
-        strip_rounds = foot < 0 ? 0 : strip_lds_rounds(p->zc, foot);
+        strip_rounds = strip_rounds_of(p, stream);
         if (strip_rounds == 0) {
             strip_lds = false;
             fetch = JH_FETCH_GATHER;
@@ -2239,6 +2254,13 @@ jh_status jh_bp_autotune(jh_bp_plan* p, const void* d_q, int dtype, int nviews, 
                         p->fetch = f;
                         p->zc = zcs[zi];
                         p->ty = tys[ti];
+                        if (f == JH_FETCH_LDS && stage_strip(p, dtype) && p->weight != JH_WEIGHT_FDK &&
+                            strip_rounds_of(p, s) == 0) {
+                            // the strip footprint exceeds the LDS at this tile: the launch
+                            // would gather, which the gather entry already times
+                            if (h_ms) h_ms[n] = -1.0f;
+                            continue;
+                        }
                         const int out = p->weight == JH_WEIGHT_FDK ? kOutFdk : kOutNormalised;
                         launch_bp(p, d_q, dtype, 0, nviews, d_vol, out, s);  // warm-up
                         JH_HIP(hipEventRecord(e0, s));

@@ -373,7 +373,10 @@ def test_bp_autotune_keeps_results(gpu):
     q = bp.stage(filt, "bf16")
     before = bp.run(q, "bf16").cpu().numpy()
     table = bp.autotune(q, "bf16", reps=1)
-    assert len(table) == 18 and all(v > 0 for v in table.values())
+    # every gather tile is timed; an LDS tile is left out where its footprint
+    # exceeds the LDS (the launch would gather)
+    assert all(("gather", z, t) in table for z in (8, 16, 32) for t in (1, 2, 4))
+    assert all(v > 0 for v in table.values())
     after = bp.run(q, "bf16").cpu().numpy()
     print("autotuned", bp.fetch(), "tile", bp.tile(), "rel-RMSE vs default", rel_rmse(after, before))
     # the z tile origin enters the float sample coordinates, so another tile
