@@ -364,6 +364,13 @@ jh_status jh_bp_set_tile(jh_bp_plan* plan, int zc, int ty);
 jh_status jh_bp_set_fetch(jh_bp_plan* plan, int fetch);
 jh_status jh_bp_get_fetch(jh_bp_plan* plan, int* fetch);
 
+/* The fetch form a count-normalised launch on storage type dtype takes now at
+ * the plan's tile shape: JH_FETCH_LDS only where it applies (geometry class
+ * 0; for BF16 / FP16 column strips, workgroups of 4 waves and a footprint
+ * that fits the LDS, whose table this call builds if needed), else
+ * JH_FETCH_GATHER.  Synchronises the stream when it builds the table. */
+jh_status jh_bp_fetch_in_use(jh_bp_plan* plan, int dtype, jh_stream_t stream, int* fetch);
+
 /* Staged layout of BF16/FP16 projections, 8 bytes per pixel either way:
  * JH_LAYOUT_QUAD (a pixel holds the 2 x 2 neighbourhood of a bilinear
  * sample) or JH_LAYOUT_STRIP (a pixel holds four rows v .. v+3 of its

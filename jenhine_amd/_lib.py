@@ -33,7 +33,7 @@ EXPORTS = [
     "jh_bp_create", "jh_bp_uweights", "jh_bp_stage_bytes", "jh_bp_stage", "jh_bp_run", "jh_bp_reset",
     "jh_bp_accumulate", "jh_bp_finalize", "jh_bp_accumulate_noalloc", "jh_bp_set_tile", "jh_bp_get_tile", "jh_bp_geometry_mode",
     "jh_bp_destroy", "jh_bp_pack", "jh_bp_backproject", "jh_fp_project", "jh_bp_autotune",
-    "jh_bp_set_fetch", "jh_bp_get_fetch", "jh_bp_set_layout", "jh_bp_get_layout", "jh_bp_set_zfilter",
+    "jh_bp_set_fetch", "jh_bp_get_fetch", "jh_bp_fetch_in_use", "jh_bp_set_layout", "jh_bp_get_layout", "jh_bp_set_zfilter",
     "jh_bp_set_ctnum", "jh_bp_set_weight", "jh_bp_get_weight", "jh_bp_set_f16_exponent", "jh_bp_get_f16_exponent",
     "jh_zfilter_run",
     "jh_ctnum_run",
@@ -115,6 +115,7 @@ _SIGS = {
     "jh_zfilter_run": (c_int, [P, P, c_int, c_int, c_int, P]),
     "jh_ctnum_run": (c_int, [P, c_size_t, c_float, c_float, c_float, P]),
     "jh_bp_get_fetch": (c_int, [P, P]),
+    "jh_bp_fetch_in_use": (c_int, [P, c_int, P, P]),
     "jh_bp_set_layout": (c_int, [P, c_int]),
     "jh_bp_get_layout": (c_int, [P, P]),
     "jh_graph_create": (c_int, [P, c_int]),

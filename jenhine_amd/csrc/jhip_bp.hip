@@ -536,7 +536,10 @@ struct BPArgs {
     uint32_t frame_bytes;
     int ct;              // apply the CT-number map to normalised output
     float ct_scale, ct_norm1;
-    int lds_px;          // LDS-staged fetch: pixels per coefficient buffer (two buffers); strip kernel: copy rounds per view
+    int lds_px;          // LDS-staged fetch: pixels per coefficient buffer (two buffers)
+    const uint32_t* sbox;  // LDS-staged strips: rectangle origins u0 | v0 << 16 per tile and view
+    int nbox_views;        // views per tile in sbox (the plan's N)
+    int sl_stride, sl_rows;  // LDS-staged strips: chunks per rectangle row (odd), strip rows
     int pxb;             // bytes per staged pixel (16: FP32 quads; 8: BF16/FP16 quads and strips)
     int row_px;          // staged pixels per detector row (W, or W + 1 for strips)
     float wscale;        // distance-weighted outputs: vol = wscale * acc
@@ -1181,34 +1184,46 @@ __global__ void __launch_bounds__(256) bp_colcount_kernel(const Geo* __restrict_
 
 // ---------------------------------------------------------------------------
 // LDS-staged column strips (geometry class 0, BF16 / FP16 column strips,
-// count-normalised forms).
+// count-normalised forms, workgroups of 16 x 16 voxel columns).
 //
-// Per view the workgroup copies the strip pixels of its block's detector
-// footprint -- strip rows [v0, v1] (a strip pixel holds its row and the three
-// below), pixels [u0, u1] (a sample's window is pixels u, u+1; the padding
-// column W included) -- from the staged frame into LDS unchanged, 16 bytes
-// per lane and instruction with buffer_load ... lds (no registers; a chunk
-// reaching past the frame reads 0), and samples there with the operations of
-// the gather form: a sample's 16-byte window is one ds_read2_b64, and one
-// window serves a pair of z samples where the gather form pairs them, so the
-// volume is the gather form's to the bit.  A dedicated copy wave keeps the
-// next views' copies in flight while the other waves sample (see
-// bp_tile_strip_lds).  What this buys: each footprint pixel passes the
-// texture path once per workgroup instead of once per sampling lane, and
-// the window reads (8 B per update) run at the LDS rate, twice the texture
-// path's 64 B per clock and CU.
+// Per view the workgroup copies a rectangle of strip pixels around its
+// block's detector footprint -- strip rows [v0, v0 + rows), pixels from u0 in
+// 16-byte chunks of two pixels, `stride` chunks per row (the largest
+// footprint over the plan's views, measured on the device, rounded up to an
+// odd number of chunks so that consecutive LDS rows start in other banks) --
+// from the staged frame into LDS unchanged, and samples there with the
+// operations of the gather form: a sample's 16-byte window is one
+// ds_read2_b64, and one window serves a pair of z samples where the gather
+// form pairs them, so the volume is the gather form's to the bit.
+//
+// Every thread copies a fixed set of the rectangle's chunks (offsets formed
+// once per workgroup, plus the view's origin) through registers: view v + 1's
+// loads are issued after view v - 1 was written to LDS and written after view
+// v is sampled, into the buffer view v - 1 used, so two buffers, one set of
+// registers and one barrier per view suffice, and no load is waited for
+// while a view is sampled.  (buffer_load ... lds, which needs no registers,
+// was measured at ~70 texture-data clocks per 64-lane 16-byte copy, 2.5x a
+// gather, and the copies then cost as much as the gathers they replace.)
+// The rectangle origins (u0, v0) of every tile and view come from a table
+// built once per tile shape on the device, read with scalar loads.  Partial
+// columns (samples outside the detector rows) keep their count corrections
+// in registers.
+// What this buys: each footprint pixel passes the texture path once per
+// workgroup instead of once per sampling lane (the gather form is bound by
+// the texture data path, ~28 clocks per 64-lane 16-byte gather), and the
+// window reads run from LDS.
 // ---------------------------------------------------------------------------
 
 // The strip rectangle a block of voxel columns [cx0, cx1] x [cy0, cy1], z
 // voxels fk0 .. fk0 + ZC - 1, samples in a view: first pixel u0, first strip
-// row v0, 16-byte chunks per row and in all.  Evaluated by the 4 lanes of
+// row v0, 16-byte chunks per row and strip rows.  Evaluated by the 4 lanes of
 // every lane group of 4 (one per corner column; the result is the group's):
 // u/w and v/w are linear-fractional in the voxel coordinates, so their
 // extremes over the block are at its corners; the reciprocal is the one the
 // samples take, and one pixel of margin on each side covers the difference.
 // A window is pixels u, u+1 (the padding column W included) of one strip row.
 struct StripBox {
-    int u0, v0, cpr, nch;
+    int u0, v0, cpr, rows;
 };
 
 template <int ZC>
@@ -1239,58 +1254,45 @@ __device__ __forceinline__ StripBox strip_footprint(const Geo& g, float cx0, flo
     b.v0 = (int)fminf(fmaxf(floorf(vmin) - 1.0f, 0.0f), Vm1);
     const int v1 = (int)fminf(fmaxf(floorf(vmax) + 1.0f, 0.0f), Vm1);
     b.cpr = (u1 - b.u0 + 2) >> 1;
-    b.nch = b.cpr * (v1 - b.v0 + 1);
+    b.rows = v1 - b.v0 + 1;
     return b;
 }
 
-// The largest strip rectangle (in chunks) over the tiles of a tile shape and
-// views view0 .. view0 + nviews - 1, into *out (atomic max): lane group g of
-// the grid takes tile g / nviews, view g % nviews (workgroup and tile
-// extents as the backprojection kernel's).
+// The rectangle origins of every tile of a tile shape (workgroups of ex x ey
+// voxel columns, ZC deep) in every view, box[tile * N + view] = u0 | v0 << 16,
+// and the largest rectangle: chunks per row into ext[0], strip rows into
+// ext[1] (atomic max, one per wave).  Lane group g of the grid takes tile
+// g / N, view g % N.
 template <int ZC>
-__global__ void strip_footprint_max_kernel(const Geo* __restrict__ geo, int view0, int nviews, int ntx, int nty, int ntz,
-                                           int ex, int ey, int X, int Y, int W, int V, int* __restrict__ out)
+__global__ void strip_box_kernel(const Geo* __restrict__ geo, int N, int ntx, int nty, int ntz, int ex, int ey, int X,
+                                 int Y, int W, int V, uint32_t* __restrict__ box, int* __restrict__ ext)
 {
     const size_t grp = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) >> 2;
-    const size_t total = (size_t)ntx * nty * ntz * nviews;
+    const size_t total = (size_t)ntx * nty * ntz * N;
     const size_t gi = grp < total ? grp : total - 1;  // the lanes of a group stay together
-    const int vv = (int)(gi % nviews);
-    const size_t tile = gi / nviews;
+    const int vv = (int)(gi % N);
+    const size_t tile = gi / N;
     const int tx = (int)(tile % ntx), tyi = (int)((tile / ntx) % nty), tz = (int)(tile / ((size_t)ntx * nty));
     const float cx0 = (float)min(tx * ex, X - 1), cx1 = (float)min(tx * ex + ex - 1, X - 1);
     const float cy0 = (float)min(tyi * ey, Y - 1), cy1 = (float)min(tyi * ey + ey - 1, Y - 1);
-    const StripBox b = strip_footprint<ZC>(geo[view0 + vv], cx0, cx1, cy0, cy1, (float)(tz * ZC), W, V);
-    // the wave's largest footprint, then one atomic per wave (one per lane
-    // group made the 2^25 groups of a 512^3 x 600 shape queue on one word)
-    int m = grp < total ? b.nch : 0;
+    const StripBox b = strip_footprint<ZC>(geo[vv], cx0, cx1, cy0, cy1, (float)(tz * ZC), W, V);
+    if ((threadIdx.x & 3) == 0 && grp < total) box[gi] = (uint32_t)b.u0 | ((uint32_t)b.v0 << 16);
+    // the wave's largest rectangle, then one atomic per wave
+    int mc = grp < total ? b.cpr : 0, mr = grp < total ? b.rows : 0;
 #pragma unroll
-    for (int s = 4; s < 64; s <<= 1) m = max(m, __shfl_xor(m, s));
-    if ((threadIdx.x & 63) == 0) atomicMax(out, m);
-}
-
-// s_waitcnt vmcnt(k), 0 <= k <= 63 (expcnt and lgkmcnt at their maxima): at
-// most k vector-memory operations of this wave are outstanding (they complete
-// in issue order).  The count is an immediate of the instruction.
-__device__ __forceinline__ void wait_vmcnt(int k)
-{
-#define JH_WAIT_CASE(n) \
-    case n: __builtin_amdgcn_s_waitcnt(((n) & 15) | (((n) >> 4) << 14) | 0x0F70); break;
-#define JH_WAIT_CASE8(n) \
-    JH_WAIT_CASE(n) JH_WAIT_CASE(n + 1) JH_WAIT_CASE(n + 2) JH_WAIT_CASE(n + 3) JH_WAIT_CASE(n + 4) \
-    JH_WAIT_CASE(n + 5) JH_WAIT_CASE(n + 6) JH_WAIT_CASE(n + 7)
-    switch (k) {
-        JH_WAIT_CASE8(0) JH_WAIT_CASE8(8) JH_WAIT_CASE8(16) JH_WAIT_CASE8(24) JH_WAIT_CASE8(32)
-        JH_WAIT_CASE8(40) JH_WAIT_CASE8(48) JH_WAIT_CASE8(56)
-    default: __builtin_amdgcn_s_waitcnt(0x0F70); break;
+    for (int s = 4; s < 64; s <<= 1) {
+        mc = max(mc, __shfl_xor(mc, s));
+        mr = max(mr, __shfl_xor(mr, s));
     }
-#undef JH_WAIT_CASE8
-#undef JH_WAIT_CASE
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(ext, mc);
+        atomicMax(ext + 1, mr);
+    }
 }
 
-// A view's geometry through the constant address space: scalar loads, which
-// the copies into LDS (vector-memory operations that write memory) do not
-// hold up -- a vector load of it would make the wave wait for every copy in
-// flight before its own result.
+// A view's geometry through the constant address space (scalar loads): the
+// sampling loop then waits for no vector-memory load, so the next views'
+// copies stay in flight.
 __device__ __forceinline__ Geo geo_of(const Geo* geo, int view)
 {
     const __attribute__((address_space(4))) float* p = (const __attribute__((address_space(4))) float*)(geo + view);
@@ -1300,111 +1302,86 @@ __device__ __forceinline__ Geo geo_of(const Geo* geo, int view)
     return g;
 }
 
-constexpr int kStripAhead = 2;               // views in flight beyond the one sampled
-constexpr int kStripBufs = kStripAhead + 1;  // footprint buffers (and boxes) per workgroup
+// Copy instructions per thread and view: the rectangle holds at most
+// kStripRounds(ZC) x 256 chunks (dental-scale footprints of 16 x 16 columns:
+// about 19 chunks x (1.44 ZC + 7) rows).
+constexpr int kStripRounds(int zc) { return zc >= 32 ? 5 : zc >= 16 ? 3 : 2; }
 
-// The workgroup: blockDim.y - 1 sampling waves (the voxel columns of
-// bp_tile) and one copy wave, the last.  Per view v the copy wave computes
-// the footprint of view v + kStripAhead, issues its copy (a.lds_px rounds of
-// 64 chunks of 16 bytes, whatever the footprint's size: the rounds past it
-// copy its last chunk again into the buffer's slack), then waits until view
-// v + 1 has landed (the copies complete in issue order), while the sampling
-// waves sample view v; one barrier per view.  Only the copy wave has copies
-// in flight, so the waits the compiler places before the sampling waves'
-// LDS reads (for copies that might write what they read) do not stall them.
 template <int ZC, int QT, int OUT>
 __device__ __forceinline__ void bp_tile_strip_lds(const BPArgs& a, unsigned tile)
 {
     static_assert(is_strip<QT>() && (OUT == kOutNormalised || OUT == kOutAccumulate),
                   "column strips, count-normalised forms");
     typedef Sampler<QT> S;
-    __shared__ int box[kStripBufs][4];  // u0, v0, chunks per row, chunks
-    __shared__ float cnt_part[ZC * 256];
-    const int nsw = blockDim.y - 1;  // sampling waves
-    const bool copier = threadIdx.y == nsw;
+    constexpr int MR = kStripRounds(ZC);
     const int tx = tile % a.ntx;
     const unsigned rest = tile / a.ntx;
     const int tyi = rest % a.nty;
     const int tz = rest / a.nty;
-    // the voxel columns of bp_tile with nsw waves
-    const int px = nsw >= 2 ? 2 : 1;
-    const int wx = threadIdx.y % px, wy = threadIdx.y / px;
-    const int ex = px * 8, ey = (nsw / px) * 8;
-    const int i = tx * ex + wx * 8 + (threadIdx.x & 7);
-    const int j = tyi * ey + wy * 8 + (threadIdx.x >> 3);
+    // 16 x 16 voxel columns: four 8 x 8 waves (as bp_tile with four waves)
+    const int wx = threadIdx.y & 1, wy = threadIdx.y >> 1;
+    const int i = tx * 16 + wx * 8 + (threadIdx.x & 7);
+    const int j = tyi * 16 + wy * 8 + (threadIdx.x >> 3);
     const int k0 = tz * ZC;
     const float fi = (float)min(i, a.X - 1);
     const float fj = (float)min(j, a.Y - 1);
     const float fk0 = (float)k0;
-    const float cx0 = (float)min(tx * ex, a.X - 1), cx1 = (float)min(tx * ex + ex - 1, a.X - 1);
-    const float cy0 = (float)min(tyi * ey, a.Y - 1), cy1 = (float)min(tyi * ey + ey - 1, a.Y - 1);
     const float Wm1 = (float)(a.W - 1), Vm1 = (float)(a.V - 1);
     const uint32_t rowb = (uint32_t)a.row_px * 8u;  // strip pixels are 8 bytes
-    const int tid = threadIdx.y * 64 + threadIdx.x;
-    const int rounds = a.lds_px;      // copy rounds per view (64 chunks each)
-    const int cap = rounds * 64;      // 16-byte chunks per buffer
     const int n = a.nviews;
+    const int stride = a.sl_stride;                   // chunks per rectangle row (odd)
+    const uint32_t lrow = (uint32_t)stride * 16u;     // LDS bytes per rectangle row
+    const int rect_chunks = stride * a.sl_rows;       // at most MR x 256 (host-checked)
+    constexpr uint32_t kBufBytes = MR * 256 * 16;  // two buffers
     unsigned char* const lbuf = reinterpret_cast<unsigned char*>(lds_dyn);
+    const int tid = threadIdx.y * 64 + threadIdx.x;
+    const __attribute__((address_space(4))) uint32_t* boxes =
+        (const __attribute__((address_space(4))) uint32_t*)a.sbox + (size_t)tile * a.nbox_views + a.view0;
 
-    // copy wave: footprint of view vv into box slot and buffer slot (both vv % kStripBufs)
-    auto stage = [&](int vv) {
-        const int slot = vv % kStripBufs;
-        const StripBox bx = strip_footprint<ZC>(geo_of(a.geo, a.view0 + vv), cx0, cx1, cy0, cy1, fk0, a.W, a.V);
-        if (threadIdx.x == 0) {
-            box[slot][0] = bx.u0;
-            box[slot][1] = bx.v0;
-            box[slot][2] = bx.cpr;
-            box[slot][3] = bx.nch;
-        }
+    // this thread's chunks c = tid + 256 r of the rectangle, row c / stride,
+    // chunk c % stride (chunks past the rectangle copy its last chunk again,
+    // to the same place)
+    uint32_t goff[MR], loff[MR];
+#pragma unroll
+    for (int r = 0; r < MR; ++r) {
+        const int c = min(tid + 256 * r, rect_chunks - 1);
+        const int row = c / stride, col = c - row * stride;
+        goff[r] = (uint32_t)row * rowb + (uint32_t)col * 16u;
+        loff[r] = (uint32_t)row * lrow + (uint32_t)col * 16u;
+    }
+    // (views past the last load the last again: every view issues MR loads,
+    // so the wait before a store counts statically)
+    auto load = [&](int vv, u32x4_t (&regs)[MR]) {
+        vv = min(vv, n - 1);
+        const uint32_t bx = boxes[vv];
         const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
             const_cast<char*>(reinterpret_cast<const char*>(a.q)) + (size_t)vv * a.frame_bytes, (short)0,
             (int)a.frame_bytes, kBufferFlags);
-        const float rinv = 1.0f / (float)bx.cpr;
-        const uint32_t gbase = (uint32_t)bx.v0 * rowb + (uint32_t)bx.u0 * 8u;
-        const int last = min(bx.nch, cap) - 1;  // the host sizes cap for the largest footprint
-        unsigned char* const dst = lbuf + (size_t)slot * cap * 16;
-        for (int r = 0; r < rounds; ++r) {
-            const int c = min(r * 64 + (int)threadIdx.x, last);
-            const int row = (int)(((float)c + 0.5f) * rinv);  // exact: (c + 0.5) / cpr is 0.5 / cpr from an integer
-            const uint32_t off = gbase + (uint32_t)row * rowb + (uint32_t)(c - row * bx.cpr) * 16u;
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (__attribute__((address_space(3))) void*)(dst + r * 64 * 16),
-                                                     16, off, 0, 0, 0);
-        }
+        // (the origin goes into the vector offset: the range check of a raw
+        // buffer does not count the scalar offset)
+        const uint32_t base = (bx >> 16) * rowb + (bx & 0xffffu) * 8u;
+#pragma unroll
+        for (int r = 0; r < MR; ++r) regs[r] = __builtin_amdgcn_raw_buffer_load_b128(rs, goff[r] + base, 0, 0);
+    };
+    auto store = [&](int vv, const u32x4_t (&regs)[MR]) {
+        unsigned char* const dst = lbuf + (vv & 1) * kBufBytes;
+#pragma unroll
+        for (int r = 0; r < MR; ++r) *reinterpret_cast<u32x4_t*>(dst + loff[r]) = regs[r];
     };
 
-    float acc[ZC];
+    float acc[ZC], cpart[ZC];
 #pragma unroll
-    for (int kk = 0; kk < ZC; ++kk) acc[kk] = 0.0f;
-    // the partial-column count corrections, one slot per thread and sample
-    // (touched by that thread only), as in bp_tile
-    if (!copier) {
-#pragma unroll
-        for (int kk = 0; kk < ZC; ++kk) cnt_part[kk * 256 + tid] = 0.0f;
-    }
+    for (int kk = 0; kk < ZC; ++kk) { acc[kk] = 0.0f; cpart[kk] = 0.0f; }
 
-    if (copier && n > 0) {
-        for (int w = 0; w < kStripAhead && w < n; ++w) stage(w);
-        wait_vmcnt((min(kStripAhead, n) - 1) * rounds);  // view 0 has landed
-    }
-    for (int v = 0; v < n; ++v) {
-        // view v has landed (the copy wave waited for it), its box is
-        // visible, and every sampling wave is done with view v - 1
-        __syncthreads();
-        if (copier) {
-            if (v + kStripAhead < n) stage(v + kStripAhead);
-            // views in flight after view v + 1: v + 2 .. min(v + kStripAhead, n - 1)
-            const int after = min(v + kStripAhead, n - 1) - (v + 1);
-            if (v + 1 < n) wait_vmcnt(max(after, 0) * rounds);
-            continue;
-        }
-        const int slot = v % kStripBufs;
+    // sample view v from its buffer
+    auto sample = [&](int v) {
         const Geo g = geo_of(a.geo, a.view0 + v);
         const float Uc = fmaf(g.c[3], fk0, fmaf(g.c[1], fi, fmaf(g.c[2], fj, g.c[0])));
         const float Vc = fmaf(g.c[7], fk0, fmaf(g.c[5], fi, fmaf(g.c[6], fj, g.c[4])));
         const float Wc = fmaf(g.c[11], fk0, fmaf(g.c[9], fi, fmaf(g.c[10], fj, g.c[8])));
         const float rw = __builtin_amdgcn_rcpf(Wc);
         const float su = fmaf(Uc, rw, 0.5f);
-        if (!(su >= 0.0f && su <= Wm1)) continue;
+        if (!(su >= 0.0f && su <= Wm1)) return;
         const int iu = (int)su;
         float fe;
         const typename S::wt_t wq = S::weight(__builtin_amdgcn_fractf(su), fe);
@@ -1412,14 +1389,13 @@ __device__ __forceinline__ void bp_tile_strip_lds(const BPArgs& a, unsigned tile
         const float dsv = g.c[7] * rw;
         const float svl = fmaf((float)(ZC - 1), dsv, sv0);
         const bool all_in = fminf(sv0, svl) >= 0.0f && fmaxf(sv0, svl) <= Vm1;
-        // (a footprint over the buffer, which the host rules out, samples nothing)
-        if (box[slot][3] > cap) continue;
-        const unsigned char* lbase = lbuf + (size_t)slot * cap * 16;
-        const uint32_t lrow = (uint32_t)box[slot][2] * 16u;
-        const uint32_t lb = (uint32_t)(iu - box[slot][0]) * 8u - (uint32_t)box[slot][1] * lrow;
-        // the window at byte offset o = row * lrow + lb of the footprint
+        const uint32_t bx = boxes[v];
+        // the window of detector row `row` at byte o = row * lrow + lb of the
+        // LDS (lb holds the buffer, the column and the rectangle origin)
+        const uint32_t lb = (uint32_t)(v & 1) * kBufBytes + (uint32_t)(iu - (int)(bx & 0xffffu)) * 8u -
+                            (bx >> 16) * lrow;
         const auto fetch = [&](uint32_t o) -> u32x4_t {
-            const uint2* q = reinterpret_cast<const uint2*>(lbase + o);
+            const uint2* q = reinterpret_cast<const uint2*>(lbuf + o);
             const uint2 x = q[0], y = q[1];
             return (u32x4_t){x.x, x.y, y.x, y.y};
         };
@@ -1446,7 +1422,8 @@ __device__ __forceinline__ void bp_tile_strip_lds(const BPArgs& a, unsigned tile
             // every sample counts: the column's count weight is in a.colcnt
         } else {
             // the count weight of the samples outside the detector rows is
-            // taken off a.colcnt at the end
+            // taken off a.colcnt at the end (this load is waited for, and
+            // with it the copies in flight: partial views are rare)
             const float2 uwq = a.uwp[(size_t)(a.view0 + v) * a.W + iu];
             const float cw = fmaf(fe, uwq.y - uwq.x, uwq.x);
 #pragma unroll
@@ -1469,20 +1446,35 @@ __device__ __forceinline__ void bp_tile_strip_lds(const BPArgs& a, unsigned tile
                 for (int q = 0; q < kZB; ++q) {
                     const float sum = S::accum(acc[kb + q], p[q], fv[q], wq);
                     acc[kb + q] = in[q] ? sum : acc[kb + q];
-                    cnt_part[(kb + q) * 256 + tid] += in[q] ? 0.0f : cw;
+                    cpart[kb + q] += in[q] ? 0.0f : cw;
                 }
             }
         }
+    };
+    u32x4_t regs[MR];
+    if (n > 0) {
+        load(0, regs);
+        store(0, regs);
+        load(1, regs);
+    }
+    for (int v = 0; v < n; ++v) {
+        // buffer v & 1 holds view v (every wave stored its part before the
+        // barrier), and every wave is done with view v - 1, whose buffer
+        // view v + 1 takes
+        __syncthreads();
+        sample(v);
+        store(v + 1, regs);
+        load(v + 2, regs);
     }
 
-    if (copier || i >= a.X || j >= a.Y) return;
+    if (i >= a.X || j >= a.Y) return;
     const float c = a.colcnt[(size_t)j * a.X + i];
     const size_t plane = (size_t)a.X * a.Y;
     size_t idx = ((size_t)k0 * a.Y + j) * a.X + i;
 #pragma unroll
     for (int kk = 0; kk < ZC; ++kk) {
         if (k0 + kk < a.Z) {
-            const float cnt = c - cnt_part[kk * 256 + tid];
+            const float cnt = c - cpart[kk];
             if constexpr (OUT == kOutNormalised) {
                 const float v = acc[kk] / cnt * a.qscale;
                 a.vol[idx] = a.ct ? ct_number(v, a.ct_scale, a.ct_norm1) : v;
@@ -1495,10 +1487,10 @@ __device__ __forceinline__ void bp_tile_strip_lds(const BPArgs& a, unsigned tile
     }
 }
 
-// The LDS-staged strip kernel: bp_kernel's tile schedule with one more wave
-// per workgroup (the copy wave).
+// The LDS-staged strip kernel: bp_kernel's tile schedule, 16 x 16 voxel
+// columns (64 x 4 threads) per workgroup.
 template <int ZC, int QT, int OUT>
-__global__ void __launch_bounds__(320) bp_strip_lds_kernel(BPArgs a)
+__global__ void __launch_bounds__(256) bp_strip_lds_kernel(BPArgs a)
 {
     const unsigned ntiles = (unsigned)a.ntx * a.nty * a.ntz;
     const unsigned t8 = gridDim.x >> 3;  // gridDim.x = 8 * ceil(ntiles / 8)
@@ -1601,63 +1593,54 @@ static bp_kernel_t pick_strip_lds(int zc, int dtype)
     return nullptr;
 }
 
-// The largest strip footprint (16-byte chunks) of the plan's tile shape over
-// all its views, measured once per shape on the device with the kernel's own
-// arithmetic; -1 where it cannot be measured now (the stream is capturing a
-// graph: the launch then gathers).
-static int strip_footprint_chunks(jh_bp_plan* p, int ntx, int nty, int ntz, int ex, int ey, hipStream_t stream)
+// The LDS-staged strip kernel's rectangle table for the plan's tile shape
+// (16 x 16 voxel columns, zc deep): built once per shape on the device with
+// the kernel's own arithmetic (origins of every tile and view, the largest
+// rectangle), kept in the plan.  True when the kernel can run this shape:
+// workgroups of 4 waves, a rectangle of at most kStripRounds(zc) x 256 chunks,
+// 16-bit origins; false also where the table cannot be built now (the stream
+// is capturing a graph): the launch then gathers.
+static bool strip_table(jh_bp_plan* p, hipStream_t stream)
 {
-    const int zi = p->zc >= 32 ? 2 : p->zc >= 16 ? 1 : 0, ti = p->ty >= 4 ? 2 : p->ty >= 2 ? 1 : 0;
-    int& c = p->strip_chunks[zi][ti];
-    if (c >= 0) return c;
+    if (p->ty != 4 || p->W > 65535 || p->V > 65535) return false;
+    if (p->sl_zc == p->zc) return p->sl_stride > 0;
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     JH_HIP(hipStreamIsCapturing(stream, &cs));
-    if (cs != hipStreamCaptureStatusNone) return -1;
-    const size_t lanes = (size_t)ntx * nty * ntz * p->N * 4;
+    if (cs != hipStreamCaptureStatusNone) return false;
+    const int ex = 16, ey = 16;
+    const int ntx = (p->X + ex - 1) / ex, nty = (p->Y + ey - 1) / ey, ntz = (p->Z + p->zc - 1) / p->zc;
+    const size_t entries = (size_t)ntx * nty * ntz * p->N;
+    const size_t lanes = entries * 4;
     require(lanes / 256 < (1ull << 31), "backprojection footprint grid too large");
-    DevBuf d(sizeof(int));
-    const int zero = 0;
-    JH_HIP(hipMemcpyAsync(d.get(), &zero, sizeof(int), hipMemcpyHostToDevice, stream));
+    p->sl_zc = -1;
+    p->sl_stride = 0;
+    p->sbox.release();
+    p->sbox.alloc(entries * sizeof(uint32_t));
+    DevBuf d(2 * sizeof(int));
+    const int zero[2] = {0, 0};
+    JH_HIP(hipMemcpyAsync(d.get(), zero, sizeof(zero), hipMemcpyHostToDevice, stream));
     const unsigned grid = (unsigned)((lanes + 255) / 256);
     const Geo* geo = p->geo.as<Geo>();
+    uint32_t* box = p->sbox.as<uint32_t>();
     if (p->zc >= 32)
-        strip_footprint_max_kernel<32><<<grid, 256, 0, stream>>>(geo, 0, p->N, ntx, nty, ntz, ex, ey, p->X, p->Y, p->W,
-                                                                 p->V, d.as<int>());
+        strip_box_kernel<32><<<grid, 256, 0, stream>>>(geo, p->N, ntx, nty, ntz, ex, ey, p->X, p->Y, p->W, p->V, box,
+                                                       d.as<int>());
     else if (p->zc >= 16)
-        strip_footprint_max_kernel<16><<<grid, 256, 0, stream>>>(geo, 0, p->N, ntx, nty, ntz, ex, ey, p->X, p->Y, p->W,
-                                                                 p->V, d.as<int>());
+        strip_box_kernel<16><<<grid, 256, 0, stream>>>(geo, p->N, ntx, nty, ntz, ex, ey, p->X, p->Y, p->W, p->V, box,
+                                                       d.as<int>());
     else
-        strip_footprint_max_kernel<8><<<grid, 256, 0, stream>>>(geo, 0, p->N, ntx, nty, ntz, ex, ey, p->X, p->Y, p->W,
-                                                                p->V, d.as<int>());
-    check_launch("strip_footprint_max_kernel");
-    int h = 0;
-    JH_HIP(hipMemcpyAsync(&h, d.get(), sizeof(int), hipMemcpyDeviceToHost, stream));
+        strip_box_kernel<8><<<grid, 256, 0, stream>>>(geo, p->N, ntx, nty, ntz, ex, ey, p->X, p->Y, p->W, p->V, box,
+                                                      d.as<int>());
+    check_launch("strip_box_kernel");
+    int h[2] = {0, 0};
+    JH_HIP(hipMemcpyAsync(h, d.get(), sizeof(h), hipMemcpyDeviceToHost, stream));
     JH_HIP(hipStreamSynchronize(stream));
-    c = h;
-    return c;
-}
-
-// Copy rounds (64 chunks of 16 bytes) per view of the strip kernel for a
-// largest footprint of foot chunks, or 0 where its buffers (kStripBufs of
-// them, beside its static footprint boxes and the partial-column count
-// slots, ZC per thread of the 256 sampling threads) would exceed 64 KiB or
-// its waits the 63 outstanding operations of the counter.
-static int strip_lds_rounds(int zc, int foot)
-{
-    const int rounds = (foot + 63) / 64;
-    const size_t bytes = (size_t)kStripBufs * rounds * 64 * 16 + 64 + (size_t)zc * 256 * 4;
-    return (rounds >= 1 && bytes <= 65536 && (kStripAhead - 1) * rounds <= 63) ? rounds : 0;
-}
-
-// Copy rounds per view of the strip kernel at the plan's tile shape, or 0
-// where its buffers do not fit (or the footprint cannot be measured now): the
-// launch then gathers.
-static int strip_rounds_of(jh_bp_plan* p, hipStream_t stream)
-{
-    const int ex = p->ty >= 2 ? 16 : 8, ey = p->ty >= 4 ? 16 : 8;
-    const int ntx = (p->X + ex - 1) / ex, nty = (p->Y + ey - 1) / ey, ntz = (p->Z + p->zc - 1) / p->zc;
-    const int foot = strip_footprint_chunks(p, ntx, nty, ntz, ex, ey, stream);
-    return foot < 0 ? 0 : strip_lds_rounds(p->zc, foot);
+    const int stride = h[0] | 1;  // odd: consecutive LDS rows start in other banks
+    p->sl_zc = p->zc;
+    p->sl_rows = h[1];
+    p->sl_stride = (h[0] >= 1 && h[1] >= 1 && (size_t)stride * h[1] <= (size_t)kStripRounds(p->zc) * 256) ? stride : 0;
+    if (p->sl_stride == 0) p->sbox.release();
+    return p->sl_stride > 0;
 }
 
 template <int OUT>
@@ -1735,13 +1718,9 @@ static void launch_bp(jh_bp_plan* p, const void* q, int dtype, int view0, int nv
     // count-weight rows (4 KiB + 48 B), stay within 64 KiB.
     int fetch = p->fetch;
     bool strip_lds = p->mode == 0 && fetch == JH_FETCH_LDS && st && (out == kOutNormalised || out == kOutAccumulate);
-    int strip_rounds = 0;
-    if (strip_lds) {
-        strip_rounds = strip_rounds_of(p, stream);
-        if (strip_rounds == 0) {
-            strip_lds = false;
-            fetch = JH_FETCH_GATHER;
-        }
+    if (strip_lds && !strip_table(p, stream)) {
+        strip_lds = false;
+        fetch = JH_FETCH_GATHER;
     }
     const bool staged = p->mode == 0 && fetch == JH_FETCH_LDS && !distance_weighted(out) && !strip_lds;
     bp_kernel_t k = out == kOutNormalised      ? pick_kernel<kOutNormalised>(p->mode, p->zc, dtype, fetch, st)
@@ -1749,8 +1728,13 @@ static void launch_bp(jh_bp_plan* p, const void* q, int dtype, int view0, int nv
                     : out == kOutFdk           ? pick_kernel<kOutFdk>(p->mode, p->zc, dtype, fetch, st)
                     : out == kOutFdkAccumulate ? pick_kernel<kOutFdkAccumulate>(p->mode, p->zc, dtype, fetch, st)
                                                : pick_kernel<kOutRaw>(p->mode, p->zc, dtype, fetch, st);
-    a.lds_px = strip_lds ? strip_rounds : staged ? std::min(64 * p->zc, (65536 - 4096 - 64) / 32) : 0;
-    const size_t lds_bytes = strip_lds ? (size_t)kStripBufs * strip_rounds * 64 * 16
+    a.lds_px = staged ? std::min(64 * p->zc, (65536 - 4096 - 64) / 32) : 0;
+    a.sbox = strip_lds ? p->sbox.as<uint32_t>() : nullptr;
+    a.nbox_views = p->N;
+    a.sl_stride = strip_lds ? p->sl_stride : 0;
+    a.sl_rows = strip_lds ? p->sl_rows : 0;
+    // (two rectangle buffers of kStripRounds(zc) x 256 chunks of 16 bytes)
+    const size_t lds_bytes = strip_lds ? 2 * (size_t)kStripRounds(p->zc) * 256 * 16
                              : staged  ? 2 * (size_t)a.lds_px * sizeof(float4) : 0;
     if (p->mode == 0 && !staged && (out == kOutNormalised || out == kOutAccumulate)) {
         // the count weights of the launch's views per voxel column
@@ -1765,7 +1749,7 @@ static void launch_bp(jh_bp_plan* p, const void* q, int dtype, int view0, int nv
         check_launch("bp_colcount_kernel");
     }
     // (the strip kernel's workgroups carry one more wave, its copy wave)
-    hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(64, p->ty + (strip_lds ? 1 : 0)), lds_bytes, stream, a);
+    hipLaunchKernelGGL(k, dim3((unsigned)grid), dim3(64, p->ty), lds_bytes, stream, a);
     check_launch("bp_kernel");
 }
 
@@ -1808,8 +1792,9 @@ jh_status jh_bp_create(jh_bp_plan** plan, int device, const double* h_ptm, int n
             p->f16exp = 0;
             p->acc_scale = 1.0f;
             p->acc_dtype = -1;
-            for (auto& r : p->strip_chunks)
-                for (int& c : r) c = -1;
+            p->sl_zc = -1;
+            p->sl_stride = 0;
+            p->sl_rows = 0;
             p->ptm.alloc(sizeof(double) * 12 * nviews);
             p->geo.alloc(sizeof(Geo) * nviews);
             p->uw.alloc(sizeof(float) * (size_t)nviews * W);
@@ -2146,6 +2131,18 @@ jh_status jh_bp_get_fetch(jh_bp_plan* p, int* fetch)
     });
 }
 
+jh_status jh_bp_fetch_in_use(jh_bp_plan* p, int dtype, jh_stream_t stream, int* fetch)
+{
+    return guard([&] {
+        require(p != nullptr && fetch != nullptr, "jh_bp_fetch_in_use: bad arguments");
+        require(dtype == JH_F32 || dtype == JH_BF16 || dtype == JH_F16, "jh_bp_fetch_in_use: unknown dtype");
+        DeviceScope ds(p->device);
+        int f = p->mode == 0 ? p->fetch : JH_FETCH_GATHER;
+        if (f == JH_FETCH_LDS && stage_strip(p, dtype) && !strip_table(p, as_stream(stream))) f = JH_FETCH_GATHER;
+        *fetch = f;
+    });
+}
+
 jh_status jh_bp_set_zfilter(jh_bp_plan* p, int enable)
 {
     return guard([&] {
@@ -2255,7 +2252,7 @@ jh_status jh_bp_autotune(jh_bp_plan* p, const void* d_q, int dtype, int nviews, 
                         p->zc = zcs[zi];
                         p->ty = tys[ti];
                         if (f == JH_FETCH_LDS && stage_strip(p, dtype) && p->weight != JH_WEIGHT_FDK &&
-                            strip_rounds_of(p, s) == 0) {
+                            !strip_table(p, s)) {
                             // the strip footprint exceeds the LDS at this tile: the launch
                             // would gather, which the gather entry already times
                             if (h_ms) h_ms[n] = -1.0f;
@@ -2284,6 +2281,9 @@ jh_status jh_bp_autotune(jh_bp_plan* p, const void* d_q, int dtype, int nviews, 
         p->zc = bz;
         p->ty = bt;
         p->fetch = bf;
+        // the LDS-staged strips' table for the chosen shape, built now (a
+        // launch under graph capture cannot build it)
+        if (p->fetch == JH_FETCH_LDS && p->mode == 0 && stage_strip(p, dtype)) (void)strip_table(p, s);
     });
 }
 

@@ -30,8 +30,9 @@ struct jh_bp_plan {
     int f16exp;      // FP16 storage holds q * 2^-f16exp; outputs are scaled back by 2^f16exp
     float acc_scale; // output scale of what the accumulators hold (2^f16exp of FP16 chunks, else 1)
     int acc_dtype;   // storage type of the chunks accumulated since the last reset (-1: none)
-    int strip_chunks[3][3];  // LDS-staged strips: largest footprint (16-byte chunks) over all views per
-                             // tile shape (zc 8 / 16 / 32 x ty 1 / 2 / 4); -1 until measured
+    int sl_zc;       // LDS-staged strips: the zc the rectangle table (sbox) was built for (-1: none)
+    int sl_stride;   // its chunks per rectangle row (odd; 0: the shape does not fit the LDS)
+    int sl_rows;     // its strip rows per rectangle
     jh::DevBuf ptm;  // N x 12 double
     jh::DevBuf geo;  // N x Geo
     jh::DevBuf uw;   // N x W float
@@ -40,4 +41,5 @@ struct jh_bp_plan {
     jh::DevBuf acc;  // X*Y*Z float (streaming form, allocated on first use)
     jh::DevBuf cnt;  // X*Y*Z float
     jh::DevBuf xt;   // X*Y*Z float, z-fastest copy of the volume for the projector
+    jh::DevBuf sbox; // LDS-staged strips: rectangle origins per tile and view (ntiles x N uint32)
 };

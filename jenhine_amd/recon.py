@@ -412,6 +412,13 @@ class BPCore:
         call("jh_bp_get_fetch", self._h, ctypes.byref(f))
         return {v: k for k, v in self.FETCH.items()}[f.value]
 
+    def fetch_in_use(self, dtype):
+        """The fetch form a count-normalised launch on `dtype` takes now:
+        "lds" only where the LDS-staged form applies at the current tile."""
+        f = ctypes.c_int()
+        call("jh_bp_fetch_in_use", self._h, DTYPES[dtype], _stream(), ctypes.byref(f))
+        return {v: k for k, v in self.FETCH.items()}[f.value]
+
     def tile(self):
         zc, ty = ctypes.c_int(), ctypes.c_int()
         call("jh_bp_get_tile", self._h, ctypes.byref(zc), ctypes.byref(ty))

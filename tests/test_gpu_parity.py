@@ -385,12 +385,14 @@ def test_bp_autotune_keeps_results(gpu):
 
 
 @pytest.mark.parametrize("dtype", ["f32", "bf16", "f16"])
-@pytest.mark.parametrize("tile", [(32, 4), (16, 2), (8, 1)])
+@pytest.mark.parametrize("tile", [(32, 4), (16, 4), (8, 4), (16, 2), (8, 1)])
 def test_bp_lds_fetch_matches_gather(gpu, dtype, tile):
     """The LDS-staged fetch samples the same positions from the same values:
     FP32 quads are staged as bilinear coefficients (only the association of
-    the bilinear terms differs); BF16 / FP16 column strips are staged as they
-    are and sampled with the gather form's operations (equal to the bit)."""
+    the bilinear terms differs); BF16 / FP16 column strips are copied into LDS
+    as they are and sampled with the gather form's operations (equal to the
+    bit).  Strips take the LDS form in workgroups of 4 waves (16 x 16 voxel
+    columns) only; at the other tiles the launch gathers."""
     w = SMALL
     ptm = w.ptms()
     filt = torch.from_numpy(synth.normal(21, (w.n_views, w.n_v, w.n_u))).to(gpu)
@@ -403,6 +405,8 @@ def test_bp_lds_fetch_matches_gather(gpu, dtype, tile):
     for f in ("gather", "lds"):
         bp.set_fetch(f)
         assert bp.fetch() == f
+        expect = "lds" if f == "lds" and (dtype == "f32" or tile[1] == 4) else "gather"
+        assert bp.fetch_in_use(dtype) == expect
         vol = bp.run(q, dtype).cpu().numpy()
         bp.reset()
         bp.accumulate(q[: q.numel() // 2], dtype, 0, w.n_views // 2)
@@ -624,7 +628,7 @@ DENTAL_STEP = configs.DENTAL512.scaled(n_views=60, n_u=192, n_v=192, du=0.8, X=1
 
 
 @pytest.mark.parametrize("dtype", ["bf16", "f16"])
-@pytest.mark.parametrize("tile", [(8, 4), (16, 2), (32, 1)])
+@pytest.mark.parametrize("tile", [(8, 4), (16, 4), (32, 4), (16, 2), (32, 1)])
 @pytest.mark.parametrize("geom", ["cbct-step-0.83", "dental-step-1.29"])
 def test_bp_strip_layout_equals_quad_layout_bit_exact(gpu, dtype, tile, geom):
     """The column-strip layout's paired gathers read the same four values per
@@ -655,6 +659,7 @@ def test_bp_strip_layout_equals_quad_layout_bit_exact(gpu, dtype, tile, geom):
         out[layout] = (vol, bp.finalize().cpu().numpy())
         if layout == "strip":
             bp.set_fetch("lds")
+            print(tile, geom, "strip fetch in use:", bp.fetch_in_use(dtype))
             vol = bp.run(q, dtype).cpu().numpy()
             bp.reset()
             bp.accumulate(q[:h], dtype, 0, w.n_views // 2)
