@@ -25,6 +25,10 @@ rel-RMSE against the FP64 oracle on the centre slices.
 A and the un-normalised backprojector A^T on 256^3 from 360 views of 512^2,
 with the dot-product test of the pair in the same run.
 
+--workload pipeline_dental512 times dental512 through the C++ agent graph
+(jh_fdk_pipeline) with the raw counts and the slices in host memory: PCIe
+transfers included.
+
 The roofline object covers the dominant kernel (bp_kernel), timed with events
 on the stream it runs on.  Its roof is the lowest of three, each from the
 kernel's own schedule and the counters committed under profiles/ (rocprofv3
@@ -89,8 +93,10 @@ def main():
     ap.add_argument("--filter-form", choices=("mfma", "mfma6", "mfma16"), default="mfma16",
                     help="MFMA filter of the bf16 and f16 chains: hi/lo BF16 (3 products), hi/mid/lo BF16 "
                          "(6 products) or row-scaled hi/lo FP16 (3 products)")
-    ap.add_argument("--workload", choices=("dental512", "batch16", "offset_dental512", "adjoint256"),
+    ap.add_argument("--workload", choices=("dental512", "batch16", "offset_dental512", "adjoint256",
+                                           "pipeline_dental512"),
                     default="dental512")
+    ap.add_argument("--chunk", type=int, default=64, help="views per chunk of the agent graph (pipeline_dental512)")
     ap.add_argument("--branches", type=int, default=4, help="parallel graph branches (batch16)")
     ap.add_argument("--summarize-counters", metavar="DIR", default=None,
                     help="no GPU work: read the rocprofv3 --pmc CSV files under DIR (kernel-only runs of one "
@@ -128,6 +134,8 @@ def main():
         return run_offset(args, dev)
     if args.workload == "adjoint256":
         return run_adjoint(args, dev)
+    if args.workload == "pipeline_dental512":
+        return run_pipeline(args)
     w = configs.DENTAL512
     if args.views:
         w = w.scaled(n_views=args.views)
@@ -555,6 +563,61 @@ def run_adjoint(args, dev):
         "forward": {"ms": fp_ms, "gups": units / (fp_ms * 1e-3) / 1e9},
         "back": {"ms": bp_ms, "gups": units / (bp_ms * 1e-3) / 1e9},
         "dot_test_rel_gap": gap,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def run_pipeline(args):
+    """dental512 through the C++ agent graph (jh_fdk_pipeline): raw counts in
+    host memory streamed to the device per view chunk, slices downloaded to
+    host memory, BF16 storage and the headline filter form; value = GUPS of
+    the graph's run (Start to Join), PCIe transfers included."""
+    import subprocess
+    import tempfile
+    from jenhine_amd import configs
+
+    app = os.path.join(ROOT, "jenhine_amd", "host", "build", "jh_fdk_pipeline")
+    if not os.path.exists(app):
+        sys.exit(f"{app} is not built")
+    w = configs.DENTAL512
+    if args.views:
+        w = w.scaled(n_views=args.views)
+    d = w.data()
+    scan, rec = w.scan_params(), w.recon_params()
+    filt = {"mfma": 1, "mfma6": 2, "mfma16": 4}[args.filter_form]
+    runs = []
+    with tempfile.TemporaryDirectory() as tmp:
+        paths = {}
+        for k, dt in (("raw", np.uint16), ("ptm", np.float64), ("dark", np.uint16), ("gain", np.float32)):
+            paths[k] = os.path.join(tmp, f"{k}.bin")
+            np.ascontiguousarray(d[k], dt).tofile(paths[k])
+        cmd = [app, f"views={w.n_views}", f"dso={w.sod}", f"dsd={w.sdd}", f"detu={w.n_u}", f"detv={w.n_v}",
+               f"pitch={w.du}", f"bright_field={scan.BrightField}",
+               "bh=" + "|".join(str(b) for b in scan.BeamHardeningParams), f"x={w.X}", f"y={w.Y}", f"z={w.Z}",
+               f"pitch_xy={w.pxy}", f"pitch_z={w.pz}", f"cut_start={rec.FilterCutOffStart}",
+               f"cut_end={rec.FilterCutOffEnd}", f"window={rec.FilterWindow}", f"filter={filt}", "storage=1",
+               f"chunk={args.chunk}", f"out={os.path.join(tmp, 'vol.bin')}"] + [f"{k}={v}" for k, v in paths.items()]
+        for r in range(args.warmup + args.steps):
+            out = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+            if out.returncode != 0:
+                sys.exit(out.stderr[-2000:])
+            info = json.loads(out.stdout.strip().splitlines()[-1])
+            log(f"run {r}: {info['seconds']:.3f} s, {w.gups_units / info['seconds'] / 1e9:.0f} GUPS",
+                "(warm-up)" if r < args.warmup else "")
+            if r >= args.warmup:
+                runs.append(info["seconds"])
+        agents = [l for l in out.stderr.splitlines() if "occupied" in l]
+    sec = float(np.median(runs))
+    line = {
+        "metric": "FDK GUPS at 512^3 x 600 views through the agent graph, host buffers at both ends",
+        "value": w.gups_units / sec / 1e9, "unit": "GUPS", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": sec * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16-storage/f32-accumulate",
+        "data": "synthetic (seeded ellipsoid phantom, analytic line integrals, Poisson counts)",
+        "config": {"workload": f"pipeline_dental512: jh_fdk_pipeline FDK {w.X}x{w.Y}x{w.Z} from {w.n_views} views "
+                               f"of {w.n_u}x{w.n_v} u16 in host memory, slices to host memory, {args.chunk} views "
+                               "per chunk", "parallelism": "single-gpu"},
+        "seconds_per_run": runs, "agents": agents,
     }
     print(json.dumps(line), flush=True)
 
