@@ -7,19 +7,21 @@ hardening, seeded 40-ellipsoid + tooth phantom, per-pixel gain and dark.
 
 One step = the whole reconstruction on the device with the raw counts already
 resident in HBM: preprocess -> row filter -> stage -> backprojection ->
-normalised volume.  Two forms are timed:
+normalised volume.  Three forms are timed:
   * fp32: FP32 Toeplitz filter on MFMA, FP32 projection storage;
-  * bf16: hi/mid/lo split-BF16 Toeplitz filter on MFMA (6 products, FP32
-    accumulation), BF16 projection storage (headline);
+  * bf16: row-scaled hi/lo FP16 Toeplitz filter on MFMA (3 products, FP32
+    accumulation; --filter-form selects the split-BF16 forms), BF16
+    projection storage (headline);
   * f16: the same filter, FP16 projection storage.
 value = voxel-updates per second of the headline form over the whole step
 (X*Y*Z*N / step time), in GUPS.
 
 --workload batch16 times the batched form instead: 16 scans of uneven sizes
 (volumes 128^3..400^3, 180..720 views, detectors 256^2..1024^2) reconstructed
-by one hipGraph launch (BF16 storage, 6-product filter), tiles and view chunks
-from the batch autotuner; value = aggregate GUPS of one launch, with per-scan
-rel-RMSE against the FP64 oracle on the centre slices.
+by one hipGraph launch (FP16 storage with its range exponent, the headline
+filter form), tiles and view chunks from the batch autotuner; value =
+aggregate GUPS of one launch, with per-scan rel-RMSE against the FP64 oracle
+on the centre slices.
 
 --workload adjoint256 times the differentiation form: the forward projector
 A and the un-normalised backprojector A^T on 256^3 from 360 views of 512^2,
@@ -39,7 +41,7 @@ kernel's own schedule and the counters committed under profiles/ (rocprofv3
     FP32 vector rate (157.3 TFLOP/s = 1.23e12 wave instructions/s);
   * vector-memory issue: the bytes each update requests through the texture
     path at 64 B per clock per CU (a 64-lane 16-byte gather issues in 16
-    clocks, measured by jenhine_amd/gather_rate);
+    clocks), or from LDS at 128 B per clock for the LDS-staged fetch;
 frac = achieved GUPS / the lowest roof.  The reference's per-view formulation
 (an FP32 accumulator read + write per update, 8 B) is kept as a separate
 north-star field.  cpu_baseline times the CPU oracle (the reference
