@@ -188,8 +188,8 @@ def main():
             fdk.bp.run(q, fdk.storage, out=vol)
         torch.cuda.synchronize()
         ms = (time.perf_counter() - t) / args.steps * 1e3
-        log(f"bp_kernel {fdk.storage}: {ms:.2f} ms/launch (tile {fdk.bp.tile()}, {fdk.bp.fetch()}), "
-            f"copy kernel moved {2 * src.numel() * 4} bytes")
+        log(f"bp_kernel {fdk.storage}: {ms:.2f} ms/launch (tile {fdk.bp.tile()}, {fdk.bp.fetch()} set, "
+            f"{fdk.bp.fetch_in_use(fdk.storage)} in use), copy kernel moved {2 * src.numel() * 4} bytes")
         print(json.dumps({"kernel_only": True, "bp_ms": ms, "copy_bytes": 2 * src.numel() * 4}))
         return
     units = w.gups_units
