@@ -111,12 +111,23 @@ def main():
     ap.add_argument("--counters-out", default=None,
                     help="with --summarize-counters: merge the summary into this JSON file (e.g. "
                          "profiles/bp_dental512_counters.json)")
+    ap.add_argument("--pipeline-args", metavar="DIR", default=None,
+                    help="no GPU work: write the pipeline_dental512 input files to DIR and print the "
+                         "jh_fdk_pipeline argument list (for running the app under a profiler)")
+    ap.add_argument("--summarize-timeline", metavar="DIR", default=None,
+                    help="no GPU work: read a rocprofv3 --runtime-trace CSV output under DIR (one jh_fdk_pipeline "
+                         "run) and print its kernel, copy and API summary and a 5 ms timeline")
     ap.add_argument("--batch-storage", choices=("bf16", "f16", "f32"), default="f16",
                     help="projection storage of the batch16 scans (FP16 with its range exponent: BF16 storage "
                          "misses 1e-3 on noise-dominated Ram-Lak scans)")
     args = ap.parse_args()
     if args.summarize_counters:
         return summarize_counters(args.summarize_counters, args.tile, args.fetch, args.form, args.counters_out)
+    if args.pipeline_args:
+        print(" ".join(pipeline_command(args, args.pipeline_args)[1:]))
+        return
+    if args.summarize_timeline:
+        return summarize_timeline(args.summarize_timeline)
     if args.gpus != 1:
         print(json.dumps({"error": "the benchmark measures one GPU (the path shards by z-slabs, one process "
                                    "per GPU, not built here); --gpus must be 1",
@@ -569,13 +580,9 @@ def run_adjoint(args, dev):
     print(json.dumps(line), flush=True)
 
 
-def run_pipeline(args):
-    """dental512 through the C++ agent graph (jh_fdk_pipeline): raw counts in
-    host memory streamed to the device per view chunk, slices downloaded to
-    host memory, BF16 storage and the headline filter form; value = GUPS of
-    the graph's run (Start to Join), PCIe transfers included."""
-    import subprocess
-    import tempfile
+def pipeline_command(args, tmp):
+    """dental512 inputs written to tmp and the jh_fdk_pipeline command line
+    (BF16 storage, the headline filter form, args.chunk views per chunk)."""
     from jenhine_amd import configs
 
     app = os.path.join(ROOT, "jenhine_amd", "host", "build", "jh_fdk_pipeline")
@@ -587,18 +594,34 @@ def run_pipeline(args):
     d = w.data()
     scan, rec = w.scan_params(), w.recon_params()
     filt = {"mfma": 1, "mfma6": 2, "mfma16": 4}[args.filter_form]
+    os.makedirs(tmp, exist_ok=True)
+    paths = {}
+    for k, dt in (("raw", np.uint16), ("ptm", np.float64), ("dark", np.uint16), ("gain", np.float32)):
+        paths[k] = os.path.join(tmp, f"{k}.bin")
+        np.ascontiguousarray(d[k], dt).tofile(paths[k])
+    return [app, f"views={w.n_views}", f"dso={w.sod}", f"dsd={w.sdd}", f"detu={w.n_u}", f"detv={w.n_v}",
+            f"pitch={w.du}", f"bright_field={scan.BrightField}",
+            "bh=" + "|".join(str(b) for b in scan.BeamHardeningParams), f"x={w.X}", f"y={w.Y}", f"z={w.Z}",
+            f"pitch_xy={w.pxy}", f"pitch_z={w.pz}", f"cut_start={rec.FilterCutOffStart}",
+            f"cut_end={rec.FilterCutOffEnd}", f"window={rec.FilterWindow}", f"filter={filt}", "storage=1",
+            f"chunk={args.chunk}", f"out={os.path.join(tmp, 'vol.bin')}"] + [f"{k}={v}" for k, v in paths.items()]
+
+
+def run_pipeline(args):
+    """dental512 through the C++ agent graph (jh_fdk_pipeline): raw counts in
+    host memory streamed to the device per view chunk, slices downloaded to
+    host memory, BF16 storage and the headline filter form; value = GUPS of
+    the graph's run (Start to Join), PCIe transfers included."""
+    import subprocess
+    import tempfile
+    from jenhine_amd import configs
+
+    w = configs.DENTAL512
+    if args.views:
+        w = w.scaled(n_views=args.views)
     runs = []
     with tempfile.TemporaryDirectory() as tmp:
-        paths = {}
-        for k, dt in (("raw", np.uint16), ("ptm", np.float64), ("dark", np.uint16), ("gain", np.float32)):
-            paths[k] = os.path.join(tmp, f"{k}.bin")
-            np.ascontiguousarray(d[k], dt).tofile(paths[k])
-        cmd = [app, f"views={w.n_views}", f"dso={w.sod}", f"dsd={w.sdd}", f"detu={w.n_u}", f"detv={w.n_v}",
-               f"pitch={w.du}", f"bright_field={scan.BrightField}",
-               "bh=" + "|".join(str(b) for b in scan.BeamHardeningParams), f"x={w.X}", f"y={w.Y}", f"z={w.Z}",
-               f"pitch_xy={w.pxy}", f"pitch_z={w.pz}", f"cut_start={rec.FilterCutOffStart}",
-               f"cut_end={rec.FilterCutOffEnd}", f"window={rec.FilterWindow}", f"filter={filt}", "storage=1",
-               f"chunk={args.chunk}", f"out={os.path.join(tmp, 'vol.bin')}"] + [f"{k}={v}" for k, v in paths.items()]
+        cmd = pipeline_command(args, tmp)
         for r in range(args.warmup + args.steps):
             out = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
             if out.returncode != 0:
@@ -622,6 +645,67 @@ def run_pipeline(args):
         "seconds_per_run": runs, "agents": agents,
     }
     print(json.dumps(line), flush=True)
+
+
+def summarize_timeline(d):
+    """Kernel, copy and HIP API summary of one jh_fdk_pipeline run traced with
+    rocprofv3 --runtime-trace --output-format csv: busy time and span of every
+    kernel and copy direction from the first preprocessing kernel, the copies
+    and kernels per 5 ms, and the HIP calls that took over 1 ms."""
+    import csv
+    import glob
+
+    def rows(pattern):
+        files = sorted(glob.glob(os.path.join(d, "**", pattern), recursive=True))
+        if not files:
+            sys.exit(f"no {pattern} under {d}")
+        return list(csv.DictReader(open(files[0])))
+
+    ev = [(int(k["Start_Timestamp"]), int(k["End_Timestamp"]), "kernel " + k["Kernel_Name"].split("(")[0])
+          for k in rows("*kernel_trace.csv")]
+    ev += [(int(m["Start_Timestamp"]), int(m["End_Timestamp"]), "copy " + m["Direction"].replace("MEMORY_COPY_", ""))
+           for m in rows("*memory_copy_trace.csv")]
+    starts = [e[0] for e in ev if "preprocess" in e[2]]
+    if not starts:
+        sys.exit("no preprocessing kernel in the trace")
+    t0, t1 = min(starts), max(e[1] for e in ev)
+    print(f"from the first preprocessing kernel to the last event: {(t1 - t0) / 1e6:.2f} ms")
+    agg = {}
+    for s, e, n in ev:
+        a = agg.setdefault(n, [0, 0.0, s, e])
+        a[0] += 1
+        a[1] += (e - s) / 1e6
+        a[2], a[3] = min(a[2], s), max(a[3], e)
+    print("| event | count | busy ms | first start ms | last end ms |")
+    print("|---|---|---|---|---|")
+    for n, (c, b, s, e) in sorted(agg.items(), key=lambda x: x[1][2]):
+        print(f"| `{n}` | {c} | {b:.2f} | {(s - t0) / 1e6:.2f} | {(e - t0) / 1e6:.2f} |")
+    step = 5e6
+    nb = int((t1 - t0) / step) + 1
+    bins = {"kernel": [0.0] * nb, "copy HOST_TO_DEVICE": [0.0] * nb, "copy DEVICE_TO_HOST": [0.0] * nb}
+    for s, e, n in ev:
+        key = "kernel" if n.startswith("kernel") else n
+        if key not in bins or e <= t0:
+            continue
+        x = max(s, t0)
+        while x < e:
+            i = int((x - t0) / step)
+            y = min(e, t0 + (i + 1) * step)
+            bins[key][i] += (y - x) / 1e6
+            x = y
+    print()
+    print("| ms from the first preprocessing kernel | kernel busy ms (all streams) | H2D busy ms | D2H busy ms |")
+    print("|---|---|---|---|")
+    for i in range(nb):
+        print(f"| {i * 5}–{i * 5 + 5} | {bins['kernel'][i]:.2f} | {bins['copy HOST_TO_DEVICE'][i]:.2f} | "
+              f"{bins['copy DEVICE_TO_HOST'][i]:.2f} |")
+    api = rows("*hip_api_trace.csv")
+    slow = [(int(a["Start_Timestamp"]), int(a["End_Timestamp"]), a["Function"], a["Thread_Id"]) for a in api
+            if int(a["End_Timestamp"]) - int(a["Start_Timestamp"]) > 1_000_000]
+    print()
+    print("HIP calls over 1 ms (ms from the first preprocessing kernel, thread):")
+    for s, e, f, t in sorted(slow):
+        print(f"  {(s - t0) / 1e6:8.2f} .. {(e - t0) / 1e6:8.2f}  {f}  thread {t}")
 
 
 def run_offset(args, dev):

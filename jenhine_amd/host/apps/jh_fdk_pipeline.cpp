@@ -287,12 +287,13 @@ int main(int argc, char** argv)
                                                               a.S("input_folder"), a.S("input_template"), composite));
             else
                 agents.push_back(std::make_shared<MemoryInputAgent>(raw, frame, scan.NumViews, parts, composite));
-            agents.push_back(std::make_shared<ProjectionUploadAgent>(scan.NumDetsU, scan.NumDetsV, recon.ViewsPerChunk));
+            agents.push_back(
+                std::make_shared<ProjectionUploadAgent>(scan.NumDetsU, scan.NumDetsV, recon.ViewsPerChunk, device));
             agents.push_back(std::make_shared<PreProcessingHIPAgent>(scan, dark, gain, device, recon.ViewsPerChunk));
             if (composite) agents.push_back(std::make_shared<CompositeHIPAgent>(scan, ptm, detectorOnRight, device));
             agents.push_back(std::make_shared<FilterHIPAgent>(scan, recon, ptm, device, composite, detectorOnRight));
             agents.push_back(bp);
-            agents.push_back(std::make_shared<DeviceMemoryDownloadAgent>(plane));
+            agents.push_back(std::make_shared<DeviceMemoryDownloadAgent>(plane, device));
             if (toFolder)
                 agents.push_back(std::make_shared<OutputAgent>(recon.NumPixelsX, recon.NumPixelsY,
                                                                a.S("output_folder"), a.S("output_template")));

@@ -191,7 +191,8 @@ struct PipeElementIO<hip::DeviceMemory<U>, void> {
     }
 };
 
-// Per-agent HIP stream, created on first use on the calling thread.
+// Per-agent HIP stream, created by the agent's GetReady (set-up time, on the
+// main thread, with the agent's device current) or else on first use.
 class StreamHolder {
 public:
     ~StreamHolder();
@@ -208,7 +209,7 @@ private:
 // the device DMA runs from / to pinned memory.
 constexpr int kCopyThreads = 4;
 
-// A pinned host buffer, allocated on first use.
+// A pinned host buffer, allocated at set-up or on first use.
 class PinnedBuffer {
 public:
     PinnedBuffer() = default;
@@ -223,10 +224,13 @@ private:
 };
 
 // Copies the raw frames up; viewsPerChunk aligns the device ring to the
-// chunks its consumer reads.
+// chunks its consumer reads.  GetReady creates the copy threads' streams and
+// pinned buffers and moves one frame through each stream, so that the HIP
+// runtime's one-time set-up of the copy path is not paid by the first frames.
 class ProjectionUploadAgent : public SequentialAgentBase {
 public:
-    ProjectionUploadAgent(size_t width, size_t height, size_t viewsPerChunk = 1);
+    ProjectionUploadAgent(size_t width, size_t height, size_t viewsPerChunk = 1, int device = 0);
+    void GetReady() override;
 
 private:
     void SetPipesImpl() override;
@@ -236,6 +240,7 @@ private:
     void WorkFlow3() override { Copy(3); }
     void Copy(int thread);
     const size_t width_, height_, chunk_;
+    const int device_;
     std::shared_ptr<Pipe<UINT16Vec>> in_;
     std::shared_ptr<Pipe<hip::DeviceMemory<uint16_t>>> out_;
     std::mutex tokens_;
@@ -344,9 +349,12 @@ private:
     StreamHolder stream_;
 };
 
+// Copies the slices down; GetReady sets up its streams and pinned buffers as
+// ProjectionUploadAgent's does.
 class DeviceMemoryDownloadAgent : public SequentialAgentBase {
 public:
-    explicit DeviceMemoryDownloadAgent(size_t elements);
+    explicit DeviceMemoryDownloadAgent(size_t elements, int device = 0);
+    void GetReady() override;
 
 private:
     void SetPipesImpl() override;
@@ -356,6 +364,7 @@ private:
     void WorkFlow3() override { Copy(3); }
     void Copy(int thread);
     const size_t n_;
+    const int device_;
     std::shared_ptr<Pipe<hip::DeviceMemory<float>>> in_;
     std::shared_ptr<Pipe<FloatVec>> out_;
     std::mutex tokens_;

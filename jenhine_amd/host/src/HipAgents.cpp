@@ -134,10 +134,25 @@ void* PinnedBuffer::Get(size_t bytes)
 // ProjectionUploadAgent
 // ---------------------------------------------------------------------------
 
-ProjectionUploadAgent::ProjectionUploadAgent(size_t width, size_t height, size_t viewsPerChunk)
+ProjectionUploadAgent::ProjectionUploadAgent(size_t width, size_t height, size_t viewsPerChunk, int device)
     : SequentialAgentBase("ProjectionUploadAgent", kCopyThreads), width_(width), height_(height),
-      chunk_(std::max<size_t>(1, viewsPerChunk))
+      chunk_(std::max<size_t>(1, viewsPerChunk)), device_(device)
 {
+}
+
+// Streams and pinned buffers of the copy threads, and one frame through each
+// stream: stream creation and the first transfers take milliseconds each.
+void ProjectionUploadAgent::GetReady()
+{
+    CheckHip(hipSetDevice(device_), "hipSetDevice");
+    const size_t bytes = width_ * height_ * sizeof(uint16_t);
+    hip::DeviceMemory<uint16_t> frame(width_ * height_);
+    for (int t = 0; t < kCopyThreads; ++t) {
+        void* stage = pinned_[t].Get(bytes);
+        std::memset(stage, 0, bytes);
+        CheckHip(hipMemcpyAsync(frame.Data(), stage, bytes, hipMemcpyHostToDevice, stream_[t].Get()), "upload set-up");
+    }
+    for (int t = 0; t < kCopyThreads; ++t) stream_[t].Sync();
 }
 
 void ProjectionUploadAgent::SetPipesImpl()
@@ -155,6 +170,7 @@ void ProjectionUploadAgent::SetPipesImpl()
 // matching write token together, so frames keep their stream order.
 void ProjectionUploadAgent::Copy(int thread)
 {
+    CheckHip(hipSetDevice(device_), "hipSetDevice");
     try {
         for (;;) {
             Pipe<UINT16Vec>::ReadToken rt;
@@ -227,6 +243,7 @@ void PreProcessingHIPAgent::GetReady()
         gain_ = hip::DeviceMemory<float>(gainHost_.size());
         CheckHip(hipMemcpy(gain_.Data(), gainHost_.data(), gain_.Bytes(), hipMemcpyHostToDevice), "gain upload");
     }
+    stream_.Get();
 }
 
 void PreProcessingHIPAgent::WorkFlow0()
@@ -290,6 +307,7 @@ void CompositeHIPAgent::GetReady()
     CheckJh(jh_composite_create(&plan_, device_, (int)scan_.NumUsedDetsU, (int)scan_.NumUsedDetsV,
                                 (int)(ptms_.size() / 12), ptms_.data(), scan_.DSO, right_ ? 1 : 0),
             "jh_composite_create");
+    stream_.Get();
 }
 
 void CompositeHIPAgent::WorkFlow0()
@@ -357,6 +375,7 @@ void FilterHIPAgent::GetReady()
                                 nAdj ? recon_.FilterAdjustLevelInDB.data() : nullptr, nAdj, recon_.FilterWindow,
                                 recon_.FilterMode),
             "jh_filter_create_ex");
+    stream_.Get();
 }
 
 void FilterHIPAgent::WorkFlow0()
@@ -555,9 +574,22 @@ void BPHIPAgent::WorkFlow0()
 // DeviceMemoryDownloadAgent
 // ---------------------------------------------------------------------------
 
-DeviceMemoryDownloadAgent::DeviceMemoryDownloadAgent(size_t elements)
-    : SequentialAgentBase("DeviceMemoryDownloadAgent", kCopyThreads), n_(elements)
+DeviceMemoryDownloadAgent::DeviceMemoryDownloadAgent(size_t elements, int device)
+    : SequentialAgentBase("DeviceMemoryDownloadAgent", kCopyThreads), n_(elements), device_(device)
 {
+}
+
+// As ProjectionUploadAgent::GetReady, one slice down through each stream.
+void DeviceMemoryDownloadAgent::GetReady()
+{
+    CheckHip(hipSetDevice(device_), "hipSetDevice");
+    const size_t bytes = n_ * sizeof(float);
+    hip::DeviceMemory<float> slice(n_);
+    CheckHip(hipMemset(slice.Data(), 0, bytes), "download set-up");
+    for (int t = 0; t < kCopyThreads; ++t)
+        CheckHip(hipMemcpyAsync(pinned_[t].Get(bytes), slice.Data(), bytes, hipMemcpyDeviceToHost, stream_[t].Get()),
+                 "download set-up");
+    for (int t = 0; t < kCopyThreads; ++t) stream_[t].Sync();
 }
 
 void DeviceMemoryDownloadAgent::SetPipesImpl()
@@ -574,6 +606,7 @@ void DeviceMemoryDownloadAgent::SetPipesImpl()
 // together, so slices keep their order.
 void DeviceMemoryDownloadAgent::Copy(int thread)
 {
+    CheckHip(hipSetDevice(device_), "hipSetDevice");
     try {
         for (;;) {
             Pipe<hip::DeviceMemory<float>>::ReadToken rt;
