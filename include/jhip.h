@@ -295,8 +295,12 @@ jh_status jh_bp_run(jh_bp_plan* plan, const void* d_q, int dtype, float* d_vol, 
 jh_status jh_bp_reset(jh_bp_plan* plan, jh_stream_t stream);
 
 /* Accumulate views view0..view0+nviews-1 (d_q holds exactly those views) into
- * the plan's accumulator and count volumes.  Replaces BPCore::ProcessSlice
- * (BPCore.cpp:157-196) and BPCUDACore::DeployBackProj (BPCUDACore.cu:89-114). */
+ * the plan's accumulator and count volumes.  Each launch continues the running
+ * sums of the earlier ones in view order, so the finished volume does not
+ * depend on how the views were split into calls: it equals jh_bp_run's to
+ * the bit when the views are accumulated in order.  Replaces
+ * BPCore::ProcessSlice (BPCore.cpp:157-196) and BPCUDACore::DeployBackProj
+ * (BPCUDACore.cu:89-114). */
 jh_status jh_bp_accumulate(jh_bp_plan* plan, const void* d_q, int dtype, int view0, int nviews,
                            jh_stream_t stream);
 
@@ -332,14 +336,16 @@ jh_status jh_fp_project(jh_bp_plan* plan, const float* d_vol, float* d_proj, int
  * jh_bp_reset / jh_bp_accumulate / jh_bp_finalize in one entry point whose
  * accumulator and count volumes live in a static device array of the
  * library (two volumes of at most 8 Mi = 8388608 voxels, e.g. 256 x 256 x
- * 128; 64 MiB held by every device the library runs on) instead of the
- * plan's buffers; it calls no allocating routine.  flags:
+ * 128, and the columns' count weights, at most 2 Mi = 2097152 columns; 72 MiB
+ * held by every device the library runs on) instead of the plan's buffers; it
+ * calls no allocating routine.  flags:
  *   JH_NOALLOC_FIRST  clear the static accumulators, then accumulate;
  *   JH_NOALLOC_LAST   after accumulating, write the finished volume (as
  *                     jh_bp_finalize, Z x Y x X f32) to d_vol.
  * views view0 .. view0+nviews-1 of d_q (staged as for jh_bp_accumulate) are
  * accumulated; nviews may be 0.  Calls between FIRST and LAST continue one
- * sequence and must keep the volume size, weighting and storage scale.
+ * sequence and must keep the volume size, weighting, geometry class and
+ * storage scale.
  * Every call on a device shares that device's array and the array keeps the
  * last call's contents: sequences that overlap on one device (from several
  * threads or streams) race and give wrong volumes.  A volume above the limit

@@ -618,6 +618,19 @@ __device__ __forceinline__ void bp_column_pairs(float* acc, const F& fetch, uint
     }
 }
 
+// A view's geometry through the constant address space: scalar loads, which
+// the compiler otherwise makes only where it can prove that no store of the
+// kernel reaches the array.  A vector load in the view loop would make each
+// view wait for every gather or copy in flight.
+__device__ __forceinline__ Geo geo_of(const Geo* geo, int view)
+{
+    const __attribute__((address_space(4))) float* p = (const __attribute__((address_space(4))) float*)(geo + view);
+    Geo g;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) g.c[k] = p[k];
+    return g;
+}
+
 // One tile: 64 blockDim.y voxel columns (8 x 8 per wave), ZC voxels deep,
 // all views of the launch.
 template <int MODE, int ZC, int QT, int OUT>
@@ -663,9 +676,36 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
 #pragma unroll
         for (int kk = 0; kk < ZC; ++kk) cnt_part[kk * 256 + tid] = 0.0f;
     }
+    // The streaming forms continue the running sums of the earlier launches
+    // (loaded here, stored at the end), so every voxel's additions run in
+    // view order whatever the launch boundaries: the result equals one launch
+    // over all views to the bit.  The count term kept per voxel is the
+    // samples' count in geometry classes 1 and 2, and the weight of the
+    // samples outside the detector rows in class 0 (whose count is the
+    // column's running count weight less that).
+    if constexpr (OUT == kOutAccumulate || OUT == kOutFdkAccumulate) {
+        // every lane loads (clamped into the volume) and selects: a branch per
+        // sample here made the compiler hold ~18 more registers through the
+        // view loop and keep fewer gathers in flight
+        const size_t plane = (size_t)a.X * a.Y;
+        const bool col = i < a.X && j < a.Y;
+        const size_t base = ((size_t)k0 * a.Y + min(j, a.Y - 1)) * a.X + min(i, a.X - 1);
+#pragma unroll
+        for (int kk = 0; kk < ZC; ++kk) {
+            const bool ok = col && k0 + kk < a.Z;
+            const size_t idx = base + (size_t)min(kk, a.Z - 1 - k0) * plane;
+            const float x = a.acc[idx];
+            acc[kk] = ok ? x : 0.0f;
+            if constexpr (!WT) {
+                const float c = a.cnt[idx];
+                if constexpr (MODE == 0) cnt_part[kk * 256 + tid] = ok ? c : 0.0f;
+                else cnt[kk] = ok ? c : 0.0f;
+            }
+        }
+    }
 
     for (int v = 0; v < a.nviews; ++v) {
-        const Geo g = a.geo[a.view0 + v];
+        const Geo g = geo_of(a.geo, a.view0 + v);
         const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
             const_cast<char*>(reinterpret_cast<const char*>(a.q)) + (size_t)v * a.frame_bytes, (short)0,
             (int)a.frame_bytes, kBufferFlags);
@@ -814,12 +854,10 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
     }
 
     if (i >= a.X || j >= a.Y) return;
-    if constexpr (!WT) {
-        if constexpr (MODE == 0) {
-            const float c = a.colcnt[(size_t)j * a.X + i];
+    if constexpr (MODE == 0 && OUT == kOutNormalised) {
+        const float c = a.colcnt[(size_t)j * a.X + i];
 #pragma unroll
-            for (int kk = 0; kk < ZC; ++kk) cnt[kk] = c - cnt_part[kk * 256 + tid];
-        }
+        for (int kk = 0; kk < ZC; ++kk) cnt[kk] = c - cnt_part[kk * 256 + tid];
     }
     const size_t plane = (size_t)a.X * a.Y;
     size_t idx = ((size_t)k0 * a.Y + j) * a.X + i;
@@ -830,10 +868,11 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
                 const float v = (OUT == kOutFdk ? a.wscale * acc[kk] : acc[kk] / cnt[kk]) * a.qscale;
                 a.vol[idx] = a.ct ? ct_number(v, a.ct_scale, a.ct_norm1) : v;
             } else if constexpr (OUT == kOutAccumulate) {
-                a.acc[idx] += acc[kk];
-                a.cnt[idx] += cnt[kk];
+                a.acc[idx] = acc[kk];
+                if constexpr (MODE == 0) a.cnt[idx] = cnt_part[kk * 256 + tid];
+                else a.cnt[idx] = cnt[kk];
             } else if constexpr (OUT == kOutFdkAccumulate) {
-                a.acc[idx] += acc[kk];
+                a.acc[idx] = acc[kk];
             } else {
                 a.vol[idx] = acc[kk];
             }
@@ -927,7 +966,7 @@ __device__ __forceinline__ void bp_tile_lds(const BPArgs& a, unsigned tile)
     // division: the one-pixel margin absorbs the difference).
     auto footprint = [&](int vv, int slot) {
         if (threadIdx.y != 0) return;
-        const Geo g = a.geo[a.view0 + vv];
+        const Geo g = geo_of(a.geo, a.view0 + vv);
         const int c = threadIdx.x & 3;
         const float cx = (c & 1) ? cx1 : cx0, cy = (c & 2) ? cy1 : cy0;
         const float U = fmaf(g.c[3], fk0, fmaf(g.c[1], cx, fmaf(g.c[2], cy, g.c[0])));
@@ -1011,10 +1050,26 @@ __device__ __forceinline__ void bp_tile_lds(const BPArgs& a, unsigned tile)
         }
     };
 
-    float acc[ZC], cnt[ZC];
+    // part: the count weight of the samples outside the detector rows (the
+    // count is the column's count weight from bp_colcount_kernel less it, as
+    // in the gather form); the streaming form continues the running sums
+    float acc[ZC], part[ZC];
 #pragma unroll
-    for (int kk = 0; kk < ZC; ++kk) { acc[kk] = 0.0f; cnt[kk] = 0.0f; }
-    float cfull = 0.0f;
+    for (int kk = 0; kk < ZC; ++kk) { acc[kk] = 0.0f; part[kk] = 0.0f; }
+    if constexpr (OUT == kOutAccumulate) {
+        // clamped loads and selects (see bp_tile)
+        const size_t plane = (size_t)a.X * a.Y;
+        const bool col = i < a.X && j < a.Y;
+        const size_t base = ((size_t)k0 * a.Y + min(j, a.Y - 1)) * a.X + min(i, a.X - 1);
+#pragma unroll
+        for (int kk = 0; kk < ZC; ++kk) {
+            const bool ok = col && k0 + kk < a.Z;
+            const size_t idx = base + (size_t)min(kk, a.Z - 1 - k0) * plane;
+            const float x = a.acc[idx], c = a.cnt[idx];
+            acc[kk] = ok ? x : 0.0f;
+            part[kk] = ok ? c : 0.0f;
+        }
+    }
 
     if (n > 0) {
         footprint(0, 0);
@@ -1029,7 +1084,7 @@ __device__ __forceinline__ void bp_tile_lds(const BPArgs& a, unsigned tile)
         const int slot = v % 3, buf = v & 1;
         if (v + 2 < n) footprint(v + 2, (v + 2) % 3);
 
-        const Geo g = a.geo[a.view0 + v];
+        const Geo g = geo_of(a.geo, a.view0 + v);
         const float Uc = fmaf(g.c[3], fk0, fmaf(g.c[1], fi, fmaf(g.c[2], fj, g.c[0])));
         const float Vc = fmaf(g.c[7], fk0, fmaf(g.c[5], fi, fmaf(g.c[6], fj, g.c[4])));
         const float Wc = fmaf(g.c[11], fk0, fmaf(g.c[9], fi, fmaf(g.c[10], fj, g.c[8])));
@@ -1053,7 +1108,6 @@ __device__ __forceinline__ void bp_tile_lds(const BPArgs& a, unsigned tile)
                 const uint32_t lb = (uint32_t)(iu - u0) * 16u - (uint32_t)v0 * lrow;
                 const char* lbase = reinterpret_cast<const char*>(lds_dyn + buf * cap);
                 if (all_in) {
-                    cfull += cw;
                     // kZB reads in flight per wave before any is consumed: at
                     // the occupancy the LDS budget allows, LDS latency is
                     // otherwise exposed
@@ -1084,7 +1138,7 @@ __device__ __forceinline__ void bp_tile_lds(const BPArgs& a, unsigned tile)
                             *reinterpret_cast<const float4*>(lbase + __umul24((uint32_t)(int)t, lrow) + lb);
                         const float sum = lds_accum<QT>(acc[k], c4, fu, fv);
                         acc[k] = in ? sum : acc[k];
-                        cnt[k] += in ? cw : 0.0f;
+                        part[k] += in ? 0.0f : cw;
                     }
                 }
             } else {
@@ -1113,7 +1167,7 @@ __device__ __forceinline__ void bp_tile_lds(const BPArgs& a, unsigned tile)
                     for (int q = 0; q < kZB; ++q) {
                         const float sum = S::accum(acc[kb + q], p[q], fv[q], wq);
                         acc[kb + q] = in[q] ? sum : acc[kb + q];
-                        cnt[kb + q] += in[q] ? cw : 0.0f;
+                        part[kb + q] += in[q] ? 0.0f : cw;
                     }
                 }
             }
@@ -1127,19 +1181,18 @@ __device__ __forceinline__ void bp_tile_lds(const BPArgs& a, unsigned tile)
     }
 
     if (i >= a.X || j >= a.Y) return;
-#pragma unroll
-    for (int kk = 0; kk < ZC; ++kk) cnt[kk] += cfull;
+    const float c = OUT == kOutNormalised ? a.colcnt[(size_t)j * a.X + i] : 0.0f;
     const size_t plane = (size_t)a.X * a.Y;
     size_t idx = ((size_t)k0 * a.Y + j) * a.X + i;
 #pragma unroll
     for (int kk = 0; kk < ZC; ++kk) {
         if (k0 + kk < a.Z) {
             if (OUT == kOutNormalised) {
-                const float v = acc[kk] / cnt[kk] * a.qscale;
+                const float v = acc[kk] / (c - part[kk]) * a.qscale;
                 a.vol[idx] = a.ct ? ct_number(v, a.ct_scale, a.ct_norm1) : v;
             } else if (OUT == kOutAccumulate) {
-                a.acc[idx] += acc[kk];
-                a.cnt[idx] += cnt[kk];
+                a.acc[idx] = acc[kk];
+                a.cnt[idx] = part[kk];
             } else {
                 a.vol[idx] = acc[kk];
             }
@@ -1151,20 +1204,22 @@ __device__ __forceinline__ void bp_tile_lds(const BPArgs& a, unsigned tile)
 // Count weights of geometry class 0 per voxel column (u and w do not depend
 // on z): the sum over the launch's views of the interpolated uW at the
 // column's u, in view order and with the sampler's u weight -- the same
-// operations as the backprojector's own sample positions.  The gather form
-// takes off the samples that fall outside the detector rows; every other
-// sample of the column counts.  One thread per column, shared by all the
-// column's z tiles (so the backprojector loads no count weight per view).
+// operations as the backprojector's own sample positions.  The
+// backprojector takes off the samples that fall outside the detector rows;
+// every other sample of the column counts.  One thread per column, shared by
+// all the column's z tiles (so the backprojector loads no count weight per
+// view).  With cont the sum continues the column's running sum in colcnt
+// (the streaming form), else it starts from zero.
 template <int QT>
 __global__ void __launch_bounds__(256) bp_colcount_kernel(const Geo* __restrict__ geo, const float2* __restrict__ uwp,
                                                           int view0, int nviews, int X, int Y, int W,
-                                                          float* __restrict__ colcnt)
+                                                          float* __restrict__ colcnt, int cont)
 {
     const int i = blockIdx.x * 64 + threadIdx.x;
     const int j = blockIdx.y * 4 + threadIdx.y;
     if (i >= X || j >= Y) return;
     const float fi = (float)i, fj = (float)j, Wm1 = (float)(W - 1);
-    float c = 0.0f;
+    float c = cont ? colcnt[(size_t)j * X + i] : 0.0f;
     for (int v = 0; v < nviews; ++v) {
         const Geo g = geo[view0 + v];
         // as bp_tile with c[3] = c[11] = 0 (fmaf(0, k0, x) = x)
@@ -1290,18 +1345,6 @@ __global__ void strip_box_kernel(const Geo* __restrict__ geo, int N, int ntx, in
     }
 }
 
-// A view's geometry through the constant address space (scalar loads): the
-// sampling loop then waits for no vector-memory load, so the next views'
-// copies stay in flight.
-__device__ __forceinline__ Geo geo_of(const Geo* geo, int view)
-{
-    const __attribute__((address_space(4))) float* p = (const __attribute__((address_space(4))) float*)(geo + view);
-    Geo g;
-#pragma unroll
-    for (int k = 0; k < 12; ++k) g.c[k] = p[k];
-    return g;
-}
-
 // Copy instructions per thread and view: the rectangle holds at most
 // kStripRounds(ZC) x 256 chunks (dental-scale footprints of 16 x 16 columns:
 // about 19 chunks x (1.44 ZC + 7) rows).
@@ -1372,6 +1415,21 @@ __device__ __forceinline__ void bp_tile_strip_lds(const BPArgs& a, unsigned tile
     float acc[ZC], cpart[ZC];
 #pragma unroll
     for (int kk = 0; kk < ZC; ++kk) { acc[kk] = 0.0f; cpart[kk] = 0.0f; }
+    // the streaming form continues the running sums (clamped loads and
+    // selects, see bp_tile)
+    if constexpr (OUT == kOutAccumulate) {
+        const size_t plane = (size_t)a.X * a.Y;
+        const bool col = i < a.X && j < a.Y;
+        const size_t base = ((size_t)k0 * a.Y + min(j, a.Y - 1)) * a.X + min(i, a.X - 1);
+#pragma unroll
+        for (int kk = 0; kk < ZC; ++kk) {
+            const bool ok = col && k0 + kk < a.Z;
+            const size_t idx = base + (size_t)min(kk, a.Z - 1 - k0) * plane;
+            const float x = a.acc[idx], c = a.cnt[idx];
+            acc[kk] = ok ? x : 0.0f;
+            cpart[kk] = ok ? c : 0.0f;
+        }
+    }
 
     // sample view v from its buffer
     auto sample = [&](int v) {
@@ -1468,19 +1526,18 @@ __device__ __forceinline__ void bp_tile_strip_lds(const BPArgs& a, unsigned tile
     }
 
     if (i >= a.X || j >= a.Y) return;
-    const float c = a.colcnt[(size_t)j * a.X + i];
+    const float c = OUT == kOutNormalised ? a.colcnt[(size_t)j * a.X + i] : 0.0f;
     const size_t plane = (size_t)a.X * a.Y;
     size_t idx = ((size_t)k0 * a.Y + j) * a.X + i;
 #pragma unroll
     for (int kk = 0; kk < ZC; ++kk) {
         if (k0 + kk < a.Z) {
-            const float cnt = c - cpart[kk];
             if constexpr (OUT == kOutNormalised) {
-                const float v = acc[kk] / cnt * a.qscale;
+                const float v = acc[kk] / (c - cpart[kk]) * a.qscale;
                 a.vol[idx] = a.ct ? ct_number(v, a.ct_scale, a.ct_norm1) : v;
             } else {
-                a.acc[idx] += acc[kk];
-                a.cnt[idx] += cnt;
+                a.acc[idx] = acc[kk];
+                a.cnt[idx] = cpart[kk];
             }
         }
         idx += plane;
@@ -1529,12 +1586,19 @@ __global__ void ctnum_kernel(float* __restrict__ x, size_t n, float scale, float
         x[t] = ct_number(x[t], scale, norm1);
 }
 
+// vol = acc / count (or wscale * acc) of the streaming sums: the count is
+// cnt, or in geometry class 0 (colsum given) the column's running count
+// weight less cnt -- the operations of the one-launch epilogue.  acc, cnt
+// and vol start at a whole plane (so t % plane is the column).
 __global__ void bp_finalize_kernel(const float* __restrict__ acc, const float* __restrict__ cnt,
-                                   float* __restrict__ vol, size_t n, int ct, float ct_scale, float ct_norm1,
-                                   int weighted, float wscale, float qscale)
+                                   const float* __restrict__ colsum, size_t plane, float* __restrict__ vol,
+                                   size_t n, int ct, float ct_scale, float ct_norm1, int weighted, float wscale,
+                                   float qscale)
 {
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
-        const float v = (weighted ? wscale * acc[t] : acc[t] / cnt[t]) * qscale;
+        float v;
+        if (weighted) v = wscale * acc[t] * qscale;
+        else v = acc[t] / (colsum ? colsum[t % plane] - cnt[t] : cnt[t]) * qscale;
         vol[t] = ct ? ct_number(v, ct_scale, ct_norm1) : v;
     }
 }
@@ -1677,8 +1741,10 @@ static size_t stage_frame_bytes(const jh_bp_plan* p, int dtype)
     return (size_t)(p->W + (stage_strip(p, dtype) ? 1 : 0)) * p->V * stage_pixel_bytes(p, dtype);
 }
 
+// acc, cnt, colsum: the streaming sums (default: the plan's).
 static void launch_bp(jh_bp_plan* p, const void* q, int dtype, int view0, int nviews, float* vol,
-                      int out, hipStream_t stream, float* acc = nullptr, float* cnt = nullptr)
+                      int out, hipStream_t stream, float* acc = nullptr, float* cnt = nullptr,
+                      float* colsum = nullptr)
 {
     BPArgs a;
     a.q = q;
@@ -1736,16 +1802,19 @@ static void launch_bp(jh_bp_plan* p, const void* q, int dtype, int view0, int nv
     // (two rectangle buffers of kStripRounds(zc) x 256 chunks of 16 bytes)
     const size_t lds_bytes = strip_lds ? 2 * (size_t)kStripRounds(p->zc) * 256 * 16
                              : staged  ? 2 * (size_t)a.lds_px * sizeof(float4) : 0;
-    if (p->mode == 0 && !staged && (out == kOutNormalised || out == kOutAccumulate)) {
-        // the count weights of the launch's views per voxel column
+    if (p->mode == 0 && (out == kOutNormalised || out == kOutAccumulate)) {
+        // the count weights of the launch's views per voxel column; the
+        // streaming form continues the running sums
+        float* cc = out == kOutAccumulate ? (colsum ? colsum : p->colsum.as<float>()) : p->colcnt.as<float>();
+        const int cont = out == kOutAccumulate ? 1 : 0;
         const dim3 cg((unsigned)((p->X + 63) / 64), (unsigned)((p->Y + 3) / 4));
         const dim3 cb(64, 4);
         if (dtype == JH_F32)
-            bp_colcount_kernel<JH_F32><<<cg, cb, 0, stream>>>(a.geo, a.uwp, view0, nviews, p->X, p->Y, p->W, p->colcnt.as<float>());
+            bp_colcount_kernel<JH_F32><<<cg, cb, 0, stream>>>(a.geo, a.uwp, view0, nviews, p->X, p->Y, p->W, cc, cont);
         else if (dtype == JH_BF16)
-            bp_colcount_kernel<JH_BF16><<<cg, cb, 0, stream>>>(a.geo, a.uwp, view0, nviews, p->X, p->Y, p->W, p->colcnt.as<float>());
+            bp_colcount_kernel<JH_BF16><<<cg, cb, 0, stream>>>(a.geo, a.uwp, view0, nviews, p->X, p->Y, p->W, cc, cont);
         else
-            bp_colcount_kernel<JH_F16><<<cg, cb, 0, stream>>>(a.geo, a.uwp, view0, nviews, p->X, p->Y, p->W, p->colcnt.as<float>());
+            bp_colcount_kernel<JH_F16><<<cg, cb, 0, stream>>>(a.geo, a.uwp, view0, nviews, p->X, p->Y, p->W, cc, cont);
         check_launch("bp_colcount_kernel");
     }
     // (the strip kernel's workgroups carry one more wave, its copy wave)
@@ -1923,6 +1992,7 @@ static void ensure_accumulators(jh_bp_plan* p)
     const size_t n = (size_t)p->X * p->Y * p->Z;
     if (!p->acc.get()) p->acc.alloc(sizeof(float) * n);
     if (!p->cnt.get()) p->cnt.alloc(sizeof(float) * n);
+    if (p->mode == 0 && !p->colsum.get()) p->colsum.alloc(sizeof(float) * (size_t)p->X * p->Y);
 }
 
 jh_status jh_bp_reset(jh_bp_plan* p, jh_stream_t stream)
@@ -1935,6 +2005,10 @@ jh_status jh_bp_reset(jh_bp_plan* p, jh_stream_t stream)
         hipStream_t s = as_stream(stream);
         fill_zero_kernel<<<grid_for(n, 256), 256, 0, s>>>(p->acc.as<float>(), n);
         fill_zero_kernel<<<grid_for(n, 256), 256, 0, s>>>(p->cnt.as<float>(), n);
+        if (p->mode == 0) {
+            const size_t nc = (size_t)p->X * p->Y;
+            fill_zero_kernel<<<grid_for(nc, 256), 256, 0, s>>>(p->colsum.as<float>(), nc);
+        }
         check_launch("fill_zero_kernel");
         p->acc_scale = 1.0f;
         p->acc_dtype = -1;
@@ -1973,9 +2047,11 @@ jh_status jh_bp_finalize(jh_bp_plan* p, float* d_vol, int z0, int nz, jh_stream_
         DeviceScope ds(p->device);
         const size_t plane = (size_t)p->X * p->Y;
         const size_t n = plane * nz;
+        const bool wt = p->weight == JH_WEIGHT_FDK;
         bp_finalize_kernel<<<grid_for(n, 256), 256, 0, as_stream(stream)>>>(
-            p->acc.as<float>() + plane * z0, p->cnt.as<float>() + plane * z0, d_vol, n, p->ct, p->ct_scale,
-            p->ct_norm1, p->weight == JH_WEIGHT_FDK ? 1 : 0, p->wscale, p->acc_scale);
+            p->acc.as<float>() + plane * z0, p->cnt.as<float>() + plane * z0,
+            p->mode == 0 && !wt ? p->colsum.as<float>() : nullptr, plane, d_vol, n, p->ct, p->ct_scale, p->ct_norm1,
+            wt ? 1 : 0, p->wscale, p->acc_scale);
         check_launch("bp_finalize_kernel");
     });
 }
@@ -1985,19 +2061,20 @@ jh_status jh_bp_finalize(jh_bp_plan* p, float* d_vol, int z0, int nz, jh_stream_
 // ---------------------------------------------------------------------------
 // Allocation-free streaming accumulation (jh_bp_accumulate_noalloc).
 //
-// The accumulator and count volumes live in one file-scope device array,
-// `scratch`, of kNoAllocVoxels voxels each, instead of plan-owned buffers:
-// every GPU the library runs on holds its own copy (64 MiB) until the process
-// ends.  Host code never passes the array's name to a kernel (in host code it
+// The accumulator and count volumes (kNoAllocVoxels voxels each) and the
+// running column count weights (kNoAllocColumns columns) live in one
+// file-scope device array, `scratch`, instead of plan-owned buffers: every GPU
+// the library runs on holds its own copy (72 MiB) until the process ends.  Host code never passes the array's name to a kernel (in host code it
 // is a stand-in, not a device address): the device address is looked up once
 // per device with hipGetSymbolAddress, with that device current, and kept.
 // ---------------------------------------------------------------------------
 
 namespace jh {
 
-constexpr size_t kNoAllocVoxels = size_t(1) << 23;  // 8 Mi voxels: e.g. 256 x 256 x 128
+constexpr size_t kNoAllocVoxels = size_t(1) << 23;   // 8 Mi voxels: e.g. 256 x 256 x 128
+constexpr size_t kNoAllocColumns = size_t(1) << 21;  // 2 Mi voxel columns: e.g. 1024 x 2048
 
-static __device__ __align__(256) float scratch[2 * kNoAllocVoxels];
+static __device__ __align__(256) float scratch[2 * kNoAllocVoxels + kNoAllocColumns];
 
 constexpr int kMaxDevices = 64;
 
@@ -2009,6 +2086,7 @@ struct NoAllocState {
     size_t nvox[kMaxDevices] = {};
     float scale[kMaxDevices] = {};
     int weight[kMaxDevices] = {};
+    int mode[kMaxDevices] = {};      // geometry class: what the count volume holds
 };
 
 static NoAllocState& noalloc_state()
@@ -2050,11 +2128,16 @@ jh_status jh_bp_accumulate_noalloc(jh_bp_plan* p, const void* d_q, int dtype, in
         require(nvox <= kNoAllocVoxels,
                 "jh_bp_accumulate_noalloc: volume of " + std::to_string(nvox) + " voxels exceeds the " +
                     std::to_string(kNoAllocVoxels) + "-voxel scratch accumulators");
+        const size_t ncol = (size_t)p->X * p->Y;
+        require(ncol <= kNoAllocColumns, "jh_bp_accumulate_noalloc: " + std::to_string(ncol) +
+                                             " voxel columns exceed the " + std::to_string(kNoAllocColumns) +
+                                             "-column scratch count weights");
         DeviceScope ds(p->device);
         NoAllocState& st = noalloc_state();
         std::lock_guard<std::mutex> lock(st.m);
         float* const acc = scratch_on(p->device);
         float* const cnt = acc + kNoAllocVoxels;
+        float* const colsum = cnt + kNoAllocVoxels;
         const int dev = p->device;
         const float sc = dtype == JH_F16 ? std::ldexp(1.0f, p->f16exp) : 1.0f;
         const bool wt = p->weight == JH_WEIGHT_FDK;
@@ -2064,22 +2147,27 @@ jh_status jh_bp_accumulate_noalloc(jh_bp_plan* p, const void* d_q, int dtype, in
             // (a fill kernel, so that a captured graph holds no memset node)
             fill_zero_kernel<<<grid_for(nvox, 256), 256, 0, s>>>(acc, nvox);
             if (!wt) fill_zero_kernel<<<grid_for(nvox, 256), 256, 0, s>>>(cnt, nvox);
+            if (!wt && p->mode == 0) fill_zero_kernel<<<grid_for(ncol, 256), 256, 0, s>>>(colsum, ncol);
             check_launch("fill_zero_kernel");
             st.open[dev] = true;
             st.nvox[dev] = nvox;
             st.scale[dev] = sc;
             st.weight[dev] = p->weight;
+            st.mode[dev] = p->mode;
         } else {
             require(st.open[dev], "jh_bp_accumulate_noalloc: no open sequence on this device (pass JH_NOALLOC_FIRST)");
-            require(st.nvox[dev] == nvox && st.weight[dev] == p->weight && (nviews == 0 || st.scale[dev] == sc),
-                    "jh_bp_accumulate_noalloc: the call does not continue the open sequence (volume, weighting or "
-                    "storage scale differ)");
+            require(st.nvox[dev] == nvox && st.weight[dev] == p->weight && st.mode[dev] == p->mode &&
+                        (nviews == 0 || st.scale[dev] == sc),
+                    "jh_bp_accumulate_noalloc: the call does not continue the open sequence (volume, weighting, "
+                    "geometry class or storage scale differ)");
         }
         if (nviews > 0)
-            launch_bp(p, d_q, dtype, view0, nviews, nullptr, wt ? kOutFdkAccumulate : kOutAccumulate, s, acc, cnt);
+            launch_bp(p, d_q, dtype, view0, nviews, nullptr, wt ? kOutFdkAccumulate : kOutAccumulate, s, acc, cnt,
+                      colsum);
         if (flags & JH_NOALLOC_LAST) {
-            bp_finalize_kernel<<<grid_for(nvox, 256), 256, 0, s>>>(acc, cnt, d_vol, nvox, p->ct, p->ct_scale,
-                                                                   p->ct_norm1, wt ? 1 : 0, p->wscale, st.scale[dev]);
+            bp_finalize_kernel<<<grid_for(nvox, 256), 256, 0, s>>>(
+                acc, cnt, !wt && p->mode == 0 ? colsum : nullptr, ncol, d_vol, nvox, p->ct, p->ct_scale, p->ct_norm1,
+                wt ? 1 : 0, p->wscale, st.scale[dev]);
             check_launch("bp_finalize_kernel");
             st.open[dev] = false;
         }

@@ -39,7 +39,8 @@ struct jh_bp_plan {
     jh::DevBuf uwp;  // N x W float2 count-weight pairs
     jh::DevBuf colcnt;  // geometry class 0: X x Y count weights of a launch's views
     jh::DevBuf acc;  // X*Y*Z float (streaming form, allocated on first use)
-    jh::DevBuf cnt;  // X*Y*Z float
+    jh::DevBuf cnt;  // X*Y*Z float: count (geometry classes 1, 2) or outside-row weight (class 0)
+    jh::DevBuf colsum;  // geometry class 0, streaming form: X x Y running count weights
     jh::DevBuf xt;   // X*Y*Z float, z-fastest copy of the volume for the projector
     jh::DevBuf sbox; // LDS-staged strips: rectangle origins per tile and view (ntiles x N uint32)
 };

@@ -241,14 +241,67 @@ def test_bp_fp32_vs_fp32_ref(gpu, tile):
     assert e <= 1e-4
 
 
-def test_bp_streaming_equals_one_shot(gpu):
+def _streamed(bp, q, dtype, sizes):
+    """Streaming accumulation of the staged views in launches of the given sizes."""
+    fb = q.numel() // sum(sizes)
+    bp.reset()
+    v0 = 0
+    for n in sizes:
+        bp.accumulate(q[v0 * fb:(v0 + n) * fb], dtype, v0, n)
+        v0 += n
+    return bp.finalize().cpu().numpy()
+
+
+# splits of 24 views into launches: whole, one view each, even, uneven
+SPLITS24 = [[24], [1] * 24, [5, 5, 5, 5, 4], [3, 10, 11], [7, 17]]
+
+
+@pytest.mark.parametrize("weight", ["count", "fdk"])
+@pytest.mark.parametrize("fetch", ["gather", "lds"])
+@pytest.mark.parametrize("dtype", ["f32", "bf16", "f16"])
+def test_bp_streaming_equals_one_shot(gpu, dtype, fetch, weight):
+    """The streaming form continues the running sums of the earlier launches
+    (accumulators, the outside-row weights and the columns' count weights), so
+    every split of the views into launches gives the one-launch volume to the
+    bit: geometry class 0, gather and LDS-staged fetch, count-normalised and
+    distance-weighted."""
+    from jenhine_amd.recon import BPCore
     w = SMALL
-    ptm = w.ptms()
-    filt = synth.normal(12, (w.n_views, w.n_v, w.n_u))
-    shape = (w.X, w.Y, w.Z, w.pxy, w.pz)
-    one, _ = _bp_gpu(gpu, ptm, filt, shape)
-    st, _ = _bp_gpu(gpu, ptm, filt, shape, chunks=5)
-    assert rel_rmse(st, one) <= 1e-6
+    filt = torch.from_numpy(synth.normal(12, (w.n_views, w.n_v, w.n_u))).to(gpu)
+    bp = BPCore(w.ptms(), w.n_u, w.n_v, w.X, w.Y, w.Z, w.pxy, w.pz, device=gpu)
+    assert bp.geometry_mode() == 0
+    bp.set_tile(16, 4)
+    bp.set_fetch(fetch)
+    if weight == "fdk":
+        bp.set_weight("fdk")
+    q = bp.stage(filt, dtype)
+    one = bp.run(q, dtype).cpu().numpy()
+    print(dtype, fetch, weight, "fetch in use", bp.fetch_in_use(dtype))
+    assert np.isfinite(one).mean() > 0.5
+    for sizes in SPLITS24:
+        np.testing.assert_array_equal(_streamed(bp, q, dtype, sizes), one, err_msg=str(sizes))
+
+
+@pytest.mark.parametrize("dtype", ["f32", "bf16", "f16"])
+def test_bp_streaming_equals_one_shot_general_geometry(gpu, dtype):
+    """As above in geometry classes 1 (u depends on z) and 2 (w too), where
+    the count is summed per sample: the reference test's matrix
+    (UTBPCUDACore.cpp:20-32) and tilted copies, 6 views."""
+    from jenhine_amd.recon import BPCore
+    P = np.array([-1.33897, -7.81194, 0.01236, 575.79443, -0.97515, -0.03862, 7.51861, 346.26642,
+                  -0.00226, -0.00004, 0.0, 1.0], np.float64)
+    tilt = P.copy()
+    tilt[10] = 0.0003
+    for base, mode in ((P, 1), (tilt, 2)):
+        ptm = np.stack([base * (1.0 + 1e-4 * k) for k in range(6)])
+        filt = torch.from_numpy(synth.normal(13, (6, 686, 644))).to(gpu)
+        bp = BPCore(ptm, 644, 686, 96, 80, 20, 0.25, 1.0, device=gpu)
+        assert bp.geometry_mode() == mode
+        q = bp.stage(filt, dtype)
+        one = bp.run(q, dtype).cpu().numpy()
+        assert np.isfinite(one).mean() > 0.5
+        for sizes in ([1] * 6, [2, 4], [5, 1]):
+            np.testing.assert_array_equal(_streamed(bp, q, dtype, sizes), one, err_msg=f"mode {mode} {sizes}")
 
 
 def test_bp_general_geometry_modes(gpu):
