@@ -44,7 +44,7 @@ kernel's own schedule and the counters committed under profiles/ (rocprofv3
     clocks), or from LDS at 128 B per clock for the LDS-staged fetch;
 frac = achieved GUPS / the lowest roof.  The reference's per-view formulation
 (an FP32 accumulator read + write per update, 8 B) is kept as a separate
-north-star field.  cpu_baseline times the CPU oracle (the reference
+field, per_view_rmw_8B.  cpu_baseline times the CPU oracle (the reference
 algorithm in portable C, FP32) on the host cores: preprocessing and filtering
 of every view and the backprojection of a bounded slab of slices.
 """
@@ -367,7 +367,7 @@ def bp_roofline(gups, units, w, stage_bytes, tuned, form="bf16"):
             "achieved": gups, "unit": "GUPS",
             "compulsory_bytes": b_sched, "gups_hbm_sched": gups_hbm,
             "window_bytes_per_update": req_per_update, f"gups_{win}": gups_win,
-            "north_star_hbm_8B": {"bytes_per_update": 8, "achieved_GBps": 8.0 * gups,
+            "per_view_rmw_8B": {"bytes_per_update": 8, "achieved_GBps": 8.0 * gups,
                                   "peak_GBps": HBM_PEAK / 1e9, "frac": 8.0 * gups * 1e9 / HBM_PEAK,
                                   "note": "the reference's per-view formulation (FP32 accumulator read + write "
                                           "per update); this kernel accumulates in registers and never moves "
