@@ -9,7 +9,7 @@
  *
  * TEST INFRASTRUCTURE ONLY.  The shipped HIP library never links, loads or
  * calls this code; only tests/, the smoke check and the CPU-baseline leg of
- * bench.py do, and only as the checker.  Compiled with -ffp-contract=off so
+ * bench.py do, and only to check results against.  Compiled with -ffp-contract=off so
  * that no multiply-add is fused behind the restated operation order.
  *
  * Semantics chosen where the reference leaves them to a closed library

@@ -1,7 +1,7 @@
 """ctypes front end of the CPU oracle (test infrastructure only).
 
 Only tests/, ``__graft_entry__.smoke()`` and the CPU-baseline leg of bench.py
-import this module, and only as the checker; the shipped library never does.
+import this module, and only to check results against; the shipped library never does.
 Every function here is a thin wrapper around ``libjh_oracle.so`` (built from
 jh_oracle.c by the Makefile beside it).
 """
