@@ -103,6 +103,10 @@ def main():
     ap.add_argument("--summarize-counters", metavar="DIR", default=None,
                     help="no GPU work: read the rocprofv3 --pmc CSV files under DIR (kernel-only runs of one "
                          "backprojection tile) and print the counter summary the roofline reads")
+    ap.add_argument("--summarize-valu", metavar="DIR", default=None,
+                    help="no GPU work: read the rocprofv3 --pmc CSV of a valu_rate run (SQ_INSTS_VALU and "
+                         "GRBM_GUI_ACTIVE) and write the clocks per wave instruction of each stream to "
+                         "profiles/valu_rate.json")
     ap.add_argument("--tile", default=None, help="zc,ty of the profiled kernel (with --summarize-counters)")
     ap.add_argument("--fetch", choices=("gather", "lds"), default="gather",
                     help="fetch form of the profiled kernel (with --summarize-counters)")
@@ -123,6 +127,8 @@ def main():
     args = ap.parse_args()
     if args.summarize_counters:
         return summarize_counters(args.summarize_counters, args.tile, args.fetch, args.form, args.counters_out)
+    if args.summarize_valu:
+        return summarize_valu(args.summarize_valu)
     if args.pipeline_args:
         print(" ".join(pipeline_command(args, args.pipeline_args)[1:]))
         return
@@ -333,7 +339,14 @@ def main():
 HBM_PEAK = 8.0e12
 CLOCK = 2.4e9
 CUS = 256
-VALU_WAVE_INSTR_PER_S = 1024 * CLOCK / 2.0
+SIMDS = 1024
+VALU_RATE_FILE = os.path.join("profiles", "valu_rate.json")
+# VALU instruction classes of the SQ_INSTS_VALU_* counters -> the valu_rate
+# stream whose clocks per wave instruction they take; the rest of the
+# instructions (v_perm_b32, v_dot2_f32_bf16, selects, compares: no class)
+# take those of v_perm_b32 (v_dot2 measures the same)
+VALU_CLASSES = {"valu_fma_f32": "fma_f32", "valu_add_f32": "add_f32", "valu_mul_f32": "fma_f32",
+                "valu_trans_f32": "perm_b32", "valu_cvt": "cvt", "valu_int32": "xad_u32"}
 TA_BYTES_PER_CLOCK = 64.0    # per CU: four lanes of a 16-byte gather per clock
 LDS_BYTES_PER_CLOCK = 128.0  # per CU: LDS read bandwidth
 COUNTERS_FILE = os.path.join("profiles", "bp_dental512_counters.json")
@@ -380,21 +393,80 @@ def bp_roofline(gups, units, w, stage_bytes, tuned, form="bf16"):
     key = counter_key(form, fetch, tuned["zc"], tuned["ty"])
     c = (c or {}).get("kernels", {}).get(key)
     gups_valu = None
-    if c is not None and c.get("valu_insts"):
+    vc = valu_cycles(c) if c is not None else None
+    if vc:
         valu_per_update = c["valu_insts"] * 64.0 / units
-        gups_valu = VALU_WAVE_INSTR_PER_S * 64.0 / valu_per_update / 1e9
-        roof.update(valu_lane_instr_per_update=valu_per_update, gups_valu=gups_valu,
+        # SIMD clocks of VALU issue per voxel-update, at the published peak clock
+        cyc_per_update = vc / units
+        gups_valu = SIMDS * CLOCK / cyc_per_update / 1e9
+        roof.update(valu_lane_instr_per_update=valu_per_update, valu_simd_clocks_per_update=cyc_per_update,
+                    gups_valu=gups_valu, valu_model=f"SQ_INSTS_VALU_* classes at the clocks per instruction "
+                                                    f"of {VALU_RATE_FILE}",
                     traffic=c["hbm_bytes"], traffic_over_compulsory=c["hbm_bytes"] / b_sched,
                     l2_hit=c["l2_hit"], valu_busy=c["valu_busy"], ta_busy=c["ta_busy"],
                     counters=f"{COUNTERS_FILE} [{key}]")
     else:
-        roof.update(traffic=None, counters=f"no counters for {key}: VALU roof not evaluated")
+        roof.update(traffic=c.get("hbm_bytes") if c else None,
+                    counters=f"no VALU class counters for {key}: VALU roof not evaluated")
     roofs = {"hbm": gups_hbm, win: gups_win}
     if gups_valu:
         roofs["valu"] = gups_valu
     bound = min(roofs, key=roofs.get)
     roof.update(bound=bound, peak=roofs[bound], frac=gups / roofs[bound], roofs=roofs)
     return roof
+
+
+def valu_rates():
+    path = os.path.join(ROOT, VALU_RATE_FILE)
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        return json.load(fh)["cycles_per_wave_instr"]
+
+
+def valu_cycles(c):
+    """SIMD clocks of VALU issue of one launch (summed over the SIMDs): each
+    SQ_INSTS_VALU_* class at the clocks per wave instruction valu_rate measured
+    for its stream, the unclassified rest at v_perm_b32's; None without the
+    class counters or the rates."""
+    r = valu_rates()
+    if r is None or not c.get("valu_insts") or any(c.get(k) is None for k in VALU_CLASSES):
+        return None
+    classified = sum(c[k] for k in VALU_CLASSES)
+    return sum(c[k] * r[VALU_CLASSES[k]] for k in VALU_CLASSES) + (c["valu_insts"] - classified) * r["perm_b32"]
+
+
+def summarize_valu(folder):
+    """Clocks per wave instruction and SIMD of each valu_rate stream from a
+    rocprofv3 --pmc run with SQ_INSTS_VALU and GRBM_GUI_ACTIVE (the second,
+    warm launch of each stream); written to profiles/valu_rate.json."""
+    import csv
+    import glob
+    names = ["fma_f32", "perm_b32", "dot2_f32_bf16", "xad_u32", "add_f32", "cvt"]
+    per = {}  # stream -> [(dispatch, instr, gui cycles summed over XCDs, ns)]
+    for path in glob.glob(os.path.join(folder, "**", "*counter_collection.csv"), recursive=True):
+        with open(path) as fh:
+            for row in csv.DictReader(fh):
+                kn = row["Kernel_Name"]
+                if not kn.startswith("void stream<"):
+                    continue
+                op = int(kn[len("void stream<"):].split(">")[0])
+                d = per.setdefault(names[op], {}).setdefault(int(row["Dispatch_Id"]), {})
+                d[row["Counter_Name"]] = d.get(row["Counter_Name"], 0.0) + float(row["Counter_Value"])
+                d["ns"] = int(row["End_Timestamp"]) - int(row["Start_Timestamp"])
+    out = {"source": "rocprofv3 --pmc SQ_INSTS_VALU GRBM_GUI_ACTIVE of jenhine_amd/build/valu_rate "
+                     "(profiles/valu_rate.md)", "cycles_per_wave_instr": {}, "clock_ghz": {}}
+    for name in names:
+        if name not in per:
+            continue
+        warm = per[name][max(per[name])]  # the later launch of the stream
+        cyc = warm["GRBM_GUI_ACTIVE"] / 8.0  # per XCD
+        out["cycles_per_wave_instr"][name] = cyc / (warm["SQ_INSTS_VALU"] / SIMDS)
+        out["clock_ghz"][name] = cyc / warm["ns"]
+    print(json.dumps(out, indent=1))
+    with open(os.path.join(ROOT, VALU_RATE_FILE), "w") as fh:
+        json.dump(out, fh, indent=1)
+        fh.write("\n")
 
 
 def summarize_counters(folder, tile, fetch, form, out_file):
@@ -447,13 +519,18 @@ def summarize_counters(folder, tile, fetch, form, out_file):
                           ("sq_wait_inst_lds", "SQ_WAIT_INST_LDS"), ("lds_bank_conflict", "SQ_LDS_BANK_CONFLICT"),
                           ("ta_busy_sum", "TA_TA_BUSY_sum"), ("ta_buffer_read_wavefronts", "TA_BUFFER_READ_WAVEFRONTS_sum"),
                           ("td_busy_sum", "TD_TD_BUSY_sum"), ("tcp_accesses", "TCP_TOTAL_CACHE_ACCESSES_sum"),
-                          ("tcp_tcc_read_req", "TCP_TCC_READ_REQ_sum")):
+                          ("tcp_tcc_read_req", "TCP_TCC_READ_REQ_sum"), ("valu_fma_f32", "SQ_INSTS_VALU_FMA_F32"),
+                          ("valu_add_f32", "SQ_INSTS_VALU_ADD_F32"), ("valu_mul_f32", "SQ_INSTS_VALU_MUL_F32"),
+                          ("valu_trans_f32", "SQ_INSTS_VALU_TRANS_F32"), ("valu_cvt", "SQ_INSTS_VALU_CVT"),
+                          ("valu_int32", "SQ_INSTS_VALU_INT32")):
         out[name] = per_dispatch(is_bp, counter)
     cyc = out["gui_active_cycles_per_xcd"]
     if cyc:
         out["clock_ghz"] = cyc / (out["ms"] * 1e6) if out["ms"] else None
-        # two clocks per wave-level VALU instruction on each of the 1024 SIMDs
-        out["valu_busy"] = out["valu_insts"] * 2.0 / (1024 * cyc) if out["valu_insts"] else None
+        # the VALU instruction mix at the measured clocks per instruction
+        vc = valu_cycles(out)
+        out["valu_cycles_per_simd"] = vc / SIMDS if vc else None
+        out["valu_busy"] = vc / (SIMDS * cyc) if vc else None
         out["ta_busy"] = out["ta_busy_sum"] / (CUS * cyc) if out["ta_busy_sum"] else None
         out["lds_bank_conflict_share"] = (out["lds_bank_conflict"] / (CUS * cyc)
                                           if out["lds_bank_conflict"] is not None else None)

@@ -46,15 +46,19 @@ def build_hip(verbose=True):
 
 
 def build_probe():
-    """gather_rate: the vector-memory issue-rate probe the roofline cites."""
+    """gather_rate and valu_rate: the vector-memory and VALU issue-rate probes
+    the roofline cites."""
     outdir = os.path.join(HERE, "build")
     os.makedirs(outdir, exist_ok=True)
-    out = os.path.join(outdir, "gather_rate")
-    _run(["/opt/rocm/bin/hipcc", "--offload-arch=gfx950", "-O3", "-std=c++17",
-          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument",
-          os.path.join(CSRC, "gather_rate.hip"), "-o", out + ".tmp"])
-    os.replace(out + ".tmp", out)
-    return out
+    outs = []
+    for name in ("gather_rate", "valu_rate"):
+        out = os.path.join(outdir, name)
+        _run(["/opt/rocm/bin/hipcc", "--offload-arch=gfx950", "-O3", "-std=c++17",
+              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-command-line-argument",
+              os.path.join(CSRC, name + ".hip"), "-o", out + ".tmp"])
+        os.replace(out + ".tmp", out)
+        outs.append(out)
+    return outs
 
 
 def build_synth():
