@@ -37,8 +37,11 @@ kernel's own schedule and the counters committed under profiles/ (rocprofv3
 --pmc of the same kernel):
   * HBM: the bytes the schedule must move at least once (the volume written
     once, every staged view read once) at 8 TB/s;
-  * VALU: the measured VALU instructions per voxel-update at the published
-    FP32 vector rate (157.3 TFLOP/s = 1.23e12 wave instructions/s);
+  * VALU: the kernel's VALU instructions per voxel-update by counter class
+    (SQ_INSTS_VALU_*), each at the clocks per wave instruction and SIMD the
+    valu_rate probe measured for its type (profiles/valu_rate.json: FP32
+    add/FMA ~2.2, permutes, BF16 dots, integer, conversions ~4.1), over 1024
+    SIMDs at the published 2.4 GHz peak clock;
   * vector-memory issue: the bytes each update requests through the texture
     path at 64 B per clock per CU (a 64-lane 16-byte gather issues in 16
     clocks), or from LDS at 128 B per clock for the LDS-staged fetch;
