@@ -63,12 +63,31 @@ def test_agent_graph_matches_python_path_and_oracle(gpu, data, tmp_path):
     fdk = FDK(W.scan_params(), W.recon_params(), data["ptm"], "fp32", "f32", gpu)
     ref = fdk(torch.from_numpy(data["raw"].view(np.int16)).to(gpu), torch.from_numpy(data["dark"].view(np.int16)).to(gpu),
               torch.from_numpy(data["gain"]).to(gpu)).cpu().numpy()
-    assert rel_rmse(vol, ref) <= 1e-6
+    # the streamed chunks continue the one-launch sums: the same volume to the bit
+    np.testing.assert_array_equal(vol, ref)
     sc, rc = oracle_params(W)
     v32 = o.fdk(data["raw"], data["ptm"], sc, rc, dark=data["dark"], gain=data["gain"])
     e = rel_rmse(vol, v32)
     print("agent graph vs FP32-ref oracle", e)
     assert e <= 1e-4
+
+
+@pytest.mark.parametrize("w", [W, configs.DENTAL512], ids=["cbct64", "dental512"])
+def test_agent_graph_headline_form_equals_python_path(gpu, w, tmp_path):
+    """The headline form (BF16 storage, row-scaled FP16 MFMA filter) through
+    the agent graph -- raw counts from host memory, 64-view chunks streamed and
+    accumulated, slices to host memory -- equals the Python path's one-shot
+    FDK (pre-weighted operand, filter with fused staging, one backprojection
+    launch) to the bit, at the test size and at dental512."""
+    from jenhine_amd.recon import FDK
+    d = w.data()
+    vol, info = _run(tmp_path, d, storage=1, filt=4, chunk=64, w=w)
+    assert info["parts"] == 1
+    fdk = FDK(w.scan_params(), w.recon_params(), d["ptm"], "mfma16", "bf16", gpu)
+    ref = fdk(torch.from_numpy(d["raw"].view(np.int16)).to(gpu), torch.from_numpy(d["dark"].view(np.int16)).to(gpu),
+              torch.from_numpy(d["gain"]).to(gpu)).cpu().numpy()
+    assert np.isfinite(ref).mean() > 0.5
+    np.testing.assert_array_equal(vol, ref)
 
 
 def test_agent_graph_bf16_mfma(gpu, data, tmp_path):
