@@ -90,8 +90,9 @@ __global__ void bp_uweight_kernel(const double* __restrict__ ptm, int N, int W, 
 // Frame layout: V rows of pixels; the data a bilinear sample whose integer
 // part is (u, v) needs is one aligned gather, with q[v][W] = 0 and
 // q[V][u] = 0 (so a sample at u <= W-1, v <= V-1 needs no clamping: at the
-// far edges the weight of the padding is exactly 0).  The gathers are
-// limited by the cache lines each wave instruction touches, not by bytes.
+// far edges the weight of the padding is exactly 0).  A gather's cost in
+// the texture path follows the cache lines each wave instruction touches more
+// than its bytes.
 //  * FP32 (16 B): (q[v][u], q[v][u+1] - q[v][u], q[v+1][u], q[v+1][u+1] -
 //    q[v+1][u]).  Each row interpolation is one FMA, q + fu d, and d is the
 //    difference the interpolation would form anyway (same result to the bit).
@@ -649,9 +650,9 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
     const int tz = rest / a.nty;
     // Each wave covers an 8 x 8 patch of voxel columns; the blockDim.y waves
     // of a workgroup tile 8 x 8 (1 wave), 16 x 8 (2) or 16 x 16 (4).  A wave's
-    // 64 samples of one view then fall on a compact detector footprint: the
-    // gathers are limited by the number of cache lines each wave touches, which
-    // a 64 x 1 row of voxels spreads over ~4x more lines.
+    // 64 samples of one view then fall on a compact detector footprint: a
+    // gather's texture-path cost follows the cache lines the wave touches,
+    // which a 64 x 1 row of voxels spreads over ~4x more lines.
     const int px = blockDim.y >= 2 ? 2 : 1;
     const int wx = threadIdx.y % px, wy = threadIdx.y / px;
     const int i = (tx * px + wx) * 8 + (threadIdx.x & 7);
@@ -733,8 +734,10 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
                 if (fminf(sv0, svl) >= 0.0f && fmaxf(sv0, svl) <= Vm1) {
                     // every sample counts: the count weight is added once.
                     // All gathers of the column are issued before any is
-                    // consumed: the kernel is bound by gather latency, and
-                    // this keeps 2 ZC of them in flight per wave.
+                    // consumed: this keeps ZC / 2 of them in flight per wave,
+                    // so their latency hides behind the VALU work (which
+                    // bounds the kernel: permutes, dots, integer and
+                    // conversion instructions issue every ~4 clocks).
                     // BF16 / FP16 strips: one b128 gather per pair of samples
                     // (k, k+1) when the row step lies clear of an integer
                     // (margin 1/64 against the rounding of sv): sample k
@@ -1264,9 +1267,10 @@ __global__ void __launch_bounds__(256) bp_colcount_kernel(const Geo* __restrict_
 // columns (samples outside the detector rows) keep their count corrections
 // in registers.
 // What this buys: each footprint pixel passes the texture path once per
-// workgroup instead of once per sampling lane (the gather form is bound by
-// the texture data path, ~28 clocks per 64-lane 16-byte gather), and the
-// window reads run from LDS.
+// workgroup instead of once per sampling lane (the gather form's texture data
+// path spends ~28 clocks per 64-lane 16-byte gather), and the window reads run
+// from LDS; what it costs: the window addressing inside the rectangle adds
+// VALU work, which bounds both forms.
 // ---------------------------------------------------------------------------
 
 // The strip rectangle a block of voxel columns [cx0, cx1] x [cy0, cy1], z
