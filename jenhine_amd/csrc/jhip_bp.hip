@@ -619,19 +619,6 @@ __device__ __forceinline__ void bp_column_pairs(float* acc, const F& fetch, uint
     }
 }
 
-// A view's geometry through the constant address space: scalar loads, which
-// the compiler otherwise makes only where it can prove that no store of the
-// kernel reaches the array.  A vector load in the view loop would make each
-// view wait for every gather or copy in flight.
-__device__ __forceinline__ Geo geo_of(const Geo* geo, int view)
-{
-    const __attribute__((address_space(4))) float* p = (const __attribute__((address_space(4))) float*)(geo + view);
-    Geo g;
-#pragma unroll
-    for (int k = 0; k < 12; ++k) g.c[k] = p[k];
-    return g;
-}
-
 // One tile: 64 blockDim.y voxel columns (8 x 8 per wave), ZC voxels deep,
 // all views of the launch.
 template <int MODE, int ZC, int QT, int OUT>

@@ -12,6 +12,19 @@ struct Geo {
     float c[12];
 };
 
+// A view's geometry through the constant address space: scalar loads, which
+// the compiler otherwise makes only where it can prove that no store of the
+// kernel reaches the array.  A vector load in the view loop would make each
+// view wait for every gather or copy in flight.
+__device__ __forceinline__ Geo geo_of(const Geo* geo, int view)
+{
+    const __attribute__((address_space(4))) float* p = (const __attribute__((address_space(4))) float*)(geo + view);
+    Geo g;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) g.c[k] = p[k];
+    return g;
+}
+
 }  // namespace jh
 
 struct jh_bp_plan {

@@ -9,11 +9,11 @@
 // optionally multiplied by uW_v(iu) (the transpose of the weighted staging).
 //
 // For geometries whose u and w do not depend on z (mode 0, circular orbits)
-// it is a deterministic gather (fp_block_kernel): a workgroup covers a block
-// of detector columns and walks the wedge of voxel columns whose samples can
-// fall in them, recomputing su and sv with the backprojector's own float
-// expressions (including the z tiling), so membership and weights are those
-// of the backprojector.  Other geometries use a scatter with float atomics
+// it is deterministic (fp_column_kernel): each wave owns one detector column,
+// walks the wedge of voxel columns whose samples can fall in it and spreads
+// their z samples over its lanes, recomputing su and sv with the
+// backprojector's own float expressions (including the z tiling), so
+// membership and weights are those of the backprojector.  Other geometries use a scatter with float atomics
 // (same weights, summation order not fixed).
 #include <cmath>
 
@@ -57,116 +57,278 @@ __device__ inline float bp_su(const Geo& g, float fi, float fj, float& rw)
     return fmaf(Uc, rw, 0.5f);
 }
 
-// Block form of the mode-0 gather: one workgroup per view and block of kFpCols
-// detector columns [u0, u0 + kFpCols), rows in passes of 64 x blockDim.y.
-// The workgroup walks the wedge of voxel columns whose su lies in
-// [u0 - 1, u0 + kFpCols) once (each voxel column feeds detector columns
-// floor(su) and floor(su) + 1); every lane owns one detector row and adds,
-// per voxel column, the column's vertical profile at its row -- the samples
-// k whose rows floor(sv_k), floor(sv_k) + 1 include it, with the
-// backprojector's own sample expressions (z tiling included) -- times
-// 1 - fu and fu into its LDS row.  Lanes own their rows, so the LDS needs no
-// atomics and the summation order is fixed.
-constexpr int kFpCols = 16;
+// Column form of the mode-0 transpose.  A workgroup of nw waves covers the
+// detector columns [c0, c0 + nw) of one view; wave w owns column c = c0 + w,
+// accumulated in LDS (V + 1 floats: row V takes the zero-weight share of
+// samples at sv = V - 1 and is never written out).  The wave walks the wedge
+// of voxel columns whose su can lie in [c - 1, c + 1), eight candidates on
+// each of eight major-axis lines per round, and keeps those that the
+// backprojector's own su puts at floor(su) = c (weight 1 - fu on column c) or
+// c - 1 (weight fu).  Each such member's samples are then spread over the
+// lanes, 64 consecutive z per step: lane k adds x_k (1 - fv_k) w_u to row
+// floor(sv_k) and x_k fv_k w_u to the next row, in two phases (every lane's
+// first row, then every lane's second row), with sv_k the backprojector's
+// expression (z tiling included).  The rows of one phase are distinct when
+// the sample spacing |dsv| is at least 1 + 2^-8 rows (the margin covers the
+// rounding of the per-tile expressions); below that the lanes take turns, in
+// ceil((1 + 2^-8) / |dsv|) passes.  A wave owns its column, so no atomics are
+// needed and the summation order is fixed.
+constexpr float kFpRowMargin = 1.00390625f;
 
-__global__ void __launch_bounds__(512) fp_block_kernel(FPArgs a)
+__device__ __forceinline__ void lds_order()
 {
-    __shared__ float accl[512 * (kFpCols + 1)];  // row r of the pass: accl[r * 17 + c], c < kFpCols
+    // the two phases touch different rows within a lane, so only this
+    // barrier keeps the compiler from moving one phase's LDS accesses across
+    // the other's; the LDS executes one wave's accesses in order
+    asm volatile("" ::: "memory");
+}
+
+// A member of a wave's column: voxel column j X + i, the weight wu its su
+// puts on the column, and its 1 / w and V coefficient at z = 0.
+struct FpMember {
+    int col;
+    float wu, rw, vc0;
+};
+
+__device__ __forceinline__ FpMember fp_pick(int cix, float wu, float rw, float vc0, int l)
+{
+    FpMember m;
+    m.col = __builtin_amdgcn_readlane(cix, l);
+    m.wu = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(wu), l));
+    m.rw = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(rw), l));
+    m.vc0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(vc0), l));
+    return m;
+}
+
+// The member's samples at the per-lane byte offsets koff (clamped into the
+// column, so every lane loads and no load needs a branch; 32-bit unsigned
+// byte offsets, so the load takes the column address as a scalar base).
+template <int NC>
+__device__ __forceinline__ void fp_load(const float* __restrict__ xt, int Z, const FpMember& m,
+                                        const unsigned* koff, float* xv)
+{
+    const char* xc = (const char*)(xt + (size_t)m.col * Z);
+#pragma unroll
+    for (int i = 0; i < NC; ++i) xv[i] = *(const float*)(xc + koff[i]);
+}
+
+// The same through a buffer resource over the whole volume (< 4 GiB): the
+// column's byte offset rides in the scalar offset, the lane's in the vector
+// offset, so a member costs no address arithmetic.  The offsets stay inside
+// the volume, which is also the resource's range.
+template <int NC>
+__device__ __forceinline__ void fp_load_buf(__amdgpu_buffer_rsrc_t xr, unsigned soff, const unsigned* koff,
+                                            float* xv)
+{
+#pragma unroll
+    for (int i = 0; i < NC; ++i)
+        xv[i] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(xr, koff[i], soff, 0));
+}
+
+// Adds NC chunks of 64 samples of one member to the wave's column.  fkt /
+// fkk: each lane's sample z tile start and offset in it, as floats; kval:
+// sample inside the volume; c7v: the V coefficient of z, in a vector
+// register (the other operands of its FMA are scalar).  MULTI: the lanes
+// take turns in `passes` passes (sample spacing below a row).
+template <int NC, bool MULTI>
+__device__ __forceinline__ void fp_apply(float* col, const float* xv, const float* fkt, const float* fkk,
+                                         const bool* kval, float c7v, const FpMember& m, float dsv, float Vm1,
+                                         int lane, int passes)
+{
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        const float Vc = fmaf(c7v, fkt[i], m.vc0);
+        const float sv0 = fmaf(Vc, m.rw, 0.5f);
+        const float sv = fmaf(fkk[i], dsv, sv0);
+        const float tt = __builtin_amdgcn_fmed3f(sv, 0.0f, Vm1);
+        const bool ok = kval[i] && tt == sv;
+        const int iv = (int)tt;
+        const float fv = __builtin_amdgcn_fractf(tt);
+        const float xw = m.wu * xv[i];
+        const float lo = fmaf(-fv, xw, xw), hi = fv * xw;
+        if constexpr (!MULTI) {
+            // one instruction stream for all lanes: every lane's first-row
+            // update is written before any lane reads its second row
+            if (ok) {
+                col[iv] += lo;
+                lds_order();
+                col[iv + 1] += hi;
+            }
+            lds_order();
+        } else {
+            for (int p = 0; p < passes; ++p) {
+                if (ok && lane % passes == p) col[iv] += lo;
+                lds_order();
+            }
+            for (int p = 0; p < passes; ++p) {
+                if (ok && lane % passes == p) col[iv + 1] += hi;
+                lds_order();
+            }
+        }
+    }
+}
+
+template <int NC>
+__device__ __forceinline__ void fp_member(float* col, const float* xv, const float* fkt, const float* fkk,
+                                          const bool* kval, float c7v, const FpMember& m, float Vm1, int lane)
+{
+    const float dsv = c7v * m.rw;
+    const float adsv = fabsf(dsv);
+    // wave-uniform (the member's values are scalar), made a scalar branch
+    if (__builtin_amdgcn_readfirstlane((int)(adsv >= kFpRowMargin))) {
+        fp_apply<NC, false>(col, xv, fkt, fkk, kval, c7v, m, dsv, Vm1, lane, 1);
+    } else {
+        const int passes = adsv > 0.0f ? min(64, (int)ceilf(kFpRowMargin / adsv)) : 64;
+        fp_apply<NC, true>(col, xv, fkt, fkk, kval, c7v, m, dsv, Vm1, lane, passes);
+    }
+}
+
+// NI > 0: Z <= 64 NI and a volume below 4 GiB, the per-lane sample
+// constants held in registers and the next member's samples loaded while the
+// current one is added; NI == 0: any size, in chunks of 4 x 64 samples.
+template <int NI>
+__global__ void __launch_bounds__(512) fp_column_kernel(FPArgs a)
+{
+    extern __shared__ float colbuf[];
+    const int lane = threadIdx.x, nw = blockDim.y;
     const int v = blockIdx.y;
-    const int u0 = blockIdx.x * kFpCols;
-    const int lane_row = threadIdx.y * 64 + threadIdx.x;
-    const int rows_per_pass = 64 * blockDim.y;
-    const Geo g = a.geo[a.view0 + v];
+    const int c0 = blockIdx.x * nw;
+    const int c = c0 + (int)threadIdx.y;
+    const int rows = a.V + 1;
+    float* const col = colbuf + threadIdx.y * rows;
+    for (int r = lane; r < rows; r += 64) col[r] = 0.0f;
+    const Geo g = geo_of(a.geo, a.view0 + v);
+    float c7v;
+    asm("v_mov_b32 %0, %1" : "=v"(c7v) : "s"(g.c[7]));
     const float Wm1 = (float)(a.W - 1), Vm1 = (float)(a.V - 1);
     const int zmask = ~(a.zc - 1);  // the backprojector's z tile start (zc is a power of two)
-    // wedge boundaries (c1 - t c9) i + (c2 - t c10) j + (c0 - t c8) = 0 at
-    // su = t + 0.5 for t = u0 - 1.5 and t = u0 + kFpCols - 0.5
-    const double t0 = (double)u0 - 1.5, t1 = (double)(u0 + kFpCols) - 0.5;
-    const double ai0 = g.c[1] - t0 * g.c[9], aj0 = g.c[2] - t0 * g.c[10], a00 = g.c[0] - t0 * g.c[8];
-    const double ai1 = g.c[1] - t1 * g.c[9], aj1 = g.c[2] - t1 * g.c[10], a01 = g.c[0] - t1 * g.c[8];
-    const double tc = (double)u0 + 0.5 * kFpCols - 0.5;
-    const bool major_i = fabs(g.c[2] - tc * g.c[10]) >= fabs(g.c[1] - tc * g.c[9]);
-    const int nmaj = major_i ? a.X : a.Y, nmin = major_i ? a.Y : a.X;
-    // minor-axis bound on line b: -(a_maj m + a_0) / a_min, linear in m
-    const double am0 = major_i ? aj0 : ai0, ak0 = major_i ? ai0 : aj0;
-    const double am1 = major_i ? aj1 : ai1, ak1 = major_i ? ai1 : aj1;
-    const double s0 = am0 != 0.0 ? -ak0 / am0 : 0.0, c0 = am0 != 0.0 ? -a00 / am0 : -1e30;
-    const double s1 = am1 != 0.0 ? -ak1 / am1 : 0.0, c1 = am1 != 0.0 ? -a01 / am1 : 1e30;
-    for (int rp = 0; rp < a.V; rp += rows_per_pass) {
-        const int iv = rp + lane_row;
-        float* const mine = accl + lane_row * (kFpCols + 1);
+    constexpr int NR = NI > 0 ? NI : 4;
+    float fkt[NR], fkk[NR];
+    bool kval[NR];
+    unsigned koff[NR];
+    if constexpr (NI > 0) {
 #pragma unroll
-        for (int c = 0; c < kFpCols; ++c) mine[c] = 0.0f;
-        const float fiv = (float)iv;
-        for (int m = 0; m < nmaj; ++m) {
-            const double b0 = am0 != 0.0 ? fma((double)m, s0, c0) : -1e30;
-            const double b1 = am1 != 0.0 ? fma((double)m, s1, c1) : 1e30;
-            const int n0 = (int)fmax(0.0, floor(fmin(b0, b1)) - 2.0);
-            const int n1 = (int)fmin((double)(nmin - 1), ceil(fmax(b0, b1)) + 2.0);
-            for (int n = n0; n <= n1; ++n) {
-                const int i = major_i ? m : n, j = major_i ? n : m;
-                const float fi = (float)i, fj = (float)j;
-                float rw;
-                const float su = bp_su(g, fi, fj, rw);
-                if (!(su >= 0.0f && su <= Wm1)) continue;
-                const int iu0 = (int)su;
-                if (iu0 < u0 - 1 || iu0 >= u0 + kFpCols) continue;
-                const float fu = __builtin_amdgcn_fractf(su);
-                const float dsv = g.c[7] * rw;
-                const float Vc0 = fmaf(g.c[5], fi, fmaf(g.c[6], fj, g.c[4]));
-                // samples k whose rows floor(sv_k) .. floor(sv_k) + 1 may
-                // include this lane's row: sv_k in [iv - 1, iv + 1), with a
-                // margin of one sample against the rounding of sv
-                const float svA = fmaf(Vc0, rw, 0.5f);
-                int k0 = 0, k1 = a.Z - 1;
-                if (fabsf(dsv) > 1e-6f) {
-                    // k with sv_k in [iv - 1, iv + 1): sv_k = svA + k dsv up to
-                    // the rounding of the backprojector's per-tile expressions
-                    // (a few 1e-6 of a row on these sizes); the margin of
-                    // 1/64 sample also absorbs the reciprocal's ulp
-                    const float r = __builtin_amdgcn_rcpf(dsv);
-                    const float ka = (fiv - 1.0f - svA) * r, kb = (fiv + 1.0f - svA) * r;
-                    k0 = (int)fmaxf(0.0f, ceilf(fminf(ka, kb) - 0.015625f));
-                    k1 = (int)fminf((float)(a.Z - 1), floorf(fmaxf(ka, kb) + 0.015625f));
+        for (int i = 0; i < NI; ++i) {
+            const int k = 64 * i + lane, kt = k & zmask;
+            fkt[i] = (float)kt;
+            fkk[i] = (float)(k - kt);
+            kval[i] = k < a.Z;
+            koff[i] = 4u * (unsigned)min(k, a.Z - 1);
+        }
+    }
+    // raw buffer over the z-fastest volume; dword 3: 32-bit data format
+    const __amdgpu_buffer_rsrc_t xr = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)a.xt, (short)0, (int)(unsigned)((size_t)a.X * a.Y * a.Z * sizeof(float)), 0x00020000);
+    const unsigned zbytes = 4u * (unsigned)a.Z;
+    if (c < a.W) {
+        // wedge boundaries (c1 - t c9) i + (c2 - t c10) j + (c0 - t c8) = 0
+        // at su = t + 0.5 for t = c - 1.5 and t = c + 0.5
+        const double t0 = (double)c - 1.5, t1 = (double)c + 0.5;
+        const double ai0 = g.c[1] - t0 * g.c[9], aj0 = g.c[2] - t0 * g.c[10], a00 = g.c[0] - t0 * g.c[8];
+        const double ai1 = g.c[1] - t1 * g.c[9], aj1 = g.c[2] - t1 * g.c[10], a01 = g.c[0] - t1 * g.c[8];
+        const double tc = (double)c - 0.5;
+        const bool major_i = fabs(g.c[2] - tc * g.c[10]) >= fabs(g.c[1] - tc * g.c[9]);
+        const int nmaj = major_i ? a.X : a.Y, nmin = major_i ? a.Y : a.X;
+        // minor-axis bound on line b: -(a_maj m + a_0) / a_min, linear in m
+        const double am0 = major_i ? aj0 : ai0, ak0 = major_i ? ai0 : aj0;
+        const double am1 = major_i ? aj1 : ai1, ak1 = major_i ? ai1 : aj1;
+        const double s0 = am0 != 0.0 ? -ak0 / am0 : 0.0, q0 = am0 != 0.0 ? -a00 / am0 : -1e30;
+        const double s1 = am1 != 0.0 ? -ak1 / am1 : 0.0, q1 = am1 != 0.0 ? -a01 / am1 : 1e30;
+        const int lm = lane >> 3, ls = lane & 7;
+        for (int mb = 0; mb < nmaj; mb += 8) {
+            const int m = mb + lm;
+            int n0 = 1, n1 = 0;
+            if (m < nmaj) {
+                const double b0 = am0 != 0.0 ? fma((double)m, s0, q0) : -1e30;
+                const double b1 = am1 != 0.0 ? fma((double)m, s1, q1) : 1e30;
+                // clamped to [0, nmin] and [-1, nmin - 1], so n0 + ls + t
+                // cannot overflow however steep the lines
+                n0 = (int)fmax(0.0, fmin((double)nmin, floor(fmin(b0, b1)) - 2.0));
+                n1 = (int)fmax(-1.0, fmin((double)(nmin - 1), ceil(fmax(b0, b1)) + 2.0));
+            }
+            for (int t = 0;; t += 8) {
+                const int n = n0 + ls + t;
+                bool mem = n <= n1;
+                if (__ballot(mem) == 0) break;
+                float wu = 0.0f, rw = 0.0f, vc0 = 0.0f;
+                int cix = 0;
+                if (mem) {
+                    const int i = major_i ? m : n, j = major_i ? n : m;
+                    const float fi = (float)i, fj = (float)j;
+                    const float su = bp_su(g, fi, fj, rw);
+                    const bool in = su >= 0.0f && su <= Wm1;
+                    const int iu0 = in ? (int)su : -2;
+                    const float fu = __builtin_amdgcn_fractf(su);
+                    mem = iu0 == c || iu0 == c - 1;
+                    wu = iu0 == c ? 1.0f - fu : fu;
+                    vc0 = fmaf(g.c[5], fi, fmaf(g.c[6], fj, g.c[4]));
+                    cix = j * a.X + i;
                 }
-                if (iv >= a.V) k1 = -1;
-                const float* xc = a.xt + ((size_t)j * a.X + i) * a.Z;
-                float prof = 0.0f;
-                // candidates in pairs: the loads of a pair (at addresses
-                // clamped into the column) are in flight together, then the
-                // exact membership test picks the samples that count
-                for (int kb = k0; kb <= k1; kb += 2) {
-                    float xv[2];
+                unsigned long long mask = __ballot(mem);
+                if (mask == 0) continue;
+                if constexpr (NI > 0) {
+                    // two members in flight: the next one's samples load
+                    // while the current one is added.  The buffers
+                    // alternate (no copy waits for a load), and the next
+                    // load is issued on every path (after the last member it
+                    // reloads that member's column), so the wait for the
+                    // current samples never counts the next ones.
+                    int l = __builtin_ctzll(mask);
+                    mask &= mask - 1;
+                    FpMember ma = fp_pick(cix, wu, rw, vc0, l), mb;
+                    float xa[NI], xb[NI];
+                    fp_load_buf<NI>(xr, (unsigned)ma.col * zbytes, koff, xa);
+                    for (;;) {
+                        bool more = mask != 0;
+                        if (more) {
+                            l = __builtin_ctzll(mask);
+                            mask &= mask - 1;
+                        }
+                        mb = fp_pick(cix, wu, rw, vc0, l);
+                        fp_load_buf<NI>(xr, (unsigned)mb.col * zbytes, koff, xb);
+                        fp_member<NI>(col, xa, fkt, fkk, kval, c7v, ma, Vm1, lane);
+                        if (!more) break;
+                        more = mask != 0;
+                        if (more) {
+                            l = __builtin_ctzll(mask);
+                            mask &= mask - 1;
+                        }
+                        ma = fp_pick(cix, wu, rw, vc0, l);
+                        fp_load_buf<NI>(xr, (unsigned)ma.col * zbytes, koff, xa);
+                        fp_member<NI>(col, xb, fkt, fkk, kval, c7v, mb, Vm1, lane);
+                        if (!more) break;
+                    }
+                } else {
+                    while (mask) {
+                        const int l = __builtin_ctzll(mask);
+                        mask &= mask - 1;
+                        const FpMember cur = fp_pick(cix, wu, rw, vc0, l);
+                        for (int kb = 0; kb < a.Z; kb += 256) {
 #pragma unroll
-                    for (int q = 0; q < 2; ++q) xv[q] = xc[min(kb + q, a.Z - 1)];
-#pragma unroll
-                    for (int q = 0; q < 2; ++q) {
-                        const int k = kb + q;
-                        const int kt = k & zmask;
-                        const float Vc = fmaf(g.c[7], (float)kt, Vc0);
-                        const float sv0 = fmaf(Vc, rw, 0.5f);
-                        const float sv = fmaf((float)(k - kt), dsv, sv0);
-                        const float tt = __builtin_amdgcn_fmed3f(sv, 0.0f, Vm1);
-                        const int ivk = (int)tt;
-                        const float fv = __builtin_amdgcn_fractf(tt);
-                        const bool take = k <= k1 && tt == sv && (ivk == iv || ivk == iv - 1);
-                        const float wv = ivk == iv ? 1.0f - fv : fv;
-                        prof = take ? fmaf(wv, xv[q], prof) : prof;
+                            for (int i = 0; i < 4; ++i) {
+                                const int k = kb + 64 * i + lane, kt = k & zmask;
+                                fkt[i] = (float)kt;
+                                fkk[i] = (float)(k - kt);
+                                kval[i] = k < a.Z;
+                                koff[i] = 4u * (unsigned)min(k, a.Z - 1);
+                            }
+                            float xa[4];
+                            fp_load<4>(a.xt, a.Z, cur, koff, xa);
+                            fp_member<4>(col, xa, fkt, fkk, kval, c7v, cur, Vm1, lane);
+                        }
                     }
                 }
-                // detector columns iu0 (weight 1 - fu) and iu0 + 1 (fu) of the block
-                if (iu0 >= u0) mine[iu0 - u0] = fmaf(1.0f - fu, prof, mine[iu0 - u0]);
-                if (iu0 + 1 < u0 + kFpCols) mine[iu0 + 1 - u0] = fmaf(fu, prof, mine[iu0 + 1 - u0]);
             }
         }
-        if (iv < a.V) {
-            float* yr = a.y + ((size_t)v * a.V + iv) * a.W;
-#pragma unroll
-            for (int c = 0; c < kFpCols; ++c) {
-                const int u = u0 + c;
-                if (u < a.W) yr[u] = mine[c] * (a.uw ? a.uw[(size_t)(a.view0 + v) * a.W + u] : 1.0f);
-            }
-        }
+    }
+    __syncthreads();
+    // rows of the nw columns, column fastest
+    const int lg = nw == 8 ? 3 : nw == 4 ? 2 : nw == 2 ? 1 : 0;
+    const int nthreads = 64 * nw;
+    const float* uwv = a.uw ? a.uw + (size_t)(a.view0 + v) * a.W : nullptr;
+    for (int e = lane + 64 * (int)threadIdx.y; e < (a.V << lg); e += nthreads) {
+        const int r = e >> lg, cc = e & (nw - 1), u = c0 + cc;
+        if (u < a.W) a.y[((size_t)v * a.V + r) * a.W + u] = colbuf[cc * rows + r] * (uwv ? uwv[u] : 1.0f);
     }
 }
 
@@ -259,9 +421,20 @@ jh_status jh_fp_project(jh_bp_plan* p, const float* d_vol, float* d_proj, int vi
             fp_transpose_kernel<<<dim3((p->X + 31) / 32, (p->Z + 31) / 32, p->Y), dim3(32, 8), 0, s>>>(
                 d_vol, p->xt.as<float>(), p->X, p->Y, p->Z);
             check_launch("fp_transpose_kernel");
-            // rows in passes of 64 x 8
-            fp_block_kernel<<<dim3((p->W + kFpCols - 1) / kFpCols, nviews), dim3(64, 8), 0, s>>>(a);
-            check_launch("fp_block_kernel");
+            // nw waves (detector columns) per workgroup, each with V + 1 floats of LDS
+            int nw = 8;
+            while (nw > 1 && (size_t)nw * (p->V + 1) * sizeof(float) > 65536) nw /= 2;
+            const size_t lds = (size_t)nw * (p->V + 1) * sizeof(float);
+            require(lds <= 65536, "jh_fp_project: detector too tall for the column projector (V <= 16383)");
+            const dim3 grid((p->W + nw - 1) / nw, nviews), block(64, nw);
+            const bool small = (size_t)p->X * p->Y * p->Z * sizeof(float) < ((size_t)1 << 32);
+            if (!small) fp_column_kernel<0><<<grid, block, lds, s>>>(a);
+            else if (p->Z <= 64) fp_column_kernel<1><<<grid, block, lds, s>>>(a);
+            else if (p->Z <= 128) fp_column_kernel<2><<<grid, block, lds, s>>>(a);
+            else if (p->Z <= 256) fp_column_kernel<4><<<grid, block, lds, s>>>(a);
+            else if (p->Z <= 512) fp_column_kernel<8><<<grid, block, lds, s>>>(a);
+            else fp_column_kernel<0><<<grid, block, lds, s>>>(a);
+            check_launch("fp_column_kernel");
         } else {
             a.xt = nullptr;
             const size_t n = (size_t)nviews * p->V * p->W;

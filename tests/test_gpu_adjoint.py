@@ -72,6 +72,29 @@ def test_dot_product_adjoint256(gpu, weighted):
     assert gap <= 1e-5
 
 
+@pytest.mark.parametrize("weighted", [False, True])
+def test_dot_product_sparse_volume_adjoint256(gpu, weighted):
+    """The dot test with 4096 random voxels set: each detector pixel then sums
+    a few terms, so rounding leaves the gap near 1e-7, and one sample that the
+    projector placed differently from the backprojector (another row or
+    column, or dropped) would show as a gap of about 1e-4."""
+    bp = _plan(ADJ, gpu)
+    rng = np.random.default_rng(41)
+    x = np.zeros((ADJ.Z, ADJ.Y, ADJ.X), np.float32)
+    idx = rng.choice(x.size, 4096, replace=False)
+    x.reshape(-1)[idx] = rng.standard_normal(4096).astype(np.float32)
+    y = torch.from_numpy(synth.normal(26, (ADJ.n_views, ADJ.n_v, ADJ.n_u))).to(gpu)
+    xt = torch.from_numpy(x).to(gpu)
+    Ax = bp.project(xt, weighted=weighted)
+    q = bp.stage(y, "f32") if weighted else bp.pack(y, "f32")
+    By = bp.backproject(q, 0, ADJ.n_views)
+    lhs = (Ax.double() * y.double()).sum().item()
+    rhs = (xt.double() * By.double()).sum().item()
+    gap = abs(lhs - rhs) / max(abs(lhs), abs(rhs))
+    print("sparse dot test weighted", weighted, lhs, rhs, gap)
+    assert gap <= 2e-6
+
+
 def test_projector_matches_cpu_oracle_and_general_modes(gpu):
     from jenhine_amd.recon import BPCore
     w = configs.CBCT128.scaled(n_views=5, n_u=48, n_v=40, X=24, Y=20, Z=14, du=1.0)
