@@ -71,8 +71,9 @@ __device__ inline float bp_su(const Geo& g, float fi, float fj, float& rw)
 // expression (z tiling included).  The rows of one phase are distinct when
 // the sample spacing |dsv| is at least 1 + 2^-8 rows (the margin covers the
 // rounding of the per-tile expressions); below that the lanes take turns, in
-// ceil((1 + 2^-8) / |dsv|) passes.  A wave owns its column, so no atomics are
-// needed and the summation order is fixed.
+// ceil((1 + 2^-8) / |dsv|) passes.  The adds are plain LDS reads and writes
+// (LDS float atomics run lane by lane: 22x slower here).  A wave owns its
+// column, so nothing else touches it and the summation order is fixed.
 constexpr float kFpRowMargin = 1.00390625f;
 
 __device__ __forceinline__ void lds_order()
@@ -128,58 +129,86 @@ __device__ __forceinline__ void fp_load_buf(__amdgpu_buffer_rsrc_t xr, unsigned 
 // Adds NC chunks of 64 samples of one member to the wave's column.  fkt /
 // fkk: each lane's sample z tile start and offset in it, as floats; kval:
 // sample inside the volume; c7v: the V coefficient of z, in a vector
-// register (the other operands of its FMA are scalar).  MULTI: the lanes
-// take turns in `passes` passes (sample spacing below a row).
-template <int NC, bool MULTI>
+// register (the other operands of its FMA are scalar).  FORM 0: every sample
+// of the member lies at least a row inside [0, V - 1] and inside the volume,
+// so no lane is masked; FORM 1: samples tested; FORM 2: as 1, the lanes
+// taking turns in `passes` passes (sample spacing below a row).
+//
+// In forms 0 and 1 the spacing is at least 1 + 2^-8 rows, so the first rows
+// floor(sv_k) of all the member's samples are distinct, and so are the second
+// rows: the chunks' first-row reads issue together, then their writes, then
+// the same for the second rows -- two LDS round trips per member.
+template <int NC, int FORM>
 __device__ __forceinline__ void fp_apply(float* col, const float* xv, const float* fkt, const float* fkk,
                                          const bool* kval, float c7v, const FpMember& m, float dsv, float Vm1,
                                          int lane, int passes)
 {
+    int iv[NC];
+    float lo[NC], hi[NC];
+    bool ok[NC];
 #pragma unroll
     for (int i = 0; i < NC; ++i) {
         const float Vc = fmaf(c7v, fkt[i], m.vc0);
         const float sv0 = fmaf(Vc, m.rw, 0.5f);
         const float sv = fmaf(fkk[i], dsv, sv0);
-        const float tt = __builtin_amdgcn_fmed3f(sv, 0.0f, Vm1);
-        const bool ok = kval[i] && tt == sv;
-        const int iv = (int)tt;
+        const float tt = FORM == 0 ? sv : __builtin_amdgcn_fmed3f(sv, 0.0f, Vm1);
+        ok[i] = FORM == 0 || (kval[i] && tt == sv);
+        iv[i] = (int)tt;
         const float fv = __builtin_amdgcn_fractf(tt);
         const float xw = m.wu * xv[i];
-        const float lo = fmaf(-fv, xw, xw), hi = fv * xw;
-        if constexpr (!MULTI) {
-            // one instruction stream for all lanes: every lane's first-row
-            // update is written before any lane reads its second row
-            if (ok) {
-                col[iv] += lo;
-                lds_order();
-                col[iv + 1] += hi;
-            }
-            lds_order();
-        } else {
+        lo[i] = fmaf(-fv, xw, xw);
+        hi[i] = fv * xw;
+    }
+    if constexpr (FORM < 2) {
+        float o[NC];
+#pragma unroll
+        for (int i = 0; i < NC; ++i)
+            if (ok[i]) o[i] = col[iv[i]];
+#pragma unroll
+        for (int i = 0; i < NC; ++i)
+            if (ok[i]) col[iv[i]] = o[i] + lo[i];
+        lds_order();
+#pragma unroll
+        for (int i = 0; i < NC; ++i)
+            if (ok[i]) o[i] = col[iv[i] + 1];
+#pragma unroll
+        for (int i = 0; i < NC; ++i)
+            if (ok[i]) col[iv[i] + 1] = o[i] + hi[i];
+        lds_order();
+    } else {
+        // one instruction stream for all lanes: a pass's first-row updates
+        // are written before any lane reads its second row
+#pragma unroll
+        for (int i = 0; i < NC; ++i) {
             for (int p = 0; p < passes; ++p) {
-                if (ok && lane % passes == p) col[iv] += lo;
+                if (ok[i] && lane % passes == p) col[iv[i]] += lo[i];
                 lds_order();
             }
             for (int p = 0; p < passes; ++p) {
-                if (ok && lane % passes == p) col[iv + 1] += hi;
+                if (ok[i] && lane % passes == p) col[iv[i] + 1] += hi[i];
                 lds_order();
             }
         }
     }
 }
 
+// single: |dsv| >= 1 + 2^-8; interior: FORM 0 applies (both wave-uniform,
+// from the candidate round's ballots).
 template <int NC>
 __device__ __forceinline__ void fp_member(float* col, const float* xv, const float* fkt, const float* fkk,
-                                          const bool* kval, float c7v, const FpMember& m, float Vm1, int lane)
+                                          const bool* kval, float c7v, const FpMember& m, float Vm1, int lane,
+                                          bool single, bool interior)
 {
     const float dsv = c7v * m.rw;
-    const float adsv = fabsf(dsv);
-    // wave-uniform (the member's values are scalar), made a scalar branch
-    if (__builtin_amdgcn_readfirstlane((int)(adsv >= kFpRowMargin))) {
-        fp_apply<NC, false>(col, xv, fkt, fkk, kval, c7v, m, dsv, Vm1, lane, 1);
+    if (single && interior) {
+        fp_apply<NC, 0>(col, xv, fkt, fkk, kval, c7v, m, dsv, Vm1, lane, 1);
+    } else if (single) {
+        fp_apply<NC, 1>(col, xv, fkt, fkk, kval, c7v, m, dsv, Vm1, lane, 1);
     } else {
-        const int passes = adsv > 0.0f ? min(64, (int)ceilf(kFpRowMargin / adsv)) : 64;
-        fp_apply<NC, true>(col, xv, fkt, fkk, kval, c7v, m, dsv, Vm1, lane, passes);
+        const float adsv = fabsf(dsv);
+        const int passes = __builtin_amdgcn_readfirstlane(
+            adsv > 0.0f ? min(64, (int)ceilf(kFpRowMargin / adsv)) : 64);
+        fp_apply<NC, 2>(col, xv, fkt, fkk, kval, c7v, m, dsv, Vm1, lane, passes);
     }
 }
 
@@ -220,6 +249,11 @@ __global__ void __launch_bounds__(512) fp_column_kernel(FPArgs a)
     const __amdgpu_buffer_rsrc_t xr = __builtin_amdgcn_make_buffer_rsrc(
         (void*)a.xt, (short)0, (int)(unsigned)((size_t)a.X * a.Y * a.Z * sizeof(float)), 0x00020000);
     const unsigned zbytes = 4u * (unsigned)a.Z;
+    // the last sample's z tile start and offset; full: every lane of every
+    // chunk holds a sample
+    const int klast = a.Z - 1;
+    const float ftlast = (float)(klast & zmask), fklast = (float)(klast - (klast & zmask));
+    const bool full = NI > 0 && a.Z == 64 * NI;
     if (c < a.W) {
         // wedge boundaries (c1 - t c9) i + (c2 - t c10) j + (c0 - t c8) = 0
         // at su = t + 0.5 for t = c - 1.5 and t = c + 0.5
@@ -252,6 +286,7 @@ __global__ void __launch_bounds__(512) fp_column_kernel(FPArgs a)
                 if (__ballot(mem) == 0) break;
                 float wu = 0.0f, rw = 0.0f, vc0 = 0.0f;
                 int cix = 0;
+                bool single = false, interior = false;
                 if (mem) {
                     const int i = major_i ? m : n, j = major_i ? n : m;
                     const float fi = (float)i, fj = (float)j;
@@ -263,9 +298,18 @@ __global__ void __launch_bounds__(512) fp_column_kernel(FPArgs a)
                     wu = iu0 == c ? 1.0f - fu : fu;
                     vc0 = fmaf(g.c[5], fi, fmaf(g.c[6], fj, g.c[4]));
                     cix = j * a.X + i;
+                    const float dsv = c7v * rw;
+                    single = fabsf(dsv) >= kFpRowMargin;
+                    // the first and last samples' sv (the samples' own
+                    // expressions), a row inside the detector: sv is linear
+                    // in k up to rounding far below a row
+                    const float svf = fmaf(vc0, rw, 0.5f);
+                    const float svl = fmaf(fklast, dsv, fmaf(fmaf(c7v, ftlast, vc0), rw, 0.5f));
+                    interior = full && fminf(svf, svl) >= 1.0f && fmaxf(svf, svl) <= Vm1 - 1.0f;
                 }
                 unsigned long long mask = __ballot(mem);
                 if (mask == 0) continue;
+                const unsigned long long msingle = __ballot(single), minterior = __ballot(interior);
                 if constexpr (NI > 0) {
                     // two members in flight: the next one's samples load
                     // while the current one is added.  The buffers
@@ -273,29 +317,27 @@ __global__ void __launch_bounds__(512) fp_column_kernel(FPArgs a)
                     // load is issued on every path (after the last member it
                     // reloads that member's column), so the wait for the
                     // current samples never counts the next ones.
-                    int l = __builtin_ctzll(mask);
+                    int la = __builtin_ctzll(mask), lb;
                     mask &= mask - 1;
-                    FpMember ma = fp_pick(cix, wu, rw, vc0, l), mb;
+                    FpMember ma = fp_pick(cix, wu, rw, vc0, la), mb;
                     float xa[NI], xb[NI];
                     fp_load_buf<NI>(xr, (unsigned)ma.col * zbytes, koff, xa);
                     for (;;) {
                         bool more = mask != 0;
-                        if (more) {
-                            l = __builtin_ctzll(mask);
-                            mask &= mask - 1;
-                        }
-                        mb = fp_pick(cix, wu, rw, vc0, l);
+                        lb = more ? __builtin_ctzll(mask) : la;
+                        mask &= mask - 1;
+                        mb = fp_pick(cix, wu, rw, vc0, lb);
                         fp_load_buf<NI>(xr, (unsigned)mb.col * zbytes, koff, xb);
-                        fp_member<NI>(col, xa, fkt, fkk, kval, c7v, ma, Vm1, lane);
+                        fp_member<NI>(col, xa, fkt, fkk, kval, c7v, ma, Vm1, lane, (msingle >> la) & 1,
+                                      (minterior >> la) & 1);
                         if (!more) break;
                         more = mask != 0;
-                        if (more) {
-                            l = __builtin_ctzll(mask);
-                            mask &= mask - 1;
-                        }
-                        ma = fp_pick(cix, wu, rw, vc0, l);
+                        la = more ? __builtin_ctzll(mask) : lb;
+                        mask &= mask - 1;
+                        ma = fp_pick(cix, wu, rw, vc0, la);
                         fp_load_buf<NI>(xr, (unsigned)ma.col * zbytes, koff, xa);
-                        fp_member<NI>(col, xb, fkt, fkk, kval, c7v, mb, Vm1, lane);
+                        fp_member<NI>(col, xb, fkt, fkk, kval, c7v, mb, Vm1, lane, (msingle >> lb) & 1,
+                                      (minterior >> lb) & 1);
                         if (!more) break;
                     }
                 } else {
@@ -314,7 +356,7 @@ __global__ void __launch_bounds__(512) fp_column_kernel(FPArgs a)
                             }
                             float xa[4];
                             fp_load<4>(a.xt, a.Z, cur, koff, xa);
-                            fp_member<4>(col, xa, fkt, fkk, kval, c7v, cur, Vm1, lane);
+                            fp_member<4>(col, xa, fkt, fkk, kval, c7v, cur, Vm1, lane, (msingle >> l) & 1, false);
                         }
                     }
                 }

@@ -135,6 +135,43 @@ def test_projector_row_steps_and_z_tiles(gpu, du, zc):
     assert err <= 1e-5 and gap <= 1e-5
 
 
+@pytest.mark.parametrize("Z,n_v", [(96, 500), (128, 300), (300, 500), (512, 700), (512, 400), (600, 500)])
+def test_projector_column_depths(gpu, Z, n_v):
+    """Column depths across the projector's sample-register forms (Z up to
+    128, 256, 512 per lane set, and the chunked form beyond), whole columns
+    inside the detector (Z 128 and 512 on 300 and 700 rows: no lane masked)
+    and columns partly off it (Z 512 on 400 rows, Z 600 on 500 rows), at
+    1.19 rows per z voxel: by the dot test with the backprojector, by the
+    sparse dot test (exact transposition), and against the CPU oracle's FP64
+    transpose within twice the CPU FP32 restatement's own distance from it --
+    on detectors of 300 to 700 rows the float sample positions alone put
+    both at 1.6e-5 to 4.8e-5, the backprojector's own distance."""
+    from jenhine_amd.recon import BPCore
+    w = configs.CBCT128.scaled(n_views=5, n_u=48, n_v=n_v, X=12, Y=10, Z=Z, pxy=0.5, pz=0.25, du=0.35)
+    x = synth.normal(27, (w.Z, w.Y, w.X))
+    y = synth.normal(28, (w.n_views, w.n_v, w.n_u))
+    bp = BPCore(w.ptms(), w.n_u, w.n_v, w.X, w.Y, w.Z, w.pxy, w.pz, device=gpu)
+    assert bp.geometry_mode() == 0
+    g = bp.project(torch.from_numpy(x).to(gpu)).cpu().numpy()
+    c = o.forward_project(x, w.ptms(), (w.n_v, w.n_u), w.pxy, w.pz, weighted=False)
+    err = np.sqrt(np.mean((g - c) ** 2)) / np.sqrt(np.mean(c ** 2))
+    bty = bp.backproject(bp.pack(torch.from_numpy(y).to(gpu), "f32"), 0, w.n_views).cpu().numpy()
+    lhs, rhs = float((g.astype(np.float64) * y).sum()), float((x.astype(np.float64) * bty).sum())
+    gap = abs(lhs - rhs) / max(abs(lhs), abs(rhs))
+    rng = np.random.default_rng(Z)
+    xs = np.zeros_like(x)
+    xs.reshape(-1)[rng.choice(xs.size, 64, replace=False)] = rng.standard_normal(64).astype(np.float32)
+    gs = bp.project(torch.from_numpy(xs).to(gpu)).cpu().numpy()
+    lhs_s = float((gs.astype(np.float64) * y).sum())
+    rhs_s = float((xs.astype(np.float64) * bty).sum())
+    gap_s = abs(lhs_s - rhs_s) / max(abs(lhs_s), abs(rhs_s))
+    c32 = o.forward_project(x, w.ptms(), (w.n_v, w.n_u), w.pxy, w.pz, weighted=False, fp64=False)
+    err32 = np.sqrt(np.mean((c32 - c) ** 2)) / np.sqrt(np.mean(c ** 2))
+    print("Z", Z, "V", n_v, "projector vs CPU FP64", err, "CPU FP32 vs FP64", err32, "dot gap", gap,
+          "sparse dot gap", gap_s)
+    assert err <= 2 * err32 and gap <= 1e-5 and gap_s <= 2e-6
+
+
 def _loss_setup(w, dev, seed=31, noise=0.01):
     d = w.data(noise=False)
     p = d["lineint"]
