@@ -304,6 +304,19 @@ jh_status jh_bp_reset(jh_bp_plan* plan, jh_stream_t stream);
 jh_status jh_bp_accumulate(jh_bp_plan* plan, const void* d_q, int dtype, int view0, int nviews,
                            jh_stream_t stream);
 
+/* jh_bp_accumulate restricted to slices z0..z0+nz-1: z0 a multiple of the
+ * backprojector's z tile (jh_bp_get_tile), nz too unless the slab ends at Z.
+ * The per-column count sums of geometry class 0 are continued only by the
+ * call whose slab starts at z0 = 0, so a set of views goes into that slab
+ * exactly once, before any slice is finalised.  Each voxel still takes its
+ * views in order, so views accumulated in order -- whole-volume calls first,
+ * then the remaining views slab by slab -- give jh_bp_run's volume to the bit.
+ * A slab can then be finalised and downloaded while the next one is
+ * accumulated.  No counterpart in the reference (BPCUDAAgent accumulates every
+ * view over the whole volume, BPCUDAAgent.cpp:231-268). */
+jh_status jh_bp_accumulate_slab(jh_bp_plan* plan, const void* d_q, int dtype, int view0, int nviews, int z0,
+                                int nz, jh_stream_t stream);
+
 /* Write slices z0..z0+nz-1 of acc/count to d_vol (nz x Y x X f32).
  * Replaces BPCore::DoneSlice (BPCore.cpp:198-205) and
  * BPCUDACore::DeployUpdateOutput + Div2 (BPCUDACore.cu:68-87,177-215). */

@@ -31,7 +31,7 @@ EXPORTS = [
     "jh_detector_on_right", "jh_composite_create", "jh_composite_side", "jh_composite_run", "jh_composite_run_operand",
     "jh_composite_destroy",
     "jh_bp_create", "jh_bp_uweights", "jh_bp_stage_bytes", "jh_bp_stage", "jh_bp_run", "jh_bp_reset",
-    "jh_bp_accumulate", "jh_bp_finalize", "jh_bp_accumulate_noalloc", "jh_bp_set_tile", "jh_bp_get_tile", "jh_bp_geometry_mode",
+    "jh_bp_accumulate", "jh_bp_accumulate_slab", "jh_bp_finalize", "jh_bp_accumulate_noalloc", "jh_bp_set_tile", "jh_bp_get_tile", "jh_bp_geometry_mode",
     "jh_bp_destroy", "jh_bp_pack", "jh_bp_backproject", "jh_fp_project", "jh_bp_autotune",
     "jh_bp_set_fetch", "jh_bp_get_fetch", "jh_bp_fetch_in_use", "jh_bp_set_layout", "jh_bp_get_layout", "jh_bp_set_zfilter",
     "jh_bp_set_ctnum", "jh_bp_set_weight", "jh_bp_get_weight", "jh_bp_set_f16_exponent", "jh_bp_get_f16_exponent",
@@ -93,6 +93,7 @@ _SIGS = {
     "jh_bp_run": (c_int, [P, P, c_int, P, P]),
     "jh_bp_reset": (c_int, [P, P]),
     "jh_bp_accumulate": (c_int, [P, P, c_int, c_int, c_int, P]),
+    "jh_bp_accumulate_slab": (c_int, [P, P, c_int, c_int, c_int, c_int, c_int, P]),
     "jh_bp_finalize": (c_int, [P, P, c_int, c_int, P]),
     "jh_bp_set_tile": (c_int, [P, c_int, c_int]),
     "jh_bp_get_tile": (c_int, [P, P, P]),

@@ -534,6 +534,7 @@ struct BPArgs {
     int W, V, X, Y, Z;
     int view0, nviews;
     int ntx, nty, ntz;
+    int tz0;             // first z tile of the launch (z-slab launches)
     uint32_t frame_bytes;
     int ct;              // apply the CT-number map to normalised output
     float ct_scale, ct_norm1;
@@ -634,7 +635,7 @@ __device__ __forceinline__ void bp_tile(const BPArgs& a, unsigned tile)
     const int tx = tile % a.ntx;
     const unsigned rest = tile / a.ntx;
     const int tyi = rest % a.nty;
-    const int tz = rest / a.nty;
+    const int tz = a.tz0 + (int)(rest / a.nty);
     // Each wave covers an 8 x 8 patch of voxel columns; the blockDim.y waves
     // of a workgroup tile 8 x 8 (1 wave), 16 x 8 (2) or 16 x 16 (4).  A wave's
     // 64 samples of one view then fall on a compact detector footprint: a
@@ -925,7 +926,7 @@ __device__ __forceinline__ void bp_tile_lds(const BPArgs& a, unsigned tile)
     const int tx = tile % a.ntx;
     const unsigned rest = tile / a.ntx;
     const int tyi = rest % a.nty;
-    const int tz = rest / a.nty;
+    const int tz = a.tz0 + (int)(rest / a.nty);
     const int px = blockDim.y >= 2 ? 2 : 1;
     const int wx = threadIdx.y % px, wy = threadIdx.y / px;
     const int ex = px * 8, ey = (blockDim.y / px) * 8;
@@ -1351,7 +1352,7 @@ __device__ __forceinline__ void bp_tile_strip_lds(const BPArgs& a, unsigned tile
     const int tx = tile % a.ntx;
     const unsigned rest = tile / a.ntx;
     const int tyi = rest % a.nty;
-    const int tz = rest / a.nty;
+    const int tz = a.tz0 + (int)(rest / a.nty);
     // 16 x 16 voxel columns: four 8 x 8 waves (as bp_tile with four waves)
     const int wx = threadIdx.y & 1, wy = threadIdx.y >> 1;
     const int i = tx * 16 + wx * 8 + (threadIdx.x & 7);
@@ -1370,7 +1371,8 @@ __device__ __forceinline__ void bp_tile_strip_lds(const BPArgs& a, unsigned tile
     unsigned char* const lbuf = reinterpret_cast<unsigned char*>(lds_dyn);
     const int tid = threadIdx.y * 64 + threadIdx.x;
     const __attribute__((address_space(4))) uint32_t* boxes =
-        (const __attribute__((address_space(4))) uint32_t*)a.sbox + (size_t)tile * a.nbox_views + a.view0;
+        (const __attribute__((address_space(4))) uint32_t*)a.sbox +
+        ((size_t)tile + (size_t)a.tz0 * a.ntx * a.nty) * a.nbox_views + a.view0;
 
     // this thread's chunks c = tid + 256 r of the rectangle, row c / stride,
     // chunk c % stride (chunks past the rectangle copy its last chunk again,
@@ -1733,9 +1735,12 @@ static size_t stage_frame_bytes(const jh_bp_plan* p, int dtype)
 }
 
 // acc, cnt, colsum: the streaming sums (default: the plan's).
+// tz0, ntz: the launch's z tiles (a z-slab of the streaming forms; the column
+// count sums are continued only by the slab that starts at z = 0); ntz < 0:
+// every tile from tz0 on.
 static void launch_bp(jh_bp_plan* p, const void* q, int dtype, int view0, int nviews, float* vol,
                       int out, hipStream_t stream, float* acc = nullptr, float* cnt = nullptr,
-                      float* colsum = nullptr)
+                      float* colsum = nullptr, int tz0 = 0, int ntz = -1)
 {
     BPArgs a;
     a.q = q;
@@ -1756,7 +1761,12 @@ static void launch_bp(jh_bp_plan* p, const void* q, int dtype, int view0, int nv
     const int ex = p->ty >= 2 ? 16 : 8, ey = p->ty >= 4 ? 16 : 8;
     a.ntx = (p->X + ex - 1) / ex;
     a.nty = (p->Y + ey - 1) / ey;
-    a.ntz = (p->Z + p->zc - 1) / p->zc;
+    const int ntz_all = (p->Z + p->zc - 1) / p->zc;
+    require(tz0 >= 0 && tz0 < ntz_all && (ntz < 0 || tz0 + ntz <= ntz_all), "backprojection: z tiles outside the volume");
+    a.tz0 = tz0;
+    a.ntz = ntz < 0 ? ntz_all - tz0 : ntz;
+    const bool slab = a.ntz < ntz_all;
+    require(!slab || out == kOutAccumulate || out == kOutFdkAccumulate, "backprojection: z-slabs are streaming only");
     const size_t fb = stage_frame_bytes(p, dtype);
     a.pxb = stage_pixel_bytes(p, dtype);
     a.row_px = p->W + (stage_strip(p, dtype) ? 1 : 0);
@@ -1793,9 +1803,10 @@ static void launch_bp(jh_bp_plan* p, const void* q, int dtype, int view0, int nv
     // (two rectangle buffers of kStripRounds(zc) x 256 chunks of 16 bytes)
     const size_t lds_bytes = strip_lds ? 2 * (size_t)kStripRounds(p->zc) * 256 * 16
                              : staged  ? 2 * (size_t)a.lds_px * sizeof(float4) : 0;
-    if (p->mode == 0 && (out == kOutNormalised || out == kOutAccumulate)) {
+    if (p->mode == 0 && (out == kOutNormalised || (out == kOutAccumulate && tz0 == 0))) {
         // the count weights of the launch's views per voxel column; the
-        // streaming form continues the running sums
+        // streaming form continues the running sums (once per view: in the
+        // launch whose tiles start at z = 0)
         float* cc = out == kOutAccumulate ? (colsum ? colsum : p->colsum.as<float>()) : p->colcnt.as<float>();
         const int cont = out == kOutAccumulate ? 1 : 0;
         const dim3 cg((unsigned)((p->X + 63) / 64), (unsigned)((p->Y + 3) / 4));
@@ -2025,6 +2036,31 @@ jh_status jh_bp_accumulate(jh_bp_plan* p, const void* d_q, int dtype, int view0,
         DeviceScope ds(p->device);
         launch_bp(p, d_q, dtype, view0, nviews, nullptr,
                   p->weight == JH_WEIGHT_FDK ? kOutFdkAccumulate : kOutAccumulate, as_stream(stream));
+    });
+}
+
+jh_status jh_bp_accumulate_slab(jh_bp_plan* p, const void* d_q, int dtype, int view0, int nviews, int z0, int nz,
+                                jh_stream_t stream)
+{
+    return guard([&] {
+        require(p != nullptr && d_q != nullptr, "jh_bp_accumulate_slab: bad arguments");
+        require(dtype == JH_F32 || dtype == JH_BF16 || dtype == JH_F16, "jh_bp_accumulate_slab: unknown dtype");
+        require(view0 >= 0 && nviews >= 0 && view0 + nviews <= p->N,
+                "jh_bp_accumulate_slab: view range outside the scan");
+        require(z0 >= 0 && nz > 0 && z0 + nz <= p->Z, "jh_bp_accumulate_slab: slice range outside the volume");
+        require(z0 % p->zc == 0 && (nz % p->zc == 0 || z0 + nz == p->Z),
+                "jh_bp_accumulate_slab: slab not aligned to the backprojector's z tile (jh_bp_get_tile)");
+        require(p->acc.get() != nullptr, "jh_bp_accumulate_slab: call jh_bp_reset first");
+        if (nviews == 0) return;
+        const float sc = dtype == JH_F16 ? std::ldexp(1.0f, p->f16exp) : 1.0f;
+        require(p->acc_dtype < 0 || sc == p->acc_scale,
+                "jh_bp_accumulate_slab: chunks of differently scaled storage since the last reset");
+        p->acc_scale = sc;
+        p->acc_dtype = dtype;
+        DeviceScope ds(p->device);
+        launch_bp(p, d_q, dtype, view0, nviews, nullptr,
+                  p->weight == JH_WEIGHT_FDK ? kOutFdkAccumulate : kOutAccumulate, as_stream(stream), nullptr,
+                  nullptr, nullptr, z0 / p->zc, (nz + p->zc - 1) / p->zc);
     });
 }
 

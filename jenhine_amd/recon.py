@@ -476,9 +476,14 @@ class BPCore:
     def reset(self):
         call("jh_bp_reset", self._h, _stream())
 
-    def accumulate(self, q, dtype, view0, n_views):
+    def accumulate(self, q, dtype, view0, n_views, z0=None, nz=None):
+        """Streaming accumulation of views view0.. into the whole volume, or
+        (z0, nz given) into slices z0..z0+nz-1 only (jh_bp_accumulate_slab)."""
         _require_cuda(q, "q")
-        call("jh_bp_accumulate", self._h, _ptr(q), DTYPES[dtype], view0, n_views, _stream())
+        if z0 is None:
+            call("jh_bp_accumulate", self._h, _ptr(q), DTYPES[dtype], view0, n_views, _stream())
+        else:
+            call("jh_bp_accumulate_slab", self._h, _ptr(q), DTYPES[dtype], view0, n_views, z0, nz, _stream())
 
     def finalize(self, z0=0, nz=None, out=None):
         nz = self.Z - z0 if nz is None else nz

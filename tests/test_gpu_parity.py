@@ -282,6 +282,73 @@ def test_bp_streaming_equals_one_shot(gpu, dtype, fetch, weight):
         np.testing.assert_array_equal(_streamed(bp, q, dtype, sizes), one, err_msg=str(sizes))
 
 
+def _slab_tail(bp, q, dtype, head, chunk, slab):
+    """The first `head` views streamed over the whole volume in launches of
+    `chunk`, the rest slab by slab (jh_bp_accumulate_slab), each slab
+    finalised before the next one is accumulated."""
+    n = bp.n_views
+    fb = q.numel() // n
+    bp.reset()
+    for v0 in range(0, head, chunk):
+        m = min(chunk, head - v0)
+        bp.accumulate(q[v0 * fb:(v0 + m) * fb], dtype, v0, m)
+    out = []
+    for z0 in range(0, bp.Z, slab):
+        nz = min(slab, bp.Z - z0)
+        if head < n:
+            bp.accumulate(q[head * fb:], dtype, head, n - head, z0=z0, nz=nz)
+        out.append(bp.finalize(z0, nz).cpu().numpy())
+    return np.concatenate(out)
+
+
+@pytest.mark.parametrize("weight", ["count", "fdk"])
+@pytest.mark.parametrize("fetch", ["gather", "lds"])
+@pytest.mark.parametrize("dtype", ["f32", "bf16", "f16"])
+def test_bp_slab_tail_equals_one_shot(gpu, dtype, fetch, weight):
+    """Views streamed over the whole volume, then the remaining views z-slab
+    by z-slab with each slab finalised in between (the agent's download
+    overlap): every voxel still takes its views in order and the column count
+    sums take each view once, so the volume is the one-launch volume to the
+    bit -- slabs of one and two z tiles and a partial last slab, no streamed
+    head, all views in the head."""
+    from jenhine_amd.recon import BPCore
+    w = SMALL
+    filt = torch.from_numpy(synth.normal(14, (w.n_views, w.n_v, w.n_u))).to(gpu)
+    bp = BPCore(w.ptms(), w.n_u, w.n_v, w.X, w.Y, w.Z, w.pxy, w.pz, device=gpu)
+    assert bp.geometry_mode() == 0
+    bp.set_tile(8, 4)
+    bp.set_fetch(fetch)
+    if weight == "fdk":
+        bp.set_weight("fdk")
+    q = bp.stage(filt, dtype)
+    one = bp.run(q, dtype).cpu().numpy()
+    n = w.n_views
+    for head, chunk, slab in ((n // 2, 5, 8), (7, 7, 16), (0, 1, 8), (n, 6, 8), (n - 1, 4, 24)):
+        got = _slab_tail(bp, q, dtype, head, chunk, slab)
+        np.testing.assert_array_equal(got, one, err_msg=f"head {head} chunk {chunk} slab {slab}")
+
+
+@pytest.mark.parametrize("dtype", ["f32", "bf16", "f16"])
+def test_bp_slab_tail_general_geometry(gpu, dtype):
+    """The slab tail in geometry classes 1 and 2 (count summed per sample)."""
+    from jenhine_amd.recon import BPCore
+    P = np.array([-1.33897, -7.81194, 0.01236, 575.79443, -0.97515, -0.03862, 7.51861, 346.26642,
+                  -0.00226, -0.00004, 0.0, 1.0], np.float64)
+    tilt = P.copy()
+    tilt[10] = 0.0003
+    for base, mode in ((P, 1), (tilt, 2)):
+        ptm = np.stack([base * (1.0 + 1e-4 * k) for k in range(6)])
+        filt = torch.from_numpy(synth.normal(13, (6, 686, 644))).to(gpu)
+        bp = BPCore(ptm, 644, 686, 96, 80, 20, 0.25, 1.0, device=gpu)
+        assert bp.geometry_mode() == mode
+        zc, _ = bp.tile()
+        q = bp.stage(filt, dtype)
+        one = bp.run(q, dtype).cpu().numpy()
+        for head in (0, 2, 5):
+            got = _slab_tail(bp, q, dtype, head, 2, zc)
+            np.testing.assert_array_equal(got, one, err_msg=f"mode {mode} head {head}")
+
+
 @pytest.mark.parametrize("dtype", ["f32", "bf16", "f16"])
 def test_bp_streaming_equals_one_shot_general_geometry(gpu, dtype):
     """As above in geometry classes 1 (u depends on z) and 2 (w too), where
