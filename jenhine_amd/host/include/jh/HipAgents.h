@@ -322,6 +322,14 @@ private:
 // slices at the shot end.  With numParts > 1 the volume is split into
 // z-slabs (the reference's "recon parts", BPCUDACore.cpp:32-38): shot p of
 // the input stream is back-projected into slab p % numParts.
+//
+// The first half of a shot's views (whole chunks) is accumulated over the
+// whole volume as it arrives; the rest is staged and held.  At the shot end
+// the held views are back-projected z-slab by z-slab (jh_bp_accumulate_slab),
+// and each slab's slices are finalised and handed on before the next slab
+// runs, so the download of the volume overlaps the last half of the
+// backprojection.  Every voxel still takes its views in order: the volume is
+// the one-launch volume to the bit.
 class BPHIPAgent : public SequentialAgentBase {
 public:
     BPHIPAgent(const ScanParams& scan, const ReconParams& recon, const std::vector<double>& ptms,
@@ -337,6 +345,8 @@ private:
     void SetPipesImpl() override;
     void WorkFlow0() override;
     jh_bp_plan* PlanFor(size_t part);
+    // back-projects the held views slab by slab and writes the part's slices
+    void FinishShot(jh_bp_plan* plan, size_t part, size_t held);
     ScanParams scan_;
     ReconParams recon_;
     std::vector<double> ptms_;
@@ -345,6 +355,9 @@ private:
     size_t part_ = SIZE_MAX;
     jh_bp_plan* plan_ = nullptr;
     hip::DeviceMemory<uint8_t> q_;
+    size_t tail_ = 0;                   // first held view of a shot
+    size_t frameBytes_ = 0;             // staged bytes per view
+    hip::DeviceMemory<uint8_t> tailQ_;  // the held views, staged
     std::shared_ptr<Pipe<hip::DeviceMemory<float>>> in_, out_;
     StreamHolder stream_;
 };

@@ -442,6 +442,8 @@ size_t BPHIPAgent::PartSlices(size_t p) const
 
 // slices finalised per launch (one contiguous write token)
 constexpr size_t kSliceChunk = 16;
+// slices per z-slab of the held views (rounded to the z tile)
+constexpr size_t kSlabSlices = 64;
 
 void BPHIPAgent::SetPipesImpl()
 {
@@ -453,6 +455,11 @@ void BPHIPAgent::SetPipesImpl()
                       {recon_.NumPixelsX, recon_.NumPixelsY});
     out_->SetBufferGranule(kSliceChunk);
     out_->SetProducer(GetAgentName(), 1, kSliceChunk);
+    // room for four slabs of slices, so that the next slab is back-projected
+    // while the download drains the last ones (the default ring, one token
+    // per writer and reader, would hold the agent to the download's pace)
+    const size_t slices = (PartSlices(0) + kSliceChunk - 1) / kSliceChunk * kSliceChunk;
+    out_->SetBufferSize(std::max<size_t>(2 * kSliceChunk, std::min<size_t>(slices, 4 * kSlabSlices)));
     MergePipeToTrunk(out_);
 }
 
@@ -497,50 +504,86 @@ void BPHIPAgent::GetReady()
 {
     CheckHip(hipSetDevice(device_), "hipSetDevice");
     PlanFor(0);
+    const size_t nv = ptms_.size() / 12;
     size_t bytes = 0;
     CheckJh(jh_bp_stage_bytes(plan_, recon_.StorageType, (int)recon_.ViewsPerChunk, &bytes), "jh_bp_stage_bytes");
     q_ = hip::DeviceMemory<uint8_t>(bytes);
+    CheckJh(jh_bp_stage_bytes(plan_, recon_.StorageType, 1, &frameBytes_), "jh_bp_stage_bytes");
+    // the views from the chunk boundary nearest below half the shot on are held
+    tail_ = nv / 2 / recon_.ViewsPerChunk * recon_.ViewsPerChunk;
+    tailQ_ = hip::DeviceMemory<uint8_t>((nv - tail_) * frameBytes_);
+}
+
+void BPHIPAgent::FinishShot(jh_bp_plan* plan, size_t part, size_t held)
+{
+    const size_t nz = PartSlices(part);
+    int zc = 0, ty = 0;
+    CheckJh(jh_bp_get_tile(plan, &zc, &ty), "jh_bp_get_tile");
+    const size_t slab = held ? std::max<size_t>((size_t)zc, kSlabSlices / (size_t)zc * (size_t)zc) : nz;
+    for (size_t z0 = 0; z0 < nz; z0 += slab) {
+        const size_t m = std::min(slab, nz - z0);
+        if (held)
+            CheckJh(jh_bp_accumulate_slab(plan, tailQ_.Data(), recon_.StorageType, (int)tail_, (int)held, (int)z0,
+                                          (int)m, stream_.Get()),
+                    "jh_bp_accumulate_slab");
+        for (size_t z = z0; z < z0 + m; z += kSliceChunk) {
+            const size_t k = std::min(kSliceChunk, z0 + m - z);
+            auto wt = out_->GetWriteToken(k, z + k == nz);
+            if (Contiguous(wt.GetBufferPtrs(), k)) {
+                CheckJh(jh_bp_finalize(plan, wt.GetBuffer(0).Data(), (int)z, (int)k, stream_.Get()), "jh_bp_finalize");
+            } else {
+                for (size_t i = 0; i < k; ++i)
+                    CheckJh(jh_bp_finalize(plan, wt.GetBuffer(i).Data(), (int)(z + i), 1, stream_.Get()),
+                            "jh_bp_finalize");
+            }
+            stream_.Sync();
+        }
+    }
 }
 
 void BPHIPAgent::WorkFlow0()
 {
     CheckHip(hipSetDevice(device_), "hipSetDevice");
     const size_t nv = ptms_.size() / 12;
-    size_t frameBytes = 0;
-    size_t shot = 0;
+    const int dt = recon_.StorageType;
+    size_t shot = 0, held = 0;
     try {
         for (;;) {
             auto rt = in_->GetReadToken();
             ThreadOccupiedScope occ(this);
             jh_bp_plan* plan = PlanFor(shot % numParts_);
-            if (!frameBytes) CheckJh(jh_bp_stage_bytes(plan, recon_.StorageType, 1, &frameBytes), "stage bytes");
-            if (rt.IsShotStart()) CheckJh(jh_bp_reset(plan, stream_.Get()), "jh_bp_reset");
-            // stage the chunk's views contiguously (one launch when the
-            // frames are contiguous and their views consecutive), then
-            // accumulate runs of consecutive view indices
+            if (rt.IsShotStart()) {
+                CheckJh(jh_bp_reset(plan, stream_.Get()), "jh_bp_reset");
+                held = 0;
+            }
             const size_t n = rt.GetSize();
-            const int view0 = (int)(rt.GetStartIndex() % nv);
-            const bool whole = view0 + n <= nv && Contiguous(rt.GetBufferPtrs(), n);
-            if (whole)
-                CheckJh(jh_bp_stage(plan, rt.GetBuffer(0).Data(), view0, (int)n, recon_.StorageType, q_.Data(),
-                                    stream_.Get()),
-                        "jh_bp_stage");
-            size_t runStart = 0;
-            for (size_t i = 0; i < n; ++i) {
-                const int view = (int)((rt.GetStartIndex() + i) % nv);
-                if (!whole)
-                    CheckJh(jh_bp_stage(plan, rt.GetBuffer(i).Data(), view, 1, recon_.StorageType,
-                                        q_.Data() + i * frameBytes, stream_.Get()),
+            const size_t first = rt.GetStartIndex() % nv;
+            if (first + n > nv) ThrowException("BPHIPAgent: a chunk spans two shots");
+            // stages the token's frames [i0, i0 + m) (views first + i0 ..)
+            // to dst: one launch when the frames are contiguous
+            auto stage = [&](size_t i0, size_t m, uint8_t* dst) {
+                const int v = (int)(first + i0);
+                if (Contiguous(rt.GetBufferPtrs() + i0, m)) {
+                    CheckJh(jh_bp_stage(plan, rt.GetBuffer(i0).Data(), v, (int)m, dt, dst, stream_.Get()),
                             "jh_bp_stage");
-                const bool last = i + 1 == n;
-                const bool wraps = !last && (int)((rt.GetStartIndex() + i + 1) % nv) != view + 1;
-                if (last || wraps) {
-                    const int v0 = (int)((rt.GetStartIndex() + runStart) % nv);
-                    CheckJh(jh_bp_accumulate(plan, q_.Data() + runStart * frameBytes, recon_.StorageType, v0,
-                                             (int)(i + 1 - runStart), stream_.Get()),
-                            "jh_bp_accumulate");
-                    runStart = i + 1;
+                } else {
+                    for (size_t i = 0; i < m; ++i)
+                        CheckJh(jh_bp_stage(plan, rt.GetBuffer(i0 + i).Data(), v + (int)i, 1, dt,
+                                            dst + i * frameBytes_, stream_.Get()),
+                                "jh_bp_stage");
                 }
+            };
+            // views before tail_ go into the whole volume now, the others are held
+            const size_t head = first < tail_ ? std::min(n, tail_ - first) : 0;
+            if (head) {
+                stage(0, head, q_.Data());
+                CheckJh(jh_bp_accumulate(plan, q_.Data(), dt, (int)first, (int)head, stream_.Get()),
+                        "jh_bp_accumulate");
+            }
+            if (head < n) {
+                if (first + head != tail_ + held) ThrowException("BPHIPAgent: held views out of order");
+                stage(head, n - head, tailQ_.Data() + held * frameBytes_);
+                held += n - head;
             }
             {
                 WaitAsyncScope wait(&occ);
@@ -548,20 +591,8 @@ void BPHIPAgent::WorkFlow0()
             }
             if (rt.IsShotEnd()) {
                 rt.Reset();  // views are no longer needed
-                const size_t nz = PartSlices(shot % numParts_);
-                for (size_t z = 0; z < nz; z += kSliceChunk) {
-                    const size_t m = std::min(kSliceChunk, nz - z);
-                    auto wt = out_->GetWriteToken(m, z + m == nz);
-                    if (Contiguous(wt.GetBufferPtrs(), m)) {
-                        CheckJh(jh_bp_finalize(plan, wt.GetBuffer(0).Data(), (int)z, (int)m, stream_.Get()),
-                                "jh_bp_finalize");
-                    } else {
-                        for (size_t i = 0; i < m; ++i)
-                            CheckJh(jh_bp_finalize(plan, wt.GetBuffer(i).Data(), (int)(z + i), 1, stream_.Get()),
-                                    "jh_bp_finalize");
-                    }
-                    stream_.Sync();
-                }
+                FinishShot(plan, shot % numParts_, held);
+                held = 0;
                 ++shot;
             }
         }
