@@ -111,11 +111,11 @@ def test_projector_matches_cpu_oracle_and_general_modes(gpu):
 
 @pytest.mark.parametrize("du,zc", [(1.0, 8), (0.25, 16), (2.2, 32)])
 def test_projector_row_steps_and_z_tiles(gpu, du, zc):
-    """The block projector's sample range per detector row is taken from the
-    row step: 0.38 rows per z voxel (du 2.2), 0.83 (du 1.0) and 3.3 (du
-    0.25), each with another
-    z tile of the backprojector (the sample positions follow its tiling):
-    against the CPU oracle's transpose and by the dot test with the
+    """Row steps on both sides of one row per z voxel: 0.38 (du 2.2) and 0.83
+    (du 1.0), where the projector's lanes take turns adding to the column
+    (several samples per row), and 3.3 (du 0.25), each with another z tile
+    of the backprojector (the sample positions follow its tiling): against
+    the CPU oracle's transpose and by the dot test with the
     backprojector."""
     from jenhine_amd.recon import BPCore
     w = configs.CBCT128.scaled(n_views=7, n_u=64, n_v=96, X=28, Y=24, Z=40, pxy=0.5, pz=0.5, du=du)
