@@ -204,6 +204,15 @@ jh_status jh_filter_run_operand_staged(jh_filter_plan* plan, jh_bp_plan* bp, con
  * counterpart in the reference. */
 jh_status jh_filter_output_bound(const jh_filter_plan* plan, double inBound, double* outBound);
 
+/* Split-FP16 form (JH_FILTER_MFMA_F16X3): the filter matrix operand read from
+ * its Toeplitz diagonals kept in LDS (enable = 1; the default where W is a
+ * multiple of 32 and the output width of 128) or copied from the packed
+ * matrix per k step (enable = 0).  Both give the same result to the bit.
+ * Enabling fails where the plan has no generated operand.  No counterpart in
+ * the reference (FilterCore filters by FFT, FilterCore.cpp:189-222). */
+jh_status jh_filter_set_generated(jh_filter_plan* plan, int enable);
+jh_status jh_filter_get_generated(jh_filter_plan* plan, int* enabled);
+
 jh_status jh_filter_destroy(jh_filter_plan* plan);
 
 /* ---------------------------------------------------------------------- */

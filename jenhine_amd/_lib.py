@@ -27,7 +27,7 @@ EXPORTS = [
     "jh_pre_create", "jh_pre_run", "jh_pre_output_bound", "jh_pre_destroy",
     "jh_filter_create", "jh_filter_create_ex", "jh_filter_shape", "jh_filter_response", "jh_filter_preweight",
     "jh_filter_run", "jh_filter_stage_fusable", "jh_filter_run_staged", "jh_filter_operand_bytes",
-    "jh_pre_run_operand", "jh_filter_run_operand_staged", "jh_filter_output_bound", "jh_filter_destroy",
+    "jh_pre_run_operand", "jh_filter_run_operand_staged", "jh_filter_output_bound", "jh_filter_set_generated", "jh_filter_get_generated", "jh_filter_destroy",
     "jh_detector_on_right", "jh_composite_create", "jh_composite_side", "jh_composite_run", "jh_composite_run_operand",
     "jh_composite_destroy",
     "jh_bp_create", "jh_bp_uweights", "jh_bp_stage_bytes", "jh_bp_stage", "jh_bp_run", "jh_bp_reset",
@@ -79,6 +79,8 @@ _SIGS = {
     "jh_filter_operand_bytes": (c_int, [P, c_int, P]),
     "jh_pre_run_operand": (c_int, [P, P, P, P, P, c_int, c_int, P, P]),
     "jh_filter_run_operand_staged": (c_int, [P, P, P, c_int, c_int, c_int, P, P]),
+    "jh_filter_set_generated": (c_int, [P, c_int]),
+    "jh_filter_get_generated": (c_int, [P, P]),
     "jh_filter_destroy": (c_int, [P]),
     "jh_detector_on_right": (c_int, [P, c_int]),
     "jh_composite_create": (c_int, [P, c_int, c_int, c_int, c_int, P, c_float, c_int]),

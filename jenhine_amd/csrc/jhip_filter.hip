@@ -124,6 +124,10 @@ struct FiltArgs {
     int tiled;           // x is the pre-weighted A operand in the MFMA tile order (pre_operand_kernel)
     int bstride;         // float4s per 16-row block of that operand
     float bscale;        // split-FP16 form: factor of the packed filter matrix (a power of two)
+    // generated split-FP16 operand (see filter_mfma_kernel)
+    const unsigned short* Bgen;
+    const unsigned short* R3;
+    int gen_stride, gen_window, gen_nT;
 };
 
 // Element (r, m) of the A operand.
@@ -444,6 +448,8 @@ __device__ inline uint32_t round16(float x, int qtype)
 }
 
 constexpr int kStagePitch = 136;  // LDS row pitch (16-bit values) of the staged tile: 4 rows apart = 16 banks
+// LDS of the generated-operand launch at most: three workgroups per CU
+constexpr size_t kGenMaxLds = 53248;
 
 // Fused staging epilogue of filter_mfma_kernel: the workgroup's 128 x 128
 // tile of filtered values, times the backprojector's uW and rounded to 16
@@ -579,9 +585,24 @@ __device__ __forceinline__ void wait_lds_copies() { __builtin_amdgcn_s_waitcnt(0
 
 // TILED: A is the pre-weighted operand (a.tiled); the split-BF16 forms then
 // load the next k step's A while the current one computes.
-template <int MODE, bool TILED>
+//
+// GEN (split-FP16 form, W a multiple of 32, Wout of 128): the B operand is
+// not copied from the packed matrix step by step but read from the Toeplitz
+// diagonals kept in LDS.  B[k][n] = d(n + off - k) for k < W, so lane l's 8
+// values of k step s and column block c are d at 8 consecutive descending
+// arguments, i.e. 8 consecutive entries of the reversed diagonal, starting at
+// i0 = 32 s + 127 - 16 c - (l % 16) + 8 (l / 16) within the workgroup's
+// window.  i0 mod 8 = (7 - l) mod 8 is fixed per lane, so of 8 copies of the
+// window, shifted by 0..7 entries, copy (7 - l) mod 8 serves lane l with
+// one aligned 16-byte read per part (the copies padded 8 banks apart).  The copies hold the same FP16 hi / lo
+// values as the packed matrix, so the product is the packed form's to the
+// bit.  The last k step (rows W .. W + 31) holds the edge rows r / 3 at
+// k = W .. W + 2 and zeros.  Each workgroup reads (W + 128) x 32 bytes of
+// copies once instead of 16 KB per k step, and the k loop has no barrier.
+template <int MODE, bool TILED, bool GEN = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) filter_mfma_kernel(FiltArgs a)
 {
+    static_assert(!GEN || MODE == kModeH3, "the generated operand is the split-FP16 form's");
     typedef FiltMode<MODE> M;
     constexpr int P = M::parts;
     constexpr int LB = M::lane_bytes;                 // B bytes per lane per column block
@@ -592,7 +613,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) f
     // (global_load_lds, no registers); after the last step, the staged tile
     // of the fused epilogue (128 rows at kStagePitch 16-bit values)
     constexpr int LDS_BYTES = 2 * TILE_BYTES > 128 * kStagePitch * 2 ? 2 * TILE_BYTES : 128 * kStagePitch * 2;
-    __shared__ __attribute__((aligned(16))) unsigned char lds[LDS_BYTES];
+    __shared__ __attribute__((aligned(16))) unsigned char lds_static[GEN ? 16 : LDS_BYTES];
+    // GEN: the copies of the diagonals, then the edge rows, then (after the
+    // k loop) the staged tile, sized by the launch
+    extern __shared__ __attribute__((aligned(16))) unsigned char filt_lds_dyn[];
+    unsigned char* const lds = GEN ? filt_lds_dyn : lds_static;
     constexpr bool H16 = MODE == kModeH3;
     __shared__ float rowfac[H16 ? 128 : 1];  // split-FP16: the result factor of each row
 
@@ -673,9 +698,34 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) f
         for (int c = 0; c < CHUNKS; ++c)
             copy16_to_lds(src + c * 256, lds + buf * TILE_BYTES + (c * 256 + (threadIdx.x & ~63u)) * 16);
     };
-    copy_b(0, 0);
-    wait_lds_copies();
-    __syncthreads();
+    // LDS copies: part p (hi, lo) of shift sg at p * 8 gS + sg * gS halves.
+    // gS = the window + 16 halves: the window (W + 128 halves) is a multiple
+    // of 64 banks, and lanes 0..7 of a 16-byte read -- one per shift, at the
+    // same offset -- would all start on one bank; 8 banks apart they do not
+    // (3x fewer bank-conflict cycles, measured)
+    const int gL = a.gen_window;                 // halves per copy a workgroup needs
+    const int gS = gL + 16;                      // halves between copies
+    const unsigned short* const gl = reinterpret_cast<const unsigned short*>(lds);
+    const int gsig = (7 - lane) & 7;             // the copy serving this lane
+    if constexpr (GEN) {
+        // this column tile's window of the 16 copies (8 shifts x hi, lo), and
+        // its 128 columns of the edge rows
+        unsigned short* const glw = reinterpret_cast<unsigned short*>(lds);
+        const int i0g = a.Wout - 128 * (ctile + 1);
+        const int per = gL / 8;
+        for (int idx = threadIdx.x; idx < 16 * per; idx += 256) {
+            const int cp = idx / per, t8 = idx - cp * per;  // cp = 2 sg + p in the global layout
+            *reinterpret_cast<u32x4_t*>(glw + (cp & 1) * 8 * gS + (cp >> 1) * gS + 8 * t8) =
+                *reinterpret_cast<const u32x4_t*>(a.Bgen + (size_t)cp * a.gen_stride + i0g + 8 * t8);
+        }
+        for (int idx = threadIdx.x; idx < 256; idx += 256)
+            glw[16 * gS + idx] = a.R3[(size_t)(idx >> 7) * a.Wout + 128 * ctile + (idx & 127)];
+        __syncthreads();
+    } else {
+        copy_b(0, 0);
+        wait_lds_copies();
+        __syncthreads();
+    }
 
     // the 8 A values lane supplies for row block rb at k step s: formed from
     // the preprocessed frame and the pre-weight map, or read from the
@@ -759,11 +809,29 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) f
                     ap[rb][1] = __builtin_bit_cast(f16x8_t, (u32x4_t){lo[0], lo[1], lo[2], lo[3]});
                 }
             }
-            copy_b(min(s + 1, nsteps - 1), (s + 1) & 1);
+            if constexpr (!GEN) copy_b(min(s + 1, nsteps - 1), (s + 1) & 1);
 #pragma unroll
             for (int c = 0; c < 8; ++c) {
-                const f16x8_t* bp = reinterpret_cast<const f16x8_t*>(bt + c * 64 * LB) + lane;
-                const f16x8_t b0 = bp[0], b1 = bp[64];
+                f16x8_t b0, b1;
+                if constexpr (GEN) {
+                    if (s < a.gen_nT) {
+                        const int t = 32 * s + 127 - 16 * c - (lane & 15) + 8 * (lane >> 4) - gsig;
+                        b0 = *reinterpret_cast<const f16x8_t*>(gl + gsig * gS + t);
+                        b1 = *reinterpret_cast<const f16x8_t*>(gl + 8 * gS + gsig * gS + t);
+                    } else {
+                        // rows W .. W + 31: lanes 0..15 hold k = W .. W + 7, of
+                        // which W .. W + 2 carry r / 3; everything else is 0
+                        const bool edge = lane < 16;
+                        const uint32_t h = edge ? gl[16 * gS + 16 * c + lane] : 0u;
+                        const uint32_t l = edge ? gl[16 * gS + 128 + 16 * c + lane] : 0u;
+                        b0 = __builtin_bit_cast(f16x8_t, (u32x4_t){h | (h << 16), h, 0u, 0u});
+                        b1 = __builtin_bit_cast(f16x8_t, (u32x4_t){l | (l << 16), l, 0u, 0u});
+                    }
+                } else {
+                    const f16x8_t* bp = reinterpret_cast<const f16x8_t*>(bt + c * 64 * LB) + lane;
+                    b0 = bp[0];
+                    b1 = bp[64];
+                }
 #pragma unroll
                 for (int rb = 0; rb < 2; ++rb) {
                     acc[rb][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ap[rb][1], b0, acc[rb][c], 0, 0, 0);
@@ -852,9 +920,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) f
                 for (int rb = 0; rb < 2; ++rb) { acur[rb][0] = anext[rb][0]; acur[rb][1] = anext[rb][1]; }
             }
         }
-        wait_lds_copies();
-        __syncthreads();
+        if constexpr (!GEN) {
+            wait_lds_copies();
+            __syncthreads();
+        }
     }
+    if constexpr (GEN) __syncthreads();  // the staged tile reuses the copies' LDS
     if constexpr (H16) {
         // the row factors 2^-e_r * bscale (exact powers of two) of rows
         // 4 (l / 16) + q of each row block (written by this wave)
@@ -1063,6 +1134,41 @@ jh_status create_filter(jh_filter_plan** plan, int device, int W, int V, int Wou
                             }
                     p->Bhl.alloc(sizeof(unsigned short) * pk.size());
                     JH_HIP(hipMemcpy(p->Bhl.get(), pk.data(), sizeof(unsigned short) * pk.size(), hipMemcpyHostToDevice));
+                    // the generated operand (filter_mfma_kernel GEN): the
+                    // reversed diagonal R[i] = B[k][n] at n + off - k =
+                    // Wout - 1 + off - i, as the same hi / lo parts, in 8
+                    // copies shifted by 0..7 entries; and the edge rows
+                    const size_t gen_lds = std::max<size_t>((size_t)128 * kStagePitch * 2,
+                                                            ((size_t)16 * (W + 144) + 256) * 2);
+                    if (W % 32 == 0 && Wout % 128 == 0 && gen_lds <= kGenMaxLds) {
+                        const int stride = (Wout + W + 8 + 7) / 8 * 8;
+                        std::vector<unsigned short> rh((size_t)stride + 8, 0), rl((size_t)stride + 8, 0);
+                        for (int i = 0; i < Wout + W - 1; ++i) {
+                            const double v = std::ldexp(hc((long)Wout - 1 + off - i), eb);
+                            const double hi = f16_rne(v, &rh[i]);
+                            f16_rne(v - hi, &rl[i]);
+                        }
+                        std::vector<unsigned short> g((size_t)16 * stride, 0);
+                        for (int sg = 0; sg < 8; ++sg)
+                            for (int t = 0; t < stride; ++t) {
+                                g[(size_t)(2 * sg) * stride + t] = rh[(size_t)t + sg];
+                                g[(size_t)(2 * sg + 1) * stride + t] = rl[(size_t)t + sg];
+                            }
+                        std::vector<unsigned short> r3((size_t)2 * Wout, 0);
+                        for (int col = 0; col < Wout; ++col) {
+                            const double v = std::ldexp(rr[col] / 3.0, eb);
+                            const double hi = f16_rne(v, &r3[col]);
+                            f16_rne(v - hi, &r3[(size_t)Wout + col]);
+                        }
+                        p->Bgen.alloc(sizeof(unsigned short) * g.size());
+                        JH_HIP(hipMemcpy(p->Bgen.get(), g.data(), sizeof(unsigned short) * g.size(), hipMemcpyHostToDevice));
+                        p->R3.alloc(sizeof(unsigned short) * r3.size());
+                        JH_HIP(hipMemcpy(p->R3.get(), r3.data(), sizeof(unsigned short) * r3.size(), hipMemcpyHostToDevice));
+                        p->gen_stride = stride;
+                        p->gen_window = W + 128;
+                        p->gen_lds = (int)gen_lds;
+                        p->use_gen = 1;
+                    }
                 } else if (P == 0) {
                     std::vector<float> pk(lanes * 8);
                     for (int s = 0; s < nst; ++s)
@@ -1162,6 +1268,11 @@ void run_filter(jh_filter_plan* p, const float* d_in, float* d_out, int view0, i
     a.tiled = tiled;
     a.bstride = p->bstride;
     a.bscale = p->bscale;
+    a.Bgen = reinterpret_cast<const unsigned short*>(p->Bgen.get());
+    a.R3 = reinterpret_cast<const unsigned short*>(p->R3.get());
+    a.gen_stride = p->gen_stride;
+    a.gen_window = p->gen_window;
+    a.gen_nT = p->W / 32;
     require((size_t)p->V * nviews < (1u << 31), "jh_filter_run: too many rows");
     hipStream_t st = as_stream(stream);
     if (p->mode == JH_FILTER_FP32_VALU) {
@@ -1182,6 +1293,9 @@ void run_filter(jh_filter_plan* p, const float* d_in, float* d_out, int view0, i
         } else if (p->mode == JH_FILTER_MFMA_BF16X3) {
             if (tiled) filter_mfma_kernel<kModeX3, true><<<grid, 256, 0, st>>>(a);
             else filter_mfma_kernel<kModeX3, false><<<grid, 256, 0, st>>>(a);
+        } else if (p->mode == JH_FILTER_MFMA_F16X3 && p->use_gen) {
+            if (tiled) filter_mfma_kernel<kModeH3, true, true><<<grid, 256, p->gen_lds, st>>>(a);
+            else filter_mfma_kernel<kModeH3, false, true><<<grid, 256, p->gen_lds, st>>>(a);
         } else if (p->mode == JH_FILTER_MFMA_F16X3) {
             if (tiled) filter_mfma_kernel<kModeH3, true><<<grid, 256, 0, st>>>(a);
             else filter_mfma_kernel<kModeH3, false><<<grid, 256, 0, st>>>(a);
@@ -1358,6 +1472,25 @@ jh_status jh_filter_run_operand_staged(jh_filter_plan* p, jh_bp_plan* bp, const 
                 "columns, BF16/FP16 column strips, no ZFilter, same device, frame size and view count)");
         run_filter(p, d_a, nullptr, view0, nviews, stream, d_q, bp->uw.as<float>() + (size_t)view0 * bp->W, dtype,
                    1, bp_stage_scale(bp, dtype));
+    });
+}
+
+jh_status jh_filter_set_generated(jh_filter_plan* p, int enable)
+{
+    return guard([&] {
+        require(p != nullptr, "jh_filter_set_generated: plan is NULL");
+        require(!enable || p->Bgen.get() != nullptr,
+                "jh_filter_set_generated: the plan has no generated operand (split-FP16 form, W % 32 == 0, "
+                "output width % 128 == 0)");
+        p->use_gen = enable ? 1 : 0;
+    });
+}
+
+jh_status jh_filter_get_generated(jh_filter_plan* p, int* enabled)
+{
+    return guard([&] {
+        require(p != nullptr && enabled != nullptr, "jh_filter_get_generated: bad arguments");
+        *enabled = p->use_gen;
     });
 }
 

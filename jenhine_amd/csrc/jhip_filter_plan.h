@@ -19,6 +19,14 @@ struct jh_filter_plan {
     std::vector<double> H;        // L/2 + 1
     jh::DevBuf B;                 // Kp x Wout float (vector-ALU form)
     jh::DevBuf Bhl;               // MFMA forms: packed FP32, BF16-part or FP16-part operand
+    // split-FP16 form, generated operand: the Toeplitz part of the operand
+    // as 8 shifted copies of its FP16 hi / lo diagonals (gen_stride halves
+    // each), the edge rows' r / 3 per column (hi / lo, Wout each)
+    jh::DevBuf Bgen, R3;
+    int gen_stride = 0;           // halves per copy
+    int gen_window = 0;           // halves per copy a workgroup keeps in LDS (W + 128)
+    int gen_lds = 0;              // LDS bytes of the generated form's launch
+    int use_gen = 0;              // generated operand in use (else packed)
     jh::DevBuf pw;                // nblocks x V x W float
     jh::DevBuf blk;               // N int: block of every view (device copy)
 };

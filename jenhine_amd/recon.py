@@ -140,6 +140,16 @@ class FilterCore:
              pts if n else None, db if n else None, n, window, m)
         self._h = h
 
+    def set_generated(self, enable=True):
+        """Split-FP16 form: filter-matrix operand from its diagonals in LDS
+        (True) or copied from the packed matrix (False); same result."""
+        call("jh_filter_set_generated", self._h, int(bool(enable)))
+
+    def generated(self):
+        v = ctypes.c_int()
+        call("jh_filter_get_generated", self._h, ctypes.byref(v))
+        return bool(v.value)
+
     def output_bound(self, in_bound):
         """Upper bound of |filtered value| for inputs bounded by in_bound."""
         b = ctypes.c_double()

@@ -900,6 +900,57 @@ def test_filter_run_staged_equals_filter_then_stage(gpu, dtype, case):
     assert torch.equal(got2, ref2)
 
 
+@pytest.mark.parametrize("W,V,mirror", [(256, 150, False), (512, 40, False), (768, 70, False), (1024, 33, False),
+                                         (256, 48, True)])
+def test_filter_generated_operand_equals_packed(gpu, W, V, mirror):
+    """The split-FP16 filter reads its matrix operand from the Toeplitz
+    diagonals kept in LDS (eight shifted copies) instead of copying the packed
+    matrix per k step: filtered rows, fused staging and the pre-weighted
+    operand path (preprocessing written into the operand) are the packed
+    form's to the bit -- widths 256 to 1024, frames shorter and longer than
+    the 128-row tile, and the composite chain's mirrored 2W -> W filter."""
+    from jenhine_amd.recon import BPCore, FilterCore, PreProcessingCore
+    if mirror:
+        w = _offset_geometry(True, n_views=6, W=W, V=V)
+    else:
+        w = SMALL.scaled(n_views=5, n_u=W, n_v=V, name=f"filter-{W}")
+    N = w.n_views
+    win = 2 * W if mirror else W
+    ptm = w.ptms()
+    scan = w.scan_params()
+    filt = FilterCore(win, V, ptm, scan.DSO, scan.HalfSampleRate, mode="mfma16", device=gpu, w_out=W,
+                      y_mirroring=mirror)
+    assert filt.generated()
+    bp = BPCore(ptm, W, V, w.X, w.Y, w.Z, w.pxy, w.pz, device=gpu)
+    x = torch.from_numpy(synth.normal(43, (N, V, win))).to(gpu)
+    rng = np.random.default_rng(W)
+    raw = torch.from_numpy(rng.integers(200, 4000, (N, V, win), dtype=np.uint16).view(np.int16)).to(gpu)
+    pre = PreProcessingCore(win, V, device=gpu)
+    op = pre.process_operand(raw, filt)
+    outs = {}
+    for gen in (True, False):
+        filt.set_generated(gen)
+        rows = filt.process(x)
+        staged = filt.process_staged(x, bp, "f16") if not mirror else None
+        from_op = filt.process_operand_staged(op, N, bp, "f16") if not mirror else filt.process(x[1:3], view0=1)
+        outs[gen] = (rows, staged, from_op)
+    for a, b in zip(outs[True], outs[False]):
+        if a is not None:
+            assert torch.equal(a, b)
+
+
+def test_filter_generated_operand_refused_where_absent(gpu):
+    """Widths that are not a multiple of 32 keep the packed operand only."""
+    from jenhine_amd.recon import FilterCore
+    w = SMALL.scaled(n_views=3, n_u=200, n_v=40, name="filter-200")
+    scan = w.scan_params()
+    filt = FilterCore(200, 40, w.ptms(), scan.DSO, scan.HalfSampleRate, mode="mfma16", device=gpu)
+    assert not filt.generated()
+    with pytest.raises(RuntimeError, match="no generated operand"):
+        filt.set_generated(True)
+    filt.set_generated(False)
+
+
 def test_filter_run_staged_refuses_plans_that_do_not_fuse(gpu):
     from jenhine_amd.recon import BPCore, FilterCore
     w = SMALL.scaled(n_views=4, n_u=64, n_v=40, name="small-frames")
