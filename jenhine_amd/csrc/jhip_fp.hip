@@ -463,8 +463,11 @@ jh_status jh_fp_project(jh_bp_plan* p, const float* d_vol, float* d_proj, int vi
             fp_transpose_kernel<<<dim3((p->X + 31) / 32, (p->Z + 31) / 32, p->Y), dim3(32, 8), 0, s>>>(
                 d_vol, p->xt.as<float>(), p->X, p->Y, p->Z);
             check_launch("fp_transpose_kernel");
-            // nw waves (detector columns) per workgroup, each with V + 1 floats of LDS
-            int nw = 8;
+            // nw waves (detector columns) per workgroup, each with V + 1 floats of
+            // LDS.  Four: a workgroup's waves wait at the write-out for its
+            // slowest column (measured at 256^3 from 360 views of 512^2: 8 waves
+            // 6.41 ms, 4 waves 6.00, 2 waves 6.15)
+            int nw = 4;
             while (nw > 1 && (size_t)nw * (p->V + 1) * sizeof(float) > 65536) nw /= 2;
             const size_t lds = (size_t)nw * (p->V + 1) * sizeof(float);
             require(lds <= 65536, "jh_fp_project: detector too tall for the column projector (V <= 16383)");
